@@ -1,0 +1,180 @@
+/*
+ * msfec.h -- C-ABI of the MI355X multiscale basis library (libmsfec_hip.so).
+ *
+ * The library replaces, for a whole batch of coarse cells at once, the
+ * per-cell object Q::QBasis of MsFEC and the parts of Q::QMultiscale that
+ * consume it.  Every entry point below cites the reference member it stands
+ * in for (paths relative to the MsFEC repository).  Plain C types only: no
+ * C++ exceptions cross this boundary, pointers are host or device memory as
+ * the accompanying *_loc argument says, streams are hipStream_t passed as
+ * void*.  All GPU work of one handle runs on the stream given to
+ * msfec_q_compute (0 = the default stream).  A handle is not thread-safe;
+ * distinct handles may be used from distinct threads.
+ *
+ * Conventions shared with the reference (deal.II):
+ *   - coarse vertex order: bit d of the vertex index selects the low/high side
+ *     in direction d (GeometryInfo<dim> / FE_Q<dim>(1) local DoF order);
+ *   - the coarse element matrix M is nb x nb row-major with row = test basis,
+ *     nb = 2^dim, exactly like get_global_element_matrix();
+ *   - fine nodes and fine cells of one coarse cell are numbered
+ *     lexicographically, x fastest (deal.II numbers fine DoFs by cell traversal;
+ *     values agree node by node at equal coordinates).
+ */
+#ifndef MSFEC_H
+#define MSFEC_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int msfec_status;
+enum {
+  MSFEC_OK = 0,
+  MSFEC_ERR_INVALID = 1,     /* bad argument or unsupported combination    */
+  MSFEC_ERR_HIP = 2,         /* a HIP runtime call failed                  */
+  MSFEC_ERR_STATE = 3,       /* call out of order (e.g. get before compute) */
+  MSFEC_ERR_UNSUPPORTED = 4, /* valid request this build does not handle   */
+  MSFEC_ERR_CELL = 5         /* at least one cell failed; see cell status  */
+};
+
+/* where a pointer argument lives */
+enum { MSFEC_HOST = 0, MSFEC_DEVICE = 1 };
+
+/* diffusion coefficient layouts (per coarse cell, fine cells lexicographic):
+ *   SCALAR_CELL: N^dim doubles, A = a I constant on each fine cell
+ *   TENSOR_QP  : N^dim x 2^dim x ncomp doubles, symmetric tensor at each
+ *                QGauss(2) point (points lexicographic, x fastest), ncomp = 3
+ *                (xx,xy,yy) in 2-D and 6 (xx,xy,xz,yy,yz,zz) in 3-D.  This is
+ *                what Diffusion_A::value_list produces at the quadrature points
+ *                (source/equation_data/eqn_coeff_A.cc:163-191). */
+enum { MSFEC_COEF_SCALAR_CELL = 0, MSFEC_COEF_TENSOR_QP = 1 };
+/* load f: constant, one value per fine cell, or per fine cell per Gauss point */
+enum { MSFEC_RHS_CONST = 0, MSFEC_RHS_CELL = 1, MSFEC_RHS_QP = 2 };
+/* optional reaction R u (the reference's QBasis has none: 0 reproduces it) */
+enum { MSFEC_REACTION_NONE = 0, MSFEC_REACTION_CONST = 1, MSFEC_REACTION_CELL = 2 };
+/* local solver */
+enum { MSFEC_SOLVER_BAND_FP64 = 0 };
+/* per-cell status codes */
+enum { MSFEC_CELL_OK = 0, MSFEC_CELL_NOT_SPD = 2, MSFEC_CELL_NONFINITE = 3 };
+/* flags */
+enum { MSFEC_FLAG_TIMING = 1 /* record per-kernel HIP events in compute */ };
+
+/* The fields of ParametersBasis the basis reads
+ * (include/Q/q_parameters.h:78-101, source/Q/q_parameters.cc:252-262), plus
+ * the problem data the reference reads from the parameter file
+ * (source/equation_data/eqn_coeff_A.cc, eqn_rhs.cc). */
+typedef struct {
+  int dim;             /* 2 or 3                                           */
+  int n_refine_local;  /* ParametersBasis::n_refine_local                  */
+  int coef_mode;       /* MSFEC_COEF_*                                     */
+  int rhs_mode;        /* MSFEC_RHS_*                                      */
+  double rhs_const;    /* f for MSFEC_RHS_CONST                            */
+  int reaction_mode;   /* MSFEC_REACTION_*                                 */
+  double reaction_const;
+  int solver;          /* MSFEC_SOLVER_* (replaces use_direct_solver)      */
+  int flags;           /* MSFEC_FLAG_*                                     */
+} msfec_q_params;
+
+typedef struct {
+  int n_cells, dim, n_refine_local;
+  int n_basis;         /* nb = 2^dim                                       */
+  int n_fine_nodes;    /* (2^l + 1)^dim                                    */
+  int n_fine_cells;    /* 2^(l dim)                                        */
+  int n_interior;      /* (2^l - 1)^dim                                    */
+  int bandwidth;       /* half-bandwidth of the interior system            */
+  size_t coef_count;   /* doubles expected by msfec_q_set_coefficient      */
+  size_t rhs_count;    /* doubles expected by msfec_q_set_rhs (0: none)    */
+  size_t reaction_count;
+  size_t device_bytes; /* device memory held by the handle                 */
+} msfec_q_info;
+
+typedef struct msfec_q_batch *msfec_q_handle;
+
+/* Create a batch of n_cells coarse cells.  vertices: n_cells x 2^dim x dim
+ * doubles; each cell must be an axis-aligned box (the coarse meshes of the
+ * reference are refined hyper_cubes, q_global.cc:96-98).
+ * Replaces QBasis::QBasis (source/Q/q_basis.cc:7-56, include/Q/q_basis.h:77-82)
+ * for every cell of the map built in QMultiscale::initialize_and_compute_basis
+ * (source/Q/q_global.cc:38-71). */
+msfec_status msfec_q_create(const msfec_q_params *params, int n_cells,
+                            const double *vertices, int vertices_loc,
+                            msfec_q_handle *out);
+
+/* Coefficient / load / reaction data (layouts above).  count must equal the
+ * value reported by msfec_q_info.  Replaces the evaluation of Diffusion_A,
+ * RightHandSideParsed and ReactionRate inside QBasis::assemble_system
+ * (source/Q/q_basis.cc:196-220). */
+msfec_status msfec_q_set_coefficient(msfec_q_handle h, const double *data,
+                                     size_t count, int loc);
+msfec_status msfec_q_set_rhs(msfec_q_handle h, const double *data, size_t count,
+                             int loc);
+msfec_status msfec_q_set_reaction(msfec_q_handle h, const double *data,
+                                  size_t count, int loc);
+
+/* Compute every basis of every cell: fine assembly, Dirichlet lifting of the
+ * coarse Q1 shape functions, the 2^dim local solves and the coarse element
+ * matrix/RHS.  Asynchronous on `stream`.
+ * Replaces QBasis::run (source/Q/q_basis.cc:518-603), i.e. make_grid,
+ * setup_system, assemble_system, the condense + solve_direct/solve_iterative
+ * loop and assemble_global_element_matrix (:95-357). */
+msfec_status msfec_q_compute(msfec_q_handle h, void *stream);
+
+/* Wait for the work of the last compute. */
+msfec_status msfec_q_synchronize(msfec_q_handle h);
+
+/* M: n_cells x nb x nb, b: n_cells x nb (either may be NULL).  Synchronizes.
+ * Replaces QBasis::get_global_element_matrix / get_global_element_rhs
+ * (source/Q/q_basis.cc:361-373). */
+msfec_status msfec_q_get_element(msfec_q_handle h, double *M, double *b, int loc);
+
+/* Psi: n_cells x nb x n_fine_nodes, basis-major.  Synchronizes.
+ * Replaces the per-cell solution_vector array of QBasis
+ * (include/Q/q_basis.h:253). */
+msfec_status msfec_q_get_basis(msfec_q_handle h, double *Psi, int loc);
+
+/* Per-cell status (MSFEC_CELL_*), n_cells ints on the host.  Synchronizes.
+ * The reference aborts the run on a failed local solve (q_basis.cc:337); here
+ * the failing cells are reported and the others stay valid. */
+msfec_status msfec_q_get_cell_status(msfec_q_handle h, int *status);
+
+/* u_fine[c][node] = sum_k w[c][k] Psi[c][k][node].  weights: n_cells x nb.
+ * Replaces QBasis::set_global_weights (source/Q/q_basis.cc:394-416). */
+msfec_status msfec_q_reconstruct(msfec_q_handle h, const double *weights,
+                                 int weights_loc, double *u_fine, int out_loc);
+
+/* Sizes of the batch. */
+msfec_status msfec_q_get_info(msfec_q_handle h, msfec_q_info *info);
+
+/* Per-kernel times (ms), averaged over every compute made with
+ * MSFEC_FLAG_TIMING since the previous call (which this call then forgets):
+ * [0] assembly, [1] factor+solve, [2] element matrix.  Synchronizes.  The
+ * number of entries written is returned in *n. */
+msfec_status msfec_q_kernel_times(msfec_q_handle h, float *ms, int max, int *n);
+
+void msfec_q_destroy(msfec_q_handle h);
+
+/* Coarse multiscale problem on a structured grid of nc^dim coarse cells of
+ * [0,1]^dim (cells and nodes lexicographic, x fastest) with homogeneous
+ * Dirichlet data on the whole boundary.  Assembles the element matrices and
+ * right-hand sides of a basis batch whose cells follow that order and solves
+ * by Jacobi-preconditioned CG in FP64 on the device until
+ * ||r|| <= max(abs_tol, rel_tol ||b||).  u (n_nodes doubles, loc) receives the
+ * nodal solution, zero on the boundary; iterations and the final residual are
+ * returned.  Replaces QMultiscale::assemble_system, solve_iterative and the
+ * weight extraction of send_global_weights_to_cell
+ * (source/Q/q_global.cc:198-297, 327-363, 368-392). */
+msfec_status msfec_coarse_solve(msfec_q_handle h, int nc, double abs_tol,
+                                double rel_tol, int max_iter, double *u,
+                                int u_loc, double *weights, int weights_loc,
+                                int *iterations, double *residual);
+
+/* Thread-local description of the last error of this thread. */
+const char *msfec_last_error(void);
+const char *msfec_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MSFEC_H */

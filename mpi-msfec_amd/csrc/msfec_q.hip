@@ -1,0 +1,1284 @@
+// MI355X (gfx950) multiscale Q1 basis construction: kernels and C-ABI.
+//
+// One coarse cell = one independent local problem (the per-cell QBasis of
+// MsFEC, source/Q/q_basis.cc).  The batch is processed by three kernels:
+//
+//   q_assemble_kernel   fine Q1 stencil (lower half + diagonal) of every node,
+//                       the load vector F and the Dirichlet-lifting right-hand
+//                       sides -K_IB g_k (q_basis.cc:117-257)
+//   q_band_kernel       banded LDL^T of the interior system K_II with the
+//                       forward solve fused, then the backward solve
+//                       (the condense + solve loop, q_basis.cc:545-565)
+//   q_element_kernel    Psi = [g on the boundary, x inside], M = Psi^T K Psi,
+//                       b = Psi^T F (assemble_global_element_matrix,
+//                       q_basis.cc:262-300)
+//
+// The banded factorisation keeps a 64-slot sliding window of the active
+// Schur complement in registers of one wavefront: lane (p,q) = (lane/8,
+// lane%8) owns the 8x8 tile (8p..8p+7) x (8q..8q+7) of the window, slot
+// s = index mod 64.  Each elimination step reads the pivot row through LDS,
+// applies a rank-1 update to the register tiles, streams the finished column
+// of L to HBM and recycles the pivot's slot for index j+64, whose (still
+// unmodified) matrix row is inserted from the stencil table.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <array>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "msfec.h"
+
+namespace {
+
+constexpr int kWin = 64;       // sliding-window slots (>= bandwidth + 1)
+constexpr int kMaxNb = 8;      // bases per cell (2^dim)
+constexpr int kMaxLow = 14;    // lower stencil entries incl. diagonal (3-D)
+
+struct QConf {
+  int dim, N, S, nn, nfc, nv, ni, n, nst, nlow, center;
+  int coef_mode, rhs_mode, reaction_mode;
+  double rhs_const, reaction_const;
+  int off_full[kMaxLow];   // full-grid offset (>0) of lower neighbour t
+  int off_int[kMaxLow];    // interior-grid offset (>0) of lower neighbour t
+  int dir_low[kMaxLow];    // direction index e of lower neighbour t (t>0)
+};
+
+// ---------------------------------------------------------------------------
+// Element matrix entries on an axis-aligned box (QGauss(2) is exact for the
+// scalar-coefficient case, so the closed tensor-product form is used there).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double box_stiff(int dim, int a, int b, const double* h) {
+  double s = 0.0;
+  for (int d = 0; d < dim; ++d) {
+    double t = ((((a ^ b) >> d) & 1) ? -1.0 : 1.0) / h[d];
+    for (int e = 0; e < dim; ++e)
+      if (e != d) t *= ((((a ^ b) >> e) & 1) ? (1.0 / 6.0) : (1.0 / 3.0)) * h[e];
+    s += t;
+  }
+  return s;
+}
+
+__device__ __forceinline__ double box_mass(int dim, int a, int b, const double* h) {
+  double t = 1.0;
+  for (int e = 0; e < dim; ++e) t *= ((((a ^ b) >> e) & 1) ? (1.0 / 6.0) : (1.0 / 3.0)) * h[e];
+  return t;
+}
+
+__device__ __constant__ double kGQ[2] = {0.5 - 0.28867513459481288225,
+                                         0.5 + 0.28867513459481288225};
+
+// grad phi_a at reference point xi (physical derivative on the box)
+__device__ __forceinline__ void box_grad(int dim, int a, const double* xi, const double* h,
+                                         double* g) {
+  for (int d = 0; d < dim; ++d) {
+    double v = (((a >> d) & 1) ? 1.0 : -1.0) / h[d];
+    for (int e = 0; e < dim; ++e)
+      if (e != d) v *= ((a >> e) & 1) ? xi[e] : 1.0 - xi[e];
+    g[d] = v;
+  }
+}
+
+__device__ __forceinline__ double box_val(int dim, int a, const double* xi) {
+  double v = 1.0;
+  for (int e = 0; e < dim; ++e) v *= ((a >> e) & 1) ? xi[e] : 1.0 - xi[e];
+  return v;
+}
+
+// K_z(a,b) for fine cell z
+__device__ double cell_entry(const QConf& c, const double* coef, const double* reac,
+                             int z, int a, int b, const double* h) {
+  double v;
+  if (c.coef_mode == MSFEC_COEF_SCALAR_CELL) {
+    v = coef[z] * box_stiff(c.dim, a, b, h);
+  } else {
+    const int nc = (c.dim == 2) ? 3 : 6;
+    double vol = 1.0;
+    for (int d = 0; d < c.dim; ++d) vol *= h[d];
+    v = 0.0;
+    for (int q = 0; q < c.nv; ++q) {
+      double xi[3];
+      double w = 1.0;
+      for (int d = 0; d < c.dim; ++d) {
+        xi[d] = kGQ[(q >> d) & 1];
+        w *= 0.5;
+      }
+      double ga[3], gb[3];
+      box_grad(c.dim, a, xi, h, ga);
+      box_grad(c.dim, b, xi, h, gb);
+      const double* s = coef + ((size_t)z * c.nv + q) * nc;
+      double Ag[3];
+      if (c.dim == 2) {
+        Ag[0] = s[0] * gb[0] + s[1] * gb[1];
+        Ag[1] = s[1] * gb[0] + s[2] * gb[1];
+      } else {
+        Ag[0] = s[0] * gb[0] + s[1] * gb[1] + s[2] * gb[2];
+        Ag[1] = s[1] * gb[0] + s[3] * gb[1] + s[4] * gb[2];
+        Ag[2] = s[2] * gb[0] + s[4] * gb[1] + s[5] * gb[2];
+      }
+      double t = 0.0;
+      for (int d = 0; d < c.dim; ++d) t += ga[d] * Ag[d];
+      v += t * w * vol;
+    }
+  }
+  if (c.reaction_mode != MSFEC_REACTION_NONE) {
+    const double r = (c.reaction_mode == MSFEC_REACTION_CONST) ? c.reaction_const : reac[z];
+    v += r * box_mass(c.dim, a, b, h);
+  }
+  return v;
+}
+
+// K[n][n+dd] for node coordinates cn and direction dd (any of the 3^dim)
+__device__ double pair_value(const QConf& c, const double* coef, const double* reac,
+                             const int* cn, const int* dd, const double* h) {
+  // cells z = cn - alpha with alpha_d fixed where dd_d != 0
+  double s = 0.0;
+  for (int al = 0; al < c.nv; ++al) {
+    bool ok = true;
+    int z = 0, mul = 1, a = 0, b = 0;
+    for (int d = 0; d < c.dim; ++d) {
+      const int ad = (al >> d) & 1;
+      const int bd = ad + dd[d];
+      if (bd < 0 || bd > 1) { ok = false; break; }
+      const int zd = cn[d] - ad;
+      if (zd < 0 || zd >= c.N) { ok = false; break; }
+      z += zd * mul;
+      mul *= c.N;
+      a |= ad << d;
+      b |= bd << d;
+    }
+    if (ok) s += cell_entry(c, coef, reac, z, a, b, h);
+  }
+  return s;
+}
+
+__device__ __forceinline__ void cell_box(const QConf& c, const double* verts, int cell,
+                                         double* lo, double* h) {
+  const double* v = verts + (size_t)cell * c.nv * c.dim;
+  for (int d = 0; d < c.dim; ++d) {
+    lo[d] = v[d];
+    h[d] = (v[(c.nv - 1) * c.dim + d] - v[d]) / c.N;
+  }
+}
+
+// g_k at fine node with grid coordinates m: trilinear coarse basis on the box
+__device__ __forceinline__ double lift_value(const QConf& c, int k, const int* m) {
+  double v = 1.0;
+  for (int d = 0; d < c.dim; ++d) {
+    const double xi = (double)m[d] / (double)c.N;
+    v *= ((k >> d) & 1) ? xi : 1.0 - xi;
+  }
+  return v;
+}
+
+// ---------------------------------------------------------------------------
+// K1: stencil table, load vector and lifting right-hand sides.
+// Slo[cell][node][t]: t = 0 diagonal, t = 1+e lower direction e < center.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void q_assemble_kernel(
+    QConf c, const double* __restrict__ verts, const double* __restrict__ coef_all,
+    size_t coef_stride, const double* __restrict__ f_all, size_t f_stride,
+    const double* __restrict__ reac_all, size_t reac_stride, double* __restrict__ Slo,
+    double* __restrict__ Fv, double* __restrict__ Rhs) {
+  const int cell = blockIdx.x;
+  const double* coef = coef_all + (size_t)cell * coef_stride;
+  const double* fcell = f_all ? f_all + (size_t)cell * f_stride : nullptr;
+  const double* reac = reac_all ? reac_all + (size_t)cell * reac_stride : nullptr;
+  double lo[3], h[3];
+  cell_box(c, verts, cell, lo, h);
+  double vol = 1.0;
+  for (int d = 0; d < c.dim; ++d) vol *= h[d];
+  double* slo = Slo + (size_t)cell * c.nn * c.nlow;
+  double* fv = Fv + (size_t)cell * c.nn;
+  double* rhs = Rhs + (size_t)cell * c.n * kMaxNb;
+
+  for (int node = threadIdx.x; node < c.nn; node += blockDim.x) {
+    int cn[3] = {0, 0, 0};
+    {
+      int r = node;
+      for (int d = 0; d < c.dim; ++d) {
+        cn[d] = r % c.S;
+        r /= c.S;
+      }
+    }
+    // lower stencil
+    for (int t = 0; t < c.nlow; ++t) {
+      int dd[3] = {0, 0, 0};
+      if (t > 0) {
+        int e = c.dir_low[t];
+        for (int d = 0; d < c.dim; ++d) {
+          dd[d] = e % 3 - 1;
+          e /= 3;
+        }
+      }
+      bool inside = true;
+      for (int d = 0; d < c.dim; ++d) {
+        const int m = cn[d] + dd[d];
+        if (m < 0 || m > c.N) inside = false;
+      }
+      slo[(size_t)node * c.nlow + t] = inside ? pair_value(c, coef, reac, cn, dd, h) : 0.0;
+    }
+    // load vector: sum over the cells touching the node
+    double fs = 0.0;
+    for (int al = 0; al < c.nv; ++al) {
+      int z = 0, mul = 1;
+      bool ok = true;
+      for (int d = 0; d < c.dim; ++d) {
+        const int zd = cn[d] - ((al >> d) & 1);
+        if (zd < 0 || zd >= c.N) { ok = false; break; }
+        z += zd * mul;
+        mul *= c.N;
+      }
+      if (!ok) continue;
+      if (c.rhs_mode == MSFEC_RHS_QP) {
+        for (int q = 0; q < c.nv; ++q) {
+          double xi[3], w = 1.0;
+          for (int d = 0; d < c.dim; ++d) {
+            xi[d] = kGQ[(q >> d) & 1];
+            w *= 0.5;
+          }
+          fs += box_val(c.dim, al, xi) * fcell[(size_t)z * c.nv + q] * w * vol;
+        }
+      } else {
+        const double fz = (c.rhs_mode == MSFEC_RHS_CELL) ? fcell[z] : c.rhs_const;
+        fs += fz * vol / c.nv;
+      }
+    }
+    fv[node] = fs;
+    // lifting right-hand sides for interior nodes
+    bool interior = true;
+    for (int d = 0; d < c.dim; ++d)
+      if (cn[d] == 0 || cn[d] == c.N) interior = false;
+    if (interior) {
+      int ii = 0, mul = 1;
+      for (int d = 0; d < c.dim; ++d) {
+        ii += (cn[d] - 1) * mul;
+        mul *= c.ni;
+      }
+      double acc[kMaxNb];
+      for (int k = 0; k < kMaxNb; ++k) acc[k] = 0.0;
+      for (int e = 0; e < c.nst; ++e) {
+        if (e == c.center) continue;
+        int dd[3] = {0, 0, 0}, m[3] = {0, 0, 0}, r = e;
+        bool bnd = false;
+        for (int d = 0; d < c.dim; ++d) {
+          dd[d] = r % 3 - 1;
+          r /= 3;
+          m[d] = cn[d] + dd[d];
+          if (m[d] == 0 || m[d] == c.N) bnd = true;
+        }
+        if (!bnd) continue;
+        const double kv = pair_value(c, coef, reac, cn, dd, h);
+        for (int k = 0; k < c.nv; ++k) acc[k] -= kv * lift_value(c, k, m);
+      }
+      for (int k = 0; k < kMaxNb; ++k) rhs[(size_t)ii * kMaxNb + k] = acc[k];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// K2: banded LDL^T in a register-resident sliding window (one wave per cell)
+// ---------------------------------------------------------------------------
+struct BandArgs {
+  int n;          // interior unknowns
+  int nlow;       // lower stencil entries incl. diagonal
+  int nn;         // fine nodes (stride of the stencil table)
+  int off_int[kMaxLow];
+  const double* Slo;     // [cell][nn][nlow]
+  const double* Rhs;     // [cell][n][8]
+  const int* inode;      // interior index -> node id
+  const unsigned* imask; // interior index -> bit t set if lower neighbour t is interior
+  double* Lcol;          // [cell][n][64]  (o = 0: pivot d_j, o > 0: L[j+o][j])
+  double* Wf;            // [cell][n][8]   D^-1 L^-1 rhs
+  double* X;             // [cell][n][8]   interior solution
+  int* status;           // [cell]
+};
+
+template <int I>
+__device__ __forceinline__ void band_insert(const BandArgs& a, int cell, int jn, int G,
+                                            double (&A)[8][8], double (&y)[8],
+                                            double* nbuf) {
+  const int lane = threadIdx.x;
+  const int p = lane >> 3, q = lane & 7;
+  const int s = 8 * G + I;
+  nbuf[lane] = 0.0;
+  __syncthreads();
+  if (jn < a.n) {
+    if (lane < a.nlow) {
+      const unsigned msk = a.imask[jn];
+      if ((msk >> lane) & 1u) {
+        const double v = a.Slo[((size_t)cell * a.nn + a.inode[jn]) * a.nlow + lane];
+        nbuf[(s - a.off_int[lane]) & (kWin - 1)] = v;
+      }
+    }
+  } else if (lane == 0) {
+    nbuf[s] = 1.0;  // past the end: an isolated unit diagonal
+  }
+  __syncthreads();
+  const int base = (p == G) ? 8 * q : 8 * p;
+  double v[8];
+#pragma unroll
+  for (int b = 0; b < 8; ++b) v[b] = nbuf[base + b];
+  if (p == G) {
+#pragma unroll
+    for (int b = 0; b < 8; ++b) A[I][b] = v[b];
+  }
+  if (q == G) {
+#pragma unroll
+    for (int r = 0; r < 8; ++r) A[r][I] = v[r];
+  }
+  if (lane == s) {
+    if (jn < a.n) {
+      const double* src = a.Rhs + ((size_t)cell * a.n + jn) * kMaxNb;
+#pragma unroll
+      for (int k = 0; k < 8; ++k) y[k] = src[k];
+    } else {
+#pragma unroll
+      for (int k = 0; k < 8; ++k) y[k] = 0.0;
+    }
+  }
+}
+
+template <int I>
+__device__ __forceinline__ void band_step(const BandArgs& a, int cell, int j, int G,
+                                          double (&A)[8][8], double (&y)[8], double* ubuf,
+                                          double* ybuf, double* nbuf, int& bad) {
+  const int lane = threadIdx.x;
+  const int p = lane >> 3, q = lane & 7;
+  const int s = 8 * G + I;
+  // pivot row of the window -> LDS (the tiles of block-row G hold it in row I)
+  if (p == G) {
+#pragma unroll
+    for (int b = 0; b < 8; ++b) ubuf[8 * q + b] = A[I][b];
+  }
+  if (lane == s) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) ybuf[k] = y[k];
+  }
+  __syncthreads();
+  const double d = ubuf[s];
+  if (!(d > 0.0)) bad |= (d == d && d < INFINITY) ? MSFEC_CELL_NOT_SPD : MSFEC_CELL_NONFINITE;
+  const double rd = 1.0 / d;
+  double l[8], u[8], ys[8];
+#pragma unroll
+  for (int r = 0; r < 8; ++r) l[r] = ubuf[8 * p + r] * rd;
+#pragma unroll
+  for (int b = 0; b < 8; ++b) u[b] = ubuf[8 * q + b];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) ys[k] = ybuf[k];
+  const double um = ubuf[lane];
+  const double lm = um * rd;
+  const double wk = ybuf[lane & 7] * rd;
+  // column j of L (offset o = slot distance from the pivot) and w_j
+  a.Lcol[((size_t)cell * a.n + j) * kWin + ((lane - s) & (kWin - 1))] = (lane == s) ? d : lm;
+  if (lane < 8) a.Wf[((size_t)cell * a.n + j) * kMaxNb + lane] = wk;
+  // rank-1 update of the window and of the right-hand sides
+#pragma unroll
+  for (int r = 0; r < 8; ++r)
+#pragma unroll
+    for (int b = 0; b < 8; ++b) A[r][b] = fma(-l[r], u[b], A[r][b]);
+  if (lane != s) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) y[k] = fma(-lm, ys[k], y[k]);
+  }
+  __syncthreads();
+  band_insert<I>(a, cell, j + kWin, G, A, y, nbuf);
+}
+
+template <int I>
+__device__ __forceinline__ void band_init_one(const BandArgs& a, int cell, int G,
+                                              double (&A)[8][8], double (&y)[8],
+                                              double* nbuf) {
+  band_insert<I>(a, cell, 8 * G + I, G, A, y, nbuf);
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(64) void q_band_kernel(BandArgs a) {
+  const int cell = blockIdx.x;
+  const int lane = threadIdx.x;
+  __shared__ double ubuf[kWin];
+  __shared__ double nbuf[kWin];
+  __shared__ double ybuf[8];
+  __shared__ double xring[8][2 * kWin];
+
+  double A[8][8];
+  double y[8];
+#pragma unroll
+  for (int r = 0; r < 8; ++r)
+#pragma unroll
+    for (int b = 0; b < 8; ++b) A[r][b] = 0.0;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) y[k] = 0.0;
+
+  // initial window: indices 0..63 in slots 0..63
+  for (int G = 0; G < 8; ++G) {
+    band_init_one<0>(a, cell, G, A, y, nbuf);
+    band_init_one<1>(a, cell, G, A, y, nbuf);
+    band_init_one<2>(a, cell, G, A, y, nbuf);
+    band_init_one<3>(a, cell, G, A, y, nbuf);
+    band_init_one<4>(a, cell, G, A, y, nbuf);
+    band_init_one<5>(a, cell, G, A, y, nbuf);
+    band_init_one<6>(a, cell, G, A, y, nbuf);
+    band_init_one<7>(a, cell, G, A, y, nbuf);
+  }
+
+  int bad = 0;
+  const int n = a.n;
+  for (int j0 = 0; j0 < n; j0 += 8) {
+    const int G = (j0 >> 3) & 7;
+    if (j0 + 0 < n) band_step<0>(a, cell, j0 + 0, G, A, y, ubuf, ybuf, nbuf, bad);
+    if (j0 + 1 < n) band_step<1>(a, cell, j0 + 1, G, A, y, ubuf, ybuf, nbuf, bad);
+    if (j0 + 2 < n) band_step<2>(a, cell, j0 + 2, G, A, y, ubuf, ybuf, nbuf, bad);
+    if (j0 + 3 < n) band_step<3>(a, cell, j0 + 3, G, A, y, ubuf, ybuf, nbuf, bad);
+    if (j0 + 4 < n) band_step<4>(a, cell, j0 + 4, G, A, y, ubuf, ybuf, nbuf, bad);
+    if (j0 + 5 < n) band_step<5>(a, cell, j0 + 5, G, A, y, ubuf, ybuf, nbuf, bad);
+    if (j0 + 6 < n) band_step<6>(a, cell, j0 + 6, G, A, y, ubuf, ybuf, nbuf, bad);
+    if (j0 + 7 < n) band_step<7>(a, cell, j0 + 7, G, A, y, ubuf, ybuf, nbuf, bad);
+  }
+
+  // backward solve x_j = w_j - sum_{o>=1} L[j+o][j] x_{j+o}
+  // lane = (k, m): RHS k = lane & 7, offsets 8m..8m+7 with m = lane >> 3
+  __threadfence_block();
+  __syncthreads();
+  const int k = lane & 7, m = lane >> 3;
+  for (int t = lane; t < 8 * 2 * kWin; t += 64) (&xring[0][0])[t] = 0.0;
+  __syncthreads();
+  const double* Lc = a.Lcol + (size_t)cell * n * kWin;
+  const double* Wc = a.Wf + (size_t)cell * n * kMaxNb;
+  double* Xc = a.X + (size_t)cell * n * kMaxNb;
+  for (int j = n - 1; j >= 0; --j) {
+    const double* lrow = Lc + (size_t)j * kWin + 8 * m;
+    const int base = (j + 8 * m) & (kWin - 1);
+    double part = 0.0;
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      const double lv = (m == 0 && t == 0) ? 0.0 : lrow[t];
+      part = fma(lv, xring[k][base + t], part);
+    }
+    part += __shfl_xor(part, 8);
+    part += __shfl_xor(part, 16);
+    part += __shfl_xor(part, 32);
+    const double xj = Wc[(size_t)j * kMaxNb + k] - part;
+    if (m == 0) {
+      xring[k][j & (kWin - 1)] = xj;
+      xring[k][(j & (kWin - 1)) + kWin] = xj;
+      Xc[(size_t)j * kMaxNb + k] = xj;
+    }
+    __syncthreads();
+  }
+  // cell status (max over lanes)
+  int worst = bad;
+  for (int o = 32; o > 0; o >>= 1) worst = max(worst, __shfl_xor(worst, o));
+  if (lane == 0) a.status[cell] = worst;
+}
+
+// ---------------------------------------------------------------------------
+// K3: Psi, element matrix and element right-hand side
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void q_element_kernel(QConf c, const double* __restrict__ Slo,
+                                                        const double* __restrict__ Fv,
+                                                        const double* __restrict__ X,
+                                                        double* __restrict__ Psi,
+                                                        double* __restrict__ Mout,
+                                                        double* __restrict__ bout) {
+  extern __shared__ double smem[];
+  const int cell = blockIdx.x;
+  const int nn = c.nn, nv = c.nv;
+  double* psi = smem;             // [nv][nn]
+  double* kpsi = smem + nv * nn;  // [nv][nn]
+  double* red = kpsi + nv * nn;   // [4][64 + 8]
+  const double* slo = Slo + (size_t)cell * nn * c.nlow;
+  const double* fv = Fv + (size_t)cell * nn;
+  const double* xc = X + (size_t)cell * c.n * kMaxNb;
+  double* psio = Psi + (size_t)cell * nv * nn;
+
+  for (int node = threadIdx.x; node < nn; node += blockDim.x) {
+    int cn[3] = {0, 0, 0}, r = node;
+    bool interior = true;
+    int ii = 0, mul = 1;
+    for (int d = 0; d < c.dim; ++d) {
+      cn[d] = r % c.S;
+      r /= c.S;
+      if (cn[d] == 0 || cn[d] == c.N) interior = false;
+      ii += (cn[d] - 1) * mul;
+      mul *= c.ni;
+    }
+    for (int k = 0; k < nv; ++k) {
+      const double v = interior ? xc[(size_t)ii * kMaxNb + k] : lift_value(c, k, cn);
+      psi[k * nn + node] = v;
+      psio[(size_t)k * nn + node] = v;
+    }
+  }
+  __syncthreads();
+  // K psi with the symmetric stencil: lower entries from the own row, upper
+  // entries from the neighbour's row
+  for (int node = threadIdx.x; node < nn; node += blockDim.x) {
+    int cn[3] = {0, 0, 0}, r = node;
+    for (int d = 0; d < c.dim; ++d) {
+      cn[d] = r % c.S;
+      r /= c.S;
+    }
+    double acc[kMaxNb];
+    for (int k = 0; k < kMaxNb; ++k) acc[k] = 0.0;
+    for (int e = 0; e < c.nst; ++e) {
+      int dd[3] = {0, 0, 0}, rr = e, off = 0, mm = 1;
+      bool ok = true;
+      for (int d = 0; d < c.dim; ++d) {
+        dd[d] = rr % 3 - 1;
+        rr /= 3;
+        const int mcoord = cn[d] + dd[d];
+        if (mcoord < 0 || mcoord > c.N) ok = false;
+        off += dd[d] * mm;
+        mm *= c.S;
+      }
+      if (!ok) continue;
+      const int nb = node + off;
+      double kv;
+      if (e == c.center) kv = slo[(size_t)node * c.nlow];
+      else if (e < c.center) kv = slo[(size_t)node * c.nlow + 1 + e];
+      else kv = slo[(size_t)nb * c.nlow + 1 + (c.nst - 1 - e)];
+      for (int k = 0; k < nv; ++k) acc[k] = fma(kv, psi[k * nn + nb], acc[k]);
+    }
+    for (int k = 0; k < nv; ++k) kpsi[k * nn + node] = acc[k];
+  }
+  __syncthreads();
+  // M[i][j] = psi_i . (K psi_j): 64 (i,j) pairs x 4 node partitions
+  {
+    const int t = threadIdx.x;
+    const int ij = t & 63, part = t >> 6;
+    const int i = ij >> 3, j = ij & 7;
+    double s = 0.0;
+    if (i < nv && j < nv)
+      for (int node = part; node < nn; node += 4) s = fma(psi[i * nn + node], kpsi[j * nn + node], s);
+    red[part * 72 + ij] = s;
+    double sb = 0.0;
+    if (ij < nv)
+      for (int node = part; node < nn; node += 4) sb = fma(psi[ij * nn + node], fv[node], sb);
+    if (ij < 8) red[part * 72 + 64 + ij] = sb;
+  }
+  __syncthreads();
+  if (threadIdx.x < 72) {
+    const int t = threadIdx.x;
+    const double s = (red[t] + red[72 + t]) + (red[144 + t] + red[216 + t]);
+    if (t < 64) {
+      const int i = t >> 3, j = t & 7;
+      if (i < nv && j < nv) Mout[((size_t)cell * nv + i) * nv + j] = s;
+    } else if (t - 64 < nv) {
+      bout[(size_t)cell * nv + (t - 64)] = s;
+    }
+  }
+}
+
+__global__ void q_reconstruct_kernel(int nv, int nn, int n_cells, const double* __restrict__ Psi,
+                                     const double* __restrict__ w, double* __restrict__ u) {
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (size_t)n_cells * nn) return;
+  const size_t cell = idx / nn, node = idx % nn;
+  double s = 0.0;
+  for (int k = 0; k < nv; ++k) s = s + w[cell * nv + k] * Psi[(cell * nv + k) * nn + node];
+  u[idx] = s;
+}
+
+// ---------------------------------------------------------------------------
+// Coarse system: stencil assembly by gather (deterministic) and a
+// single-workgroup Jacobi-PCG in FP64.
+// ---------------------------------------------------------------------------
+__global__ void coarse_assemble_kernel(int dim, int nc, const double* __restrict__ M,
+                                       const double* __restrict__ b, double* __restrict__ K,
+                                       double* __restrict__ rhs) {
+  const int nst = (dim == 2) ? 9 : 27, nv = 1 << dim;
+  int nnode = 1;
+  for (int d = 0; d < dim; ++d) nnode *= nc + 1;
+  const int node = blockIdx.x * blockDim.x + threadIdx.x;
+  if (node >= nnode) return;
+  int cn[3] = {0, 0, 0}, r = node;
+  bool bnd = false;
+  for (int d = 0; d < dim; ++d) {
+    cn[d] = r % (nc + 1);
+    r /= nc + 1;
+    if (cn[d] == 0 || cn[d] == nc) bnd = true;
+  }
+  double rs = 0.0;
+  for (int e = 0; e < nst; ++e) {
+    int dd[3] = {0, 0, 0}, rr = e;
+    bool nbnd = false, inside = true;
+    for (int d = 0; d < dim; ++d) {
+      dd[d] = rr % 3 - 1;
+      rr /= 3;
+      const int m = cn[d] + dd[d];
+      if (m < 0 || m > nc) inside = false;
+      if (m == 0 || m == nc) nbnd = true;
+    }
+    double s = 0.0;
+    if (inside && !bnd && !nbnd) {
+      for (int al = 0; al < nv; ++al) {
+        bool ok = true;
+        int z = 0, mul = 1, a = 0, bb = 0;
+        for (int d = 0; d < dim; ++d) {
+          const int ad = (al >> d) & 1, bd = ad + dd[d];
+          const int zd = cn[d] - ad;
+          if (bd < 0 || bd > 1 || zd < 0 || zd >= nc) { ok = false; break; }
+          z += zd * mul;
+          mul *= nc;
+          a |= ad << d;
+          bb |= bd << d;
+        }
+        if (ok) s += M[((size_t)z * nv + a) * nv + bb];
+      }
+    } else if (bnd && e == (nst - 1) / 2) {
+      s = 1.0;  // constrained node: unit row, zero right-hand side
+    }
+    K[(size_t)node * nst + e] = s;
+  }
+  if (!bnd) {
+    for (int al = 0; al < nv; ++al) {
+      bool ok = true;
+      int z = 0, mul = 1;
+      for (int d = 0; d < dim; ++d) {
+        const int zd = cn[d] - ((al >> d) & 1);
+        if (zd < 0 || zd >= nc) { ok = false; break; }
+        z += zd * mul;
+        mul *= nc;
+      }
+      if (ok) rs += b[(size_t)z * nv + al];
+    }
+  }
+  rhs[node] = rs;
+}
+
+__device__ double block_sum(double v, double* red) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  __syncthreads();
+  if (lane == 0) red[w] = v;
+  __syncthreads();
+  double s = 0.0;
+  const int nw = blockDim.x >> 6;
+  for (int i = 0; i < nw; ++i) s += red[i];
+  return s;
+}
+
+__global__ __launch_bounds__(1024) void coarse_cg_kernel(int dim, int nc, const double* __restrict__ K,
+                                                         const double* __restrict__ rhs,
+                                                         double* __restrict__ u, double* r,
+                                                         double* z, double* pv, double* qv,
+                                                         double abs_tol, double rel_tol,
+                                                         int max_iter, int* iters_out,
+                                                         double* res_out) {
+  __shared__ double red[16];
+  const int nst = (dim == 2) ? 9 : 27, center = (nst - 1) / 2;
+  int nnode = 1;
+  for (int d = 0; d < dim; ++d) nnode *= nc + 1;
+  const int S = nc + 1;
+  double bb = 0.0;
+  for (int i = threadIdx.x; i < nnode; i += blockDim.x) {
+    u[i] = 0.0;
+    r[i] = rhs[i];
+    pv[i] = 0.0;
+    bb += rhs[i] * rhs[i];
+  }
+  const double bnorm2 = block_sum(bb, red);
+  const double stop = fmax(abs_tol * abs_tol, rel_tol * rel_tol * bnorm2);
+  double rnorm2 = bnorm2, rz_old = 1.0;
+  int it = 0;
+  while (rnorm2 > stop && it < max_iter) {
+    double rz = 0.0;
+    for (int i = threadIdx.x; i < nnode; i += blockDim.x) {
+      const double zi = r[i] / K[(size_t)i * nst + center];
+      z[i] = zi;
+      rz += r[i] * zi;
+    }
+    rz = block_sum(rz, red);
+    const double beta = (it == 0) ? 0.0 : rz / rz_old;
+    rz_old = rz;
+    for (int i = threadIdx.x; i < nnode; i += blockDim.x) pv[i] = z[i] + beta * pv[i];
+    __syncthreads();
+    double pq = 0.0;
+    for (int i = threadIdx.x; i < nnode; i += blockDim.x) {
+      int cn[3] = {0, 0, 0}, rr = i;
+      for (int d = 0; d < dim; ++d) {
+        cn[d] = rr % S;
+        rr /= S;
+      }
+      double s = 0.0;
+      for (int e = 0; e < nst; ++e) {
+        int er = e, off = 0, mul = 1;
+        bool ok = true;
+        for (int d = 0; d < dim; ++d) {
+          const int dd = er % 3 - 1;
+          er /= 3;
+          const int m = cn[d] + dd;
+          if (m < 0 || m >= S) ok = false;
+          off += dd * mul;
+          mul *= S;
+        }
+        if (ok) s = fma(K[(size_t)i * nst + e], pv[i + off], s);
+      }
+      qv[i] = s;
+      pq += pv[i] * s;
+    }
+    pq = block_sum(pq, red);
+    const double alpha = rz / pq;
+    double rr2 = 0.0;
+    for (int i = threadIdx.x; i < nnode; i += blockDim.x) {
+      u[i] = fma(alpha, pv[i], u[i]);
+      const double ri = fma(-alpha, qv[i], r[i]);
+      r[i] = ri;
+      rr2 += ri * ri;
+    }
+    rnorm2 = block_sum(rr2, red);
+    ++it;
+  }
+  if (threadIdx.x == 0) {
+    *iters_out = (rnorm2 > stop) ? -it : it;
+    *res_out = sqrt(rnorm2);
+  }
+}
+
+__global__ void coarse_weights_kernel(int dim, int nc, const double* __restrict__ u,
+                                      double* __restrict__ w) {
+  const int nv = 1 << dim;
+  int ncell = 1;
+  for (int d = 0; d < dim; ++d) ncell *= nc;
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= ncell * nv) return;
+  const int cell = idx / nv, v = idx % nv;
+  int r = cell, node = 0, mul = 1;
+  for (int d = 0; d < dim; ++d) {
+    node += (r % nc + ((v >> d) & 1)) * mul;
+    r /= nc;
+    mul *= nc + 1;
+  }
+  w[idx] = u[node];
+}
+
+}  // namespace
+
+// ===========================================================================
+// Host side: handle, error reporting, launches
+// ===========================================================================
+namespace {
+
+thread_local std::string g_last_error;
+
+msfec_status fail(msfec_status st, const std::string& msg) {
+  g_last_error = msg;
+  return st;
+}
+
+#define MSFEC_HIP(call)                                                              \
+  do {                                                                               \
+    hipError_t e_ = (call);                                                          \
+    if (e_ != hipSuccess)                                                            \
+      return fail(MSFEC_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+int ipow(int b, int e) {
+  int r = 1;
+  while (e-- > 0) r *= b;
+  return r;
+}
+
+}  // namespace
+
+struct msfec_q_batch {
+  msfec_q_params p{};
+  QConf conf{};
+  int n_cells = 0;
+  int bw = 0;
+  size_t coef_count = 0, rhs_count = 0, reac_count = 0;
+  // device buffers
+  double* d_verts = nullptr;
+  double* d_coef = nullptr;
+  double* d_f = nullptr;
+  double* d_reac = nullptr;
+  double* d_slo = nullptr;
+  double* d_F = nullptr;
+  double* d_rhs = nullptr;
+  double* d_L = nullptr;
+  double* d_W = nullptr;
+  double* d_X = nullptr;
+  double* d_psi = nullptr;
+  double* d_M = nullptr;
+  double* d_b = nullptr;
+  int* d_status = nullptr;
+  int* d_inode = nullptr;
+  unsigned* d_imask = nullptr;
+  size_t bytes = 0;
+  bool coef_set = false, rhs_set = false, reac_set = false, computed = false;
+  hipStream_t stream = nullptr;
+  // per-compute kernel events (MSFEC_FLAG_TIMING): sets recorded since the
+  // last msfec_q_kernel_times call, reused across calls
+  std::vector<std::array<hipEvent_t, 4>> ev_sets;
+  size_t ev_used = 0;
+  // coarse scratch
+  double* d_cK = nullptr;
+  double* d_crhs = nullptr;
+  double* d_cvec = nullptr;
+  int* d_cit = nullptr;
+  int coarse_nc = -1;
+};
+
+namespace {
+
+template <typename T>
+msfec_status dalloc(msfec_q_batch* h, T** p, size_t count) {
+  if (count == 0) count = 1;
+  MSFEC_HIP(hipMalloc((void**)p, count * sizeof(T)));
+  h->bytes += count * sizeof(T);
+  return MSFEC_OK;
+}
+
+void free_all(msfec_q_batch* h) {
+  void* ptrs[] = {h->d_verts, h->d_coef, h->d_f,     h->d_reac,  h->d_slo,   h->d_F,
+                  h->d_rhs,   h->d_L,    h->d_W,     h->d_X,     h->d_psi,   h->d_M,
+                  h->d_b,     h->d_status, h->d_inode, h->d_imask, h->d_cK,  h->d_crhs,
+                  h->d_cvec,  h->d_cit};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  for (auto& set : h->ev_sets)
+    for (auto& e : set)
+      if (e) (void)hipEventDestroy(e);
+  h->ev_sets.clear();
+}
+
+msfec_status copy_in(double* dst, const double* src, size_t count, int loc) {
+  if (!src) return fail(MSFEC_ERR_INVALID, "null data pointer");
+  MSFEC_HIP(hipMemcpy(dst, src, count * sizeof(double),
+                      loc == MSFEC_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice));
+  return MSFEC_OK;
+}
+
+msfec_status copy_out(double* dst, const double* src, size_t count, int loc) {
+  MSFEC_HIP(hipMemcpy(dst, src, count * sizeof(double),
+                      loc == MSFEC_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost));
+  return MSFEC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* msfec_last_error(void) { return g_last_error.c_str(); }
+
+const char* msfec_version(void) { return "msfec-mi355x 0.1 (band-window FP64, gfx950)"; }
+
+msfec_status msfec_q_create(const msfec_q_params* params, int n_cells, const double* vertices,
+                            int vertices_loc, msfec_q_handle* out) {
+  if (!params || !out || !vertices) return fail(MSFEC_ERR_INVALID, "null argument");
+  *out = nullptr;
+  const msfec_q_params& p = *params;
+  if (p.dim != 2 && p.dim != 3) return fail(MSFEC_ERR_INVALID, "dim must be 2 or 3");
+  if (n_cells <= 0) return fail(MSFEC_ERR_INVALID, "n_cells must be positive");
+  if (p.n_refine_local < 1) return fail(MSFEC_ERR_INVALID, "n_refine_local must be >= 1");
+  if (p.coef_mode != MSFEC_COEF_SCALAR_CELL && p.coef_mode != MSFEC_COEF_TENSOR_QP)
+    return fail(MSFEC_ERR_INVALID, "unknown coef_mode");
+  if (p.rhs_mode < MSFEC_RHS_CONST || p.rhs_mode > MSFEC_RHS_QP)
+    return fail(MSFEC_ERR_INVALID, "unknown rhs_mode");
+  if (p.reaction_mode < MSFEC_REACTION_NONE || p.reaction_mode > MSFEC_REACTION_CELL)
+    return fail(MSFEC_ERR_INVALID, "unknown reaction_mode");
+  if (p.solver != MSFEC_SOLVER_BAND_FP64) return fail(MSFEC_ERR_INVALID, "unknown solver");
+  const int N = 1 << p.n_refine_local;
+  const int ni = N - 1;
+  const int bw = (p.dim == 2) ? ni + 1 : ni * ni + ni + 1;
+  if (bw + 1 > kWin)
+    return fail(MSFEC_ERR_UNSUPPORTED,
+                "interior half-bandwidth " + std::to_string(bw) +
+                    " exceeds the 63 of the band solver (3-D: n_refine_local <= 3, 2-D: <= 5)");
+  // validate cells on the host (axis-aligned boxes)
+  const int nv = 1 << p.dim;
+  std::vector<double> hv((size_t)n_cells * nv * p.dim);
+  if (vertices_loc == MSFEC_DEVICE) {
+    if (hipMemcpy(hv.data(), vertices, hv.size() * sizeof(double), hipMemcpyDeviceToHost) !=
+        hipSuccess)
+      return fail(MSFEC_ERR_HIP, "copy of device vertices failed");
+  } else {
+    std::memcpy(hv.data(), vertices, hv.size() * sizeof(double));
+  }
+  for (int c = 0; c < n_cells; ++c) {
+    const double* v = hv.data() + (size_t)c * nv * p.dim;
+    for (int d = 0; d < p.dim; ++d) {
+      const double lo = v[d], hi = v[(nv - 1) * p.dim + d];
+      if (!(hi > lo) || !std::isfinite(lo) || !std::isfinite(hi))
+        return fail(MSFEC_ERR_INVALID, "cell " + std::to_string(c) + " is degenerate");
+      for (int k = 0; k < nv; ++k) {
+        const double want = ((k >> d) & 1) ? hi : lo;
+        if (std::fabs(v[k * p.dim + d] - want) > 1e-12 * (1.0 + std::fabs(want)))
+          return fail(MSFEC_ERR_UNSUPPORTED,
+                      "cell " + std::to_string(c) + " is not an axis-aligned box");
+      }
+    }
+  }
+
+  msfec_q_batch* h = new (std::nothrow) msfec_q_batch();
+  if (!h) return fail(MSFEC_ERR_INVALID, "out of host memory");
+  h->p = p;
+  h->n_cells = n_cells;
+  h->bw = bw;
+  QConf& c = h->conf;
+  c.dim = p.dim;
+  c.N = N;
+  c.S = N + 1;
+  c.nn = ipow(c.S, p.dim);
+  c.nfc = ipow(N, p.dim);
+  c.nv = nv;
+  c.ni = ni;
+  c.n = ipow(ni, p.dim);
+  c.nst = ipow(3, p.dim);
+  c.center = (c.nst - 1) / 2;
+  c.nlow = c.center + 1;
+  c.coef_mode = p.coef_mode;
+  c.rhs_mode = p.rhs_mode;
+  c.reaction_mode = p.reaction_mode;
+  c.rhs_const = p.rhs_const;
+  c.reaction_const = p.reaction_const;
+  c.off_full[0] = 0;
+  c.off_int[0] = 0;
+  c.dir_low[0] = -1;
+  for (int e = 0; e < c.center; ++e) {
+    int r = e, of = 0, oi = 0, mf = 1, mi = 1;
+    for (int d = 0; d < p.dim; ++d) {
+      const int dd = r % 3 - 1;
+      r /= 3;
+      of += dd * mf;
+      oi += dd * mi;
+      mf *= c.S;
+      mi *= ni;
+    }
+    c.off_full[1 + e] = -of;
+    c.off_int[1 + e] = -oi;
+    c.dir_low[1 + e] = e;
+  }
+  const int nq = nv;
+  h->coef_count = (p.coef_mode == MSFEC_COEF_SCALAR_CELL)
+                      ? (size_t)n_cells * c.nfc
+                      : (size_t)n_cells * c.nfc * nq * (p.dim == 2 ? 3 : 6);
+  h->rhs_count = (p.rhs_mode == MSFEC_RHS_CONST) ? 0
+                 : (p.rhs_mode == MSFEC_RHS_CELL) ? (size_t)n_cells * c.nfc
+                                                  : (size_t)n_cells * c.nfc * nq;
+  h->reac_count = (p.reaction_mode == MSFEC_REACTION_CELL) ? (size_t)n_cells * c.nfc : 0;
+
+  msfec_status st = MSFEC_OK;
+#define TRY(x)              \
+  do {                      \
+    st = (x);               \
+    if (st != MSFEC_OK) {   \
+      free_all(h);          \
+      delete h;             \
+      return st;            \
+    }                       \
+  } while (0)
+  TRY(dalloc(h, &h->d_verts, hv.size()));
+  TRY(dalloc(h, &h->d_coef, h->coef_count));
+  TRY(dalloc(h, &h->d_f, h->rhs_count));
+  TRY(dalloc(h, &h->d_reac, h->reac_count));
+  TRY(dalloc(h, &h->d_slo, (size_t)n_cells * c.nn * c.nlow));
+  TRY(dalloc(h, &h->d_F, (size_t)n_cells * c.nn));
+  TRY(dalloc(h, &h->d_rhs, (size_t)n_cells * c.n * kMaxNb));
+  TRY(dalloc(h, &h->d_L, (size_t)n_cells * c.n * kWin));
+  TRY(dalloc(h, &h->d_W, (size_t)n_cells * c.n * kMaxNb));
+  TRY(dalloc(h, &h->d_X, (size_t)n_cells * c.n * kMaxNb));
+  TRY(dalloc(h, &h->d_psi, (size_t)n_cells * nv * c.nn));
+  TRY(dalloc(h, &h->d_M, (size_t)n_cells * nv * nv));
+  TRY(dalloc(h, &h->d_b, (size_t)n_cells * nv));
+  TRY(dalloc(h, &h->d_status, (size_t)n_cells));
+  TRY(dalloc(h, &h->d_inode, (size_t)c.n));
+  TRY(dalloc(h, &h->d_imask, (size_t)c.n));
+  // interior index -> node id and neighbour masks
+  std::vector<int> inode(c.n);
+  std::vector<unsigned> imask(c.n);
+  for (int i = 0; i < c.n; ++i) {
+    int r = i, node = 0, mul = 1, co[3] = {0, 0, 0};
+    for (int d = 0; d < p.dim; ++d) {
+      co[d] = r % ni;
+      r /= ni;
+      node += (co[d] + 1) * mul;
+      mul *= c.S;
+    }
+    inode[i] = node;
+    unsigned msk = 1u;  // diagonal
+    for (int e = 0; e < c.center; ++e) {
+      int rr = e;
+      bool ok = true;
+      for (int d = 0; d < p.dim; ++d) {
+        const int dd = rr % 3 - 1;
+        rr /= 3;
+        const int m = co[d] + dd;
+        if (m < 0 || m >= ni) ok = false;
+      }
+      if (ok) msk |= 1u << (1 + e);
+    }
+    imask[i] = msk;
+  }
+  if (hipMemcpy(h->d_inode, inode.data(), c.n * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(h->d_imask, imask.data(), c.n * sizeof(unsigned), hipMemcpyHostToDevice) !=
+          hipSuccess ||
+      hipMemcpy(h->d_verts, hv.data(), hv.size() * sizeof(double), hipMemcpyHostToDevice) !=
+          hipSuccess) {
+    free_all(h);
+    delete h;
+    return fail(MSFEC_ERR_HIP, "upload of tables failed");
+  }
+#undef TRY
+  *out = h;
+  return MSFEC_OK;
+}
+
+msfec_status msfec_q_get_info(msfec_q_handle h, msfec_q_info* info) {
+  if (!h || !info) return fail(MSFEC_ERR_INVALID, "null argument");
+  info->n_cells = h->n_cells;
+  info->dim = h->conf.dim;
+  info->n_refine_local = h->p.n_refine_local;
+  info->n_basis = h->conf.nv;
+  info->n_fine_nodes = h->conf.nn;
+  info->n_fine_cells = h->conf.nfc;
+  info->n_interior = h->conf.n;
+  info->bandwidth = h->bw;
+  info->coef_count = h->coef_count;
+  info->rhs_count = h->rhs_count;
+  info->reaction_count = h->reac_count;
+  info->device_bytes = h->bytes;
+  return MSFEC_OK;
+}
+
+msfec_status msfec_q_set_coefficient(msfec_q_handle h, const double* data, size_t count, int loc) {
+  if (!h) return fail(MSFEC_ERR_INVALID, "null handle");
+  if (count != h->coef_count)
+    return fail(MSFEC_ERR_INVALID, "coefficient count " + std::to_string(count) + " != expected " +
+                                       std::to_string(h->coef_count));
+  msfec_status st = copy_in(h->d_coef, data, count, loc);
+  if (st == MSFEC_OK) h->coef_set = true;
+  return st;
+}
+
+msfec_status msfec_q_set_rhs(msfec_q_handle h, const double* data, size_t count, int loc) {
+  if (!h) return fail(MSFEC_ERR_INVALID, "null handle");
+  if (h->rhs_count == 0) return fail(MSFEC_ERR_STATE, "rhs_mode is constant; no data expected");
+  if (count != h->rhs_count)
+    return fail(MSFEC_ERR_INVALID, "rhs count mismatch (expected " + std::to_string(h->rhs_count) + ")");
+  msfec_status st = copy_in(h->d_f, data, count, loc);
+  if (st == MSFEC_OK) h->rhs_set = true;
+  return st;
+}
+
+msfec_status msfec_q_set_reaction(msfec_q_handle h, const double* data, size_t count, int loc) {
+  if (!h) return fail(MSFEC_ERR_INVALID, "null handle");
+  if (h->reac_count == 0) return fail(MSFEC_ERR_STATE, "reaction_mode takes no per-cell data");
+  if (count != h->reac_count)
+    return fail(MSFEC_ERR_INVALID, "reaction count mismatch (expected " + std::to_string(h->reac_count) + ")");
+  msfec_status st = copy_in(h->d_reac, data, count, loc);
+  if (st == MSFEC_OK) h->reac_set = true;
+  return st;
+}
+
+msfec_status msfec_q_compute(msfec_q_handle h, void* stream) {
+  if (!h) return fail(MSFEC_ERR_INVALID, "null handle");
+  if (!h->coef_set) return fail(MSFEC_ERR_STATE, "coefficient not set");
+  if (h->rhs_count && !h->rhs_set) return fail(MSFEC_ERR_STATE, "rhs data not set");
+  if (h->reac_count && !h->reac_set) return fail(MSFEC_ERR_STATE, "reaction data not set");
+  hipStream_t s = (hipStream_t)stream;
+  h->stream = s;
+  const QConf& c = h->conf;
+  const bool tm = (h->p.flags & MSFEC_FLAG_TIMING) != 0;
+  hipEvent_t* ev = nullptr;
+  if (tm) {
+    if (h->ev_used == h->ev_sets.size()) {
+      if (h->ev_sets.size() >= 4096) return fail(MSFEC_ERR_STATE, "too many timed computes pending");
+      std::array<hipEvent_t, 4> set{};
+      for (auto& e : set) MSFEC_HIP(hipEventCreate(&e));
+      h->ev_sets.push_back(set);
+    }
+    ev = h->ev_sets[h->ev_used++].data();
+  }
+  const size_t cstride = h->coef_count / h->n_cells;
+  const size_t fstride = h->rhs_count ? h->rhs_count / h->n_cells : 0;
+  const size_t rstride = h->reac_count ? h->reac_count / h->n_cells : 0;
+  if (tm) MSFEC_HIP(hipEventRecord(ev[0], s));
+  hipLaunchKernelGGL(q_assemble_kernel, dim3(h->n_cells), dim3(256), 0, s, c, h->d_verts, h->d_coef,
+                     cstride, h->rhs_count ? h->d_f : nullptr, fstride,
+                     h->reac_count ? h->d_reac : nullptr, rstride, h->d_slo, h->d_F, h->d_rhs);
+  MSFEC_HIP(hipGetLastError());
+  if (tm) MSFEC_HIP(hipEventRecord(ev[1], s));
+  BandArgs a;
+  a.n = c.n;
+  a.nlow = c.nlow;
+  a.nn = c.nn;
+  for (int t = 0; t < kMaxLow; ++t) a.off_int[t] = c.off_int[t];
+  a.Slo = h->d_slo;
+  a.Rhs = h->d_rhs;
+  a.inode = h->d_inode;
+  a.imask = h->d_imask;
+  a.Lcol = h->d_L;
+  a.Wf = h->d_W;
+  a.X = h->d_X;
+  a.status = h->d_status;
+  hipLaunchKernelGGL(q_band_kernel, dim3(h->n_cells), dim3(64), 0, s, a);
+  MSFEC_HIP(hipGetLastError());
+  if (tm) MSFEC_HIP(hipEventRecord(ev[2], s));
+  const size_t shm = (size_t)(2 * c.nv * c.nn + 4 * 72) * sizeof(double);
+  if (shm > 160 * 1024) return fail(MSFEC_ERR_UNSUPPORTED, "element kernel LDS too large");
+  hipLaunchKernelGGL(q_element_kernel, dim3(h->n_cells), dim3(256), shm, s, c, h->d_slo, h->d_F,
+                     h->d_X, h->d_psi, h->d_M, h->d_b);
+  MSFEC_HIP(hipGetLastError());
+  if (tm) MSFEC_HIP(hipEventRecord(ev[3], s));
+  h->computed = true;
+  return MSFEC_OK;
+}
+
+msfec_status msfec_q_synchronize(msfec_q_handle h) {
+  if (!h) return fail(MSFEC_ERR_INVALID, "null handle");
+  MSFEC_HIP(hipStreamSynchronize(h->stream));
+  return MSFEC_OK;
+}
+
+msfec_status msfec_q_kernel_times(msfec_q_handle h, float* ms, int max, int* n) {
+  if (!h || !ms) return fail(MSFEC_ERR_INVALID, "null argument");
+  if (h->ev_used == 0) return fail(MSFEC_ERR_STATE, "no timed compute (set MSFEC_FLAG_TIMING)");
+  double acc[3] = {0.0, 0.0, 0.0};
+  for (size_t c = 0; c < h->ev_used; ++c) {
+    MSFEC_HIP(hipEventSynchronize(h->ev_sets[c][3]));
+    for (int i = 0; i < 3; ++i) {
+      float t = 0.f;
+      MSFEC_HIP(hipEventElapsedTime(&t, h->ev_sets[c][i], h->ev_sets[c][i + 1]));
+      acc[i] += t;
+    }
+  }
+  int k = 0;
+  for (; k < 3 && k < max; ++k) ms[k] = (float)(acc[k] / (double)h->ev_used);
+  if (n) *n = k;
+  h->ev_used = 0;
+  return MSFEC_OK;
+}
+
+msfec_status msfec_q_get_element(msfec_q_handle h, double* M, double* b, int loc) {
+  if (!h) return fail(MSFEC_ERR_INVALID, "null handle");
+  if (!h->computed) return fail(MSFEC_ERR_STATE, "compute not called");
+  MSFEC_HIP(hipStreamSynchronize(h->stream));
+  const size_t nv = h->conf.nv;
+  if (M) {
+    msfec_status st = copy_out(M, h->d_M, (size_t)h->n_cells * nv * nv, loc);
+    if (st) return st;
+  }
+  if (b) {
+    msfec_status st = copy_out(b, h->d_b, (size_t)h->n_cells * nv, loc);
+    if (st) return st;
+  }
+  return MSFEC_OK;
+}
+
+msfec_status msfec_q_get_basis(msfec_q_handle h, double* Psi, int loc) {
+  if (!h || !Psi) return fail(MSFEC_ERR_INVALID, "null argument");
+  if (!h->computed) return fail(MSFEC_ERR_STATE, "compute not called");
+  MSFEC_HIP(hipStreamSynchronize(h->stream));
+  return copy_out(Psi, h->d_psi, (size_t)h->n_cells * h->conf.nv * h->conf.nn, loc);
+}
+
+msfec_status msfec_q_get_cell_status(msfec_q_handle h, int* status) {
+  if (!h || !status) return fail(MSFEC_ERR_INVALID, "null argument");
+  if (!h->computed) return fail(MSFEC_ERR_STATE, "compute not called");
+  MSFEC_HIP(hipStreamSynchronize(h->stream));
+  MSFEC_HIP(hipMemcpy(status, h->d_status, h->n_cells * sizeof(int), hipMemcpyDeviceToHost));
+  return MSFEC_OK;
+}
+
+msfec_status msfec_q_reconstruct(msfec_q_handle h, const double* weights, int weights_loc,
+                                 double* u_fine, int out_loc) {
+  if (!h || !weights || !u_fine) return fail(MSFEC_ERR_INVALID, "null argument");
+  if (!h->computed) return fail(MSFEC_ERR_STATE, "compute not called");
+  const size_t nw = (size_t)h->n_cells * h->conf.nv, nu = (size_t)h->n_cells * h->conf.nn;
+  double* dw = nullptr;
+  double* du = nullptr;
+  const double* wsrc = weights;
+  if (weights_loc != MSFEC_DEVICE) {
+    MSFEC_HIP(hipMalloc((void**)&dw, nw * sizeof(double)));
+    MSFEC_HIP(hipMemcpyAsync(dw, weights, nw * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    wsrc = dw;
+  }
+  double* dst = u_fine;
+  if (out_loc != MSFEC_DEVICE) {
+    MSFEC_HIP(hipMalloc((void**)&du, nu * sizeof(double)));
+    dst = du;
+  }
+  const int threads = 256;
+  const unsigned blocks = (unsigned)((nu + threads - 1) / threads);
+  hipLaunchKernelGGL(q_reconstruct_kernel, dim3(blocks), dim3(threads), 0, h->stream, h->conf.nv,
+                     h->conf.nn, h->n_cells, h->d_psi, wsrc, dst);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  if (e == hipSuccess && du) e = hipMemcpy(u_fine, du, nu * sizeof(double), hipMemcpyDeviceToHost);
+  if (dw) (void)hipFree(dw);
+  if (du) (void)hipFree(du);
+  if (e != hipSuccess) return fail(MSFEC_ERR_HIP, std::string("reconstruct: ") + hipGetErrorString(e));
+  return MSFEC_OK;
+}
+
+msfec_status msfec_coarse_solve(msfec_q_handle h, int nc, double abs_tol, double rel_tol,
+                                int max_iter, double* u, int u_loc, double* weights,
+                                int weights_loc, int* iterations, double* residual) {
+  if (!h) return fail(MSFEC_ERR_INVALID, "null handle");
+  if (!h->computed) return fail(MSFEC_ERR_STATE, "compute not called");
+  const int dim = h->conf.dim, nv = h->conf.nv;
+  if (nc <= 0 || ipow(nc, dim) != h->n_cells)
+    return fail(MSFEC_ERR_INVALID, "nc^dim must equal the number of cells of the batch");
+  const int nnode = ipow(nc + 1, dim), nst = ipow(3, dim);
+  if (h->coarse_nc != nc) {
+    if (h->d_cK) (void)hipFree(h->d_cK);
+    if (h->d_crhs) (void)hipFree(h->d_crhs);
+    if (h->d_cvec) (void)hipFree(h->d_cvec);
+    if (h->d_cit) (void)hipFree(h->d_cit);
+    h->d_cK = h->d_crhs = h->d_cvec = nullptr;
+    h->d_cit = nullptr;
+    MSFEC_HIP(hipMalloc((void**)&h->d_cK, (size_t)nnode * nst * sizeof(double)));
+    MSFEC_HIP(hipMalloc((void**)&h->d_crhs, (size_t)nnode * sizeof(double)));
+    MSFEC_HIP(hipMalloc((void**)&h->d_cvec, ((size_t)5 * nnode + 1 + (size_t)h->n_cells * nv) * sizeof(double)));
+    MSFEC_HIP(hipMalloc((void**)&h->d_cit, sizeof(int)));
+    h->coarse_nc = nc;
+  }
+  double* du = h->d_cvec;
+  double* r = du + nnode;
+  double* z = r + nnode;
+  double* pv = z + nnode;
+  double* qv = pv + nnode;
+  double* res = qv + nnode;
+  double* dw = res + 1;
+  hipLaunchKernelGGL(coarse_assemble_kernel, dim3((nnode + 255) / 256), dim3(256), 0, h->stream, dim,
+                     nc, h->d_M, h->d_b, h->d_cK, h->d_crhs);
+  MSFEC_HIP(hipGetLastError());
+  hipLaunchKernelGGL(coarse_cg_kernel, dim3(1), dim3(1024), 0, h->stream, dim, nc, h->d_cK,
+                     h->d_crhs, du, r, z, pv, qv, abs_tol, rel_tol, max_iter, h->d_cit, res);
+  MSFEC_HIP(hipGetLastError());
+  const int nw = h->n_cells * nv;
+  hipLaunchKernelGGL(coarse_weights_kernel, dim3((nw + 255) / 256), dim3(256), 0, h->stream, dim, nc,
+                     du, dw);
+  MSFEC_HIP(hipGetLastError());
+  MSFEC_HIP(hipStreamSynchronize(h->stream));
+  int it = 0;
+  double rr = 0.0;
+  MSFEC_HIP(hipMemcpy(&it, h->d_cit, sizeof(int), hipMemcpyDeviceToHost));
+  MSFEC_HIP(hipMemcpy(&rr, res, sizeof(double), hipMemcpyDeviceToHost));
+  if (iterations) *iterations = it;
+  if (residual) *residual = rr;
+  if (u) {
+    msfec_status st = copy_out(u, du, nnode, u_loc);
+    if (st) return st;
+  }
+  if (weights) {
+    msfec_status st = copy_out(weights, dw, (size_t)nw, weights_loc);
+    if (st) return st;
+  }
+  if (it < 0) return fail(MSFEC_ERR_CELL, "coarse CG did not converge");
+  return MSFEC_OK;
+}
+
+void msfec_q_destroy(msfec_q_handle h) {
+  if (!h) return;
+  free_all(h);
+  delete h;
+}
+
+}  // extern "C"

@@ -1,0 +1,11 @@
+"""MI355X multiscale basis construction (MsFEC Q1 pairing), Python face.
+
+The compute path is libmsfec_hip.so (hand-written HIP for gfx950) behind the
+C-ABI in include/msfec.h; this package only binds it (ctypes) and generates
+seeded synthetic inputs.
+"""
+from . import _lib
+from ._lib import MsfecError
+from .qbasis import QBasisBatch
+
+__all__ = ["QBasisBatch", "MsfecError", "_lib"]

@@ -1,0 +1,117 @@
+"""ctypes binding of libmsfec_hip.so (the C-ABI declared in include/msfec.h).
+
+The library is built in-tree next to this file by ``msfec_amd.build.build()``.
+There is no fallback: if the shared object is missing or fails to load, every
+entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_NAME = "libmsfec_hip.so"
+LIB_PATH = os.path.join(HERE, LIB_NAME)
+
+# C enums (include/msfec.h)
+OK, ERR_INVALID, ERR_HIP, ERR_STATE, ERR_UNSUPPORTED, ERR_CELL = 0, 1, 2, 3, 4, 5
+HOST, DEVICE = 0, 1
+COEF_SCALAR_CELL, COEF_TENSOR_QP = 0, 1
+RHS_CONST, RHS_CELL, RHS_QP = 0, 1, 2
+REACTION_NONE, REACTION_CONST, REACTION_CELL = 0, 1, 2
+SOLVER_BAND_FP64 = 0
+CELL_OK, CELL_NOT_SPD, CELL_NONFINITE = 0, 2, 3
+FLAG_TIMING = 1
+
+#: every symbol include/msfec.h declares
+EXPORTED = (
+    "msfec_q_create", "msfec_q_set_coefficient", "msfec_q_set_rhs", "msfec_q_set_reaction",
+    "msfec_q_compute", "msfec_q_synchronize", "msfec_q_get_element", "msfec_q_get_basis",
+    "msfec_q_get_cell_status", "msfec_q_reconstruct", "msfec_q_get_info",
+    "msfec_q_kernel_times", "msfec_q_destroy", "msfec_coarse_solve", "msfec_last_error",
+    "msfec_version",
+)
+
+
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("dim", ctypes.c_int),
+        ("n_refine_local", ctypes.c_int),
+        ("coef_mode", ctypes.c_int),
+        ("rhs_mode", ctypes.c_int),
+        ("rhs_const", ctypes.c_double),
+        ("reaction_mode", ctypes.c_int),
+        ("reaction_const", ctypes.c_double),
+        ("solver", ctypes.c_int),
+        ("flags", ctypes.c_int),
+    ]
+
+
+class Info(ctypes.Structure):
+    _fields_ = [
+        ("n_cells", ctypes.c_int),
+        ("dim", ctypes.c_int),
+        ("n_refine_local", ctypes.c_int),
+        ("n_basis", ctypes.c_int),
+        ("n_fine_nodes", ctypes.c_int),
+        ("n_fine_cells", ctypes.c_int),
+        ("n_interior", ctypes.c_int),
+        ("bandwidth", ctypes.c_int),
+        ("coef_count", ctypes.c_size_t),
+        ("rhs_count", ctypes.c_size_t),
+        ("reaction_count", ctypes.c_size_t),
+        ("device_bytes", ctypes.c_size_t),
+    ]
+
+
+class MsfecError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"msfec status {status}: {msg}")
+        self.status = status
+
+
+_lib = None
+
+
+def load(path: str | None = None):
+    """Load (once) and return the ctypes library object."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = path or LIB_PATH
+    if not os.path.exists(path):
+        raise RuntimeError(
+            f"{path} is missing: build it with msfec_amd.build.build() (hipcc, gfx950); "
+            "there is no CPU fallback")
+    L = ctypes.CDLL(path)
+    vp, dp, ip = ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)
+    L.msfec_q_create.argtypes = [ctypes.POINTER(Params), ctypes.c_int, vp, ctypes.c_int,
+                                 ctypes.POINTER(vp)]
+    L.msfec_q_set_coefficient.argtypes = [vp, vp, ctypes.c_size_t, ctypes.c_int]
+    L.msfec_q_set_rhs.argtypes = [vp, vp, ctypes.c_size_t, ctypes.c_int]
+    L.msfec_q_set_reaction.argtypes = [vp, vp, ctypes.c_size_t, ctypes.c_int]
+    L.msfec_q_compute.argtypes = [vp, vp]
+    L.msfec_q_synchronize.argtypes = [vp]
+    L.msfec_q_get_element.argtypes = [vp, vp, vp, ctypes.c_int]
+    L.msfec_q_get_basis.argtypes = [vp, vp, ctypes.c_int]
+    L.msfec_q_get_cell_status.argtypes = [vp, ip]
+    L.msfec_q_reconstruct.argtypes = [vp, vp, ctypes.c_int, vp, ctypes.c_int]
+    L.msfec_q_get_info.argtypes = [vp, ctypes.POINTER(Info)]
+    L.msfec_q_kernel_times.argtypes = [vp, ctypes.POINTER(ctypes.c_float), ctypes.c_int, ip]
+    L.msfec_q_destroy.argtypes = [vp]
+    L.msfec_q_destroy.restype = None
+    L.msfec_coarse_solve.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                     ctypes.c_int, vp, ctypes.c_int, vp, ctypes.c_int, ip, dp]
+    L.msfec_last_error.restype = ctypes.c_char_p
+    L.msfec_version.restype = ctypes.c_char_p
+    for name in EXPORTED:
+        fn = getattr(L, name)
+        if name not in ("msfec_q_destroy", "msfec_last_error", "msfec_version"):
+            fn.restype = ctypes.c_int
+    _lib = L
+    return L
+
+
+def check(status: int):
+    if status != OK:
+        raise MsfecError(status, load().msfec_last_error().decode(errors="replace"))

@@ -1,0 +1,177 @@
+"""ctypes front end of the FP64 CPU oracle (oracle/msfec_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, ``__graft_entry__.smoke()`` and
+the cpu_baseline leg of bench.py, always as the checker (or, for the CPU
+baseline, as the thing timed beside the GPU path).  The product library never
+imports it.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+COEF_SCALAR_CELL, COEF_TENSOR_QP, COEF_ANALYTIC = 0, 1, 2
+RHS_CONST, RHS_CELL, RHS_QP = 0, 1, 2
+REACTION_NONE, REACTION_CONST, REACTION_CELL = 0, 1, 2
+SOLVER_DIRECT, SOLVER_CG_SSOR = 0, 1
+
+
+class Problem(ctypes.Structure):
+    _fields_ = [
+        ("dim", ctypes.c_int),
+        ("n_refine", ctypes.c_int),
+        ("coef_mode", ctypes.c_int),
+        ("rhs_mode", ctypes.c_int),
+        ("rhs_const", ctypes.c_double),
+        ("reaction_mode", ctypes.c_int),
+        ("reaction_const", ctypes.c_double),
+        ("freq", ctypes.c_int * 3),
+        ("scale", ctypes.c_double * 3),
+        ("alpha", ctypes.c_double * 3),
+        ("rotate", ctypes.c_int),
+        ("cg_maxit", ctypes.c_int),
+        ("cg_tol", ctypes.c_double),
+        ("ssor_omega", ctypes.c_double),
+    ]
+
+
+def build(force: bool = False) -> str:
+    """Compile liboracle.so with the Makefile next to this file."""
+    if force or not os.path.exists(LIB_PATH):
+        subprocess.run(["make", "-s", "-C", HERE, "-B" if force else "all"], check=True)
+    return LIB_PATH
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        dp = ctypes.POINTER(ctypes.c_double)
+        ip = ctypes.POINTER(ctypes.c_int)
+        pp = ctypes.POINTER(Problem)
+        L.oq_basis_batch.argtypes = [pp, ctypes.c_int, dp, dp, dp, dp, ctypes.c_int,
+                                     ctypes.c_int, dp, dp, dp, ip, ip]
+        L.oq_basis_batch.restype = ctypes.c_int
+        L.oq_basis_q1_coeffs.argtypes = [ctypes.c_int, dp, dp]
+        L.oq_basis_q1_coeffs.restype = ctypes.c_int
+        L.oq_basis_q1_value.argtypes = [ctypes.c_int, dp, ctypes.c_int, dp]
+        L.oq_basis_q1_value.restype = ctypes.c_double
+        L.oq_diffusion_A.argtypes = [pp, dp, dp]
+        L.oq_diffusion_A.restype = None
+        L.oq_element_matrix.argtypes = [ctypes.c_int, dp, dp, dp, ctypes.c_double, dp, dp]
+        L.oq_element_matrix.restype = None
+        L.oq_coarse_solve.argtypes = [ctypes.c_int, ctypes.c_int, dp, dp, ctypes.c_double,
+                                      ctypes.c_int, dp]
+        L.oq_coarse_solve.restype = ctypes.c_int
+        L.oq_reconstruct.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, dp, dp]
+        L.oq_reconstruct.restype = None
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    if a is None:
+        return ctypes.POINTER(ctypes.c_double)()
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+def make_problem(dim, n_refine, coef_mode=COEF_SCALAR_CELL, rhs_mode=RHS_CONST,
+                 rhs_const=1.0, reaction_mode=REACTION_NONE, reaction_const=0.0,
+                 freq=(0, 0, 0), scale=(1.0, 1.0, 1.0), alpha=(1.0, 1.0, 1.0),
+                 rotate=False, cg_maxit=1000, cg_tol=1e-12, ssor_omega=1.6):
+    p = Problem()
+    p.dim, p.n_refine, p.coef_mode, p.rhs_mode = dim, n_refine, coef_mode, rhs_mode
+    p.rhs_const, p.reaction_mode, p.reaction_const = rhs_const, reaction_mode, reaction_const
+    p.freq[:] = list(freq)
+    p.scale[:] = list(scale)
+    p.alpha[:] = list(alpha)
+    p.rotate = int(bool(rotate))
+    p.cg_maxit, p.cg_tol, p.ssor_omega = cg_maxit, cg_tol, ssor_omega
+    return p
+
+
+def basis_batch(prob: Problem, verts, coef=None, f=None, reaction=None,
+                solver=SOLVER_DIRECT, n_threads=0, want_psi=True):
+    """Run the oracle on a batch of cells. Returns dict(M, b, Psi, iters, status)."""
+    dim, nref = prob.dim, prob.n_refine
+    nv = 1 << dim
+    nn = ((1 << nref) + 1) ** dim
+    verts = np.ascontiguousarray(verts, dtype=np.float64)
+    n_cells = verts.shape[0]
+    coef = None if coef is None else np.ascontiguousarray(coef, dtype=np.float64)
+    f = None if f is None else np.ascontiguousarray(f, dtype=np.float64)
+    reaction = None if reaction is None else np.ascontiguousarray(reaction, dtype=np.float64)
+    M = np.zeros((n_cells, nv, nv))
+    b = np.zeros((n_cells, nv))
+    Psi = np.zeros((n_cells, nv, nn)) if want_psi else None
+    iters = np.zeros(n_cells, dtype=np.int32)
+    status = np.zeros(n_cells, dtype=np.int32)
+    lib().oq_basis_batch(ctypes.byref(prob), n_cells, _dp(verts), _dp(coef), _dp(f),
+                         _dp(reaction), solver, n_threads, _dp(M), _dp(b), _dp(Psi),
+                         _ip(iters), _ip(status))
+    return {"M": M, "b": b, "Psi": Psi, "iters": iters, "status": status}
+
+
+def q1_coeffs(dim, verts):
+    verts = np.ascontiguousarray(verts, dtype=np.float64)
+    nv = 1 << dim
+    C = np.zeros((nv, nv))
+    rc = lib().oq_basis_q1_coeffs(dim, _dp(verts), _dp(C))
+    if rc != 0:
+        raise ValueError("singular vertex-monomial matrix")
+    return C
+
+
+def q1_value(dim, C, index, x):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    return lib().oq_basis_q1_value(dim, _dp(np.ascontiguousarray(C)), index, _dp(x))
+
+
+def diffusion_A(prob: Problem, x):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    A = np.zeros(9)
+    lib().oq_diffusion_A(ctypes.byref(prob), _dp(x), _dp(A))
+    return A.reshape(3, 3)
+
+
+def element_matrix(dim, h, Aq, fq, rq=0.0):
+    nv = 1 << dim
+    Ke = np.zeros((nv, nv))
+    Fe = np.zeros(nv)
+    lib().oq_element_matrix(dim, _dp(np.ascontiguousarray(h, dtype=np.float64)),
+                            _dp(np.ascontiguousarray(Aq, dtype=np.float64)),
+                            _dp(np.ascontiguousarray(fq, dtype=np.float64)), float(rq),
+                            _dp(Ke), _dp(Fe))
+    return Ke, Fe
+
+
+def coarse_solve(dim, nc, M, b, rtol=1e-14, maxit=100000):
+    u = np.zeros((nc + 1) ** dim)
+    it = lib().oq_coarse_solve(dim, nc, _dp(np.ascontiguousarray(M, dtype=np.float64)),
+                               _dp(np.ascontiguousarray(b, dtype=np.float64)), rtol, maxit,
+                               _dp(u))
+    return u, it
+
+
+def reconstruct(dim, nc, n_refine, u_coarse, Psi):
+    nn = ((1 << n_refine) + 1) ** dim
+    ncell = nc ** dim
+    out = np.zeros((ncell, nn))
+    lib().oq_reconstruct(dim, nc, n_refine, _dp(np.ascontiguousarray(u_coarse, dtype=np.float64)),
+                         _dp(np.ascontiguousarray(Psi, dtype=np.float64)), _dp(out))
+    return out

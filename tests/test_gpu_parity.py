@@ -1,0 +1,200 @@
+"""Parity of the HIP basis construction (libmsfec_hip.so through the C-ABI)
+with the FP64 CPU oracle, on seeded inputs.
+
+Tolerances: both sides compute in FP64 with direct solves, so M, b and Psi
+must agree to a few hundred ulps of the condition number; the bound used is
+1e-11 relative (max over cells of ||X_gpu - X_oracle|| / ||X_oracle||), well
+inside the 1e-10 acceptance bound of the basis metric.
+"""
+import numpy as np
+import pytest
+
+from conftest import box_vertices
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-11
+
+
+def _lib():
+    from msfec_amd import _lib as L
+    return L
+
+
+def rel(a, b):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    num = np.linalg.norm((a - b).reshape(a.shape[0], -1), axis=1)
+    den = np.linalg.norm(b.reshape(b.shape[0], -1), axis=1)
+    return float(np.max(num / np.maximum(den, 1e-300)))
+
+
+def run_gpu(verts, dim, l, coef, coef_mode=0, f=None, rhs_mode=0, rhs_const=1.0,
+            reac=None, reaction_mode=0, reaction_const=0.0):
+    from msfec_amd import QBasisBatch
+    with QBasisBatch(verts, dim=dim, n_refine_local=l, coef_mode=coef_mode, rhs_mode=rhs_mode,
+                     rhs_const=rhs_const, reaction_mode=reaction_mode,
+                     reaction_const=reaction_const) as qb:
+        qb.set_coefficient(coef)
+        if f is not None:
+            qb.set_rhs(f)
+        if reac is not None:
+            qb.set_reaction(reac)
+        qb.run()
+        return dict(M=qb.get_global_element_matrix(), b=qb.get_global_element_rhs(),
+                    Psi=qb.get_basis(), status=qb.get_cell_status())
+
+
+def run_oracle(O, verts, dim, l, coef, coef_mode=0, f=None, rhs_mode=0, rhs_const=1.0,
+               reac=None, reaction_mode=0, reaction_const=0.0, **kw):
+    p = O.make_problem(dim, l, coef_mode=coef_mode, rhs_mode=rhs_mode, rhs_const=rhs_const,
+                       reaction_mode=reaction_mode, reaction_const=reaction_const, **kw)
+    return O.basis_batch(p, verts, coef=coef, f=f, reaction=reac, solver=O.SOLVER_DIRECT)
+
+
+def random_boxes(rng, dim, n_cells):
+    out = []
+    for _ in range(n_cells):
+        lo = rng.random(dim)
+        hi = lo + 0.02 + 0.1 * rng.random(dim)
+        out.append(box_vertices(dim, lo, hi))
+    return np.array(out)
+
+
+@pytest.mark.parametrize("dim,l", [(2, 1), (2, 2), (2, 3), (2, 4), (2, 5), (3, 1), (3, 2), (3, 3)])
+def test_scalar_lognormal(oracle_mod, dim, l):
+    print(f"case scalar_lognormal dim={dim} l={l}", flush=True)
+    rng = np.random.default_rng(100 + 10 * dim + l)
+    n_cells = 7
+    verts = random_boxes(rng, dim, n_cells)
+    N = 1 << l
+    coef = np.exp(1.5 * rng.standard_normal((n_cells, N ** dim)))
+    g = run_gpu(verts, dim, l, coef)
+    o = run_oracle(oracle_mod, verts, dim, l, coef)
+    assert (g["status"] == 0).all() and (o["status"] == 0).all()
+    assert rel(g["M"], o["M"]) < TOL
+    assert rel(g["b"], o["b"]) < TOL
+    assert rel(g["Psi"], o["Psi"]) < TOL
+
+
+@pytest.mark.parametrize("dim,l", [(2, 3), (3, 2)])
+def test_tensor_qp_and_rhs_modes(oracle_mod, dim, l):
+    print(f"case tensor_qp dim={dim} l={l}", flush=True)
+    rng = np.random.default_rng(7 + dim)
+    n_cells, N, nq = 5, 1 << l, 1 << dim
+    verts = random_boxes(rng, dim, n_cells)
+    ncomp = 3 if dim == 2 else 6
+    # random SPD tensors per quadrature point: B B^T + 0.5 I
+    B = rng.standard_normal((n_cells, N ** dim, nq, dim, dim))
+    A = np.einsum("...ij,...kj->...ik", B, B) + 0.5 * np.eye(dim)
+    if dim == 2:
+        T = np.stack([A[..., 0, 0], A[..., 0, 1], A[..., 1, 1]], -1)
+    else:
+        T = np.stack([A[..., 0, 0], A[..., 0, 1], A[..., 0, 2], A[..., 1, 1], A[..., 1, 2],
+                      A[..., 2, 2]], -1)
+    assert T.shape[-1] == ncomp
+    for rhs_mode in (1, 2):
+        f = rng.standard_normal((n_cells, N ** dim) if rhs_mode == 1 else (n_cells, N ** dim, nq))
+        g = run_gpu(verts, dim, l, T, coef_mode=1, f=f, rhs_mode=rhs_mode)
+        o = run_oracle(oracle_mod, verts, dim, l, T, coef_mode=1, f=f, rhs_mode=rhs_mode)
+        assert rel(g["M"], o["M"]) < TOL
+        assert rel(g["b"], o["b"]) < TOL
+        assert rel(g["Psi"], o["Psi"]) < TOL
+
+
+def test_analytic_diffusion_A(oracle_mod):
+    """The reference's rotated oscillatory tensor (eqn_coeff_A.cc), evaluated
+    at the quadrature points on the host, vs the oracle evaluating it inline."""
+    print("case analytic_diffusion_A", flush=True)
+    from msfec_amd import fields as F
+    dim, l, nc = 3, 2, 4
+    verts = F.coarse_vertices(dim, nc)[:16]
+    pts = F.qp_coordinates(verts, l)
+    T = F.diffusion_A_tensor(pts)
+    g = run_gpu(verts, dim, l, T, coef_mode=1)
+    o = run_oracle(oracle_mod, verts, dim, l, None, coef_mode=2, freq=(17, 16, 15),
+                   alpha=(0.9, 0.9, 0.9), rotate=True)
+    assert rel(g["M"], o["M"]) < 1e-10
+    assert rel(g["Psi"], o["Psi"]) < 1e-10
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_reaction(oracle_mod, mode):
+    print(f"case reaction mode={mode}", flush=True)
+    rng = np.random.default_rng(31 + mode)
+    dim, l, n_cells = 3, 2, 4
+    N = 1 << l
+    verts = random_boxes(rng, dim, n_cells)
+    coef = np.exp(rng.standard_normal((n_cells, N ** dim)))
+    reac = rng.random((n_cells, N ** dim)) if mode == 2 else None
+    g = run_gpu(verts, dim, l, coef, reac=reac, reaction_mode=mode, reaction_const=3.0)
+    o = run_oracle(oracle_mod, verts, dim, l, coef, reac=reac, reaction_mode=mode,
+                   reaction_const=3.0)
+    assert rel(g["M"], o["M"]) < TOL
+    assert rel(g["Psi"], o["Psi"]) < TOL
+
+
+def test_bad_cells_reported_not_fatal():
+    """A cell with a non-positive coefficient is flagged; the others stay valid."""
+    print("case bad_cells", flush=True)
+    rng = np.random.default_rng(5)
+    dim, l, n_cells = 3, 2, 3
+    N = 1 << l
+    verts = random_boxes(rng, dim, n_cells)
+    coef = np.exp(rng.standard_normal((n_cells, N ** dim)))
+    coef[1, :] = -1.0
+    coef[2, 3] = np.nan
+    L = _lib()
+    g = run_gpu(verts, dim, l, coef)
+    assert g["status"][0] == L.CELL_OK
+    assert g["status"][1] == L.CELL_NOT_SPD
+    assert g["status"][2] in (L.CELL_NOT_SPD, L.CELL_NONFINITE)
+    assert np.isfinite(g["M"][0]).all()
+
+
+def test_headline_batch_invariants(oracle_mod):
+    """4096 cells, 3-D, 8^3 fine cells each, lognormal sigma=1.5 field with
+    correlation length 1/64: structural invariants on every cell and oracle
+    parity on a seeded sample."""
+    print("case headline_batch", flush=True)
+    from msfec_amd import fields as F
+    dim, l, nc = 3, 3, 16
+    verts = F.coarse_vertices(dim, nc)
+    coef = F.lognormal_coefficient(dim, nc, l, 1.5, 1.0 / 64, seed=20261202)
+    g = run_gpu(verts, dim, l, coef)
+    assert (g["status"] == 0).all()
+    M, b, Psi = g["M"], g["b"], g["Psi"]
+    scale = np.abs(M).max(axis=(1, 2))
+    assert np.max(np.abs(M - M.transpose(0, 2, 1)).max(axis=(1, 2)) / scale) < 1e-12
+    # constants are A-harmonic: M 1 = 0 and sum_i psi_i = 1
+    assert np.max(np.abs(M.sum(axis=2)).max(axis=1) / scale) < 1e-11
+    assert np.abs(Psi.sum(axis=1) - 1.0).max() < 1e-11
+    # sum_i b_i = int_K f = |K| for f = 1
+    assert np.abs(b.sum(axis=1) - (1.0 / nc) ** 3).max() < 1e-15
+    # diagonal positive, eigenvalues >= 0
+    ev = np.linalg.eigvalsh(M)
+    assert (ev[:, 0] > -1e-12 * scale).all()
+    sample = np.random.default_rng(1).choice(verts.shape[0], 24, replace=False)
+    o = run_oracle(oracle_mod, verts[sample], dim, l, coef[sample])
+    assert rel(M[sample], o["M"]) < TOL
+    assert rel(b[sample], o["b"]) < TOL
+    assert rel(Psi[sample], o["Psi"]) < TOL
+
+
+@pytest.mark.parametrize("dim,nc,l", [(2, 8, 4), (3, 4, 3)])
+def test_coarse_solve_and_reconstruct(oracle_mod, dim, nc, l):
+    print(f"case coarse dim={dim} nc={nc} l={l}", flush=True)
+    from msfec_amd import QBasisBatch, fields as F
+    verts = F.coarse_vertices(dim, nc)
+    coef = F.lognormal_coefficient(dim, nc, l, 1.0, 0.0, seed=3)
+    with QBasisBatch(verts, dim=dim, n_refine_local=l) as qb:
+        qb.set_coefficient(coef)
+        qb.run()
+        M, b, Psi = qb.get_global_element_matrix(), qb.get_global_element_rhs(), qb.get_basis()
+        u, w, it, res = qb.coarse_solve(nc, abs_tol=0.0, rel_tol=1e-13)
+        uf = qb.set_global_weights(w)
+    uo, ito = oracle_mod.coarse_solve(dim, nc, M, b, rtol=1e-14)
+    assert it > 0 and ito > 0
+    assert np.abs(u - uo).max() <= 1e-10 * np.abs(uo).max()
+    ufo = oracle_mod.reconstruct(dim, nc, l, uo, Psi)
+    assert np.abs(uf - ufo).max() <= 1e-10 * np.abs(ufo).max()
