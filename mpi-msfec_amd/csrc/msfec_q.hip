@@ -283,47 +283,236 @@ __global__ __launch_bounds__(256) void q_assemble_kernel(
 
 // ---------------------------------------------------------------------------
 // K2: banded LDL^T in a register-resident sliding window (one wave per cell)
+//
+// Staging: the matrix rows and lifting right-hand sides that enter the window
+// during the next 8 elimination steps are prepared in LDS once per group of 8
+// steps, either computed from the LDS-resident fine-cell coefficients
+// (scalar coefficient on a box: K[n][n+dd] = kv[mask(dd)] * sum of the
+// coefficients of the fine cells shared by n and n+dd, plus the mass
+// analogue for the reaction term) or read from the tables of
+// q_assemble_kernel (tensor coefficients, quadrature-point loads).
+// Synchronisation inside the wave uses wavefront-scope fences only, so the
+// streamed stores of L never stall the elimination.
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 struct BandArgs {
-  int n;          // interior unknowns
-  int nlow;       // lower stencil entries incl. diagonal
-  int nn;         // fine nodes (stride of the stencil table)
+  int n, ni, N, S, nn, nfc;
   int off_int[kMaxLow];
-  const double* Slo;     // [cell][nn][nlow]
-  const double* Rhs;     // [cell][n][8]
-  const int* inode;      // interior index -> node id
-  const unsigned* imask; // interior index -> bit t set if lower neighbour t is interior
-  double* Lcol;          // [cell][n][64]  (o = 0: pivot d_j, o > 0: L[j+o][j])
-  double* Wf;            // [cell][n][8]   D^-1 L^-1 rhs
-  double* X;             // [cell][n][8]   interior solution
-  int* status;           // [cell]
+  // general path tables (q_assemble_kernel)
+  const double* Slo;      // [cell][nn][nlow]
+  const double* Rhs;      // [cell][n][8]
+  const int* inode;       // interior index -> node id
+  const unsigned* imask;  // bit t: lower neighbour t is interior
+  // fast path data
+  const double* verts;    // [cell][2^dim][dim]
+  const double* coef;     // [cell][nfc]
+  const double* reac;     // [cell][nfc] or null
+  const double* fcell;    // [cell][nfc] or null
+  int reaction_mode, rhs_mode;
+  double reaction_const, rhs_const;
+  // scratch and outputs
+  double* Lcol;  // [cell][n][64]  o = 0: pivot d_j, o > 0: L[j+o][j]
+  double* Wf;    // [cell][n][8]   D^-1 L^-1 rhs
+  double* X;     // [cell][n][8]   interior solution
+  double* Psi;   // [cell][nv][nn]
+  double* M;     // [cell][nv][nv]
+  double* bvec;  // [cell][nv]
+  int* status;   // [cell]
 };
 
-template <int I>
-__device__ __forceinline__ void band_insert(const BandArgs& a, int cell, int jn, int G,
-                                            double (&A)[8][8], double (&y)[8],
-                                            double* nbuf) {
+template <int DIM>
+struct Dirs {
+  static constexpr int NV = 1 << DIM;
+  static constexpr int NST = (DIM == 3) ? 27 : 9;
+  static constexpr int CENTER = (NST - 1) / 2;
+  static constexpr int NLOW = CENTER + 1;
+};
+
+struct BandSmem {
+  double ubuf[kWin];
+  double nbuf[kWin];
+  double ybuf[8];
+  double kv[8];
+  double mv[8];
+  double stg_low[8][16];
+  double stg_rhs[8][8];
+  // phase-dependent pool:
+  //   forward : RHS partials [64][8]
+  //   backward: w of the group [8][8] | L columns of the group [8][64] | x window [8][64]
+  //   epilogue: psi of 64 nodes [64][8] | (K psi) of 64 nodes [64][8] | F of 64 nodes [64]
+  double pool[64 + 8 * kWin + 8 * kWin];
+};
+#define SH_PART(sh) ((sh).pool)
+#define SH_WBUF(sh) ((sh).pool)
+#define SH_LBUF(sh) ((sh).pool + 64)
+#define SH_XRING(sh) ((sh).pool + 64 + 8 * kWin)
+#define SH_PN(sh) ((sh).pool)
+#define SH_KP(sh) ((sh).pool + 8 * kWin)
+#define SH_FS(sh) ((sh).pool + 16 * kWin)
+
+// direction e -> component d in {-1,0,1}
+template <int DIM>
+__device__ __forceinline__ int dcomp(int e, int d) {
+  return (d == 0) ? (e % 3) - 1 : (d == 1) ? ((e / 3) % 3) - 1 : (e / 9) - 1;
+}
+
+// Sum of the coefficients of the fine cells shared by node c and c+dd
+// (cells clipped to the grid); cstride = (1, N, N^2).
+template <int DIM>
+__device__ __forceinline__ void shared_sums(const double* cf, const double* rf, bool has_rf,
+                                            const int* c, const int* dd, int N, double& sa,
+                                            double& sr, int& cnt) {
+  sa = 0.0;
+  sr = 0.0;
+  cnt = 0;
+#pragma unroll
+  for (int al = 0; al < (1 << DIM); ++al) {
+    bool ok = true;
+    int z = 0, mul = 1;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+      const int ad = (al >> d) & 1;
+      if (dd[d] == -1 && ad != 1) ok = false;
+      if (dd[d] == 1 && ad != 0) ok = false;
+      const int zd = c[d] - ad;
+      if (zd < 0 || zd >= N) ok = false;
+      z += zd * mul;
+      mul *= N;
+    }
+    if (ok) {
+      sa += cf[z];
+      if (has_rf) sr += rf[z];
+      ++cnt;
+    }
+  }
+}
+
+// g_k at a fine node with grid coordinates m (trilinear coarse basis on the box)
+template <int DIM>
+__device__ __forceinline__ double lift_k(int k, const int* m, double invN) {
+  double v = 1.0;
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) {
+    const double xi = (double)m[d] * invN;
+    v *= ((k >> d) & 1) ? xi : 1.0 - xi;
+  }
+  return v;
+}
+
+// Stage the rows jbase..jbase+7 (lower stencil with zeros for non-interior
+// neighbours, and the lifting right-hand sides) into LDS.
+template <int DIM, bool FAST>
+__device__ __forceinline__ void stage_rows(const BandArgs& a, int cell, int jbase, BandSmem& sh,
+                                           const double* cf, const double* rf) {
+  using D = Dirs<DIM>;
+  const int lane = threadIdx.x;
+  const int r = lane >> 3, s = lane & 7;
+  const int jn = jbase + r;
+  if (FAST) {
+    double rhs[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) rhs[k] = 0.0;
+    if (jn < a.n) {
+      int c[3] = {0, 0, 0};
+      {
+        int t = jn;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+          c[d] = t % a.ni + 1;
+          t /= a.ni;
+        }
+      }
+      const bool has_rf = (a.reaction_mode == MSFEC_REACTION_CELL);
+      const double rc = (a.reaction_mode == MSFEC_REACTION_CONST) ? a.reaction_const : 0.0;
+      const double invN = 1.0 / (double)a.N;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int e = s + 8 * u;
+        if (e >= D::NST || e == D::CENTER) continue;
+        int dd[3] = {0, 0, 0}, m[3] = {0, 0, 0}, mask = 0;
+        bool bnd = false;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+          dd[d] = dcomp<DIM>(e, d);
+          if (dd[d] != 0) mask |= 1 << d;
+          m[d] = c[d] + dd[d];
+          if (m[d] == 0 || m[d] == a.N) bnd = true;
+        }
+        double sa, sr;
+        int cnt;
+        shared_sums<DIM>(cf, rf, has_rf, c, dd, a.N, sa, sr, cnt);
+        const double kval = sh.kv[mask] * sa + sh.mv[mask] * (sr + rc * cnt);
+        if (e < D::CENTER) sh.stg_low[r][1 + e] = bnd ? 0.0 : kval;
+        if (bnd) {
+#pragma unroll
+          for (int k = 0; k < D::NV; ++k) rhs[k] = fma(-kval, lift_k<DIM>(k, m, invN), rhs[k]);
+        }
+      }
+      if (s == 0) {
+        int dd[3] = {0, 0, 0};
+        double sa, sr;
+        int cnt;
+        shared_sums<DIM>(cf, rf, has_rf, c, dd, a.N, sa, sr, cnt);
+        sh.stg_low[r][0] = sh.kv[0] * sa + sh.mv[0] * (sr + rc * cnt);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) SH_PART(sh)[lane * 8 + k] = rhs[k];
+    wave_sync();
+    {
+      const int k = s;
+      double t = 0.0;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) t += SH_PART(sh)[((r << 3) + q) * 8 + k];
+      sh.stg_rhs[r][k] = t;
+    }
+    wave_sync();
+  } else {
+    if (jn < a.n) {
+      if (s == 0) {
+        const unsigned msk = a.imask[jn];
+        const double* src = a.Slo + ((size_t)cell * a.nn + a.inode[jn]) * D::NLOW;
+#pragma unroll
+        for (int t = 0; t < D::NLOW; ++t) sh.stg_low[r][t] = ((msk >> t) & 1u) ? src[t] : 0.0;
+      }
+      sh.stg_rhs[r][s] = a.Rhs[((size_t)cell * a.n + jn) * kMaxNb + s];
+    }
+    wave_sync();
+  }
+}
+
+// Insert index jn (staged as row I) into slot s = 8G+I of the window.
+template <int DIM, int I>
+__device__ __forceinline__ void band_insert(const BandArgs& a, int jn, int G, int my_off,
+                                            double (&A)[8][8], double (&y)[8], BandSmem& sh) {
+  using D = Dirs<DIM>;
   const int lane = threadIdx.x;
   const int p = lane >> 3, q = lane & 7;
   const int s = 8 * G + I;
-  nbuf[lane] = 0.0;
-  __syncthreads();
+  sh.nbuf[lane] = 0.0;
+  wave_sync();
   if (jn < a.n) {
-    if (lane < a.nlow) {
-      const unsigned msk = a.imask[jn];
-      if ((msk >> lane) & 1u) {
-        const double v = a.Slo[((size_t)cell * a.nn + a.inode[jn]) * a.nlow + lane];
-        nbuf[(s - a.off_int[lane]) & (kWin - 1)] = v;
-      }
+    // only nonzero entries: they belong to distinct interior neighbours and
+    // hence to distinct slots, while the offsets of non-neighbour directions
+    // can coincide (e.g. with the diagonal when there is a single interior
+    // node per direction)
+    if (lane < D::NLOW) {
+      const double v = sh.stg_low[I][lane];
+      if (v != 0.0) sh.nbuf[(s - my_off) & (kWin - 1)] = v;
     }
   } else if (lane == 0) {
-    nbuf[s] = 1.0;  // past the end: an isolated unit diagonal
+    sh.nbuf[s] = 1.0;  // past the end: an isolated unit diagonal
   }
-  __syncthreads();
+  wave_sync();
   const int base = (p == G) ? 8 * q : 8 * p;
   double v[8];
 #pragma unroll
-  for (int b = 0; b < 8; ++b) v[b] = nbuf[base + b];
+  for (int b = 0; b < 8; ++b) v[b] = sh.nbuf[base + b];
   if (p == G) {
 #pragma unroll
     for (int b = 0; b < 8; ++b) A[I][b] = v[b];
@@ -333,51 +522,45 @@ __device__ __forceinline__ void band_insert(const BandArgs& a, int cell, int jn,
     for (int r = 0; r < 8; ++r) A[r][I] = v[r];
   }
   if (lane == s) {
-    if (jn < a.n) {
-      const double* src = a.Rhs + ((size_t)cell * a.n + jn) * kMaxNb;
+    const bool live = jn < a.n;
 #pragma unroll
-      for (int k = 0; k < 8; ++k) y[k] = src[k];
-    } else {
-#pragma unroll
-      for (int k = 0; k < 8; ++k) y[k] = 0.0;
-    }
+    for (int k = 0; k < 8; ++k) y[k] = live ? sh.stg_rhs[I][k] : 0.0;
   }
 }
 
-template <int I>
-__device__ __forceinline__ void band_step(const BandArgs& a, int cell, int j, int G,
-                                          double (&A)[8][8], double (&y)[8], double* ubuf,
-                                          double* ybuf, double* nbuf, int& bad) {
+template <int DIM, int I>
+__device__ __forceinline__ void band_step(const BandArgs& a, int cell, int j, int G, int my_off,
+                                          double (&A)[8][8], double (&y)[8], BandSmem& sh,
+                                          int& bad, double& sacc) {
   const int lane = threadIdx.x;
   const int p = lane >> 3, q = lane & 7;
   const int s = 8 * G + I;
-  // pivot row of the window -> LDS (the tiles of block-row G hold it in row I)
   if (p == G) {
 #pragma unroll
-    for (int b = 0; b < 8; ++b) ubuf[8 * q + b] = A[I][b];
+    for (int b = 0; b < 8; ++b) sh.ubuf[8 * q + b] = A[I][b];
   }
   if (lane == s) {
 #pragma unroll
-    for (int k = 0; k < 8; ++k) ybuf[k] = y[k];
+    for (int k = 0; k < 8; ++k) sh.ybuf[k] = y[k];
   }
-  __syncthreads();
-  const double d = ubuf[s];
-  if (!(d > 0.0)) bad |= (d == d && d < INFINITY) ? MSFEC_CELL_NOT_SPD : MSFEC_CELL_NONFINITE;
+  wave_sync();
+  const double d = sh.ubuf[s];
+  if (!(d > 0.0)) bad = max(bad, (d == d && d > -INFINITY) ? (int)MSFEC_CELL_NOT_SPD
+                                                             : (int)MSFEC_CELL_NONFINITE);
   const double rd = 1.0 / d;
   double l[8], u[8], ys[8];
 #pragma unroll
-  for (int r = 0; r < 8; ++r) l[r] = ubuf[8 * p + r] * rd;
+  for (int r = 0; r < 8; ++r) l[r] = sh.ubuf[8 * p + r] * rd;
 #pragma unroll
-  for (int b = 0; b < 8; ++b) u[b] = ubuf[8 * q + b];
+  for (int b = 0; b < 8; ++b) u[b] = sh.ubuf[8 * q + b];
 #pragma unroll
-  for (int k = 0; k < 8; ++k) ys[k] = ybuf[k];
-  const double um = ubuf[lane];
-  const double lm = um * rd;
-  const double wk = ybuf[lane & 7] * rd;
-  // column j of L (offset o = slot distance from the pivot) and w_j
+  for (int k = 0; k < 8; ++k) ys[k] = sh.ybuf[k];
+  const double lm = sh.ubuf[lane] * rd;
+  const double wk = sh.ybuf[lane & 7] * rd;
+  // Schur term of the element matrix: lane (i,k) accumulates z_j[i] z_j[k] / d_j
+  sacc = fma(sh.ybuf[lane >> 3], wk, sacc);
   a.Lcol[((size_t)cell * a.n + j) * kWin + ((lane - s) & (kWin - 1))] = (lane == s) ? d : lm;
   if (lane < 8) a.Wf[((size_t)cell * a.n + j) * kMaxNb + lane] = wk;
-  // rank-1 update of the window and of the right-hand sides
 #pragma unroll
   for (int r = 0; r < 8; ++r)
 #pragma unroll
@@ -386,25 +569,233 @@ __device__ __forceinline__ void band_step(const BandArgs& a, int cell, int j, in
 #pragma unroll
     for (int k = 0; k < 8; ++k) y[k] = fma(-lm, ys[k], y[k]);
   }
-  __syncthreads();
-  band_insert<I>(a, cell, j + kWin, G, A, y, nbuf);
+  wave_sync();
+  band_insert<DIM, I>(a, j + kWin, G, my_off, A, y, sh);
 }
 
-template <int I>
-__device__ __forceinline__ void band_init_one(const BandArgs& a, int cell, int G,
-                                              double (&A)[8][8], double (&y)[8],
-                                              double* nbuf) {
-  band_insert<I>(a, cell, 8 * G + I, G, A, y, nbuf);
-  __syncthreads();
+// Running interior coordinates of the backward sweep (j decreasing).
+struct BackPos {
+  int x, y, z;  // interior coordinates of j
+};
+
+template <int DIM, bool FAST, int R>
+__device__ __forceinline__ void back_step(const BandArgs& a, int cell, int j, BandSmem& sh,
+                                          BackPos& pos, double& bint, const double* ff) {
+  using D = Dirs<DIM>;
+  const int lane = threadIdx.x;
+  const int k = lane & 7, m = lane >> 3;
+  double part = 0.0;
+#pragma unroll
+  for (int t = 0; t < 8; ++t) {
+    const double lv = (m == 0 && t == 0) ? 0.0 : SH_LBUF(sh)[R * kWin + 8 * m + t];
+    part = fma(lv, SH_XRING(sh)[k * kWin + ((j + 8 * m + t) & (kWin - 1))], part);
+  }
+  part += __shfl_xor(part, 8);
+  part += __shfl_xor(part, 16);
+  part += __shfl_xor(part, 32);
+  const double xj = SH_WBUF(sh)[R * 8 + k] - part;
+  if (m == 0) {
+    SH_XRING(sh)[k * kWin + (j & (kWin - 1))] = xj;
+    if (FAST) {
+      // interior entry of Psi and its share of b = Psi^T F
+      const int cx = pos.x + 1, cy = pos.y + 1, cz = (DIM == 3) ? pos.z + 1 : 0;
+      const int node = cx + a.S * (cy + a.S * cz);
+      if (k < D::NV) a.Psi[((size_t)cell * D::NV + k) * a.nn + node] = xj;
+      double fj;
+      if (a.rhs_mode == MSFEC_RHS_CELL) {
+        double t = 0.0;
+#pragma unroll
+        for (int al = 0; al < D::NV; ++al) {
+          const int zx = cx - (al & 1), zy = cy - ((al >> 1) & 1);
+          const int zz = (DIM == 3) ? cz - ((al >> 2) & 1) : 0;
+          t += ff[zx + a.N * (zy + a.N * zz)];
+        }
+        fj = t;
+      } else {
+        fj = a.rhs_const * D::NV;
+      }
+      bint = fma(xj, fj, bint);  // scaled by |z| / 2^dim at the end
+    } else {
+      a.X[((size_t)cell * a.n + j) * kMaxNb + k] = xj;
+    }
+  }
+  // step to j-1 (uniform)
+  if (--pos.x < 0) {
+    pos.x = a.ni - 1;
+    if (--pos.y < 0) {
+      pos.y = a.ni - 1;
+      --pos.z;
+    }
+  }
+  wave_sync();
 }
 
-__global__ __launch_bounds__(64) void q_band_kernel(BandArgs a) {
+// Fast-path epilogue.  With R = -K_IB G_B the lifting right-hand sides and
+// K_II = L D L^T, the element matrix psi_i^T K psi_j of the reference
+// (q_basis.cc:280-293) is exactly
+//   M = G_B^T K_BB G_B - R^T K_II^{-1} R = P - sum_j z_j z_j^T / d_j ,
+// z = L^{-1} R the forward-eliminated right-hand sides.  The sum (sacc) is
+// accumulated during the forward sweep; P involves boundary nodes only and
+// is formed here from the LDS coefficients, together with the boundary part
+// of Psi and of b = Psi^T F (the interior parts were produced by the
+// backward sweep).  Nodes are processed 64 at a time, one per lane; lane
+// (i,k) then accumulates P[i][k] over the chunk from LDS.
+template <int DIM>
+__device__ __forceinline__ void fast_epilogue(const BandArgs& a, int cell, BandSmem& sh,
+                                              const double* cf, const double* rf,
+                                              const double* ff, double vol, double sacc,
+                                              double bint) {
+  using D = Dirs<DIM>;
+  const int lane = threadIdx.x;
+  double* psi_out = a.Psi + (size_t)cell * D::NV * a.nn;
+  const double invN = 1.0 / (double)a.N;
+  const bool has_rf = (a.reaction_mode == MSFEC_REACTION_CELL);
+  const double rc = (a.reaction_mode == MSFEC_REACTION_CONST) ? a.reaction_const : 0.0;
+  double* pn_s = SH_PN(sh);
+  double* kp_s = SH_KP(sh);
+  double* fs_s = SH_FS(sh);
+  double pacc = 0.0, bacc = 0.0;
+  const int mi_ = lane >> 3, mk_ = lane & 7;
+  for (int base = 0; base < a.nn; base += 64) {
+    const int node = base + lane;
+    int c[3] = {0, 0, 0};
+    bool bnd = false;
+    {
+      int t = (node < a.nn) ? node : 0;
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) {
+        c[d] = t % a.S;
+        t /= a.S;
+        if (c[d] == 0 || c[d] == a.N) bnd = true;
+      }
+    }
+    const bool act = (node < a.nn) && bnd;
+    double fs = 0.0;
+    double kp[D::NV];
+#pragma unroll
+    for (int k = 0; k < D::NV; ++k) kp[k] = 0.0;
+    if (act) {
+#pragma unroll
+      for (int k = 0; k < D::NV; ++k) {
+        const double v = lift_k<DIM>(k, c, invN);
+        psi_out[(size_t)k * a.nn + node] = v;
+        pn_s[lane * 8 + k] = v;
+      }
+      // F[node] = sum over the cells touching the node of f_z |z| / 2^dim
+#pragma unroll
+      for (int al = 0; al < D::NV; ++al) {
+        bool ok = true;
+        int z = 0, mul = 1;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+          const int zd = c[d] - ((al >> d) & 1);
+          if (zd < 0 || zd >= a.N) ok = false;
+          z += zd * mul;
+          mul *= a.N;
+        }
+        if (ok) fs += (a.rhs_mode == MSFEC_RHS_CELL) ? ff[z] : a.rhs_const;
+      }
+      fs *= vol / D::NV;
+      // (K_BB g_k)[node]: boundary neighbours only
+#pragma unroll
+      for (int e = 0; e < D::NST; ++e) {
+        int dd[3] = {0, 0, 0}, m[3] = {0, 0, 0}, mask = 0;
+        bool inside = true, mb = false;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+          dd[d] = dcomp<DIM>(e, d);
+          if (dd[d] != 0) mask |= 1 << d;
+          m[d] = c[d] + dd[d];
+          if (m[d] < 0 || m[d] > a.N) inside = false;
+          if (m[d] == 0 || m[d] == a.N) mb = true;
+        }
+        if (!inside || !mb) continue;
+        double sa, sr;
+        int cnt;
+        shared_sums<DIM>(cf, rf, has_rf, c, dd, a.N, sa, sr, cnt);
+        const double kval = sh.kv[mask] * sa + sh.mv[mask] * (sr + rc * cnt);
+#pragma unroll
+        for (int k = 0; k < D::NV; ++k) kp[k] = fma(kval, lift_k<DIM>(k, m, invN), kp[k]);
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < D::NV; ++k) pn_s[lane * 8 + k] = 0.0;
+    }
+    fs_s[lane] = fs;
+#pragma unroll
+    for (int k = 0; k < D::NV; ++k) kp_s[lane * 8 + k] = kp[k];
+    wave_sync();
+    if (mi_ < D::NV && mk_ < D::NV) {
+#pragma unroll 8
+      for (int l = 0; l < 64; ++l) pacc = fma(pn_s[l * 8 + mi_], kp_s[l * 8 + mk_], pacc);
+    }
+    if (lane < D::NV) {
+#pragma unroll 8
+      for (int l = 0; l < 64; ++l) bacc = fma(pn_s[l * 8 + lane], fs_s[l], bacc);
+    }
+    wave_sync();
+  }
+  if (mi_ < D::NV && mk_ < D::NV) a.M[((size_t)cell * D::NV + mi_) * D::NV + mk_] = pacc - sacc;
+  if (lane < D::NV) a.bvec[(size_t)cell * D::NV + lane] = bacc + bint * (vol / D::NV);
+}
+
+template <int DIM, bool FAST>
+__global__ __launch_bounds__(64, 2) void q_band_kernel(BandArgs a) {
+  using D = Dirs<DIM>;
   const int cell = blockIdx.x;
   const int lane = threadIdx.x;
-  __shared__ double ubuf[kWin];
-  __shared__ double nbuf[kWin];
-  __shared__ double ybuf[8];
-  __shared__ double xring[8][2 * kWin];
+  __shared__ BandSmem sh;
+  extern __shared__ double dyn[];  // fast path: fine-cell coefficient | reaction | load
+  const double* cf = dyn;
+  const double* rf = dyn + a.nfc;
+  const double* ff = dyn + a.nfc * ((a.reaction_mode == MSFEC_REACTION_CELL) ? 2 : 1);
+  double vol = 1.0;
+  if (FAST) {
+    const double* cg = a.coef + (size_t)cell * a.nfc;
+    for (int z = lane; z < a.nfc; z += 64) dyn[z] = cg[z];
+    if (a.reaction_mode == MSFEC_REACTION_CELL) {
+      const double* rg = a.reac + (size_t)cell * a.nfc;
+      for (int z = lane; z < a.nfc; z += 64) dyn[a.nfc + z] = rg[z];
+    }
+    if (a.rhs_mode == MSFEC_RHS_CELL) {
+      const double* fg = a.fcell + (size_t)cell * a.nfc;
+      double* fd = dyn + a.nfc * ((a.reaction_mode == MSFEC_REACTION_CELL) ? 2 : 1);
+      for (int z = lane; z < a.nfc; z += 64) fd[z] = fg[z];
+    }
+    const double* v = a.verts + (size_t)cell * D::NV * DIM;
+    double hh[3] = {1.0, 1.0, 1.0};
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+      hh[d] = (v[(D::NV - 1) * DIM + d] - v[d]) / a.N;
+      vol *= hh[d];
+    }
+    if (lane < 8) {
+      // element stiffness / mass of a box as a function of a^b (bit d set:
+      // the two vertices differ in direction d)
+      const int x = lane;
+      double ks = 0.0, ms = 1.0;
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) {
+        double t = (((x >> d) & 1) ? -1.0 : 1.0) / hh[d];
+#pragma unroll
+        for (int e = 0; e < DIM; ++e)
+          if (e != d) t *= (((x >> e) & 1) ? (1.0 / 6.0) : (1.0 / 3.0)) * hh[e];
+        ks += t;
+        ms *= (((x >> d) & 1) ? (1.0 / 6.0) : (1.0 / 3.0)) * hh[d];
+      }
+      sh.kv[x] = ks;
+      sh.mv[x] = ms;
+    }
+    wave_sync();
+  }
+
+  // interior-grid offset of the lower stencil entry this lane scatters
+  // (kept in a register: indexing the kernel argument by lane would reload it
+  // from memory in every step)
+  int my_off = 0;
+#pragma unroll
+  for (int t = 0; t < kMaxLow; ++t)
+    if (lane == t) my_off = a.off_int[t];
 
   double A[8][8];
   double y[8];
@@ -417,64 +808,89 @@ __global__ __launch_bounds__(64) void q_band_kernel(BandArgs a) {
 
   // initial window: indices 0..63 in slots 0..63
   for (int G = 0; G < 8; ++G) {
-    band_init_one<0>(a, cell, G, A, y, nbuf);
-    band_init_one<1>(a, cell, G, A, y, nbuf);
-    band_init_one<2>(a, cell, G, A, y, nbuf);
-    band_init_one<3>(a, cell, G, A, y, nbuf);
-    band_init_one<4>(a, cell, G, A, y, nbuf);
-    band_init_one<5>(a, cell, G, A, y, nbuf);
-    band_init_one<6>(a, cell, G, A, y, nbuf);
-    band_init_one<7>(a, cell, G, A, y, nbuf);
+    stage_rows<DIM, FAST>(a, cell, 8 * G, sh, cf, rf);
+    band_insert<DIM, 0>(a, 8 * G + 0, G, my_off, A, y, sh);
+    wave_sync();
+    band_insert<DIM, 1>(a, 8 * G + 1, G, my_off, A, y, sh);
+    wave_sync();
+    band_insert<DIM, 2>(a, 8 * G + 2, G, my_off, A, y, sh);
+    wave_sync();
+    band_insert<DIM, 3>(a, 8 * G + 3, G, my_off, A, y, sh);
+    wave_sync();
+    band_insert<DIM, 4>(a, 8 * G + 4, G, my_off, A, y, sh);
+    wave_sync();
+    band_insert<DIM, 5>(a, 8 * G + 5, G, my_off, A, y, sh);
+    wave_sync();
+    band_insert<DIM, 6>(a, 8 * G + 6, G, my_off, A, y, sh);
+    wave_sync();
+    band_insert<DIM, 7>(a, 8 * G + 7, G, my_off, A, y, sh);
+    wave_sync();
   }
 
   int bad = 0;
+  double sacc = 0.0;
   const int n = a.n;
   for (int j0 = 0; j0 < n; j0 += 8) {
     const int G = (j0 >> 3) & 7;
-    if (j0 + 0 < n) band_step<0>(a, cell, j0 + 0, G, A, y, ubuf, ybuf, nbuf, bad);
-    if (j0 + 1 < n) band_step<1>(a, cell, j0 + 1, G, A, y, ubuf, ybuf, nbuf, bad);
-    if (j0 + 2 < n) band_step<2>(a, cell, j0 + 2, G, A, y, ubuf, ybuf, nbuf, bad);
-    if (j0 + 3 < n) band_step<3>(a, cell, j0 + 3, G, A, y, ubuf, ybuf, nbuf, bad);
-    if (j0 + 4 < n) band_step<4>(a, cell, j0 + 4, G, A, y, ubuf, ybuf, nbuf, bad);
-    if (j0 + 5 < n) band_step<5>(a, cell, j0 + 5, G, A, y, ubuf, ybuf, nbuf, bad);
-    if (j0 + 6 < n) band_step<6>(a, cell, j0 + 6, G, A, y, ubuf, ybuf, nbuf, bad);
-    if (j0 + 7 < n) band_step<7>(a, cell, j0 + 7, G, A, y, ubuf, ybuf, nbuf, bad);
+    stage_rows<DIM, FAST>(a, cell, j0 + kWin, sh, cf, rf);
+    if (j0 + 0 < n) band_step<DIM, 0>(a, cell, j0 + 0, G, my_off, A, y, sh, bad, sacc);
+    if (j0 + 1 < n) band_step<DIM, 1>(a, cell, j0 + 1, G, my_off, A, y, sh, bad, sacc);
+    if (j0 + 2 < n) band_step<DIM, 2>(a, cell, j0 + 2, G, my_off, A, y, sh, bad, sacc);
+    if (j0 + 3 < n) band_step<DIM, 3>(a, cell, j0 + 3, G, my_off, A, y, sh, bad, sacc);
+    if (j0 + 4 < n) band_step<DIM, 4>(a, cell, j0 + 4, G, my_off, A, y, sh, bad, sacc);
+    if (j0 + 5 < n) band_step<DIM, 5>(a, cell, j0 + 5, G, my_off, A, y, sh, bad, sacc);
+    if (j0 + 6 < n) band_step<DIM, 6>(a, cell, j0 + 6, G, my_off, A, y, sh, bad, sacc);
+    if (j0 + 7 < n) band_step<DIM, 7>(a, cell, j0 + 7, G, my_off, A, y, sh, bad, sacc);
+    wave_sync();
   }
 
-  // backward solve x_j = w_j - sum_{o>=1} L[j+o][j] x_{j+o}
-  // lane = (k, m): RHS k = lane & 7, offsets 8m..8m+7 with m = lane >> 3
+  // backward solve x_j = w_j - sum_{o>=1} L[j+o][j] x_{j+o}; lane = (k, m):
+  // RHS k = lane & 7, offsets 8m..8m+7 with m = lane >> 3.  The columns of
+  // the next group of 8 steps are prefetched into registers.
   __threadfence_block();
   __syncthreads();
-  const int k = lane & 7, m = lane >> 3;
-  for (int t = lane; t < 8 * 2 * kWin; t += 64) (&xring[0][0])[t] = 0.0;
-  __syncthreads();
+  for (int t = lane; t < 8 * kWin; t += 64) SH_XRING(sh)[t] = 0.0;
   const double* Lc = a.Lcol + (size_t)cell * n * kWin;
   const double* Wc = a.Wf + (size_t)cell * n * kMaxNb;
-  double* Xc = a.X + (size_t)cell * n * kMaxNb;
-  for (int j = n - 1; j >= 0; --j) {
-    const double* lrow = Lc + (size_t)j * kWin + 8 * m;
-    const int base = (j + 8 * m) & (kWin - 1);
-    double part = 0.0;
+  const int nchunk = (n + 7) >> 3;
+  BackPos pos{a.ni - 1, a.ni - 1, (DIM == 3) ? a.ni - 1 : 0};
+  double bint = 0.0;
+  double lp[8], wp = 0.0;
+  {
+    const int jj = 8 * (nchunk - 1) + (lane >> 3);
 #pragma unroll
-    for (int t = 0; t < 8; ++t) {
-      const double lv = (m == 0 && t == 0) ? 0.0 : lrow[t];
-      part = fma(lv, xring[k][base + t], part);
-    }
-    part += __shfl_xor(part, 8);
-    part += __shfl_xor(part, 16);
-    part += __shfl_xor(part, 32);
-    const double xj = Wc[(size_t)j * kMaxNb + k] - part;
-    if (m == 0) {
-      xring[k][j & (kWin - 1)] = xj;
-      xring[k][(j & (kWin - 1)) + kWin] = xj;
-      Xc[(size_t)j * kMaxNb + k] = xj;
-    }
-    __syncthreads();
+    for (int t = 0; t < 8; ++t) lp[t] = (jj < n) ? Lc[(size_t)jj * kWin + 8 * (lane & 7) + t] : 0.0;
+    wp = (jj < n) ? Wc[(size_t)jj * kMaxNb + (lane & 7)] : 0.0;
   }
-  // cell status (max over lanes)
+  for (int cch = nchunk - 1; cch >= 0; --cch) {
+    wave_sync();
+#pragma unroll
+    for (int t = 0; t < 8; ++t) SH_LBUF(sh)[(lane >> 3) * kWin + 8 * (lane & 7) + t] = lp[t];
+    SH_WBUF(sh)[lane] = wp;
+    wave_sync();
+    if (cch > 0) {
+      const int jj = 8 * (cch - 1) + (lane >> 3);
+#pragma unroll
+      for (int t = 0; t < 8; ++t) lp[t] = Lc[(size_t)jj * kWin + 8 * (lane & 7) + t];
+      wp = Wc[(size_t)jj * kMaxNb + (lane & 7)];
+    }
+    const int jb = 8 * cch;
+    if (jb + 7 < n) back_step<DIM, FAST, 7>(a, cell, jb + 7, sh, pos, bint, ff);
+    if (jb + 6 < n) back_step<DIM, FAST, 6>(a, cell, jb + 6, sh, pos, bint, ff);
+    if (jb + 5 < n) back_step<DIM, FAST, 5>(a, cell, jb + 5, sh, pos, bint, ff);
+    if (jb + 4 < n) back_step<DIM, FAST, 4>(a, cell, jb + 4, sh, pos, bint, ff);
+    if (jb + 3 < n) back_step<DIM, FAST, 3>(a, cell, jb + 3, sh, pos, bint, ff);
+    if (jb + 2 < n) back_step<DIM, FAST, 2>(a, cell, jb + 2, sh, pos, bint, ff);
+    if (jb + 1 < n) back_step<DIM, FAST, 1>(a, cell, jb + 1, sh, pos, bint, ff);
+    back_step<DIM, FAST, 0>(a, cell, jb + 0, sh, pos, bint, ff);
+  }
   int worst = bad;
   for (int o = 32; o > 0; o >>= 1) worst = max(worst, __shfl_xor(worst, o));
   if (lane == 0) a.status[cell] = worst;
+  if (FAST) {
+    wave_sync();
+    fast_epilogue<DIM>(a, cell, sh, cf, rf, ff, vol, sacc, bint);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -822,6 +1238,9 @@ struct msfec_q_batch {
   double* d_cvec = nullptr;
   int* d_cit = nullptr;
   int coarse_nc = -1;
+  // scalar coefficient and piecewise-constant load: one fused kernel per
+  // batch; otherwise assembly tables + band kernel + element kernel
+  bool fast = false;
 };
 
 namespace {
@@ -962,6 +1381,7 @@ msfec_status msfec_q_create(const msfec_q_params* params, int n_cells, const dou
                  : (p.rhs_mode == MSFEC_RHS_CELL) ? (size_t)n_cells * c.nfc
                                                   : (size_t)n_cells * c.nfc * nq;
   h->reac_count = (p.reaction_mode == MSFEC_REACTION_CELL) ? (size_t)n_cells * c.nfc : 0;
+  h->fast = (p.coef_mode == MSFEC_COEF_SCALAR_CELL) && (p.rhs_mode != MSFEC_RHS_QP);
 
   msfec_status st = MSFEC_OK;
 #define TRY(x)              \
@@ -977,9 +1397,11 @@ msfec_status msfec_q_create(const msfec_q_params* params, int n_cells, const dou
   TRY(dalloc(h, &h->d_coef, h->coef_count));
   TRY(dalloc(h, &h->d_f, h->rhs_count));
   TRY(dalloc(h, &h->d_reac, h->reac_count));
-  TRY(dalloc(h, &h->d_slo, (size_t)n_cells * c.nn * c.nlow));
-  TRY(dalloc(h, &h->d_F, (size_t)n_cells * c.nn));
-  TRY(dalloc(h, &h->d_rhs, (size_t)n_cells * c.n * kMaxNb));
+  if (!h->fast) {
+    TRY(dalloc(h, &h->d_slo, (size_t)n_cells * c.nn * c.nlow));
+    TRY(dalloc(h, &h->d_F, (size_t)n_cells * c.nn));
+    TRY(dalloc(h, &h->d_rhs, (size_t)n_cells * c.n * kMaxNb));
+  }
   TRY(dalloc(h, &h->d_L, (size_t)n_cells * c.n * kWin));
   TRY(dalloc(h, &h->d_W, (size_t)n_cells * c.n * kMaxNb));
   TRY(dalloc(h, &h->d_X, (size_t)n_cells * c.n * kMaxNb));
@@ -1098,33 +1520,66 @@ msfec_status msfec_q_compute(msfec_q_handle h, void* stream) {
   const size_t cstride = h->coef_count / h->n_cells;
   const size_t fstride = h->rhs_count ? h->rhs_count / h->n_cells : 0;
   const size_t rstride = h->reac_count ? h->reac_count / h->n_cells : 0;
+  const bool fast = h->fast;
   if (tm) MSFEC_HIP(hipEventRecord(ev[0], s));
-  hipLaunchKernelGGL(q_assemble_kernel, dim3(h->n_cells), dim3(256), 0, s, c, h->d_verts, h->d_coef,
-                     cstride, h->rhs_count ? h->d_f : nullptr, fstride,
-                     h->reac_count ? h->d_reac : nullptr, rstride, h->d_slo, h->d_F, h->d_rhs);
-  MSFEC_HIP(hipGetLastError());
+  if (!fast) {
+    hipLaunchKernelGGL(q_assemble_kernel, dim3(h->n_cells), dim3(256), 0, s, c, h->d_verts,
+                       h->d_coef, cstride, h->rhs_count ? h->d_f : nullptr, fstride,
+                       h->reac_count ? h->d_reac : nullptr, rstride, h->d_slo, h->d_F, h->d_rhs);
+    MSFEC_HIP(hipGetLastError());
+  }
   if (tm) MSFEC_HIP(hipEventRecord(ev[1], s));
-  BandArgs a;
+  BandArgs a{};
   a.n = c.n;
-  a.nlow = c.nlow;
+  a.ni = c.ni;
+  a.N = c.N;
+  a.S = c.S;
   a.nn = c.nn;
+  a.nfc = c.nfc;
   for (int t = 0; t < kMaxLow; ++t) a.off_int[t] = c.off_int[t];
   a.Slo = h->d_slo;
   a.Rhs = h->d_rhs;
   a.inode = h->d_inode;
   a.imask = h->d_imask;
+  a.verts = h->d_verts;
+  a.coef = h->d_coef;
+  a.reac = h->reac_count ? h->d_reac : nullptr;
+  a.fcell = h->rhs_count ? h->d_f : nullptr;
+  a.reaction_mode = c.reaction_mode;
+  a.rhs_mode = c.rhs_mode;
+  a.reaction_const = c.reaction_const;
+  a.rhs_const = c.rhs_const;
   a.Lcol = h->d_L;
   a.Wf = h->d_W;
   a.X = h->d_X;
+  a.Psi = h->d_psi;
+  a.M = h->d_M;
+  a.bvec = h->d_b;
   a.status = h->d_status;
-  hipLaunchKernelGGL(q_band_kernel, dim3(h->n_cells), dim3(64), 0, s, a);
+  if (fast) {
+    const size_t dyn = (size_t)c.nfc *
+                       (1 + (c.reaction_mode == MSFEC_REACTION_CELL ? 1 : 0) +
+                        (c.rhs_mode == MSFEC_RHS_CELL ? 1 : 0)) *
+                       sizeof(double);
+    if (c.dim == 3)
+      hipLaunchKernelGGL((q_band_kernel<3, true>), dim3(h->n_cells), dim3(64), dyn, s, a);
+    else
+      hipLaunchKernelGGL((q_band_kernel<2, true>), dim3(h->n_cells), dim3(64), dyn, s, a);
+  } else {
+    if (c.dim == 3)
+      hipLaunchKernelGGL((q_band_kernel<3, false>), dim3(h->n_cells), dim3(64), 0, s, a);
+    else
+      hipLaunchKernelGGL((q_band_kernel<2, false>), dim3(h->n_cells), dim3(64), 0, s, a);
+  }
   MSFEC_HIP(hipGetLastError());
   if (tm) MSFEC_HIP(hipEventRecord(ev[2], s));
-  const size_t shm = (size_t)(2 * c.nv * c.nn + 4 * 72) * sizeof(double);
-  if (shm > 160 * 1024) return fail(MSFEC_ERR_UNSUPPORTED, "element kernel LDS too large");
-  hipLaunchKernelGGL(q_element_kernel, dim3(h->n_cells), dim3(256), shm, s, c, h->d_slo, h->d_F,
-                     h->d_X, h->d_psi, h->d_M, h->d_b);
-  MSFEC_HIP(hipGetLastError());
+  if (!fast) {
+    const size_t shm = (size_t)(2 * c.nv * c.nn + 4 * 72) * sizeof(double);
+    if (shm > 160 * 1024) return fail(MSFEC_ERR_UNSUPPORTED, "element kernel LDS too large");
+    hipLaunchKernelGGL(q_element_kernel, dim3(h->n_cells), dim3(256), shm, s, c, h->d_slo,
+                       h->d_F, h->d_X, h->d_psi, h->d_M, h->d_b);
+    MSFEC_HIP(hipGetLastError());
+  }
   if (tm) MSFEC_HIP(hipEventRecord(ev[3], s));
   h->computed = true;
   return MSFEC_OK;

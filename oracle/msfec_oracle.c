@@ -604,6 +604,12 @@ int oq_basis_cell(const oq_problem *p, const double *verts, const double *coef,
     }
   m.K = (double *)calloc((size_t)m.nn * m.nst, sizeof(double));
   m.F = (double *)calloc((size_t)m.nn, sizeof(double));
+  /* callers that only want M and b may pass Psi = NULL */
+  double *psi_own = NULL;
+  if (!Psi) {
+    psi_own = (double *)malloc(sizeof(double) * (size_t)m.nv * m.nn);
+    Psi = psi_own;
+  }
   double *g = (double *)calloc((size_t)m.nv * m.nn, sizeof(double));
   double *tmp = (double *)malloc(sizeof(double) * m.nn);
   int status = assemble(p, &m, coef, f, reac);
@@ -650,6 +656,7 @@ int oq_basis_cell(const oq_problem *p, const double *verts, const double *coef,
   free(m.F);
   free(g);
   free(tmp);
+  free(psi_own);
   return status;
 }
 

@@ -64,7 +64,8 @@ typedef struct {
  *           ncomp = 3 (xx,xy,yy) in 2-D or 6 (xx,xy,xz,yy,yz,zz) in 3-D
  *   f     : RHS_CELL -> N^dim; RHS_QP -> N^dim x 2^dim
  *   reac  : REACTION_CELL -> N^dim
- * Outputs per cell: M (nb x nb, row = test basis), b (nb), Psi (nb x (N+1)^dim),
+ * Outputs per cell: M (nb x nb, row = test basis), b (nb), Psi (nb x (N+1)^dim;
+ * NULL when not wanted),
  * nb = 2^dim, fine nodes lexicographic x fastest.
  * Returns 0 on success, 1 if CG did not converge, 2 if a pivot was not positive,
  * -1 on invalid input. */

@@ -198,3 +198,40 @@ def test_coarse_solve_and_reconstruct(oracle_mod, dim, nc, l):
     assert np.abs(u - uo).max() <= 1e-10 * np.abs(uo).max()
     ufo = oracle_mod.reconstruct(dim, nc, l, uo, Psi)
     assert np.abs(uf - ufo).max() <= 1e-10 * np.abs(ufo).max()
+
+
+@pytest.mark.parametrize("dim,l", [(2, 4), (3, 3)])
+def test_fused_path_rhs_cell_and_reaction(oracle_mod, dim, l):
+    """Scalar coefficient with a per-cell load and a per-cell reaction runs
+    through the fused kernel; checked against the oracle."""
+    print(f"case fused rhs_cell dim={dim} l={l}", flush=True)
+    rng = np.random.default_rng(40 + dim)
+    n_cells, N = 6, 1 << l
+    verts = random_boxes(rng, dim, n_cells)
+    coef = np.exp(rng.standard_normal((n_cells, N ** dim)))
+    f = rng.standard_normal((n_cells, N ** dim))
+    reac = rng.random((n_cells, N ** dim))
+    g = run_gpu(verts, dim, l, coef, f=f, rhs_mode=1, reac=reac, reaction_mode=2)
+    o = run_oracle(oracle_mod, verts, dim, l, coef, f=f, rhs_mode=1, reac=reac, reaction_mode=2)
+    assert rel(g["M"], o["M"]) < TOL
+    assert rel(g["b"], o["b"]) < TOL
+    assert rel(g["Psi"], o["Psi"]) < TOL
+
+
+@pytest.mark.parametrize("dim,l", [(2, 3), (3, 3)])
+def test_fused_equals_general_path(dim, l):
+    """A I given as a scalar (fused kernel) and as a tensor at the quadrature
+    points (assembly + band + element kernels) give the same basis."""
+    print(f"case fused_vs_general dim={dim} l={l}", flush=True)
+    rng = np.random.default_rng(50 + dim)
+    n_cells, N, nq = 5, 1 << l, 1 << dim
+    verts = random_boxes(rng, dim, n_cells)
+    a = np.exp(1.5 * rng.standard_normal((n_cells, N ** dim)))
+    T = np.zeros((n_cells, N ** dim, nq, 3 if dim == 2 else 6))
+    for c in ([0, 2] if dim == 2 else [0, 3, 5]):
+        T[..., c] = a[:, :, None]
+    g1 = run_gpu(verts, dim, l, a)
+    g2 = run_gpu(verts, dim, l, T, coef_mode=1)
+    assert rel(g1["M"], g2["M"]) < TOL
+    assert rel(g1["b"], g2["b"]) < TOL
+    assert rel(g1["Psi"], g2["Psi"]) < TOL
