@@ -1,0 +1,151 @@
+// QBasis / QBatch shim over the C-ABI (see msfec_q_basis.hpp).
+#include "msfec_q_basis.hpp"
+
+#include <string>
+
+namespace msfec {
+
+namespace {
+void check(msfec_status st, const char* what) {
+  if (st != MSFEC_OK)
+    throw std::runtime_error(std::string(what) + ": " + msfec_last_error());
+}
+}  // namespace
+
+QBatch::QBatch(const ParametersBasis& prm) : prm_(prm) {
+  if (prm.dim != 2 && prm.dim != 3) throw std::invalid_argument("dim must be 2 or 3");
+  const int S = (1 << prm.n_refine_local) + 1;
+  n_fine_nodes_ = (prm.dim == 2) ? S * S : S * S * S;
+}
+
+QBatch::~QBatch() {
+  if (handle_) msfec_q_destroy(handle_);
+}
+
+std::size_t QBatch::add_cell(const std::vector<double>& vertices, const std::vector<double>& coef,
+                             const std::vector<double>& f, const std::vector<double>& reac) {
+  const std::size_t nvd = (std::size_t)(1 << prm_.dim) * prm_.dim;
+  if (vertices.size() != nvd) throw std::invalid_argument("wrong number of vertex coordinates");
+  if (n_cells_ > 0 && coef.size() * n_cells_ != coef_.size())
+    throw std::invalid_argument("coefficient block size differs between cells");
+  verts_.insert(verts_.end(), vertices.begin(), vertices.end());
+  coef_.insert(coef_.end(), coef.begin(), coef.end());
+  f_.insert(f_.end(), f.begin(), f.end());
+  reac_.insert(reac_.end(), reac.begin(), reac.end());
+  computed_ = false;
+  return n_cells_++;
+}
+
+void QBatch::compute() {
+  if (computed_) return;
+  if (n_cells_ == 0) throw std::logic_error("no cells registered");
+  if (handle_) {
+    msfec_q_destroy(handle_);
+    handle_ = nullptr;
+  }
+  msfec_q_params p{};
+  p.dim = prm_.dim;
+  p.n_refine_local = (int)prm_.n_refine_local;
+  p.coef_mode = prm_.coef_mode;
+  p.rhs_mode = prm_.rhs_mode;
+  p.rhs_const = prm_.rhs_const;
+  p.reaction_mode = prm_.reaction_mode;
+  p.reaction_const = prm_.reaction_const;
+  p.solver = MSFEC_SOLVER_BAND_FP64;
+  p.flags = 0;
+  check(msfec_q_create(&p, (int)n_cells_, verts_.data(), MSFEC_HOST, &handle_), "msfec_q_create");
+  check(msfec_q_set_coefficient(handle_, coef_.data(), coef_.size(), MSFEC_HOST),
+        "msfec_q_set_coefficient");
+  if (!f_.empty()) check(msfec_q_set_rhs(handle_, f_.data(), f_.size(), MSFEC_HOST), "msfec_q_set_rhs");
+  if (!reac_.empty())
+    check(msfec_q_set_reaction(handle_, reac_.data(), reac_.size(), MSFEC_HOST),
+          "msfec_q_set_reaction");
+  check(msfec_q_compute(handle_, nullptr), "msfec_q_compute");
+  const int nb = n_basis();
+  M_.resize(n_cells_ * nb * nb);
+  b_.resize(n_cells_ * nb);
+  psi_.resize(n_cells_ * nb * (std::size_t)n_fine_nodes_);
+  status_.resize(n_cells_);
+  check(msfec_q_get_element(handle_, M_.data(), b_.data(), MSFEC_HOST), "msfec_q_get_element");
+  check(msfec_q_get_basis(handle_, psi_.data(), MSFEC_HOST), "msfec_q_get_basis");
+  check(msfec_q_get_cell_status(handle_, status_.data()), "msfec_q_get_cell_status");
+  computed_ = true;
+}
+
+const double* QBatch::element_matrix(std::size_t slot) const {
+  return M_.data() + slot * n_basis() * n_basis();
+}
+const double* QBatch::element_rhs(std::size_t slot) const { return b_.data() + slot * n_basis(); }
+const double* QBatch::basis(std::size_t slot, int i) const {
+  return psi_.data() + (slot * n_basis() + i) * (std::size_t)n_fine_nodes_;
+}
+int QBatch::cell_status(std::size_t slot) const { return status_[slot]; }
+
+std::vector<double> QBatch::coarse_solve(int nc, double abs_tol, double rel_tol, int max_iter,
+                                         int* iterations, double* residual) {
+  compute();
+  std::vector<double> w(n_cells_ * n_basis());
+  check(msfec_coarse_solve(handle_, nc, abs_tol, rel_tol, max_iter, nullptr, MSFEC_HOST, w.data(),
+                           MSFEC_HOST, iterations, residual),
+        "msfec_coarse_solve");
+  return w;
+}
+
+QBasis::QBasis(std::shared_ptr<QBatch> batch, const std::vector<double>& corner_points,
+               const std::vector<double>& coef, const std::vector<double>& f,
+               const std::vector<double>& reac)
+    : batch_(std::move(batch)) {
+  if (!batch_) throw std::invalid_argument("null batch");
+  slot_ = batch_->add_cell(corner_points, coef, f, reac);
+}
+
+void QBasis::run() { run_called_ = true; }
+
+void QBasis::ensure() const {
+  if (!run_called_) throw std::logic_error("QBasis::run() must be called first");
+  batch_->compute();
+  if (batch_->cell_status(slot_) != MSFEC_CELL_OK)
+    throw std::runtime_error("local solve failed in cell slot " + std::to_string(slot_));
+  if (M_.empty()) {
+    const int nb = batch_->n_basis();
+    M_.assign(batch_->element_matrix(slot_), batch_->element_matrix(slot_) + nb * nb);
+    b_.assign(batch_->element_rhs(slot_), batch_->element_rhs(slot_) + nb);
+  }
+}
+
+const std::vector<double>& QBasis::get_global_element_matrix() const {
+  ensure();
+  return M_;
+}
+
+const std::vector<double>& QBasis::get_global_element_rhs() const {
+  ensure();
+  return b_;
+}
+
+std::vector<double> QBasis::get_basis(int i) const {
+  ensure();
+  const double* p = batch_->basis(slot_, i);
+  return std::vector<double>(p, p + batch_->n_fine_nodes());
+}
+
+void QBasis::set_global_weights(const std::vector<double>& weights) {
+  ensure();
+  const int nb = batch_->n_basis();
+  if ((int)weights.size() != nb) throw std::invalid_argument("expected 2^dim weights");
+  weights_ = weights;
+  solution_.assign(batch_->n_fine_nodes(), 0.0);
+  for (int i = 0; i < nb; ++i) {
+    const double* p = batch_->basis(slot_, i);
+    for (int n = 0; n < batch_->n_fine_nodes(); ++n) solution_[n] = solution_[n] + weights_[i] * p[n];
+  }
+}
+
+const std::vector<double>& QBasis::get_global_solution() const { return solution_; }
+
+int QBasis::status() const {
+  batch_->compute();
+  return batch_->cell_status(slot_);
+}
+
+}  // namespace msfec

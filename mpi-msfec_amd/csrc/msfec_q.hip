@@ -333,27 +333,31 @@ struct Dirs {
   static constexpr int NLOW = CENTER + 1;
 };
 
+// LDS layouts pad every group of 8 doubles to 9 so that the 8 distinct
+// groups read together by a wave fall into distinct banks.
+__device__ __forceinline__ int pad8(int x) { return 9 * (x >> 3) + (x & 7); }
+
 struct BandSmem {
-  double ubuf[kWin];
-  double nbuf[kWin];
+  double ubuf[72];   // pivot row, slot x at pad8(x)
+  double nbuf[72];   // incoming row, slot x at pad8(x)
   double ybuf[8];
   double kv[8];
   double mv[8];
   double stg_low[8][16];
   double stg_rhs[8][8];
   // phase-dependent pool:
-  //   forward : RHS partials [64][8]
-  //   backward: w of the group [8][8] | L columns of the group [8][64] | x window [8][64]
-  //   epilogue: psi of 64 nodes [64][8] | (K psi) of 64 nodes [64][8] | F of 64 nodes [64]
-  double pool[64 + 8 * kWin + 8 * kWin];
+  //   forward : RHS partials [64][9]
+  //   backward: w of the group [8][8] | L columns of the group [8][72] | x window [64][9]
+  //   epilogue: psi of 64 nodes [64][9] | (K psi)_B of 64 nodes [64][9] | F of 64 nodes [64]
+  double pool[64 + 576 + 576];
 };
 #define SH_PART(sh) ((sh).pool)
 #define SH_WBUF(sh) ((sh).pool)
 #define SH_LBUF(sh) ((sh).pool + 64)
-#define SH_XRING(sh) ((sh).pool + 64 + 8 * kWin)
+#define SH_XRING(sh) ((sh).pool + 64 + 576)
 #define SH_PN(sh) ((sh).pool)
-#define SH_KP(sh) ((sh).pool + 8 * kWin)
-#define SH_FS(sh) ((sh).pool + 16 * kWin)
+#define SH_KP(sh) ((sh).pool + 576)
+#define SH_FS(sh) ((sh).pool + 1152)
 
 // direction e -> component d in {-1,0,1}
 template <int DIM>
@@ -462,13 +466,13 @@ __device__ __forceinline__ void stage_rows(const BandArgs& a, int cell, int jbas
       }
     }
 #pragma unroll
-    for (int k = 0; k < 8; ++k) SH_PART(sh)[lane * 8 + k] = rhs[k];
+    for (int k = 0; k < 8; ++k) SH_PART(sh)[lane * 9 + k] = rhs[k];
     wave_sync();
     {
       const int k = s;
       double t = 0.0;
 #pragma unroll
-      for (int q = 0; q < 8; ++q) t += SH_PART(sh)[((r << 3) + q) * 8 + k];
+      for (int q = 0; q < 8; ++q) t += SH_PART(sh)[((r << 3) + q) * 9 + k];
       sh.stg_rhs[r][k] = t;
     }
     wave_sync();
@@ -494,7 +498,7 @@ __device__ __forceinline__ void band_insert(const BandArgs& a, int jn, int G, in
   const int lane = threadIdx.x;
   const int p = lane >> 3, q = lane & 7;
   const int s = 8 * G + I;
-  sh.nbuf[lane] = 0.0;
+  sh.nbuf[pad8(lane)] = 0.0;
   wave_sync();
   if (jn < a.n) {
     // only nonzero entries: they belong to distinct interior neighbours and
@@ -503,13 +507,13 @@ __device__ __forceinline__ void band_insert(const BandArgs& a, int jn, int G, in
     // node per direction)
     if (lane < D::NLOW) {
       const double v = sh.stg_low[I][lane];
-      if (v != 0.0) sh.nbuf[(s - my_off) & (kWin - 1)] = v;
+      if (v != 0.0) sh.nbuf[pad8((s - my_off) & (kWin - 1))] = v;
     }
   } else if (lane == 0) {
-    sh.nbuf[s] = 1.0;  // past the end: an isolated unit diagonal
+    sh.nbuf[pad8(s)] = 1.0;  // past the end: an isolated unit diagonal
   }
   wave_sync();
-  const int base = (p == G) ? 8 * q : 8 * p;
+  const int base = (p == G) ? 9 * q : 9 * p;
   double v[8];
 #pragma unroll
   for (int b = 0; b < 8; ++b) v[b] = sh.nbuf[base + b];
@@ -537,25 +541,25 @@ __device__ __forceinline__ void band_step(const BandArgs& a, int cell, int j, in
   const int s = 8 * G + I;
   if (p == G) {
 #pragma unroll
-    for (int b = 0; b < 8; ++b) sh.ubuf[8 * q + b] = A[I][b];
+    for (int b = 0; b < 8; ++b) sh.ubuf[9 * q + b] = A[I][b];
   }
   if (lane == s) {
 #pragma unroll
     for (int k = 0; k < 8; ++k) sh.ybuf[k] = y[k];
   }
   wave_sync();
-  const double d = sh.ubuf[s];
+  const double d = sh.ubuf[pad8(s)];
   if (!(d > 0.0)) bad = max(bad, (d == d && d > -INFINITY) ? (int)MSFEC_CELL_NOT_SPD
                                                              : (int)MSFEC_CELL_NONFINITE);
   const double rd = 1.0 / d;
   double l[8], u[8], ys[8];
 #pragma unroll
-  for (int r = 0; r < 8; ++r) l[r] = sh.ubuf[8 * p + r] * rd;
+  for (int r = 0; r < 8; ++r) l[r] = sh.ubuf[9 * p + r] * rd;
 #pragma unroll
-  for (int b = 0; b < 8; ++b) u[b] = sh.ubuf[8 * q + b];
+  for (int b = 0; b < 8; ++b) u[b] = sh.ubuf[9 * q + b];
 #pragma unroll
   for (int k = 0; k < 8; ++k) ys[k] = sh.ybuf[k];
-  const double lm = sh.ubuf[lane] * rd;
+  const double lm = sh.ubuf[pad8(lane)] * rd;
   const double wk = sh.ybuf[lane & 7] * rd;
   // Schur term of the element matrix: lane (i,k) accumulates z_j[i] z_j[k] / d_j
   sacc = fma(sh.ybuf[lane >> 3], wk, sacc);
@@ -587,15 +591,15 @@ __device__ __forceinline__ void back_step(const BandArgs& a, int cell, int j, Ba
   double part = 0.0;
 #pragma unroll
   for (int t = 0; t < 8; ++t) {
-    const double lv = (m == 0 && t == 0) ? 0.0 : SH_LBUF(sh)[R * kWin + 8 * m + t];
-    part = fma(lv, SH_XRING(sh)[k * kWin + ((j + 8 * m + t) & (kWin - 1))], part);
+    const double lv = (m == 0 && t == 0) ? 0.0 : SH_LBUF(sh)[R * 72 + 9 * m + t];
+    part = fma(lv, SH_XRING(sh)[9 * ((j + 8 * m + t) & (kWin - 1)) + k], part);
   }
   part += __shfl_xor(part, 8);
   part += __shfl_xor(part, 16);
   part += __shfl_xor(part, 32);
   const double xj = SH_WBUF(sh)[R * 8 + k] - part;
   if (m == 0) {
-    SH_XRING(sh)[k * kWin + (j & (kWin - 1))] = xj;
+    SH_XRING(sh)[9 * (j & (kWin - 1)) + k] = xj;
     if (FAST) {
       // interior entry of Psi and its share of b = Psi^T F
       const int cx = pos.x + 1, cy = pos.y + 1, cz = (DIM == 3) ? pos.z + 1 : 0;
@@ -679,7 +683,7 @@ __device__ __forceinline__ void fast_epilogue(const BandArgs& a, int cell, BandS
       for (int k = 0; k < D::NV; ++k) {
         const double v = lift_k<DIM>(k, c, invN);
         psi_out[(size_t)k * a.nn + node] = v;
-        pn_s[lane * 8 + k] = v;
+        pn_s[lane * 9 + k] = v;
       }
       // F[node] = sum over the cells touching the node of f_z |z| / 2^dim
 #pragma unroll
@@ -719,19 +723,19 @@ __device__ __forceinline__ void fast_epilogue(const BandArgs& a, int cell, BandS
       }
     } else {
 #pragma unroll
-      for (int k = 0; k < D::NV; ++k) pn_s[lane * 8 + k] = 0.0;
+      for (int k = 0; k < D::NV; ++k) pn_s[lane * 9 + k] = 0.0;
     }
     fs_s[lane] = fs;
 #pragma unroll
-    for (int k = 0; k < D::NV; ++k) kp_s[lane * 8 + k] = kp[k];
+    for (int k = 0; k < D::NV; ++k) kp_s[lane * 9 + k] = kp[k];
     wave_sync();
     if (mi_ < D::NV && mk_ < D::NV) {
 #pragma unroll 8
-      for (int l = 0; l < 64; ++l) pacc = fma(pn_s[l * 8 + mi_], kp_s[l * 8 + mk_], pacc);
+      for (int l = 0; l < 64; ++l) pacc = fma(pn_s[l * 9 + mi_], kp_s[l * 9 + mk_], pacc);
     }
     if (lane < D::NV) {
 #pragma unroll 8
-      for (int l = 0; l < 64; ++l) bacc = fma(pn_s[l * 8 + lane], fs_s[l], bacc);
+      for (int l = 0; l < 64; ++l) bacc = fma(pn_s[l * 9 + lane], fs_s[l], bacc);
     }
     wave_sync();
   }
@@ -849,7 +853,7 @@ __global__ __launch_bounds__(64, 2) void q_band_kernel(BandArgs a) {
   // the next group of 8 steps are prefetched into registers.
   __threadfence_block();
   __syncthreads();
-  for (int t = lane; t < 8 * kWin; t += 64) SH_XRING(sh)[t] = 0.0;
+  for (int t = lane; t < 576; t += 64) SH_XRING(sh)[t] = 0.0;
   const double* Lc = a.Lcol + (size_t)cell * n * kWin;
   const double* Wc = a.Wf + (size_t)cell * n * kMaxNb;
   const int nchunk = (n + 7) >> 3;
@@ -865,7 +869,7 @@ __global__ __launch_bounds__(64, 2) void q_band_kernel(BandArgs a) {
   for (int cch = nchunk - 1; cch >= 0; --cch) {
     wave_sync();
 #pragma unroll
-    for (int t = 0; t < 8; ++t) SH_LBUF(sh)[(lane >> 3) * kWin + 8 * (lane & 7) + t] = lp[t];
+    for (int t = 0; t < 8; ++t) SH_LBUF(sh)[(lane >> 3) * 72 + 9 * (lane & 7) + t] = lp[t];
     SH_WBUF(sh)[lane] = wp;
     wave_sync();
     if (cch > 0) {

@@ -28,6 +28,24 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found")
 
 
+CPP = os.path.join(PKG_ROOT, "cpp")
+DRIVER = os.path.join(CPP, "q_multiscale_driver")
+
+
+def build_driver(verbose: bool = False) -> str:
+    """The QBasis/QBatch C++ shim and the QMultiscale-shaped driver: host C++
+    only, linked against the HIP library through the C-ABI."""
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", f"-I{INCLUDE}", f"-I{CPP}",
+           os.path.join(CPP, "msfec_q_basis.cpp"), os.path.join(CPP, "q_multiscale_driver.cpp"),
+           f"-L{HERE}", "-l:libmsfec_hip.so", "-Wl,-rpath,$ORIGIN/../msfec_amd",
+           "-o", DRIVER + ".tmp"]
+    if verbose:
+        print(" ".join(cmd))
+    subprocess.run(cmd, check=True)
+    os.replace(DRIVER + ".tmp", DRIVER)
+    return DRIVER
+
+
 def build(verbose: bool = False) -> str:
     cmd = [hipcc(), *HIPCC_FLAGS, f"-I{INCLUDE}",
            *[os.path.join(CSRC, s) for s in SOURCES], "-o", LIB + ".tmp"]
@@ -35,6 +53,7 @@ def build(verbose: bool = False) -> str:
         print(" ".join(cmd))
     subprocess.run(cmd, check=True)
     os.replace(LIB + ".tmp", LIB)
+    build_driver(verbose)
     return LIB
 
 

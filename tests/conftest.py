@@ -17,7 +17,8 @@ def box_vertices(dim, lo, hi):
     """2^dim vertices of an axis-aligned box in deal.II vertex order."""
     import numpy as np
     nv = 1 << dim
-    return np.array([[hi[d] if (v >> d) & 1 else lo[d] for d in range(dim)] for v in range(nv)])
+    return np.array([[hi[d] if (v >> d) & 1 else lo[d] for d in range(dim)] for v in range(nv)],
+                    dtype=np.float64)
 
 
 @pytest.fixture(scope="session")

@@ -1,0 +1,103 @@
+"""The C-ABI library loads, exports every symbol include/msfec.h declares, and
+rejects invalid requests before touching the GPU.  CPU only: no compute call."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import box_vertices
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    text = open(os.path.join(REPO, "include", "msfec.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(msfec_[a-z_0-9]+)\s*\(", text)))
+
+
+@pytest.fixture(scope="module")
+def lib():
+    from msfec_amd import _lib
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.fail("libmsfec_hip.so is not built (run __graft_entry__.build())")
+    return _lib
+
+
+def test_exports_every_declared_symbol(lib):
+    L = lib.load()
+    names = declared_functions()
+    assert "msfec_q_create" in names and len(names) >= 15
+    for n in names:
+        assert hasattr(L, n), n
+    assert set(names) == set(lib.EXPORTED)
+
+
+def test_version_string(lib):
+    assert b"gfx950" in lib.load().msfec_version()
+
+
+def _create(lib, dim=3, l=2, verts=None, **kw):
+    L = lib.load()
+    p = lib.Params()
+    p.dim, p.n_refine_local, p.coef_mode, p.rhs_mode = dim, l, kw.get("coef_mode", 0), 0
+    p.rhs_const, p.reaction_mode, p.reaction_const, p.solver = 1.0, 0, 0.0, kw.get("solver", 0)
+    if verts is None:
+        verts = box_vertices(dim, [0.0] * dim, [1.0] * dim)[None]
+    verts = np.ascontiguousarray(verts, dtype=np.float64)
+    h = ctypes.c_void_p()
+    st = L.msfec_q_create(ctypes.byref(p), verts.shape[0], ctypes.c_void_p(verts.ctypes.data),
+                          lib.HOST, ctypes.byref(h))
+    return st, L.msfec_last_error().decode()
+
+
+def test_rejects_bad_dimension(lib):
+    st, msg = _create(lib, dim=4)
+    assert st == lib.ERR_INVALID and "dim" in msg
+
+
+def test_rejects_too_wide_band(lib):
+    st, msg = _create(lib, dim=3, l=4)
+    assert st == lib.ERR_UNSUPPORTED and "bandwidth" in msg
+
+
+def test_rejects_non_box_cell(lib):
+    v = box_vertices(3, [0, 0, 0], [1, 1, 1])
+    v[5, 1] += 0.1
+    st, msg = _create(lib, verts=v[None])
+    assert st == lib.ERR_UNSUPPORTED and "axis-aligned" in msg
+
+
+def test_rejects_degenerate_cell(lib):
+    v = box_vertices(2, [0, 0], [0, 1])
+    st, msg = _create(lib, dim=2, verts=v[None])
+    assert st == lib.ERR_INVALID
+
+
+def test_rejects_unknown_modes(lib):
+    st, _ = _create(lib, coef_mode=7)
+    assert st == lib.ERR_INVALID
+    st, _ = _create(lib, solver=9)
+    assert st == lib.ERR_INVALID
+
+
+def test_null_arguments(lib):
+    L = lib.load()
+    assert L.msfec_q_create(None, 1, None, 0, None) == lib.ERR_INVALID
+    assert L.msfec_q_compute(None, None) == lib.ERR_INVALID
+    assert L.msfec_q_get_element(None, None, None, 0) == lib.ERR_INVALID
+    L.msfec_q_destroy(None)
+
+
+def test_product_path_has_no_fallback():
+    """The package refuses to run without the HIP library (no CPU path)."""
+    from msfec_amd import _lib
+    with pytest.raises(RuntimeError, match="no CPU fallback"):
+        saved = _lib._lib
+        try:
+            _lib._lib = None
+            _lib.load("/nonexistent/libmsfec_hip.so")
+        finally:
+            _lib._lib = saved
