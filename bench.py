@@ -115,27 +115,36 @@ def main():
     # The library owns its device buffers and stream work; torch is not
     # imported so that exactly one HIP runtime (the one the library was built
     # against) lives in the process.
-    from msfec_amd import QBasisBatch, fields as F
+    from msfec_amd import QBasisBatch, _lib as L, fields as F
 
     verts = F.coarse_vertices(DIM, NC)
     coef = F.lognormal_coefficient(DIM, NC, NREF, SIGMA, CORR, SEED)
 
-    qb = QBasisBatch(verts, dim=DIM, n_refine_local=NREF, timing=True)
-    qb.set_coefficient(coef)          # host -> HBM copy, before the timed region
-    info = qb.info
-    for _ in range(args.warmup):
-        qb.run()
-    qb.synchronize()
-    if args.warmup > 0:
-        qb.kernel_times()  # drop the warm-up events
+    def measure(solver, steps, warmup):
+        qb = QBasisBatch(verts, dim=DIM, n_refine_local=NREF, solver=solver, timing=True)
+        qb.set_coefficient(coef)          # host -> HBM copy, before the timed region
+        for _ in range(warmup):
+            qb.run()
+        qb.synchronize()
+        if warmup > 0:
+            qb.kernel_times()  # drop the warm-up events
+        qb.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            qb.run()
+        qb.synchronize()
+        wall = time.perf_counter() - t0
+        kt = qb.kernel_times()  # HIP events on the launch stream, averaged over the timed steps
+        return qb, wall, kt
 
-    qb.synchronize()
-    wall0 = time.perf_counter()
-    for _ in range(args.steps):
-        qb.run()
-    qb.synchronize()
-    wall = time.perf_counter() - wall0
-    kt = qb.kernel_times()  # HIP events on the launch stream, averaged over the timed steps
+    # the vector-ALU variant of the same FP64 band algorithm, timed beside
+    qb0, wall0, kt0 = measure(L.SOLVER_BAND_FP64, max(2, args.steps // 4), min(args.warmup, 1))
+    valu = {"cells_per_s": round(qb0.info.n_cells / (wall0 / max(2, args.steps // 4)), 1),
+            "band_kernel_ms": round(kt0[1], 4)}
+    qb0.close()
+    # headline: the matrix-core variant
+    qb, wall, kt = measure(L.SOLVER_BAND_FP64_MFMA, args.steps, args.warmup)
+    info = qb.info
     ms_step = wall * 1e3 / args.steps
     status = qb.get_cell_status()
     cells_per_s = info.n_cells / (ms_step * 1e-3)
@@ -154,7 +163,7 @@ def main():
     else:
         roof = {"bound": "mfma", "achieved": round(tflops, 3), "peak": FP64_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(f_frac, 4), "traffic": None}
-    roof.update({"kernel": "q_band_kernel", "kernel_ms": round(band_ms, 4),
+    roof.update({"kernel": "q_band_mfma_kernel", "kernel_ms": round(band_ms, 4),
                  "flops_per_cell": flops_cell, "bytes_per_cell": bytes_cell,
                  "fp64_tflops": round(tflops, 3), "hbm_gbs": round(gbs, 1),
                  "dense_equiv_tflops": round(dense_flops_per_cell(n, 8) * info.n_cells /
@@ -175,9 +184,11 @@ def main():
         "data": "synthetic: seeded lognormal A=exp(g), g Gaussian sigma 1.5, corr. length 1/64 "
                 "on the 128^3 fine grid (seed 20261202), f=1",
         "config": {"workload": "3D Q1 basis construction, 16^3=4096 coarse cells x 8^3 fine "
-                               "cells (343 interior DoFs, 8 RHS), band LDL^T FP64",
-                   "n_cells": info.n_cells, "n_refine_local": NREF, "solver": "band-fp64",
-                   "parallelism": "single GPU"},
+                               "cells (343 interior DoFs, 8 RHS), banded Cholesky FP64 on the "
+                               "matrix cores",
+                   "n_cells": info.n_cells, "n_refine_local": NREF,
+                   "solver": "band-fp64-mfma", "parallelism": "single GPU"},
+        "fp64_vector_alu_path": valu,
         "kernel_ms": {"assemble": round(kt[0], 4), "band_factor_solve": round(kt[1], 4),
                       "element": round(kt[2], 4)},
         "wall_s_timed": round(wall, 3),

@@ -54,8 +54,12 @@ enum { MSFEC_COEF_SCALAR_CELL = 0, MSFEC_COEF_TENSOR_QP = 1 };
 enum { MSFEC_RHS_CONST = 0, MSFEC_RHS_CELL = 1, MSFEC_RHS_QP = 2 };
 /* optional reaction R u (the reference's QBasis has none: 0 reproduces it) */
 enum { MSFEC_REACTION_NONE = 0, MSFEC_REACTION_CONST = 1, MSFEC_REACTION_CELL = 2 };
-/* local solver */
-enum { MSFEC_SOLVER_BAND_FP64 = 0 };
+/* local solver: banded Cholesky of the interior system in FP64, eliminating
+ * one pivot per step on the vector ALUs (BAND_FP64) or four pivots per step
+ * with FP64 matrix-core updates (BAND_FP64_MFMA, the default of the Python
+ * and C++ front ends).  Both are direct solvers, the analogue of
+ * use_direct_solver_basis = true (source/Q/q_basis.cc:304-330). */
+enum { MSFEC_SOLVER_BAND_FP64 = 0, MSFEC_SOLVER_BAND_FP64_MFMA = 1 };
 /* per-cell status codes */
 enum { MSFEC_CELL_OK = 0, MSFEC_CELL_NOT_SPD = 2, MSFEC_CELL_NONFINITE = 3 };
 /* flags */

@@ -302,6 +302,7 @@ __device__ __forceinline__ void wave_sync() {
 
 struct BandArgs {
   int n, ni, N, S, nn, nfc;
+  int nq;  // super-steps of 4 pivots (MFMA variant)
   int off_int[kMaxLow];
   // general path tables (q_assemble_kernel)
   const double* Slo;      // [cell][nn][nlow]
@@ -549,8 +550,9 @@ __device__ __forceinline__ void band_step(const BandArgs& a, int cell, int j, in
   }
   wave_sync();
   const double d = sh.ubuf[pad8(s)];
-  if (!(d > 0.0)) bad = max(bad, (d == d && d > -INFINITY) ? (int)MSFEC_CELL_NOT_SPD
-                                                             : (int)MSFEC_CELL_NONFINITE);
+  // keep the first failure: later pivots of a failed cell are meaningless
+  if (!(d > 0.0) && bad == 0)
+    bad = (d == d && d > -INFINITY) ? (int)MSFEC_CELL_NOT_SPD : (int)MSFEC_CELL_NONFINITE;
   const double rd = 1.0 / d;
   double l[8], u[8], ys[8];
 #pragma unroll
@@ -893,6 +895,412 @@ __global__ __launch_bounds__(64, 2) void q_band_kernel(BandArgs a) {
   if (lane == 0) a.status[cell] = worst;
   if (FAST) {
     wave_sync();
+    fast_epilogue<DIM>(a, cell, sh, cf, rf, ff, vol, sacc, bint);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// K2 (MFMA variant): the same sliding-window band factorisation, eliminating
+// 4 pivots per super-step with FP64 matrix-core updates.
+//
+// The 64-slot window is a symmetric 4x4 grid of 16x16 tiles of which the 10
+// tiles (P >= Q) are kept, each as the 4-register accumulator of
+// v_mfma_f64_16x16x4f64 (entry (i,j) in lane j + 16 (i mod 4), register
+// i / 4).  The 4 pivots of a super-step occupy slots s0..s0+3 = rows 4g..4g+3
+// of tile row T: in every tile of that row they are register g of all lanes.
+// A super-step
+//   1. copies the 4 pivot rows to LDS,
+//   2. factors the 4x4 pivot block C C^T (every lane, redundantly),
+//   3. forms the panel V = A21 C^-T, one value per lane and tile row, in the
+//      A/B operand layout of the MFMA (lane i + 16k holds V[16P+i][k]),
+//   4. updates the window T_PQ -= V_P V_Q^T (10 MFMAs), the right-hand sides
+//      Y_P -= V_P Z with Z = C^-1 y_pivot (4 MFMAs) and the Schur term of the
+//      element matrix S += Z^T Z (1 MFMA),
+//   5. streams V and Z to HBM for the backward sweep and inserts the next 4
+//      rows of the band into the freed slots.
+// ---------------------------------------------------------------------------
+typedef double mf_d4 __attribute__((ext_vector_type(4)));
+
+struct MfmaSmem {
+  double piv[4 * 72];   // pivot rows, slot x at pad8(x)
+  double nrow[4 * 72];  // incoming rows, slot x at pad8(x)
+  double yp[64];        // pivot right-hand sides [4][16]
+};
+
+#define MF_TILE(P, Q) ((P) * ((P) + 1) / 2 + (Q))
+
+__device__ __forceinline__ mf_d4 mf_fma(double a, double b, mf_d4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// pivot rows of the window -> LDS
+template <int G>
+__device__ __forceinline__ void mf_extract(const mf_d4 (&Tl)[10], int T, double* piv) {
+  const int lane = threadIdx.x, lh = lane >> 4, ll = lane & 15;
+#pragma unroll
+  for (int P = 0; P < 4; ++P) {
+#pragma unroll
+    for (int Q = 0; Q <= P; ++Q) {
+      if (P == T) piv[lh * 72 + pad8(16 * Q + ll)] = Tl[MF_TILE(P, Q)][G];
+      if (Q == T && P > T && (ll >> 2) == G) {
+        const int t = ll - 4 * G;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) piv[t * 72 + pad8(16 * P + 4 * r + lh)] = Tl[MF_TILE(P, Q)][r];
+      }
+    }
+  }
+}
+
+// incoming rows (LDS) -> window; rows and columns of slots s0..s0+3
+template <int G>
+__device__ __forceinline__ void mf_place(mf_d4 (&Tl)[10], int T, const double* nrow) {
+  const int lane = threadIdx.x, lh = lane >> 4, ll = lane & 15;
+#pragma unroll
+  for (int P = 0; P < 4; ++P) {
+#pragma unroll
+    for (int Q = 0; Q <= P; ++Q) {
+      if (P == T) Tl[MF_TILE(P, Q)][G] = nrow[lh * 72 + pad8(16 * Q + ll)];
+      if (Q == T && (ll >> 2) == G) {
+        const int t = ll - 4 * G;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) Tl[MF_TILE(P, Q)][r] = nrow[t * 72 + pad8(16 * P + 4 * r + lh)];
+      }
+    }
+  }
+}
+
+// Build the 4 incoming rows jbase..jbase+3 (staged as rows rb..rb+3) in LDS
+// and place them, with their right-hand sides, into slots s0..s0+3.
+template <int DIM, int G>
+__device__ __forceinline__ void mf_insert(const BandArgs& a, int jbase, int rb, int T, int my_off,
+                                          mf_d4 (&Tl)[10], mf_d4 (&Yt)[4], BandSmem& sh,
+                                          MfmaSmem& ms) {
+  using D = Dirs<DIM>;
+  const int lane = threadIdx.x, lh = lane >> 4, ll = lane & 15;
+  const int s0 = 16 * T + 4 * G;
+#pragma unroll
+  for (int t = 0; t < 4; ++t) ms.nrow[t * 72 + pad8(lane)] = 0.0;
+  wave_sync();
+  {
+    const int t = lh, e = ll, jn = jbase + t;
+    if (jn < a.n) {
+      if (e < D::NLOW) {
+        const double v = sh.stg_low[rb + t][e];
+        if (v != 0.0) {
+          ms.nrow[t * 72 + pad8((s0 + t - my_off) & (kWin - 1))] = v;
+          // couplings between two incoming rows are needed in both rows
+          if (e > 0 && my_off <= t) ms.nrow[(t - my_off) * 72 + pad8(s0 + t)] = v;
+        }
+      }
+    } else if (e == 0) {
+      ms.nrow[t * 72 + pad8(s0 + t)] = 1.0;  // past the end: isolated unit diagonal
+    }
+  }
+  wave_sync();
+  mf_place<G>(Tl, T, ms.nrow);
+  const double yv = (ll < 8 && jbase + lh < a.n) ? sh.stg_rhs[rb + lh][ll & 7] : 0.0;
+#pragma unroll
+  for (int P = 0; P < 4; ++P)
+    if (P == T) Yt[P][G] = yv;
+  wave_sync();
+}
+
+template <int DIM, int G>
+__device__ __forceinline__ void mf_superstep(const BandArgs& a, int cell, int j0, int T,
+                                             mf_d4 (&Tl)[10], mf_d4 (&Yt)[4], mf_d4& Sacc,
+                                             BandSmem& sh, MfmaSmem& ms, int& bad) {
+  const int lane = threadIdx.x, lh = lane >> 4, ll = lane & 15;
+  const int s0 = 16 * T + 4 * G;
+  mf_extract<G>(Tl, T, ms.piv);
+  // pivot right-hand sides: register G of the right-hand-side tile T
+  {
+    double yv = 0.0;
+#pragma unroll
+    for (int P = 0; P < 4; ++P)
+      if (P == T) yv = Yt[P][G];
+    ms.yp[lh * 16 + ll] = yv;
+  }
+  wave_sync();
+  // 4x4 Cholesky of the pivot block and its inverse (all lanes)
+  double a00 = ms.piv[0 * 72 + pad8(s0 + 0)];
+  double a10 = ms.piv[1 * 72 + pad8(s0 + 0)], a11 = ms.piv[1 * 72 + pad8(s0 + 1)];
+  double a20 = ms.piv[2 * 72 + pad8(s0 + 0)], a21 = ms.piv[2 * 72 + pad8(s0 + 1)];
+  double a22 = ms.piv[2 * 72 + pad8(s0 + 2)];
+  double a30 = ms.piv[3 * 72 + pad8(s0 + 0)], a31 = ms.piv[3 * 72 + pad8(s0 + 1)];
+  double a32 = ms.piv[3 * 72 + pad8(s0 + 2)], a33 = ms.piv[3 * 72 + pad8(s0 + 3)];
+  if (!(a00 > 0.0) && bad == 0) bad = (a00 == a00) ? (int)MSFEC_CELL_NOT_SPD : (int)MSFEC_CELL_NONFINITE;
+  const double c00 = sqrt(a00), i00 = 1.0 / c00;
+  const double c10 = a10 * i00, c20 = a20 * i00, c30 = a30 * i00;
+  const double d1 = a11 - c10 * c10;
+  if (!(d1 > 0.0) && bad == 0) bad = (d1 == d1) ? (int)MSFEC_CELL_NOT_SPD : (int)MSFEC_CELL_NONFINITE;
+  const double c11 = sqrt(d1), i11 = 1.0 / c11;
+  const double c21 = (a21 - c20 * c10) * i11, c31 = (a31 - c30 * c10) * i11;
+  const double d2 = a22 - c20 * c20 - c21 * c21;
+  if (!(d2 > 0.0) && bad == 0) bad = (d2 == d2) ? (int)MSFEC_CELL_NOT_SPD : (int)MSFEC_CELL_NONFINITE;
+  const double c22 = sqrt(d2), i22 = 1.0 / c22;
+  const double c32 = (a32 - c30 * c20 - c31 * c21) * i22;
+  const double d3 = a33 - c30 * c30 - c31 * c31 - c32 * c32;
+  if (!(d3 > 0.0) && bad == 0) bad = (d3 == d3) ? (int)MSFEC_CELL_NOT_SPD : (int)MSFEC_CELL_NONFINITE;
+  const double c33 = sqrt(d3), i33 = 1.0 / c33;
+  // C^-1 (lower): row k of the inverse
+  const double i10 = -c10 * i00 * i11;
+  const double i21 = -c21 * i11 * i22;
+  const double i20 = -(c20 * i00 + c21 * i10) * i22;
+  const double i32 = -c32 * i22 * i33;
+  const double i31 = -(c31 * i11 + c32 * i21) * i33;
+  const double i30 = -(c30 * i00 + c31 * i10 + c32 * i20) * i33;
+  double ck0, ck1, ck2, ck3;  // row lh of C^-1
+  if (lh == 0) {
+    ck0 = i00; ck1 = 0.0; ck2 = 0.0; ck3 = 0.0;
+  } else if (lh == 1) {
+    ck0 = i10; ck1 = i11; ck2 = 0.0; ck3 = 0.0;
+  } else if (lh == 2) {
+    ck0 = i20; ck1 = i21; ck2 = i22; ck3 = 0.0;
+  } else {
+    ck0 = i30; ck1 = i31; ck2 = i32; ck3 = i33;
+  }
+  // panel V = A21 C^-T in operand layout, and Z = C^-1 y_pivot
+  double V[4], nV[4];
+#pragma unroll
+  for (int P = 0; P < 4; ++P) {
+    const int x = pad8(16 * P + ll);
+    V[P] = ms.piv[0 * 72 + x] * ck0 + ms.piv[1 * 72 + x] * ck1 + ms.piv[2 * 72 + x] * ck2 +
+           ms.piv[3 * 72 + x] * ck3;
+    nV[P] = -V[P];
+  }
+  const double Z = ms.yp[0 * 16 + ll] * ck0 + ms.yp[1 * 16 + ll] * ck1 + ms.yp[2 * 16 + ll] * ck2 +
+                   ms.yp[3 * 16 + ll] * ck3;
+  // window, right-hand sides and Schur term
+#pragma unroll
+  for (int P = 0; P < 4; ++P)
+#pragma unroll
+    for (int Q = 0; Q <= P; ++Q) Tl[MF_TILE(P, Q)] = mf_fma(nV[P], V[Q], Tl[MF_TILE(P, Q)]);
+#pragma unroll
+  for (int P = 0; P < 4; ++P) Yt[P] = mf_fma(nV[P], Z, Yt[P]);
+  Sacc = mf_fma(Z, Z, Sacc);
+  // stream the panel (rows relative to s0; rows 0..3 are C) and Z to HBM
+  const int qs = j0 >> 2;
+  double* lst = a.Lcol + ((size_t)cell * a.nq + qs) * 256;
+#pragma unroll
+  for (int P = 0; P < 4; ++P) lst[((16 * P + ll - s0) & (kWin - 1)) * 4 + lh] = V[P];
+  if (ll < 8) a.Wf[((size_t)cell * a.nq + qs) * 32 + lh * 8 + ll] = Z;
+  wave_sync();
+}
+
+// Backward block step: x_blk = C^-T (z - V_rest^T x_window) for the 4 pivots
+// of super-step qs; lane (k = lane>>4, r = (lane>>1)&7, h = lane&1) sums half
+// of the 60 terms of (V_rest^T x)[k][r].
+template <int DIM, bool FAST>
+__device__ __forceinline__ void mf_back(const BandArgs& a, int cell, int qs, const double* vb,
+                                        const double* zb, double* xr, double* tb, double& bint,
+                                        const double* ff) {
+  using D = Dirs<DIM>;
+  const int lane = threadIdx.x;
+  const int k = lane >> 4, r = (lane >> 1) & 7, h = lane & 1;
+  const int j0 = 4 * qs;
+  double part = 0.0;
+#pragma unroll 6
+  for (int m = 0; m < 30; ++m) {
+    const int o = 4 + h + 2 * m;
+    part = fma(vb[o * 5 + k], xr[9 * ((j0 + o) & (kWin - 1)) + r], part);
+  }
+  part += __shfl_xor(part, 1);
+  if (h == 0) tb[k * 8 + r] = part;
+  wave_sync();
+  if (lane < 8) {
+    const int rr = lane;
+    const double c00 = vb[0 * 5 + 0], c10 = vb[1 * 5 + 0], c11 = vb[1 * 5 + 1];
+    const double c20 = vb[2 * 5 + 0], c21 = vb[2 * 5 + 1], c22 = vb[2 * 5 + 2];
+    const double c30 = vb[3 * 5 + 0], c31 = vb[3 * 5 + 1], c32 = vb[3 * 5 + 2], c33 = vb[3 * 5 + 3];
+    const double r0 = zb[0 * 8 + rr] - tb[0 * 8 + rr], r1 = zb[1 * 8 + rr] - tb[1 * 8 + rr];
+    const double r2 = zb[2 * 8 + rr] - tb[2 * 8 + rr], r3 = zb[3 * 8 + rr] - tb[3 * 8 + rr];
+    const double x3 = r3 / c33;
+    const double x2 = (r2 - c32 * x3) / c22;
+    const double x1 = (r1 - c21 * x2 - c31 * x3) / c11;
+    const double x0 = (r0 - c10 * x1 - c20 * x2 - c30 * x3) / c00;
+    const double xs[4] = {x0, x1, x2, x3};
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int j = j0 + t;
+      if (j >= a.n) continue;
+      xr[9 * (j & (kWin - 1)) + rr] = xs[t];
+      if (FAST) {
+        int c[3] = {0, 0, 0}, tt = j;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+          c[d] = tt % a.ni + 1;
+          tt /= a.ni;
+        }
+        const int node = c[0] + a.S * (c[1] + a.S * c[2]);
+        if (rr < D::NV) a.Psi[((size_t)cell * D::NV + rr) * a.nn + node] = xs[t];
+        double fj;
+        if (a.rhs_mode == MSFEC_RHS_CELL) {
+          double s = 0.0;
+#pragma unroll
+          for (int al = 0; al < D::NV; ++al) {
+            const int zx = c[0] - (al & 1), zy = c[1] - ((al >> 1) & 1);
+            const int zz = (DIM == 3) ? c[2] - ((al >> 2) & 1) : 0;
+            s += ff[zx + a.N * (zy + a.N * zz)];
+          }
+          fj = s;
+        } else {
+          fj = a.rhs_const * D::NV;
+        }
+        bint = fma(xs[t], fj, bint);
+      } else {
+        a.X[((size_t)cell * a.n + j) * kMaxNb + rr] = xs[t];
+      }
+    }
+  }
+  wave_sync();
+}
+
+template <int DIM, bool FAST>
+__global__ __launch_bounds__(64, 2) void q_band_mfma_kernel(BandArgs a) {
+  using D = Dirs<DIM>;
+  const int cell = blockIdx.x;
+  const int lane = threadIdx.x;
+  __shared__ BandSmem sh;
+  __shared__ MfmaSmem ms;
+  extern __shared__ double dyn[];  // fast path: fine-cell coefficient | reaction | load
+  const double* cf = dyn;
+  const double* rf = dyn + a.nfc;
+  const double* ff = dyn + a.nfc * ((a.reaction_mode == MSFEC_REACTION_CELL) ? 2 : 1);
+  double vol = 1.0;
+  if (FAST) {
+    const double* cg = a.coef + (size_t)cell * a.nfc;
+    for (int z = lane; z < a.nfc; z += 64) dyn[z] = cg[z];
+    if (a.reaction_mode == MSFEC_REACTION_CELL) {
+      const double* rg = a.reac + (size_t)cell * a.nfc;
+      for (int z = lane; z < a.nfc; z += 64) dyn[a.nfc + z] = rg[z];
+    }
+    if (a.rhs_mode == MSFEC_RHS_CELL) {
+      const double* fg = a.fcell + (size_t)cell * a.nfc;
+      double* fd = dyn + a.nfc * ((a.reaction_mode == MSFEC_REACTION_CELL) ? 2 : 1);
+      for (int z = lane; z < a.nfc; z += 64) fd[z] = fg[z];
+    }
+    const double* v = a.verts + (size_t)cell * D::NV * DIM;
+    double hh[3] = {1.0, 1.0, 1.0};
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+      hh[d] = (v[(D::NV - 1) * DIM + d] - v[d]) / a.N;
+      vol *= hh[d];
+    }
+    if (lane < 8) {
+      const int x = lane;
+      double ks = 0.0, msv = 1.0;
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) {
+        double t = (((x >> d) & 1) ? -1.0 : 1.0) / hh[d];
+#pragma unroll
+        for (int e = 0; e < DIM; ++e)
+          if (e != d) t *= (((x >> e) & 1) ? (1.0 / 6.0) : (1.0 / 3.0)) * hh[e];
+        ks += t;
+        msv *= (((x >> d) & 1) ? (1.0 / 6.0) : (1.0 / 3.0)) * hh[d];
+      }
+      sh.kv[x] = ks;
+      sh.mv[x] = msv;
+    }
+    wave_sync();
+  }
+  // interior-grid offset of the stencil entry scattered by this lane (lane & 15)
+  int my_off = 0;
+#pragma unroll
+  for (int t = 0; t < kMaxLow; ++t)
+    if ((lane & 15) == t) my_off = a.off_int[t];
+
+  mf_d4 Tl[10];
+  mf_d4 Yt[4];
+  mf_d4 Sacc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int t = 0; t < 10; ++t) Tl[t] = (mf_d4){0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int P = 0; P < 4; ++P) Yt[P] = (mf_d4){0.0, 0.0, 0.0, 0.0};
+
+  // initial window: indices 0..63 in slots 0..63
+  for (int T = 0; T < 4; ++T) {
+    stage_rows<DIM, FAST>(a, cell, 16 * T, sh, cf, rf);
+    mf_insert<DIM, 0>(a, 16 * T + 0, 0, T, my_off, Tl, Yt, sh, ms);
+    mf_insert<DIM, 1>(a, 16 * T + 4, 4, T, my_off, Tl, Yt, sh, ms);
+    stage_rows<DIM, FAST>(a, cell, 16 * T + 8, sh, cf, rf);
+    mf_insert<DIM, 2>(a, 16 * T + 8, 0, T, my_off, Tl, Yt, sh, ms);
+    mf_insert<DIM, 3>(a, 16 * T + 12, 4, T, my_off, Tl, Yt, sh, ms);
+  }
+
+  int bad = 0;
+  const int n = a.n;
+  for (int j0 = 0; j0 < n; j0 += 16) {
+    const int T = (j0 >> 4) & 3;
+    stage_rows<DIM, FAST>(a, cell, j0 + kWin, sh, cf, rf);
+    if (j0 < n) {
+      mf_superstep<DIM, 0>(a, cell, j0, T, Tl, Yt, Sacc, sh, ms, bad);
+      mf_insert<DIM, 0>(a, j0 + kWin, 0, T, my_off, Tl, Yt, sh, ms);
+    }
+    if (j0 + 4 < n) {
+      mf_superstep<DIM, 1>(a, cell, j0 + 4, T, Tl, Yt, Sacc, sh, ms, bad);
+      mf_insert<DIM, 1>(a, j0 + 4 + kWin, 4, T, my_off, Tl, Yt, sh, ms);
+    }
+    stage_rows<DIM, FAST>(a, cell, j0 + 8 + kWin, sh, cf, rf);
+    if (j0 + 8 < n) {
+      mf_superstep<DIM, 2>(a, cell, j0 + 8, T, Tl, Yt, Sacc, sh, ms, bad);
+      mf_insert<DIM, 2>(a, j0 + 8 + kWin, 0, T, my_off, Tl, Yt, sh, ms);
+    }
+    if (j0 + 12 < n) {
+      mf_superstep<DIM, 3>(a, cell, j0 + 12, T, Tl, Yt, Sacc, sh, ms, bad);
+      mf_insert<DIM, 3>(a, j0 + 12 + kWin, 4, T, my_off, Tl, Yt, sh, ms);
+    }
+  }
+
+  // Schur term of the element matrix: entry (i,j) of Sacc sits in lane
+  // j + 16 (i mod 4), register i / 4; lane (i,k) of the epilogue needs (i,k)
+  double sacc = 0.0;
+  {
+    double* sb = SH_PART(sh);
+#pragma unroll
+    for (int rr = 0; rr < 4; ++rr) {
+      const int i = 4 * rr + (lane >> 4), jj = lane & 15;
+      if (i < 8 && jj < 8) sb[i * 8 + jj] = Sacc[rr];
+    }
+    wave_sync();
+    sacc = sb[lane];
+    wave_sync();
+  }
+
+  // backward sweep over the super-steps, panels prefetched one step ahead
+  __threadfence_block();
+  __syncthreads();
+  double* xr = SH_XRING(sh);
+  double* vb = SH_LBUF(sh);        // [64][5]
+  double* zb = SH_LBUF(sh) + 320;  // [4][8]
+  double* tb = SH_LBUF(sh) + 352;  // [4][8]
+  for (int t = lane; t < 576; t += 64) xr[t] = 0.0;
+  const int nq = (n + 3) >> 2;
+  const double* Lc = a.Lcol + (size_t)cell * a.nq * 256;
+  const double* Zc = a.Wf + (size_t)cell * a.nq * 32;
+  double vp[4], zp = 0.0;
+#pragma unroll
+  for (int t = 0; t < 4; ++t) vp[t] = Lc[(size_t)(nq - 1) * 256 + lane * 4 + t];
+  if (lane < 32) zp = Zc[(size_t)(nq - 1) * 32 + lane];
+  double bint = 0.0;
+  for (int qs = nq - 1; qs >= 0; --qs) {
+    wave_sync();
+#pragma unroll
+    for (int t = 0; t < 4; ++t) vb[lane * 5 + t] = vp[t];
+    if (lane < 32) zb[lane] = zp;
+    wave_sync();
+    if (qs > 0) {
+#pragma unroll
+      for (int t = 0; t < 4; ++t) vp[t] = Lc[(size_t)(qs - 1) * 256 + lane * 4 + t];
+      if (lane < 32) zp = Zc[(size_t)(qs - 1) * 32 + lane];
+    }
+    mf_back<DIM, FAST>(a, cell, qs, vb, zb, xr, tb, bint, ff);
+  }
+  int worst = bad;
+  for (int o = 32; o > 0; o >>= 1) worst = max(worst, __shfl_xor(worst, o));
+  if (lane == 0) a.status[cell] = worst;
+  if (FAST) {
+    wave_sync();
+    // bint lives in lanes 0..7 (RHS r = lane); the epilogue reads it from lane k
     fast_epilogue<DIM>(a, cell, sh, cf, rf, ff, vol, sacc, bint);
   }
 }
@@ -1305,7 +1713,8 @@ msfec_status msfec_q_create(const msfec_q_params* params, int n_cells, const dou
     return fail(MSFEC_ERR_INVALID, "unknown rhs_mode");
   if (p.reaction_mode < MSFEC_REACTION_NONE || p.reaction_mode > MSFEC_REACTION_CELL)
     return fail(MSFEC_ERR_INVALID, "unknown reaction_mode");
-  if (p.solver != MSFEC_SOLVER_BAND_FP64) return fail(MSFEC_ERR_INVALID, "unknown solver");
+  if (p.solver != MSFEC_SOLVER_BAND_FP64 && p.solver != MSFEC_SOLVER_BAND_FP64_MFMA)
+    return fail(MSFEC_ERR_INVALID, "unknown solver");
   const int N = 1 << p.n_refine_local;
   const int ni = N - 1;
   const int bw = (p.dim == 2) ? ni + 1 : ni * ni + ni + 1;
@@ -1406,8 +1815,13 @@ msfec_status msfec_q_create(const msfec_q_params* params, int n_cells, const dou
     TRY(dalloc(h, &h->d_F, (size_t)n_cells * c.nn));
     TRY(dalloc(h, &h->d_rhs, (size_t)n_cells * c.n * kMaxNb));
   }
-  TRY(dalloc(h, &h->d_L, (size_t)n_cells * c.n * kWin));
-  TRY(dalloc(h, &h->d_W, (size_t)n_cells * c.n * kMaxNb));
+  {
+    // VALU variant: one column of 64 per pivot; MFMA variant: a 64 x 4 panel
+    // per super-step of 4 pivots
+    const size_t nq = (size_t)(c.n + 3) / 4;
+    TRY(dalloc(h, &h->d_L, (size_t)n_cells * std::max((size_t)c.n * kWin, nq * 256)));
+    TRY(dalloc(h, &h->d_W, (size_t)n_cells * std::max((size_t)c.n * kMaxNb, nq * 32)));
+  }
   TRY(dalloc(h, &h->d_X, (size_t)n_cells * c.n * kMaxNb));
   TRY(dalloc(h, &h->d_psi, (size_t)n_cells * nv * c.nn));
   TRY(dalloc(h, &h->d_M, (size_t)n_cells * nv * nv));
@@ -1560,21 +1974,32 @@ msfec_status msfec_q_compute(msfec_q_handle h, void* stream) {
   a.M = h->d_M;
   a.bvec = h->d_b;
   a.status = h->d_status;
-  if (fast) {
-    const size_t dyn = (size_t)c.nfc *
-                       (1 + (c.reaction_mode == MSFEC_REACTION_CELL ? 1 : 0) +
-                        (c.rhs_mode == MSFEC_RHS_CELL ? 1 : 0)) *
-                       sizeof(double);
-    if (c.dim == 3)
-      hipLaunchKernelGGL((q_band_kernel<3, true>), dim3(h->n_cells), dim3(64), dyn, s, a);
-    else
-      hipLaunchKernelGGL((q_band_kernel<2, true>), dim3(h->n_cells), dim3(64), dyn, s, a);
+  a.nq = (c.n + 3) / 4;
+  const bool mfma = (h->p.solver == MSFEC_SOLVER_BAND_FP64_MFMA);
+  const size_t dyn = fast ? (size_t)c.nfc *
+                                (1 + (c.reaction_mode == MSFEC_REACTION_CELL ? 1 : 0) +
+                                 (c.rhs_mode == MSFEC_RHS_CELL ? 1 : 0)) *
+                                sizeof(double)
+                          : 0;
+#define LAUNCH_BAND(K, D, F) hipLaunchKernelGGL((K<D, F>), dim3(h->n_cells), dim3(64), dyn, s, a)
+  if (mfma) {
+    if (c.dim == 3) {
+      if (fast) LAUNCH_BAND(q_band_mfma_kernel, 3, true);
+      else LAUNCH_BAND(q_band_mfma_kernel, 3, false);
+    } else {
+      if (fast) LAUNCH_BAND(q_band_mfma_kernel, 2, true);
+      else LAUNCH_BAND(q_band_mfma_kernel, 2, false);
+    }
   } else {
-    if (c.dim == 3)
-      hipLaunchKernelGGL((q_band_kernel<3, false>), dim3(h->n_cells), dim3(64), 0, s, a);
-    else
-      hipLaunchKernelGGL((q_band_kernel<2, false>), dim3(h->n_cells), dim3(64), 0, s, a);
+    if (c.dim == 3) {
+      if (fast) LAUNCH_BAND(q_band_kernel, 3, true);
+      else LAUNCH_BAND(q_band_kernel, 3, false);
+    } else {
+      if (fast) LAUNCH_BAND(q_band_kernel, 2, true);
+      else LAUNCH_BAND(q_band_kernel, 2, false);
+    }
   }
+#undef LAUNCH_BAND
   MSFEC_HIP(hipGetLastError());
   if (tm) MSFEC_HIP(hipEventRecord(ev[2], s));
   if (!fast) {

@@ -19,7 +19,7 @@ HOST, DEVICE = 0, 1
 COEF_SCALAR_CELL, COEF_TENSOR_QP = 0, 1
 RHS_CONST, RHS_CELL, RHS_QP = 0, 1, 2
 REACTION_NONE, REACTION_CONST, REACTION_CELL = 0, 1, 2
-SOLVER_BAND_FP64 = 0
+SOLVER_BAND_FP64, SOLVER_BAND_FP64_MFMA = 0, 1
 CELL_OK, CELL_NOT_SPD, CELL_NONFINITE = 0, 2, 3
 FLAG_TIMING = 1
 

@@ -29,12 +29,15 @@ def rel(a, b):
     return float(np.max(num / np.maximum(den, 1e-300)))
 
 
+SOLVERS = (0, 1)  # MSFEC_SOLVER_BAND_FP64 (vector ALU), MSFEC_SOLVER_BAND_FP64_MFMA
+
+
 def run_gpu(verts, dim, l, coef, coef_mode=0, f=None, rhs_mode=0, rhs_const=1.0,
-            reac=None, reaction_mode=0, reaction_const=0.0):
+            reac=None, reaction_mode=0, reaction_const=0.0, solver=1):
     from msfec_amd import QBasisBatch
     with QBasisBatch(verts, dim=dim, n_refine_local=l, coef_mode=coef_mode, rhs_mode=rhs_mode,
                      rhs_const=rhs_const, reaction_mode=reaction_mode,
-                     reaction_const=reaction_const) as qb:
+                     reaction_const=reaction_const, solver=solver) as qb:
         qb.set_coefficient(coef)
         if f is not None:
             qb.set_rhs(f)
@@ -61,15 +64,16 @@ def random_boxes(rng, dim, n_cells):
     return np.array(out)
 
 
+@pytest.mark.parametrize("solver", SOLVERS)
 @pytest.mark.parametrize("dim,l", [(2, 1), (2, 2), (2, 3), (2, 4), (2, 5), (3, 1), (3, 2), (3, 3)])
-def test_scalar_lognormal(oracle_mod, dim, l):
-    print(f"case scalar_lognormal dim={dim} l={l}", flush=True)
+def test_scalar_lognormal(oracle_mod, dim, l, solver):
+    print(f"case scalar_lognormal dim={dim} l={l} solver={solver}", flush=True)
     rng = np.random.default_rng(100 + 10 * dim + l)
     n_cells = 7
     verts = random_boxes(rng, dim, n_cells)
     N = 1 << l
     coef = np.exp(1.5 * rng.standard_normal((n_cells, N ** dim)))
-    g = run_gpu(verts, dim, l, coef)
+    g = run_gpu(verts, dim, l, coef, solver=solver)
     o = run_oracle(oracle_mod, verts, dim, l, coef)
     assert (g["status"] == 0).all() and (o["status"] == 0).all()
     assert rel(g["M"], o["M"]) < TOL
@@ -77,9 +81,10 @@ def test_scalar_lognormal(oracle_mod, dim, l):
     assert rel(g["Psi"], o["Psi"]) < TOL
 
 
+@pytest.mark.parametrize("solver", SOLVERS)
 @pytest.mark.parametrize("dim,l", [(2, 3), (3, 2)])
-def test_tensor_qp_and_rhs_modes(oracle_mod, dim, l):
-    print(f"case tensor_qp dim={dim} l={l}", flush=True)
+def test_tensor_qp_and_rhs_modes(oracle_mod, dim, l, solver):
+    print(f"case tensor_qp dim={dim} l={l} solver={solver}", flush=True)
     rng = np.random.default_rng(7 + dim)
     n_cells, N, nq = 5, 1 << l, 1 << dim
     verts = random_boxes(rng, dim, n_cells)
@@ -95,7 +100,7 @@ def test_tensor_qp_and_rhs_modes(oracle_mod, dim, l):
     assert T.shape[-1] == ncomp
     for rhs_mode in (1, 2):
         f = rng.standard_normal((n_cells, N ** dim) if rhs_mode == 1 else (n_cells, N ** dim, nq))
-        g = run_gpu(verts, dim, l, T, coef_mode=1, f=f, rhs_mode=rhs_mode)
+        g = run_gpu(verts, dim, l, T, coef_mode=1, f=f, rhs_mode=rhs_mode, solver=solver)
         o = run_oracle(oracle_mod, verts, dim, l, T, coef_mode=1, f=f, rhs_mode=rhs_mode)
         assert rel(g["M"], o["M"]) < TOL
         assert rel(g["b"], o["b"]) < TOL
@@ -118,25 +123,28 @@ def test_analytic_diffusion_A(oracle_mod):
     assert rel(g["Psi"], o["Psi"]) < 1e-10
 
 
+@pytest.mark.parametrize("solver", SOLVERS)
 @pytest.mark.parametrize("mode", [1, 2])
-def test_reaction(oracle_mod, mode):
-    print(f"case reaction mode={mode}", flush=True)
+def test_reaction(oracle_mod, mode, solver):
+    print(f"case reaction mode={mode} solver={solver}", flush=True)
     rng = np.random.default_rng(31 + mode)
     dim, l, n_cells = 3, 2, 4
     N = 1 << l
     verts = random_boxes(rng, dim, n_cells)
     coef = np.exp(rng.standard_normal((n_cells, N ** dim)))
     reac = rng.random((n_cells, N ** dim)) if mode == 2 else None
-    g = run_gpu(verts, dim, l, coef, reac=reac, reaction_mode=mode, reaction_const=3.0)
+    g = run_gpu(verts, dim, l, coef, reac=reac, reaction_mode=mode, reaction_const=3.0,
+                solver=solver)
     o = run_oracle(oracle_mod, verts, dim, l, coef, reac=reac, reaction_mode=mode,
                    reaction_const=3.0)
     assert rel(g["M"], o["M"]) < TOL
     assert rel(g["Psi"], o["Psi"]) < TOL
 
 
-def test_bad_cells_reported_not_fatal():
+@pytest.mark.parametrize("solver", SOLVERS)
+def test_bad_cells_reported_not_fatal(solver):
     """A cell with a non-positive coefficient is flagged; the others stay valid."""
-    print("case bad_cells", flush=True)
+    print(f"case bad_cells solver={solver}", flush=True)
     rng = np.random.default_rng(5)
     dim, l, n_cells = 3, 2, 3
     N = 1 << l
@@ -145,23 +153,24 @@ def test_bad_cells_reported_not_fatal():
     coef[1, :] = -1.0
     coef[2, 3] = np.nan
     L = _lib()
-    g = run_gpu(verts, dim, l, coef)
+    g = run_gpu(verts, dim, l, coef, solver=solver)
     assert g["status"][0] == L.CELL_OK
     assert g["status"][1] == L.CELL_NOT_SPD
     assert g["status"][2] in (L.CELL_NOT_SPD, L.CELL_NONFINITE)
     assert np.isfinite(g["M"][0]).all()
 
 
-def test_headline_batch_invariants(oracle_mod):
+@pytest.mark.parametrize("solver", SOLVERS)
+def test_headline_batch_invariants(oracle_mod, solver):
     """4096 cells, 3-D, 8^3 fine cells each, lognormal sigma=1.5 field with
     correlation length 1/64: structural invariants on every cell and oracle
     parity on a seeded sample."""
-    print("case headline_batch", flush=True)
+    print(f"case headline_batch solver={solver}", flush=True)
     from msfec_amd import fields as F
     dim, l, nc = 3, 3, 16
     verts = F.coarse_vertices(dim, nc)
     coef = F.lognormal_coefficient(dim, nc, l, 1.5, 1.0 / 64, seed=20261202)
-    g = run_gpu(verts, dim, l, coef)
+    g = run_gpu(verts, dim, l, coef, solver=solver)
     assert (g["status"] == 0).all()
     M, b, Psi = g["M"], g["b"], g["Psi"]
     scale = np.abs(M).max(axis=(1, 2))
@@ -200,8 +209,9 @@ def test_coarse_solve_and_reconstruct(oracle_mod, dim, nc, l):
     assert np.abs(uf - ufo).max() <= 1e-10 * np.abs(ufo).max()
 
 
+@pytest.mark.parametrize("solver", SOLVERS)
 @pytest.mark.parametrize("dim,l", [(2, 4), (3, 3)])
-def test_fused_path_rhs_cell_and_reaction(oracle_mod, dim, l):
+def test_fused_path_rhs_cell_and_reaction(oracle_mod, dim, l, solver):
     """Scalar coefficient with a per-cell load and a per-cell reaction runs
     through the fused kernel; checked against the oracle."""
     print(f"case fused rhs_cell dim={dim} l={l}", flush=True)
@@ -211,7 +221,7 @@ def test_fused_path_rhs_cell_and_reaction(oracle_mod, dim, l):
     coef = np.exp(rng.standard_normal((n_cells, N ** dim)))
     f = rng.standard_normal((n_cells, N ** dim))
     reac = rng.random((n_cells, N ** dim))
-    g = run_gpu(verts, dim, l, coef, f=f, rhs_mode=1, reac=reac, reaction_mode=2)
+    g = run_gpu(verts, dim, l, coef, f=f, rhs_mode=1, reac=reac, reaction_mode=2, solver=solver)
     o = run_oracle(oracle_mod, verts, dim, l, coef, f=f, rhs_mode=1, reac=reac, reaction_mode=2)
     assert rel(g["M"], o["M"]) < TOL
     assert rel(g["b"], o["b"]) < TOL
@@ -235,3 +245,18 @@ def test_fused_equals_general_path(dim, l):
     assert rel(g1["M"], g2["M"]) < TOL
     assert rel(g1["b"], g2["b"]) < TOL
     assert rel(g1["Psi"], g2["Psi"]) < TOL
+
+
+@pytest.mark.parametrize("dim,l", [(2, 5), (3, 3)])
+def test_solvers_agree(dim, l):
+    """Vector-ALU and matrix-core variants of the band factorisation agree to
+    rounding (different summation orders, same algorithm)."""
+    print(f"case solvers_agree dim={dim} l={l}", flush=True)
+    rng = np.random.default_rng(60 + dim)
+    n_cells, N = 9, 1 << l
+    verts = random_boxes(rng, dim, n_cells)
+    coef = np.exp(2.0 * rng.standard_normal((n_cells, N ** dim)))
+    g0 = run_gpu(verts, dim, l, coef, solver=0)
+    g1 = run_gpu(verts, dim, l, coef, solver=1)
+    assert rel(g0["M"], g1["M"]) < TOL
+    assert rel(g0["Psi"], g1["Psi"]) < TOL
