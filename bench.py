@@ -29,6 +29,12 @@ import numpy as np  # noqa: E402
 DIM, NREF, NC = 3, 3, 16
 SIGMA, CORR, SEED = 1.5, 1.0 / 64, 20261202
 FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 (vector and matrix)
+# local solvers: name -> (solver id, band kernel name); HEADLINE is the value line
+SOLVERS = {"band-fp64-valu": (0, "q_band_kernel"), "band-fp64-mfma": (1, "q_band_mfma_kernel")}
+SOLVER_TEXT = {"band-fp64-valu": "banded Cholesky FP64 on the vector ALUs",
+               "band-fp64-mfma": "banded Cholesky FP64 on the matrix cores"}
+HEADLINE = "band-fp64-valu"
+TRAFFIC_FILE = os.path.join(REPO, "profiles", "hbm_traffic.json")
 HBM_PEAK_GBS = 8000.0     # MI355X HBM3E
 
 
@@ -54,6 +60,22 @@ def band_bytes_per_cell(n: int, p: int, nlow: int, nrhs: int = 8) -> float:
     lifting right-hand sides in, the band factor L written once and read once
     by the backward solve, forward results written and read, solution out."""
     return 8.0 * (n * nlow + n * nrhs + 2 * n * (p + 1) + 2 * n * nrhs + n * nrhs)
+
+
+def measured_traffic(kernel: str, n_cells: int):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary (FETCH_SIZE and WRITE_SIZE runs, each scaled by its reading for
+    a copy of known size), or None when no summary matches this kernel and
+    batch size."""
+    try:
+        with open(TRAFFIC_FILE) as fh:
+            t = json.load(fh)
+        k = t["kernels"][kernel]
+        if t["n_cells"] != n_cells:
+            return None
+        return round((k["fetch_kib"] * t["fetch_scale"] + k["write_kib"] * t["write_scale"]) * 1024)
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 def rel_err(a, b):
@@ -98,6 +120,31 @@ def cpu_baseline(verts, coef, budget_s: float = 20.0):
     }
 
 
+def copy_probe(mib: int, reps: int = 3):
+    """Device-to-device copies of a known size on the library's HIP runtime.
+    Profiled with the FETCH_SIZE / WRITE_SIZE counters, the copy kernel's
+    reading against its known bytes gives the scale factor applied to the
+    same counters of the basis kernels."""
+    import ctypes
+    from msfec_amd import _lib as L
+    L.load()  # the runtime the library links (libamdhip64.so.7)
+    hip = ctypes.CDLL("libamdhip64.so.7")
+    nbytes = mib << 20
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    for p in (src, dst):
+        if hip.hipMalloc(ctypes.byref(p), ctypes.c_size_t(nbytes)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    if hip.hipMemset(src, 1, ctypes.c_size_t(nbytes)) != 0:
+        raise RuntimeError("hipMemset failed")
+    for _ in range(reps):
+        if hip.hipMemcpy(dst, src, ctypes.c_size_t(nbytes), 3) != 0:  # hipMemcpyDeviceToDevice
+            raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    print(json.dumps({"copy_probe_bytes": nbytes, "copy_probe_kib": nbytes // 1024, "reps": reps}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -105,11 +152,16 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run MIB-sized device copies (counter calibration)")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; the per-cell basis "
-                                   "construction shards over GPUs by cells (see DESIGN.md), "
-                                   "but no multi-GPU path is built"}))
+        print(json.dumps({"error": "this build runs on one GPU only: the per-cell basis "
+                                   "construction would shard over GPUs by coarse cells, but "
+                                   "no multi-GPU path is built"}))
         sys.exit(2)
 
     # The library owns its device buffers and stream work; torch is not
@@ -137,13 +189,18 @@ def main():
         kt = qb.kernel_times()  # HIP events on the launch stream, averaged over the timed steps
         return qb, wall, kt
 
-    # the vector-ALU variant of the same FP64 band algorithm, timed beside
-    qb0, wall0, kt0 = measure(L.SOLVER_BAND_FP64, max(2, args.steps // 4), min(args.warmup, 1))
-    valu = {"cells_per_s": round(qb0.info.n_cells / (wall0 / max(2, args.steps // 4)), 1),
-            "band_kernel_ms": round(kt0[1], 4)}
-    qb0.close()
-    # headline: the matrix-core variant
-    qb, wall, kt = measure(L.SOLVER_BAND_FP64_MFMA, args.steps, args.warmup)
+    # the other local solvers, timed beside the headline on fewer steps
+    others = {}
+    for name, (solver, kname) in SOLVERS.items():
+        if name == HEADLINE:
+            continue
+        k = max(2, args.steps // 4)
+        q, w, t = measure(solver, k, min(args.warmup, 1))
+        others[name] = {"cells_per_s": round(q.info.n_cells / (w / k), 1),
+                        "kernel": kname, "band_kernel_ms": round(t[1], 4)}
+        q.close()
+    solver, kname = SOLVERS[HEADLINE]
+    qb, wall, kt = measure(solver, args.steps, args.warmup)
     info = qb.info
     ms_step = wall * 1e3 / args.steps
     status = qb.get_cell_status()
@@ -157,15 +214,18 @@ def main():
     tflops = flops_cell * info.n_cells / (band_ms * 1e-3) / 1e12
     gbs = bytes_cell * info.n_cells / (band_ms * 1e-3) / 1e9
     f_frac, b_frac = tflops / FP64_PEAK_TFLOPS, gbs / HBM_PEAK_GBS
+    traffic = measured_traffic(kname, info.n_cells)
     if b_frac >= f_frac:
         roof = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(b_frac, 4), "traffic": None}
+                "unit": "GB/s", "frac": round(b_frac, 4), "traffic": traffic}
     else:
         roof = {"bound": "mfma", "achieved": round(tflops, 3), "peak": FP64_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "frac": round(f_frac, 4), "traffic": None}
-    roof.update({"kernel": "q_band_mfma_kernel", "kernel_ms": round(band_ms, 4),
+                "unit": "TFLOP/s", "frac": round(f_frac, 4), "traffic": traffic}
+    roof.update({"kernel": kname, "kernel_ms": round(band_ms, 4),
                  "flops_per_cell": flops_cell, "bytes_per_cell": bytes_cell,
                  "fp64_tflops": round(tflops, 3), "hbm_gbs": round(gbs, 1),
+                 "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
+                                 "copy-calibrated, profiles/hbm_traffic.json)",
                  "dense_equiv_tflops": round(dense_flops_per_cell(n, 8) * info.n_cells /
                                              (band_ms * 1e-3) / 1e12, 3)})
 
@@ -184,11 +244,10 @@ def main():
         "data": "synthetic: seeded lognormal A=exp(g), g Gaussian sigma 1.5, corr. length 1/64 "
                 "on the 128^3 fine grid (seed 20261202), f=1",
         "config": {"workload": "3D Q1 basis construction, 16^3=4096 coarse cells x 8^3 fine "
-                               "cells (343 interior DoFs, 8 RHS), banded Cholesky FP64 on the "
-                               "matrix cores",
+                               "cells (343 interior DoFs, 8 RHS), " + SOLVER_TEXT[HEADLINE],
                    "n_cells": info.n_cells, "n_refine_local": NREF,
-                   "solver": "band-fp64-mfma", "parallelism": "single GPU"},
-        "fp64_vector_alu_path": valu,
+                   "solver": HEADLINE, "parallelism": "single GPU"},
+        "other_solvers": others,
         "kernel_ms": {"assemble": round(kt[0], 4), "band_factor_solve": round(kt[1], 4),
                       "element": round(kt[2], 4)},
         "wall_s_timed": round(wall, 3),

@@ -55,9 +55,9 @@ enum { MSFEC_RHS_CONST = 0, MSFEC_RHS_CELL = 1, MSFEC_RHS_QP = 2 };
 /* optional reaction R u (the reference's QBasis has none: 0 reproduces it) */
 enum { MSFEC_REACTION_NONE = 0, MSFEC_REACTION_CONST = 1, MSFEC_REACTION_CELL = 2 };
 /* local solver: banded Cholesky of the interior system in FP64, eliminating
- * one pivot per step on the vector ALUs (BAND_FP64) or four pivots per step
- * with FP64 matrix-core updates (BAND_FP64_MFMA, the default of the Python
- * and C++ front ends).  Both are direct solvers, the analogue of
+ * one pivot per step on the vector ALUs (BAND_FP64, the default of the
+ * Python and C++ front ends) or four pivots per step with FP64 matrix-core
+ * updates (BAND_FP64_MFMA).  Both are direct solvers, the analogue of
  * use_direct_solver_basis = true (source/Q/q_basis.cc:304-330). */
 enum { MSFEC_SOLVER_BAND_FP64 = 0, MSFEC_SOLVER_BAND_FP64_MFMA = 1 };
 /* per-cell status codes */

@@ -34,7 +34,7 @@ struct ParametersBasis {
   double rhs_const = 1.0;
   int reaction_mode = MSFEC_REACTION_NONE;
   double reaction_const = 0.0;
-  int solver = MSFEC_SOLVER_BAND_FP64_MFMA;
+  int solver = MSFEC_SOLVER_BAND_FP64;
 };
 
 class QBatch {

@@ -32,7 +32,7 @@ def _ptr(a):
 class QBasisBatch:
     def __init__(self, vertices, dim=3, n_refine_local=3, coef_mode=L.COEF_SCALAR_CELL,
                  rhs_mode=L.RHS_CONST, rhs_const=1.0, reaction_mode=L.REACTION_NONE,
-                 reaction_const=0.0, solver=L.SOLVER_BAND_FP64_MFMA, timing=False):
+                 reaction_const=0.0, solver=L.SOLVER_BAND_FP64, timing=False):
         self._lib = L.load()
         self._h = ctypes.c_void_p()
         p = L.Params()

@@ -24,7 +24,7 @@ static int ipow(int b, int e) {
  * BasisQ1 -- include/functions/basis_q1.tpp:21-67 (constructor inverts the
  * vertex-monomial matrix with FullMatrix::invert) and :94-152 (value_list).
  * Pinned by: delta property at vertices, partition of unity, closed-form
- * tensor-product Lagrange values on boxes (tests/test_oracle_basis.py).
+ * tensor-product Lagrange values on boxes (tests/test_oracle.py).
  * ------------------------------------------------------------------------- */
 static void monomials(int dim, const double *x, double *m) {
   if (dim == 2) {
@@ -99,7 +99,7 @@ double oq_basis_q1_value(int dim, const double *C, int index, const double *x) {
  * Diffusion_A -- source/equation_data/eqn_coeff_A.cc:7-57 (rotation from the
  * fixed Euler angles alpha=pi/3, beta=pi/6, gamma=pi/4) and :163-191.
  * Pinned by: orthogonality of the rotation, eigenvalues equal to the diagonal
- * entries, rotate=false giving the diagonal (tests/test_oracle_coeff.py).
+ * entries, rotate=false giving the diagonal (tests/test_oracle.py).
  * ------------------------------------------------------------------------- */
 static void rotation(int rotate, double R[3][3]) {
   const double a = OQ_PI / 3, b = OQ_PI / 6, g = OQ_PI / 4;
@@ -150,7 +150,7 @@ void oq_diffusion_A(const oq_problem *p, const double *x, double *A) {
  * FEValues on an axis-aligned box: shape values are tensor products of 1-D
  * linear Lagrange functions; JxW = w_q * prod(h).  Pinned by the closed-form
  * Q1 stiffness (a h/12) S in 3-D and (a/6) S in 2-D, the Q1 mass matrix, and a
- * numpy brute-force 3-point Gauss assembly (tests/test_oracle_assembly.py).
+ * numpy brute-force 3-point Gauss assembly (tests/test_oracle.py and tests/golden/make_golden.py).
  * ------------------------------------------------------------------------- */
 static const double GQ[2] = {0.5 - 0.28867513459481288225, 0.5 + 0.28867513459481288225};
 
@@ -696,7 +696,7 @@ int oq_basis_batch(const oq_problem *p, int n_cells, const double *verts,
 /* ---------------------------------------------------------------------------
  * Coarse problem -- source/Q/q_global.cc.
  * Pinned by: A = aI gives the standard coarse Q1 FEM, checked against a dense
- * numpy solve of the closed-form coarse stiffness (tests/test_oracle_coarse.py).
+ * numpy solve of the closed-form coarse stiffness (tests/test_oracle.py).
  * ------------------------------------------------------------------------- */
 static void coarse_cell_nodes(int dim, int nc, int c, int *nodes) {
   int cijk[3] = {0, 0, 0}, r = c;

@@ -20,7 +20,7 @@
  *   source/Q/q_global.cc:92-193,198-297,327-392  coarse grid, assembly, CG
  *
  * Pinning (what checks each function against something other than itself) is
- * listed per function below and exercised by tests/test_oracle_*.py.
+ * listed per function in msfec_oracle.c and exercised by tests/test_oracle.py.
  */
 #ifndef MSFEC_ORACLE_H
 #define MSFEC_ORACLE_H
