@@ -39,6 +39,7 @@ namespace {
 constexpr int kWin = 64;       // sliding-window slots (>= bandwidth + 1)
 constexpr int kMaxNb = 8;      // bases per cell (2^dim)
 constexpr int kMaxLow = 14;    // lower stencil entries incl. diagonal (3-D)
+constexpr int kRowStride = 24; // row table: 16 lower-stencil slots + 8 lifting rhs
 
 struct QConf {
   int dim, N, S, nn, nfc, nv, ni, n, nst, nlow, center;
@@ -310,6 +311,7 @@ struct BandArgs {
   const int* inode;       // interior index -> node id
   const unsigned* imask;  // bit t: lower neighbour t is interior
   // fast path data
+  const double* Rows;     // [cell][n][kRowStride]: lower stencil (16) | lifting rhs (8)
   const double* verts;    // [cell][2^dim][dim]
   const double* coef;     // [cell][nfc]
   const double* reac;     // [cell][nfc] or null
@@ -409,85 +411,170 @@ __device__ __forceinline__ double lift_k(int k, const int* m, double invN) {
   return v;
 }
 
+// Element stiffness / mass of a box as functions of a^b (bit d set: the two
+// vertices differ in direction d), for lanes x < 8.
+template <int DIM>
+__device__ __forceinline__ void box_tables(int x, const double* hh, double& ks, double& ms) {
+  ks = 0.0;
+  ms = 1.0;
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) {
+    double t = (((x >> d) & 1) ? -1.0 : 1.0) / hh[d];
+#pragma unroll
+    for (int e = 0; e < DIM; ++e)
+      if (e != d) t *= (((x >> e) & 1) ? (1.0 / 6.0) : (1.0 / 3.0)) * hh[e];
+    ks += t;
+    ms *= (((x >> d) & 1) ? (1.0 / 6.0) : (1.0 / 3.0)) * hh[d];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// K1f: interior row table for a scalar coefficient per fine cell (fast path).
+// Row j of cell c: Rows[c][j][0..NLOW-1] = lower stencil (t = 0 diagonal,
+// t = 1+e lower direction e, zero where the neighbour is on the boundary),
+// Rows[c][j][16..23] = lifting right-hand sides -sum_{m in B} K[j][m] g_k(m).
+// K[n][n+dd] = kv[mask(dd)] * (sum of the coefficients of the fine cells
+// shared by n and n+dd) + the mass analogue for the reaction term.  One block
+// per cell, one thread per interior row; all branching stays out of the
+// serial factorisation.
+// ---------------------------------------------------------------------------
+template <int DIM>
+__global__ __launch_bounds__(256) void q_rows_kernel(BandArgs a) {
+  using D = Dirs<DIM>;
+  const int cell = blockIdx.x;
+  extern __shared__ double dyn[];  // coefficient | reaction
+  __shared__ double kv[8], mv[8];
+  double* cf = dyn;
+  double* rf = dyn + a.nfc;
+  const bool has_rf = (a.reaction_mode == MSFEC_REACTION_CELL);
+  const double* cg = a.coef + (size_t)cell * a.nfc;
+  for (int z = threadIdx.x; z < a.nfc; z += blockDim.x) cf[z] = cg[z];
+  if (has_rf) {
+    const double* rg = a.reac + (size_t)cell * a.nfc;
+    for (int z = threadIdx.x; z < a.nfc; z += blockDim.x) rf[z] = rg[z];
+  }
+  if (threadIdx.x < 8) {
+    const double* v = a.verts + (size_t)cell * D::NV * DIM;
+    double hh[3] = {1.0, 1.0, 1.0};
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) hh[d] = (v[(D::NV - 1) * DIM + d] - v[d]) / a.N;
+    double ks, ms;
+    box_tables<DIM>(threadIdx.x, hh, ks, ms);
+    kv[threadIdx.x] = ks;
+    mv[threadIdx.x] = ms;
+  }
+  __syncthreads();
+  const double rc = (a.reaction_mode == MSFEC_REACTION_CONST) ? a.reaction_const : 0.0;
+  const double invN = 1.0 / (double)a.N;
+  for (int jn = threadIdx.x; jn < a.n; jn += blockDim.x) {
+    int c[3] = {0, 0, 0};
+    {
+      int t = jn;
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) {
+        c[d] = t % a.ni + 1;
+        t /= a.ni;
+      }
+    }
+    double lo[16], rhs[8];
+#pragma unroll
+    for (int t = 0; t < 16; ++t) lo[t] = 0.0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) rhs[k] = 0.0;
+#pragma unroll
+    for (int e = 0; e < D::NST; ++e) {
+      int dd[3] = {0, 0, 0}, m[3] = {0, 0, 0}, mask = 0;
+      bool bnd = false;
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) {
+        dd[d] = dcomp<DIM>(e, d);
+        if (dd[d] != 0) mask |= 1 << d;
+        m[d] = c[d] + dd[d];
+        if (m[d] == 0 || m[d] == a.N) bnd = true;
+      }
+      double sa, sr;
+      int cnt;
+      shared_sums<DIM>(cf, rf, has_rf, c, dd, a.N, sa, sr, cnt);
+      const double kval = kv[mask] * sa + mv[mask] * (sr + rc * cnt);
+      if (e == D::CENTER) {
+        lo[0] = kval;
+      } else if (bnd) {
+#pragma unroll
+        for (int k = 0; k < D::NV; ++k) rhs[k] = fma(-kval, lift_k<DIM>(k, m, invN), rhs[k]);
+      } else if (e < D::CENTER) {
+        lo[1 + e] = kval;
+      }
+    }
+    double* dst = const_cast<double*>(a.Rows) + ((size_t)cell * a.n + jn) * kRowStride;
+#pragma unroll
+    for (int t = 0; t < 16; ++t) dst[t] = lo[t];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) dst[16 + k] = rhs[k];
+  }
+}
+
+// Rows jbase..jbase+7 of the row table are 192 contiguous doubles: every
+// lane loads 3 of them into registers (issued one staging group ahead) and
+// later stores them to the LDS staging area.
+struct RowPrefetch {
+  double v[3];
+};
+
+__device__ __forceinline__ void rows_load(const BandArgs& a, int cell, int jbase, RowPrefetch& pf) {
+  const int lane = threadIdx.x;
+  const double* src = a.Rows + ((size_t)cell * a.n + jbase) * kRowStride;
+  const int lim = (a.n - jbase) * kRowStride;  // valid doubles from src on
+#pragma unroll
+  for (int u = 0; u < 3; ++u) {
+    const int idx = lane + 64 * u;
+    pf.v[u] = (idx < lim) ? src[idx] : 0.0;
+  }
+}
+
+__device__ __forceinline__ void rows_store(const RowPrefetch& pf, BandSmem& sh) {
+  const int lane = threadIdx.x;
+#pragma unroll
+  for (int u = 0; u < 3; ++u) {
+    const int idx = lane + 64 * u;
+    const int r = idx / kRowStride, t = idx - kRowStride * r;
+    if (t < 16) sh.stg_low[r][t] = pf.v[u];
+    else sh.stg_rhs[r][t - 16] = pf.v[u];
+  }
+  wave_sync();
+}
+
 // Stage the rows jbase..jbase+7 (lower stencil with zeros for non-interior
-// neighbours, and the lifting right-hand sides) into LDS.
-template <int DIM, bool FAST>
-__device__ __forceinline__ void stage_rows(const BandArgs& a, int cell, int jbase, BandSmem& sh,
-                                           const double* cf, const double* rf) {
+// neighbours, and the lifting right-hand sides) into LDS from the tables of
+// q_assemble_kernel (general path).
+template <int DIM>
+__device__ __forceinline__ void stage_rows_general(const BandArgs& a, int cell, int jbase,
+                                                   BandSmem& sh) {
   using D = Dirs<DIM>;
   const int lane = threadIdx.x;
   const int r = lane >> 3, s = lane & 7;
   const int jn = jbase + r;
+  if (jn < a.n) {
+    if (s == 0) {
+      const unsigned msk = a.imask[jn];
+      const double* src = a.Slo + ((size_t)cell * a.nn + a.inode[jn]) * D::NLOW;
+#pragma unroll
+      for (int t = 0; t < D::NLOW; ++t) sh.stg_low[r][t] = ((msk >> t) & 1u) ? src[t] : 0.0;
+    }
+    sh.stg_rhs[r][s] = a.Rhs[((size_t)cell * a.n + jn) * kMaxNb + s];
+  }
+  wave_sync();
+}
+
+// One staging group: the fast path stores the prefetched rows and issues the
+// loads of the next group; the general path reads its tables directly.
+template <int DIM, bool FAST>
+__device__ __forceinline__ void stage_group(const BandArgs& a, int cell, int jbase, BandSmem& sh,
+                                            RowPrefetch& pf) {
   if (FAST) {
-    double rhs[8];
-#pragma unroll
-    for (int k = 0; k < 8; ++k) rhs[k] = 0.0;
-    if (jn < a.n) {
-      int c[3] = {0, 0, 0};
-      {
-        int t = jn;
-#pragma unroll
-        for (int d = 0; d < DIM; ++d) {
-          c[d] = t % a.ni + 1;
-          t /= a.ni;
-        }
-      }
-      const bool has_rf = (a.reaction_mode == MSFEC_REACTION_CELL);
-      const double rc = (a.reaction_mode == MSFEC_REACTION_CONST) ? a.reaction_const : 0.0;
-      const double invN = 1.0 / (double)a.N;
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const int e = s + 8 * u;
-        if (e >= D::NST || e == D::CENTER) continue;
-        int dd[3] = {0, 0, 0}, m[3] = {0, 0, 0}, mask = 0;
-        bool bnd = false;
-#pragma unroll
-        for (int d = 0; d < DIM; ++d) {
-          dd[d] = dcomp<DIM>(e, d);
-          if (dd[d] != 0) mask |= 1 << d;
-          m[d] = c[d] + dd[d];
-          if (m[d] == 0 || m[d] == a.N) bnd = true;
-        }
-        double sa, sr;
-        int cnt;
-        shared_sums<DIM>(cf, rf, has_rf, c, dd, a.N, sa, sr, cnt);
-        const double kval = sh.kv[mask] * sa + sh.mv[mask] * (sr + rc * cnt);
-        if (e < D::CENTER) sh.stg_low[r][1 + e] = bnd ? 0.0 : kval;
-        if (bnd) {
-#pragma unroll
-          for (int k = 0; k < D::NV; ++k) rhs[k] = fma(-kval, lift_k<DIM>(k, m, invN), rhs[k]);
-        }
-      }
-      if (s == 0) {
-        int dd[3] = {0, 0, 0};
-        double sa, sr;
-        int cnt;
-        shared_sums<DIM>(cf, rf, has_rf, c, dd, a.N, sa, sr, cnt);
-        sh.stg_low[r][0] = sh.kv[0] * sa + sh.mv[0] * (sr + rc * cnt);
-      }
-    }
-#pragma unroll
-    for (int k = 0; k < 8; ++k) SH_PART(sh)[lane * 9 + k] = rhs[k];
-    wave_sync();
-    {
-      const int k = s;
-      double t = 0.0;
-#pragma unroll
-      for (int q = 0; q < 8; ++q) t += SH_PART(sh)[((r << 3) + q) * 9 + k];
-      sh.stg_rhs[r][k] = t;
-    }
-    wave_sync();
+    rows_store(pf, sh);
+    rows_load(a, cell, jbase + 8, pf);
   } else {
-    if (jn < a.n) {
-      if (s == 0) {
-        const unsigned msk = a.imask[jn];
-        const double* src = a.Slo + ((size_t)cell * a.nn + a.inode[jn]) * D::NLOW;
-#pragma unroll
-        for (int t = 0; t < D::NLOW; ++t) sh.stg_low[r][t] = ((msk >> t) & 1u) ? src[t] : 0.0;
-      }
-      sh.stg_rhs[r][s] = a.Rhs[((size_t)cell * a.n + jn) * kMaxNb + s];
-    }
-    wave_sync();
+    stage_rows_general<DIM>(a, cell, jbase, sh);
   }
 }
 
@@ -776,21 +863,10 @@ __global__ __launch_bounds__(64, 2) void q_band_kernel(BandArgs a) {
       vol *= hh[d];
     }
     if (lane < 8) {
-      // element stiffness / mass of a box as a function of a^b (bit d set:
-      // the two vertices differ in direction d)
-      const int x = lane;
-      double ks = 0.0, ms = 1.0;
-#pragma unroll
-      for (int d = 0; d < DIM; ++d) {
-        double t = (((x >> d) & 1) ? -1.0 : 1.0) / hh[d];
-#pragma unroll
-        for (int e = 0; e < DIM; ++e)
-          if (e != d) t *= (((x >> e) & 1) ? (1.0 / 6.0) : (1.0 / 3.0)) * hh[e];
-        ks += t;
-        ms *= (((x >> d) & 1) ? (1.0 / 6.0) : (1.0 / 3.0)) * hh[d];
-      }
-      sh.kv[x] = ks;
-      sh.mv[x] = ms;
+      double ks, ms;
+      box_tables<DIM>(lane, hh, ks, ms);
+      sh.kv[lane] = ks;
+      sh.mv[lane] = ms;
     }
     wave_sync();
   }
@@ -813,8 +889,10 @@ __global__ __launch_bounds__(64, 2) void q_band_kernel(BandArgs a) {
   for (int k = 0; k < 8; ++k) y[k] = 0.0;
 
   // initial window: indices 0..63 in slots 0..63
+  RowPrefetch pf;
+  if (FAST) rows_load(a, cell, 0, pf);
   for (int G = 0; G < 8; ++G) {
-    stage_rows<DIM, FAST>(a, cell, 8 * G, sh, cf, rf);
+    stage_group<DIM, FAST>(a, cell, 8 * G, sh, pf);
     band_insert<DIM, 0>(a, 8 * G + 0, G, my_off, A, y, sh);
     wave_sync();
     band_insert<DIM, 1>(a, 8 * G + 1, G, my_off, A, y, sh);
@@ -838,7 +916,7 @@ __global__ __launch_bounds__(64, 2) void q_band_kernel(BandArgs a) {
   const int n = a.n;
   for (int j0 = 0; j0 < n; j0 += 8) {
     const int G = (j0 >> 3) & 7;
-    stage_rows<DIM, FAST>(a, cell, j0 + kWin, sh, cf, rf);
+    stage_group<DIM, FAST>(a, cell, j0 + kWin, sh, pf);
     if (j0 + 0 < n) band_step<DIM, 0>(a, cell, j0 + 0, G, my_off, A, y, sh, bad, sacc);
     if (j0 + 1 < n) band_step<DIM, 1>(a, cell, j0 + 1, G, my_off, A, y, sh, bad, sacc);
     if (j0 + 2 < n) band_step<DIM, 2>(a, cell, j0 + 2, G, my_off, A, y, sh, bad, sacc);
@@ -1218,11 +1296,13 @@ __global__ __launch_bounds__(64, 2) void q_band_mfma_kernel(BandArgs a) {
   for (int P = 0; P < 4; ++P) Yt[P] = (mf_d4){0.0, 0.0, 0.0, 0.0};
 
   // initial window: indices 0..63 in slots 0..63
+  RowPrefetch pf;
+  if (FAST) rows_load(a, cell, 0, pf);
   for (int T = 0; T < 4; ++T) {
-    stage_rows<DIM, FAST>(a, cell, 16 * T, sh, cf, rf);
+    stage_group<DIM, FAST>(a, cell, 16 * T, sh, pf);
     mf_insert<DIM, 0>(a, 16 * T + 0, 0, T, my_off, Tl, Yt, sh, ms);
     mf_insert<DIM, 1>(a, 16 * T + 4, 4, T, my_off, Tl, Yt, sh, ms);
-    stage_rows<DIM, FAST>(a, cell, 16 * T + 8, sh, cf, rf);
+    stage_group<DIM, FAST>(a, cell, 16 * T + 8, sh, pf);
     mf_insert<DIM, 2>(a, 16 * T + 8, 0, T, my_off, Tl, Yt, sh, ms);
     mf_insert<DIM, 3>(a, 16 * T + 12, 4, T, my_off, Tl, Yt, sh, ms);
   }
@@ -1231,7 +1311,7 @@ __global__ __launch_bounds__(64, 2) void q_band_mfma_kernel(BandArgs a) {
   const int n = a.n;
   for (int j0 = 0; j0 < n; j0 += 16) {
     const int T = (j0 >> 4) & 3;
-    stage_rows<DIM, FAST>(a, cell, j0 + kWin, sh, cf, rf);
+    stage_group<DIM, FAST>(a, cell, j0 + kWin, sh, pf);
     if (j0 < n) {
       mf_superstep<DIM, 0>(a, cell, j0, T, Tl, Yt, Sacc, sh, ms, bad);
       mf_insert<DIM, 0>(a, j0 + kWin, 0, T, my_off, Tl, Yt, sh, ms);
@@ -1240,7 +1320,7 @@ __global__ __launch_bounds__(64, 2) void q_band_mfma_kernel(BandArgs a) {
       mf_superstep<DIM, 1>(a, cell, j0 + 4, T, Tl, Yt, Sacc, sh, ms, bad);
       mf_insert<DIM, 1>(a, j0 + 4 + kWin, 4, T, my_off, Tl, Yt, sh, ms);
     }
-    stage_rows<DIM, FAST>(a, cell, j0 + 8 + kWin, sh, cf, rf);
+    stage_group<DIM, FAST>(a, cell, j0 + 8 + kWin, sh, pf);
     if (j0 + 8 < n) {
       mf_superstep<DIM, 2>(a, cell, j0 + 8, T, Tl, Yt, Sacc, sh, ms, bad);
       mf_insert<DIM, 2>(a, j0 + 8 + kWin, 0, T, my_off, Tl, Yt, sh, ms);
@@ -1625,6 +1705,7 @@ struct msfec_q_batch {
   double* d_coef = nullptr;
   double* d_f = nullptr;
   double* d_reac = nullptr;
+  double* d_rows = nullptr;
   double* d_slo = nullptr;
   double* d_F = nullptr;
   double* d_rhs = nullptr;
@@ -1666,7 +1747,7 @@ msfec_status dalloc(msfec_q_batch* h, T** p, size_t count) {
 }
 
 void free_all(msfec_q_batch* h) {
-  void* ptrs[] = {h->d_verts, h->d_coef, h->d_f,     h->d_reac,  h->d_slo,   h->d_F,
+  void* ptrs[] = {h->d_rows,  h->d_verts, h->d_coef, h->d_f,     h->d_reac,  h->d_slo,   h->d_F,
                   h->d_rhs,   h->d_L,    h->d_W,     h->d_X,     h->d_psi,   h->d_M,
                   h->d_b,     h->d_status, h->d_inode, h->d_imask, h->d_cK,  h->d_crhs,
                   h->d_cvec,  h->d_cit};
@@ -1810,6 +1891,7 @@ msfec_status msfec_q_create(const msfec_q_params* params, int n_cells, const dou
   TRY(dalloc(h, &h->d_coef, h->coef_count));
   TRY(dalloc(h, &h->d_f, h->rhs_count));
   TRY(dalloc(h, &h->d_reac, h->reac_count));
+  if (h->fast) TRY(dalloc(h, &h->d_rows, (size_t)n_cells * c.n * kRowStride));
   if (!h->fast) {
     TRY(dalloc(h, &h->d_slo, (size_t)n_cells * c.nn * c.nlow));
     TRY(dalloc(h, &h->d_F, (size_t)n_cells * c.nn));
@@ -1940,13 +2022,6 @@ msfec_status msfec_q_compute(msfec_q_handle h, void* stream) {
   const size_t rstride = h->reac_count ? h->reac_count / h->n_cells : 0;
   const bool fast = h->fast;
   if (tm) MSFEC_HIP(hipEventRecord(ev[0], s));
-  if (!fast) {
-    hipLaunchKernelGGL(q_assemble_kernel, dim3(h->n_cells), dim3(256), 0, s, c, h->d_verts,
-                       h->d_coef, cstride, h->rhs_count ? h->d_f : nullptr, fstride,
-                       h->reac_count ? h->d_reac : nullptr, rstride, h->d_slo, h->d_F, h->d_rhs);
-    MSFEC_HIP(hipGetLastError());
-  }
-  if (tm) MSFEC_HIP(hipEventRecord(ev[1], s));
   BandArgs a{};
   a.n = c.n;
   a.ni = c.ni;
@@ -1955,6 +2030,7 @@ msfec_status msfec_q_compute(msfec_q_handle h, void* stream) {
   a.nn = c.nn;
   a.nfc = c.nfc;
   for (int t = 0; t < kMaxLow; ++t) a.off_int[t] = c.off_int[t];
+  a.Rows = h->d_rows;
   a.Slo = h->d_slo;
   a.Rhs = h->d_rhs;
   a.inode = h->d_inode;
@@ -1975,6 +2051,22 @@ msfec_status msfec_q_compute(msfec_q_handle h, void* stream) {
   a.bvec = h->d_b;
   a.status = h->d_status;
   a.nq = (c.n + 3) / 4;
+  if (fast) {
+    const size_t rdyn =
+        (size_t)c.nfc * (1 + (c.reaction_mode == MSFEC_REACTION_CELL ? 1 : 0)) * sizeof(double);
+    if (c.dim == 3)
+      hipLaunchKernelGGL((q_rows_kernel<3>), dim3(h->n_cells), dim3(256), rdyn, s, a);
+    else
+      hipLaunchKernelGGL((q_rows_kernel<2>), dim3(h->n_cells), dim3(256), rdyn, s, a);
+    MSFEC_HIP(hipGetLastError());
+  }
+  if (!fast) {
+    hipLaunchKernelGGL(q_assemble_kernel, dim3(h->n_cells), dim3(256), 0, s, c, h->d_verts,
+                       h->d_coef, cstride, h->rhs_count ? h->d_f : nullptr, fstride,
+                       h->reac_count ? h->d_reac : nullptr, rstride, h->d_slo, h->d_F, h->d_rhs);
+    MSFEC_HIP(hipGetLastError());
+  }
+  if (tm) MSFEC_HIP(hipEventRecord(ev[1], s));
   const bool mfma = (h->p.solver == MSFEC_SOLVER_BAND_FP64_MFMA);
   const size_t dyn = fast ? (size_t)c.nfc *
                                 (1 + (c.reaction_mode == MSFEC_REACTION_CELL ? 1 : 0) +
