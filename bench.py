@@ -29,11 +29,17 @@ import numpy as np  # noqa: E402
 DIM, NREF, NC = 3, 3, 16
 SIGMA, CORR, SEED = 1.5, 1.0 / 64, 20261202
 FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 (vector and matrix)
-# local solvers: name -> (solver id, band kernel name); HEADLINE is the value line
-SOLVERS = {"band-fp64-valu": (0, "q_band_kernel"), "band-fp64-mfma": (1, "q_band_mfma_kernel")}
-SOLVER_TEXT = {"band-fp64-valu": "banded Cholesky FP64 on the vector ALUs",
+FP32_MFMA_PEAK_TFLOPS = 157.3  # MI355X FP32 matrix cores, dense
+# local solvers: name -> (solver id, dominant kernel, its arithmetic); HEADLINE
+# is the value line, the others are timed beside it
+SOLVERS = {"band-mixed": (2, "q_blk_factor_kernel<float>", "f32"),
+           "band-fp64-valu": (0, "q_band_kernel", "f64"),
+           "band-fp64-mfma": (1, "q_blk_factor_kernel<double>", "f64")}
+SOLVER_TEXT = {"band-mixed": "banded Cholesky FP32 on the matrix cores + iterative refinement "
+                             "with FP64 residuals until (|dx|/|x|)^2 <= 1e-13 per cell",
+               "band-fp64-valu": "banded Cholesky FP64 on the vector ALUs",
                "band-fp64-mfma": "banded Cholesky FP64 on the matrix cores"}
-HEADLINE = "band-fp64-valu"
+HEADLINE = "band-mixed"
 TRAFFIC_FILE = os.path.join(REPO, "profiles", "hbm_traffic.json")
 HBM_PEAK_GBS = 8000.0     # MI355X HBM3E
 
@@ -56,10 +62,17 @@ def dense_flops_per_cell(n: int, nrhs: int) -> float:
 
 
 def band_bytes_per_cell(n: int, p: int, nlow: int, nrhs: int = 8) -> float:
-    """Compulsory HBM bytes of the band kernel per cell: stencil rows and
+    """Compulsory HBM bytes of the FP64 band kernel per cell: stencil rows and
     lifting right-hand sides in, the band factor L written once and read once
     by the backward solve, forward results written and read, solution out."""
     return 8.0 * (n * nlow + n * nrhs + 2 * n * (p + 1) + 2 * n * nrhs + n * nrhs)
+
+
+def mixed_factor_bytes_per_cell(n: int, p: int, nlow: int, nrhs: int = 8) -> float:
+    """The same for the FP32 factor kernel of the mixed solver: FP64 rows and
+    right-hand sides in, the FP32 factor and forward results written and read
+    back, the FP64 first solution out."""
+    return 8.0 * (n * nlow + n * nrhs) + 4.0 * (2 * n * (p + 1) + 2 * n * nrhs) + 8.0 * n * nrhs
 
 
 def measured_traffic(kernel: str, n_cells: int):
@@ -191,15 +204,15 @@ def main():
 
     # the other local solvers, timed beside the headline on fewer steps
     others = {}
-    for name, (solver, kname) in SOLVERS.items():
+    for name, (solver, kname, _) in SOLVERS.items():
         if name == HEADLINE:
             continue
         k = max(2, args.steps // 4)
         q, w, t = measure(solver, k, min(args.warmup, 1))
         others[name] = {"cells_per_s": round(q.info.n_cells / (w / k), 1),
-                        "kernel": kname, "band_kernel_ms": round(t[1], 4)}
+                        "kernel": kname, "kernel_ms": round(t[1] + t[2] + t[3], 4)}
         q.close()
-    solver, kname = SOLVERS[HEADLINE]
+    solver, kname, arith = SOLVERS[HEADLINE]
     qb, wall, kt = measure(solver, args.steps, args.warmup)
     info = qb.info
     ms_step = wall * 1e3 / args.steps
@@ -209,25 +222,32 @@ def main():
     n, p = info.n_interior, info.bandwidth
     nlow = 14
     flops_cell = band_flops_per_cell(n, p, 8)
-    bytes_cell = band_bytes_per_cell(n, p, nlow)
-    band_ms = kt[1]
+    if arith == "f32":
+        bytes_cell = mixed_factor_bytes_per_cell(n, p, nlow)
+        peak_tflops = FP32_MFMA_PEAK_TFLOPS
+    else:
+        bytes_cell = band_bytes_per_cell(n, p, nlow)
+        peak_tflops = FP64_PEAK_TFLOPS
+    band_ms = kt[1]  # the dominant kernel: factor + forward + backward sweep
     tflops = flops_cell * info.n_cells / (band_ms * 1e-3) / 1e12
     gbs = bytes_cell * info.n_cells / (band_ms * 1e-3) / 1e9
-    f_frac, b_frac = tflops / FP64_PEAK_TFLOPS, gbs / HBM_PEAK_GBS
+    f_frac, b_frac = tflops / peak_tflops, gbs / HBM_PEAK_GBS
     traffic = measured_traffic(kname, info.n_cells)
     if b_frac >= f_frac:
         roof = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(b_frac, 4), "traffic": traffic}
     else:
-        roof = {"bound": "mfma", "achieved": round(tflops, 3), "peak": FP64_PEAK_TFLOPS,
+        roof = {"bound": "mfma", "achieved": round(tflops, 3), "peak": peak_tflops,
                 "unit": "TFLOP/s", "frac": round(f_frac, 4), "traffic": traffic}
     roof.update({"kernel": kname, "kernel_ms": round(band_ms, 4),
                  "flops_per_cell": flops_cell, "bytes_per_cell": bytes_cell,
-                 "fp64_tflops": round(tflops, 3), "hbm_gbs": round(gbs, 1),
+                 "tflops": round(tflops, 3), "tflops_frac": round(f_frac, 4),
+                 "hbm_gbs": round(gbs, 1), "hbm_frac": round(b_frac, 4),
                  "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
                                  "copy-calibrated, profiles/hbm_traffic.json)",
                  "dense_equiv_tflops": round(dense_flops_per_cell(n, 8) * info.n_cells /
                                              (band_ms * 1e-3) / 1e12, 3)})
+    fp64_best = max((v["cells_per_s"] for k, v in others.items() if "fp64" in k), default=None)
 
     out = {
         "metric": "basis cells/sec, 3D Q 8^3 fine/cell: mixed vs FP64 GPU at ≤1e-10 rel err",
@@ -240,7 +260,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32+f64" if arith == "f32" else "f64",
         "data": "synthetic: seeded lognormal A=exp(g), g Gaussian sigma 1.5, corr. length 1/64 "
                 "on the 128^3 fine grid (seed 20261202), f=1",
         "config": {"workload": "3D Q1 basis construction, 16^3=4096 coarse cells x 8^3 fine "
@@ -248,8 +268,10 @@ def main():
                    "n_cells": info.n_cells, "n_refine_local": NREF,
                    "solver": HEADLINE, "parallelism": "single GPU"},
         "other_solvers": others,
-        "kernel_ms": {"assemble": round(kt[0], 4), "band_factor_solve": round(kt[1], 4),
-                      "element": round(kt[2], 4)},
+        "speedup_vs_fp64": round(cells_per_s / fp64_best, 3) if fp64_best else None,
+        "kernel_ms": {"rows": round(kt[0], 4), "factor_solve": round(kt[1], 4),
+                      "refine_residual": round(kt[2], 4), "refine_sweep": round(kt[3], 4),
+                      "element": round(kt[4], 4)},
         "wall_s_timed": round(wall, 3),
         "failed_cells": int((status != 0).sum()),
         "roofline": roof,

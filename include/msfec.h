@@ -54,14 +54,20 @@ enum { MSFEC_COEF_SCALAR_CELL = 0, MSFEC_COEF_TENSOR_QP = 1 };
 enum { MSFEC_RHS_CONST = 0, MSFEC_RHS_CELL = 1, MSFEC_RHS_QP = 2 };
 /* optional reaction R u (the reference's QBasis has none: 0 reproduces it) */
 enum { MSFEC_REACTION_NONE = 0, MSFEC_REACTION_CONST = 1, MSFEC_REACTION_CELL = 2 };
-/* local solver: banded Cholesky of the interior system in FP64, eliminating
- * one pivot per step on the vector ALUs (BAND_FP64, the default of the
- * Python and C++ front ends) or four pivots per step with FP64 matrix-core
- * updates (BAND_FP64_MFMA).  Both are direct solvers, the analogue of
- * use_direct_solver_basis = true (source/Q/q_basis.cc:304-330). */
-enum { MSFEC_SOLVER_BAND_FP64 = 0, MSFEC_SOLVER_BAND_FP64_MFMA = 1 };
-/* per-cell status codes */
-enum { MSFEC_CELL_OK = 0, MSFEC_CELL_NOT_SPD = 2, MSFEC_CELL_NONFINITE = 3 };
+/* local solver: banded Cholesky of the interior system, the analogue of
+ * use_direct_solver_basis = true (source/Q/q_basis.cc:304-330):
+ *   BAND_FP64      FP64, one pivot per step on the vector ALUs (the default
+ *                  of the Python and C++ front ends);
+ *   BAND_FP64_MFMA FP64, four pivots per step with FP64 matrix-core updates;
+ *   BAND_MIXED     FP32 factor on the matrix cores and iterative refinement
+ *                  with the residual in FP64: a cell stops refining once the
+ *                  predicted error (|dx| / |x|)^2 is below 1e-13, after at
+ *                  most max_refine steps (scalar coefficient per fine cell and
+ *                  constant or per-cell load only). */
+enum { MSFEC_SOLVER_BAND_FP64 = 0, MSFEC_SOLVER_BAND_FP64_MFMA = 1, MSFEC_SOLVER_BAND_MIXED = 2 };
+/* per-cell status codes.  NOT_CONVERGED: BAND_MIXED stopped after
+ * max_refine steps with a predicted error above 1e-10 relative. */
+enum { MSFEC_CELL_OK = 0, MSFEC_CELL_NOT_CONVERGED = 1, MSFEC_CELL_NOT_SPD = 2, MSFEC_CELL_NONFINITE = 3 };
 /* flags */
 enum { MSFEC_FLAG_TIMING = 1 /* record per-kernel HIP events in compute */ };
 
@@ -79,6 +85,7 @@ typedef struct {
   double reaction_const;
   int solver;          /* MSFEC_SOLVER_* (replaces use_direct_solver)      */
   int flags;           /* MSFEC_FLAG_*                                     */
+  int max_refine;      /* BAND_MIXED: refinement steps at most (0 = 3)     */
 } msfec_q_params;
 
 typedef struct {
@@ -153,8 +160,11 @@ msfec_status msfec_q_get_info(msfec_q_handle h, msfec_q_info *info);
 
 /* Per-kernel times (ms), averaged over every compute made with
  * MSFEC_FLAG_TIMING since the previous call (which this call then forgets):
- * [0] assembly, [1] factor+solve, [2] element matrix.  Synchronizes.  The
- * number of entries written is returned in *n. */
+ * [0] assembly (row table or stencil tables), [1] factor + solve (for
+ * BAND_MIXED: the FP32 factor and first solution), [2] refinement residual
+ * (BAND_MIXED), [3] refinement sweep and element matrix (BAND_MIXED),
+ * [4] element matrix (tensor coefficients).  Entries that do not apply are
+ * zero.  Synchronizes.  The number of entries written is returned in *n. */
 msfec_status msfec_q_kernel_times(msfec_q_handle h, float *ms, int max, int *n);
 
 void msfec_q_destroy(msfec_q_handle h);

@@ -35,6 +35,7 @@ struct ParametersBasis {
   int reaction_mode = MSFEC_REACTION_NONE;
   double reaction_const = 0.0;
   int solver = MSFEC_SOLVER_BAND_FP64;
+  int max_refine = 0;  // BAND_MIXED: refinement steps at most (0 = library default)
 };
 
 class QBatch {

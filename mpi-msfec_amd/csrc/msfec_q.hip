@@ -319,6 +319,11 @@ struct BandArgs {
   int reaction_mode, rhs_mode;
   double reaction_const, rhs_const;
   // scratch and outputs
+  void* pan;              // matrix-core kernels: [cell][nq][4][64] panels (FP64 or FP32)
+  void* zst;              // matrix-core kernels: [cell][nq][4][8] forward results
+  double* Rres;           // mixed solver: [cell][n][8] refinement residual
+  int* conv;              // mixed solver: [cell] 1 once the cell stops refining
+  int last_pass;          // mixed solver: this refinement pass is the last one
   double* Lcol;  // [cell][n][64]  o = 0: pivot d_j, o > 0: L[j+o][j]
   double* Wf;    // [cell][n][8]   D^-1 L^-1 rhs
   double* X;     // [cell][n][8]   interior solution
@@ -352,7 +357,8 @@ struct BandSmem {
   //   forward : RHS partials [64][9]
   //   backward: w of the group [8][8] | L columns of the group [8][72] | x window [64][9]
   //   epilogue: psi of 64 nodes [64][9] | (K psi)_B of 64 nodes [64][9] | F of 64 nodes [64]
-  double pool[64 + 576 + 576];
+  //   matrix-core backward: x ring [8][132] | z [4][8] | partial sums [4][8] | table rows [4][24] | x [4][8]
+  double pool[1280];
 };
 #define SH_PART(sh) ((sh).pool)
 #define SH_WBUF(sh) ((sh).pool)
@@ -442,11 +448,16 @@ template <int DIM>
 __global__ __launch_bounds__(256) void q_rows_kernel(BandArgs a) {
   using D = Dirs<DIM>;
   const int cell = blockIdx.x;
-  extern __shared__ double dyn[];  // coefficient | reaction
+  extern __shared__ double dyn[];  // coefficient | reaction | load
   __shared__ double kv[8], mv[8];
   double* cf = dyn;
   double* rf = dyn + a.nfc;
   const bool has_rf = (a.reaction_mode == MSFEC_REACTION_CELL);
+  double* ff = dyn + a.nfc * (has_rf ? 2 : 1);
+  if (a.rhs_mode == MSFEC_RHS_CELL) {
+    const double* fg = a.fcell + (size_t)cell * a.nfc;
+    for (int z = threadIdx.x; z < a.nfc; z += blockDim.x) ff[z] = fg[z];
+  }
   const double* cg = a.coef + (size_t)cell * a.nfc;
   for (int z = threadIdx.x; z < a.nfc; z += blockDim.x) cf[z] = cg[z];
   if (has_rf) {
@@ -505,6 +516,26 @@ __global__ __launch_bounds__(256) void q_rows_kernel(BandArgs a) {
         lo[1 + e] = kval;
       }
     }
+    // slot 14: sum of the loads of the fine cells around the node (the
+    // interior load is this times |cell| / 2^dim); slot 15: the node index
+    double fj;
+    if (a.rhs_mode == MSFEC_RHS_CELL) {
+      fj = 0.0;
+#pragma unroll
+      for (int al = 0; al < D::NV; ++al) {
+        int z = 0, mul = 1;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+          z += (c[d] - ((al >> d) & 1)) * mul;
+          mul *= a.N;
+        }
+        fj += ff[z];
+      }
+    } else {
+      fj = a.rhs_const * D::NV;
+    }
+    lo[14] = fj;
+    lo[15] = (double)(c[0] + a.S * (c[1] + a.S * c[2]));
     double* dst = const_cast<double*>(a.Rows) + ((size_t)cell * a.n + jn) * kRowStride;
 #pragma unroll
     for (int t = 0; t < 16; ++t) dst[t] = lo[t];
@@ -1386,6 +1417,655 @@ __global__ __launch_bounds__(64, 2) void q_band_mfma_kernel(BandArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// K2m: the band factorisation on the matrix cores with the tile rows of the
+// window unrolled at compile time, for FP64 tiles (solver BAND_FP64_MFMA) or
+// FP32 tiles (the factor of the mixed-precision solver).
+//
+// Layout.  The window is the lower half of a 4x4 grid of 16x16 tiles held as
+// MFMA accumulators.  Window slot x = (row index mod 64) sits in tile row
+// x / 16.  Inside a tile, accumulator register g of lane l holds row slot
+// 4g + (l >> 4) and column slot cslot(l & 15); the same lane, as an A/B
+// operand, holds row slot cslot(l & 15) and pivot k = l >> 4.  cslot is the
+// identity for v_mfma_f64_16x16x4f64 (accumulator (i,j) in lane
+// j + 16 (i mod 4), register i / 4) and 4 (x mod 4) + x / 4 for
+// v_mfma_f32_16x16x4f32 (accumulator (i,j) in lane j + 16 (i / 4), register
+// i mod 4): with it, the 4 pivots of a super-step are one register of every
+// tile in both precisions.  Tile rows are template parameters, so every tile
+// access is a fixed register.
+//
+// A super-step (tile row T, pivots 16T + 4G .. +3) copies the pivot rows to
+// LDS, factors the 4x4 pivot block C C^T in FP64 (all lanes), forms the panel
+// V = A21 C^-T in operand layout, updates the 10 window tiles, the 4
+// right-hand-side tiles and the Schur term with 15 MFMAs, streams the panel
+// (with C^-1 in place of its first 4 rows) and Z = C^-1 y to HBM, and
+// inserts the next 4 rows of the band.  The backward sweep reads the panels
+// back: x_blk = C^-T (z - sum_{o>=4} V[o]^T x[j0 + o]).
+// ---------------------------------------------------------------------------
+template <typename R>
+struct Mf;
+template <>
+struct Mf<double> {
+  typedef double v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ v4 mma(double a, double b, v4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int cslot(int ll) { return ll; }
+  // physical accumulator row of register g in lane group t
+  static __device__ __forceinline__ int acc_row(int g, int t) { return 4 * g + t; }
+};
+template <>
+struct Mf<float> {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ v4 mma(float a, float b, v4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int cslot(int ll) { return 4 * (ll & 3) + (ll >> 2); }
+  static __device__ __forceinline__ int acc_row(int g, int t) { return 4 * t + g; }
+};
+
+// backward-sweep strides: panel rows padded to 68 (zeros at 64..67), x ring
+// rows of 132 (copy at +64, zeros at 128..131)
+constexpr int kPanRow = 68;
+constexpr int kRing = 132;
+
+template <typename R>
+struct BkSmem {
+  R piv[4 * 72];   // forward: pivot rows, slot x at pad8(x); backward: panel [4][68]
+  R nrow[4 * 72];  // forward: incoming rows
+  R yp[64];        // forward: pivot right-hand sides [4][16]
+};
+
+// What the backward sweep produces.
+enum { BK_FINAL64 = 0, BK_X0 = 1, BK_REFINE = 2 };
+
+// The row index as an opaque scalar: slot computations such as
+// (row - offset) mod 64 stay inside the super-step instead of being hoisted
+// out of the loop as one register per unrolled super-step.
+__device__ __forceinline__ int bk_opaque(int x) {
+  asm volatile("" : "+s"(x));
+  return x;
+}
+
+// The lane index as an opaque value: the per-lane LDS addresses of a
+// super-step are recomputed in it (a few integer operations) instead of
+// being hoisted out of the loop as dozens of live registers.
+__device__ __forceinline__ int bk_lane() {
+  int x = threadIdx.x;
+  asm volatile("" : "+v"(x));
+  return x;
+}
+
+template <typename R, int T, int G>
+__device__ __forceinline__ void bk_extract(const typename Mf<R>::v4 (&Tl)[10], R* piv) {
+  const int lane = bk_lane(), lh = lane >> 4, cs = Mf<R>::cslot(lane & 15);
+#pragma unroll
+  for (int Q = 0; Q <= T; ++Q) piv[lh * 72 + pad8(16 * Q + cs)] = Tl[MF_TILE(T, Q)][G];
+  if ((cs >> 2) == G) {
+    const int t = cs & 3;
+#pragma unroll
+    for (int P = T + 1; P < 4; ++P)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) piv[t * 72 + pad8(16 * P + 4 * r + lh)] = Tl[MF_TILE(P, T)][r];
+  }
+}
+
+// Insert the rows jbase..jbase+3 (staged as rows rb..rb+3) into slots
+// 16T + 4G .. +3: scatter their nonzeros into LDS, then place rows and columns.
+template <typename R, int DIM, int T, int G>
+__device__ __forceinline__ void bk_insert(const BandArgs& a, int jbase, int rb, int my_off,
+                                          typename Mf<R>::v4 (&Tl)[10],
+                                          typename Mf<R>::v4 (&Yt)[4], BandSmem& sh,
+                                          BkSmem<R>& ms) {
+  using D = Dirs<DIM>;
+  const int lane = bk_lane(), lh = lane >> 4, ll = lane & 15, cs = Mf<R>::cslot(ll);
+  constexpr int s0 = 16 * T + 4 * G;
+#pragma unroll
+  for (int t = 0; t < 4; ++t) ms.nrow[t * 72 + pad8(lane)] = (R)0;
+  wave_sync();
+  {
+    const int t = lh, e = ll, jn = jbase + t;
+    if (jn < a.n) {
+      if (e < D::NLOW) {
+        const double v = sh.stg_low[rb + t][e];
+        if (v != 0.0) {
+          // slot of row jn - off; jbase = s0 mod 64
+          ms.nrow[t * 72 + pad8((bk_opaque(jbase) + t - my_off) & (kWin - 1))] = (R)v;
+          // couplings between two incoming rows are needed in both rows
+          if (e > 0 && my_off <= t) ms.nrow[(t - my_off) * 72 + pad8(s0 + t)] = (R)v;
+        }
+      }
+    } else if (e == 0) {
+      ms.nrow[t * 72 + pad8(s0 + t)] = (R)1;  // past the end: isolated unit diagonal
+    }
+  }
+  wave_sync();
+#pragma unroll
+  for (int Q = 0; Q <= T; ++Q) Tl[MF_TILE(T, Q)][G] = ms.nrow[lh * 72 + pad8(16 * Q + cs)];
+  if ((cs >> 2) == G) {
+    const int t = cs & 3;
+#pragma unroll
+    for (int P = T; P < 4; ++P)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) Tl[MF_TILE(P, T)][r] = ms.nrow[t * 72 + pad8(16 * P + 4 * r + lh)];
+  }
+  Yt[T][G] = (ll < 8 && jbase + lh < a.n) ? (R)sh.stg_rhs[rb + lh][ll] : (R)0;
+  wave_sync();
+}
+
+// 1/sqrt: the hardware estimate, refined by two Newton steps in FP64 (one
+// step is not enough for FP64; the FP32 estimate is already within 1 ulp).
+__device__ __forceinline__ double bk_rsqrt(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  y = y * fma(-0.5 * x * y, y, 1.5);
+  y = y * fma(-0.5 * x * y, y, 1.5);
+  return y;
+}
+__device__ __forceinline__ float bk_rsqrt(float x) { return __builtin_amdgcn_rsqf(x); }
+
+template <typename R>
+__device__ __forceinline__ void bk_check(R d, int& bad) {
+  // keep the first failure: later pivots of a failed cell are meaningless
+  if (!(d > (R)0) && bad == 0) bad = (d == d) ? (int)MSFEC_CELL_NOT_SPD : (int)MSFEC_CELL_NONFINITE;
+}
+
+template <typename R, int DIM, int T, int G>
+__device__ __forceinline__ void bk_superstep(const BandArgs& a, int cell, int j0,
+                                             typename Mf<R>::v4 (&Tl)[10],
+                                             typename Mf<R>::v4 (&Yt)[4],
+                                             typename Mf<R>::v4& Sacc, BkSmem<R>& ms, int& bad,
+                                             const R (&ek)[4]) {
+  typedef Mf<R> M;
+  const int lane = bk_lane(), lh = lane >> 4, ll = lane & 15, cs = M::cslot(ll);
+  constexpr int s0 = 16 * T + 4 * G;
+  bk_extract<R, T, G>(Tl, ms.piv);
+  ms.yp[lh * 16 + ll] = Yt[T][G];
+  wave_sync();
+  const R* pv = ms.piv;
+  // C C^T = pivot block (every lane); ci_t = 1 / C[t][t]
+  const R a00 = pv[0 * 72 + pad8(s0 + 0)];
+  bk_check(a00, bad);
+  const R ci0 = bk_rsqrt(a00);
+  const R c10 = pv[1 * 72 + pad8(s0 + 0)] * ci0, c20 = pv[2 * 72 + pad8(s0 + 0)] * ci0;
+  const R c30 = pv[3 * 72 + pad8(s0 + 0)] * ci0;
+  const R d1 = pv[1 * 72 + pad8(s0 + 1)] - c10 * c10;
+  bk_check(d1, bad);
+  const R ci1 = bk_rsqrt(d1);
+  const R c21 = (pv[2 * 72 + pad8(s0 + 1)] - c20 * c10) * ci1;
+  const R c31 = (pv[3 * 72 + pad8(s0 + 1)] - c30 * c10) * ci1;
+  const R d2 = pv[2 * 72 + pad8(s0 + 2)] - c20 * c20 - c21 * c21;
+  bk_check(d2, bad);
+  const R ci2 = bk_rsqrt(d2);
+  const R c32 = (pv[3 * 72 + pad8(s0 + 2)] - c30 * c20 - c31 * c21) * ci2;
+  const R d3 = pv[3 * 72 + pad8(s0 + 3)] - c30 * c30 - c31 * c31 - c32 * c32;
+  bk_check(d3, bad);
+  const R ci3 = bk_rsqrt(d3);
+  // row lh of C^-1: x C = e_lh solved from the last column backwards
+  const R x3 = ek[3] * ci3;
+  const R x2 = (ek[2] - x3 * c32) * ci2;
+  const R x1 = (ek[1] - x2 * c21 - x3 * c31) * ci1;
+  const R x0 = (ek[0] - x1 * c10 - x2 * c20 - x3 * c30) * ci0;
+  // panel in operand layout: lane (row slot 16P + cs, pivot lh)
+  R V[4], nV[4];
+#pragma unroll
+  for (int P = 0; P < 4; ++P) {
+    const int x = pad8(16 * P + cs);
+    V[P] = pv[0 * 72 + x] * x0 + pv[1 * 72 + x] * x1 + pv[2 * 72 + x] * x2 + pv[3 * 72 + x] * x3;
+    nV[P] = -V[P];
+  }
+  const R Z = ms.yp[0 * 16 + ll] * x0 + ms.yp[1 * 16 + ll] * x1 + ms.yp[2 * 16 + ll] * x2 +
+              ms.yp[3 * 16 + ll] * x3;
+#pragma unroll
+  for (int P = 0; P < 4; ++P)
+#pragma unroll
+    for (int Q = 0; Q <= P; ++Q) Tl[MF_TILE(P, Q)] = M::mma(nV[P], V[Q], Tl[MF_TILE(P, Q)]);
+#pragma unroll
+  for (int P = 0; P < 4; ++P) Yt[P] = M::mma(nV[P], Z, Yt[P]);
+  Sacc = M::mma(Z, Z, Sacc);
+  // panel [k][o], o = offset from the first pivot.  Offsets 0..3 hold C
+  // (lower part) with 1 / C[t][t] on the diagonal.
+  const int qs = j0 >> 2;
+  R* pst = reinterpret_cast<R*>(a.pan) + ((size_t)cell * a.nq + qs) * 256;
+  const R xd = (lh == 0) ? x0 : (lh == 1) ? x1 : (lh == 2) ? x2 : x3;  // C^-1[lh][lh]
+  const int jr = bk_opaque(j0);  // = s0 mod 64
+#pragma unroll
+  for (int P = 0; P < 4; ++P) {
+    const int o = (16 * P + cs - jr) & (kWin - 1);
+    pst[lh * 64 + o] = (P == T && o == lh) ? xd : V[P];
+  }
+  if (ll < 8) reinterpret_cast<R*>(a.zst)[((size_t)cell * a.nq + qs) * 32 + lh * 8 + ll] = Z;
+  wave_sync();
+}
+
+// The 4 super-steps of tile row T (pivots j0 .. j0+15); the next 16 rows of
+// the band enter the freed slots.
+template <typename R, int DIM, int T>
+__device__ __forceinline__ void bk_tile_row(const BandArgs& a, int cell, int j0, int my_off,
+                                            typename Mf<R>::v4 (&Tl)[10],
+                                            typename Mf<R>::v4 (&Yt)[4],
+                                            typename Mf<R>::v4& Sacc, BandSmem& sh,
+                                            BkSmem<R>& ms, RowPrefetch& pf, int& bad,
+                                            const R (&ek)[4]) {
+  const int n = a.n;
+  stage_group<DIM, true>(a, cell, j0 + kWin, sh, pf);
+  bk_superstep<R, DIM, T, 0>(a, cell, j0, Tl, Yt, Sacc, ms, bad, ek);
+  bk_insert<R, DIM, T, 0>(a, j0 + kWin, 0, my_off, Tl, Yt, sh, ms);
+  if (j0 + 4 < n) {
+    bk_superstep<R, DIM, T, 1>(a, cell, j0 + 4, Tl, Yt, Sacc, ms, bad, ek);
+    bk_insert<R, DIM, T, 1>(a, j0 + 4 + kWin, 4, my_off, Tl, Yt, sh, ms);
+  }
+  stage_group<DIM, true>(a, cell, j0 + 8 + kWin, sh, pf);
+  if (j0 + 8 < n) {
+    bk_superstep<R, DIM, T, 2>(a, cell, j0 + 8, Tl, Yt, Sacc, ms, bad, ek);
+    bk_insert<R, DIM, T, 2>(a, j0 + 8 + kWin, 0, my_off, Tl, Yt, sh, ms);
+  }
+  if (j0 + 12 < n) {
+    bk_superstep<R, DIM, T, 3>(a, cell, j0 + 12, Tl, Yt, Sacc, ms, bad, ek);
+    bk_insert<R, DIM, T, 3>(a, j0 + 12 + kWin, 4, my_off, Tl, Yt, sh, ms);
+  }
+}
+
+// Initial fill of tile row T with the rows 16T .. 16T+15.
+template <typename R, int DIM, int T>
+__device__ __forceinline__ void bk_fill_row(const BandArgs& a, int cell, int my_off,
+                                            typename Mf<R>::v4 (&Tl)[10],
+                                            typename Mf<R>::v4 (&Yt)[4], BandSmem& sh,
+                                            BkSmem<R>& ms, RowPrefetch& pf) {
+  stage_group<DIM, true>(a, cell, 16 * T, sh, pf);
+  bk_insert<R, DIM, T, 0>(a, 16 * T + 0, 0, my_off, Tl, Yt, sh, ms);
+  bk_insert<R, DIM, T, 1>(a, 16 * T + 4, 4, my_off, Tl, Yt, sh, ms);
+  stage_group<DIM, true>(a, cell, 16 * T + 8, sh, pf);
+  bk_insert<R, DIM, T, 2>(a, 16 * T + 8, 0, my_off, Tl, Yt, sh, ms);
+  bk_insert<R, DIM, T, 3>(a, 16 * T + 12, 4, my_off, Tl, Yt, sh, ms);
+}
+
+// Backward sweep.  Lane (k = lane >> 4, r = (lane >> 1) & 7, h = lane & 1)
+// sums the offsets o in [4 + 32h, 36 + 32h) of (V^T x)[k][r]; lanes 0..7
+// (r = lane) then apply C^-T.  Per super-step the panel, z and the 4 table
+// rows (lifting right-hand sides, interior load, node index) are prefetched
+// one step ahead.  MODE BK_FINAL64: Psi and the interior part of b;
+// BK_X0: the solution to X; BK_REFINE: the ring holds the correction,
+// x = X + correction goes to Psi and b, and lane (i,k) accumulates the Schur
+// term in its energy form S = R^T X + X^T r (r = Rres = R - K_II X), exact up
+// to the second order in the error of the first solution X:
+// S = R^T x* - e^T K_II e for X = x* + e.
+template <typename R, int DIM, int MODE>
+__device__ __forceinline__ void bk_backward(const BandArgs& a, int cell, BandSmem& sh,
+                                            BkSmem<R>& ms, double& bint, double& sacc,
+                                            double& dmax, double& xmax) {
+  using D = Dirs<DIM>;
+  const int lane = threadIdx.x;
+  const int k = lane >> 4, r = (lane >> 1) & 7, h = lane & 1;
+  const int n = a.n, nq = (n + 3) >> 2;
+  R* pb = ms.piv;                     // [4][68]
+  double* xr = SH_PART(sh);           // [8][132]
+  double* zb = xr + 8 * kRing;        // [4][8]
+  double* tb = zb + 32;               // [4][8]
+  double* rw = tb + 32;               // [4][24] table rows of the step
+  double* xb = rw + 96;               // [4][8] first solution X of the step (BK_REFINE)
+  double* r0b = xb + 32;              // [4][8] its residual (BK_REFINE)
+  for (int t = lane; t < 8 * kRing; t += 64) xr[t] = 0.0;
+  for (int t = lane; t < 4 * kPanRow; t += 64) pb[t] = (R)0;
+  const R* Pc = reinterpret_cast<const R*>(a.pan) + (size_t)cell * a.nq * 256;
+  const R* Zc = reinterpret_cast<const R*>(a.zst) + (size_t)cell * a.nq * 32;
+  const double* Rc = a.Rows + (size_t)cell * n * kRowStride;
+  // prefetch registers: 4 panel values, 1 z value, 2 table values per lane
+  R pp[4], zp = (R)0;
+  double rp0 = 0.0, rp1 = 0.0, xp = 0.0, qp = 0.0;
+  auto load = [&](int q) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) pp[u] = Pc[(size_t)q * 256 + lane * 4 + u];
+    if (lane < 32) zp = Zc[(size_t)q * 32 + lane];
+    const int lim = (n - 4 * q) * kRowStride;  // valid doubles of the 4 rows
+    rp0 = (lane < lim) ? Rc[(size_t)4 * q * kRowStride + lane] : 0.0;
+    rp1 = (lane + 64 < min(lim, 96)) ? Rc[(size_t)4 * q * kRowStride + 64 + lane] : 0.0;
+    if (MODE == BK_REFINE && lane < 32 && 4 * q + (lane >> 3) < n) {
+      xp = a.X[((size_t)cell * n + 4 * q) * kMaxNb + lane];
+      qp = a.Rres[((size_t)cell * n + 4 * q) * kMaxNb + lane];
+    } else if (MODE == BK_REFINE) {
+      xp = 0.0;
+      qp = 0.0;
+    }
+  };
+  load(nq - 1);
+  for (int q = nq - 1; q >= 0; --q) {
+    const int j0 = 4 * q;
+    wave_sync();
+#pragma unroll
+    for (int u = 0; u < 4; ++u) pb[(lane >> 4) * kPanRow + (lane & 15) * 4 + u] = pp[u];
+    if (lane < 32) zb[lane] = (double)zp;
+    rw[lane] = rp0;
+    if (lane < 32) rw[64 + lane] = rp1;
+    if (MODE == BK_REFINE && lane < 32) {
+      xb[lane] = xp;
+      r0b[lane] = qp;
+    }
+    wave_sync();
+    if (q > 0) load(q - 1);
+    if (MODE == BK_REFINE) {
+      // energy-form Schur term: lane (i = lane >> 3, kk = lane & 7)
+      const int i = lane >> 3, kk = lane & 7;
+#pragma unroll
+      for (int t = 0; t < 4; ++t)
+        sacc += rw[t * kRowStride + 16 + i] * xb[t * 8 + kk] + xb[t * 8 + i] * r0b[t * 8 + kk];
+    }
+    const int base = (j0 & (kWin - 1)) + 4 + 32 * h;
+    double part = 0.0;
+#pragma unroll
+    for (int m = 0; m < 32; ++m)
+      part = fma((double)pb[k * kPanRow + 4 + 32 * h + m], xr[r * kRing + base + m], part);
+    part += __shfl_xor(part, 1);
+    if (h == 0) tb[k * 8 + r] = part;
+    wave_sync();
+    if (lane < 8) {
+      const int rr = lane;
+      const double w0 = zb[0 * 8 + rr] - tb[0 * 8 + rr], w1 = zb[1 * 8 + rr] - tb[1 * 8 + rr];
+      const double w2 = zb[2 * 8 + rr] - tb[2 * 8 + rr], w3 = zb[3 * 8 + rr] - tb[3 * 8 + rr];
+      // C^T x = w with C (t,u) at pb[u * 68 + t] and 1 / C[t][t] at pb[t * 68 + t]
+      const double x3 = w3 * (double)pb[3 * kPanRow + 3];
+      const double x2 = (w2 - (double)pb[2 * kPanRow + 3] * x3) * (double)pb[2 * kPanRow + 2];
+      const double x1 = (w1 - (double)pb[1 * kPanRow + 2] * x2 - (double)pb[1 * kPanRow + 3] * x3) *
+                        (double)pb[1 * kPanRow + 1];
+      const double x0 = (w0 - (double)pb[0 * kPanRow + 1] * x1 - (double)pb[0 * kPanRow + 2] * x2 -
+                         (double)pb[0 * kPanRow + 3] * x3) *
+                        (double)pb[0 * kPanRow + 0];
+      const double xs[4] = {x0, x1, x2, x3};
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const int j = j0 + t;
+        if (j >= n) continue;
+        const int p = j & (kWin - 1);
+        xr[rr * kRing + p] = xs[t];
+        xr[rr * kRing + p + 64] = xs[t];
+        if (MODE == BK_X0) {
+          a.X[((size_t)cell * n + j) * kMaxNb + rr] = xs[t];
+        } else {
+          const double xf = (MODE == BK_REFINE) ? xb[t * 8 + rr] + xs[t] : xs[t];
+          if (MODE == BK_REFINE) {
+            a.X[((size_t)cell * n + j) * kMaxNb + rr] = xf;  // for the next pass
+            dmax = fmax(dmax, fabs(xs[t]));
+            xmax = fmax(xmax, fabs(xf));
+          }
+          const int node = (int)rw[t * kRowStride + 15];
+          if (rr < D::NV) a.Psi[((size_t)cell * D::NV + rr) * a.nn + node] = xf;
+          bint = fma(xf, rw[t * kRowStride + 14], bint);
+        }
+      }
+    }
+  }
+  wave_sync();
+}
+
+// Sacc (accumulator layout) -> lane (i = lane >> 3, k = lane & 7)
+template <typename R>
+__device__ __forceinline__ double bk_sacc(const typename Mf<R>::v4& Sacc, double* buf) {
+  const int lane = threadIdx.x;
+#pragma unroll
+  for (int g = 0; g < 4; ++g) {
+    const int i = Mf<R>::acc_row(g, lane >> 4), j = lane & 15;
+    if (i < 8 && j < 8) buf[i * 8 + j] = (double)Sacc[g];
+  }
+  wave_sync();
+  const double v = buf[lane];
+  wave_sync();
+  return v;
+}
+
+// Prologue shared by the fast-path kernels: coefficient, reaction and load of
+// the cell to LDS, fine-cell size, element tables.
+template <int DIM>
+__device__ __forceinline__ double bk_prologue(const BandArgs& a, int cell, BandSmem& sh,
+                                              double* dyn) {
+  using D = Dirs<DIM>;
+  const int lane = threadIdx.x;
+  const double* cg = a.coef + (size_t)cell * a.nfc;
+  for (int z = lane; z < a.nfc; z += 64) dyn[z] = cg[z];
+  if (a.reaction_mode == MSFEC_REACTION_CELL) {
+    const double* rg = a.reac + (size_t)cell * a.nfc;
+    for (int z = lane; z < a.nfc; z += 64) dyn[a.nfc + z] = rg[z];
+  }
+  if (a.rhs_mode == MSFEC_RHS_CELL) {
+    const double* fg = a.fcell + (size_t)cell * a.nfc;
+    double* fd = dyn + a.nfc * ((a.reaction_mode == MSFEC_REACTION_CELL) ? 2 : 1);
+    for (int z = lane; z < a.nfc; z += 64) fd[z] = fg[z];
+  }
+  const double* v = a.verts + (size_t)cell * D::NV * DIM;
+  double hh[3] = {1.0, 1.0, 1.0};
+  double vol = 1.0;
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) {
+    hh[d] = (v[(D::NV - 1) * DIM + d] - v[d]) / a.N;
+    vol *= hh[d];
+  }
+  if (lane < 8) {
+    double ks, msv;
+    box_tables<DIM>(lane, hh, ks, msv);
+    sh.kv[lane] = ks;
+    sh.mv[lane] = msv;
+  }
+  wave_sync();
+  return vol;
+}
+
+// Factor + forward elimination of the lifting right-hand sides, then the
+// backward sweep.  MODE BK_FINAL64 (FP64 tiles): Psi, M, b of the cell;
+// MODE BK_X0 (FP32 tiles): the first solution X of the mixed solver.
+// waves per SIMD the register budget is sized for: FP64 tiles need about
+// 200 VGPRs, FP32 tiles half of that
+template <typename R>
+struct BkOcc {
+  static constexpr int waves = sizeof(R) == 8 ? 2 : 3;
+};
+
+template <typename R, int DIM, int MODE>
+__global__ __launch_bounds__(64, BkOcc<R>::waves) void q_blk_factor_kernel(BandArgs a) {
+  typedef typename Mf<R>::v4 V4;
+  const int cell = blockIdx.x;
+  const int lane = threadIdx.x;
+  __shared__ BandSmem sh;
+  __shared__ BkSmem<R> ms;
+  extern __shared__ double dyn[];  // coefficient | reaction | load
+  const double* cf = dyn;
+  const double* rf = dyn + a.nfc;
+  const double* ff = dyn + a.nfc * ((a.reaction_mode == MSFEC_REACTION_CELL) ? 2 : 1);
+  double vol = 1.0;
+  if (MODE == BK_FINAL64) vol = bk_prologue<DIM>(a, cell, sh, dyn);
+  int my_off = 0;
+#pragma unroll
+  for (int t = 0; t < kMaxLow; ++t)
+    if ((lane & 15) == t) my_off = a.off_int[t];
+
+  V4 Tl[10], Yt[4], Sacc = {0, 0, 0, 0};
+#pragma unroll
+  for (int t = 0; t < 10; ++t) Tl[t] = (V4){0, 0, 0, 0};
+#pragma unroll
+  for (int P = 0; P < 4; ++P) Yt[P] = (V4){0, 0, 0, 0};
+  R ek[4];  // unit vector e_(lane >> 4)
+#pragma unroll
+  for (int t = 0; t < 4; ++t) ek[t] = ((lane >> 4) == t) ? (R)1 : (R)0;
+  RowPrefetch pf;
+  rows_load(a, cell, 0, pf);
+  bk_fill_row<R, DIM, 0>(a, cell, my_off, Tl, Yt, sh, ms, pf);
+  bk_fill_row<R, DIM, 1>(a, cell, my_off, Tl, Yt, sh, ms, pf);
+  bk_fill_row<R, DIM, 2>(a, cell, my_off, Tl, Yt, sh, ms, pf);
+  bk_fill_row<R, DIM, 3>(a, cell, my_off, Tl, Yt, sh, ms, pf);
+  int bad = 0;
+  const int n = a.n;
+  for (int j0 = 0; j0 < n; j0 += 64) {
+    bk_tile_row<R, DIM, 0>(a, cell, j0, my_off, Tl, Yt, Sacc, sh, ms, pf, bad, ek);
+    if (j0 + 16 < n) bk_tile_row<R, DIM, 1>(a, cell, j0 + 16, my_off, Tl, Yt, Sacc, sh, ms, pf, bad, ek);
+    if (j0 + 32 < n) bk_tile_row<R, DIM, 2>(a, cell, j0 + 32, my_off, Tl, Yt, Sacc, sh, ms, pf, bad, ek);
+    if (j0 + 48 < n) bk_tile_row<R, DIM, 3>(a, cell, j0 + 48, my_off, Tl, Yt, Sacc, sh, ms, pf, bad, ek);
+  }
+  double sacc = (MODE == BK_FINAL64) ? bk_sacc<R>(Sacc, SH_PART(sh)) : 0.0;
+  // panels of this wave must be visible to its own loads in the backward sweep
+  __threadfence_block();
+  __syncthreads();
+  double bint = 0.0, dmax = 0.0, xmax = 0.0;
+  bk_backward<R, DIM, MODE>(a, cell, sh, ms, bint, sacc, dmax, xmax);
+  int worst = bad;
+  for (int o = 32; o > 0; o >>= 1) worst = max(worst, __shfl_xor(worst, o));
+  if (lane == 0) {
+    a.status[cell] = worst;
+    if (MODE == BK_X0) a.conv[cell] = (worst != 0) ? 1 : 0;  // failed cells do not refine
+  }
+  if (MODE == BK_FINAL64) fast_epilogue<DIM>(a, cell, sh, cf, rf, ff, vol, sacc, bint);
+}
+
+// ---------------------------------------------------------------------------
+// Mixed-precision solver, refinement residual: Rres = R - K_II X in FP64,
+// one thread per (cell, interior row, right-hand side).  K_II comes from the
+// row table: lower entries of row j, upper entries as the lower entries of
+// the rows j + off (zero where the neighbour is not interior).
+// ---------------------------------------------------------------------------
+template <int DIM>
+__global__ __launch_bounds__(256) void q_resid_kernel(BandArgs a, int n_cells) {
+  using D = Dirs<DIM>;
+  const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = a.n;
+  if (gid >= (long long)n_cells * n * kMaxNb) return;
+  const int r = (int)(gid & (kMaxNb - 1));
+  const long long cj = gid >> 3;
+  const int cell = (int)(cj / n), j = (int)(cj - (long long)cell * n);
+  if (a.conv[cell]) return;
+  const double* rows = a.Rows + (size_t)cell * n * kRowStride;
+  const double* X = a.X + (size_t)cell * n * kMaxNb;
+  const double* rj = rows + (size_t)j * kRowStride;
+  double acc = rj[16 + r] - rj[0] * X[(size_t)j * kMaxNb + r];
+#pragma unroll
+  for (int t = 1; t < D::NLOW; ++t) {
+    const int off = a.off_int[t];
+    const double lo = rj[t];
+    if (lo != 0.0 && j >= off) acc = fma(-lo, X[(size_t)(j - off) * kMaxNb + r], acc);
+    if (j + off < n) {
+      const double up = rows[(size_t)(j + off) * kRowStride + t];
+      if (up != 0.0) acc = fma(-up, X[(size_t)(j + off) * kMaxNb + r], acc);
+    }
+  }
+  a.Rres[((size_t)cell * n + j) * kMaxNb + r] = acc;
+}
+
+// ---------------------------------------------------------------------------
+// Mixed-precision solver, refinement sweep: forward elimination of Rres with
+// the stored FP32 panels (4 MFMAs per super-step), backward sweep of the
+// correction, x = X + correction, then the element matrix and Psi.
+// ---------------------------------------------------------------------------
+template <int DIM, int T, int G>
+__device__ __forceinline__ void rf_step(const BandArgs& a, int cell, int j0,
+                                        typename Mf<float>::v4 (&Yt)[4], BkSmem<float>& ms,
+                                        float (&pp)[4]) {
+  typedef Mf<float> M;
+  const int lane = threadIdx.x, lh = lane >> 4, ll = lane & 15, cs = M::cslot(ll);
+  constexpr int s0 = 16 * T + 4 * G;
+  const int n = a.n;
+  float* pb = ms.piv;  // [4][68]
+  const int qs = j0 >> 2;
+  // panel of this super-step (prefetched) and the pivot right-hand sides to LDS
+#pragma unroll
+  for (int u = 0; u < 4; ++u) pb[lh * kPanRow + ll * 4 + u] = pp[u];
+  ms.yp[lh * 16 + ll] = Yt[T][G];
+  wave_sync();
+  if (j0 + 4 < n) {
+    const float* Pn = reinterpret_cast<const float*>(a.pan) + ((size_t)cell * a.nq + qs + 1) * 256;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) pp[u] = Pn[lane * 4 + u];
+  }
+  // Z = C^-1 y_pivot by forward substitution (C (t,u) at pb[u * 68 + t],
+  // 1 / C[t][t] at pb[t * 68 + t]); lane (k = lh, rhs ll) keeps row lh
+  const float y0 = ms.yp[0 * 16 + ll], y1 = ms.yp[1 * 16 + ll];
+  const float y2 = ms.yp[2 * 16 + ll], y3 = ms.yp[3 * 16 + ll];
+  const float z0 = y0 * pb[0 * kPanRow + 0];
+  const float z1 = (y1 - pb[0 * kPanRow + 1] * z0) * pb[1 * kPanRow + 1];
+  const float z2 = (y2 - pb[0 * kPanRow + 2] * z0 - pb[1 * kPanRow + 2] * z1) * pb[2 * kPanRow + 2];
+  const float z3 = (y3 - pb[0 * kPanRow + 3] * z0 - pb[1 * kPanRow + 3] * z1 -
+                    pb[2 * kPanRow + 3] * z2) *
+                   pb[3 * kPanRow + 3];
+  const float Z = (lh == 0) ? z0 : (lh == 1) ? z1 : (lh == 2) ? z2 : z3;
+  float nV[4];
+  const int jr = bk_opaque(j0);  // = s0 mod 64
+#pragma unroll
+  for (int P = 0; P < 4; ++P) nV[P] = -pb[lh * kPanRow + ((16 * P + cs - jr) & (kWin - 1))];
+#pragma unroll
+  for (int P = 0; P < 4; ++P) Yt[P] = M::mma(nV[P], Z, Yt[P]);
+  if (ll < 8) reinterpret_cast<float*>(a.zst)[((size_t)cell * a.nq + qs) * 32 + lh * 8 + ll] = Z;
+  // next rows of the band enter the pivot slots
+  const int jn = j0 + kWin + lh;
+  const float rv = (float)a.Rres[((size_t)cell * n + min(jn, n - 1)) * kMaxNb + (ll & 7)];
+  Yt[T][G] = (ll < 8 && jn < n) ? rv : 0.0f;
+  wave_sync();
+}
+
+template <int DIM, int T>
+__device__ __forceinline__ void rf_tile_row(const BandArgs& a, int cell, int j0,
+                                            typename Mf<float>::v4 (&Yt)[4], BkSmem<float>& ms,
+                                            float (&pp)[4]) {
+  const int n = a.n;
+  rf_step<DIM, T, 0>(a, cell, j0, Yt, ms, pp);
+  if (j0 + 4 < n) rf_step<DIM, T, 1>(a, cell, j0 + 4, Yt, ms, pp);
+  if (j0 + 8 < n) rf_step<DIM, T, 2>(a, cell, j0 + 8, Yt, ms, pp);
+  if (j0 + 12 < n) rf_step<DIM, T, 3>(a, cell, j0 + 12, Yt, ms, pp);
+}
+
+template <int DIM>
+__global__ __launch_bounds__(64, 2) void q_blk_refine_kernel(BandArgs a) {
+  typedef typename Mf<float>::v4 V4;
+  const int cell = blockIdx.x;
+  const int lane = threadIdx.x, lh = lane >> 4, ll = lane & 15;
+  if (a.conv[cell]) return;  // converged or failed: the whole wave leaves
+  __shared__ BandSmem sh;
+  __shared__ BkSmem<float> ms;
+  extern __shared__ double dyn[];
+  const double* cf = dyn;
+  const double* rf = dyn + a.nfc;
+  const double* ff = dyn + a.nfc * ((a.reaction_mode == MSFEC_REACTION_CELL) ? 2 : 1);
+  const double vol = bk_prologue<DIM>(a, cell, sh, dyn);
+  const int n = a.n;
+  V4 Yt[4];
+  // rows 0..63 of the residual: slot 4g + lh of tile row P is register g
+#pragma unroll
+  for (int P = 0; P < 4; ++P)
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+      const int j = 16 * P + 4 * g + lh;
+      const float rv = (float)a.Rres[((size_t)cell * n + min(j, n - 1)) * kMaxNb + (ll & 7)];
+      Yt[P][g] = (ll < 8 && j < n) ? rv : 0.0f;
+    }
+  float pp[4];
+  {
+    const float* Pn = reinterpret_cast<const float*>(a.pan) + (size_t)cell * a.nq * 256;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) pp[u] = Pn[lane * 4 + u];
+  }
+  for (int t = lane; t < 4 * kPanRow; t += 64) ms.piv[t] = 0.0f;
+  wave_sync();
+  for (int j0 = 0; j0 < n; j0 += 64) {
+    rf_tile_row<DIM, 0>(a, cell, j0, Yt, ms, pp);
+    if (j0 + 16 < n) rf_tile_row<DIM, 1>(a, cell, j0 + 16, Yt, ms, pp);
+    if (j0 + 32 < n) rf_tile_row<DIM, 2>(a, cell, j0 + 32, Yt, ms, pp);
+    if (j0 + 48 < n) rf_tile_row<DIM, 3>(a, cell, j0 + 48, Yt, ms, pp);
+  }
+  __threadfence_block();
+  __syncthreads();
+  double bint = 0.0, sacc = 0.0, dmax = 0.0, xmax = 0.0;
+  bk_backward<float, DIM, BK_REFINE>(a, cell, sh, ms, bint, sacc, dmax, xmax);
+  for (int o = 32; o > 0; o >>= 1) {
+    dmax = fmax(dmax, __shfl_xor(dmax, o));
+    xmax = fmax(xmax, __shfl_xor(xmax, o));
+  }
+  // The error left after this step is about (|dx| / |x|)^2 |x| (the
+  // contraction of the refinement is about |dx| / |x| itself).
+  const double q2 = (dmax * dmax) / fmax(xmax * xmax, 1e-300);
+  if (lane == 0) {
+    if (q2 <= 1e-13) {
+      a.conv[cell] = 1;
+    } else if (a.last_pass) {
+      a.conv[cell] = 1;
+      if (q2 > 1e-10 && a.status[cell] == MSFEC_CELL_OK) a.status[cell] = MSFEC_CELL_NOT_CONVERGED;
+    }
+  }
+  fast_epilogue<DIM>(a, cell, sh, cf, rf, ff, vol, sacc, bint);
+}
+
+// ---------------------------------------------------------------------------
 // K3: Psi, element matrix and element right-hand side
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void q_element_kernel(QConf c, const double* __restrict__ Slo,
@@ -1706,6 +2386,8 @@ struct msfec_q_batch {
   double* d_f = nullptr;
   double* d_reac = nullptr;
   double* d_rows = nullptr;
+  double* d_rres = nullptr;
+  int* d_conv = nullptr;
   double* d_slo = nullptr;
   double* d_F = nullptr;
   double* d_rhs = nullptr;
@@ -1723,7 +2405,7 @@ struct msfec_q_batch {
   hipStream_t stream = nullptr;
   // per-compute kernel events (MSFEC_FLAG_TIMING): sets recorded since the
   // last msfec_q_kernel_times call, reused across calls
-  std::vector<std::array<hipEvent_t, 4>> ev_sets;
+  std::vector<std::array<hipEvent_t, 6>> ev_sets;
   size_t ev_used = 0;
   // coarse scratch
   double* d_cK = nullptr;
@@ -1747,7 +2429,7 @@ msfec_status dalloc(msfec_q_batch* h, T** p, size_t count) {
 }
 
 void free_all(msfec_q_batch* h) {
-  void* ptrs[] = {h->d_rows,  h->d_verts, h->d_coef, h->d_f,     h->d_reac,  h->d_slo,   h->d_F,
+  void* ptrs[] = {h->d_conv,  h->d_rres,  h->d_rows,  h->d_verts, h->d_coef, h->d_f,     h->d_reac,  h->d_slo,   h->d_F,
                   h->d_rhs,   h->d_L,    h->d_W,     h->d_X,     h->d_psi,   h->d_M,
                   h->d_b,     h->d_status, h->d_inode, h->d_imask, h->d_cK,  h->d_crhs,
                   h->d_cvec,  h->d_cit};
@@ -1794,8 +2476,16 @@ msfec_status msfec_q_create(const msfec_q_params* params, int n_cells, const dou
     return fail(MSFEC_ERR_INVALID, "unknown rhs_mode");
   if (p.reaction_mode < MSFEC_REACTION_NONE || p.reaction_mode > MSFEC_REACTION_CELL)
     return fail(MSFEC_ERR_INVALID, "unknown reaction_mode");
-  if (p.solver != MSFEC_SOLVER_BAND_FP64 && p.solver != MSFEC_SOLVER_BAND_FP64_MFMA)
+  if (p.solver != MSFEC_SOLVER_BAND_FP64 && p.solver != MSFEC_SOLVER_BAND_FP64_MFMA &&
+      p.solver != MSFEC_SOLVER_BAND_MIXED)
     return fail(MSFEC_ERR_INVALID, "unknown solver");
+  if (p.max_refine < 0 || p.max_refine > 16)
+    return fail(MSFEC_ERR_INVALID, "max_refine must be in 0..16");
+  if (p.solver == MSFEC_SOLVER_BAND_MIXED &&
+      (p.coef_mode != MSFEC_COEF_SCALAR_CELL || p.rhs_mode == MSFEC_RHS_QP))
+    return fail(MSFEC_ERR_UNSUPPORTED,
+                "the mixed-precision solver needs a scalar coefficient per fine cell and a "
+                "constant or per-cell load");
   const int N = 1 << p.n_refine_local;
   const int ni = N - 1;
   const int bw = (p.dim == 2) ? ni + 1 : ni * ni + ni + 1;
@@ -1892,6 +2582,10 @@ msfec_status msfec_q_create(const msfec_q_params* params, int n_cells, const dou
   TRY(dalloc(h, &h->d_f, h->rhs_count));
   TRY(dalloc(h, &h->d_reac, h->reac_count));
   if (h->fast) TRY(dalloc(h, &h->d_rows, (size_t)n_cells * c.n * kRowStride));
+  if (p.solver == MSFEC_SOLVER_BAND_MIXED) {
+    TRY(dalloc(h, &h->d_rres, (size_t)n_cells * c.n * kMaxNb));
+    TRY(dalloc(h, &h->d_conv, (size_t)n_cells));
+  }
   if (!h->fast) {
     TRY(dalloc(h, &h->d_slo, (size_t)n_cells * c.nn * c.nlow));
     TRY(dalloc(h, &h->d_F, (size_t)n_cells * c.nn));
@@ -2011,7 +2705,7 @@ msfec_status msfec_q_compute(msfec_q_handle h, void* stream) {
   if (tm) {
     if (h->ev_used == h->ev_sets.size()) {
       if (h->ev_sets.size() >= 4096) return fail(MSFEC_ERR_STATE, "too many timed computes pending");
-      std::array<hipEvent_t, 4> set{};
+      std::array<hipEvent_t, 6> set{};
       for (auto& e : set) MSFEC_HIP(hipEventCreate(&e));
       h->ev_sets.push_back(set);
     }
@@ -2044,6 +2738,11 @@ msfec_status msfec_q_compute(msfec_q_handle h, void* stream) {
   a.reaction_const = c.reaction_const;
   a.rhs_const = c.rhs_const;
   a.Lcol = h->d_L;
+  a.pan = h->d_L;
+  a.zst = h->d_W;
+  a.Rres = h->d_rres;
+  a.conv = h->d_conv;
+  a.last_pass = 0;
   a.Wf = h->d_W;
   a.X = h->d_X;
   a.Psi = h->d_psi;
@@ -2052,8 +2751,10 @@ msfec_status msfec_q_compute(msfec_q_handle h, void* stream) {
   a.status = h->d_status;
   a.nq = (c.n + 3) / 4;
   if (fast) {
-    const size_t rdyn =
-        (size_t)c.nfc * (1 + (c.reaction_mode == MSFEC_REACTION_CELL ? 1 : 0)) * sizeof(double);
+    const size_t rdyn = (size_t)c.nfc *
+                        (1 + (c.reaction_mode == MSFEC_REACTION_CELL ? 1 : 0) +
+                         (c.rhs_mode == MSFEC_RHS_CELL ? 1 : 0)) *
+                        sizeof(double);
     if (c.dim == 3)
       hipLaunchKernelGGL((q_rows_kernel<3>), dim3(h->n_cells), dim3(256), rdyn, s, a);
     else
@@ -2068,13 +2769,42 @@ msfec_status msfec_q_compute(msfec_q_handle h, void* stream) {
   }
   if (tm) MSFEC_HIP(hipEventRecord(ev[1], s));
   const bool mfma = (h->p.solver == MSFEC_SOLVER_BAND_FP64_MFMA);
+  const bool mixed = (h->p.solver == MSFEC_SOLVER_BAND_MIXED);
   const size_t dyn = fast ? (size_t)c.nfc *
                                 (1 + (c.reaction_mode == MSFEC_REACTION_CELL ? 1 : 0) +
                                  (c.rhs_mode == MSFEC_RHS_CELL ? 1 : 0)) *
                                 sizeof(double)
                           : 0;
 #define LAUNCH_BAND(K, D, F) hipLaunchKernelGGL((K<D, F>), dim3(h->n_cells), dim3(64), dyn, s, a)
-  if (mfma) {
+  if (mixed) {
+    // FP32 factor and first solution, FP64 residual, FP32 correction sweep
+    const long long nth = (long long)h->n_cells * c.n * kMaxNb;
+    const unsigned nblk = (unsigned)((nth + 255) / 256);
+    if (c.dim == 3)
+      hipLaunchKernelGGL((q_blk_factor_kernel<float, 3, BK_X0>), dim3(h->n_cells), dim3(64), 0, s, a);
+    else
+      hipLaunchKernelGGL((q_blk_factor_kernel<float, 2, BK_X0>), dim3(h->n_cells), dim3(64), 0, s, a);
+    if (tm) MSFEC_HIP(hipEventRecord(ev[2], s));
+    // refinement passes; cells that have converged leave each kernel at once
+    const int passes = (h->p.max_refine > 0) ? h->p.max_refine : 3;
+    for (int pass = 1; pass <= passes; ++pass) {
+      a.last_pass = (pass == passes) ? 1 : 0;
+      if (c.dim == 3)
+        hipLaunchKernelGGL((q_resid_kernel<3>), dim3(nblk), dim3(256), 0, s, a, h->n_cells);
+      else
+        hipLaunchKernelGGL((q_resid_kernel<2>), dim3(nblk), dim3(256), 0, s, a, h->n_cells);
+      if (tm && pass == 1) MSFEC_HIP(hipEventRecord(ev[3], s));
+      if (c.dim == 3)
+        hipLaunchKernelGGL((q_blk_refine_kernel<3>), dim3(h->n_cells), dim3(64), dyn, s, a);
+      else
+        hipLaunchKernelGGL((q_blk_refine_kernel<2>), dim3(h->n_cells), dim3(64), dyn, s, a);
+    }
+  } else if (mfma && fast) {
+    if (c.dim == 3)
+      hipLaunchKernelGGL((q_blk_factor_kernel<double, 3, BK_FINAL64>), dim3(h->n_cells), dim3(64), dyn, s, a);
+    else
+      hipLaunchKernelGGL((q_blk_factor_kernel<double, 2, BK_FINAL64>), dim3(h->n_cells), dim3(64), dyn, s, a);
+  } else if (mfma) {
     if (c.dim == 3) {
       if (fast) LAUNCH_BAND(q_band_mfma_kernel, 3, true);
       else LAUNCH_BAND(q_band_mfma_kernel, 3, false);
@@ -2093,7 +2823,11 @@ msfec_status msfec_q_compute(msfec_q_handle h, void* stream) {
   }
 #undef LAUNCH_BAND
   MSFEC_HIP(hipGetLastError());
-  if (tm) MSFEC_HIP(hipEventRecord(ev[2], s));
+  if (tm && !mixed) {
+    MSFEC_HIP(hipEventRecord(ev[2], s));
+    MSFEC_HIP(hipEventRecord(ev[3], s));
+  }
+  if (tm) MSFEC_HIP(hipEventRecord(ev[4], s));
   if (!fast) {
     const size_t shm = (size_t)(2 * c.nv * c.nn + 4 * 72) * sizeof(double);
     if (shm > 160 * 1024) return fail(MSFEC_ERR_UNSUPPORTED, "element kernel LDS too large");
@@ -2101,7 +2835,7 @@ msfec_status msfec_q_compute(msfec_q_handle h, void* stream) {
                        h->d_F, h->d_X, h->d_psi, h->d_M, h->d_b);
     MSFEC_HIP(hipGetLastError());
   }
-  if (tm) MSFEC_HIP(hipEventRecord(ev[3], s));
+  if (tm) MSFEC_HIP(hipEventRecord(ev[5], s));
   h->computed = true;
   return MSFEC_OK;
 }
@@ -2115,17 +2849,17 @@ msfec_status msfec_q_synchronize(msfec_q_handle h) {
 msfec_status msfec_q_kernel_times(msfec_q_handle h, float* ms, int max, int* n) {
   if (!h || !ms) return fail(MSFEC_ERR_INVALID, "null argument");
   if (h->ev_used == 0) return fail(MSFEC_ERR_STATE, "no timed compute (set MSFEC_FLAG_TIMING)");
-  double acc[3] = {0.0, 0.0, 0.0};
+  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
   for (size_t c = 0; c < h->ev_used; ++c) {
-    MSFEC_HIP(hipEventSynchronize(h->ev_sets[c][3]));
-    for (int i = 0; i < 3; ++i) {
+    MSFEC_HIP(hipEventSynchronize(h->ev_sets[c][5]));
+    for (int i = 0; i < 5; ++i) {
       float t = 0.f;
       MSFEC_HIP(hipEventElapsedTime(&t, h->ev_sets[c][i], h->ev_sets[c][i + 1]));
       acc[i] += t;
     }
   }
   int k = 0;
-  for (; k < 3 && k < max; ++k) ms[k] = (float)(acc[k] / (double)h->ev_used);
+  for (; k < 5 && k < max; ++k) ms[k] = (float)(acc[k] / (double)h->ev_used);
   if (n) *n = k;
   h->ev_used = 0;
   return MSFEC_OK;

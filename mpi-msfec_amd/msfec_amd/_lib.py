@@ -19,8 +19,8 @@ HOST, DEVICE = 0, 1
 COEF_SCALAR_CELL, COEF_TENSOR_QP = 0, 1
 RHS_CONST, RHS_CELL, RHS_QP = 0, 1, 2
 REACTION_NONE, REACTION_CONST, REACTION_CELL = 0, 1, 2
-SOLVER_BAND_FP64, SOLVER_BAND_FP64_MFMA = 0, 1
-CELL_OK, CELL_NOT_SPD, CELL_NONFINITE = 0, 2, 3
+SOLVER_BAND_FP64, SOLVER_BAND_FP64_MFMA, SOLVER_BAND_MIXED = 0, 1, 2
+CELL_OK, CELL_NOT_CONVERGED, CELL_NOT_SPD, CELL_NONFINITE = 0, 1, 2, 3
 FLAG_TIMING = 1
 
 #: every symbol include/msfec.h declares
@@ -44,6 +44,7 @@ class Params(ctypes.Structure):
         ("reaction_const", ctypes.c_double),
         ("solver", ctypes.c_int),
         ("flags", ctypes.c_int),
+        ("max_refine", ctypes.c_int),
     ]
 
 

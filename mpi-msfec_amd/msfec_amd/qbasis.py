@@ -32,13 +32,14 @@ def _ptr(a):
 class QBasisBatch:
     def __init__(self, vertices, dim=3, n_refine_local=3, coef_mode=L.COEF_SCALAR_CELL,
                  rhs_mode=L.RHS_CONST, rhs_const=1.0, reaction_mode=L.REACTION_NONE,
-                 reaction_const=0.0, solver=L.SOLVER_BAND_FP64, timing=False):
+                 reaction_const=0.0, solver=L.SOLVER_BAND_FP64, timing=False, max_refine=0):
         self._lib = L.load()
         self._h = ctypes.c_void_p()
         p = L.Params()
         p.dim, p.n_refine_local, p.coef_mode, p.rhs_mode = dim, n_refine_local, coef_mode, rhs_mode
         p.rhs_const, p.reaction_mode, p.reaction_const = rhs_const, reaction_mode, reaction_const
         p.solver, p.flags = solver, (L.FLAG_TIMING if timing else 0)
+        p.max_refine = max_refine
         vptr, vloc, keep = _ptr(vertices)
         nv = 1 << dim
         n_cells = int(np.prod(vertices.shape)) // (nv * dim)

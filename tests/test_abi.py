@@ -83,6 +83,13 @@ def test_rejects_unknown_modes(lib):
     assert st == lib.ERR_INVALID
 
 
+def test_mixed_solver_needs_scalar_coefficient(lib):
+    """The mixed-precision solver works on the scalar-coefficient row table;
+    a tensor coefficient is refused before any device work."""
+    st, msg = _create(lib, coef_mode=1, solver=lib.SOLVER_BAND_MIXED)
+    assert st == lib.ERR_UNSUPPORTED and "mixed" in msg
+
+
 def test_null_arguments(lib):
     L = lib.load()
     assert L.msfec_q_create(None, 1, None, 0, None) == lib.ERR_INVALID
