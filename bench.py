@@ -30,16 +30,18 @@ DIM, NREF, NC = 3, 3, 16
 SIGMA, CORR, SEED = 1.5, 1.0 / 64, 20261202
 FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 (vector and matrix)
 FP32_MFMA_PEAK_TFLOPS = 157.3  # MI355X FP32 matrix cores, dense
-# local solvers: name -> (solver id, dominant kernel, its arithmetic); HEADLINE
-# is the value line, the others are timed beside it
-SOLVERS = {"band-mixed": (2, "q_blk_factor_kernel<float>", "f32"),
-           "band-fp64-valu": (0, "q_band_kernel", "f64"),
-           "band-fp64-mfma": (1, "q_blk_factor_kernel<double>", "f64")}
+# local solvers: name -> (solver id, dominant kernel, its arithmetic).  The
+# value line is HEADLINE, the fastest path that meets the 1e-10 bound on this
+# workload; the others, the mixed-precision one among them, are timed beside
+# it and reported with their ratio to it.
+SOLVERS = {"band-fp64-mfma": (1, "q_blk_factor_kernel<double>", "f64"),
+           "band-mixed": (2, "q_blk_factor_kernel<float>", "f32"),
+           "band-fp64-valu": (0, "q_band_kernel", "f64")}
 SOLVER_TEXT = {"band-mixed": "banded Cholesky FP32 on the matrix cores + iterative refinement "
-                             "with FP64 residuals until (|dx|/|x|)^2 <= 1e-13 per cell",
+                             "with FP64 residuals until (|dx|/|x|)^2 <= 1e-11 per cell",
                "band-fp64-valu": "banded Cholesky FP64 on the vector ALUs",
                "band-fp64-mfma": "banded Cholesky FP64 on the matrix cores"}
-HEADLINE = "band-mixed"
+HEADLINE = "band-fp64-mfma"
 TRAFFIC_FILE = os.path.join(REPO, "profiles", "hbm_traffic.json")
 HBM_PEAK_GBS = 8000.0     # MI355X HBM3E
 
@@ -247,7 +249,12 @@ def main():
                                  "copy-calibrated, profiles/hbm_traffic.json)",
                  "dense_equiv_tflops": round(dense_flops_per_cell(n, 8) * info.n_cells /
                                              (band_ms * 1e-3) / 1e12, 3)})
-    fp64_best = max((v["cells_per_s"] for k, v in others.items() if "fp64" in k), default=None)
+    # mixed precision against the fastest FP64 path (the headline or a path beside it)
+    rates = {k: v["cells_per_s"] for k, v in others.items()}
+    rates[HEADLINE] = cells_per_s
+    fp64_best = max((v for k, v in rates.items() if "fp64" in k), default=None)
+    mixed_vs_fp64 = (round(rates["band-mixed"] / fp64_best, 3)
+                     if fp64_best and "band-mixed" in rates else None)
 
     out = {
         "metric": "basis cells/sec, 3D Q 8^3 fine/cell: mixed vs FP64 GPU at ≤1e-10 rel err",
@@ -268,7 +275,7 @@ def main():
                    "n_cells": info.n_cells, "n_refine_local": NREF,
                    "solver": HEADLINE, "parallelism": "single GPU"},
         "other_solvers": others,
-        "speedup_vs_fp64": round(cells_per_s / fp64_best, 3) if fp64_best else None,
+        "mixed_vs_fp64": mixed_vs_fp64,
         "kernel_ms": {"rows": round(kt[0], 4), "factor_solve": round(kt[1], 4),
                       "refine_residual": round(kt[2], 4), "refine_sweep": round(kt[3], 4),
                       "element": round(kt[4], 4)},

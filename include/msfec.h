@@ -61,7 +61,7 @@ enum { MSFEC_REACTION_NONE = 0, MSFEC_REACTION_CONST = 1, MSFEC_REACTION_CELL = 
  *   BAND_FP64_MFMA FP64, four pivots per step with FP64 matrix-core updates;
  *   BAND_MIXED     FP32 factor on the matrix cores and iterative refinement
  *                  with the residual in FP64: a cell stops refining once the
- *                  predicted error (|dx| / |x|)^2 is below 1e-13, after at
+ *                  predicted error (|dx| / |x|)^2 is below 1e-11, after at
  *                  most max_refine steps (scalar coefficient per fine cell and
  *                  constant or per-cell load only). */
 enum { MSFEC_SOLVER_BAND_FP64 = 0, MSFEC_SOLVER_BAND_FP64_MFMA = 1, MSFEC_SOLVER_BAND_MIXED = 2 };

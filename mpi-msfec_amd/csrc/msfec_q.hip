@@ -40,6 +40,7 @@ constexpr int kWin = 64;       // sliding-window slots (>= bandwidth + 1)
 constexpr int kMaxNb = 8;      // bases per cell (2^dim)
 constexpr int kMaxLow = 14;    // lower stencil entries incl. diagonal (3-D)
 constexpr int kRowStride = 24; // row table: 16 lower-stencil slots + 8 lifting rhs
+constexpr int kPStride = 80;   // per cell: P (8 x 8) | b_B (8) | |cell| / 2^dim
 
 struct QConf {
   int dim, N, S, nn, nfc, nv, ni, n, nst, nlow, center;
@@ -323,6 +324,7 @@ struct BandArgs {
   void* zst;              // matrix-core kernels: [cell][nq][4][8] forward results
   double* Rres;           // mixed solver: [cell][n][8] refinement residual
   int* conv;              // mixed solver: [cell] 1 once the cell stops refining
+  double* Pb;             // fast path: [cell][kPStride] P = G_B^T K_BB G_B | b_B | |cell| / 2^dim
   int last_pass;          // mixed solver: this refinement pass is the last one
   double* Lcol;  // [cell][n][64]  o = 0: pivot d_j, o > 0: L[j+o][j]
   double* Wf;    // [cell][n][8]   D^-1 L^-1 rhs
@@ -417,6 +419,108 @@ __device__ __forceinline__ double lift_k(int k, const int* m, double invN) {
   return v;
 }
 
+// Sums over the fine cells shared by node c and each neighbour c + dd, for
+// all 3^dim directions e (dd_d = e's base-3 digit d minus 1) at once: the
+// coefficient (SA) and the reaction (SR, including the constant part).  The
+// 2^dim cells around the node are loaded once (cells outside the grid count
+// as zero) and combined one dimension at a time: dd_d = -1 keeps the cells
+// below the node in direction d, +1 those above, 0 both.
+template <int DIM>
+__device__ __forceinline__ void node_sums(const double* cf, const double* rf, bool has_rf,
+                                          double rc, const int* c, int N, double* SA,
+                                          double* SR) {
+  constexpr int NV = 1 << DIM;
+  constexpr int NST = (DIM == 3) ? 27 : 9;
+  double ca[NV], cr[NV];
+#pragma unroll
+  for (int al = 0; al < NV; ++al) {
+    bool ok = true;
+    int z = 0, mul = 1;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+      const int zd = c[d] - ((al >> d) & 1);
+      if (zd < 0 || zd >= N) ok = false;
+      z += zd * mul;
+      mul *= N;
+    }
+    ca[al] = ok ? cf[z] : 0.0;
+    cr[al] = ok ? (has_rf ? rf[z] : 0.0) + rc : 0.0;
+  }
+  // dimension 0: al bit 0 -> three direction values per remaining bits
+  double ta[NST], tr[NST];
+#pragma unroll
+  for (int e = 0; e < NST; ++e) {
+    // al bits in dimension d: dd = -1 -> cell index bit 1, +1 -> bit 0, 0 -> both
+    double sa = 0.0, sr = 0.0;
+#pragma unroll
+    for (int al = 0; al < NV; ++al) {
+      bool take = true;
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) {
+        const int dd = (d == 0) ? (e % 3) - 1 : (d == 1) ? ((e / 3) % 3) - 1 : (e / 9) - 1;
+        const int b = (al >> d) & 1;
+        if ((dd == -1 && b != 1) || (dd == 1 && b != 0)) take = false;
+      }
+      if (take) {
+        sa += ca[al];
+        sr += cr[al];
+      }
+    }
+    ta[e] = sa;
+    tr[e] = sr;
+  }
+#pragma unroll
+  for (int e = 0; e < NST; ++e) {
+    SA[e] = ta[e];
+    SR[e] = tr[e];
+  }
+}
+
+// out[k] = sum_e w[e] g_k(c + dd_e) for the 2^dim coarse bases k, with the
+// lift factorised per dimension (fl[d][dd_d + 1][k_d]): contracted one
+// dimension at a time instead of forming g_k at every neighbour.
+template <int DIM>
+__device__ __forceinline__ void lift_contract(const double* w, const double (&fl)[3][3][2],
+                                              double* out) {
+  if (DIM == 2) {
+    double t[3][2];  // [ey][kx]
+#pragma unroll
+    for (int ey = 0; ey < 3; ++ey)
+#pragma unroll
+      for (int kx = 0; kx < 2; ++kx)
+        t[ey][kx] = w[3 * ey + 0] * fl[0][0][kx] + w[3 * ey + 1] * fl[0][1][kx] +
+                    w[3 * ey + 2] * fl[0][2][kx];
+#pragma unroll
+    for (int ky = 0; ky < 2; ++ky)
+#pragma unroll
+      for (int kx = 0; kx < 2; ++kx)
+        out[kx + 2 * ky] = t[0][kx] * fl[1][0][ky] + t[1][kx] * fl[1][1][ky] + t[2][kx] * fl[1][2][ky];
+  } else {
+    double t[9][2];  // [ey + 3 ez][kx]
+#pragma unroll
+    for (int q = 0; q < 9; ++q)
+#pragma unroll
+      for (int kx = 0; kx < 2; ++kx)
+        t[q][kx] = w[3 * q + 0] * fl[0][0][kx] + w[3 * q + 1] * fl[0][1][kx] +
+                   w[3 * q + 2] * fl[0][2][kx];
+    double u[3][4];  // [ez][kx + 2 ky]
+#pragma unroll
+    for (int ez = 0; ez < 3; ++ez)
+#pragma unroll
+      for (int ky = 0; ky < 2; ++ky)
+#pragma unroll
+        for (int kx = 0; kx < 2; ++kx)
+          u[ez][kx + 2 * ky] = t[3 * ez + 0][kx] * fl[1][0][ky] + t[3 * ez + 1][kx] * fl[1][1][ky] +
+                               t[3 * ez + 2][kx] * fl[1][2][ky];
+#pragma unroll
+    for (int kz = 0; kz < 2; ++kz)
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        out[q + 4 * kz] = u[0][q] * fl[2][0][kz] + u[1][q] * fl[2][1][kz] + u[2][q] * fl[2][2][kz];
+  }
+}
+
+
 // Element stiffness / mass of a box as functions of a^b (bit d set: the two
 // vertices differ in direction d), for lanes x < 8.
 template <int DIM>
@@ -492,30 +596,35 @@ __global__ __launch_bounds__(256) void q_rows_kernel(BandArgs a) {
     for (int t = 0; t < 16; ++t) lo[t] = 0.0;
 #pragma unroll
     for (int k = 0; k < 8; ++k) rhs[k] = 0.0;
+    double SA[D::NST], SR[D::NST];
+    node_sums<DIM>(cf, rf, has_rf, rc, c, a.N, SA, SR);
+    double fl[3][3][2];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d)
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        const double t = (double)(c[d] + q - 1) * invN;
+        fl[d][q][0] = 1.0 - t;
+        fl[d][q][1] = t;
+      }
+    double wb[D::NST];  // couplings to boundary neighbours
 #pragma unroll
     for (int e = 0; e < D::NST; ++e) {
-      int dd[3] = {0, 0, 0}, m[3] = {0, 0, 0}, mask = 0;
+      int mask = 0;
       bool bnd = false;
 #pragma unroll
       for (int d = 0; d < DIM; ++d) {
-        dd[d] = dcomp<DIM>(e, d);
-        if (dd[d] != 0) mask |= 1 << d;
-        m[d] = c[d] + dd[d];
-        if (m[d] == 0 || m[d] == a.N) bnd = true;
+        const int dd = dcomp<DIM>(e, d);
+        if (dd != 0) mask |= 1 << d;
+        const int m = c[d] + dd;
+        if (m == 0 || m == a.N) bnd = true;
       }
-      double sa, sr;
-      int cnt;
-      shared_sums<DIM>(cf, rf, has_rf, c, dd, a.N, sa, sr, cnt);
-      const double kval = kv[mask] * sa + mv[mask] * (sr + rc * cnt);
-      if (e == D::CENTER) {
-        lo[0] = kval;
-      } else if (bnd) {
-#pragma unroll
-        for (int k = 0; k < D::NV; ++k) rhs[k] = fma(-kval, lift_k<DIM>(k, m, invN), rhs[k]);
-      } else if (e < D::CENTER) {
-        lo[1 + e] = kval;
-      }
+      const double kval = kv[mask] * SA[e] + mv[mask] * SR[e];
+      wb[e] = (bnd && e != D::CENTER) ? -kval : 0.0;
+      if (e == D::CENTER) lo[0] = kval;
+      else if (!bnd && e < D::CENTER) lo[1 + e] = kval;
     }
+    lift_contract<DIM>(wb, fl, rhs);
     // slot 14: sum of the loads of the fine cells around the node (the
     // interior load is this times |cell| / 2^dim); slot 15: the node index
     double fj;
@@ -542,6 +651,114 @@ __global__ __launch_bounds__(256) void q_rows_kernel(BandArgs a) {
 #pragma unroll
     for (int k = 0; k < 8; ++k) dst[16 + k] = rhs[k];
   }
+
+  // Boundary part of the element matrix and right-hand side, and the
+  // boundary values of Psi: with g_k the lifting (coarse Q1 basis k at the
+  // boundary nodes, zero inside),
+  //   P[i][k] = sum_{m in B} g_i(m) (K_BB g_k)(m),   b_B[i] = sum_{m in B} g_i(m) F(m).
+  // Nodes are taken 256 at a time; threads (i,k) < (2^dim)^2 then sum their
+  // entry over the chunk.
+  __shared__ double gs[256 * 9], kps[256 * 9], fsh[256];
+  double* psi_out = a.Psi + (size_t)cell * D::NV * a.nn;
+  double vol = 1.0;
+  {
+    const double* v = a.verts + (size_t)cell * D::NV * DIM;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) vol *= (v[(D::NV - 1) * DIM + d] - v[d]) / a.N;
+  }
+  const int tid = threadIdx.x;
+  const int pi = tid >> 3, pk = tid & 7;
+  const bool pthr = tid < 64 && pi < D::NV && pk < D::NV;
+  const bool bthr = tid >= 64 && tid < 64 + D::NV;
+  double pacc = 0.0, bacc = 0.0;
+  for (int base = 0; base < a.nn; base += 256) {
+    const int node = base + tid;
+    int c[3] = {0, 0, 0};
+    bool bnd = false;
+    {
+      int t = (node < a.nn) ? node : 0;
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) {
+        c[d] = t % a.S;
+        t /= a.S;
+        if (c[d] == 0 || c[d] == a.N) bnd = true;
+      }
+    }
+    double g[D::NV], kp[D::NV], fsum = 0.0;
+#pragma unroll
+    for (int k = 0; k < D::NV; ++k) {
+      g[k] = 0.0;
+      kp[k] = 0.0;
+    }
+    if (node < a.nn && bnd) {
+#pragma unroll
+      for (int k = 0; k < D::NV; ++k) {
+        g[k] = lift_k<DIM>(k, c, invN);
+        psi_out[(size_t)k * a.nn + node] = g[k];
+      }
+      // F[node] = sum over the fine cells touching the node of f_z |z| / 2^dim
+#pragma unroll
+      for (int al = 0; al < D::NV; ++al) {
+        bool ok = true;
+        int z = 0, mul = 1;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+          const int zd = c[d] - ((al >> d) & 1);
+          if (zd < 0 || zd >= a.N) ok = false;
+          z += zd * mul;
+          mul *= a.N;
+        }
+        if (ok) fsum += (a.rhs_mode == MSFEC_RHS_CELL) ? ff[z] : a.rhs_const;
+      }
+      fsum *= vol / D::NV;
+      // (K_BB g_k)[node]: boundary neighbours only
+      double SA[D::NST], SR[D::NST];
+      node_sums<DIM>(cf, rf, has_rf, rc, c, a.N, SA, SR);
+      double fl[3][3][2];
+#pragma unroll
+      for (int d = 0; d < DIM; ++d)
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+          const double t = (double)(c[d] + q - 1) * invN;
+          fl[d][q][0] = 1.0 - t;
+          fl[d][q][1] = t;
+        }
+      double wb[D::NST];  // couplings to boundary neighbours inside the grid
+#pragma unroll
+      for (int e = 0; e < D::NST; ++e) {
+        int mask = 0;
+        bool inside = true, mb = false;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+          const int dd = dcomp<DIM>(e, d);
+          if (dd != 0) mask |= 1 << d;
+          const int m = c[d] + dd;
+          if (m < 0 || m > a.N) inside = false;
+          if (m == 0 || m == a.N) mb = true;
+        }
+        wb[e] = (inside && mb) ? kv[mask] * SA[e] + mv[mask] * SR[e] : 0.0;
+      }
+      lift_contract<DIM>(wb, fl, kp);
+    }
+    __syncthreads();  // the previous chunk has been summed
+#pragma unroll
+    for (int k = 0; k < D::NV; ++k) {
+      gs[tid * 9 + k] = g[k];
+      kps[tid * 9 + k] = kp[k];
+    }
+    fsh[tid] = fsum;
+    __syncthreads();
+    if (pthr) {
+      for (int l = 0; l < 256; ++l) pacc = fma(gs[l * 9 + pi], kps[l * 9 + pk], pacc);
+    }
+    if (bthr) {
+      for (int l = 0; l < 256; ++l) bacc = fma(gs[l * 9 + (tid - 64)], fsh[l], bacc);
+    }
+  }
+  double* pb = a.Pb + (size_t)cell * kPStride;
+  if (pthr) pb[pi * 8 + pk] = pacc;
+  if (bthr) pb[64 + (tid - 64)] = bacc;
+  if (tid == 0) pb[72] = vol / D::NV;
 }
 
 // Rows jbase..jbase+7 of the row table are 192 contiguous doubles: every
@@ -573,6 +790,7 @@ __device__ __forceinline__ void rows_store(const RowPrefetch& pf, BandSmem& sh) 
   }
   wave_sync();
 }
+
 
 // Stage the rows jbase..jbase+7 (lower stencil with zeros for non-interior
 // neighbours, and the lifting right-hand sides) into LDS from the tables of
@@ -757,110 +975,21 @@ __device__ __forceinline__ void back_step(const BandArgs& a, int cell, int j, Ba
 // Fast-path epilogue.  With R = -K_IB G_B the lifting right-hand sides and
 // K_II = L D L^T, the element matrix psi_i^T K psi_j of the reference
 // (q_basis.cc:280-293) is exactly
-//   M = G_B^T K_BB G_B - R^T K_II^{-1} R = P - sum_j z_j z_j^T / d_j ,
-// z = L^{-1} R the forward-eliminated right-hand sides.  The sum (sacc) is
-// accumulated during the forward sweep; P involves boundary nodes only and
-// is formed here from the LDS coefficients, together with the boundary part
-// of Psi and of b = Psi^T F (the interior parts were produced by the
-// backward sweep).  Nodes are processed 64 at a time, one per lane; lane
-// (i,k) then accumulates P[i][k] over the chunk from LDS.
+//   M = G_B^T K_BB G_B - R^T K_II^{-1} R = P - S,
+// with S = sum_j z_j z_j^T / d_j (z = L^{-1} R, accumulated in the forward
+// sweep) or, for the mixed solver, the energy form R^T X + X^T (R - K_II X).
+// P, the boundary part b_B of b = Psi^T F and the boundary values of Psi
+// come from q_rows_kernel; lane (i,k) holds S[i][k], lane k the interior
+// part of b (before the factor |cell| / 2^dim).
 template <int DIM>
-__device__ __forceinline__ void fast_epilogue(const BandArgs& a, int cell, BandSmem& sh,
-                                              const double* cf, const double* rf,
-                                              const double* ff, double vol, double sacc,
-                                              double bint) {
+__device__ __forceinline__ void fast_finish(const BandArgs& a, int cell, double sacc, double bint) {
   using D = Dirs<DIM>;
   const int lane = threadIdx.x;
-  double* psi_out = a.Psi + (size_t)cell * D::NV * a.nn;
-  const double invN = 1.0 / (double)a.N;
-  const bool has_rf = (a.reaction_mode == MSFEC_REACTION_CELL);
-  const double rc = (a.reaction_mode == MSFEC_REACTION_CONST) ? a.reaction_const : 0.0;
-  double* pn_s = SH_PN(sh);
-  double* kp_s = SH_KP(sh);
-  double* fs_s = SH_FS(sh);
-  double pacc = 0.0, bacc = 0.0;
-  const int mi_ = lane >> 3, mk_ = lane & 7;
-  for (int base = 0; base < a.nn; base += 64) {
-    const int node = base + lane;
-    int c[3] = {0, 0, 0};
-    bool bnd = false;
-    {
-      int t = (node < a.nn) ? node : 0;
-#pragma unroll
-      for (int d = 0; d < DIM; ++d) {
-        c[d] = t % a.S;
-        t /= a.S;
-        if (c[d] == 0 || c[d] == a.N) bnd = true;
-      }
-    }
-    const bool act = (node < a.nn) && bnd;
-    double fs = 0.0;
-    double kp[D::NV];
-#pragma unroll
-    for (int k = 0; k < D::NV; ++k) kp[k] = 0.0;
-    if (act) {
-#pragma unroll
-      for (int k = 0; k < D::NV; ++k) {
-        const double v = lift_k<DIM>(k, c, invN);
-        psi_out[(size_t)k * a.nn + node] = v;
-        pn_s[lane * 9 + k] = v;
-      }
-      // F[node] = sum over the cells touching the node of f_z |z| / 2^dim
-#pragma unroll
-      for (int al = 0; al < D::NV; ++al) {
-        bool ok = true;
-        int z = 0, mul = 1;
-#pragma unroll
-        for (int d = 0; d < DIM; ++d) {
-          const int zd = c[d] - ((al >> d) & 1);
-          if (zd < 0 || zd >= a.N) ok = false;
-          z += zd * mul;
-          mul *= a.N;
-        }
-        if (ok) fs += (a.rhs_mode == MSFEC_RHS_CELL) ? ff[z] : a.rhs_const;
-      }
-      fs *= vol / D::NV;
-      // (K_BB g_k)[node]: boundary neighbours only
-#pragma unroll
-      for (int e = 0; e < D::NST; ++e) {
-        int dd[3] = {0, 0, 0}, m[3] = {0, 0, 0}, mask = 0;
-        bool inside = true, mb = false;
-#pragma unroll
-        for (int d = 0; d < DIM; ++d) {
-          dd[d] = dcomp<DIM>(e, d);
-          if (dd[d] != 0) mask |= 1 << d;
-          m[d] = c[d] + dd[d];
-          if (m[d] < 0 || m[d] > a.N) inside = false;
-          if (m[d] == 0 || m[d] == a.N) mb = true;
-        }
-        if (!inside || !mb) continue;
-        double sa, sr;
-        int cnt;
-        shared_sums<DIM>(cf, rf, has_rf, c, dd, a.N, sa, sr, cnt);
-        const double kval = sh.kv[mask] * sa + sh.mv[mask] * (sr + rc * cnt);
-#pragma unroll
-        for (int k = 0; k < D::NV; ++k) kp[k] = fma(kval, lift_k<DIM>(k, m, invN), kp[k]);
-      }
-    } else {
-#pragma unroll
-      for (int k = 0; k < D::NV; ++k) pn_s[lane * 9 + k] = 0.0;
-    }
-    fs_s[lane] = fs;
-#pragma unroll
-    for (int k = 0; k < D::NV; ++k) kp_s[lane * 9 + k] = kp[k];
-    wave_sync();
-    if (mi_ < D::NV && mk_ < D::NV) {
-#pragma unroll 8
-      for (int l = 0; l < 64; ++l) pacc = fma(pn_s[l * 9 + mi_], kp_s[l * 9 + mk_], pacc);
-    }
-    if (lane < D::NV) {
-#pragma unroll 8
-      for (int l = 0; l < 64; ++l) bacc = fma(pn_s[l * 9 + lane], fs_s[l], bacc);
-    }
-    wave_sync();
-  }
-  if (mi_ < D::NV && mk_ < D::NV) a.M[((size_t)cell * D::NV + mi_) * D::NV + mk_] = pacc - sacc;
-  if (lane < D::NV) a.bvec[(size_t)cell * D::NV + lane] = bacc + bint * (vol / D::NV);
+  const int mi = lane >> 3, mk = lane & 7;
+  const double* pb = a.Pb + (size_t)cell * kPStride;
+  if (mi < D::NV && mk < D::NV)
+    a.M[((size_t)cell * D::NV + mi) * D::NV + mk] = pb[mi * 8 + mk] - sacc;
+  if (lane < D::NV) a.bvec[(size_t)cell * D::NV + lane] = pb[64 + lane] + bint * pb[72];
 }
 
 template <int DIM, bool FAST>
@@ -1004,7 +1133,7 @@ __global__ __launch_bounds__(64, 2) void q_band_kernel(BandArgs a) {
   if (lane == 0) a.status[cell] = worst;
   if (FAST) {
     wave_sync();
-    fast_epilogue<DIM>(a, cell, sh, cf, rf, ff, vol, sacc, bint);
+    fast_finish<DIM>(a, cell, sacc, bint);
   }
 }
 
@@ -1412,7 +1541,7 @@ __global__ __launch_bounds__(64, 2) void q_band_mfma_kernel(BandArgs a) {
   if (FAST) {
     wave_sync();
     // bint lives in lanes 0..7 (RHS r = lane); the epilogue reads it from lane k
-    fast_epilogue<DIM>(a, cell, sh, cf, rf, ff, vol, sacc, bint);
+    fast_finish<DIM>(a, cell, sacc, bint);
   }
 }
 
@@ -1636,6 +1765,14 @@ __device__ __forceinline__ void bk_superstep(const BandArgs& a, int cell, int j0
   wave_sync();
 }
 
+// Staging for the matrix-core kernels: two prefetch sets used alternately,
+// so that rows are loaded 16 rows (4 super-steps) before they are stored.
+__device__ __forceinline__ void bk_stage(const BandArgs& a, int cell, int jbase, BandSmem& sh,
+                                         RowPrefetch& pf) {
+  rows_store(pf, sh);
+  rows_load(a, cell, jbase + 16, pf);
+}
+
 // The 4 super-steps of tile row T (pivots j0 .. j0+15); the next 16 rows of
 // the band enter the freed slots.
 template <typename R, int DIM, int T>
@@ -1643,17 +1780,17 @@ __device__ __forceinline__ void bk_tile_row(const BandArgs& a, int cell, int j0,
                                             typename Mf<R>::v4 (&Tl)[10],
                                             typename Mf<R>::v4 (&Yt)[4],
                                             typename Mf<R>::v4& Sacc, BandSmem& sh,
-                                            BkSmem<R>& ms, RowPrefetch& pf, int& bad,
+                                            BkSmem<R>& ms, RowPrefetch (&pf)[2], int& bad,
                                             const R (&ek)[4]) {
   const int n = a.n;
-  stage_group<DIM, true>(a, cell, j0 + kWin, sh, pf);
+  bk_stage(a, cell, j0 + kWin, sh, pf[0]);
   bk_superstep<R, DIM, T, 0>(a, cell, j0, Tl, Yt, Sacc, ms, bad, ek);
   bk_insert<R, DIM, T, 0>(a, j0 + kWin, 0, my_off, Tl, Yt, sh, ms);
   if (j0 + 4 < n) {
     bk_superstep<R, DIM, T, 1>(a, cell, j0 + 4, Tl, Yt, Sacc, ms, bad, ek);
     bk_insert<R, DIM, T, 1>(a, j0 + 4 + kWin, 4, my_off, Tl, Yt, sh, ms);
   }
-  stage_group<DIM, true>(a, cell, j0 + 8 + kWin, sh, pf);
+  bk_stage(a, cell, j0 + 8 + kWin, sh, pf[1]);
   if (j0 + 8 < n) {
     bk_superstep<R, DIM, T, 2>(a, cell, j0 + 8, Tl, Yt, Sacc, ms, bad, ek);
     bk_insert<R, DIM, T, 2>(a, j0 + 8 + kWin, 0, my_off, Tl, Yt, sh, ms);
@@ -1669,11 +1806,11 @@ template <typename R, int DIM, int T>
 __device__ __forceinline__ void bk_fill_row(const BandArgs& a, int cell, int my_off,
                                             typename Mf<R>::v4 (&Tl)[10],
                                             typename Mf<R>::v4 (&Yt)[4], BandSmem& sh,
-                                            BkSmem<R>& ms, RowPrefetch& pf) {
-  stage_group<DIM, true>(a, cell, 16 * T, sh, pf);
+                                            BkSmem<R>& ms, RowPrefetch (&pf)[2]) {
+  bk_stage(a, cell, 16 * T, sh, pf[0]);
   bk_insert<R, DIM, T, 0>(a, 16 * T + 0, 0, my_off, Tl, Yt, sh, ms);
   bk_insert<R, DIM, T, 1>(a, 16 * T + 4, 4, my_off, Tl, Yt, sh, ms);
-  stage_group<DIM, true>(a, cell, 16 * T + 8, sh, pf);
+  bk_stage(a, cell, 16 * T + 8, sh, pf[1]);
   bk_insert<R, DIM, T, 2>(a, 16 * T + 8, 0, my_off, Tl, Yt, sh, ms);
   bk_insert<R, DIM, T, 3>(a, 16 * T + 12, 4, my_off, Tl, Yt, sh, ms);
 }
@@ -1708,39 +1845,48 @@ __device__ __forceinline__ void bk_backward(const BandArgs& a, int cell, BandSme
   const R* Pc = reinterpret_cast<const R*>(a.pan) + (size_t)cell * a.nq * 256;
   const R* Zc = reinterpret_cast<const R*>(a.zst) + (size_t)cell * a.nq * 32;
   const double* Rc = a.Rows + (size_t)cell * n * kRowStride;
-  // prefetch registers: 4 panel values, 1 z value, 2 table values per lane
-  R pp[4], zp = (R)0;
-  double rp0 = 0.0, rp1 = 0.0, xp = 0.0, qp = 0.0;
-  auto load = [&](int q) {
+  // Prefetch ring: the data of step q is loaded 4 steps ahead into register
+  // set (nq - 1 - q) mod 4 (4 panel values, z, 2 table values and, when
+  // refining, X and its residual per lane).
+  struct Pre {
+    R pp[4], zp;
+    double rp0, rp1, xp, qp;
+  } pre[4];
+  auto load = [&](int q, Pre& f) {
+    if (q < 0) return;
 #pragma unroll
-    for (int u = 0; u < 4; ++u) pp[u] = Pc[(size_t)q * 256 + lane * 4 + u];
-    if (lane < 32) zp = Zc[(size_t)q * 32 + lane];
+    for (int u = 0; u < 4; ++u) f.pp[u] = Pc[(size_t)q * 256 + lane * 4 + u];
+    f.zp = (lane < 32) ? Zc[(size_t)q * 32 + lane] : (R)0;
     const int lim = (n - 4 * q) * kRowStride;  // valid doubles of the 4 rows
-    rp0 = (lane < lim) ? Rc[(size_t)4 * q * kRowStride + lane] : 0.0;
-    rp1 = (lane + 64 < min(lim, 96)) ? Rc[(size_t)4 * q * kRowStride + 64 + lane] : 0.0;
-    if (MODE == BK_REFINE && lane < 32 && 4 * q + (lane >> 3) < n) {
-      xp = a.X[((size_t)cell * n + 4 * q) * kMaxNb + lane];
-      qp = a.Rres[((size_t)cell * n + 4 * q) * kMaxNb + lane];
-    } else if (MODE == BK_REFINE) {
-      xp = 0.0;
-      qp = 0.0;
+    f.rp0 = (lane < lim) ? Rc[(size_t)4 * q * kRowStride + lane] : 0.0;
+    f.rp1 = (lane + 64 < min(lim, 96)) ? Rc[(size_t)4 * q * kRowStride + 64 + lane] : 0.0;
+    if (MODE == BK_REFINE) {
+      const bool ok = lane < 32 && 4 * q + (lane >> 3) < n;
+      const size_t xi = ((size_t)cell * n + 4 * q) * kMaxNb + (ok ? lane : 0);
+      f.xp = ok ? a.X[xi] : 0.0;
+      f.qp = ok ? a.Rres[xi] : 0.0;
     }
   };
-  load(nq - 1);
-  for (int q = nq - 1; q >= 0; --q) {
+#pragma unroll
+  for (int u = 0; u < 4; ++u) load(nq - 1 - u, pre[u]);
+  for (int q0 = nq - 1; q0 >= 0; q0 -= 4) {
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int q = q0 - u;
+    if (q < 0) break;
     const int j0 = 4 * q;
     wave_sync();
 #pragma unroll
-    for (int u = 0; u < 4; ++u) pb[(lane >> 4) * kPanRow + (lane & 15) * 4 + u] = pp[u];
-    if (lane < 32) zb[lane] = (double)zp;
-    rw[lane] = rp0;
-    if (lane < 32) rw[64 + lane] = rp1;
+    for (int v = 0; v < 4; ++v) pb[(lane >> 4) * kPanRow + (lane & 15) * 4 + v] = pre[u].pp[v];
+    if (lane < 32) zb[lane] = (double)pre[u].zp;
+    rw[lane] = pre[u].rp0;
+    if (lane < 32) rw[64 + lane] = pre[u].rp1;
     if (MODE == BK_REFINE && lane < 32) {
-      xb[lane] = xp;
-      r0b[lane] = qp;
+      xb[lane] = pre[u].xp;
+      r0b[lane] = pre[u].qp;
     }
     wave_sync();
-    if (q > 0) load(q - 1);
+    load(q - 4, pre[u]);
     if (MODE == BK_REFINE) {
       // energy-form Schur term: lane (i = lane >> 3, kk = lane & 7)
       const int i = lane >> 3, kk = lane & 7;
@@ -1792,6 +1938,7 @@ __device__ __forceinline__ void bk_backward(const BandArgs& a, int cell, BandSme
       }
     }
   }
+  }
   wave_sync();
 }
 
@@ -1808,42 +1955,6 @@ __device__ __forceinline__ double bk_sacc(const typename Mf<R>::v4& Sacc, double
   const double v = buf[lane];
   wave_sync();
   return v;
-}
-
-// Prologue shared by the fast-path kernels: coefficient, reaction and load of
-// the cell to LDS, fine-cell size, element tables.
-template <int DIM>
-__device__ __forceinline__ double bk_prologue(const BandArgs& a, int cell, BandSmem& sh,
-                                              double* dyn) {
-  using D = Dirs<DIM>;
-  const int lane = threadIdx.x;
-  const double* cg = a.coef + (size_t)cell * a.nfc;
-  for (int z = lane; z < a.nfc; z += 64) dyn[z] = cg[z];
-  if (a.reaction_mode == MSFEC_REACTION_CELL) {
-    const double* rg = a.reac + (size_t)cell * a.nfc;
-    for (int z = lane; z < a.nfc; z += 64) dyn[a.nfc + z] = rg[z];
-  }
-  if (a.rhs_mode == MSFEC_RHS_CELL) {
-    const double* fg = a.fcell + (size_t)cell * a.nfc;
-    double* fd = dyn + a.nfc * ((a.reaction_mode == MSFEC_REACTION_CELL) ? 2 : 1);
-    for (int z = lane; z < a.nfc; z += 64) fd[z] = fg[z];
-  }
-  const double* v = a.verts + (size_t)cell * D::NV * DIM;
-  double hh[3] = {1.0, 1.0, 1.0};
-  double vol = 1.0;
-#pragma unroll
-  for (int d = 0; d < DIM; ++d) {
-    hh[d] = (v[(D::NV - 1) * DIM + d] - v[d]) / a.N;
-    vol *= hh[d];
-  }
-  if (lane < 8) {
-    double ks, msv;
-    box_tables<DIM>(lane, hh, ks, msv);
-    sh.kv[lane] = ks;
-    sh.mv[lane] = msv;
-  }
-  wave_sync();
-  return vol;
 }
 
 // Factor + forward elimination of the lifting right-hand sides, then the
@@ -1863,12 +1974,6 @@ __global__ __launch_bounds__(64, BkOcc<R>::waves) void q_blk_factor_kernel(BandA
   const int lane = threadIdx.x;
   __shared__ BandSmem sh;
   __shared__ BkSmem<R> ms;
-  extern __shared__ double dyn[];  // coefficient | reaction | load
-  const double* cf = dyn;
-  const double* rf = dyn + a.nfc;
-  const double* ff = dyn + a.nfc * ((a.reaction_mode == MSFEC_REACTION_CELL) ? 2 : 1);
-  double vol = 1.0;
-  if (MODE == BK_FINAL64) vol = bk_prologue<DIM>(a, cell, sh, dyn);
   int my_off = 0;
 #pragma unroll
   for (int t = 0; t < kMaxLow; ++t)
@@ -1882,8 +1987,9 @@ __global__ __launch_bounds__(64, BkOcc<R>::waves) void q_blk_factor_kernel(BandA
   R ek[4];  // unit vector e_(lane >> 4)
 #pragma unroll
   for (int t = 0; t < 4; ++t) ek[t] = ((lane >> 4) == t) ? (R)1 : (R)0;
-  RowPrefetch pf;
-  rows_load(a, cell, 0, pf);
+  RowPrefetch pf[2];
+  rows_load(a, cell, 0, pf[0]);
+  rows_load(a, cell, 8, pf[1]);
   bk_fill_row<R, DIM, 0>(a, cell, my_off, Tl, Yt, sh, ms, pf);
   bk_fill_row<R, DIM, 1>(a, cell, my_off, Tl, Yt, sh, ms, pf);
   bk_fill_row<R, DIM, 2>(a, cell, my_off, Tl, Yt, sh, ms, pf);
@@ -1908,40 +2014,52 @@ __global__ __launch_bounds__(64, BkOcc<R>::waves) void q_blk_factor_kernel(BandA
     a.status[cell] = worst;
     if (MODE == BK_X0) a.conv[cell] = (worst != 0) ? 1 : 0;  // failed cells do not refine
   }
-  if (MODE == BK_FINAL64) fast_epilogue<DIM>(a, cell, sh, cf, rf, ff, vol, sacc, bint);
+  if (MODE == BK_FINAL64) fast_finish<DIM>(a, cell, sacc, bint);
 }
 
 // ---------------------------------------------------------------------------
-// Mixed-precision solver, refinement residual: Rres = R - K_II X in FP64,
-// one thread per (cell, interior row, right-hand side).  K_II comes from the
+// Mixed-precision solver, refinement residual: Rres = R - K_II X in FP64.
+// One block per cell stages the cell's X (n x 8 doubles, at most 61.5 KB) in
+// LDS; one thread per (interior row, right-hand side).  K_II comes from the
 // row table: lower entries of row j, upper entries as the lower entries of
 // the rows j + off (zero where the neighbour is not interior).
 // ---------------------------------------------------------------------------
 template <int DIM>
-__global__ __launch_bounds__(256) void q_resid_kernel(BandArgs a, int n_cells) {
+__global__ __launch_bounds__(256) void q_resid_kernel(BandArgs a, int stage_lo) {
   using D = Dirs<DIM>;
-  const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int cell = blockIdx.x;
+  if (a.conv[cell]) return;  // uniform over the block
+  extern __shared__ double xs[];  // [n][8] | lower stencil [n][NLOW] when stage_lo
   const int n = a.n;
-  if (gid >= (long long)n_cells * n * kMaxNb) return;
-  const int r = (int)(gid & (kMaxNb - 1));
-  const long long cj = gid >> 3;
-  const int cell = (int)(cj / n), j = (int)(cj - (long long)cell * n);
-  if (a.conv[cell]) return;
+  double* los = xs + (size_t)n * kMaxNb;
   const double* rows = a.Rows + (size_t)cell * n * kRowStride;
   const double* X = a.X + (size_t)cell * n * kMaxNb;
-  const double* rj = rows + (size_t)j * kRowStride;
-  double acc = rj[16 + r] - rj[0] * X[(size_t)j * kMaxNb + r];
-#pragma unroll
-  for (int t = 1; t < D::NLOW; ++t) {
-    const int off = a.off_int[t];
-    const double lo = rj[t];
-    if (lo != 0.0 && j >= off) acc = fma(-lo, X[(size_t)(j - off) * kMaxNb + r], acc);
-    if (j + off < n) {
-      const double up = rows[(size_t)(j + off) * kRowStride + t];
-      if (up != 0.0) acc = fma(-up, X[(size_t)(j + off) * kMaxNb + r], acc);
+  for (int i = threadIdx.x; i < n * kMaxNb; i += blockDim.x) xs[i] = X[i];
+  if (stage_lo)
+    for (int i = threadIdx.x; i < n * D::NLOW; i += blockDim.x) {
+      const int j = i / D::NLOW, t = i - j * D::NLOW;
+      los[i] = rows[(size_t)j * kRowStride + t];
     }
+  __syncthreads();
+  for (int i = threadIdx.x; i < n * kMaxNb; i += blockDim.x) {
+    const int j = i >> 3, r = i & (kMaxNb - 1);
+    const double* rj = rows + (size_t)j * kRowStride;
+    auto lo = [&](int jj, int t) {
+      return stage_lo ? los[jj * D::NLOW + t] : rows[(size_t)jj * kRowStride + t];
+    };
+    double acc = rj[16 + r] - lo(j, 0) * xs[i];
+#pragma unroll
+    for (int t = 1; t < D::NLOW; ++t) {
+      const int off = a.off_int[t];
+      const double l = lo(j, t);
+      if (l != 0.0 && j >= off) acc = fma(-l, xs[(j - off) * kMaxNb + r], acc);
+      if (j + off < n) {
+        const double up = lo(j + off, t);
+        if (up != 0.0) acc = fma(-up, xs[(j + off) * kMaxNb + r], acc);
+      }
+    }
+    a.Rres[((size_t)cell * n + j) * kMaxNb + r] = acc;
   }
-  a.Rres[((size_t)cell * n + j) * kMaxNb + r] = acc;
 }
 
 // ---------------------------------------------------------------------------
@@ -1949,26 +2067,40 @@ __global__ __launch_bounds__(256) void q_resid_kernel(BandArgs a, int n_cells) {
 // the stored FP32 panels (4 MFMAs per super-step), backward sweep of the
 // correction, x = X + correction, then the element matrix and Psi.
 // ---------------------------------------------------------------------------
+// Per-lane prefetch for one super-step of the refinement sweep: the panel
+// (4 values) and one value of the residual row entering the window.
+struct RfPre {
+  float pp[4];
+  float rq;
+};
+
+__device__ __forceinline__ void rf_load(const BandArgs& a, int cell, int q, RfPre& f) {
+  const int lane = threadIdx.x, lh = lane >> 4, ll = lane & 15;
+  const int n = a.n;
+  if (4 * q >= n) return;
+  const float* Pn = reinterpret_cast<const float*>(a.pan) + ((size_t)cell * a.nq + q) * 256;
+#pragma unroll
+  for (int u = 0; u < 4; ++u) f.pp[u] = Pn[lane * 4 + u];
+  const int jn = 4 * q + kWin + lh;
+  const float rv = (float)a.Rres[((size_t)cell * n + min(jn, n - 1)) * kMaxNb + (ll & 7)];
+  f.rq = (ll < 8 && jn < n) ? rv : 0.0f;
+}
+
 template <int DIM, int T, int G>
 __device__ __forceinline__ void rf_step(const BandArgs& a, int cell, int j0,
                                         typename Mf<float>::v4 (&Yt)[4], BkSmem<float>& ms,
-                                        float (&pp)[4]) {
+                                        RfPre& f) {
   typedef Mf<float> M;
-  const int lane = threadIdx.x, lh = lane >> 4, ll = lane & 15, cs = M::cslot(ll);
-  constexpr int s0 = 16 * T + 4 * G;
-  const int n = a.n;
+  const int lane = bk_lane(), lh = lane >> 4, ll = lane & 15, cs = M::cslot(ll);
   float* pb = ms.piv;  // [4][68]
   const int qs = j0 >> 2;
-  // panel of this super-step (prefetched) and the pivot right-hand sides to LDS
+  // panel of this super-step and the pivot right-hand sides to LDS
 #pragma unroll
-  for (int u = 0; u < 4; ++u) pb[lh * kPanRow + ll * 4 + u] = pp[u];
+  for (int u = 0; u < 4; ++u) pb[lh * kPanRow + ll * 4 + u] = f.pp[u];
   ms.yp[lh * 16 + ll] = Yt[T][G];
+  const float rin = f.rq;
   wave_sync();
-  if (j0 + 4 < n) {
-    const float* Pn = reinterpret_cast<const float*>(a.pan) + ((size_t)cell * a.nq + qs + 1) * 256;
-#pragma unroll
-    for (int u = 0; u < 4; ++u) pp[u] = Pn[lane * 4 + u];
-  }
+  rf_load(a, cell, qs + 4, f);  // 4 super-steps ahead
   // Z = C^-1 y_pivot by forward substitution (C (t,u) at pb[u * 68 + t],
   // 1 / C[t][t] at pb[t * 68 + t]); lane (k = lh, rhs ll) keeps row lh
   const float y0 = ms.yp[0 * 16 + ll], y1 = ms.yp[1 * 16 + ll];
@@ -1987,22 +2119,20 @@ __device__ __forceinline__ void rf_step(const BandArgs& a, int cell, int j0,
 #pragma unroll
   for (int P = 0; P < 4; ++P) Yt[P] = M::mma(nV[P], Z, Yt[P]);
   if (ll < 8) reinterpret_cast<float*>(a.zst)[((size_t)cell * a.nq + qs) * 32 + lh * 8 + ll] = Z;
-  // next rows of the band enter the pivot slots
-  const int jn = j0 + kWin + lh;
-  const float rv = (float)a.Rres[((size_t)cell * n + min(jn, n - 1)) * kMaxNb + (ll & 7)];
-  Yt[T][G] = (ll < 8 && jn < n) ? rv : 0.0f;
+  // the next rows of the band enter the pivot slots
+  Yt[T][G] = rin;
   wave_sync();
 }
 
 template <int DIM, int T>
 __device__ __forceinline__ void rf_tile_row(const BandArgs& a, int cell, int j0,
                                             typename Mf<float>::v4 (&Yt)[4], BkSmem<float>& ms,
-                                            float (&pp)[4]) {
+                                            RfPre (&f)[4]) {
   const int n = a.n;
-  rf_step<DIM, T, 0>(a, cell, j0, Yt, ms, pp);
-  if (j0 + 4 < n) rf_step<DIM, T, 1>(a, cell, j0 + 4, Yt, ms, pp);
-  if (j0 + 8 < n) rf_step<DIM, T, 2>(a, cell, j0 + 8, Yt, ms, pp);
-  if (j0 + 12 < n) rf_step<DIM, T, 3>(a, cell, j0 + 12, Yt, ms, pp);
+  rf_step<DIM, T, 0>(a, cell, j0, Yt, ms, f[0]);
+  if (j0 + 4 < n) rf_step<DIM, T, 1>(a, cell, j0 + 4, Yt, ms, f[1]);
+  if (j0 + 8 < n) rf_step<DIM, T, 2>(a, cell, j0 + 8, Yt, ms, f[2]);
+  if (j0 + 12 < n) rf_step<DIM, T, 3>(a, cell, j0 + 12, Yt, ms, f[3]);
 }
 
 template <int DIM>
@@ -2013,11 +2143,9 @@ __global__ __launch_bounds__(64, 2) void q_blk_refine_kernel(BandArgs a) {
   if (a.conv[cell]) return;  // converged or failed: the whole wave leaves
   __shared__ BandSmem sh;
   __shared__ BkSmem<float> ms;
-  extern __shared__ double dyn[];
-  const double* cf = dyn;
-  const double* rf = dyn + a.nfc;
-  const double* ff = dyn + a.nfc * ((a.reaction_mode == MSFEC_REACTION_CELL) ? 2 : 1);
-  const double vol = bk_prologue<DIM>(a, cell, sh, dyn);
+  RfPre pre[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) rf_load(a, cell, u, pre[u]);
   const int n = a.n;
   V4 Yt[4];
   // rows 0..63 of the residual: slot 4g + lh of tile row P is register g
@@ -2029,19 +2157,13 @@ __global__ __launch_bounds__(64, 2) void q_blk_refine_kernel(BandArgs a) {
       const float rv = (float)a.Rres[((size_t)cell * n + min(j, n - 1)) * kMaxNb + (ll & 7)];
       Yt[P][g] = (ll < 8 && j < n) ? rv : 0.0f;
     }
-  float pp[4];
-  {
-    const float* Pn = reinterpret_cast<const float*>(a.pan) + (size_t)cell * a.nq * 256;
-#pragma unroll
-    for (int u = 0; u < 4; ++u) pp[u] = Pn[lane * 4 + u];
-  }
   for (int t = lane; t < 4 * kPanRow; t += 64) ms.piv[t] = 0.0f;
   wave_sync();
   for (int j0 = 0; j0 < n; j0 += 64) {
-    rf_tile_row<DIM, 0>(a, cell, j0, Yt, ms, pp);
-    if (j0 + 16 < n) rf_tile_row<DIM, 1>(a, cell, j0 + 16, Yt, ms, pp);
-    if (j0 + 32 < n) rf_tile_row<DIM, 2>(a, cell, j0 + 32, Yt, ms, pp);
-    if (j0 + 48 < n) rf_tile_row<DIM, 3>(a, cell, j0 + 48, Yt, ms, pp);
+    rf_tile_row<DIM, 0>(a, cell, j0, Yt, ms, pre);
+    if (j0 + 16 < n) rf_tile_row<DIM, 1>(a, cell, j0 + 16, Yt, ms, pre);
+    if (j0 + 32 < n) rf_tile_row<DIM, 2>(a, cell, j0 + 32, Yt, ms, pre);
+    if (j0 + 48 < n) rf_tile_row<DIM, 3>(a, cell, j0 + 48, Yt, ms, pre);
   }
   __threadfence_block();
   __syncthreads();
@@ -2052,17 +2174,18 @@ __global__ __launch_bounds__(64, 2) void q_blk_refine_kernel(BandArgs a) {
     xmax = fmax(xmax, __shfl_xor(xmax, o));
   }
   // The error left after this step is about (|dx| / |x|)^2 |x| (the
-  // contraction of the refinement is about |dx| / |x| itself).
+  // contraction of the refinement is about |dx| / |x| itself); a cell stops
+  // once that is below 1e-11, a tenth of the 1e-10 acceptance bound.
   const double q2 = (dmax * dmax) / fmax(xmax * xmax, 1e-300);
   if (lane == 0) {
-    if (q2 <= 1e-13) {
+    if (q2 <= 1e-11) {
       a.conv[cell] = 1;
     } else if (a.last_pass) {
       a.conv[cell] = 1;
       if (q2 > 1e-10 && a.status[cell] == MSFEC_CELL_OK) a.status[cell] = MSFEC_CELL_NOT_CONVERGED;
     }
   }
-  fast_epilogue<DIM>(a, cell, sh, cf, rf, ff, vol, sacc, bint);
+  fast_finish<DIM>(a, cell, sacc, bint);
 }
 
 // ---------------------------------------------------------------------------
@@ -2387,6 +2510,7 @@ struct msfec_q_batch {
   double* d_reac = nullptr;
   double* d_rows = nullptr;
   double* d_rres = nullptr;
+  double* d_pb = nullptr;
   int* d_conv = nullptr;
   double* d_slo = nullptr;
   double* d_F = nullptr;
@@ -2429,7 +2553,7 @@ msfec_status dalloc(msfec_q_batch* h, T** p, size_t count) {
 }
 
 void free_all(msfec_q_batch* h) {
-  void* ptrs[] = {h->d_conv,  h->d_rres,  h->d_rows,  h->d_verts, h->d_coef, h->d_f,     h->d_reac,  h->d_slo,   h->d_F,
+  void* ptrs[] = {h->d_pb,    h->d_conv,  h->d_rres,  h->d_rows,  h->d_verts, h->d_coef, h->d_f,     h->d_reac,  h->d_slo,   h->d_F,
                   h->d_rhs,   h->d_L,    h->d_W,     h->d_X,     h->d_psi,   h->d_M,
                   h->d_b,     h->d_status, h->d_inode, h->d_imask, h->d_cK,  h->d_crhs,
                   h->d_cvec,  h->d_cit};
@@ -2581,7 +2705,10 @@ msfec_status msfec_q_create(const msfec_q_params* params, int n_cells, const dou
   TRY(dalloc(h, &h->d_coef, h->coef_count));
   TRY(dalloc(h, &h->d_f, h->rhs_count));
   TRY(dalloc(h, &h->d_reac, h->reac_count));
-  if (h->fast) TRY(dalloc(h, &h->d_rows, (size_t)n_cells * c.n * kRowStride));
+  if (h->fast) {
+    TRY(dalloc(h, &h->d_rows, (size_t)n_cells * c.n * kRowStride));
+    TRY(dalloc(h, &h->d_pb, (size_t)n_cells * kPStride));
+  }
   if (p.solver == MSFEC_SOLVER_BAND_MIXED) {
     TRY(dalloc(h, &h->d_rres, (size_t)n_cells * c.n * kMaxNb));
     TRY(dalloc(h, &h->d_conv, (size_t)n_cells));
@@ -2742,6 +2869,7 @@ msfec_status msfec_q_compute(msfec_q_handle h, void* stream) {
   a.zst = h->d_W;
   a.Rres = h->d_rres;
   a.conv = h->d_conv;
+  a.Pb = h->d_pb;
   a.last_pass = 0;
   a.Wf = h->d_W;
   a.X = h->d_X;
@@ -2778,8 +2906,9 @@ msfec_status msfec_q_compute(msfec_q_handle h, void* stream) {
 #define LAUNCH_BAND(K, D, F) hipLaunchKernelGGL((K<D, F>), dim3(h->n_cells), dim3(64), dyn, s, a)
   if (mixed) {
     // FP32 factor and first solution, FP64 residual, FP32 correction sweep
-    const long long nth = (long long)h->n_cells * c.n * kMaxNb;
-    const unsigned nblk = (unsigned)((nth + 255) / 256);
+    // X of the cell in LDS, and its lower stencil too when both fit in 64 KB
+    const int stage_lo = ((size_t)c.n * (kMaxNb + c.nlow) * sizeof(double) <= 65536) ? 1 : 0;
+    const size_t rsm = (size_t)c.n * (kMaxNb + (stage_lo ? c.nlow : 0)) * sizeof(double);
     if (c.dim == 3)
       hipLaunchKernelGGL((q_blk_factor_kernel<float, 3, BK_X0>), dim3(h->n_cells), dim3(64), 0, s, a);
     else
@@ -2790,28 +2919,24 @@ msfec_status msfec_q_compute(msfec_q_handle h, void* stream) {
     for (int pass = 1; pass <= passes; ++pass) {
       a.last_pass = (pass == passes) ? 1 : 0;
       if (c.dim == 3)
-        hipLaunchKernelGGL((q_resid_kernel<3>), dim3(nblk), dim3(256), 0, s, a, h->n_cells);
+        hipLaunchKernelGGL((q_resid_kernel<3>), dim3(h->n_cells), dim3(256), rsm, s, a, stage_lo);
       else
-        hipLaunchKernelGGL((q_resid_kernel<2>), dim3(nblk), dim3(256), 0, s, a, h->n_cells);
+        hipLaunchKernelGGL((q_resid_kernel<2>), dim3(h->n_cells), dim3(256), rsm, s, a, stage_lo);
       if (tm && pass == 1) MSFEC_HIP(hipEventRecord(ev[3], s));
       if (c.dim == 3)
-        hipLaunchKernelGGL((q_blk_refine_kernel<3>), dim3(h->n_cells), dim3(64), dyn, s, a);
+        hipLaunchKernelGGL((q_blk_refine_kernel<3>), dim3(h->n_cells), dim3(64), 0, s, a);
       else
-        hipLaunchKernelGGL((q_blk_refine_kernel<2>), dim3(h->n_cells), dim3(64), dyn, s, a);
+        hipLaunchKernelGGL((q_blk_refine_kernel<2>), dim3(h->n_cells), dim3(64), 0, s, a);
     }
   } else if (mfma && fast) {
     if (c.dim == 3)
-      hipLaunchKernelGGL((q_blk_factor_kernel<double, 3, BK_FINAL64>), dim3(h->n_cells), dim3(64), dyn, s, a);
+      hipLaunchKernelGGL((q_blk_factor_kernel<double, 3, BK_FINAL64>), dim3(h->n_cells), dim3(64), 0, s, a);
     else
-      hipLaunchKernelGGL((q_blk_factor_kernel<double, 2, BK_FINAL64>), dim3(h->n_cells), dim3(64), dyn, s, a);
+      hipLaunchKernelGGL((q_blk_factor_kernel<double, 2, BK_FINAL64>), dim3(h->n_cells), dim3(64), 0, s, a);
   } else if (mfma) {
-    if (c.dim == 3) {
-      if (fast) LAUNCH_BAND(q_band_mfma_kernel, 3, true);
-      else LAUNCH_BAND(q_band_mfma_kernel, 3, false);
-    } else {
-      if (fast) LAUNCH_BAND(q_band_mfma_kernel, 2, true);
-      else LAUNCH_BAND(q_band_mfma_kernel, 2, false);
-    }
+    // tensor coefficients: stencil tables of q_assemble_kernel
+    if (c.dim == 3) LAUNCH_BAND(q_band_mfma_kernel, 3, false);
+    else LAUNCH_BAND(q_band_mfma_kernel, 2, false);
   } else {
     if (c.dim == 3) {
       if (fast) LAUNCH_BAND(q_band_kernel, 3, true);
