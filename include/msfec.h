@@ -184,6 +184,22 @@ msfec_status msfec_coarse_solve(msfec_q_handle h, int nc, double abs_tol,
                                 int u_loc, double *weights, int weights_loc,
                                 int *iterations, double *residual);
 
+/* Adjoint gradient of the coarse misfit J = 1/2 sum over the free coarse
+ * nodes of (u_H - u_target)^2 with respect to the log-coefficient g of every
+ * fine cell (coefficient a = exp(g), MSFEC_COEF_SCALAR_CELL), for a batch laid
+ * out as in msfec_coarse_solve and computed with BAND_FP64_MFMA or BAND_MIXED
+ * (the stored panels give w = K_II^-1 F_I).  u_target: (nc+1)^dim nodal
+ * values (boundary entries ignored); J on the host; grad: n_cells x N^dim in
+ * the layout of the coefficient.  The coarse solves run to rel_tol.
+ *   dJ/dg_e = -a_e vt_e^T Khat_e (ut_e + w_e)
+ * with ut, vt the fine reconstructions of the coarse solution and of the
+ * adjoint K_H v = u_H - u_target, and Khat_e the unit-coefficient element
+ * stiffness.  No reference counterpart: the reference has no adjoint; this is
+ * the differentiation target of the coarse-scale inverse problem. */
+msfec_status msfec_q_adjoint_gradient(msfec_q_handle h, int nc, const double *u_target,
+                                      int target_loc, double rel_tol, int max_iter,
+                                      double *J, double *grad, int grad_loc);
+
 /* Thread-local description of the last error of this thread. */
 const char *msfec_last_error(void);
 const char *msfec_version(void);

@@ -2067,104 +2067,116 @@ __global__ __launch_bounds__(256) void q_resid_kernel(BandArgs a, int stage_lo) 
 // the stored FP32 panels (4 MFMAs per super-step), backward sweep of the
 // correction, x = X + correction, then the element matrix and Psi.
 // ---------------------------------------------------------------------------
-// Per-lane prefetch for one super-step of the refinement sweep: the panel
-// (4 values) and one value of the residual row entering the window.
+// Per-lane prefetch for one super-step of a sweep with the stored panels:
+// the panel (4 values) and one value of the right-hand-side row entering the
+// window (from a.Rres).
+template <typename R>
 struct RfPre {
-  float pp[4];
-  float rq;
+  R pp[4];
+  R rq;
 };
 
-__device__ __forceinline__ void rf_load(const BandArgs& a, int cell, int q, RfPre& f) {
+template <typename R>
+__device__ __forceinline__ void rf_load(const BandArgs& a, int cell, int q, RfPre<R>& f) {
   const int lane = threadIdx.x, lh = lane >> 4, ll = lane & 15;
   const int n = a.n;
   if (4 * q >= n) return;
-  const float* Pn = reinterpret_cast<const float*>(a.pan) + ((size_t)cell * a.nq + q) * 256;
+  const R* Pn = reinterpret_cast<const R*>(a.pan) + ((size_t)cell * a.nq + q) * 256;
 #pragma unroll
   for (int u = 0; u < 4; ++u) f.pp[u] = Pn[lane * 4 + u];
   const int jn = 4 * q + kWin + lh;
-  const float rv = (float)a.Rres[((size_t)cell * n + min(jn, n - 1)) * kMaxNb + (ll & 7)];
-  f.rq = (ll < 8 && jn < n) ? rv : 0.0f;
+  const R rv = (R)a.Rres[((size_t)cell * n + min(jn, n - 1)) * kMaxNb + (ll & 7)];
+  f.rq = (ll < 8 && jn < n) ? rv : (R)0;
 }
 
-template <int DIM, int T, int G>
+// One super-step of the forward elimination of a.Rres with the stored
+// panels: Z = C^-1 y_pivot, Y_P -= V_P Z (4 MFMAs), Z to HBM, next rows in.
+template <typename R, int DIM, int T, int G>
 __device__ __forceinline__ void rf_step(const BandArgs& a, int cell, int j0,
-                                        typename Mf<float>::v4 (&Yt)[4], BkSmem<float>& ms,
-                                        RfPre& f) {
-  typedef Mf<float> M;
+                                        typename Mf<R>::v4 (&Yt)[4], BkSmem<R>& ms,
+                                        RfPre<R>& f) {
+  typedef Mf<R> M;
   const int lane = bk_lane(), lh = lane >> 4, ll = lane & 15, cs = M::cslot(ll);
-  float* pb = ms.piv;  // [4][68]
+  R* pb = ms.piv;  // [4][68]
   const int qs = j0 >> 2;
   // panel of this super-step and the pivot right-hand sides to LDS
 #pragma unroll
   for (int u = 0; u < 4; ++u) pb[lh * kPanRow + ll * 4 + u] = f.pp[u];
   ms.yp[lh * 16 + ll] = Yt[T][G];
-  const float rin = f.rq;
+  const R rin = f.rq;
   wave_sync();
-  rf_load(a, cell, qs + 4, f);  // 4 super-steps ahead
+  rf_load<R>(a, cell, qs + 4, f);  // 4 super-steps ahead
   // Z = C^-1 y_pivot by forward substitution (C (t,u) at pb[u * 68 + t],
   // 1 / C[t][t] at pb[t * 68 + t]); lane (k = lh, rhs ll) keeps row lh
-  const float y0 = ms.yp[0 * 16 + ll], y1 = ms.yp[1 * 16 + ll];
-  const float y2 = ms.yp[2 * 16 + ll], y3 = ms.yp[3 * 16 + ll];
-  const float z0 = y0 * pb[0 * kPanRow + 0];
-  const float z1 = (y1 - pb[0 * kPanRow + 1] * z0) * pb[1 * kPanRow + 1];
-  const float z2 = (y2 - pb[0 * kPanRow + 2] * z0 - pb[1 * kPanRow + 2] * z1) * pb[2 * kPanRow + 2];
-  const float z3 = (y3 - pb[0 * kPanRow + 3] * z0 - pb[1 * kPanRow + 3] * z1 -
-                    pb[2 * kPanRow + 3] * z2) *
-                   pb[3 * kPanRow + 3];
-  const float Z = (lh == 0) ? z0 : (lh == 1) ? z1 : (lh == 2) ? z2 : z3;
-  float nV[4];
+  const R y0 = ms.yp[0 * 16 + ll], y1 = ms.yp[1 * 16 + ll];
+  const R y2 = ms.yp[2 * 16 + ll], y3 = ms.yp[3 * 16 + ll];
+  const R z0 = y0 * pb[0 * kPanRow + 0];
+  const R z1 = (y1 - pb[0 * kPanRow + 1] * z0) * pb[1 * kPanRow + 1];
+  const R z2 = (y2 - pb[0 * kPanRow + 2] * z0 - pb[1 * kPanRow + 2] * z1) * pb[2 * kPanRow + 2];
+  const R z3 = (y3 - pb[0 * kPanRow + 3] * z0 - pb[1 * kPanRow + 3] * z1 -
+                pb[2 * kPanRow + 3] * z2) *
+               pb[3 * kPanRow + 3];
+  const R Z = (lh == 0) ? z0 : (lh == 1) ? z1 : (lh == 2) ? z2 : z3;
+  R nV[4];
   const int jr = bk_opaque(j0);  // = s0 mod 64
 #pragma unroll
   for (int P = 0; P < 4; ++P) nV[P] = -pb[lh * kPanRow + ((16 * P + cs - jr) & (kWin - 1))];
 #pragma unroll
   for (int P = 0; P < 4; ++P) Yt[P] = M::mma(nV[P], Z, Yt[P]);
-  if (ll < 8) reinterpret_cast<float*>(a.zst)[((size_t)cell * a.nq + qs) * 32 + lh * 8 + ll] = Z;
+  if (ll < 8) reinterpret_cast<R*>(a.zst)[((size_t)cell * a.nq + qs) * 32 + lh * 8 + ll] = Z;
   // the next rows of the band enter the pivot slots
   Yt[T][G] = rin;
   wave_sync();
 }
 
-template <int DIM, int T>
+template <typename R, int DIM, int T>
 __device__ __forceinline__ void rf_tile_row(const BandArgs& a, int cell, int j0,
-                                            typename Mf<float>::v4 (&Yt)[4], BkSmem<float>& ms,
-                                            RfPre (&f)[4]) {
+                                            typename Mf<R>::v4 (&Yt)[4], BkSmem<R>& ms,
+                                            RfPre<R> (&f)[4]) {
   const int n = a.n;
-  rf_step<DIM, T, 0>(a, cell, j0, Yt, ms, f[0]);
-  if (j0 + 4 < n) rf_step<DIM, T, 1>(a, cell, j0 + 4, Yt, ms, f[1]);
-  if (j0 + 8 < n) rf_step<DIM, T, 2>(a, cell, j0 + 8, Yt, ms, f[2]);
-  if (j0 + 12 < n) rf_step<DIM, T, 3>(a, cell, j0 + 12, Yt, ms, f[3]);
+  rf_step<R, DIM, T, 0>(a, cell, j0, Yt, ms, f[0]);
+  if (j0 + 4 < n) rf_step<R, DIM, T, 1>(a, cell, j0 + 4, Yt, ms, f[1]);
+  if (j0 + 8 < n) rf_step<R, DIM, T, 2>(a, cell, j0 + 8, Yt, ms, f[2]);
+  if (j0 + 12 < n) rf_step<R, DIM, T, 3>(a, cell, j0 + 12, Yt, ms, f[3]);
 }
 
-template <int DIM>
-__global__ __launch_bounds__(64, 2) void q_blk_refine_kernel(BandArgs a) {
-  typedef typename Mf<float>::v4 V4;
-  const int cell = blockIdx.x;
+// Forward sweep of a.Rres over all super-steps of the cell.
+template <typename R, int DIM>
+__device__ __forceinline__ void rf_forward(const BandArgs& a, int cell, BkSmem<R>& ms) {
+  typedef typename Mf<R>::v4 V4;
   const int lane = threadIdx.x, lh = lane >> 4, ll = lane & 15;
-  if (a.conv[cell]) return;  // converged or failed: the whole wave leaves
-  __shared__ BandSmem sh;
-  __shared__ BkSmem<float> ms;
-  RfPre pre[4];
-#pragma unroll
-  for (int u = 0; u < 4; ++u) rf_load(a, cell, u, pre[u]);
   const int n = a.n;
+  RfPre<R> pre[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) rf_load<R>(a, cell, u, pre[u]);
   V4 Yt[4];
-  // rows 0..63 of the residual: slot 4g + lh of tile row P is register g
+  // rows 0..63: slot 4g + lh of tile row P is register g
 #pragma unroll
   for (int P = 0; P < 4; ++P)
 #pragma unroll
     for (int g = 0; g < 4; ++g) {
       const int j = 16 * P + 4 * g + lh;
-      const float rv = (float)a.Rres[((size_t)cell * n + min(j, n - 1)) * kMaxNb + (ll & 7)];
-      Yt[P][g] = (ll < 8 && j < n) ? rv : 0.0f;
+      const R rv = (R)a.Rres[((size_t)cell * n + min(j, n - 1)) * kMaxNb + (ll & 7)];
+      Yt[P][g] = (ll < 8 && j < n) ? rv : (R)0;
     }
-  for (int t = lane; t < 4 * kPanRow; t += 64) ms.piv[t] = 0.0f;
+  for (int t = lane; t < 4 * kPanRow; t += 64) ms.piv[t] = (R)0;
   wave_sync();
   for (int j0 = 0; j0 < n; j0 += 64) {
-    rf_tile_row<DIM, 0>(a, cell, j0, Yt, ms, pre);
-    if (j0 + 16 < n) rf_tile_row<DIM, 1>(a, cell, j0 + 16, Yt, ms, pre);
-    if (j0 + 32 < n) rf_tile_row<DIM, 2>(a, cell, j0 + 32, Yt, ms, pre);
-    if (j0 + 48 < n) rf_tile_row<DIM, 3>(a, cell, j0 + 48, Yt, ms, pre);
+    rf_tile_row<R, DIM, 0>(a, cell, j0, Yt, ms, pre);
+    if (j0 + 16 < n) rf_tile_row<R, DIM, 1>(a, cell, j0 + 16, Yt, ms, pre);
+    if (j0 + 32 < n) rf_tile_row<R, DIM, 2>(a, cell, j0 + 32, Yt, ms, pre);
+    if (j0 + 48 < n) rf_tile_row<R, DIM, 3>(a, cell, j0 + 48, Yt, ms, pre);
   }
+}
+
+template <int DIM>
+__global__ __launch_bounds__(64, 2) void q_blk_refine_kernel(BandArgs a) {
+  const int cell = blockIdx.x;
+  const int lane = threadIdx.x;
+  if (a.conv[cell]) return;  // converged or failed: the whole wave leaves
+  __shared__ BandSmem sh;
+  __shared__ BkSmem<float> ms;
+  rf_forward<float, DIM>(a, cell, ms);
   __threadfence_block();
   __syncthreads();
   double bint = 0.0, sacc = 0.0, dmax = 0.0, xmax = 0.0;
@@ -2186,6 +2198,144 @@ __global__ __launch_bounds__(64, 2) void q_blk_refine_kernel(BandArgs a) {
     }
   }
   fast_finish<DIM>(a, cell, sacc, bint);
+}
+
+// ---------------------------------------------------------------------------
+// Solve K_II x = a.Rres (8 columns) with the panels stored by the last
+// factorisation (FP64 or FP32), x to a.X.  Used by the adjoint for
+// w = K_II^{-1} F_I.
+// ---------------------------------------------------------------------------
+template <typename R, int DIM>
+__global__ __launch_bounds__(64, 2) void q_blk_solve_kernel(BandArgs a) {
+  const int cell = blockIdx.x;
+  __shared__ BandSmem sh;
+  __shared__ BkSmem<R> ms;
+  rf_forward<R, DIM>(a, cell, ms);
+  __threadfence_block();
+  __syncthreads();
+  double bint = 0.0, sacc = 0.0, dmax = 0.0, xmax = 0.0;
+  bk_backward<R, DIM, BK_X0>(a, cell, sh, ms, bint, sacc, dmax, xmax);
+}
+
+// Right-hand side of w: column 0 = F_I (interior load), columns 1..7 = 0.
+__global__ void q_adj_rhs_kernel(BandArgs a, int n_cells) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long long)n_cells * a.n * kMaxNb) return;
+  const int r = (int)(i & (kMaxNb - 1));
+  const long long cj = i >> 3;
+  const int cell = (int)(cj / a.n);
+  const double fj = a.Rows[(size_t)cj * kRowStride + 14] * a.Pb[(size_t)cell * kPStride + 72];
+  a.Rres[i] = (r == 0) ? fj : 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint gradient of J = 1/2 |u_H - u_target|^2 (free coarse nodes) with
+// respect to the log-coefficient g (a = exp(g)) of every fine cell:
+//   dJ/dg_e = -a_e vt_e^T Khat_e (ut_e + w_e),
+// ut, vt the fine reconstructions of the coarse solution u and adjoint v,
+// w = K_II^{-1} F_I (column 0 of a.X, zero on the cell boundary) and Khat_e
+// the unit-coefficient element stiffness of fine cell e.  One block per
+// coarse cell: the nodal fields go to LDS, then one thread per fine cell.
+// ---------------------------------------------------------------------------
+template <int DIM>
+__global__ __launch_bounds__(256) void q_adj_grad_kernel(BandArgs a, int nc,
+                                                         const double* __restrict__ u,
+                                                         const double* __restrict__ v,
+                                                         double* __restrict__ grad) {
+  using D = Dirs<DIM>;
+  const int cell = blockIdx.x;
+  extern __shared__ double fld[];  // [nn] ut + w | [nn] vt
+  double* su = fld;
+  double* sv = fld + a.nn;
+  __shared__ double kvu[8];
+  double uk[D::NV], vk[D::NV];
+  {
+    int cijk[3] = {0, 0, 0}, r = cell;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+      cijk[d] = r % nc;
+      r /= nc;
+    }
+#pragma unroll
+    for (int k = 0; k < D::NV; ++k) {
+      int id = 0, mul = 1;
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) {
+        id += (cijk[d] + ((k >> d) & 1)) * mul;
+        mul *= nc + 1;
+      }
+      uk[k] = u[id];
+      vk[k] = v[id];
+    }
+  }
+  if (threadIdx.x < 8) {
+    const double* vt = a.verts + (size_t)cell * D::NV * DIM;
+    double hh[3] = {1.0, 1.0, 1.0};
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) hh[d] = (vt[(D::NV - 1) * DIM + d] - vt[d]) / a.N;
+    double ks, ms;
+    box_tables<DIM>(threadIdx.x, hh, ks, ms);
+    kvu[threadIdx.x] = ks;
+  }
+  const double* psi = a.Psi + (size_t)cell * D::NV * a.nn;
+  for (int node = threadIdx.x; node < a.nn; node += blockDim.x) {
+    double ut = 0.0, vt = 0.0;
+#pragma unroll
+    for (int k = 0; k < D::NV; ++k) {
+      const double p = psi[(size_t)k * a.nn + node];
+      ut = fma(uk[k], p, ut);
+      vt = fma(vk[k], p, vt);
+    }
+    int c[3] = {0, 0, 0}, t = node;
+    bool inner = true;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+      c[d] = t % a.S;
+      t /= a.S;
+      if (c[d] == 0 || c[d] == a.N) inner = false;
+    }
+    if (inner) {
+      int j = 0, mul = 1;
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) {
+        j += (c[d] - 1) * mul;
+        mul *= a.ni;
+      }
+      ut += a.X[((size_t)cell * a.n + j) * kMaxNb];
+    }
+    su[node] = ut;
+    sv[node] = vt;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < a.nfc; e += blockDim.x) {
+    int c[3] = {0, 0, 0}, t = e;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+      c[d] = t % a.N;
+      t /= a.N;
+    }
+    double ue[D::NV], ve[D::NV];
+#pragma unroll
+    for (int al = 0; al < D::NV; ++al) {
+      int nd = 0, mul = 1;
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) {
+        nd += (c[d] + ((al >> d) & 1)) * mul;
+        mul *= a.S;
+      }
+      ue[al] = su[nd];
+      ve[al] = sv[nd];
+    }
+    double s = 0.0;
+#pragma unroll
+    for (int al = 0; al < D::NV; ++al) {
+      double kr = 0.0;
+#pragma unroll
+      for (int be = 0; be < D::NV; ++be) kr = fma(kvu[al ^ be], ue[be], kr);
+      s = fma(ve[al], kr, s);
+    }
+    grad[(size_t)cell * a.nfc + e] = -a.coef[(size_t)cell * a.nfc + e] * s;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -2580,6 +2730,49 @@ msfec_status copy_out(double* dst, const double* src, size_t count, int loc) {
 
 }  // namespace
 
+namespace {
+BandArgs band_args(const msfec_q_batch* h) {
+  const QConf& c = h->conf;
+  BandArgs a{};
+  a.n = c.n;
+  a.ni = c.ni;
+  a.N = c.N;
+  a.S = c.S;
+  a.nn = c.nn;
+  a.nfc = c.nfc;
+  for (int t = 0; t < kMaxLow; ++t) a.off_int[t] = c.off_int[t];
+  a.Rows = h->d_rows;
+  a.Slo = h->d_slo;
+  a.Rhs = h->d_rhs;
+  a.inode = h->d_inode;
+  a.imask = h->d_imask;
+  a.verts = h->d_verts;
+  a.coef = h->d_coef;
+  a.reac = h->reac_count ? h->d_reac : nullptr;
+  a.fcell = h->rhs_count ? h->d_f : nullptr;
+  a.reaction_mode = c.reaction_mode;
+  a.rhs_mode = c.rhs_mode;
+  a.reaction_const = c.reaction_const;
+  a.rhs_const = c.rhs_const;
+  a.Lcol = h->d_L;
+  a.pan = h->d_L;
+  a.zst = h->d_W;
+  a.Rres = h->d_rres;
+  a.conv = h->d_conv;
+  a.Pb = h->d_pb;
+  a.last_pass = 0;
+  a.Wf = h->d_W;
+  a.X = h->d_X;
+  a.Psi = h->d_psi;
+  a.M = h->d_M;
+  a.bvec = h->d_b;
+  a.status = h->d_status;
+  a.nq = (c.n + 3) / 4;
+  return a;
+}
+
+}  // namespace
+
 extern "C" {
 
 const char* msfec_last_error(void) { return g_last_error.c_str(); }
@@ -2843,41 +3036,7 @@ msfec_status msfec_q_compute(msfec_q_handle h, void* stream) {
   const size_t rstride = h->reac_count ? h->reac_count / h->n_cells : 0;
   const bool fast = h->fast;
   if (tm) MSFEC_HIP(hipEventRecord(ev[0], s));
-  BandArgs a{};
-  a.n = c.n;
-  a.ni = c.ni;
-  a.N = c.N;
-  a.S = c.S;
-  a.nn = c.nn;
-  a.nfc = c.nfc;
-  for (int t = 0; t < kMaxLow; ++t) a.off_int[t] = c.off_int[t];
-  a.Rows = h->d_rows;
-  a.Slo = h->d_slo;
-  a.Rhs = h->d_rhs;
-  a.inode = h->d_inode;
-  a.imask = h->d_imask;
-  a.verts = h->d_verts;
-  a.coef = h->d_coef;
-  a.reac = h->reac_count ? h->d_reac : nullptr;
-  a.fcell = h->rhs_count ? h->d_f : nullptr;
-  a.reaction_mode = c.reaction_mode;
-  a.rhs_mode = c.rhs_mode;
-  a.reaction_const = c.reaction_const;
-  a.rhs_const = c.rhs_const;
-  a.Lcol = h->d_L;
-  a.pan = h->d_L;
-  a.zst = h->d_W;
-  a.Rres = h->d_rres;
-  a.conv = h->d_conv;
-  a.Pb = h->d_pb;
-  a.last_pass = 0;
-  a.Wf = h->d_W;
-  a.X = h->d_X;
-  a.Psi = h->d_psi;
-  a.M = h->d_M;
-  a.bvec = h->d_b;
-  a.status = h->d_status;
-  a.nq = (c.n + 3) / 4;
+  BandArgs a = band_args(h);
   if (fast) {
     const size_t rdyn = (size_t)c.nfc *
                         (1 + (c.reaction_mode == MSFEC_REACTION_CELL ? 1 : 0) +
@@ -3107,6 +3266,150 @@ msfec_status msfec_coarse_solve(msfec_q_handle h, int nc, double abs_tol, double
     if (st) return st;
   }
   if (it < 0) return fail(MSFEC_ERR_CELL, "coarse CG did not converge");
+  return MSFEC_OK;
+}
+
+msfec_status msfec_q_adjoint_gradient(msfec_q_handle h, int nc, const double* u_target,
+                                      int target_loc, double rel_tol, int max_iter, double* J,
+                                      double* grad, int grad_loc) {
+  if (!h || !u_target || !J || !grad) return fail(MSFEC_ERR_INVALID, "null argument");
+  if (!h->computed) return fail(MSFEC_ERR_STATE, "compute not called");
+  if (!h->fast || h->conf.coef_mode != MSFEC_COEF_SCALAR_CELL)
+    return fail(MSFEC_ERR_UNSUPPORTED, "the adjoint needs a scalar coefficient per fine cell");
+  if (h->p.solver != MSFEC_SOLVER_BAND_FP64_MFMA && h->p.solver != MSFEC_SOLVER_BAND_MIXED)
+    return fail(MSFEC_ERR_UNSUPPORTED,
+                "the adjoint reuses the stored panels of BAND_FP64_MFMA or BAND_MIXED");
+  const QConf& c = h->conf;
+  const int dim = c.dim, nv = c.nv;
+  if (nc <= 0 || ipow(nc, dim) != h->n_cells)
+    return fail(MSFEC_ERR_INVALID, "nc^dim must equal the number of cells of the batch");
+  hipStream_t s = h->stream;
+  MSFEC_HIP(hipStreamSynchronize(s));
+  {
+    std::vector<int> st(h->n_cells);
+    MSFEC_HIP(hipMemcpy(st.data(), h->d_status, st.size() * sizeof(int), hipMemcpyDeviceToHost));
+    for (int v : st)
+      if (v != MSFEC_CELL_OK) return fail(MSFEC_ERR_CELL, "a local solve failed; no gradient");
+  }
+  const int nnode = ipow(nc + 1, dim), nst = ipow(3, dim);
+  if (h->coarse_nc != nc) {
+    if (h->d_cK) (void)hipFree(h->d_cK);
+    if (h->d_crhs) (void)hipFree(h->d_crhs);
+    if (h->d_cvec) (void)hipFree(h->d_cvec);
+    if (h->d_cit) (void)hipFree(h->d_cit);
+    h->d_cK = h->d_crhs = h->d_cvec = nullptr;
+    h->d_cit = nullptr;
+    MSFEC_HIP(hipMalloc((void**)&h->d_cK, (size_t)nnode * nst * sizeof(double)));
+    MSFEC_HIP(hipMalloc((void**)&h->d_crhs, (size_t)nnode * sizeof(double)));
+    MSFEC_HIP(hipMalloc((void**)&h->d_cvec, ((size_t)5 * nnode + 1 + (size_t)h->n_cells * nv) * sizeof(double)));
+    MSFEC_HIP(hipMalloc((void**)&h->d_cit, sizeof(int)));
+    h->coarse_nc = nc;
+  }
+  double* du = h->d_cvec;
+  double* r = du + nnode;
+  double* z = r + nnode;
+  double* pv = z + nnode;
+  double* qv = pv + nnode;
+  double* res = qv + nnode;
+  // buffers of the adjoint: v, right-hand sides of w, the gradient (device)
+  double *dv = nullptr, *dgrad = nullptr;
+  MSFEC_HIP(hipMalloc((void**)&dv, (size_t)nnode * sizeof(double)));
+  if (!h->d_rres) {
+    if (hipMalloc((void**)&h->d_rres, (size_t)h->n_cells * c.n * kMaxNb * sizeof(double)) !=
+        hipSuccess) {
+      (void)hipFree(dv);
+      return fail(MSFEC_ERR_HIP, "allocation of the adjoint right-hand sides failed");
+    }
+  }
+  const size_t ngrad = (size_t)h->n_cells * c.nfc;
+  if (grad_loc == MSFEC_DEVICE) {
+    dgrad = grad;
+  } else if (hipMalloc((void**)&dgrad, ngrad * sizeof(double)) != hipSuccess) {
+    (void)hipFree(dv);
+    return fail(MSFEC_ERR_HIP, "allocation of the gradient failed");
+  }
+  auto cleanup = [&]() {
+    (void)hipFree(dv);
+    if (grad_loc != MSFEC_DEVICE) (void)hipFree(dgrad);
+  };
+#define ADJ_HIP(call)                                 \
+  do {                                                \
+    hipError_t e_ = (call);                           \
+    if (e_ != hipSuccess) {                           \
+      cleanup();                                      \
+      return fail(MSFEC_ERR_HIP, hipGetErrorString(e_)); \
+    }                                                 \
+  } while (0)
+  // forward coarse problem
+  hipLaunchKernelGGL(coarse_assemble_kernel, dim3((nnode + 255) / 256), dim3(256), 0, s, dim, nc,
+                     h->d_M, h->d_b, h->d_cK, h->d_crhs);
+  ADJ_HIP(hipGetLastError());
+  hipLaunchKernelGGL(coarse_cg_kernel, dim3(1), dim3(1024), 0, s, dim, nc, h->d_cK, h->d_crhs, du,
+                     r, z, pv, qv, 0.0, rel_tol, max_iter, h->d_cit, res);
+  ADJ_HIP(hipGetLastError());
+  int it = 0;
+  ADJ_HIP(hipMemcpyAsync(&it, h->d_cit, sizeof(int), hipMemcpyDeviceToHost, s));
+  std::vector<double> hu(nnode), ht(nnode), hd(nnode);
+  ADJ_HIP(hipMemcpyAsync(hu.data(), du, nnode * sizeof(double), hipMemcpyDeviceToHost, s));
+  ADJ_HIP(hipStreamSynchronize(s));
+  if (it < 0) {
+    cleanup();
+    return fail(MSFEC_ERR_CELL, "coarse CG did not converge");
+  }
+  if (target_loc == MSFEC_DEVICE) {
+    ADJ_HIP(hipMemcpy(ht.data(), u_target, nnode * sizeof(double), hipMemcpyDeviceToHost));
+  } else {
+    std::memcpy(ht.data(), u_target, nnode * sizeof(double));
+  }
+  // J and the adjoint right-hand side u_H - u_target on the free nodes
+  double jsum = 0.0;
+  for (int n = 0; n < nnode; ++n) {
+    bool bnd = false;
+    int t = n;
+    for (int d = 0; d < dim; ++d) {
+      const int cc = t % (nc + 1);
+      t /= nc + 1;
+      if (cc == 0 || cc == nc) bnd = true;
+    }
+    hd[n] = bnd ? 0.0 : hu[n] - ht[n];
+    jsum += 0.5 * hd[n] * hd[n];
+  }
+  *J = jsum;
+  ADJ_HIP(hipMemcpyAsync(h->d_crhs, hd.data(), nnode * sizeof(double), hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(coarse_cg_kernel, dim3(1), dim3(1024), 0, s, dim, nc, h->d_cK, h->d_crhs, dv,
+                     r, z, pv, qv, 0.0, rel_tol, max_iter, h->d_cit, res);
+  ADJ_HIP(hipGetLastError());
+  // w = K_II^{-1} F_I with the stored panels
+  BandArgs a = band_args(h);
+  a.Rres = h->d_rres;
+  {
+    const long long nth = (long long)h->n_cells * c.n * kMaxNb;
+    hipLaunchKernelGGL(q_adj_rhs_kernel, dim3((unsigned)((nth + 255) / 256)), dim3(256), 0, s, a,
+                       h->n_cells);
+    ADJ_HIP(hipGetLastError());
+  }
+  const bool f64 = (h->p.solver == MSFEC_SOLVER_BAND_FP64_MFMA);
+  if (dim == 3) {
+    if (f64) hipLaunchKernelGGL((q_blk_solve_kernel<double, 3>), dim3(h->n_cells), dim3(64), 0, s, a);
+    else hipLaunchKernelGGL((q_blk_solve_kernel<float, 3>), dim3(h->n_cells), dim3(64), 0, s, a);
+  } else {
+    if (f64) hipLaunchKernelGGL((q_blk_solve_kernel<double, 2>), dim3(h->n_cells), dim3(64), 0, s, a);
+    else hipLaunchKernelGGL((q_blk_solve_kernel<float, 2>), dim3(h->n_cells), dim3(64), 0, s, a);
+  }
+  ADJ_HIP(hipGetLastError());
+  const size_t gsm = (size_t)2 * c.nn * sizeof(double);
+  if (dim == 3)
+    hipLaunchKernelGGL((q_adj_grad_kernel<3>), dim3(h->n_cells), dim3(256), gsm, s, a, nc, du, dv, dgrad);
+  else
+    hipLaunchKernelGGL((q_adj_grad_kernel<2>), dim3(h->n_cells), dim3(256), gsm, s, a, nc, du, dv, dgrad);
+  ADJ_HIP(hipGetLastError());
+  ADJ_HIP(hipMemcpyAsync(&it, h->d_cit, sizeof(int), hipMemcpyDeviceToHost, s));
+  if (grad_loc != MSFEC_DEVICE)
+    ADJ_HIP(hipMemcpyAsync(grad, dgrad, ngrad * sizeof(double), hipMemcpyDeviceToHost, s));
+  ADJ_HIP(hipStreamSynchronize(s));
+#undef ADJ_HIP
+  cleanup();
+  if (it < 0) return fail(MSFEC_ERR_CELL, "adjoint coarse CG did not converge");
   return MSFEC_OK;
 }
 

@@ -28,8 +28,8 @@ EXPORTED = (
     "msfec_q_create", "msfec_q_set_coefficient", "msfec_q_set_rhs", "msfec_q_set_reaction",
     "msfec_q_compute", "msfec_q_synchronize", "msfec_q_get_element", "msfec_q_get_basis",
     "msfec_q_get_cell_status", "msfec_q_reconstruct", "msfec_q_get_info",
-    "msfec_q_kernel_times", "msfec_q_destroy", "msfec_coarse_solve", "msfec_last_error",
-    "msfec_version",
+    "msfec_q_kernel_times", "msfec_q_destroy", "msfec_coarse_solve", "msfec_q_adjoint_gradient",
+    "msfec_last_error", "msfec_version",
 )
 
 
@@ -103,6 +103,8 @@ def load(path: str | None = None):
     L.msfec_q_destroy.restype = None
     L.msfec_coarse_solve.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                      ctypes.c_int, vp, ctypes.c_int, vp, ctypes.c_int, ip, dp]
+    L.msfec_q_adjoint_gradient.argtypes = [vp, ctypes.c_int, vp, ctypes.c_int, ctypes.c_double,
+                                           ctypes.c_int, dp, vp, ctypes.c_int]
     L.msfec_last_error.restype = ctypes.c_char_p
     L.msfec_version.restype = ctypes.c_char_p
     for name in EXPORTED:

@@ -121,6 +121,19 @@ class QBasisBatch:
                                              ctypes.byref(it), ctypes.byref(res)))
         return u, w, it.value, res.value
 
+    def adjoint_gradient(self, nc, u_target, rel_tol=1e-14, max_iter=100000):
+        """(J, dJ/dg) for J = 1/2 |u_H - u_target|^2 over the free coarse nodes
+        and the log-coefficient g of every fine cell (a = exp(g)); the batch
+        must have been computed (run()) with solver BAND_FP64_MFMA or
+        BAND_MIXED on the nc^dim structured grid of coarse_solve."""
+        ut = np.ascontiguousarray(u_target, dtype=np.float64)
+        grad = np.empty((self.n_cells, self.info.n_fine_cells))
+        J = ctypes.c_double(0.0)
+        L.check(self._lib.msfec_q_adjoint_gradient(
+            self._h, nc, ctypes.c_void_p(ut.ctypes.data), L.HOST, rel_tol, max_iter,
+            ctypes.byref(J), ctypes.c_void_p(grad.ctypes.data), L.HOST))
+        return J.value, grad
+
     def kernel_times(self):
         ms = (ctypes.c_float * 8)()
         n = ctypes.c_int(0)

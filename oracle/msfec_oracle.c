@@ -340,11 +340,15 @@ static void stencil_apply(const oq_fine *m, const double *x, double *y) {
  * q_basis.cc:304-330: UMFPACK of the condensed matrix yields the same
  * interior solution K_II x_I = -K_IB g_B). Pinned by scipy cho_solve golden
  * vectors and the A = aI closed form psi_i = phi_i. */
-static int solve_direct(const oq_fine *m, const double *g, double *Psi) {
+/* ... and, when w != NULL, w = K_II^{-1} F_I on the nodes (0 on the boundary),
+ * the extra solve of the adjoint (the derivative of b = Psi^T F) */
+static int solve_direct_w(const oq_fine *m, const double *g, double *Psi, double *w) {
   const int dim = m->dim, ni = m->N - 1, nb = m->nv;
   const int n = ipow(ni, dim);
   if (n == 0) {
     for (int k = 0; k < nb; ++k) memcpy(Psi + (size_t)k * m->nn, g + (size_t)k * m->nn, sizeof(double) * m->nn);
+    if (w)
+      for (int v = 0; v < m->nn; ++v) w[v] = 0.0;
     return 0;
   }
   int *inode = (int *)malloc(sizeof(int) * n);
@@ -418,6 +422,22 @@ static int solve_direct(const oq_fine *m, const double *g, double *Psi) {
       /* distribute: boundary entries = g, interior = x */
       for (int v = 0; v < m->nn; ++v)
         Psi[(size_t)k * m->nn + v] = (iidx[v] >= 0) ? x[iidx[v]] : g[(size_t)k * m->nn + v];
+    }
+    if (w) {
+      double *x = (double *)malloc(sizeof(double) * n);
+      for (int i = 0; i < n; ++i) x[i] = m->F[inode[i]];
+      for (int i = 0; i < n; ++i) {
+        double t = x[i];
+        for (int c = 0; c < i; ++c) t -= L[(size_t)i * n + c] * x[c];
+        x[i] = t / L[(size_t)i * n + i];
+      }
+      for (int i = n - 1; i >= 0; --i) {
+        double t = x[i];
+        for (int r = i + 1; r < n; ++r) t -= L[(size_t)r * n + i] * x[r];
+        x[i] = t / L[(size_t)i * n + i];
+      }
+      for (int v = 0; v < m->nn; ++v) w[v] = (iidx[v] >= 0) ? x[iidx[v]] : 0.0;
+      free(x);
     }
   }
   free(inode);
@@ -577,9 +597,19 @@ static int solve_cg_ssor(const oq_problem *p, const oq_fine *m, const double *g,
   return status;
 }
 
+static int basis_cell(const oq_problem *p, const double *verts, const double *coef,
+                      const double *f, const double *reac, int solver, double *M,
+                      double *b, double *Psi, int *iters, double *w);
+
 int oq_basis_cell(const oq_problem *p, const double *verts, const double *coef,
                   const double *f, const double *reac, int solver, double *M,
                   double *b, double *Psi, int *iters) {
+  return basis_cell(p, verts, coef, f, reac, solver, M, b, Psi, iters, NULL);
+}
+
+static int basis_cell(const oq_problem *p, const double *verts, const double *coef,
+                      const double *f, const double *reac, int solver, double *M,
+                      double *b, double *Psi, int *iters, double *w) {
   const int dim = p->dim;
   if (dim != 2 && dim != 3) return -1;
   if (p->n_refine < 0 || p->n_refine > 6) return -1;
@@ -629,7 +659,7 @@ int oq_basis_cell(const oq_problem *p, const double *verts, const double *coef,
   }
   if (status == 0) {
     if (iters) *iters = 0;
-    if (solver == OQ_SOLVER_DIRECT) status = solve_direct(&m, g, Psi);
+    if (solver == OQ_SOLVER_DIRECT) status = solve_direct_w(&m, g, Psi, w);
     else status = solve_cg_ssor(p, &m, g, Psi, iters);
   }
   if (status == 0 || status == 1) {
@@ -724,8 +754,23 @@ static int coarse_is_boundary(int dim, int nc, int node) {
   return 0;
 }
 
+static int coarse_cg(int dim, int nc, const double *M, const double *b,
+                     const double *rhs_nodes, double rtol, int maxit, double *u);
+
 int oq_coarse_solve(int dim, int nc, const double *M, const double *b,
                     double rtol, int maxit, double *u) {
+  return coarse_cg(dim, nc, M, b, NULL, rtol, maxit, u);
+}
+
+int oq_coarse_solve_rhs(int dim, int nc, const double *M, const double *rhs_nodes,
+                        double rtol, int maxit, double *u) {
+  return coarse_cg(dim, nc, M, NULL, rhs_nodes, rtol, maxit, u);
+}
+
+/* assembly with homogeneous Dirichlet constraints and Jacobi-PCG; the
+ * right-hand side is assembled from b, or taken nodewise from rhs_nodes */
+static int coarse_cg(int dim, int nc, const double *M, const double *b,
+                     const double *rhs_nodes, double rtol, int maxit, double *u) {
   const int nv = 1 << dim, ncell = ipow(nc, dim), nn = ipow(nc + 1, dim);
   const int nst = ipow(3, dim);
   double *K = (double *)calloc((size_t)nn * nst, sizeof(double));
@@ -756,9 +801,11 @@ int oq_coarse_solve(int dim, int nc, const double *M, const double *b,
         for (int d = 0; d < dim; ++d) dd[d] = ((bb >> d) & 1) - ((a >> d) & 1);
         K[(size_t)nodes[a] * nst + dir_index(dim, dd)] += M[((size_t)c * nv + a) * nv + bb];
       }
-      rhs[nodes[a]] += b[(size_t)c * nv + a];
+      if (b) rhs[nodes[a]] += b[(size_t)c * nv + a];
     }
   }
+  if (rhs_nodes)
+    for (int n = 0; n < nn; ++n) rhs[n] = rhs_nodes[n];
   const int center = (nst - 1) / 2;
   double bnorm2 = 0.0, rnorm2 = 0.0;
   for (int n = 0; n < nn; ++n) {
@@ -839,4 +886,150 @@ void oq_reconstruct(int dim, int nc, int n_refine, const double *u_coarse,
       for (int n = 0; n < nn; ++n) uf[n] = uf[n] + w * pk[n];
     }
   }
+}
+
+/* ---------------------------------------------------------------------------
+ * Adjoint gradient of the coarse misfit (no reference counterpart: the
+ * differentiation target of the MI355X build).  Scalar coefficient a = exp(g)
+ * per fine cell, structured nc^dim coarse grid as in oq_coarse_solve.
+ *   J(g) = 1/2 sum over free coarse nodes (u_H - u_target)^2,
+ *   K_H v = u_H - u_target (adjoint, same constraints),
+ *   dJ/da_e = -vt_e^T Khat_e (ut_e + w_e),  dJ/dg_e = a_e dJ/da_e,
+ * with ut = sum_k u_k psi_k and vt = sum_k v_k psi_k the fine reconstructions
+ * of the coarse solution and adjoint, w = K_II^{-1} F_I (zero on the cell
+ * boundary) and Khat_e the fine element stiffness of a unit coefficient.
+ * (dM_ij/da_e = psi_i^T Khat_e psi_j because the bases are K-harmonic, and
+ * db_i/da_e = -w^T Khat_e psi_i.)  Pinned by central finite differences of
+ * oq_misfit (tests/test_oracle.py).
+ * ------------------------------------------------------------------------- */
+static int coarse_all(const oq_problem *p, int nc, const double *verts, const double *coef,
+                      const double *f, double *M, double *b, double *Psi, double *W) {
+  const int dim = p->dim, nv = 1 << dim, N = 1 << p->n_refine;
+  const int nfc = ipow(N, dim), nn = ipow(N + 1, dim), ncell = ipow(nc, dim);
+  int worst = 0;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 1) reduction(max : worst)
+#endif
+  for (int c = 0; c < ncell; ++c) {
+    int it = 0;
+    const int st = basis_cell(p, verts + (size_t)c * nv * dim, coef + (size_t)c * nfc,
+                              (p->rhs_mode == OQ_RHS_CELL && f) ? f + (size_t)c * nfc : NULL,
+                              NULL, OQ_SOLVER_DIRECT, M + (size_t)c * nv * nv,
+                              b + (size_t)c * nv, Psi + (size_t)c * nv * nn, &it,
+                              W ? W + (size_t)c * nn : NULL);
+    if (st > worst) worst = st;
+  }
+  return worst;
+}
+
+int oq_misfit(const oq_problem *p, int nc, const double *verts, const double *coef,
+              const double *f, const double *u_target, double rtol, double *J, double *u) {
+  if (p->coef_mode != OQ_COEF_SCALAR_CELL) return -1;
+  const int dim = p->dim, nv = 1 << dim, N = 1 << p->n_refine;
+  const int nn = ipow(N + 1, dim), ncell = ipow(nc, dim), ncn = ipow(nc + 1, dim);
+  double *M = (double *)malloc(sizeof(double) * (size_t)ncell * nv * nv);
+  double *b = (double *)malloc(sizeof(double) * (size_t)ncell * nv);
+  double *Psi = (double *)malloc(sizeof(double) * (size_t)ncell * nv * nn);
+  int st = coarse_all(p, nc, verts, coef, f, M, b, Psi, NULL);
+  if (st == 0 && oq_coarse_solve(dim, nc, M, b, rtol, 100000, u) < 0) st = 1;
+  double s = 0.0;
+  for (int n = 0; n < ncn; ++n)
+    if (!coarse_is_boundary(dim, nc, n)) s += 0.5 * (u[n] - u_target[n]) * (u[n] - u_target[n]);
+  *J = s;
+  free(M);
+  free(b);
+  free(Psi);
+  return st;
+}
+
+int oq_adjoint_gradient(const oq_problem *p, int nc, const double *verts, const double *coef,
+                        const double *f, const double *u_target, double rtol, double *J,
+                        double *grad) {
+  if (p->coef_mode != OQ_COEF_SCALAR_CELL || p->reaction_mode != OQ_REACTION_NONE) return -1;
+  const int dim = p->dim, nv = 1 << dim, N = 1 << p->n_refine, S = N + 1;
+  const int nfc = ipow(N, dim), nn = ipow(S, dim), ncell = ipow(nc, dim), ncn = ipow(nc + 1, dim);
+  double *M = (double *)malloc(sizeof(double) * (size_t)ncell * nv * nv);
+  double *b = (double *)malloc(sizeof(double) * (size_t)ncell * nv);
+  double *Psi = (double *)malloc(sizeof(double) * (size_t)ncell * nv * nn);
+  double *W = (double *)malloc(sizeof(double) * (size_t)ncell * nn);
+  double *u = (double *)malloc(sizeof(double) * ncn);
+  double *v = (double *)malloc(sizeof(double) * ncn);
+  double *d = (double *)malloc(sizeof(double) * ncn);
+  int st = coarse_all(p, nc, verts, coef, f, M, b, Psi, W);
+  if (st == 0 && oq_coarse_solve(dim, nc, M, b, rtol, 100000, u) < 0) st = 1;
+  double s = 0.0;
+  for (int n = 0; n < ncn; ++n) {
+    const int bd = coarse_is_boundary(dim, nc, n);
+    d[n] = bd ? 0.0 : u[n] - u_target[n];
+    s += 0.5 * d[n] * d[n];
+  }
+  *J = s;
+  if (st == 0 && oq_coarse_solve_rhs(dim, nc, M, d, rtol, 100000, v) < 0) st = 1;
+  if (st == 0) {
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 1)
+#endif
+    for (int c = 0; c < ncell; ++c) {
+      int nodes[8];
+      coarse_cell_nodes(dim, nc, c, nodes);
+      const double *vc = verts + (size_t)c * nv * dim;
+      double h[3] = {1.0, 1.0, 1.0};
+      for (int dd = 0; dd < dim; ++dd) h[dd] = (vc[(nv - 1) * dim + dd] - vc[dd]) / N;
+      /* unit-coefficient element stiffness */
+      double Aq[8 * 6] = {0}, fq[8] = {0}, Kh[64], Fe[8];
+      const int ncomp = (dim == 2) ? 3 : 6;
+      for (int q = 0; q < nv; ++q) {
+        if (dim == 2) {
+          Aq[q * ncomp + 0] = 1.0;
+          Aq[q * ncomp + 2] = 1.0;
+        } else {
+          Aq[q * ncomp + 0] = 1.0;
+          Aq[q * ncomp + 3] = 1.0;
+          Aq[q * ncomp + 5] = 1.0;
+        }
+      }
+      oq_element_matrix(dim, h, Aq, fq, 0.0, Kh, Fe);
+      double *ut = (double *)calloc((size_t)nn, sizeof(double));
+      double *vt = (double *)calloc((size_t)nn, sizeof(double));
+      for (int k = 0; k < nv; ++k) {
+        const double *pk = Psi + ((size_t)c * nv + k) * nn;
+        for (int n = 0; n < nn; ++n) {
+          ut[n] += u[nodes[k]] * pk[n];
+          vt[n] += v[nodes[k]] * pk[n];
+        }
+      }
+      const double *wc = W + (size_t)c * nn;
+      for (int e = 0; e < nfc; ++e) {
+        int cijk[3] = {0, 0, 0}, r = e;
+        for (int dd = 0; dd < dim; ++dd) {
+          cijk[dd] = r % N;
+          r /= N;
+        }
+        double ve[8], ue[8];
+        for (int a = 0; a < nv; ++a) {
+          int na = 0, mul = 1;
+          for (int dd = 0; dd < dim; ++dd) {
+            na += (cijk[dd] + ((a >> dd) & 1)) * mul;
+            mul *= S;
+          }
+          ve[a] = vt[na];
+          ue[a] = ut[na] + wc[na];
+        }
+        double t = 0.0;
+        for (int a = 0; a < nv; ++a)
+          for (int bb = 0; bb < nv; ++bb) t += ve[a] * Kh[a * nv + bb] * ue[bb];
+        grad[(size_t)c * nfc + e] = -coef[(size_t)c * nfc + e] * t;
+      }
+      free(ut);
+      free(vt);
+    }
+  }
+  free(M);
+  free(b);
+  free(Psi);
+  free(W);
+  free(u);
+  free(v);
+  free(d);
+  return st;
 }

@@ -115,6 +115,24 @@ int oq_coarse_solve(int dim, int nc, const double *M, const double *b,
 void oq_reconstruct(int dim, int nc, int n_refine, const double *u_coarse,
                     const double *Psi, double *u_fine);
 
+/* The same coarse system with a nodal right-hand side (boundary entries are
+ * ignored): the adjoint solve. */
+int oq_coarse_solve_rhs(int dim, int nc, const double *M, const double *rhs_nodes,
+                        double rtol, int maxit, double *u);
+
+/* ---------------- coarse misfit and its adjoint gradient ------------------ */
+/* J = 1/2 sum over free coarse nodes (u_H - u_target)^2 for a scalar
+ * coefficient per fine cell (cells of the nc^dim grid in lexicographic order,
+ * direct local solves, coarse CG to rtol).  u receives u_H.  Returns 0 on
+ * success. */
+int oq_misfit(const oq_problem *p, int nc, const double *verts, const double *coef,
+              const double *f, const double *u_target, double rtol, double *J, double *u);
+/* dJ/dg per fine cell for a = exp(g) (layout of coef); J as in oq_misfit.
+ * No reaction term.  See msfec_oracle.c for the formula. */
+int oq_adjoint_gradient(const oq_problem *p, int nc, const double *verts, const double *coef,
+                        const double *f, const double *u_target, double rtol, double *J,
+                        double *grad);
+
 #ifdef __cplusplus
 }
 #endif

@@ -76,6 +76,14 @@ def lib():
         L.oq_coarse_solve.restype = ctypes.c_int
         L.oq_reconstruct.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, dp, dp]
         L.oq_reconstruct.restype = None
+        L.oq_coarse_solve_rhs.argtypes = [ctypes.c_int, ctypes.c_int, dp, dp, ctypes.c_double,
+                                          ctypes.c_int, dp]
+        L.oq_coarse_solve_rhs.restype = ctypes.c_int
+        L.oq_misfit.argtypes = [pp, ctypes.c_int, dp, dp, dp, dp, ctypes.c_double, dp, dp]
+        L.oq_misfit.restype = ctypes.c_int
+        L.oq_adjoint_gradient.argtypes = [pp, ctypes.c_int, dp, dp, dp, dp, ctypes.c_double,
+                                          dp, dp]
+        L.oq_adjoint_gradient.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -175,3 +183,42 @@ def reconstruct(dim, nc, n_refine, u_coarse, Psi):
     lib().oq_reconstruct(dim, nc, n_refine, _dp(np.ascontiguousarray(u_coarse, dtype=np.float64)),
                          _dp(np.ascontiguousarray(Psi, dtype=np.float64)), _dp(out))
     return out
+
+
+def coarse_solve_rhs(dim, nc, M, rhs_nodes, rtol=1e-14, maxit=100000):
+    """The coarse system of coarse_solve with a nodal right-hand side."""
+    u = np.zeros((nc + 1) ** dim)
+    it = lib().oq_coarse_solve_rhs(dim, nc, _dp(np.ascontiguousarray(M, dtype=np.float64)),
+                                   _dp(np.ascontiguousarray(rhs_nodes, dtype=np.float64)), rtol,
+                                   maxit, _dp(u))
+    return u, it
+
+
+def misfit(prob, nc, verts, coef, u_target, f=None, rtol=1e-14):
+    """J = 1/2 sum over free coarse nodes (u_H - u_target)^2; returns (J, u_H)."""
+    dim = prob.dim
+    u = np.zeros((nc + 1) ** dim)
+    J = ctypes.c_double(0.0)
+    st = lib().oq_misfit(ctypes.byref(prob), nc, _dp(np.ascontiguousarray(verts, dtype=np.float64)),
+                         _dp(np.ascontiguousarray(coef, dtype=np.float64)),
+                         _dp(None if f is None else np.ascontiguousarray(f, dtype=np.float64)),
+                         _dp(np.ascontiguousarray(u_target, dtype=np.float64)), rtol,
+                         ctypes.byref(J), _dp(u))
+    if st != 0:
+        raise RuntimeError(f"oq_misfit failed ({st})")
+    return J.value, u
+
+
+def adjoint_gradient(prob, nc, verts, coef, u_target, f=None, rtol=1e-14):
+    """(J, dJ/dg) for a = exp(g) per fine cell, dJ/dg in the layout of coef."""
+    grad = np.zeros(np.asarray(coef).shape)
+    J = ctypes.c_double(0.0)
+    st = lib().oq_adjoint_gradient(ctypes.byref(prob), nc,
+                                   _dp(np.ascontiguousarray(verts, dtype=np.float64)),
+                                   _dp(np.ascontiguousarray(coef, dtype=np.float64)),
+                                   _dp(None if f is None else np.ascontiguousarray(f, dtype=np.float64)),
+                                   _dp(np.ascontiguousarray(u_target, dtype=np.float64)), rtol,
+                                   ctypes.byref(J), _dp(grad))
+    if st != 0:
+        raise RuntimeError(f"oq_adjoint_gradient failed ({st})")
+    return J.value, grad

@@ -330,3 +330,55 @@ def test_coarse_constant_coefficient_equals_standard_q1(oracle_mod):
             node_f = sum((((v >> d) & 1) * (S - 1)) * S ** d for d in range(dim))
             node_c = sum((cc[d] + ((v >> d) & 1)) * (nc + 1) ** d for d in range(dim))
             assert abs(uf[c, node_f] - u[node_c]) < 1e-14
+
+
+@pytest.mark.parametrize("dim,nc,l", [(2, 3, 2), (3, 2, 2)])
+def test_adjoint_gradient_matches_finite_differences(oracle_mod, dim, nc, l):
+    """The oracle's adjoint gradient of J = 1/2 |u_H - u_target|^2 with respect
+    to the log-coefficient, against central differences of its own misfit
+    along random directions (truncation O(eps^2) ~ 1e-10)."""
+    from msfec_amd import fields as F
+    O = oracle_mod
+    verts = F.coarse_vertices(dim, nc)
+    N = 1 << l
+    rng = np.random.default_rng(21 + dim)
+    g = 0.8 * rng.standard_normal((nc ** dim, N ** dim))
+    p = O.make_problem(dim, l)
+    _, ut = O.misfit(p, nc, verts, np.exp(0.5 * rng.standard_normal(g.shape)),
+                     np.zeros((nc + 1) ** dim))
+    J, grad = O.adjoint_gradient(p, nc, verts, np.exp(g), ut)
+    J2, _ = O.misfit(p, nc, verts, np.exp(g), ut)
+    assert abs(J - J2) <= 1e-14 * abs(J)
+    eps = 1e-5
+    for _ in range(3):
+        dg = rng.standard_normal(g.shape)
+        Jp, _ = O.misfit(p, nc, verts, np.exp(g + eps * dg), ut)
+        Jm, _ = O.misfit(p, nc, verts, np.exp(g - eps * dg), ut)
+        fd = (Jp - Jm) / (2 * eps)
+        ad = float((grad * dg).sum())
+        # relative to |grad| |dg|, the largest |<grad, dg>| can be: a random
+        # direction can make the derivative itself nearly vanish
+        scale = np.linalg.norm(grad) * np.linalg.norm(dg)
+        assert abs(fd - ad) <= 1e-8 * scale
+
+
+def test_coarse_solve_rhs_equals_assembled(oracle_mod):
+    """The nodal right-hand-side variant reproduces the assembled solve."""
+    O = oracle_mod
+    dim, nc = 2, 4
+    rng = np.random.default_rng(3)
+    # a valid coarse system: element matrices of A = I on unit squares
+    Ke, _ = O.element_matrix(2, np.array([0.25, 0.25]), np.tile([1.0, 0.0, 1.0], (4, 1)),
+                             np.zeros(4), 0.0)
+    M = np.tile(Ke, (nc * nc, 1, 1))
+    b = rng.random((nc * nc, 4))
+    u1, it1 = O.coarse_solve(dim, nc, M, b)
+    rhs = np.zeros((nc + 1) ** dim)
+    for c in range(nc * nc):
+        cx, cy = c % nc, c // nc
+        for v in range(4):
+            node = (cx + (v & 1)) + (nc + 1) * (cy + (v >> 1))
+            rhs[node] += b[c, v]
+    u2, it2 = O.coarse_solve_rhs(dim, nc, M, rhs)
+    assert it1 > 0 and it2 > 0
+    assert np.abs(u1 - u2).max() <= 1e-13 * np.abs(u1).max()
