@@ -1624,35 +1624,63 @@ __device__ __forceinline__ int bk_lane() {
   return x;
 }
 
-template <typename R, int T, int G>
-__device__ __forceinline__ void bk_extract(const typename Mf<R>::v4 (&Tl)[10], R* piv) {
-  const int lane = bk_lane(), lh = lane >> 4, cs = Mf<R>::cslot(lane & 15);
+// Per-lane LDS offsets (in elements) of the super-step accesses, computed
+// once per kernel: every access is one of these plus a compile-time constant,
+// since pad8(16 Q + x) = 18 Q + pad8(x) and pad8(4 r + lh) = lh + {0,4,9,13}[r].
+struct BkAddr {
+  int lane, lh, ll, cs;
+  int row;  // lh * 72 + pad8(cs): row part of tile (T, Q) at + 18 Q
+  int col;  // (cs & 3) * 72 + lh: column part of tile (P, T) at + 18 P + kColR[r]
+  int vrd;  // pad8(cs): panel reads at + 72 t + 18 P
+  int zr;   // pad8(lane): zeroing of the incoming rows
+  bool gcol[4];  // (cs >> 2) == G: this lane holds pivot column cs & 3 of group G
+};
+
+__device__ constexpr int kColR[4] = {0, 4, 9, 13};
+
+template <typename R>
+__device__ __forceinline__ BkAddr bk_addr() {
+  BkAddr ad;
+  ad.lane = threadIdx.x;
+  ad.lh = ad.lane >> 4;
+  ad.ll = ad.lane & 15;
+  ad.cs = Mf<R>::cslot(ad.ll);
+  ad.row = ad.lh * 72 + pad8(ad.cs);
+  ad.col = (ad.cs & 3) * 72 + ad.lh;
+  ad.vrd = pad8(ad.cs);
+  ad.zr = pad8(ad.lane);
 #pragma unroll
-  for (int Q = 0; Q <= T; ++Q) piv[lh * 72 + pad8(16 * Q + cs)] = Tl[MF_TILE(T, Q)][G];
-  if ((cs >> 2) == G) {
-    const int t = cs & 3;
+  for (int g = 0; g < 4; ++g) ad.gcol[g] = (ad.cs >> 2) == g;
+  return ad;
+}
+
+template <typename R, int T, int G>
+__device__ __forceinline__ void bk_extract(const BkAddr& ad, const typename Mf<R>::v4 (&Tl)[10],
+                                           R* piv) {
+#pragma unroll
+  for (int Q = 0; Q <= T; ++Q) piv[ad.row + 18 * Q] = Tl[MF_TILE(T, Q)][G];
+  if (ad.gcol[G]) {
 #pragma unroll
     for (int P = T + 1; P < 4; ++P)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) piv[t * 72 + pad8(16 * P + 4 * r + lh)] = Tl[MF_TILE(P, T)][r];
+      for (int r = 0; r < 4; ++r) piv[ad.col + 18 * P + kColR[r]] = Tl[MF_TILE(P, T)][r];
   }
 }
 
 // Insert the rows jbase..jbase+3 (staged as rows rb..rb+3) into slots
 // 16T + 4G .. +3: scatter their nonzeros into LDS, then place rows and columns.
 template <typename R, int DIM, int T, int G>
-__device__ __forceinline__ void bk_insert(const BandArgs& a, int jbase, int rb, int my_off,
-                                          typename Mf<R>::v4 (&Tl)[10],
+__device__ __forceinline__ void bk_insert(const BandArgs& a, const BkAddr& ad, int jbase, int rb,
+                                          int my_off, typename Mf<R>::v4 (&Tl)[10],
                                           typename Mf<R>::v4 (&Yt)[4], BandSmem& sh,
                                           BkSmem<R>& ms) {
   using D = Dirs<DIM>;
-  const int lane = bk_lane(), lh = lane >> 4, ll = lane & 15, cs = Mf<R>::cslot(ll);
   constexpr int s0 = 16 * T + 4 * G;
 #pragma unroll
-  for (int t = 0; t < 4; ++t) ms.nrow[t * 72 + pad8(lane)] = (R)0;
+  for (int t = 0; t < 4; ++t) ms.nrow[ad.zr + 72 * t] = (R)0;
   wave_sync();
   {
-    const int t = lh, e = ll, jn = jbase + t;
+    const int t = ad.lh, e = ad.ll, jn = jbase + t;
     if (jn < a.n) {
       if (e < D::NLOW) {
         const double v = sh.stg_low[rb + t][e];
@@ -1660,24 +1688,24 @@ __device__ __forceinline__ void bk_insert(const BandArgs& a, int jbase, int rb, 
           // slot of row jn - off; jbase = s0 mod 64
           ms.nrow[t * 72 + pad8((bk_opaque(jbase) + t - my_off) & (kWin - 1))] = (R)v;
           // couplings between two incoming rows are needed in both rows
-          if (e > 0 && my_off <= t) ms.nrow[(t - my_off) * 72 + pad8(s0 + t)] = (R)v;
+          // (pad8(s0 + t) = pad8(s0) + t as s0 is a multiple of 4)
+          if (e > 0 && my_off <= t) ms.nrow[(t - my_off) * 72 + pad8(s0) + t] = (R)v;
         }
       }
     } else if (e == 0) {
-      ms.nrow[t * 72 + pad8(s0 + t)] = (R)1;  // past the end: isolated unit diagonal
+      ms.nrow[t * 72 + pad8(s0) + t] = (R)1;  // past the end: isolated unit diagonal
     }
   }
   wave_sync();
 #pragma unroll
-  for (int Q = 0; Q <= T; ++Q) Tl[MF_TILE(T, Q)][G] = ms.nrow[lh * 72 + pad8(16 * Q + cs)];
-  if ((cs >> 2) == G) {
-    const int t = cs & 3;
+  for (int Q = 0; Q <= T; ++Q) Tl[MF_TILE(T, Q)][G] = ms.nrow[ad.row + 18 * Q];
+  if (ad.gcol[G]) {
 #pragma unroll
     for (int P = T; P < 4; ++P)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) Tl[MF_TILE(P, T)][r] = ms.nrow[t * 72 + pad8(16 * P + 4 * r + lh)];
+      for (int r = 0; r < 4; ++r) Tl[MF_TILE(P, T)][r] = ms.nrow[ad.col + 18 * P + kColR[r]];
   }
-  Yt[T][G] = (ll < 8 && jbase + lh < a.n) ? (R)sh.stg_rhs[rb + lh][ll] : (R)0;
+  Yt[T][G] = (ad.ll < 8 && jbase + ad.lh < a.n) ? (R)sh.stg_rhs[rb + ad.lh][ad.ll & 7] : (R)0;
   wave_sync();
 }
 
@@ -1698,16 +1726,16 @@ __device__ __forceinline__ void bk_check(R d, int& bad) {
 }
 
 template <typename R, int DIM, int T, int G>
-__device__ __forceinline__ void bk_superstep(const BandArgs& a, int cell, int j0,
+__device__ __forceinline__ void bk_superstep(const BandArgs& a, const BkAddr& ad, int cell, int j0,
                                              typename Mf<R>::v4 (&Tl)[10],
                                              typename Mf<R>::v4 (&Yt)[4],
                                              typename Mf<R>::v4& Sacc, BkSmem<R>& ms, int& bad,
                                              const R (&ek)[4]) {
   typedef Mf<R> M;
-  const int lane = bk_lane(), lh = lane >> 4, ll = lane & 15, cs = M::cslot(ll);
+  const int lh = ad.lh, ll = ad.ll, cs = ad.cs;
   constexpr int s0 = 16 * T + 4 * G;
-  bk_extract<R, T, G>(Tl, ms.piv);
-  ms.yp[lh * 16 + ll] = Yt[T][G];
+  bk_extract<R, T, G>(ad, Tl, ms.piv);
+  ms.yp[ad.lane] = Yt[T][G];
   wave_sync();
   const R* pv = ms.piv;
   // C C^T = pivot block (every lane); ci_t = 1 / C[t][t]
@@ -1737,7 +1765,7 @@ __device__ __forceinline__ void bk_superstep(const BandArgs& a, int cell, int j0
   R V[4], nV[4];
 #pragma unroll
   for (int P = 0; P < 4; ++P) {
-    const int x = pad8(16 * P + cs);
+    const int x = ad.vrd + 18 * P;
     V[P] = pv[0 * 72 + x] * x0 + pv[1 * 72 + x] * x1 + pv[2 * 72 + x] * x2 + pv[3 * 72 + x] * x3;
     nV[P] = -V[P];
   }
@@ -1776,7 +1804,8 @@ __device__ __forceinline__ void bk_stage(const BandArgs& a, int cell, int jbase,
 // The 4 super-steps of tile row T (pivots j0 .. j0+15); the next 16 rows of
 // the band enter the freed slots.
 template <typename R, int DIM, int T>
-__device__ __forceinline__ void bk_tile_row(const BandArgs& a, int cell, int j0, int my_off,
+__device__ __forceinline__ void bk_tile_row(const BandArgs& a, const BkAddr& ad, int cell, int j0,
+                                            int my_off,
                                             typename Mf<R>::v4 (&Tl)[10],
                                             typename Mf<R>::v4 (&Yt)[4],
                                             typename Mf<R>::v4& Sacc, BandSmem& sh,
@@ -1784,54 +1813,96 @@ __device__ __forceinline__ void bk_tile_row(const BandArgs& a, int cell, int j0,
                                             const R (&ek)[4]) {
   const int n = a.n;
   bk_stage(a, cell, j0 + kWin, sh, pf[0]);
-  bk_superstep<R, DIM, T, 0>(a, cell, j0, Tl, Yt, Sacc, ms, bad, ek);
-  bk_insert<R, DIM, T, 0>(a, j0 + kWin, 0, my_off, Tl, Yt, sh, ms);
+  bk_superstep<R, DIM, T, 0>(a, ad, cell, j0, Tl, Yt, Sacc, ms, bad, ek);
+  bk_insert<R, DIM, T, 0>(a, ad, j0 + kWin, 0, my_off, Tl, Yt, sh, ms);
   if (j0 + 4 < n) {
-    bk_superstep<R, DIM, T, 1>(a, cell, j0 + 4, Tl, Yt, Sacc, ms, bad, ek);
-    bk_insert<R, DIM, T, 1>(a, j0 + 4 + kWin, 4, my_off, Tl, Yt, sh, ms);
+    bk_superstep<R, DIM, T, 1>(a, ad, cell, j0 + 4, Tl, Yt, Sacc, ms, bad, ek);
+    bk_insert<R, DIM, T, 1>(a, ad, j0 + 4 + kWin, 4, my_off, Tl, Yt, sh, ms);
   }
   bk_stage(a, cell, j0 + 8 + kWin, sh, pf[1]);
   if (j0 + 8 < n) {
-    bk_superstep<R, DIM, T, 2>(a, cell, j0 + 8, Tl, Yt, Sacc, ms, bad, ek);
-    bk_insert<R, DIM, T, 2>(a, j0 + 8 + kWin, 0, my_off, Tl, Yt, sh, ms);
+    bk_superstep<R, DIM, T, 2>(a, ad, cell, j0 + 8, Tl, Yt, Sacc, ms, bad, ek);
+    bk_insert<R, DIM, T, 2>(a, ad, j0 + 8 + kWin, 0, my_off, Tl, Yt, sh, ms);
   }
   if (j0 + 12 < n) {
-    bk_superstep<R, DIM, T, 3>(a, cell, j0 + 12, Tl, Yt, Sacc, ms, bad, ek);
-    bk_insert<R, DIM, T, 3>(a, j0 + 12 + kWin, 4, my_off, Tl, Yt, sh, ms);
+    bk_superstep<R, DIM, T, 3>(a, ad, cell, j0 + 12, Tl, Yt, Sacc, ms, bad, ek);
+    bk_insert<R, DIM, T, 3>(a, ad, j0 + 12 + kWin, 4, my_off, Tl, Yt, sh, ms);
   }
 }
 
 // Initial fill of tile row T with the rows 16T .. 16T+15.
 template <typename R, int DIM, int T>
-__device__ __forceinline__ void bk_fill_row(const BandArgs& a, int cell, int my_off,
+__device__ __forceinline__ void bk_fill_row(const BandArgs& a, const BkAddr& ad, int cell,
+                                            int my_off,
                                             typename Mf<R>::v4 (&Tl)[10],
                                             typename Mf<R>::v4 (&Yt)[4], BandSmem& sh,
                                             BkSmem<R>& ms, RowPrefetch (&pf)[2]) {
   bk_stage(a, cell, 16 * T, sh, pf[0]);
-  bk_insert<R, DIM, T, 0>(a, 16 * T + 0, 0, my_off, Tl, Yt, sh, ms);
-  bk_insert<R, DIM, T, 1>(a, 16 * T + 4, 4, my_off, Tl, Yt, sh, ms);
+  bk_insert<R, DIM, T, 0>(a, ad, 16 * T + 0, 0, my_off, Tl, Yt, sh, ms);
+  bk_insert<R, DIM, T, 1>(a, ad, 16 * T + 4, 4, my_off, Tl, Yt, sh, ms);
   bk_stage(a, cell, 16 * T + 8, sh, pf[1]);
-  bk_insert<R, DIM, T, 2>(a, 16 * T + 8, 0, my_off, Tl, Yt, sh, ms);
-  bk_insert<R, DIM, T, 3>(a, 16 * T + 12, 4, my_off, Tl, Yt, sh, ms);
+  bk_insert<R, DIM, T, 2>(a, ad, 16 * T + 8, 0, my_off, Tl, Yt, sh, ms);
+  bk_insert<R, DIM, T, 3>(a, ad, 16 * T + 12, 4, my_off, Tl, Yt, sh, ms);
 }
 
-// Backward sweep.  Lane (k = lane >> 4, r = (lane >> 1) & 7, h = lane & 1)
-// sums the offsets o in [4 + 32h, 36 + 32h) of (V^T x)[k][r]; lanes 0..7
-// (r = lane) then apply C^-T.  Per super-step the panel, z and the 4 table
-// rows (lifting right-hand sides, interior load, node index) are prefetched
-// one step ahead.  MODE BK_FINAL64: Psi and the interior part of b;
-// BK_X0: the solution to X; BK_REFINE: the ring holds the correction,
-// x = X + correction goes to Psi and b, and lane (i,k) accumulates the Schur
-// term in its energy form S = R^T X + X^T r (r = Rres = R - K_II X), exact up
-// to the second order in the error of the first solution X:
+// Wait for every outstanding vector memory operation (vmcnt(0); expcnt and
+// lgkmcnt left at their maxima).
+__device__ __forceinline__ void vm_wait_all() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
+// Sum over the 8 lanes of an aligned group (DPP: swap neighbours, swap
+// pairs, mirror within the half row).
+__device__ __forceinline__ double dpp_f64(double v, int ctrl_sel) {
+  const long long b = __builtin_bit_cast(long long, v);
+  int lo = (int)b, hi = (int)(b >> 32);
+  if (ctrl_sel == 0) {
+    lo = __builtin_amdgcn_mov_dpp(lo, 0xB1, 0xF, 0xF, false);  // quad_perm [1,0,3,2]
+    hi = __builtin_amdgcn_mov_dpp(hi, 0xB1, 0xF, 0xF, false);
+  } else if (ctrl_sel == 1) {
+    lo = __builtin_amdgcn_mov_dpp(lo, 0x4E, 0xF, 0xF, false);  // quad_perm [2,3,0,1]
+    hi = __builtin_amdgcn_mov_dpp(hi, 0x4E, 0xF, 0xF, false);
+  } else {
+    lo = __builtin_amdgcn_mov_dpp(lo, 0x141, 0xF, 0xF, false);  // row_half_mirror
+    hi = __builtin_amdgcn_mov_dpp(hi, 0x141, 0xF, 0xF, false);
+  }
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ double sum8(double v) {
+  v += dpp_f64(v, 0);
+  v += dpp_f64(v, 1);
+  v += dpp_f64(v, 2);
+  return v;
+}
+
+// Backward sweep.  (V^T x)[k][r] over the 60 offsets of a panel: lane
+// (k0 = 2 ((lane >> 3) & 1), r0 = 2 (lane >> 4), c = lane & 7) sums a 2 x 2
+// block of it over every 8th offset, the 8 lanes of the block add up by DPP;
+// lanes 0..7 (r = lane) then apply C^-T.  The steps go in groups of 4: at the start of
+// a group one wait covers the data of the group (the panels, z, the 4 table
+// rows of each step: lifting right-hand sides, interior load, node index),
+// loaded during the previous group into the other of two register sets, and
+// the loads of the next group are issued.  The steps issue no global stores:
+// the solution rows stay in the LDS ring (64 rows) and are written out 64
+// rows at a time, one row per lane, right after a group's wait, so no wait
+// ever covers a store or load issued less than a group earlier.
+// MODE BK_FINAL64: Psi and the interior part of b; BK_X0: the solution to X;
+// BK_REFINE: the ring holds the correction, x = X + correction (kept in
+// xfr [8][64]) goes to X, Psi and b, and lane (i,k) accumulates the Schur
+// term in its energy form S = R^T X + X^T r (r = Rres = R - K_II X), exact
+// up to the second order in the error of the first solution X:
 // S = R^T x* - e^T K_II e for X = x* + e.
+template <typename R>
+struct BkPre {
+  R pp[4], zp;
+  double rp0, rp1, xp, qp;
+};
+
 template <typename R, int DIM, int MODE>
 __device__ __forceinline__ void bk_backward(const BandArgs& a, int cell, BandSmem& sh,
-                                            BkSmem<R>& ms, double& bint, double& sacc,
-                                            double& dmax, double& xmax) {
+                                            BkSmem<R>& ms, double* xfr, double& bint,
+                                            double& sacc, double& dmax, double& xmax) {
   using D = Dirs<DIM>;
   const int lane = threadIdx.x;
-  const int k = lane >> 4, r = (lane >> 1) & 7, h = lane & 1;
+  const int c = lane & 7, k0 = 2 * ((lane >> 3) & 1), r0 = 2 * (lane >> 4);
   const int n = a.n, nq = (n + 3) >> 2;
   R* pb = ms.piv;                     // [4][68]
   double* xr = SH_PART(sh);           // [8][132]
@@ -1840,19 +1911,13 @@ __device__ __forceinline__ void bk_backward(const BandArgs& a, int cell, BandSme
   double* rw = tb + 32;               // [4][24] table rows of the step
   double* xb = rw + 96;               // [4][8] first solution X of the step (BK_REFINE)
   double* r0b = xb + 32;              // [4][8] its residual (BK_REFINE)
+  double* nodeb = &sh.stg_low[0][0];  // [64] node index of the ring rows
   for (int t = lane; t < 8 * kRing; t += 64) xr[t] = 0.0;
   for (int t = lane; t < 4 * kPanRow; t += 64) pb[t] = (R)0;
   const R* Pc = reinterpret_cast<const R*>(a.pan) + (size_t)cell * a.nq * 256;
   const R* Zc = reinterpret_cast<const R*>(a.zst) + (size_t)cell * a.nq * 32;
   const double* Rc = a.Rows + (size_t)cell * n * kRowStride;
-  // Prefetch ring: the data of step q is loaded 4 steps ahead into register
-  // set (nq - 1 - q) mod 4 (4 panel values, z, 2 table values and, when
-  // refining, X and its residual per lane).
-  struct Pre {
-    R pp[4], zp;
-    double rp0, rp1, xp, qp;
-  } pre[4];
-  auto load = [&](int q, Pre& f) {
+  auto load = [&](int q, BkPre<R>& f) {
     if (q < 0) return;
 #pragma unroll
     for (int u = 0; u < 4; ++u) f.pp[u] = Pc[(size_t)q * 256 + lane * 4 + u];
@@ -1867,26 +1932,44 @@ __device__ __forceinline__ void bk_backward(const BandArgs& a, int cell, BandSme
       f.qp = ok ? a.Rres[xi] : 0.0;
     }
   };
+  // rows [jlo, jhi) (at most 64, all still in the ring) to X and / or Psi;
+  // lane = row
+  auto flush = [&](int jlo, int jhi) {
+    wave_sync();
+    const int j = jlo + lane;
+    if (j < jhi) {
+      const int p = j & (kWin - 1);
+      double v[kMaxNb];
 #pragma unroll
-  for (int u = 0; u < 4; ++u) load(nq - 1 - u, pre[u]);
-  for (int q0 = nq - 1; q0 >= 0; q0 -= 4) {
+      for (int rr = 0; rr < kMaxNb; ++rr)
+        v[rr] = (MODE == BK_REFINE) ? xfr[rr * 64 + p] : xr[rr * kRing + p];
+      if (MODE != BK_FINAL64) {
+        double2* xd = reinterpret_cast<double2*>(a.X + ((size_t)cell * n + j) * kMaxNb);
 #pragma unroll
-  for (int u = 0; u < 4; ++u) {
-    const int q = q0 - u;
-    if (q < 0) break;
+        for (int rr = 0; rr < kMaxNb; rr += 2) xd[rr >> 1] = make_double2(v[rr], v[rr + 1]);
+      }
+      if (MODE != BK_X0) {
+        const int node = (int)nodeb[p];
+#pragma unroll
+        for (int rr = 0; rr < D::NV; ++rr) a.Psi[((size_t)cell * D::NV + rr) * a.nn + node] = v[rr];
+      }
+    }
+  };
+  auto step = [&](int q, const BkPre<R>& f) {
     const int j0 = 4 * q;
     wave_sync();
 #pragma unroll
-    for (int v = 0; v < 4; ++v) pb[(lane >> 4) * kPanRow + (lane & 15) * 4 + v] = pre[u].pp[v];
-    if (lane < 32) zb[lane] = (double)pre[u].zp;
-    rw[lane] = pre[u].rp0;
-    if (lane < 32) rw[64 + lane] = pre[u].rp1;
+    for (int v = 0; v < 4; ++v) pb[(lane >> 4) * kPanRow + (lane & 15) * 4 + v] = f.pp[v];
+    if (lane < 32) zb[lane] = (double)f.zp;
+    rw[lane] = f.rp0;
+    if (lane < 32) rw[64 + lane] = f.rp1;
     if (MODE == BK_REFINE && lane < 32) {
-      xb[lane] = pre[u].xp;
-      r0b[lane] = pre[u].qp;
+      xb[lane] = f.xp;
+      r0b[lane] = f.qp;
     }
     wave_sync();
-    load(q - 4, pre[u]);
+    if (MODE != BK_X0 && lane < 4 && j0 + lane < n)
+      nodeb[(j0 + lane) & (kWin - 1)] = rw[lane * kRowStride + 15];
     if (MODE == BK_REFINE) {
       // energy-form Schur term: lane (i = lane >> 3, kk = lane & 7)
       const int i = lane >> 3, kk = lane & 7;
@@ -1894,13 +1977,31 @@ __device__ __forceinline__ void bk_backward(const BandArgs& a, int cell, BandSme
       for (int t = 0; t < 4; ++t)
         sacc += rw[t * kRowStride + 16 + i] * xb[t * 8 + kk] + xb[t * 8 + i] * r0b[t * 8 + kk];
     }
-    const int base = (j0 & (kWin - 1)) + 4 + 32 * h;
-    double part = 0.0;
+    // (V^T x)[k][r] over the offsets 4..67 (the panel is zero at 64..67):
+    // lane (k0, r0, c) sums the 2 x 2 block k0 + {0,1}, r0 + {0,1} over the
+    // offsets 4 + c + 8i, then the 8 lanes of the block add up.
+    const double* xw = xr + r0 * kRing + (j0 & (kWin - 1)) + 4 + c;
+    const R* vw = pb + k0 * kPanRow + 4 + c;
+    double s00 = 0.0, s01 = 0.0, s10 = 0.0, s11 = 0.0;
 #pragma unroll
-    for (int m = 0; m < 32; ++m)
-      part = fma((double)pb[k * kPanRow + 4 + 32 * h + m], xr[r * kRing + base + m], part);
-    part += __shfl_xor(part, 1);
-    if (h == 0) tb[k * 8 + r] = part;
+    for (int i = 0; i < 8; ++i) {
+      const double v0 = (double)vw[8 * i], v1 = (double)vw[kPanRow + 8 * i];
+      const double x0 = xw[8 * i], x1 = xw[kRing + 8 * i];
+      s00 = fma(v0, x0, s00);
+      s01 = fma(v0, x1, s01);
+      s10 = fma(v1, x0, s10);
+      s11 = fma(v1, x1, s11);
+    }
+    s00 = sum8(s00);
+    s01 = sum8(s01);
+    s10 = sum8(s10);
+    s11 = sum8(s11);
+    if (c == 0) {
+      tb[k0 * 8 + r0] = s00;
+      tb[k0 * 8 + r0 + 1] = s01;
+      tb[(k0 + 1) * 8 + r0] = s10;
+      tb[(k0 + 1) * 8 + r0 + 1] = s11;
+    }
     wave_sync();
     if (lane < 8) {
       const int rr = lane;
@@ -1922,23 +2023,44 @@ __device__ __forceinline__ void bk_backward(const BandArgs& a, int cell, BandSme
         const int p = j & (kWin - 1);
         xr[rr * kRing + p] = xs[t];
         xr[rr * kRing + p + 64] = xs[t];
-        if (MODE == BK_X0) {
-          a.X[((size_t)cell * n + j) * kMaxNb + rr] = xs[t];
-        } else {
+        if (MODE != BK_X0) {
           const double xf = (MODE == BK_REFINE) ? xb[t * 8 + rr] + xs[t] : xs[t];
           if (MODE == BK_REFINE) {
-            a.X[((size_t)cell * n + j) * kMaxNb + rr] = xf;  // for the next pass
+            xfr[rr * 64 + p] = xf;
             dmax = fmax(dmax, fabs(xs[t]));
             xmax = fmax(xmax, fabs(xf));
           }
-          const int node = (int)rw[t * kRowStride + 15];
-          if (rr < D::NV) a.Psi[((size_t)cell * D::NV + rr) * a.nn + node] = xf;
           bint = fma(xf, rw[t * kRowStride + 14], bint);
         }
       }
     }
+  };
+  int jflush = n;  // rows >= jflush are written out
+  // group of the steps qg, qg - 1, .., qg - 3 with its data in cur
+  auto group = [&](int qg, const BkPre<R> (&cur)[4], BkPre<R> (&nxt)[4]) {
+    vm_wait_all();
+    const int done = nq - 1 - qg;  // steps before this group
+    if (done > 0 && (done & 15) == 0) {
+      flush(4 * (qg + 1), jflush);
+      jflush = 4 * (qg + 1);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) load(qg - 4 - u, nxt[u]);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (qg - u < 0) break;
+      step(qg - u, cur[u]);
+    }
+  };
+  BkPre<R> pa[4], pc[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) load(nq - 1 - u, pa[u]);
+  for (int q0 = nq - 1; q0 >= 0; q0 -= 8) {
+    group(q0, pa, pc);
+    if (q0 - 4 < 0) break;
+    group(q0 - 4, pc, pa);
   }
-  }
+  if (jflush > 0) flush(0, jflush);
   wave_sync();
 }
 
@@ -1984,30 +2106,31 @@ __global__ __launch_bounds__(64, BkOcc<R>::waves) void q_blk_factor_kernel(BandA
   for (int t = 0; t < 10; ++t) Tl[t] = (V4){0, 0, 0, 0};
 #pragma unroll
   for (int P = 0; P < 4; ++P) Yt[P] = (V4){0, 0, 0, 0};
+  const BkAddr ad = bk_addr<R>();
   R ek[4];  // unit vector e_(lane >> 4)
 #pragma unroll
   for (int t = 0; t < 4; ++t) ek[t] = ((lane >> 4) == t) ? (R)1 : (R)0;
   RowPrefetch pf[2];
   rows_load(a, cell, 0, pf[0]);
   rows_load(a, cell, 8, pf[1]);
-  bk_fill_row<R, DIM, 0>(a, cell, my_off, Tl, Yt, sh, ms, pf);
-  bk_fill_row<R, DIM, 1>(a, cell, my_off, Tl, Yt, sh, ms, pf);
-  bk_fill_row<R, DIM, 2>(a, cell, my_off, Tl, Yt, sh, ms, pf);
-  bk_fill_row<R, DIM, 3>(a, cell, my_off, Tl, Yt, sh, ms, pf);
+  bk_fill_row<R, DIM, 0>(a, ad, cell, my_off, Tl, Yt, sh, ms, pf);
+  bk_fill_row<R, DIM, 1>(a, ad, cell, my_off, Tl, Yt, sh, ms, pf);
+  bk_fill_row<R, DIM, 2>(a, ad, cell, my_off, Tl, Yt, sh, ms, pf);
+  bk_fill_row<R, DIM, 3>(a, ad, cell, my_off, Tl, Yt, sh, ms, pf);
   int bad = 0;
   const int n = a.n;
   for (int j0 = 0; j0 < n; j0 += 64) {
-    bk_tile_row<R, DIM, 0>(a, cell, j0, my_off, Tl, Yt, Sacc, sh, ms, pf, bad, ek);
-    if (j0 + 16 < n) bk_tile_row<R, DIM, 1>(a, cell, j0 + 16, my_off, Tl, Yt, Sacc, sh, ms, pf, bad, ek);
-    if (j0 + 32 < n) bk_tile_row<R, DIM, 2>(a, cell, j0 + 32, my_off, Tl, Yt, Sacc, sh, ms, pf, bad, ek);
-    if (j0 + 48 < n) bk_tile_row<R, DIM, 3>(a, cell, j0 + 48, my_off, Tl, Yt, Sacc, sh, ms, pf, bad, ek);
+    bk_tile_row<R, DIM, 0>(a, ad, cell, j0, my_off, Tl, Yt, Sacc, sh, ms, pf, bad, ek);
+    if (j0 + 16 < n) bk_tile_row<R, DIM, 1>(a, ad, cell, j0 + 16, my_off, Tl, Yt, Sacc, sh, ms, pf, bad, ek);
+    if (j0 + 32 < n) bk_tile_row<R, DIM, 2>(a, ad, cell, j0 + 32, my_off, Tl, Yt, Sacc, sh, ms, pf, bad, ek);
+    if (j0 + 48 < n) bk_tile_row<R, DIM, 3>(a, ad, cell, j0 + 48, my_off, Tl, Yt, Sacc, sh, ms, pf, bad, ek);
   }
   double sacc = (MODE == BK_FINAL64) ? bk_sacc<R>(Sacc, SH_PART(sh)) : 0.0;
   // panels of this wave must be visible to its own loads in the backward sweep
   __threadfence_block();
   __syncthreads();
   double bint = 0.0, dmax = 0.0, xmax = 0.0;
-  bk_backward<R, DIM, MODE>(a, cell, sh, ms, bint, sacc, dmax, xmax);
+  bk_backward<R, DIM, MODE>(a, cell, sh, ms, nullptr, bint, sacc, dmax, xmax);
   int worst = bad;
   for (int o = 32; o > 0; o >>= 1) worst = max(worst, __shfl_xor(worst, o));
   if (lane == 0) {
@@ -2176,11 +2299,12 @@ __global__ __launch_bounds__(64, 2) void q_blk_refine_kernel(BandArgs a) {
   if (a.conv[cell]) return;  // converged or failed: the whole wave leaves
   __shared__ BandSmem sh;
   __shared__ BkSmem<float> ms;
+  __shared__ double xfr[kMaxNb * 64];  // refined solution of the ring rows
   rf_forward<float, DIM>(a, cell, ms);
   __threadfence_block();
   __syncthreads();
   double bint = 0.0, sacc = 0.0, dmax = 0.0, xmax = 0.0;
-  bk_backward<float, DIM, BK_REFINE>(a, cell, sh, ms, bint, sacc, dmax, xmax);
+  bk_backward<float, DIM, BK_REFINE>(a, cell, sh, ms, xfr, bint, sacc, dmax, xmax);
   for (int o = 32; o > 0; o >>= 1) {
     dmax = fmax(dmax, __shfl_xor(dmax, o));
     xmax = fmax(xmax, __shfl_xor(xmax, o));
@@ -2214,7 +2338,7 @@ __global__ __launch_bounds__(64, 2) void q_blk_solve_kernel(BandArgs a) {
   __threadfence_block();
   __syncthreads();
   double bint = 0.0, sacc = 0.0, dmax = 0.0, xmax = 0.0;
-  bk_backward<R, DIM, BK_X0>(a, cell, sh, ms, bint, sacc, dmax, xmax);
+  bk_backward<R, DIM, BK_X0>(a, cell, sh, ms, nullptr, bint, sacc, dmax, xmax);
 }
 
 // Right-hand side of w: column 0 = F_I (interior load), columns 1..7 = 0.
