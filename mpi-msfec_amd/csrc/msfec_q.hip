@@ -1719,17 +1719,44 @@ __device__ __forceinline__ double bk_rsqrt(double x) {
 }
 __device__ __forceinline__ float bk_rsqrt(float x) { return __builtin_amdgcn_rsqf(x); }
 
+// Pivot monitor: the smallest pivot (NaN ignored) and the sum of all pivots.
+// A non-positive pivot turns every later one into NaN, and with finite
+// entries the first failing pivot is a number <= 0, so dmin <= 0 marks a
+// non-SPD cell; otherwise a non-finite sum marks non-finite entries.
 template <typename R>
-__device__ __forceinline__ void bk_check(R d, int& bad) {
-  // keep the first failure: later pivots of a failed cell are meaningless
-  if (!(d > (R)0) && bad == 0) bad = (d == d) ? (int)MSFEC_CELL_NOT_SPD : (int)MSFEC_CELL_NONFINITE;
+struct BkMon {
+  R dmin, dsum;
+  __device__ __forceinline__ void init() {
+    dmin = (R)1;
+    dsum = (R)0;
+  }
+  __device__ __forceinline__ int status() const {
+    if (!(dmin > (R)0)) return MSFEC_CELL_NOT_SPD;
+    return isfinite(dsum) ? (int)MSFEC_CELL_OK : (int)MSFEC_CELL_NONFINITE;
+  }
+};
+
+template <typename R>
+__device__ __forceinline__ void bk_check(R d, BkMon<R>& mon) {
+  mon.dmin = fmin(mon.dmin, d);
+  mon.dsum += d;
+}
+
+// Order of the 15 updates of a super-step (tile row T, group G): first the
+// tiles and the right-hand-side tile that the insertion of the next rows
+// overwrites (row and column T), then, at the last group of the tile row,
+// those that the next super-step extracts (row and column T + 1), then the
+// rest, so that the next steps wait for as few matrix-core results as possible.
+template <int T, int G>
+__device__ constexpr int bk_prio(int P, int Q) {
+  return (P == T || Q == T) ? 0 : (G == 3 && (P == ((T + 1) & 3) || Q == ((T + 1) & 3))) ? 1 : 2;
 }
 
 template <typename R, int DIM, int T, int G>
 __device__ __forceinline__ void bk_superstep(const BandArgs& a, const BkAddr& ad, int cell, int j0,
                                              typename Mf<R>::v4 (&Tl)[10],
                                              typename Mf<R>::v4 (&Yt)[4],
-                                             typename Mf<R>::v4& Sacc, BkSmem<R>& ms, int& bad,
+                                             typename Mf<R>::v4& Sacc, BkSmem<R>& ms, BkMon<R>& bad,
                                              const R (&ek)[4]) {
   typedef Mf<R> M;
   const int lh = ad.lh, ll = ad.ll, cs = ad.cs;
@@ -1772,11 +1799,15 @@ __device__ __forceinline__ void bk_superstep(const BandArgs& a, const BkAddr& ad
   const R Z = ms.yp[0 * 16 + ll] * x0 + ms.yp[1 * 16 + ll] * x1 + ms.yp[2 * 16 + ll] * x2 +
               ms.yp[3 * 16 + ll] * x3;
 #pragma unroll
-  for (int P = 0; P < 4; ++P)
+  for (int pr = 0; pr < 3; ++pr) {
 #pragma unroll
-    for (int Q = 0; Q <= P; ++Q) Tl[MF_TILE(P, Q)] = M::mma(nV[P], V[Q], Tl[MF_TILE(P, Q)]);
+    for (int P = 0; P < 4; ++P) {
 #pragma unroll
-  for (int P = 0; P < 4; ++P) Yt[P] = M::mma(nV[P], Z, Yt[P]);
+      for (int Q = 0; Q <= P; ++Q)
+        if (bk_prio<T, G>(P, Q) == pr) Tl[MF_TILE(P, Q)] = M::mma(nV[P], V[Q], Tl[MF_TILE(P, Q)]);
+      if (bk_prio<T, G>(P, P) == pr) Yt[P] = M::mma(nV[P], Z, Yt[P]);
+    }
+  }
   Sacc = M::mma(Z, Z, Sacc);
   // panel [k][o], o = offset from the first pivot.  Offsets 0..3 hold C
   // (lower part) with 1 / C[t][t] on the diagonal.
@@ -1809,7 +1840,7 @@ __device__ __forceinline__ void bk_tile_row(const BandArgs& a, const BkAddr& ad,
                                             typename Mf<R>::v4 (&Tl)[10],
                                             typename Mf<R>::v4 (&Yt)[4],
                                             typename Mf<R>::v4& Sacc, BandSmem& sh,
-                                            BkSmem<R>& ms, RowPrefetch (&pf)[2], int& bad,
+                                            BkSmem<R>& ms, RowPrefetch (&pf)[2], BkMon<R>& bad,
                                             const R (&ek)[4]) {
   const int n = a.n;
   bk_stage(a, cell, j0 + kWin, sh, pf[0]);
@@ -1920,7 +1951,7 @@ __device__ __forceinline__ void bk_backward(const BandArgs& a, int cell, BandSme
   auto load = [&](int q, BkPre<R>& f) {
     if (q < 0) return;
 #pragma unroll
-    for (int u = 0; u < 4; ++u) f.pp[u] = Pc[(size_t)q * 256 + lane * 4 + u];
+    for (int u = 0; u < 4; ++u) f.pp[u] = Pc[(size_t)q * 256 + u * 64 + lane];  // [k = u][o = lane]
     f.zp = (lane < 32) ? Zc[(size_t)q * 32 + lane] : (R)0;
     const int lim = (n - 4 * q) * kRowStride;  // valid doubles of the 4 rows
     f.rp0 = (lane < lim) ? Rc[(size_t)4 * q * kRowStride + lane] : 0.0;
@@ -1959,7 +1990,7 @@ __device__ __forceinline__ void bk_backward(const BandArgs& a, int cell, BandSme
     const int j0 = 4 * q;
     wave_sync();
 #pragma unroll
-    for (int v = 0; v < 4; ++v) pb[(lane >> 4) * kPanRow + (lane & 15) * 4 + v] = f.pp[v];
+    for (int v = 0; v < 4; ++v) pb[v * kPanRow + lane] = f.pp[v];
     if (lane < 32) zb[lane] = (double)f.zp;
     rw[lane] = f.rp0;
     if (lane < 32) rw[64 + lane] = f.rp1;
@@ -2117,7 +2148,8 @@ __global__ __launch_bounds__(64, BkOcc<R>::waves) void q_blk_factor_kernel(BandA
   bk_fill_row<R, DIM, 1>(a, ad, cell, my_off, Tl, Yt, sh, ms, pf);
   bk_fill_row<R, DIM, 2>(a, ad, cell, my_off, Tl, Yt, sh, ms, pf);
   bk_fill_row<R, DIM, 3>(a, ad, cell, my_off, Tl, Yt, sh, ms, pf);
-  int bad = 0;
+  BkMon<R> bad;
+  bad.init();
   const int n = a.n;
   for (int j0 = 0; j0 < n; j0 += 64) {
     bk_tile_row<R, DIM, 0>(a, ad, cell, j0, my_off, Tl, Yt, Sacc, sh, ms, pf, bad, ek);
@@ -2131,7 +2163,7 @@ __global__ __launch_bounds__(64, BkOcc<R>::waves) void q_blk_factor_kernel(BandA
   __syncthreads();
   double bint = 0.0, dmax = 0.0, xmax = 0.0;
   bk_backward<R, DIM, MODE>(a, cell, sh, ms, nullptr, bint, sacc, dmax, xmax);
-  int worst = bad;
+  int worst = bad.status();
   for (int o = 32; o > 0; o >>= 1) worst = max(worst, __shfl_xor(worst, o));
   if (lane == 0) {
     a.status[cell] = worst;
@@ -2206,7 +2238,7 @@ __device__ __forceinline__ void rf_load(const BandArgs& a, int cell, int q, RfPr
   if (4 * q >= n) return;
   const R* Pn = reinterpret_cast<const R*>(a.pan) + ((size_t)cell * a.nq + q) * 256;
 #pragma unroll
-  for (int u = 0; u < 4; ++u) f.pp[u] = Pn[lane * 4 + u];
+  for (int u = 0; u < 4; ++u) f.pp[u] = Pn[u * 64 + lane];  // [k = u][o = lane]
   const int jn = 4 * q + kWin + lh;
   const R rv = (R)a.Rres[((size_t)cell * n + min(jn, n - 1)) * kMaxNb + (ll & 7)];
   f.rq = (ll < 8 && jn < n) ? rv : (R)0;
@@ -2224,7 +2256,7 @@ __device__ __forceinline__ void rf_step(const BandArgs& a, int cell, int j0,
   const int qs = j0 >> 2;
   // panel of this super-step and the pivot right-hand sides to LDS
 #pragma unroll
-  for (int u = 0; u < 4; ++u) pb[lh * kPanRow + ll * 4 + u] = f.pp[u];
+  for (int u = 0; u < 4; ++u) pb[u * kPanRow + lane] = f.pp[u];
   ms.yp[lh * 16 + ll] = Yt[T][G];
   const R rin = f.rq;
   wave_sync();

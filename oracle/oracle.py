@@ -84,6 +84,11 @@ def lib():
         L.oq_adjoint_gradient.argtypes = [pp, ctypes.c_int, dp, dp, dp, dp, ctypes.c_double,
                                           dp, dp]
         L.oq_adjoint_gradient.restype = ctypes.c_int
+        L.oq_ned_basis_cell.argtypes = [ctypes.c_int, dp, dp, dp, dp, dp, ctypes.c_int, dp, dp,
+                                        dp]
+        L.oq_ned_basis_cell.restype = ctypes.c_int
+        L.oq_ned_assemble.argtypes = [ctypes.c_int, dp, dp, dp, dp, dp, dp, dp]
+        L.oq_ned_assemble.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -222,3 +227,71 @@ def adjoint_gradient(prob, nc, verts, coef, u_target, f=None, rtol=1e-14):
     if st != 0:
         raise RuntimeError(f"oq_adjoint_gradient failed ({st})")
     return J.value, grad
+
+
+# ---------------------------------------------------------------------------
+# 2-D Q1 x Nedelec basis (ned_oracle.c)
+# ---------------------------------------------------------------------------
+def ned_sizes(l):
+    """(ns, nh, nv, N): nodes, horizontal edges, vertical edges, all dofs."""
+    m = 1 << l
+    ns, nh, nv = (m + 1) ** 2, m * (m + 1), (m + 1) * m
+    return ns, nh, nv, ns + nh + nv
+
+
+def _ned_coefs(l, a, binv, r, f):
+    m = 1 << l
+    nf = m * m
+    a = np.ascontiguousarray(np.broadcast_to(np.asarray(a, dtype=np.float64), (nf,)))
+    binv = np.ascontiguousarray(np.broadcast_to(np.asarray(binv, dtype=np.float64), (nf,)))
+    r = np.ascontiguousarray(np.broadcast_to(np.asarray(r, dtype=np.float64), (nf,)))
+    f = np.ascontiguousarray(np.broadcast_to(np.asarray(f, dtype=np.float64), (nf, 2)))
+    return a, binv, r, f
+
+
+def ned_basis_cell(l, verts, a, binv=1.0, r=0.0, f=(1.0, 1.0), full_rhs=True):
+    """One coarse box cell: returns (M 8x8, b 8, Psi 8 x N, status)."""
+    a, binv, r, f = _ned_coefs(l, a, binv, r, f)
+    v = np.ascontiguousarray(verts, dtype=np.float64).reshape(4, 2)
+    N = ned_sizes(l)[3]
+    M = np.zeros((8, 8))
+    b = np.zeros(8)
+    Psi = np.zeros((8, N))
+    st = lib().oq_ned_basis_cell(l, _dp(v), _dp(a), _dp(binv), _dp(r), _dp(f), int(full_rhs),
+                                 _dp(M), _dp(b), _dp(Psi))
+    return M, b, Psi, st
+
+
+def ned_basis_batch(l, verts, a, binv=1.0, r=0.0, f=(1.0, 1.0), full_rhs=True):
+    """Cells of verts (n, 4, 2) with per-cell coefficient blocks a (n, m*m) etc."""
+    verts = np.asarray(verts, dtype=np.float64)
+    n = verts.shape[0]
+    m = 1 << l
+
+    def per_cell(x, shape):
+        x = np.asarray(x, dtype=np.float64)
+        return np.broadcast_to(x, (n,) + shape) if x.ndim <= len(shape) else x.reshape((n,) + shape)
+
+    a_, b_, r_ = per_cell(a, (m * m,)), per_cell(binv, (m * m,)), per_cell(r, (m * m,))
+    f_ = per_cell(f, (m * m, 2)) if np.asarray(f).ndim >= 2 else np.broadcast_to(
+        np.asarray(f, dtype=np.float64), (n, m * m, 2))
+    out = {"M": [], "b": [], "Psi": [], "status": []}
+    for c in range(n):
+        M, b, P, st = ned_basis_cell(l, verts[c], a_[c], b_[c], r_[c], f_[c], full_rhs)
+        out["M"].append(M)
+        out["b"].append(b)
+        out["Psi"].append(P)
+        out["status"].append(st)
+    return {k: np.array(v) for k, v in out.items()}
+
+
+def ned_assemble(l, verts, a, binv=1.0, r=0.0, f=(1.0, 1.0)):
+    """The assembled full system (row = test function) and the load (v, f)."""
+    a, binv, r, f = _ned_coefs(l, a, binv, r, f)
+    v = np.ascontiguousarray(verts, dtype=np.float64).reshape(4, 2)
+    N = ned_sizes(l)[3]
+    A = np.zeros((N, N))
+    F = np.zeros(N)
+    st = lib().oq_ned_assemble(l, _dp(v), _dp(a), _dp(binv), _dp(r), _dp(f), _dp(A), _dp(F))
+    assert st == 0
+    return A, F
