@@ -200,6 +200,66 @@ msfec_status msfec_q_adjoint_gradient(msfec_q_handle h, int nc, const double *u_
                                       int target_loc, double rel_tol, int max_iter,
                                       double *J, double *grad, int grad_loc);
 
+/* ---------------------------------------------------------------------------
+ * Q_Ned: the mixed Q1 x lowest-order Nedelec multiscale basis, two-dimensional
+ * (the reference's QNedBasis, source/Q_Ned/q_ned_basis.cc, is its 3-D form).
+ * Per coarse box cell and fine cell (2^l x 2^l, lexicographic, x fastest):
+ * a (curl-curl coefficient), binv (B^-1), r (reaction), f (2 components).
+ * 8 bases: 0..3 vertex bases (deal.II vertex order: sigma = phi_n and
+ * u x n = grad phi_n on the boundary, source (v, grad phi_n) when full_rhs),
+ * 4..7 edge bases (deal.II line order x = x0, x = x1, y = y0, y = y1:
+ * sigma = 0 and u x n = N_E on the boundary).  Fine degrees of freedom of a
+ * basis (ns + nu per cell, m = 2^l): sigma at the (m+1)^2 nodes (lexicographic),
+ * then u as line integrals of the tangential component (tangent +x / +y) on
+ * the m (m+1) horizontal edges (edge (i,j) at j m + i) and the (m+1) m
+ * vertical edges (edge (i,j) at j (m+1) + i).
+ * ------------------------------------------------------------------------- */
+typedef struct {
+  int n_refine_local; /* l, 1..4 (the interior half-bandwidth must stay <= 63) */
+  int full_rhs;       /* QNedBasis' full_rhs parameter (q_ned_basis.cc:432-444) */
+  int flags;          /* MSFEC_FLAG_*                                          */
+} msfec_ned_params;
+
+typedef struct msfec_ned_batch *msfec_ned_handle;
+
+/* Create a batch.  vertices: n_cells x 4 x 2 doubles, axis-aligned boxes.
+ * Replaces QNedBasis::QNedBasis + setup_grid/setup_system_matrix
+ * (source/Q_Ned/q_ned_basis.cc:27-217, include/Q_Ned/q_ned_basis.h:98-112). */
+msfec_status msfec_ned_create(const msfec_ned_params *params, int n_cells,
+                              const double *vertices, int vertices_loc,
+                              msfec_ned_handle *out);
+
+/* Per-fine-cell data, n_cells x 4^l doubles each (f: x 2).  Stands for the
+ * Diffusion_A / DiffusionInverse_B / ReactionRate / RightHandSide evaluations
+ * of QNedBasis::assemble_system (q_ned_basis.cc:392-412, 445-452). */
+msfec_status msfec_ned_set_coefficients(msfec_ned_handle h, const double *a,
+                                        const double *binv, const double *r,
+                                        const double *f, int loc);
+
+/* All cells: assembly, lifting, the interior solves of the 8 bases and the
+ * element matrix, enqueued on stream.  Replaces QNedBasis::run
+ * (q_ned_basis.cc:1281-1429) with solve_direct (:570-625). */
+msfec_status msfec_ned_compute(msfec_ned_handle h, void *stream);
+msfec_status msfec_ned_synchronize(msfec_ned_handle h);
+
+/* M: n_cells x 8 x 8 (row = test basis), b: n_cells x 8, as built by
+ * QNedBasis::assemble_global_element_matrix (q_ned_basis.cc:841-939) and
+ * returned by get_global_element_matrix/_rhs (include/Q_Ned/q_ned_basis.h:117-129). */
+msfec_status msfec_ned_get_element(msfec_ned_handle h, double *M, double *b, int loc);
+
+/* Psi: n_cells x 8 x (ns + nu), the fine coefficients of the bases
+ * (basis_h1_v / basis_curl_v of QNedBasis, renumbered as above). */
+msfec_status msfec_ned_get_basis(msfec_ned_handle h, double *Psi, int loc);
+
+/* Per-cell status: MSFEC_CELL_OK, MSFEC_CELL_NOT_SPD for a zero pivot,
+ * MSFEC_CELL_NONFINITE for non-finite data. */
+msfec_status msfec_ned_get_cell_status(msfec_ned_handle h, int *status);
+
+/* Mean per-kernel times in ms (rows, factor + solves, element) of the computes
+ * made with MSFEC_FLAG_TIMING since the previous call. */
+msfec_status msfec_ned_kernel_times(msfec_ned_handle h, float *ms, int max, int *n);
+msfec_status msfec_ned_destroy(msfec_ned_handle h);
+
 /* Thread-local description of the last error of this thread. */
 const char *msfec_last_error(void);
 const char *msfec_version(void);

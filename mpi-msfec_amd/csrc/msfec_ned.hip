@@ -1,0 +1,823 @@
+// Q_Ned: the mixed Q1 x lowest-order Nedelec multiscale basis in two
+// dimensions on MI355X (gfx950), behind the msfec_ned_* entry points of
+// include/msfec.h.
+//
+// Reference: source/Q_Ned/q_ned_basis.cc (three-dimensional QNedBasis).  Per
+// coarse box cell, on its 2^l x 2^l fine grid:
+//   (tau, B^-1 sigma) - (grad tau, u) = 0
+//   (v, grad sigma) + (curl v, A curl u) + (v, R u) = (v, grad phi_n)  [vertex bases, full_rhs]
+// (q_ned_basis.cc:483-491, 498-505), lifted from sigma = phi_n, u x n =
+// grad phi_n (vertex bases, :223-294) or sigma = 0, u x n = N_E (edge bases,
+// :296-357), solved directly (:570-625), then the element matrix of
+// :841-939.
+//
+// Layout of the work:
+//   ned_rows_kernel    one thread per interior row: the row of the interior
+//                      system with the sigma rows negated (which makes it
+//                      symmetric), in band order, its lower entries (at most
+//                      16) and the 8 condensed right-hand sides; the
+//                      boundary values of the 8 bases to Psi.
+//   ned_factor_kernel  one 256-thread workgroup per cell: LDL^T of the
+//                      equilibrated band matrix in a 64-row LDS window, the
+//                      forward elimination of the 8 right-hand sides with it,
+//                      then the backward sweep; the interior of Psi.
+//   ned_element_kernel one workgroup per cell: the 8 x 8 element matrix and
+//                      the load vector from Psi staged in LDS.
+// Band order: node row j (nodes (i,j)), interleaved with the horizontal
+// edges of row j and the vertical edges between rows j-1 and j, every edge
+// after its interior end nodes.  With that order every leading principal
+// minor of the symmetric matrix [[-M, G^T], [G, C]] is non-singular (an
+// edge field that is a discrete gradient of nodes already eliminated and
+// orthogonal to their tests vanishes), so LDL^T needs no pivoting; the
+// half-bandwidth is 3m + 3 (51 at l = 4).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <set>
+#include <string>
+#include <vector>
+
+#include "msfec.h"
+
+namespace msfec_detail {
+extern thread_local std::string g_last_error;
+}
+
+namespace {
+
+constexpr int kNLow = 16;                // lower entries of a band row, diagonal first
+constexpr int kNW = 64;                  // window slots: the half-bandwidth stays <= 63
+constexpr int kNR = 8;                   // bases per cell
+constexpr int kRowLen = kNLow + kNR;     // row table: lower values, right-hand sides
+constexpr int kZLen = kNR + 1;           // z_k of the 8 bases, then the row scale s_k
+constexpr int kNThreads = 256;
+constexpr int kMaxPairs = 63 * 64 / 2;
+
+struct NedArgs {
+  int m, ns, nh, N;  // fine mesh: m cells per direction; nodes, horizontal edges, all dofs
+  int n, p, npairs, full_rhs;
+  const int* order;  // [n] band row -> fine dof
+  const int* pos;    // [N] fine dof -> band row, -1 on the boundary
+  const int* loff;   // [n][kNLow] offsets row - column of the lower entries, -1 unused
+  const int* pairs;  // [npairs] (io << 8) | jo, 1 <= jo <= io <= p
+  const double* verts;
+  const double* a;
+  const double* binv;
+  const double* r;
+  const double* f;
+  double* rows;  // [cells][n][kRowLen]
+  double* pan;   // [cells][n][kNW]: d_k at 0, l_{k+o,k} at o = 1..p
+  double* zs;    // [cells][n][kZLen]
+  double* psi;   // [cells][8][N]
+  double* M;     // [cells][64]
+  double* b;     // [cells][8]
+  int* status;
+};
+
+struct NedBox {
+  double hx, hy;
+};
+
+__device__ __forceinline__ NedBox ned_box(const NedArgs& a, int cell) {
+  const double* v = a.verts + (size_t)cell * 8;
+  return NedBox{(v[2] - v[0]) / a.m, (v[5] - v[1]) / a.m};
+}
+
+// fine dof -> (type, i, j): 0 node (i,j); 1 horizontal edge (i,j)-(i+1,j);
+// 2 vertical edge (i,j)-(i,j+1)
+__device__ __forceinline__ void ned_decode(const NedArgs& a, int d, int& type, int& i, int& j) {
+  if (d < a.ns) {
+    type = 0;
+    j = d / (a.m + 1);
+    i = d - j * (a.m + 1);
+  } else if (d < a.ns + a.nh) {
+    type = 1;
+    d -= a.ns;
+    j = d / a.m;
+    i = d - j * a.m;
+  } else {
+    type = 2;
+    d -= a.ns + a.nh;
+    j = d / (a.m + 1);
+    i = d - j * (a.m + 1);
+  }
+}
+
+// dofs of fine cell (i, j): nodes in deal.II vertex order, then the lines in
+// deal.II line order (x = left, x = right, y = bottom, y = top)
+__device__ __forceinline__ void ned_cell_dofs(const NedArgs& a, int i, int j, int (&d)[8]) {
+  const int m = a.m;
+  d[0] = j * (m + 1) + i;
+  d[1] = d[0] + 1;
+  d[2] = d[0] + m + 1;
+  d[3] = d[2] + 1;
+  d[4] = a.ns + a.nh + j * (m + 1) + i;
+  d[5] = d[4] + 1;
+  d[6] = a.ns + j * m + i;
+  d[7] = d[6] + m;
+}
+
+// Exact integrals of the mixed form over a fine box (hx, hy) with constant
+// coefficients (the QGauss(3) rule of the reference integrates them
+// exactly).  K[test][trial], local order as ned_cell_dofs.
+//   Q1 mass           binv hx hy / 36 [4 2 2 1; 2 4 1 2; 2 1 4 2; 1 2 2 4]
+//   G[e][k]           (N_e, grad phi_k)
+//   curl N_e          s_e / (hx hy), s = (-1, 1, 1, -1)
+//   Nedelec mass      hx / (3 hy), hx / (6 hy) for the vertical pair,
+//                     hy / (3 hx), hy / (6 hx) for the horizontal pair
+__device__ __forceinline__ void ned_local(double hx, double hy, double ac, double bc, double rc,
+                                          double (&K)[8][8]) {
+  const double mq = bc * hx * hy / 36.0;
+  const double gx = hx / hy, gy = hy / hx;
+  const double Mq[4][4] = {{4, 2, 2, 1}, {2, 4, 1, 2}, {2, 1, 4, 2}, {1, 2, 2, 4}};
+  const double G[4][4] = {{-gx / 3, -gx / 6, gx / 3, gx / 6},
+                          {-gx / 6, -gx / 3, gx / 6, gx / 3},
+                          {-gy / 3, gy / 3, -gy / 6, gy / 6},
+                          {-gy / 6, gy / 6, -gy / 3, gy / 3}};
+  const double s[4] = {-1, 1, 1, -1};
+  const double cc = ac / (hx * hy);
+#pragma unroll
+  for (int p = 0; p < 4; ++p)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) K[p][k] = mq * Mq[p][k];
+#pragma unroll
+  for (int e = 0; e < 4; ++e)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      K[k][4 + e] = -G[e][k];
+      K[4 + e][k] = G[e][k];
+    }
+#pragma unroll
+  for (int e = 0; e < 4; ++e)
+#pragma unroll
+    for (int h = 0; h < 4; ++h) {
+      double mass = 0.0;
+      if ((e >> 1) == (h >> 1)) mass = ((e >> 1) == 0 ? gx : gy) * (e == h ? 1.0 / 3 : 1.0 / 6);
+      K[4 + e][4 + h] = cc * s[e] * s[h] + rc * mass;
+    }
+}
+
+// (N_e, grad phi_n) over fine cell (i, j) for the coarse bilinear phi_n:
+// d_x phi_n depends on y only, d_y phi_n on x only, both linearly.
+__device__ __forceinline__ double ned_source(int m, double hx, double hy, int i, int j, int n,
+                                             int e) {
+  const double Hx = hx * m, Hy = hy * m;
+  auto lin = [](int hi, double t) { return hi ? t : 1.0 - t; };
+  if (e < 2) {  // vertical line: (0, lin/hy) . grad phi_n -> d_y phi_n(x)
+    const double dy = ((n >> 1) ? 1.0 : -1.0) / Hy;
+    const double h0 = lin(n & 1, (double)i / m) * dy, h1 = lin(n & 1, (double)(i + 1) / m) * dy;
+    return e == 0 ? hx * (h0 / 3 + h1 / 6) : hx * (h0 / 6 + h1 / 3);
+  }
+  const double dx = ((n & 1) ? 1.0 : -1.0) / Hx;
+  const double g0 = lin(n >> 1, (double)j / m) * dx, g1 = lin(n >> 1, (double)(j + 1) / m) * dx;
+  return e == 2 ? hy * (g0 / 3 + g1 / 6) : hy * (g0 / 6 + g1 / 3);
+}
+
+// Boundary value of basis b at boundary dof d (setup_basis_dofs_h1/_curl):
+// vertex basis: phi_n at nodes, phi_n(end) - phi_n(start) on edges (the
+// line integral of grad phi_n); edge basis E: 1/m on the fine edges of
+// coarse edge E (the line integral of N_E), 0 elsewhere.
+__device__ __forceinline__ double ned_bval(const NedArgs& a, int d, int b) {
+  int type, i, j;
+  ned_decode(a, d, type, i, j);
+  const double m = a.m;
+  auto phi = [](int n, double X, double Y) {
+    return ((n & 1) ? X : 1.0 - X) * ((n >> 1) ? Y : 1.0 - Y);
+  };
+  if (type == 0) return b < 4 ? phi(b, i / m, j / m) : 0.0;
+  if (type == 1) {
+    if (b < 4) return phi(b, (i + 1) / m, j / m) - phi(b, i / m, j / m);
+    return ((b == 6 && j == 0) || (b == 7 && j == a.m)) ? 1.0 / m : 0.0;
+  }
+  if (b < 4) return phi(b, i / m, (j + 1) / m) - phi(b, i / m, j / m);
+  return ((b == 4 && i == 0) || (b == 5 && i == a.m)) ? 1.0 / m : 0.0;
+}
+
+__global__ __launch_bounds__(kNThreads) void ned_rows_kernel(NedArgs a) {
+  const int cell = blockIdx.x;
+  const NedBox bx = ned_box(a, cell);
+  const int m = a.m, nf = m * m;
+  const double* ca = a.a + (size_t)cell * nf;
+  const double* cb = a.binv + (size_t)cell * nf;
+  const double* cr = a.r + (size_t)cell * nf;
+  for (int r = threadIdx.x; r < a.n; r += blockDim.x) {
+    const int d = a.order[r];
+    int type, i, j;
+    ned_decode(a, d, type, i, j);
+    // fine cells around the dof and its local index in each
+    int ci[4], cj[4], lp[4], nc;
+    if (type == 0) {
+      ci[0] = i - 1; cj[0] = j - 1; lp[0] = 3;
+      ci[1] = i;     cj[1] = j - 1; lp[1] = 2;
+      ci[2] = i - 1; cj[2] = j;     lp[2] = 1;
+      ci[3] = i;     cj[3] = j;     lp[3] = 0;
+      nc = 4;
+    } else if (type == 1) {
+      ci[0] = i; cj[0] = j - 1; lp[0] = 7;
+      ci[1] = i; cj[1] = j;     lp[1] = 6;
+      ci[2] = ci[3] = i; cj[2] = cj[3] = j; lp[2] = lp[3] = 6;
+      nc = 2;
+    } else {
+      ci[0] = i - 1; cj[0] = j; lp[0] = 5;
+      ci[1] = i;     cj[1] = j; lp[1] = 4;
+      ci[2] = ci[3] = i; cj[2] = cj[3] = j; lp[2] = lp[3] = 4;
+      nc = 2;
+    }
+    int lo[kNLow];
+#pragma unroll
+    for (int t = 0; t < kNLow; ++t) lo[t] = a.loff[(size_t)r * kNLow + t];
+    double low[kNLow], rhs[kNR];
+#pragma unroll
+    for (int t = 0; t < kNLow; ++t) low[t] = 0.0;
+#pragma unroll
+    for (int t = 0; t < kNR; ++t) rhs[t] = 0.0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if (q >= nc) break;
+      const int fc = cj[q] * m + ci[q];
+      double K[8][8];
+      ned_local(bx.hx, bx.hy, ca[fc], cb[fc], cr[fc], K);
+      int dofs[8];
+      ned_cell_dofs(a, ci[q], cj[q], dofs);
+      const int p = lp[q];
+      const double sg = p < 4 ? -1.0 : 1.0;  // sigma rows negated
+      double kr[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        // row p of K, selected without dynamic register indexing
+        double v = 0.0;
+#pragma unroll
+        for (int pp = 0; pp < 8; ++pp) v = (pp == p) ? K[pp][k] : v;
+        kr[k] = sg * v;
+      }
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const int pc = a.pos[dofs[k]];
+        if (pc >= 0) {
+          const int o = r - pc;
+#pragma unroll
+          for (int t = 0; t < kNLow; ++t)
+            if (lo[t] == o) low[t] += kr[k];
+        } else {
+#pragma unroll
+          for (int bb = 0; bb < kNR; ++bb) rhs[bb] -= kr[k] * ned_bval(a, dofs[k], bb);
+        }
+      }
+      if (a.full_rhs && p >= 4)
+#pragma unroll
+        for (int n = 0; n < 4; ++n) rhs[n] += ned_source(m, bx.hx, bx.hy, ci[q], cj[q], n, p - 4);
+    }
+    double* out = a.rows + ((size_t)cell * a.n + r) * kRowLen;
+#pragma unroll
+    for (int t = 0; t < kNLow; ++t) out[t] = low[t];
+#pragma unroll
+    for (int t = 0; t < kNR; ++t) out[kNLow + t] = rhs[t];
+  }
+  for (int d = threadIdx.x; d < a.N; d += blockDim.x)
+    if (a.pos[d] < 0)
+#pragma unroll
+      for (int bb = 0; bb < kNR; ++bb) a.psi[((size_t)cell * kNR + bb) * a.N + d] = ned_bval(a, d, bb);
+}
+
+// LDL^T of the equilibrated band matrix S K S (S = |diag K|^-1/2), rows
+// entering a 64-row LDS window as earlier ones leave; the right-hand sides
+// are eliminated alongside; then x = S (L^-T D^-1 z).
+__global__ __launch_bounds__(kNThreads) void ned_factor_kernel(NedArgs a) {
+  const int cell = blockIdx.x, tid = threadIdx.x;
+  const int n = a.n, p = a.p;
+  __shared__ double W[kNW][kNW + 1];
+  __shared__ double Y[kNW][kNR];
+  __shared__ double srow[kNW];
+  __shared__ double lcol[kNW];
+  __shared__ double rowbuf[kRowLen];
+  __shared__ int offbuf[kNLow];
+  __shared__ int pairs[kMaxPairs];
+  __shared__ int bad;
+  const double* rows = a.rows + (size_t)cell * n * kRowLen;
+  double* pan = a.pan + (size_t)cell * n * kNW;
+  double* zs = a.zs + (size_t)cell * n * kZLen;
+  for (int q = tid; q < a.npairs; q += kNThreads) pairs[q] = a.pairs[q];
+  if (tid == 0) bad = MSFEC_CELL_OK;
+
+  // row i (staged in rowbuf / offbuf) into slot i mod 64: thread o <= p
+  // writes column i - o, scaled by s_i s_col
+  auto insert = [&](int i) {
+    const int si = i & (kNW - 1);
+    const double dg = fabs(rowbuf[0]);
+    const double s_i = dg > 0.0 ? 1.0 / sqrt(dg) : 1.0;
+    if (tid <= p && i - tid >= 0) {
+      double v = 0.0;
+#pragma unroll
+      for (int t = 0; t < kNLow; ++t)
+        if (offbuf[t] == tid) v = rowbuf[t];
+      const int col = i - tid;
+      const double sc = tid == 0 ? s_i : srow[col & (kNW - 1)];
+      W[si][col & (kNW - 1)] = v * s_i * sc;
+    }
+    if (tid >= kNW && tid < kNW + kNR) Y[si][tid - kNW] = rowbuf[kNLow + tid - kNW] * s_i;
+    // slot si is read by no other thread here (its columns are i - o, o >= 1)
+    if (tid == 0) {
+      srow[si] = s_i;
+      zs[(size_t)i * kZLen + kNR] = s_i;
+    }
+  };
+  auto stage = [&](int i) {
+    if (tid < kRowLen) rowbuf[tid] = rows[(size_t)i * kRowLen + tid];
+    else if (tid < kRowLen + kNLow) offbuf[tid - kRowLen] = a.loff[(size_t)i * kNLow + tid - kRowLen];
+  };
+
+  __syncthreads();
+  for (int i = 0; i < min(n, kNW); ++i) {
+    stage(i);
+    __syncthreads();
+    insert(i);
+    __syncthreads();
+  }
+  for (int k = 0; k < n; ++k) {
+    const int sk = k & (kNW - 1);
+    const int inew = k + kNW;
+    // the entering row, loaded during the elimination step
+    double pv = 0.0;
+    int po = -1;
+    if (inew < n) {
+      if (tid < kRowLen) pv = rows[(size_t)inew * kRowLen + tid];
+      else if (tid < kRowLen + kNLow) po = a.loff[(size_t)inew * kNLow + tid - kRowLen];
+    }
+    const double dk = W[sk][sk];
+    if (tid >= 1 && tid <= p) {
+      const int i = k + tid;
+      const double l = (i < n) ? W[i & (kNW - 1)][sk] / dk : 0.0;
+      lcol[tid] = l;
+      pan[(size_t)k * kNW + tid] = l;
+    }
+    if (tid == 0) {
+      pan[(size_t)k * kNW] = dk;
+      if (!(dk != 0.0)) bad = MSFEC_CELL_NOT_SPD;
+      else if (!isfinite(dk) && bad == MSFEC_CELL_OK) bad = MSFEC_CELL_NONFINITE;
+    }
+    if (tid >= kNW && tid < kNW + kNR) zs[(size_t)k * kZLen + tid - kNW] = Y[sk][tid - kNW];
+    __syncthreads();
+    for (int q = tid; q < a.npairs; q += kNThreads) {
+      const int pr = pairs[q], io = pr >> 8, jo = pr & 255;
+      if (k + io < n)
+        W[(k + io) & (kNW - 1)][(k + jo) & (kNW - 1)] -= lcol[io] * W[(k + jo) & (kNW - 1)][sk];
+    }
+    for (int q = tid; q < p * kNR; q += kNThreads) {
+      const int io = 1 + q / kNR, bb = q - (io - 1) * kNR;
+      if (k + io < n) Y[(k + io) & (kNW - 1)][bb] -= lcol[io] * Y[sk][bb];
+    }
+    if (inew < n) {
+      if (tid < kRowLen) rowbuf[tid] = pv;
+      else if (tid < kRowLen + kNLow) offbuf[tid - kRowLen] = po;
+    }
+    __syncthreads();
+    if (inew < n) insert(inew);
+    __syncthreads();
+  }
+
+  // backward sweep: x'_k = z_k / d_k - sum_o l_{k+o,k} x'_{k+o}; lane group
+  // of 32 per basis, x' ring in Y
+  for (int t = tid; t < kNW * kNR; t += kNThreads) Y[t / kNR][t % kNR] = 0.0;
+  __syncthreads();
+  const int bb = tid >> 5, l32 = tid & 31;
+  for (int k = n - 1; k >= 0; --k) {
+    const double* pk = pan + (size_t)k * kNW;
+    double part = 0.0;
+    for (int o = l32 + 1; o <= p; o += 32)
+      if (k + o < n) part += pk[o] * Y[(k + o) & (kNW - 1)][bb];
+#pragma unroll
+    for (int w = 16; w > 0; w >>= 1) part += __shfl_xor(part, w);
+    __syncthreads();
+    if (l32 == 0) {
+      const double x = zs[(size_t)k * kZLen + bb] / pk[0] - part;
+      Y[k & (kNW - 1)][bb] = x;
+      a.psi[((size_t)cell * kNR + bb) * a.N + a.order[k]] = x * zs[(size_t)k * kZLen + kNR];
+    }
+    __syncthreads();
+  }
+  if (tid == 0) a.status[cell] = bad;
+}
+
+// Element matrix (q_ned_basis.cc:841-939): M[kt][kc] = psi_kt^T A psi_kc on
+// the sigma block for vertex bases and the u block for edge bases; b[kt] =
+// (psi_kt u, f) for the edge bases.  Psi of the cell staged in LDS; 3 groups
+// of 72 threads (64 matrix entries + 8 loads) share the fine cells.
+__global__ __launch_bounds__(kNThreads) void ned_element_kernel(NedArgs a) {
+  extern __shared__ double sm[];
+  const int cell = blockIdx.x, tid = threadIdx.x;
+  const int N = a.N, m = a.m, nf = m * m;
+  double* ps = sm;               // [8][N]
+  double* part = sm + kNR * N;   // [3][72]
+  for (int t = tid; t < kNR * N; t += kNThreads) ps[t] = a.psi[(size_t)cell * kNR * N + t];
+  __syncthreads();
+  const NedBox bx = ned_box(a, cell);
+  const int e = tid % 72, grp = tid / 72;
+  double acc = 0.0;
+  if (grp < 3) {
+    const int kt = e < 64 ? e >> 3 : 4 + ((e - 64) & 3);
+    const int kc = e & 7;
+    const int r0 = kt < 4 ? 0 : 4, c0 = kc < 4 ? 0 : 4;
+    const bool isb = e >= 64;
+    for (int fc = grp; fc < nf; fc += 3) {
+      const int i = fc % m, j = fc / m;
+      int d[8];
+      ned_cell_dofs(a, i, j, d);
+      const size_t c = (size_t)cell * nf + fc;
+      if (!isb) {
+        double K[8][8];
+        ned_local(bx.hx, bx.hy, a.a[c], a.binv[c], a.r[c], K);
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+          double t = 0.0;
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            // K[r0 + p][c0 + q] without dynamic register indexing
+            const double kv = r0 == 0 ? (c0 == 0 ? K[p][q] : K[p][4 + q])
+                                      : (c0 == 0 ? K[4 + p][q] : K[4 + p][4 + q]);
+            t += kv * ps[kc * N + d[c0 + q]];
+          }
+          acc += ps[kt * N + d[r0 + p]] * t;
+        }
+      } else if (e < 68) {
+        const double fx = a.f[2 * c], fy = a.f[2 * c + 1];
+        const double lf[4] = {fy * bx.hx / 2, fy * bx.hx / 2, fx * bx.hy / 2, fx * bx.hy / 2};
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc += ps[kt * N + d[4 + q]] * lf[q];
+      }
+    }
+    part[grp * 72 + e] = acc;
+  }
+  __syncthreads();
+  if (tid < 72) {
+    const double v = part[tid] + part[72 + tid] + part[144 + tid];
+    if (tid < 64) a.M[(size_t)cell * 64 + tid] = v;
+    else if (tid < 68) {
+      a.b[(size_t)cell * kNR + tid - 64] = 0.0;  // vertex bases: no right-hand side
+      a.b[(size_t)cell * kNR + 4 + tid - 64] = v;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+using msfec_detail::g_last_error;
+
+msfec_status ned_fail(msfec_status st, const std::string& msg) {
+  g_last_error = msg;
+  return st;
+}
+
+#define NED_HIP(call)                                                                     \
+  do {                                                                                    \
+    hipError_t e_ = (call);                                                               \
+    if (e_ != hipSuccess)                                                                 \
+      return ned_fail(MSFEC_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// band order, its inverse, lower offsets and the pair list of one refinement level
+struct NedConf {
+  int l = 0, m = 0, ns = 0, nh = 0, nv = 0, N = 0, n = 0, p = 0;
+  std::vector<int> order, pos, loff, pairs;
+};
+
+bool ned_conf(int l, NedConf& c, std::string& why) {
+  c.l = l;
+  c.m = 1 << l;
+  const int m = c.m;
+  c.ns = (m + 1) * (m + 1);
+  c.nh = m * (m + 1);
+  c.nv = (m + 1) * m;
+  c.N = c.ns + c.nh + c.nv;
+  auto node = [&](int i, int j) { return j * (m + 1) + i; };
+  auto hed = [&](int i, int j) { return c.ns + j * m + i; };
+  auto ved = [&](int i, int j) { return c.ns + c.nh + j * (m + 1) + i; };
+  c.order.clear();
+  for (int j = 1; j < m; ++j)
+    for (int i = 0; i <= m; ++i) {
+      if (i >= 1 && i <= m - 1) c.order.push_back(node(i, j));
+      if (i >= 1) c.order.push_back(hed(i - 1, j));
+      if (i >= 1 && i <= m - 1) c.order.push_back(ved(i, j - 1));
+    }
+  for (int i = 1; i < m; ++i) c.order.push_back(ved(i, m - 1));
+  c.n = (int)c.order.size();
+  c.pos.assign(c.N, -1);
+  for (int r = 0; r < c.n; ++r) c.pos[c.order[r]] = r;
+  // dofs sharing a fine cell with each dof
+  std::vector<std::set<int>> nb(c.N);
+  for (int j = 0; j < m; ++j)
+    for (int i = 0; i < m; ++i) {
+      const int d[8] = {node(i, j),     node(i + 1, j),     node(i, j + 1), node(i + 1, j + 1),
+                        ved(i, j),      ved(i + 1, j),      hed(i, j),      hed(i, j + 1)};
+      for (int x : d)
+        for (int y : d) nb[x].insert(y);
+    }
+  c.loff.assign((size_t)c.n * kNLow, -1);
+  c.p = 0;
+  for (int r = 0; r < c.n; ++r) {
+    std::vector<int> offs;
+    for (int g : nb[c.order[r]])
+      if (c.pos[g] >= 0 && c.pos[g] <= r) offs.push_back(r - c.pos[g]);
+    std::sort(offs.begin(), offs.end());
+    if ((int)offs.size() > kNLow || offs.empty() || offs[0] != 0) {
+      why = "band row structure exceeds the supported stencil";
+      return false;
+    }
+    for (size_t t = 0; t < offs.size(); ++t) c.loff[(size_t)r * kNLow + t] = offs[t];
+    c.p = std::max(c.p, offs.back());
+  }
+  if (c.p > kNW - 1) {
+    why = "half-bandwidth " + std::to_string(c.p) + " exceeds 63 (l <= 4)";
+    return false;
+  }
+  c.pairs.clear();
+  for (int io = 1; io <= c.p; ++io)
+    for (int jo = 1; jo <= io; ++jo) c.pairs.push_back((io << 8) | jo);
+  return true;
+}
+
+}  // namespace
+
+struct msfec_ned_batch {
+  msfec_ned_params prm{};
+  NedConf conf;
+  int n_cells = 0;
+  double* d_verts = nullptr;
+  double* d_a = nullptr;
+  double* d_binv = nullptr;
+  double* d_r = nullptr;
+  double* d_f = nullptr;
+  double* d_rows = nullptr;
+  double* d_pan = nullptr;
+  double* d_zs = nullptr;
+  double* d_psi = nullptr;
+  double* d_M = nullptr;
+  double* d_b = nullptr;
+  int* d_status = nullptr;
+  int* d_order = nullptr;
+  int* d_pos = nullptr;
+  int* d_loff = nullptr;
+  int* d_pairs = nullptr;
+  bool coef_set = false, computed = false;
+  hipStream_t stream = nullptr;
+  std::vector<std::array<hipEvent_t, 4>> ev_sets;
+  size_t ev_used = 0;
+};
+
+namespace {
+
+template <typename T>
+msfec_status ned_alloc(T** p, size_t count) {
+  NED_HIP(hipMalloc((void**)p, std::max<size_t>(count, 1) * sizeof(T)));
+  return MSFEC_OK;
+}
+
+void ned_free(msfec_ned_batch* h) {
+  void* ptrs[] = {h->d_verts, h->d_a,   h->d_binv,   h->d_r,     h->d_f,   h->d_rows,
+                  h->d_pan,   h->d_zs,  h->d_psi,    h->d_M,     h->d_b,   h->d_status,
+                  h->d_order, h->d_pos, h->d_loff,   h->d_pairs};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  for (auto& set : h->ev_sets)
+    for (auto& e : set)
+      if (e) (void)hipEventDestroy(e);
+  h->ev_sets.clear();
+}
+
+msfec_status ned_in(double* dst, const double* src, size_t count, int loc) {
+  if (!src) return ned_fail(MSFEC_ERR_INVALID, "null data pointer");
+  NED_HIP(hipMemcpy(dst, src, count * sizeof(double),
+                    loc == MSFEC_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice));
+  return MSFEC_OK;
+}
+
+msfec_status ned_out(double* dst, const double* src, size_t count, int loc) {
+  if (!dst) return ned_fail(MSFEC_ERR_INVALID, "null output pointer");
+  NED_HIP(hipMemcpy(dst, src, count * sizeof(double),
+                    loc == MSFEC_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost));
+  return MSFEC_OK;
+}
+
+NedArgs ned_args(const msfec_ned_batch* h) {
+  const NedConf& c = h->conf;
+  NedArgs a{};
+  a.m = c.m;
+  a.ns = c.ns;
+  a.nh = c.nh;
+  a.N = c.N;
+  a.n = c.n;
+  a.p = c.p;
+  a.npairs = (int)c.pairs.size();
+  a.full_rhs = h->prm.full_rhs ? 1 : 0;
+  a.order = h->d_order;
+  a.pos = h->d_pos;
+  a.loff = h->d_loff;
+  a.pairs = h->d_pairs;
+  a.verts = h->d_verts;
+  a.a = h->d_a;
+  a.binv = h->d_binv;
+  a.r = h->d_r;
+  a.f = h->d_f;
+  a.rows = h->d_rows;
+  a.pan = h->d_pan;
+  a.zs = h->d_zs;
+  a.psi = h->d_psi;
+  a.M = h->d_M;
+  a.b = h->d_b;
+  a.status = h->d_status;
+  return a;
+}
+
+}  // namespace
+
+extern "C" {
+
+msfec_status msfec_ned_create(const msfec_ned_params* params, int n_cells, const double* vertices,
+                              int vertices_loc, msfec_ned_handle* out) {
+  if (!params || !vertices || !out) return ned_fail(MSFEC_ERR_INVALID, "null argument");
+  *out = nullptr;
+  if (n_cells <= 0) return ned_fail(MSFEC_ERR_INVALID, "n_cells must be positive");
+  const int l = params->n_refine_local;
+  if (l < 1 || l > 6) return ned_fail(MSFEC_ERR_INVALID, "n_refine_local must be in 1..6");
+  if (vertices_loc != MSFEC_HOST && vertices_loc != MSFEC_DEVICE)
+    return ned_fail(MSFEC_ERR_INVALID, "bad vertices location");
+  std::vector<double> hv((size_t)n_cells * 8);
+  if (vertices_loc == MSFEC_DEVICE) {
+    NED_HIP(hipMemcpy(hv.data(), vertices, hv.size() * sizeof(double), hipMemcpyDeviceToHost));
+  } else {
+    std::copy(vertices, vertices + hv.size(), hv.begin());
+  }
+  for (int c = 0; c < n_cells; ++c) {
+    const double* v = &hv[(size_t)c * 8];
+    const double Hx = v[2] - v[0], Hy = v[5] - v[1];
+    const bool box = Hx > 0 && Hy > 0 && std::fabs(v[3] - v[1]) <= 1e-14 * Hy &&
+                     std::fabs(v[4] - v[0]) <= 1e-14 * Hx && std::fabs(v[6] - v[2]) <= 1e-14 * Hx &&
+                     std::fabs(v[7] - v[5]) <= 1e-14 * Hy;
+    if (!box)
+      return ned_fail(MSFEC_ERR_INVALID,
+                      "cell " + std::to_string(c) + " is not an axis-aligned box in deal.II vertex order");
+  }
+  msfec_ned_batch* h = new msfec_ned_batch();
+  h->prm = *params;
+  h->n_cells = n_cells;
+  std::string why;
+  if (!ned_conf(l, h->conf, why)) {
+    delete h;
+    return ned_fail(MSFEC_ERR_UNSUPPORTED, why);
+  }
+  const NedConf& c = h->conf;
+  const size_t nc = (size_t)n_cells, nf = (size_t)c.m * c.m;
+  msfec_status st = MSFEC_OK;
+#define NED_TRY(x)                 \
+  do {                             \
+    st = (x);                      \
+    if (st != MSFEC_OK) {          \
+      ned_free(h);                 \
+      delete h;                    \
+      return st;                   \
+    }                              \
+  } while (0)
+  NED_TRY(ned_alloc(&h->d_verts, nc * 8));
+  NED_TRY(ned_alloc(&h->d_a, nc * nf));
+  NED_TRY(ned_alloc(&h->d_binv, nc * nf));
+  NED_TRY(ned_alloc(&h->d_r, nc * nf));
+  NED_TRY(ned_alloc(&h->d_f, nc * nf * 2));
+  NED_TRY(ned_alloc(&h->d_rows, nc * c.n * kRowLen));
+  NED_TRY(ned_alloc(&h->d_pan, nc * c.n * kNW));
+  NED_TRY(ned_alloc(&h->d_zs, nc * c.n * kZLen));
+  NED_TRY(ned_alloc(&h->d_psi, nc * kNR * c.N));
+  NED_TRY(ned_alloc(&h->d_M, nc * 64));
+  NED_TRY(ned_alloc(&h->d_b, nc * kNR));
+  NED_TRY(ned_alloc(&h->d_status, nc));
+  NED_TRY(ned_alloc(&h->d_order, c.order.size()));
+  NED_TRY(ned_alloc(&h->d_pos, c.pos.size()));
+  NED_TRY(ned_alloc(&h->d_loff, c.loff.size()));
+  NED_TRY(ned_alloc(&h->d_pairs, c.pairs.size()));
+#undef NED_TRY
+  auto up = [&](void* dst, const void* src, size_t bytes) -> msfec_status {
+    NED_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return MSFEC_OK;
+  };
+  msfec_status s2 = MSFEC_OK;
+  if (s2 == MSFEC_OK) s2 = up(h->d_verts, hv.data(), hv.size() * sizeof(double));
+  if (s2 == MSFEC_OK) s2 = up(h->d_order, c.order.data(), c.order.size() * sizeof(int));
+  if (s2 == MSFEC_OK) s2 = up(h->d_pos, c.pos.data(), c.pos.size() * sizeof(int));
+  if (s2 == MSFEC_OK) s2 = up(h->d_loff, c.loff.data(), c.loff.size() * sizeof(int));
+  if (s2 == MSFEC_OK && !c.pairs.empty())
+    s2 = up(h->d_pairs, c.pairs.data(), c.pairs.size() * sizeof(int));
+  if (s2 != MSFEC_OK) {
+    ned_free(h);
+    delete h;
+    return s2;
+  }
+  *out = h;
+  return MSFEC_OK;
+}
+
+msfec_status msfec_ned_set_coefficients(msfec_ned_handle h, const double* a, const double* binv,
+                                        const double* r, const double* f, int loc) {
+  if (!h) return ned_fail(MSFEC_ERR_INVALID, "null handle");
+  if (loc != MSFEC_HOST && loc != MSFEC_DEVICE) return ned_fail(MSFEC_ERR_INVALID, "bad location");
+  const size_t cnt = (size_t)h->n_cells * h->conf.m * h->conf.m;
+  msfec_status st = ned_in(h->d_a, a, cnt, loc);
+  if (st == MSFEC_OK) st = ned_in(h->d_binv, binv, cnt, loc);
+  if (st == MSFEC_OK) st = ned_in(h->d_r, r, cnt, loc);
+  if (st == MSFEC_OK) st = ned_in(h->d_f, f, 2 * cnt, loc);
+  if (st != MSFEC_OK) return st;
+  h->coef_set = true;
+  return MSFEC_OK;
+}
+
+msfec_status msfec_ned_compute(msfec_ned_handle h, void* stream) {
+  if (!h) return ned_fail(MSFEC_ERR_INVALID, "null handle");
+  if (!h->coef_set) return ned_fail(MSFEC_ERR_STATE, "coefficients not set");
+  hipStream_t s = (hipStream_t)stream;
+  h->stream = s;
+  const NedArgs a = ned_args(h);
+  const bool timing = (h->prm.flags & MSFEC_FLAG_TIMING) != 0;
+  std::array<hipEvent_t, 4>* ev = nullptr;
+  if (timing) {
+    if (h->ev_used == h->ev_sets.size()) {
+      std::array<hipEvent_t, 4> set{};
+      for (auto& e : set) NED_HIP(hipEventCreate(&e));
+      h->ev_sets.push_back(set);
+    }
+    ev = &h->ev_sets[h->ev_used++];
+    NED_HIP(hipEventRecord((*ev)[0], s));
+  }
+  const size_t elem_lds = ((size_t)kNR * h->conf.N + 3 * 72) * sizeof(double);
+  if (elem_lds > 160 * 1024) return ned_fail(MSFEC_ERR_UNSUPPORTED, "cell too large for LDS staging");
+  hipLaunchKernelGGL(ned_rows_kernel, dim3(h->n_cells), dim3(kNThreads), 0, s, a);
+  NED_HIP(hipGetLastError());
+  if (ev) NED_HIP(hipEventRecord((*ev)[1], s));
+  hipLaunchKernelGGL(ned_factor_kernel, dim3(h->n_cells), dim3(kNThreads), 0, s, a);
+  NED_HIP(hipGetLastError());
+  if (ev) NED_HIP(hipEventRecord((*ev)[2], s));
+  hipLaunchKernelGGL(ned_element_kernel, dim3(h->n_cells), dim3(kNThreads), elem_lds, s, a);
+  NED_HIP(hipGetLastError());
+  if (ev) NED_HIP(hipEventRecord((*ev)[3], s));
+  h->computed = true;
+  return MSFEC_OK;
+}
+
+msfec_status msfec_ned_synchronize(msfec_ned_handle h) {
+  if (!h) return ned_fail(MSFEC_ERR_INVALID, "null handle");
+  NED_HIP(hipStreamSynchronize(h->stream));
+  return MSFEC_OK;
+}
+
+msfec_status msfec_ned_get_element(msfec_ned_handle h, double* M, double* b, int loc) {
+  if (!h) return ned_fail(MSFEC_ERR_INVALID, "null handle");
+  if (!h->computed) return ned_fail(MSFEC_ERR_STATE, "compute not called");
+  NED_HIP(hipStreamSynchronize(h->stream));
+  msfec_status st = MSFEC_OK;
+  if (M) st = ned_out(M, h->d_M, (size_t)h->n_cells * 64, loc);
+  if (st == MSFEC_OK && b) st = ned_out(b, h->d_b, (size_t)h->n_cells * kNR, loc);
+  return st;
+}
+
+msfec_status msfec_ned_get_basis(msfec_ned_handle h, double* Psi, int loc) {
+  if (!h) return ned_fail(MSFEC_ERR_INVALID, "null handle");
+  if (!h->computed) return ned_fail(MSFEC_ERR_STATE, "compute not called");
+  NED_HIP(hipStreamSynchronize(h->stream));
+  return ned_out(Psi, h->d_psi, (size_t)h->n_cells * kNR * h->conf.N, loc);
+}
+
+msfec_status msfec_ned_get_cell_status(msfec_ned_handle h, int* status) {
+  if (!h || !status) return ned_fail(MSFEC_ERR_INVALID, "null argument");
+  if (!h->computed) return ned_fail(MSFEC_ERR_STATE, "compute not called");
+  NED_HIP(hipStreamSynchronize(h->stream));
+  NED_HIP(hipMemcpy(status, h->d_status, (size_t)h->n_cells * sizeof(int), hipMemcpyDeviceToHost));
+  return MSFEC_OK;
+}
+
+msfec_status msfec_ned_kernel_times(msfec_ned_handle h, float* ms, int max, int* n) {
+  if (!h || !ms) return ned_fail(MSFEC_ERR_INVALID, "null argument");
+  if (h->ev_used == 0) return ned_fail(MSFEC_ERR_STATE, "no timed compute (set MSFEC_FLAG_TIMING)");
+  double acc[3] = {0.0, 0.0, 0.0};
+  for (size_t c = 0; c < h->ev_used; ++c) {
+    NED_HIP(hipEventSynchronize(h->ev_sets[c][3]));
+    for (int i = 0; i < 3; ++i) {
+      float t = 0.f;
+      NED_HIP(hipEventElapsedTime(&t, h->ev_sets[c][i], h->ev_sets[c][i + 1]));
+      acc[i] += t;
+    }
+  }
+  int k = 0;
+  for (; k < 3 && k < max; ++k) ms[k] = (float)(acc[k] / (double)h->ev_used);
+  if (n) *n = k;
+  h->ev_used = 0;
+  return MSFEC_OK;
+}
+
+msfec_status msfec_ned_destroy(msfec_ned_handle h) {
+  if (!h) return MSFEC_OK;
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  ned_free(h);
+  delete h;
+  return MSFEC_OK;
+}
+
+}  // extern "C"

@@ -2779,9 +2779,15 @@ __global__ void coarse_weights_kernel(int dim, int nc, const double* __restrict_
 // ===========================================================================
 // Host side: handle, error reporting, launches
 // ===========================================================================
+// the last error message of the calling thread, shared with the Q_Ned
+// entry points (msfec_ned.hip)
+namespace msfec_detail {
+thread_local std::string g_last_error;
+}
+
 namespace {
 
-thread_local std::string g_last_error;
+using msfec_detail::g_last_error;
 
 msfec_status fail(msfec_status st, const std::string& msg) {
   g_last_error = msg;

@@ -1,4 +1,5 @@
-"""MI355X multiscale basis construction (MsFEC Q1 pairing), Python face.
+"""MI355X multiscale basis construction (MsFEC Q1 and 2-D Q1 x Nedelec pairings),
+Python face.
 
 The compute path is libmsfec_hip.so (hand-written HIP for gfx950) behind the
 C-ABI in include/msfec.h; this package only binds it (ctypes) and generates
@@ -6,6 +7,7 @@ seeded synthetic inputs.
 """
 from . import _lib
 from ._lib import MsfecError
+from .ned import NedBasisBatch
 from .qbasis import QBasisBatch
 
-__all__ = ["QBasisBatch", "MsfecError", "_lib"]
+__all__ = ["QBasisBatch", "NedBasisBatch", "MsfecError", "_lib"]

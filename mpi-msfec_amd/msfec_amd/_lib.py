@@ -29,6 +29,9 @@ EXPORTED = (
     "msfec_q_compute", "msfec_q_synchronize", "msfec_q_get_element", "msfec_q_get_basis",
     "msfec_q_get_cell_status", "msfec_q_reconstruct", "msfec_q_get_info",
     "msfec_q_kernel_times", "msfec_q_destroy", "msfec_coarse_solve", "msfec_q_adjoint_gradient",
+    "msfec_ned_create", "msfec_ned_set_coefficients", "msfec_ned_compute",
+    "msfec_ned_synchronize", "msfec_ned_get_element", "msfec_ned_get_basis",
+    "msfec_ned_get_cell_status", "msfec_ned_kernel_times", "msfec_ned_destroy",
     "msfec_last_error", "msfec_version",
 )
 
@@ -45,6 +48,14 @@ class Params(ctypes.Structure):
         ("solver", ctypes.c_int),
         ("flags", ctypes.c_int),
         ("max_refine", ctypes.c_int),
+    ]
+
+
+class NedParams(ctypes.Structure):
+    _fields_ = [
+        ("n_refine_local", ctypes.c_int),
+        ("full_rhs", ctypes.c_int),
+        ("flags", ctypes.c_int),
     ]
 
 
@@ -105,6 +116,16 @@ def load(path: str | None = None):
                                      ctypes.c_int, vp, ctypes.c_int, vp, ctypes.c_int, ip, dp]
     L.msfec_q_adjoint_gradient.argtypes = [vp, ctypes.c_int, vp, ctypes.c_int, ctypes.c_double,
                                            ctypes.c_int, dp, vp, ctypes.c_int]
+    L.msfec_ned_create.argtypes = [ctypes.POINTER(NedParams), ctypes.c_int, vp, ctypes.c_int,
+                                   ctypes.POINTER(vp)]
+    L.msfec_ned_set_coefficients.argtypes = [vp, vp, vp, vp, vp, ctypes.c_int]
+    L.msfec_ned_compute.argtypes = [vp, vp]
+    L.msfec_ned_synchronize.argtypes = [vp]
+    L.msfec_ned_get_element.argtypes = [vp, vp, vp, ctypes.c_int]
+    L.msfec_ned_get_basis.argtypes = [vp, vp, ctypes.c_int]
+    L.msfec_ned_get_cell_status.argtypes = [vp, ip]
+    L.msfec_ned_kernel_times.argtypes = [vp, ctypes.POINTER(ctypes.c_float), ctypes.c_int, ip]
+    L.msfec_ned_destroy.argtypes = [vp]
     L.msfec_last_error.restype = ctypes.c_char_p
     L.msfec_version.restype = ctypes.c_char_p
     for name in EXPORTED:

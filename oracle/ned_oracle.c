@@ -199,9 +199,9 @@ static void ned_boundary(const ned_mesh *g, double *G, unsigned char *isb) {
     }
 }
 
-/* dense LU with partial pivoting, nrhs right-hand sides in B (n x nrhs,
- * row-major); the stand-in for SparseDirectUMFPACK (q_ned_basis.cc:604-607) */
-static int lu_solve(int n, double *A, int nrhs, double *B) {
+/* dense LU with partial pivoting (the stand-in for SparseDirectUMFPACK,
+ * q_ned_basis.cc:604-607): factor in place, row interchanges in piv */
+static int lu_factor(int n, double *A, int *piv) {
   for (int k = 0; k < n; ++k) {
     int p = k;
     double best = fabs(A[(size_t)k * n + k]);
@@ -210,25 +210,39 @@ static int lu_solve(int n, double *A, int nrhs, double *B) {
         best = fabs(A[(size_t)i * n + k]);
         p = i;
       }
+    piv[k] = p;
     if (!(best > 0)) return 2;
-    if (p != k) {
+    if (p != k)
       for (int c = 0; c < n; ++c) {
         const double tmp = A[(size_t)k * n + c];
         A[(size_t)k * n + c] = A[(size_t)p * n + c];
         A[(size_t)p * n + c] = tmp;
       }
-      for (int c = 0; c < nrhs; ++c) {
-        const double tmp = B[(size_t)k * nrhs + c];
-        B[(size_t)k * nrhs + c] = B[(size_t)p * nrhs + c];
-        B[(size_t)p * nrhs + c] = tmp;
-      }
-    }
     const double inv = 1.0 / A[(size_t)k * n + k];
     for (int i = k + 1; i < n; ++i) {
       const double lik = A[(size_t)i * n + k] * inv;
-      if (lik == 0.0) continue;
       A[(size_t)i * n + k] = lik;
+      if (lik == 0.0) continue;
       for (int c = k + 1; c < n; ++c) A[(size_t)i * n + c] -= lik * A[(size_t)k * n + c];
+    }
+  }
+  return 0;
+}
+
+/* B (n x nrhs, row-major) <- A^-1 B with the factors of lu_factor (whole
+ * rows are interchanged there, so all interchanges come first) */
+static void lu_apply(int n, const double *A, const int *piv, int nrhs, double *B) {
+  for (int k = 0; k < n; ++k)
+    if (piv[k] != k)
+      for (int c = 0; c < nrhs; ++c) {
+        const double tmp = B[(size_t)k * nrhs + c];
+        B[(size_t)k * nrhs + c] = B[(size_t)piv[k] * nrhs + c];
+        B[(size_t)piv[k] * nrhs + c] = tmp;
+      }
+  for (int k = 0; k < n; ++k) {
+    for (int i = k + 1; i < n; ++i) {
+      const double lik = A[(size_t)i * n + k];
+      if (lik == 0.0) continue;
       for (int c = 0; c < nrhs; ++c) B[(size_t)i * nrhs + c] -= lik * B[(size_t)k * nrhs + c];
     }
   }
@@ -238,7 +252,47 @@ static int lu_solve(int n, double *A, int nrhs, double *B) {
       for (int j = k + 1; j < n; ++j) s -= A[(size_t)k * n + j] * B[(size_t)j * nrhs + c];
       B[(size_t)k * nrhs + c] = s / A[(size_t)k * n + k];
     }
-  return 0;
+}
+
+/* A x = B for nrhs columns: symmetric diagonal equilibration (the sigma and
+ * u rows differ by powers of the fine mesh size), LU with partial pivoting,
+ * then 3 steps of iterative refinement with FP64 residuals.  B is overwritten
+ * with x. */
+static int dense_solve(int n, const double *A, int nrhs, double *B) {
+  double *s = malloc(sizeof(double) * n);
+  double *F = malloc(sizeof(double) * (size_t)n * n);
+  double *R = malloc(sizeof(double) * (size_t)n * nrhs);
+  double *X = malloc(sizeof(double) * (size_t)n * nrhs);
+  int *piv = malloc(sizeof(int) * n);
+  for (int i = 0; i < n; ++i) {
+    const double dii = fabs(A[(size_t)i * n + i]);
+    s[i] = dii > 0 ? 1.0 / sqrt(dii) : 1.0;
+  }
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) F[(size_t)i * n + j] = s[i] * A[(size_t)i * n + j] * s[j];
+  int st = lu_factor(n, F, piv);
+  if (st == 0) {
+    memset(X, 0, sizeof(double) * (size_t)n * nrhs);
+    for (int pass = 0; pass < 4; ++pass) {
+      /* R = s (B - A X) */
+      for (int i = 0; i < n; ++i)
+        for (int c = 0; c < nrhs; ++c) {
+          double acc = B[(size_t)i * nrhs + c];
+          for (int j = 0; j < n; ++j) acc -= A[(size_t)i * n + j] * X[(size_t)j * nrhs + c];
+          R[(size_t)i * nrhs + c] = s[i] * acc;
+        }
+      lu_apply(n, F, piv, nrhs, R);
+      for (int i = 0; i < n; ++i)
+        for (int c = 0; c < nrhs; ++c) X[(size_t)i * nrhs + c] += s[i] * R[(size_t)i * nrhs + c];
+    }
+    memcpy(B, X, sizeof(double) * (size_t)n * nrhs);
+  }
+  free(piv);
+  free(X);
+  free(R);
+  free(F);
+  free(s);
+  return st;
 }
 
 int oq_ned_assemble(int l, const double *verts, const double *a, const double *binv,
@@ -279,7 +333,7 @@ int oq_ned_basis_cell(int l, const double *verts, const double *a, const double 
       B[(size_t)p * 8 + n] = s;
     }
   }
-  const int st = lu_solve(ni, AI, 8, B);
+  const int st = dense_solve(ni, AI, 8, B);
   /* distribute: psi = g on the boundary, the solution inside */
   double *P = malloc(sizeof(double) * 8 * N);
   memcpy(P, G, sizeof(double) * 8 * N);

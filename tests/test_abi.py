@@ -108,3 +108,30 @@ def test_product_path_has_no_fallback():
             _lib.load("/nonexistent/libmsfec_hip.so")
         finally:
             _lib._lib = saved
+
+
+def _ned_create(lib, l=2, verts=None):
+    L = lib.load()
+    p = lib.NedParams()
+    p.n_refine_local, p.full_rhs, p.flags = l, 1, 0
+    if verts is None:
+        verts = box_vertices(2, [0.0, 0.0], [1.0, 1.0])[None]
+    verts = np.ascontiguousarray(verts, dtype=np.float64)
+    h = ctypes.c_void_p()
+    st = L.msfec_ned_create(ctypes.byref(p), verts.shape[0], ctypes.c_void_p(verts.ctypes.data),
+                            lib.HOST, ctypes.byref(h))
+    return st, L.msfec_last_error().decode()
+
+
+def test_ned_rejects_non_box_and_bad_level(lib):
+    v = box_vertices(2, [0, 0], [1, 1])
+    v[3, 0] += 0.1
+    st, msg = _ned_create(lib, verts=v[None])
+    assert st == lib.ERR_INVALID and "axis-aligned" in msg
+    st, msg = _ned_create(lib, l=0)
+    assert st == lib.ERR_INVALID
+    st, msg = _ned_create(lib, l=5)
+    assert st == lib.ERR_UNSUPPORTED and "bandwidth" in msg
+    L = lib.load()
+    assert L.msfec_ned_create(None, 1, None, 0, None) == lib.ERR_INVALID
+    assert L.msfec_ned_compute(None, None) == lib.ERR_INVALID
