@@ -48,10 +48,10 @@ extern thread_local std::string g_last_error;
 namespace {
 
 constexpr int kNLow = 16;                // lower entries of a band row, diagonal first
-constexpr int kNW = 64;                  // window slots: the half-bandwidth stays <= 63
+constexpr int kNW = 64;                  // window slots: the half-bandwidth stays <= 62
 constexpr int kNR = 8;                   // bases per cell
 constexpr int kRowLen = kNLow + kNR;     // row table: lower values, right-hand sides
-constexpr int kZLen = kNR + 1;           // z_k of the 8 bases, then the row scale s_k
+constexpr int kZLen = kNR + 2;           // z_k of the 8 bases, the row scale s_k, 1 / d_k
 constexpr int kNThreads = 256;
 constexpr int kMaxPairs = 63 * 64 / 2;
 
@@ -281,9 +281,55 @@ __global__ __launch_bounds__(kNThreads) void ned_rows_kernel(NedArgs a) {
       for (int bb = 0; bb < kNR; ++bb) a.psi[((size_t)cell * kNR + bb) * a.N + d] = ned_bval(a, d, bb);
 }
 
+// Wait for every outstanding vector memory operation (vmcnt(0); expcnt and
+// lgkmcnt left at their maxima).
+__device__ __forceinline__ void ned_vm_wait() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
+// Entering rows of one group of 4 elimination steps, held by threads
+// 0..kRowLen+kNLow-1 (value or offset) between their load and their use.
+struct NedRowSet {
+  double v[4];
+  int o[4];
+};
+
+// Backward data of one group of 4 steps: per lane of wave 0 the panel entry
+// at offset lane, z of its basis, the row scale, 1 / pivot and the row's dof.
+struct NedBackSet {
+  double pv[4], z[4], s[4], di[4];
+  int ord[4];
+};
+
+__device__ __forceinline__ void ned_wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// one 64-bit value moved by a DPP lane pattern (32-bit halves)
+template <int CTRL>
+__device__ __forceinline__ double ned_dpp(double v) {
+  const long long b = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_mov_dpp((int)b, CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, false);
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
+}
+
+// sum over the 8 lanes of an aligned group: swap neighbours
+// (quad_perm [1,0,3,2]), swap pairs ([2,3,0,1]), mirror the half row
+__device__ __forceinline__ double ned_sum8(double v) {
+  v += ned_dpp<0xB1>(v);
+  v += ned_dpp<0x4E>(v);
+  v += ned_dpp<0x141>(v);
+  return v;
+}
+
 // LDL^T of the equilibrated band matrix S K S (S = |diag K|^-1/2), rows
 // entering a 64-row LDS window as earlier ones leave; the right-hand sides
-// are eliminated alongside; then x = S (L^-T D^-1 z).
+// are eliminated alongside; then x = S (L^-T D^-1 z).  The steps run in
+// groups of 4: one wait at the start of a group covers the loads issued a
+// group earlier (the entering rows, or the panels of the backward sweep) and
+// the stores of the group before; the group's own panels and solution rows
+// stay in LDS until the next group's start.  No step waits for memory.
 __global__ __launch_bounds__(kNThreads) void ned_factor_kernel(NedArgs a) {
   const int cell = blockIdx.x, tid = threadIdx.x;
   const int n = a.n, p = a.p;
@@ -291,8 +337,12 @@ __global__ __launch_bounds__(kNThreads) void ned_factor_kernel(NedArgs a) {
   __shared__ double Y[kNW][kNR];
   __shared__ double srow[kNW];
   __shared__ double lcol[kNW];
-  __shared__ double rowbuf[kRowLen];
-  __shared__ int offbuf[kNLow];
+  __shared__ double ring[32][kRowLen];  // entering rows: the first fill, then 4 per group
+  __shared__ int oring[32][kNLow];
+  __shared__ double pbuf[4][kNW];       // panels of the group (d at 0)
+  __shared__ double zbuf[4][kZLen];     // z and s of the group
+  __shared__ int ordbuf[4];
+  __shared__ double psib[4][kNR];       // backward: Psi rows of the group
   __shared__ int pairs[kMaxPairs];
   __shared__ int bad;
   const double* rows = a.rows + (size_t)cell * n * kRowLen;
@@ -301,63 +351,78 @@ __global__ __launch_bounds__(kNThreads) void ned_factor_kernel(NedArgs a) {
   for (int q = tid; q < a.npairs; q += kNThreads) pairs[q] = a.pairs[q];
   if (tid == 0) bad = MSFEC_CELL_OK;
 
-  // row i (staged in rowbuf / offbuf) into slot i mod 64: thread o <= p
-  // writes column i - o, scaled by s_i s_col
-  auto insert = [&](int i) {
+  // row i staged at ring[u] into slot i mod 64: thread o <= p writes column
+  // i - o, scaled by s_i s_col
+  auto insert = [&](int i, int u) {
     const int si = i & (kNW - 1);
-    const double dg = fabs(rowbuf[0]);
+    const double dg = fabs(ring[u][0]);
     const double s_i = dg > 0.0 ? 1.0 / sqrt(dg) : 1.0;
     if (tid <= p && i - tid >= 0) {
       double v = 0.0;
 #pragma unroll
       for (int t = 0; t < kNLow; ++t)
-        if (offbuf[t] == tid) v = rowbuf[t];
+        if (oring[u][t] == tid) v = ring[u][t];
       const int col = i - tid;
       const double sc = tid == 0 ? s_i : srow[col & (kNW - 1)];
       W[si][col & (kNW - 1)] = v * s_i * sc;
     }
-    if (tid >= kNW && tid < kNW + kNR) Y[si][tid - kNW] = rowbuf[kNLow + tid - kNW] * s_i;
+    if (tid >= kNW && tid < kNW + kNR) Y[si][tid - kNW] = ring[u][kNLow + tid - kNW] * s_i;
     // slot si is read by no other thread here (its columns are i - o, o >= 1)
-    if (tid == 0) {
-      srow[si] = s_i;
-      zs[(size_t)i * kZLen + kNR] = s_i;
+    if (tid == 0) srow[si] = s_i;
+  };
+  auto load_row = [&](int i, double& v, int& o) {
+    if (i < n) {
+      if (tid < kRowLen) v = rows[(size_t)i * kRowLen + tid];
+      else if (tid < kRowLen + kNLow) o = a.loff[(size_t)i * kNLow + tid - kRowLen];
     }
   };
-  auto stage = [&](int i) {
-    if (tid < kRowLen) rowbuf[tid] = rows[(size_t)i * kRowLen + tid];
-    else if (tid < kRowLen + kNLow) offbuf[tid - kRowLen] = a.loff[(size_t)i * kNLow + tid - kRowLen];
+  auto stage_row = [&](int u, double v, int o) {
+    if (tid < kRowLen) ring[u][tid] = v;
+    else if (tid < kRowLen + kNLow) oring[u][tid - kRowLen] = o;
   };
 
-  __syncthreads();
-  for (int i = 0; i < min(n, kNW); ++i) {
-    stage(i);
-    __syncthreads();
-    insert(i);
-    __syncthreads();
-  }
-  for (int k = 0; k < n; ++k) {
-    const int sk = k & (kNW - 1);
-    const int inew = k + kNW;
-    // the entering row, loaded during the elimination step
-    double pv = 0.0;
-    int po = -1;
-    if (inew < n) {
-      if (tid < kRowLen) pv = rows[(size_t)inew * kRowLen + tid];
-      else if (tid < kRowLen + kNLow) po = a.loff[(size_t)inew * kNLow + tid - kRowLen];
+  // first fill: rows 0..63, 32 at a time
+  for (int h = 0; h < 2; ++h) {
+    double v[32];
+    int o[32];
+#pragma unroll
+    for (int u = 0; u < 32; ++u) {
+      v[u] = 0.0;
+      o[u] = -1;
+      load_row(32 * h + u, v[u], o[u]);
     }
+#pragma unroll
+    for (int u = 0; u < 32; ++u) stage_row(u, v[u], o[u]);
+    __syncthreads();
+    for (int u = 0; u < 32 && 32 * h + u < n; ++u) {
+      insert(32 * h + u, u);
+      __syncthreads();
+    }
+  }
+
+  // forward elimination in groups of 4 steps
+  // Step k: the row entering after step k - 1 (row k + 63, staged at ring
+  // slot rslot(k - 1)) is inserted while pivot k is read (they touch
+  // different slots since p <= 62), then the rank-1 update.
+  auto rslot = [](int j) { return ((j >> 2) & 1) * 4 + (j & 3); };
+  auto fstep = [&](int k, int u) {
+    const int sk = k & (kNW - 1);
+    if (k >= 1 && k - 1 + kNW < n) insert(k - 1 + kNW, rslot(k - 1));
     const double dk = W[sk][sk];
     if (tid >= 1 && tid <= p) {
       const int i = k + tid;
       const double l = (i < n) ? W[i & (kNW - 1)][sk] / dk : 0.0;
       lcol[tid] = l;
-      pan[(size_t)k * kNW + tid] = l;
+      pbuf[u][tid] = l;
     }
     if (tid == 0) {
-      pan[(size_t)k * kNW] = dk;
+      pbuf[u][0] = dk;
       if (!(dk != 0.0)) bad = MSFEC_CELL_NOT_SPD;
       else if (!isfinite(dk) && bad == MSFEC_CELL_OK) bad = MSFEC_CELL_NONFINITE;
     }
-    if (tid >= kNW && tid < kNW + kNR) zs[(size_t)k * kZLen + tid - kNW] = Y[sk][tid - kNW];
+    if (tid >= kNW && tid < kNW + kNR) zbuf[u][tid - kNW] = Y[sk][tid - kNW];
+    if (tid == kNW + kNR) zbuf[u][kNR] = srow[sk];
+    if (tid == kNW + kNR + 1) zbuf[u][kNR + 1] = 1.0 / dk;
     __syncthreads();
     for (int q = tid; q < a.npairs; q += kNThreads) {
       const int pr = pairs[q], io = pr >> 8, jo = pr & 255;
@@ -368,36 +433,128 @@ __global__ __launch_bounds__(kNThreads) void ned_factor_kernel(NedArgs a) {
       const int io = 1 + q / kNR, bb = q - (io - 1) * kNR;
       if (k + io < n) Y[(k + io) & (kNW - 1)][bb] -= lcol[io] * Y[sk][bb];
     }
-    if (inew < n) {
-      if (tid < kRowLen) rowbuf[tid] = pv;
-      else if (tid < kRowLen + kNLow) offbuf[tid - kRowLen] = po;
+    __syncthreads();
+  };
+  // panels and z of steps kg .. kg+3 from LDS to HBM
+  auto fflush = [&](int kg) {
+    for (int t = tid; t < 4 * kNW; t += kNThreads) {
+      const int u = t / kNW, o = t - u * kNW, k = kg + u;
+      if (k < n && o <= p) pan[(size_t)k * kNW + o] = pbuf[u][o];
     }
-    __syncthreads();
-    if (inew < n) insert(inew);
-    __syncthreads();
-  }
-
-  // backward sweep: x'_k = z_k / d_k - sum_o l_{k+o,k} x'_{k+o}; lane group
-  // of 32 per basis, x' ring in Y
-  for (int t = tid; t < kNW * kNR; t += kNThreads) Y[t / kNR][t % kNR] = 0.0;
-  __syncthreads();
-  const int bb = tid >> 5, l32 = tid & 31;
-  for (int k = n - 1; k >= 0; --k) {
-    const double* pk = pan + (size_t)k * kNW;
-    double part = 0.0;
-    for (int o = l32 + 1; o <= p; o += 32)
-      if (k + o < n) part += pk[o] * Y[(k + o) & (kNW - 1)][bb];
+    if (tid < 4 * kZLen) {
+      const int u = tid / kZLen, c = tid - u * kZLen, k = kg + u;
+      if (k < n) zs[(size_t)k * kZLen + c] = zbuf[u][c];
+    }
+  };
+  auto fgroup = [&](int kg, const NedRowSet& cur, NedRowSet& nxt) {
+    ned_vm_wait();
+    if (kg > 0) fflush(kg - 4);
 #pragma unroll
-    for (int w = 16; w > 0; w >>= 1) part += __shfl_xor(part, w);
+    for (int u = 0; u < 4; ++u) stage_row(rslot(kg + u), cur.v[u], cur.o[u]);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) load_row(kg + kNW + 4 + u, nxt.v[u], nxt.o[u]);
     __syncthreads();
-    if (l32 == 0) {
-      const double x = zs[(size_t)k * kZLen + bb] / pk[0] - part;
-      Y[k & (kNW - 1)][bb] = x;
-      a.psi[((size_t)cell * kNR + bb) * a.N + a.order[k]] = x * zs[(size_t)k * kZLen + kNR];
-    }
-    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (kg + u < n) fstep(kg + u, u);
+  };
+  NedRowSet ra, rb;
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    ra.v[u] = rb.v[u] = 0.0;
+    ra.o[u] = rb.o[u] = -1;
+    load_row(kNW + u, ra.v[u], ra.o[u]);
   }
-  if (tid == 0) a.status[cell] = bad;
+  int klast = 0;
+  for (int k0 = 0; k0 < n; k0 += 8) {
+    fgroup(k0, ra, rb);
+    klast = k0;
+    if (k0 + 4 >= n) break;
+    fgroup(k0 + 4, rb, ra);
+    klast = k0 + 4;
+  }
+  __syncthreads();
+  fflush(klast);
+
+  // backward sweep on wave 0: x'_k = z_k / d_k - sum_o l_{k+o,k} x'_{k+o};
+  // lane (bb = lane >> 3, c = lane & 7) sums the offsets o = 1 + c + 8t of
+  // basis bb, the 8 lanes of a basis add up by DPP; x' ring in Y.  Groups of
+  // 4 steps: the panels (lane o holds l_{k+o,k}) and z, s, 1/d of the next
+  // group load while the current one runs.
+  // the panels and z written above, visible to every thread of the block
+  ned_vm_wait();
+  __threadfence_block();
+  __syncthreads();
+  if (tid < kNW) {
+    for (int t = tid; t < kNW * kNR; t += kNW) Y[t / kNR][t % kNR] = 0.0;
+    const int bb = tid >> 3, c = tid & 7;
+    auto bload = [&](int k, NedBackSet& f, int u) {
+      if (k < 0) return;
+      f.pv[u] = pan[(size_t)k * kNW + tid];
+      f.z[u] = zs[(size_t)k * kZLen + bb];
+      f.s[u] = zs[(size_t)k * kZLen + kNR];
+      f.di[u] = zs[(size_t)k * kZLen + kNR + 1];
+      f.ord[u] = a.order[k];
+    };
+    auto bstep = [&](int k, int u, const NedBackSet& f) {
+      double part = 0.0;
+#pragma unroll
+      for (int t = 0; t < 8; ++t) {
+        const int o = 1 + c + 8 * t;
+        if (o <= p && k + o < n) part = fma(pbuf[u][o], Y[(k + o) & (kNW - 1)][bb], part);
+      }
+      part = ned_sum8(part);
+      if (c == 0) {
+        const double x = f.z[u] * f.di[u] - part;
+        Y[k & (kNW - 1)][bb] = x;
+        psib[u][bb] = x * f.s[u];
+      }
+      ned_wave_sync();
+    };
+    // Psi of steps kg, kg-1, kg-2, kg-3 (psib[u], u = kg - k) to HBM
+    auto bflush = [&](int kg) {
+      if (tid < 4 * kNR) {
+        const int u = tid / kNR, cc = tid - u * kNR;
+        if (kg - u >= 0) a.psi[((size_t)cell * kNR + cc) * a.N + ordbuf[u]] = psib[u][cc];
+      }
+    };
+    auto bgroup = [&](int kg, const NedBackSet& cur, NedBackSet& nxt, int kprev) {
+      ned_vm_wait();
+      if (kprev >= 0) bflush(kprev);
+      ned_wave_sync();
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        pbuf[u][tid] = cur.pv[u];
+        if (tid == 0) ordbuf[u] = cur.ord[u];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) bload(kg - 4 - u, nxt, u);
+      ned_wave_sync();
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (kg - u >= 0) bstep(kg - u, u, cur);
+    };
+    NedBackSet ba, bc;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      ba.pv[u] = ba.z[u] = ba.s[u] = ba.di[u] = 0.0;
+      bc.pv[u] = bc.z[u] = bc.s[u] = bc.di[u] = 0.0;
+      ba.ord[u] = bc.ord[u] = 0;
+      bload(n - 1 - u, ba, u);
+    }
+    ned_wave_sync();
+    int kprev = -1;
+    for (int k0 = n - 1; k0 >= 0; k0 -= 8) {
+      bgroup(k0, ba, bc, kprev);
+      kprev = k0;
+      if (k0 - 4 < 0) break;
+      bgroup(k0 - 4, bc, ba, kprev);
+      kprev = k0 - 4;
+    }
+    ned_wave_sync();
+    bflush(kprev);
+    if (tid == 0) a.status[cell] = bad;
+  }
 }
 
 // Element matrix (q_ned_basis.cc:841-939): M[kt][kc] = psi_kt^T A psi_kc on
@@ -528,8 +685,8 @@ bool ned_conf(int l, NedConf& c, std::string& why) {
     for (size_t t = 0; t < offs.size(); ++t) c.loff[(size_t)r * kNLow + t] = offs[t];
     c.p = std::max(c.p, offs.back());
   }
-  if (c.p > kNW - 1) {
-    why = "half-bandwidth " + std::to_string(c.p) + " exceeds 63 (l <= 4)";
+  if (c.p > kNW - 2) {
+    why = "half-bandwidth " + std::to_string(c.p) + " exceeds 62 (l <= 4)";
     return false;
   }
   c.pairs.clear();

@@ -32,17 +32,22 @@ CPP = os.path.join(PKG_ROOT, "cpp")
 DRIVER = os.path.join(CPP, "q_multiscale_driver")
 
 
+NED_DRIVER = os.path.join(CPP, "q_ned_driver")
+
+
 def build_driver(verbose: bool = False) -> str:
-    """The QBasis/QBatch C++ shim and the QMultiscale-shaped driver: host C++
-    only, linked against the HIP library through the C-ABI."""
-    cmd = ["g++", "-O2", "-std=c++17", "-Wall", f"-I{INCLUDE}", f"-I{CPP}",
-           os.path.join(CPP, "msfec_q_basis.cpp"), os.path.join(CPP, "q_multiscale_driver.cpp"),
-           f"-L{HERE}", "-l:libmsfec_hip.so", "-Wl,-rpath,$ORIGIN/../msfec_amd",
-           "-o", DRIVER + ".tmp"]
-    if verbose:
-        print(" ".join(cmd))
-    subprocess.run(cmd, check=True)
-    os.replace(DRIVER + ".tmp", DRIVER)
+    """The QBasis/QBatch and QNedBasis/QNedBatch C++ shims and their drivers:
+    host C++ only, linked against the HIP library through the C-ABI."""
+    for shim, main, exe in (("msfec_q_basis.cpp", "q_multiscale_driver.cpp", DRIVER),
+                            ("msfec_q_ned_basis.cpp", "q_ned_driver.cpp", NED_DRIVER)):
+        cmd = ["g++", "-O2", "-std=c++17", "-Wall", f"-I{INCLUDE}", f"-I{CPP}",
+               os.path.join(CPP, shim), os.path.join(CPP, main),
+               f"-L{HERE}", "-l:libmsfec_hip.so", "-Wl,-rpath,$ORIGIN/../msfec_amd",
+               "-o", exe + ".tmp"]
+        if verbose:
+            print(" ".join(cmd))
+        subprocess.run(cmd, check=True)
+        os.replace(exe + ".tmp", exe)
     return DRIVER
 
 

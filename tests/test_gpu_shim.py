@@ -42,3 +42,44 @@ def test_multiscale_driver(oracle_mod, dim, l, nc):
     u, _ = oracle_mod.coarse_solve(dim, nc, ref["M"], ref["b"], rtol=1e-14)
     ufo = oracle_mod.reconstruct(dim, nc, l, u, ref["Psi"])
     assert np.abs(uf - ufo).max() < 1e-9 * np.abs(ufo).max()
+
+
+NED_DRIVER = os.path.join(REPO, "mpi-msfec_amd", "cpp", "q_ned_driver")
+
+
+def test_ned_driver(oracle_mod):
+    """The C++ QNedBasis shim: map of cells, run(), getters and
+    set_global_weights (sigma block from the vertex bases, u block from the
+    edge bases, q_ned_basis.cc:1158-1182) against the oracle."""
+    print("case ned_shim_driver l=3 nc=4", flush=True)
+    from msfec_amd import fields as F
+    assert os.path.exists(NED_DRIVER), "driver not built"
+    l, nc = 3, 4
+    a = F.lognormal_coefficient(2, nc, l, 1.0, 0.0, seed=13)
+    ncell = nc * nc
+    ns, nh, nv, N = oracle_mod.ned_sizes(l)
+    with tempfile.TemporaryDirectory() as td:
+        fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+        with open(fin, "wb") as fh:
+            fh.write(np.array([l, nc], dtype=np.int32).tobytes())
+            fh.write(np.ascontiguousarray(a, dtype=np.float64).tobytes())
+        r = subprocess.run([NED_DRIVER, fin, fout], capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        d = np.frombuffer(open(fout, "rb").read(), dtype=np.float64)
+    o = 0
+    M = d[o:o + ncell * 64].reshape(ncell, 8, 8); o += ncell * 64
+    b = d[o:o + ncell * 8].reshape(ncell, 8); o += ncell * 8
+    P = d[o:o + ncell * 8 * N].reshape(ncell, 8, N); o += ncell * 8 * N
+    sol = d[o:o + ncell * N].reshape(ncell, N)
+    verts = F.coarse_vertices(2, nc)
+    ref = oracle_mod.ned_basis_batch(l, verts, a, 1.0, 0.0, (1.0, 1.0))
+    for i in (0, 4):
+        for j in (0, 4):
+            blk = ref["M"][:, i:i + 4, j:j + 4]
+            assert np.abs(M[:, i:i + 4, j:j + 4] - blk).max() < 1e-10 * np.abs(blk).max()
+    assert np.abs(b - ref["b"]).max() < 1e-10 * np.abs(ref["b"]).max()
+    assert np.abs(P[:, :, ns:] - ref["Psi"][:, :, ns:]).max() < 1e-11 * np.abs(ref["Psi"]).max()
+    w = np.arange(1.0, 9.0)
+    expect = np.concatenate([np.einsum("i,cid->cd", w[:4], P[:, :4, :ns]),
+                             np.einsum("i,cid->cd", w[4:], P[:, 4:, ns:])], axis=1)
+    assert np.abs(sol - expect).max() < 1e-13 * np.abs(expect).max()
