@@ -201,6 +201,86 @@ def ned_config(steps: int, warmup: int, check: bool, cpu: bool):
     return out
 
 
+def uneven_config(steps: int, warmup: int, check: bool, cpu: bool):
+    """The uneven 3-D configuration, timed beside the headline: 16^3 coarse
+    cells; 410 seeded inclusions (A = 1e4, radius 0.02) in a background A = 1;
+    the cells holding an inclusion centre refined 4 levels (3375 interior
+    unknowns, matrix-free CG), the others 2 or 3 levels by a seeded coin
+    (band solver or CG, whichever the one-off timing on 256 cells of the
+    level picks); f = 1."""
+    from msfec_amd import UnevenBatch, fields as F, uneven as U
+    nc = 16
+    cen = F.inclusion_centres(3, 410, SEED)
+    lv = F.uneven_levels(3, nc, cen, SEED + 1)
+    blocks = F.inclusion_blocks(3, nc, lv, cen, 0.02, 1e4)
+    verts = F.coarse_vertices(3, nc)
+    ub = UnevenBatch(verts, lv, dim=3, coef_blocks=blocks, timing=True)
+    for _ in range(warmup):
+        ub.run()
+    ub.synchronize()
+    for g in ub.groups.values():
+        if warmup > 0:
+            g["qb"].kernel_times()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ub.run()
+    ub.synchronize()
+    wall = time.perf_counter() - t0
+    per_level = {}
+    for lvl, g in ub.groups.items():
+        kt = g["qb"].kernel_times()
+        per_level[str(lvl)] = {"cells": int(g["cells"].shape[0]),
+                               "solver": ub.summary()[lvl]["solver"],
+                               "kernel_ms": [round(t, 4) for t in kt[:4]]}
+    out = {"workload": "uneven 3D Q1 batch: 16^3 cells, levels 2/3 (band or CG, timed choice) "
+                       "and 4 (3375 interior DoFs, matrix-free Jacobi-PCG FP64) for the cells "
+                       "holding one of 410 inclusions (A=1e4, r=0.02)",
+           "cells_per_s": round(ub.n_cells / (wall / steps), 1),
+           "ms_per_step": round(wall * 1e3 / steps, 4),
+           "levels": per_level,
+           "autotune": {f"{k[0]}d_l{k[1]}": v for k, v in U.tuning_table().items()},
+           "failed_cells": int((ub.get_cell_status() != 0).sum())}
+    g4 = ub.groups[4]
+    it = g4["qb"].cg_iterations()
+    out["cg_iterations_l4"] = {"mean": round(float(it.mean()), 1), "max": int(it.max())}
+    if check or cpu:
+        from oracle import oracle as O
+        O.build()
+        rng = np.random.default_rng(11)
+        samp = {2: rng.choice(ub.groups[2]["cells"], 8, replace=False),
+                3: rng.choice(ub.groups[3]["cells"], 8, replace=False),
+                4: rng.choice(g4["cells"], 1, replace=False)}
+        if check:
+            M, b = ub.get_global_element_matrix(), ub.get_global_element_rhs()
+            errs = {}
+            for lvl, idx in samp.items():
+                o = O.basis_batch(O.make_problem(3, lvl), verts[idx],
+                                  coef=np.stack([blocks[c] for c in idx]), solver=O.SOLVER_DIRECT)
+                errs[f"l{lvl}"] = {"M": rel_err(M[idx], o["M"]), "b": rel_err(b[idx], o["b"]),
+                                   "cells": len(idx)}
+            out["rel_err_vs_fp64_oracle"] = errs
+        if cpu:
+            # the reference's default CG + SSOR on a sample of each level,
+            # scaled to the batch's level mix
+            cores = min(os.cpu_count() or 1, 16)
+            t_cell = {}
+            for lvl in (2, 3, 4):
+                idx = rng.choice(ub.groups[lvl]["cells"], cores if lvl < 4 else cores, replace=False)
+                t0 = time.perf_counter()
+                O.basis_batch(O.make_problem(3, lvl), verts[idx],
+                              coef=np.stack([blocks[c] for c in idx]), solver=O.SOLVER_CG_SSOR,
+                              n_threads=cores, want_psi=False)
+                t_cell[lvl] = (time.perf_counter() - t0) / len(idx)
+            total = sum(t_cell[lvl] * ub.groups[lvl]["cells"].shape[0] for lvl in (2, 3, 4))
+            out["cpu_baseline"] = {"value": round(ub.n_cells / total, 1), "unit": "cells/s",
+                                   "cores": cores, "kind": "port",
+                                   "sample": f"{cores} cells per level, CG+SSOR(1.6) abs tol 1e-12, "
+                                             f"OpenMP {cores} threads, per-level times scaled to "
+                                             "the batch's level counts"}
+    ub.close()
+    return out
+
+
 def copy_probe(mib: int, reps: int = 3):
     """Device-to-device copies of a known size on the library's HIP runtime.
     Profiled with the FETCH_SIZE / WRITE_SIZE counters, the copy kernel's
@@ -235,6 +315,8 @@ def main():
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--no-ned", action="store_true",
                     help="skip the 2-D Q1 x Nedelec configuration timed beside the headline")
+    ap.add_argument("--no-uneven", action="store_true",
+                    help="skip the uneven 3-D configuration timed beside the headline")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run MIB-sized device copies (counter calibration)")
     args = ap.parse_args()
@@ -367,10 +449,17 @@ def main():
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(verts, coef)
     qb.close()
+    others_cfg = {}
     if not args.no_ned:
-        out["other_configs"] = {"q_ned_2d_32x32_l4": ned_config(
+        others_cfg["q_ned_2d_32x32_l4"] = ned_config(
             max(2, args.steps // 2), min(args.warmup, 2), not args.no_check,
-            not args.no_cpu_baseline)}
+            not args.no_cpu_baseline)
+    if not args.no_uneven:
+        others_cfg["q_uneven_3d_16x16x16"] = uneven_config(
+            max(2, args.steps // 4), min(args.warmup, 1), not args.no_check,
+            not args.no_cpu_baseline)
+    if others_cfg:
+        out["other_configs"] = others_cfg
     print(json.dumps(out))
 
 

@@ -63,10 +63,22 @@ enum { MSFEC_REACTION_NONE = 0, MSFEC_REACTION_CONST = 1, MSFEC_REACTION_CELL = 
  *                  with the residual in FP64: a cell stops refining once the
  *                  predicted error (|dx| / |x|)^2 is below 1e-11, after at
  *                  most max_refine steps (scalar coefficient per fine cell and
- *                  constant or per-cell load only). */
-enum { MSFEC_SOLVER_BAND_FP64 = 0, MSFEC_SOLVER_BAND_FP64_MFMA = 1, MSFEC_SOLVER_BAND_MIXED = 2 };
+ *                  constant or per-cell load only);
+ *   CG             matrix-free conjugate gradients in FP64 with the Jacobi
+ *                  preconditioner, the analogue of use_direct_solver_basis =
+ *                  false (solve_iterative, q_basis.cc:335-357, which uses SSOR):
+ *                  one workgroup per (cell, basis), no bandwidth limit (3-D up
+ *                  to n_refine_local = 4, 2-D up to 6), stops at
+ *                  ||r|| <= cg_tol ||b|| (default 1e-13) or after cg_max_iter
+ *                  iterations (default 5000); the element matrix uses the
+ *                  energy form of the Schur term, second order in the solve's
+ *                  error (scalar coefficient per fine cell and constant or
+ *                  per-cell load only). */
+enum { MSFEC_SOLVER_BAND_FP64 = 0, MSFEC_SOLVER_BAND_FP64_MFMA = 1, MSFEC_SOLVER_BAND_MIXED = 2,
+       MSFEC_SOLVER_CG = 3 };
 /* per-cell status codes.  NOT_CONVERGED: BAND_MIXED stopped after
- * max_refine steps with a predicted error above 1e-10 relative. */
+ * max_refine steps with a predicted error above 1e-10 relative, or CG
+ * reached cg_max_iter.  NOT_SPD also reports a CG breakdown (p^T K p <= 0). */
 enum { MSFEC_CELL_OK = 0, MSFEC_CELL_NOT_CONVERGED = 1, MSFEC_CELL_NOT_SPD = 2, MSFEC_CELL_NONFINITE = 3 };
 /* flags */
 enum { MSFEC_FLAG_TIMING = 1 /* record per-kernel HIP events in compute */ };
@@ -86,6 +98,8 @@ typedef struct {
   int solver;          /* MSFEC_SOLVER_* (replaces use_direct_solver)      */
   int flags;           /* MSFEC_FLAG_*                                     */
   int max_refine;      /* BAND_MIXED: refinement steps at most (0 = 3)     */
+  double cg_tol;       /* CG: relative residual to reach (0 = 1e-13)       */
+  int cg_max_iter;     /* CG: iterations at most (0 = 5000)                */
 } msfec_q_params;
 
 typedef struct {
@@ -259,6 +273,12 @@ msfec_status msfec_ned_get_cell_status(msfec_ned_handle h, int *status);
  * made with MSFEC_FLAG_TIMING since the previous call. */
 msfec_status msfec_ned_kernel_times(msfec_ned_handle h, float *ms, int max, int *n);
 msfec_status msfec_ned_destroy(msfec_ned_handle h);
+
+/* CG iterations of the last compute, n_cells x 2^dim ints on the host
+ * (basis-minor), for a handle created with MSFEC_SOLVER_CG.  The reference
+ * reports CG iterations per basis solve in verbose mode
+ * (source/Q/q_basis.cc:349-353). */
+msfec_status msfec_q_get_cg_iterations(msfec_q_handle h, int *iterations);
 
 /* Thread-local description of the last error of this thread. */
 const char *msfec_last_error(void);

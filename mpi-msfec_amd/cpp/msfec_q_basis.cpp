@@ -53,6 +53,8 @@ void QBatch::compute() {
   p.reaction_const = prm_.reaction_const;
   p.solver = prm_.solver;
   p.max_refine = prm_.max_refine;
+  p.cg_tol = prm_.cg_tol;
+  p.cg_max_iter = prm_.cg_max_iter;
   p.flags = 0;
   check(msfec_q_create(&p, (int)n_cells_, verts_.data(), MSFEC_HOST, &handle_), "msfec_q_create");
   check(msfec_q_set_coefficient(handle_, coef_.data(), coef_.size(), MSFEC_HOST),

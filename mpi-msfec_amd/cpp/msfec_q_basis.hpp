@@ -36,6 +36,8 @@ struct ParametersBasis {
   double reaction_const = 0.0;
   int solver = MSFEC_SOLVER_BAND_FP64;
   int max_refine = 0;  // BAND_MIXED: refinement steps at most (0 = library default)
+  double cg_tol = 0.0;  // CG: relative residual (0 = library default)
+  int cg_max_iter = 0;  // CG: iterations at most (0 = library default)
 };
 
 class QBatch {

@@ -2776,6 +2776,298 @@ __global__ void coarse_weights_kernel(int dim, int nc, const double* __restrict_
 
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// Matrix-free CG (MSFEC_SOLVER_CG): the analogue of QBasis::solve_iterative
+// (source/Q/q_basis.cc:335-357) with the Jacobi preconditioner in place of
+// SSOR, for cells whose interior band does not fit the 64-row window.
+// One workgroup of 512 threads per (cell, basis); thread t owns the interior
+// rows t, t + 512, ... (at most 8).  The search direction lives in LDS over
+// all fine nodes (zero on the boundary) so that K p is the 3^dim-point
+// stencil of the neighbour sums of the fine-cell coefficients, recomputed
+// from the coefficients in LDS at every iteration (no matrix is stored).
+// The right-hand side and the diagonal come from the row table.
+// ---------------------------------------------------------------------------
+constexpr int kCgThreads = 512;
+constexpr int kCgOwn = 8;
+constexpr int kCgWaves = kCgThreads / 64;
+
+struct CgArgs {
+  double tol;
+  int max_iter;
+  int* iters;  // [cells][NV][2]: iterations, state (0 converged, 1 not, 2 breakdown, 3 non-finite)
+};
+
+// sum over the 64 lanes of the wave: DPP inside rows of 16, then across rows
+__device__ __forceinline__ double cg_wave_sum(double v) {
+  v += dpp_f64(v, 0);
+  v += dpp_f64(v, 1);
+  v += dpp_f64(v, 2);
+  v += __shfl_xor(v, 8);
+  v += __shfl_xor(v, 16);
+  v += __shfl_xor(v, 32);
+  return v;
+}
+
+// block sums of two values; every thread gets both
+__device__ __forceinline__ void cg_block_sum2(double& x, double& y, double (*red)[2]) {
+  x = cg_wave_sum(x);
+  y = cg_wave_sum(y);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    red[w][0] = x;
+    red[w][1] = y;
+  }
+  __syncthreads();
+  double sx = 0.0, sy = 0.0;
+#pragma unroll
+  for (int i = 0; i < kCgWaves; ++i) {
+    sx += red[i][0];
+    sy += red[i][1];
+  }
+  x = sx;
+  y = sy;
+  __syncthreads();
+}
+
+// (K v)(node c) for an interior node from the nodal vector v in LDS: the
+// 3^dim neighbour values, then the 2^dim fine cells around the node, each
+// a_e sum_v ks[al ^ v] v(vertex v) (+ the reaction mass when REAC), with
+// al the node's corner in the cell.  Every register index is static.
+template <int DIM, bool REAC>
+__device__ __forceinline__ double cg_apply(const double* cf, const double* rf, bool has_rf,
+                                           double rc, const double (&kr)[8], const double (&mr)[8],
+                                           const int* c, int N, int S, const double* v) {
+  constexpr int NV = 1 << DIM, NST = (DIM == 3) ? 27 : 9;
+  const int node = c[0] + S * c[1] + (DIM == 3 ? S * S * c[2] : 0);
+  double nb[NST];
+#pragma unroll
+  for (int e = 0; e < NST; ++e) {
+    int off = 0, mul = 1;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+      off += dcomp<DIM>(e, d) * mul;
+      mul *= S;
+    }
+    nb[e] = v[node + off];
+  }
+  double acc = 0.0;
+#pragma unroll
+  for (int al = 0; al < NV; ++al) {
+    int z = 0, mul = 1;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+      z += (c[d] - ((al >> d) & 1)) * mul;
+      mul *= N;
+    }
+    double sk = 0.0, sm = 0.0;
+#pragma unroll
+    for (int vt = 0; vt < NV; ++vt) {
+      int e = 0, p3 = 1;
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) {
+        e += (((vt >> d) & 1) - ((al >> d) & 1) + 1) * p3;
+        p3 *= 3;
+      }
+      sk = fma(kr[al ^ vt], nb[e], sk);
+      if (REAC) sm = fma(mr[al ^ vt], nb[e], sm);
+    }
+    acc = fma(cf[z], sk, acc);
+    if (REAC) acc = fma((has_rf ? rf[z] : 0.0) + rc, sm, acc);
+  }
+  return acc;
+}
+
+// interior row jn -> grid coordinates (1..N-1 per dimension)
+template <int DIM>
+__device__ __forceinline__ void cg_coords(int jn, int ni, int* c) {
+  c[2] = 0;
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) {
+    c[d] = jn % ni + 1;
+    jn /= ni;
+  }
+}
+
+template <int DIM, bool REAC>
+__global__ __launch_bounds__(kCgThreads) void q_cg_kernel(BandArgs a, CgArgs g) {
+  using D = Dirs<DIM>;
+  const int cell = blockIdx.x / D::NV, j = blockIdx.x - cell * D::NV;
+  const int tid = threadIdx.x;
+  // LDS: coefficient | reaction (per cell) | search direction over all nodes | K p
+  extern __shared__ double dyn[];
+  __shared__ double kv[8], mv[8];
+  __shared__ double red[kCgWaves][2];
+  const bool has_rf = (a.reaction_mode == MSFEC_REACTION_CELL);
+  double* cf = dyn;
+  double* rf = dyn + a.nfc;
+  double* pv = dyn + a.nfc * (has_rf ? 2 : 1);
+  double* qv = pv + a.nn;
+  const double* cg = a.coef + (size_t)cell * a.nfc;
+  for (int z = tid; z < a.nfc; z += kCgThreads) cf[z] = cg[z];
+  if (has_rf) {
+    const double* rg = a.reac + (size_t)cell * a.nfc;
+    for (int z = tid; z < a.nfc; z += kCgThreads) rf[z] = rg[z];
+  }
+  for (int z = tid; z < a.nn; z += kCgThreads) pv[z] = 0.0;
+  if (tid < 8) {
+    const double* v = a.verts + (size_t)cell * D::NV * DIM;
+    double hh[3] = {1.0, 1.0, 1.0};
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) hh[d] = (v[(D::NV - 1) * DIM + d] - v[d]) / a.N;
+    double ks, ms;
+    box_tables<DIM>(tid, hh, ks, ms);
+    kv[tid] = ks;
+    mv[tid] = ms;
+  }
+  const double rc = (a.reaction_mode == MSFEC_REACTION_CONST) ? a.reaction_const : 0.0;
+  const double* rows = a.Rows + (size_t)cell * a.n * kRowStride;
+  // owned rows jn = tid + t * kCgThreads: node, right-hand side, inverse diagonal
+  int node[kCgOwn];
+  double b[kCgOwn], dinv[kCgOwn], x[kCgOwn], r[kCgOwn];
+  double bb = 0.0, rz = 0.0;
+#pragma unroll
+  for (int t = 0; t < kCgOwn; ++t) {
+    const int jn = tid + t * kCgThreads;
+    int c[3];
+    cg_coords<DIM>(jn < a.n ? jn : 0, a.ni, c);
+    node[t] = c[0] + a.S * c[1] + (DIM == 3 ? a.S * a.S * c[2] : 0);
+    b[t] = (jn < a.n) ? rows[(size_t)jn * kRowStride + 16 + j] : 0.0;
+    dinv[t] = (jn < a.n) ? 1.0 / rows[(size_t)jn * kRowStride] : 0.0;
+    x[t] = 0.0;
+    r[t] = b[t];
+    bb = fma(b[t], b[t], bb);
+    rz = fma(r[t], dinv[t] * r[t], rz);
+  }
+  __syncthreads();  // coefficients, box tables and the zeroed vector
+  double kr[8], mr[8];
+#pragma unroll
+  for (int t = 0; t < 8; ++t) {
+    kr[t] = kv[t];
+    mr[t] = mv[t];
+  }
+#pragma unroll
+  for (int t = 0; t < kCgOwn; ++t)
+    if (tid + t * kCgThreads < a.n) pv[node[t]] = dinv[t] * r[t];
+  cg_block_sum2(bb, rz, red);
+  double rr = bb;
+  const double stop = g.tol * g.tol * bb;
+  int it = 0, state = 0;
+  while (rr > stop) {
+    if (it >= g.max_iter) {
+      state = 1;
+      break;
+    }
+    // q = K p into LDS (one row at a time: the stencil gather is register
+    // heavy), and p^T q
+    double pq = 0.0, dummy = 0.0;
+#pragma unroll 1
+    for (int jn = tid; jn < a.n; jn += kCgThreads) {
+      int c[3];
+      cg_coords<DIM>(jn, a.ni, c);
+      const double q = cg_apply<DIM, REAC>(cf, rf, has_rf, rc, kr, mr, c, a.N, a.S, pv);
+      qv[jn] = q;
+      pq = fma(pv[c[0] + a.S * c[1] + (DIM == 3 ? a.S * a.S * c[2] : 0)], q, pq);
+    }
+    cg_block_sum2(pq, dummy, red);
+    if (!(pq > 0.0)) {
+      state = 2;
+      break;
+    }
+    const double alpha = rz / pq;
+    double rz1 = 0.0, rr1 = 0.0;
+#pragma unroll
+    for (int t = 0; t < kCgOwn; ++t) {
+      const int jn = tid + t * kCgThreads;
+      if (jn < a.n) {
+        x[t] = fma(alpha, pv[node[t]], x[t]);
+        r[t] = fma(-alpha, qv[jn], r[t]);
+        rz1 = fma(r[t], dinv[t] * r[t], rz1);
+        rr1 = fma(r[t], r[t], rr1);
+      }
+    }
+    cg_block_sum2(rz1, rr1, red);  // also: every thread is done reading p
+    const double beta = rz1 / rz;
+    rz = rz1;
+    rr = rr1;
+#pragma unroll
+    for (int t = 0; t < kCgOwn; ++t)
+      if (tid + t * kCgThreads < a.n) pv[node[t]] = fma(beta, pv[node[t]], dinv[t] * r[t]);
+    __syncthreads();
+    ++it;
+  }
+  if (!isfinite(rr) || !isfinite(bb)) state = 3;
+  // the true residual of the returned solution, for the energy form of S
+#pragma unroll
+  for (int t = 0; t < kCgOwn; ++t)
+    if (tid + t * kCgThreads < a.n) pv[node[t]] = x[t];
+  __syncthreads();
+#pragma unroll 1
+  for (int jn = tid; jn < a.n; jn += kCgThreads) {
+    int c[3];
+    cg_coords<DIM>(jn, a.ni, c);
+    qv[jn] = cg_apply<DIM, REAC>(cf, rf, has_rf, rc, kr, mr, c, a.N, a.S, pv);
+  }
+  double* psi = a.Psi + ((size_t)cell * D::NV + j) * a.nn;
+#pragma unroll
+  for (int t = 0; t < kCgOwn; ++t) {
+    const int jn = tid + t * kCgThreads;
+    if (jn < a.n) {
+      a.X[((size_t)cell * a.n + jn) * kMaxNb + j] = x[t];
+      a.Rres[((size_t)cell * a.n + jn) * kMaxNb + j] = b[t] - qv[jn];
+      psi[node[t]] = x[t];
+    }
+  }
+  if (tid == 0) {
+    g.iters[((size_t)cell * D::NV + j) * 2] = it;
+    g.iters[((size_t)cell * D::NV + j) * 2 + 1] = state;
+  }
+}
+
+// Element matrix and right-hand side of a CG cell: S in its energy form
+// S[i][k] = R_i^T x_k + x_i^T r_k (r = R - K x, exact up to the second order
+// in the error of x), symmetrised; M = P - S and b = b_B + sum_j x_j F_j as in
+// the band kernels.  Lane (i = lane >> 3, k = lane & 7) of each of 4 waves
+// sums every 4th interior row.
+template <int DIM>
+__global__ __launch_bounds__(256) void q_cg_finish_kernel(BandArgs a, CgArgs g) {
+  using D = Dirs<DIM>;
+  const int cell = blockIdx.x, tid = threadIdx.x;
+  const int lane = tid & 63, w = tid >> 6, i = lane >> 3, k = lane & 7;
+  __shared__ double part[4][64], spart[4][64];
+  double s = 0.0, bi = 0.0;
+  if (i < D::NV && k < D::NV) {
+    const double* rows = a.Rows + (size_t)cell * a.n * kRowStride;
+    const double* X = a.X + (size_t)cell * a.n * kMaxNb;
+    const double* Rr = a.Rres + (size_t)cell * a.n * kMaxNb;
+    for (int jn = w; jn < a.n; jn += 4) {
+      const double xi = X[(size_t)jn * kMaxNb + i], xk = X[(size_t)jn * kMaxNb + k];
+      s = fma(rows[(size_t)jn * kRowStride + 16 + i], xk, s);
+      s = fma(xi, Rr[(size_t)jn * kMaxNb + k], s);
+      if (i == 0) bi = fma(xk, rows[(size_t)jn * kRowStride + 14], bi);
+    }
+  }
+  part[w][lane] = s;
+  spart[w][lane] = bi;
+  __syncthreads();
+  if (w == 0) {
+    const double sik = part[0][lane] + part[1][lane] + part[2][lane] + part[3][lane];
+    const int lt = k * 8 + i;  // lane of S[k][i]
+    const double ski = part[0][lt] + part[1][lt] + part[2][lt] + part[3][lt];
+    const double bint = spart[0][lane] + spart[1][lane] + spart[2][lane] + spart[3][lane];
+    fast_finish<DIM>(a, cell, 0.5 * (sik + ski), bint);
+    if (lane == 0) {
+      int st = MSFEC_CELL_OK;
+      for (int jj = 0; jj < D::NV; ++jj) {
+        const int state = g.iters[((size_t)cell * D::NV + jj) * 2 + 1];
+        if (state == 3) st = MSFEC_CELL_NONFINITE;
+        else if (state == 2 && st != MSFEC_CELL_NONFINITE) st = MSFEC_CELL_NOT_SPD;
+        else if (state == 1 && st == MSFEC_CELL_OK) st = MSFEC_CELL_NOT_CONVERGED;
+      }
+      a.status[cell] = st;
+    }
+  }
+}
+
 // ===========================================================================
 // Host side: handle, error reporting, launches
 // ===========================================================================
@@ -2824,6 +3116,7 @@ struct msfec_q_batch {
   double* d_rres = nullptr;
   double* d_pb = nullptr;
   int* d_conv = nullptr;
+  int* d_cgit = nullptr;
   double* d_slo = nullptr;
   double* d_F = nullptr;
   double* d_rhs = nullptr;
@@ -2865,7 +3158,7 @@ msfec_status dalloc(msfec_q_batch* h, T** p, size_t count) {
 }
 
 void free_all(msfec_q_batch* h) {
-  void* ptrs[] = {h->d_pb,    h->d_conv,  h->d_rres,  h->d_rows,  h->d_verts, h->d_coef, h->d_f,     h->d_reac,  h->d_slo,   h->d_F,
+  void* ptrs[] = {h->d_pb,    h->d_conv,  h->d_cgit,  h->d_rres,  h->d_rows,  h->d_verts, h->d_coef, h->d_f,     h->d_reac,  h->d_slo,   h->d_F,
                   h->d_rhs,   h->d_L,    h->d_W,     h->d_X,     h->d_psi,   h->d_M,
                   h->d_b,     h->d_status, h->d_inode, h->d_imask, h->d_cK,  h->d_crhs,
                   h->d_cvec,  h->d_cit};
@@ -2956,8 +3249,15 @@ msfec_status msfec_q_create(const msfec_q_params* params, int n_cells, const dou
   if (p.reaction_mode < MSFEC_REACTION_NONE || p.reaction_mode > MSFEC_REACTION_CELL)
     return fail(MSFEC_ERR_INVALID, "unknown reaction_mode");
   if (p.solver != MSFEC_SOLVER_BAND_FP64 && p.solver != MSFEC_SOLVER_BAND_FP64_MFMA &&
-      p.solver != MSFEC_SOLVER_BAND_MIXED)
+      p.solver != MSFEC_SOLVER_BAND_MIXED && p.solver != MSFEC_SOLVER_CG)
     return fail(MSFEC_ERR_INVALID, "unknown solver");
+  if (p.solver == MSFEC_SOLVER_CG &&
+      (p.coef_mode != MSFEC_COEF_SCALAR_CELL || p.rhs_mode == MSFEC_RHS_QP))
+    return fail(MSFEC_ERR_UNSUPPORTED,
+                "the CG solver needs a scalar coefficient per fine cell and a constant or "
+                "per-cell load");
+  if (p.solver == MSFEC_SOLVER_CG && (p.cg_tol < 0.0 || p.cg_max_iter < 0))
+    return fail(MSFEC_ERR_INVALID, "cg_tol and cg_max_iter must not be negative");
   if (p.max_refine < 0 || p.max_refine > 16)
     return fail(MSFEC_ERR_INVALID, "max_refine must be in 0..16");
   if (p.solver == MSFEC_SOLVER_BAND_MIXED &&
@@ -2968,7 +3268,15 @@ msfec_status msfec_q_create(const msfec_q_params* params, int n_cells, const dou
   const int N = 1 << p.n_refine_local;
   const int ni = N - 1;
   const int bw = (p.dim == 2) ? ni + 1 : ni * ni + ni + 1;
-  if (bw + 1 > kWin)
+  if (p.solver == MSFEC_SOLVER_CG) {
+    const long long nint = (long long)ipow(ni, p.dim), nnod = (long long)ipow(N + 1, p.dim);
+    const long long lds = ((long long)ipow(N, p.dim) *
+                               (1 + (p.reaction_mode == MSFEC_REACTION_CELL ? 1 : 0)) +
+                           nnod + nint) * (long long)sizeof(double);
+    if (nint > (long long)kCgThreads * kCgOwn || lds > 160 * 1024)
+      return fail(MSFEC_ERR_UNSUPPORTED,
+                  "cell too fine for the CG workgroup (3-D: n_refine_local <= 4, 2-D: <= 6)");
+  } else if (bw + 1 > kWin)
     return fail(MSFEC_ERR_UNSUPPORTED,
                 "interior half-bandwidth " + std::to_string(bw) +
                     " exceeds the 63 of the band solver (3-D: n_refine_local <= 3, 2-D: <= 5)");
@@ -3068,12 +3376,16 @@ msfec_status msfec_q_create(const msfec_q_params* params, int n_cells, const dou
     TRY(dalloc(h, &h->d_rres, (size_t)n_cells * c.n * kMaxNb));
     TRY(dalloc(h, &h->d_conv, (size_t)n_cells));
   }
+  if (p.solver == MSFEC_SOLVER_CG) {
+    TRY(dalloc(h, &h->d_rres, (size_t)n_cells * c.n * kMaxNb));
+    TRY(dalloc(h, &h->d_cgit, (size_t)n_cells * nv * 2));
+  }
   if (!h->fast) {
     TRY(dalloc(h, &h->d_slo, (size_t)n_cells * c.nn * c.nlow));
     TRY(dalloc(h, &h->d_F, (size_t)n_cells * c.nn));
     TRY(dalloc(h, &h->d_rhs, (size_t)n_cells * c.n * kMaxNb));
   }
-  {
+  if (p.solver != MSFEC_SOLVER_CG) {
     // VALU variant: one column of 64 per pivot; MFMA variant: a 64 x 4 panel
     // per super-step of 4 pivots
     const size_t nq = (size_t)(c.n + 3) / 4;
@@ -3204,6 +3516,11 @@ msfec_status msfec_q_compute(msfec_q_handle h, void* stream) {
                         (1 + (c.reaction_mode == MSFEC_REACTION_CELL ? 1 : 0) +
                          (c.rhs_mode == MSFEC_RHS_CELL ? 1 : 0)) *
                         sizeof(double);
+    if (rdyn > 48 * 1024) {
+      MSFEC_HIP(hipFuncSetAttribute(c.dim == 3 ? (const void*)q_rows_kernel<3>
+                                               : (const void*)q_rows_kernel<2>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)rdyn));
+    }
     if (c.dim == 3)
       hipLaunchKernelGGL((q_rows_kernel<3>), dim3(h->n_cells), dim3(256), rdyn, s, a);
     else
@@ -3225,7 +3542,38 @@ msfec_status msfec_q_compute(msfec_q_handle h, void* stream) {
                                 sizeof(double)
                           : 0;
 #define LAUNCH_BAND(K, D, F) hipLaunchKernelGGL((K<D, F>), dim3(h->n_cells), dim3(64), dyn, s, a)
-  if (mixed) {
+  if (h->p.solver == MSFEC_SOLVER_CG) {
+    CgArgs g{};
+    g.tol = h->p.cg_tol > 0.0 ? h->p.cg_tol : 1e-13;
+    g.max_iter = h->p.cg_max_iter > 0 ? h->p.cg_max_iter : 5000;
+    g.iters = h->d_cgit;
+    const size_t cdyn = ((size_t)c.nfc * (1 + (c.reaction_mode == MSFEC_REACTION_CELL ? 1 : 0)) +
+                         (size_t)c.nn + (size_t)c.n) * sizeof(double);
+    const bool reac = c.reaction_mode != MSFEC_REACTION_NONE;
+#define LAUNCH_CG(D, R)                                                                   \
+  do {                                                                                    \
+    MSFEC_HIP(hipFuncSetAttribute((const void*)q_cg_kernel<D, R>,                         \
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)cdyn)); \
+    hipLaunchKernelGGL((q_cg_kernel<D, R>), dim3(h->n_cells * c.nv), dim3(kCgThreads), cdyn, \
+                       s, a, g);                                                          \
+  } while (0)
+    if (c.dim == 3) {
+      if (reac) LAUNCH_CG(3, true);
+      else LAUNCH_CG(3, false);
+    } else {
+      if (reac) LAUNCH_CG(2, true);
+      else LAUNCH_CG(2, false);
+    }
+#undef LAUNCH_CG
+    MSFEC_HIP(hipGetLastError());
+    if (tm) MSFEC_HIP(hipEventRecord(ev[2], s));
+    if (c.dim == 3)
+      hipLaunchKernelGGL((q_cg_finish_kernel<3>), dim3(h->n_cells), dim3(256), 0, s, a, g);
+    else
+      hipLaunchKernelGGL((q_cg_finish_kernel<2>), dim3(h->n_cells), dim3(256), 0, s, a, g);
+    MSFEC_HIP(hipGetLastError());
+    if (tm) MSFEC_HIP(hipEventRecord(ev[3], s));
+  } else if (mixed) {
     // FP32 factor and first solution, FP64 residual, FP32 correction sweep
     // X of the cell in LDS, and its lower stencil too when both fit in 64 KB
     const int stage_lo = ((size_t)c.n * (kMaxNb + c.nlow) * sizeof(double) <= 65536) ? 1 : 0;
@@ -3269,7 +3617,7 @@ msfec_status msfec_q_compute(msfec_q_handle h, void* stream) {
   }
 #undef LAUNCH_BAND
   MSFEC_HIP(hipGetLastError());
-  if (tm && !mixed) {
+  if (tm && !mixed && h->p.solver != MSFEC_SOLVER_CG) {
     MSFEC_HIP(hipEventRecord(ev[2], s));
     MSFEC_HIP(hipEventRecord(ev[3], s));
   }
@@ -3283,6 +3631,17 @@ msfec_status msfec_q_compute(msfec_q_handle h, void* stream) {
   }
   if (tm) MSFEC_HIP(hipEventRecord(ev[5], s));
   h->computed = true;
+  return MSFEC_OK;
+}
+
+msfec_status msfec_q_get_cg_iterations(msfec_q_handle h, int* iterations) {
+  if (!h || !iterations) return fail(MSFEC_ERR_INVALID, "null argument");
+  if (h->p.solver != MSFEC_SOLVER_CG) return fail(MSFEC_ERR_STATE, "not a CG handle");
+  if (!h->computed) return fail(MSFEC_ERR_STATE, "compute not called");
+  MSFEC_HIP(hipStreamSynchronize(h->stream));
+  std::vector<int> raw((size_t)h->n_cells * h->conf.nv * 2);
+  MSFEC_HIP(hipMemcpy(raw.data(), h->d_cgit, raw.size() * sizeof(int), hipMemcpyDeviceToHost));
+  for (size_t t = 0; t < raw.size() / 2; ++t) iterations[t] = raw[2 * t];
   return MSFEC_OK;
 }
 

@@ -9,5 +9,6 @@ from . import _lib
 from ._lib import MsfecError
 from .ned import NedBasisBatch
 from .qbasis import QBasisBatch
+from .uneven import UnevenBatch
 
-__all__ = ["QBasisBatch", "NedBasisBatch", "MsfecError", "_lib"]
+__all__ = ["QBasisBatch", "NedBasisBatch", "UnevenBatch", "MsfecError", "_lib"]

@@ -19,7 +19,7 @@ HOST, DEVICE = 0, 1
 COEF_SCALAR_CELL, COEF_TENSOR_QP = 0, 1
 RHS_CONST, RHS_CELL, RHS_QP = 0, 1, 2
 REACTION_NONE, REACTION_CONST, REACTION_CELL = 0, 1, 2
-SOLVER_BAND_FP64, SOLVER_BAND_FP64_MFMA, SOLVER_BAND_MIXED = 0, 1, 2
+SOLVER_BAND_FP64, SOLVER_BAND_FP64_MFMA, SOLVER_BAND_MIXED, SOLVER_CG = 0, 1, 2, 3
 CELL_OK, CELL_NOT_CONVERGED, CELL_NOT_SPD, CELL_NONFINITE = 0, 1, 2, 3
 FLAG_TIMING = 1
 
@@ -29,6 +29,7 @@ EXPORTED = (
     "msfec_q_compute", "msfec_q_synchronize", "msfec_q_get_element", "msfec_q_get_basis",
     "msfec_q_get_cell_status", "msfec_q_reconstruct", "msfec_q_get_info",
     "msfec_q_kernel_times", "msfec_q_destroy", "msfec_coarse_solve", "msfec_q_adjoint_gradient",
+    "msfec_q_get_cg_iterations",
     "msfec_ned_create", "msfec_ned_set_coefficients", "msfec_ned_compute",
     "msfec_ned_synchronize", "msfec_ned_get_element", "msfec_ned_get_basis",
     "msfec_ned_get_cell_status", "msfec_ned_kernel_times", "msfec_ned_destroy",
@@ -48,6 +49,8 @@ class Params(ctypes.Structure):
         ("solver", ctypes.c_int),
         ("flags", ctypes.c_int),
         ("max_refine", ctypes.c_int),
+        ("cg_tol", ctypes.c_double),
+        ("cg_max_iter", ctypes.c_int),
     ]
 
 
@@ -116,6 +119,7 @@ def load(path: str | None = None):
                                      ctypes.c_int, vp, ctypes.c_int, vp, ctypes.c_int, ip, dp]
     L.msfec_q_adjoint_gradient.argtypes = [vp, ctypes.c_int, vp, ctypes.c_int, ctypes.c_double,
                                            ctypes.c_int, dp, vp, ctypes.c_int]
+    L.msfec_q_get_cg_iterations.argtypes = [vp, ip]
     L.msfec_ned_create.argtypes = [ctypes.POINTER(NedParams), ctypes.c_int, vp, ctypes.c_int,
                                    ctypes.POINTER(vp)]
     L.msfec_ned_set_coefficients.argtypes = [vp, vp, vp, vp, vp, ctypes.c_int]

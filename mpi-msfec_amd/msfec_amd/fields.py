@@ -125,3 +125,46 @@ def diffusion_A_tensor(points: np.ndarray, freq=(17, 16, 15), scale=(1.0, 1.0, 1
     A = np.einsum("ik,...k,jk->...ij", R, D, R)
     return np.stack([A[..., 0, 0], A[..., 0, 1], A[..., 0, 2], A[..., 1, 1], A[..., 1, 2],
                      A[..., 2, 2]], -1)
+
+
+def inclusion_centres(dim: int, n_inclusions: int, seed: int) -> np.ndarray:
+    """Seeded inclusion centres in the unit cube (the generator of
+    inclusion_coefficient)."""
+    return np.random.default_rng(seed).random((n_inclusions, dim))
+
+
+def uneven_levels(dim: int, nc: int, centres: np.ndarray, seed: int, fine: int = 4,
+                  coarse=(2, 3)) -> np.ndarray:
+    """Refinement level per coarse cell: ``fine`` for the cells that hold an
+    inclusion centre, otherwise one of ``coarse`` by a seeded coin."""
+    rng = np.random.default_rng(seed)
+    lv = rng.choice(np.asarray(coarse), size=nc ** dim)
+    idx = np.minimum((centres * nc).astype(int), nc - 1)
+    flat = idx[:, 0] + nc * idx[:, 1] + (nc * nc * idx[:, 2] if dim == 3 else 0)
+    lv[np.unique(flat)] = fine
+    return lv
+
+
+def inclusion_blocks(dim: int, nc: int, levels: np.ndarray, centres: np.ndarray, radius: float,
+                     value: float, background: float = 1.0) -> list:
+    """Per-cell coefficient blocks at each cell's own refinement: ``value`` on
+    the fine cells whose centre lies in a ball of ``radius`` around a centre,
+    ``background`` elsewhere (lexicographic, x fastest)."""
+    h = 1.0 / nc
+    out = []
+    r2 = radius * radius
+    for c in range(nc ** dim):
+        ix = [c % nc, (c // nc) % nc, c // (nc * nc)][:dim]
+        N = 1 << int(levels[c])
+        g = (np.arange(N) + 0.5) / N
+        axes = [(ix[d] + g) * h for d in range(dim)]
+        grids = np.meshgrid(*axes[::-1], indexing="ij")[::-1]   # grids[d] varies along axis d
+        blk = np.full((N,) * dim, background)
+        lo = np.array([ix[d] * h for d in range(dim)]) - radius
+        hi = lo + h + 2 * radius
+        near = np.all((centres >= lo) & (centres <= hi), axis=1)
+        for cc in centres[near]:
+            d2 = sum((grids[d] - cc[d]) ** 2 for d in range(dim))
+            blk[d2 <= r2] = value
+        out.append(blk.reshape(-1))
+    return out

@@ -32,7 +32,8 @@ def _ptr(a):
 class QBasisBatch:
     def __init__(self, vertices, dim=3, n_refine_local=3, coef_mode=L.COEF_SCALAR_CELL,
                  rhs_mode=L.RHS_CONST, rhs_const=1.0, reaction_mode=L.REACTION_NONE,
-                 reaction_const=0.0, solver=L.SOLVER_BAND_FP64, timing=False, max_refine=0):
+                 reaction_const=0.0, solver=L.SOLVER_BAND_FP64, timing=False, max_refine=0,
+                 cg_tol=0.0, cg_max_iter=0):
         self._lib = L.load()
         self._h = ctypes.c_void_p()
         p = L.Params()
@@ -40,6 +41,7 @@ class QBasisBatch:
         p.rhs_const, p.reaction_mode, p.reaction_const = rhs_const, reaction_mode, reaction_const
         p.solver, p.flags = solver, (L.FLAG_TIMING if timing else 0)
         p.max_refine = max_refine
+        p.cg_tol, p.cg_max_iter = cg_tol, cg_max_iter
         vptr, vloc, keep = _ptr(vertices)
         nv = 1 << dim
         n_cells = int(np.prod(vertices.shape)) // (nv * dim)
@@ -133,6 +135,13 @@ class QBasisBatch:
             self._h, nc, ctypes.c_void_p(ut.ctypes.data), L.HOST, rel_tol, max_iter,
             ctypes.byref(J), ctypes.c_void_p(grad.ctypes.data), L.HOST))
         return J.value, grad
+
+    def cg_iterations(self):
+        """(n_cells, nb) CG iterations of the last compute (SOLVER_CG)."""
+        it = np.empty((self.n_cells, self.nb), dtype=np.int32)
+        L.check(self._lib.msfec_q_get_cg_iterations(
+            self._h, it.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
+        return it
 
     def kernel_times(self):
         ms = (ctypes.c_float * 8)()

@@ -135,3 +135,12 @@ def test_ned_rejects_non_box_and_bad_level(lib):
     L = lib.load()
     assert L.msfec_ned_create(None, 1, None, 0, None) == lib.ERR_INVALID
     assert L.msfec_ned_compute(None, None) == lib.ERR_INVALID
+
+
+def test_cg_solver_limits(lib):
+    """CG lifts the band limit (3-D l = 4 accepted up to the allocation) but
+    refuses tensor coefficients and cells beyond its workgroup."""
+    st, msg = _create(lib, coef_mode=1, solver=lib.SOLVER_CG)
+    assert st == lib.ERR_UNSUPPORTED and "CG" in msg
+    st, msg = _create(lib, dim=3, l=5, solver=lib.SOLVER_CG)
+    assert st == lib.ERR_UNSUPPORTED and "CG" in msg
