@@ -667,9 +667,11 @@ __global__ __launch_bounds__(256) void q_rows_kernel(BandArgs a) {
     for (int d = 0; d < DIM; ++d) vol *= (v[(D::NV - 1) * DIM + d] - v[d]) / a.N;
   }
   const int tid = threadIdx.x;
-  const int pi = tid >> 3, pk = tid & 7;
-  const bool pthr = tid < 64 && pi < D::NV && pk < D::NV;
-  const bool bthr = tid >= 64 && tid < 64 + D::NV;
+  // thread (quarter qt, pair (pi, pk)) sums its quarter of every chunk:
+  // 4 independent chains over 64 nodes instead of one over 256
+  const int qt = tid >> 6, pi = (tid >> 3) & 7, pk = tid & 7;
+  const bool pthr = pi < D::NV && pk < D::NV;
+  const bool bthr = pi == 0 && pk < D::NV;
   double pacc = 0.0, bacc = 0.0;
   for (int base = 0; base < a.nn; base += 256) {
     const int node = base + tid;
@@ -749,15 +751,26 @@ __global__ __launch_bounds__(256) void q_rows_kernel(BandArgs a) {
     fsh[tid] = fsum;
     __syncthreads();
     if (pthr) {
-      for (int l = 0; l < 256; ++l) pacc = fma(gs[l * 9 + pi], kps[l * 9 + pk], pacc);
+      double p4[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 16
+      for (int l = 64 * qt; l < 64 * qt + 64; ++l)
+        p4[l & 3] = fma(gs[l * 9 + pi], kps[l * 9 + pk], p4[l & 3]);
+      pacc += (p4[0] + p4[1]) + (p4[2] + p4[3]);
     }
     if (bthr) {
-      for (int l = 0; l < 256; ++l) bacc = fma(gs[l * 9 + (tid - 64)], fsh[l], bacc);
+      for (int l = 64 * qt; l < 64 * qt + 64; ++l) bacc = fma(gs[l * 9 + pk], fsh[l], bacc);
     }
   }
+  // the 4 quarters add up (the chunk buffers are free again)
+  __syncthreads();
+  gs[tid] = pacc;
+  kps[tid] = bacc;
+  __syncthreads();
   double* pb = a.Pb + (size_t)cell * kPStride;
-  if (pthr) pb[pi * 8 + pk] = pacc;
-  if (bthr) pb[64 + (tid - 64)] = bacc;
+  if (tid < 64 && pthr)
+    pb[pi * 8 + pk] = (gs[tid] + gs[64 + tid]) + (gs[128 + tid] + gs[192 + tid]);
+  if (tid < 64 && bthr)
+    pb[64 + pk] = (kps[tid] + kps[64 + tid]) + (kps[128 + tid] + kps[192 + tid]);
   if (tid == 0) pb[72] = vol / D::NV;
 }
 
