@@ -548,7 +548,7 @@ __device__ __forceinline__ void box_tables(int x, const double* hh, double& ks, 
 // per cell, one thread per interior row; all branching stays out of the
 // serial factorisation.
 // ---------------------------------------------------------------------------
-template <int DIM>
+template <int DIM, bool REAC>
 __global__ __launch_bounds__(256) void q_rows_kernel(BandArgs a) {
   using D = Dirs<DIM>;
   const int cell = blockIdx.x;
@@ -619,7 +619,7 @@ __global__ __launch_bounds__(256) void q_rows_kernel(BandArgs a) {
         const int m = c[d] + dd;
         if (m == 0 || m == a.N) bnd = true;
       }
-      const double kval = kv[mask] * SA[e] + mv[mask] * SR[e];
+      const double kval = kv[mask] * SA[e] + (REAC ? mv[mask] * SR[e] : 0.0);
       wb[e] = (bnd && e != D::CENTER) ? -kval : 0.0;
       if (e == D::CENTER) lo[0] = kval;
       else if (!bnd && e < D::CENTER) lo[1 + e] = kval;
@@ -738,7 +738,7 @@ __global__ __launch_bounds__(256) void q_rows_kernel(BandArgs a) {
           if (m < 0 || m > a.N) inside = false;
           if (m == 0 || m == a.N) mb = true;
         }
-        wb[e] = (inside && mb) ? kv[mask] * SA[e] + mv[mask] * SR[e] : 0.0;
+        wb[e] = (inside && mb) ? kv[mask] * SA[e] + (REAC ? mv[mask] * SR[e] : 0.0) : 0.0;
       }
       lift_contract<DIM>(wb, fl, kp);
     }
@@ -3529,15 +3529,22 @@ msfec_status msfec_q_compute(msfec_q_handle h, void* stream) {
                         (1 + (c.reaction_mode == MSFEC_REACTION_CELL ? 1 : 0) +
                          (c.rhs_mode == MSFEC_RHS_CELL ? 1 : 0)) *
                         sizeof(double);
-    if (rdyn > 48 * 1024) {
-      MSFEC_HIP(hipFuncSetAttribute(c.dim == 3 ? (const void*)q_rows_kernel<3>
-                                               : (const void*)q_rows_kernel<2>,
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)rdyn));
+    const bool reac = c.reaction_mode != MSFEC_REACTION_NONE;
+#define LAUNCH_ROWS(D, R)                                                                  \
+  do {                                                                                     \
+    if (rdyn > 48 * 1024)                                                                  \
+      MSFEC_HIP(hipFuncSetAttribute((const void*)q_rows_kernel<D, R>,                      \
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)rdyn)); \
+    hipLaunchKernelGGL((q_rows_kernel<D, R>), dim3(h->n_cells), dim3(256), rdyn, s, a);    \
+  } while (0)
+    if (c.dim == 3) {
+      if (reac) LAUNCH_ROWS(3, true);
+      else LAUNCH_ROWS(3, false);
+    } else {
+      if (reac) LAUNCH_ROWS(2, true);
+      else LAUNCH_ROWS(2, false);
     }
-    if (c.dim == 3)
-      hipLaunchKernelGGL((q_rows_kernel<3>), dim3(h->n_cells), dim3(256), rdyn, s, a);
-    else
-      hipLaunchKernelGGL((q_rows_kernel<2>), dim3(h->n_cells), dim3(256), rdyn, s, a);
+#undef LAUNCH_ROWS
     MSFEC_HIP(hipGetLastError());
   }
   if (!fast) {
