@@ -42,6 +42,11 @@ constexpr int kMaxLow = 14;    // lower stencil entries incl. diagonal (3-D)
 constexpr int kRowStride = 24; // row table: 16 lower-stencil slots + 8 lifting rhs
 constexpr int kPStride = 80;   // per cell: P (8 x 8) | b_B (8) | |cell| / 2^dim
 
+// Panel slots per cell of the band solvers: one per super-step of 4 pivots,
+// rounded up to whole tile rows of 16 pivots (the matrix-core factorisation
+// runs every super-step of its last tile row).
+inline int panel_quads(int n) { return ((n + 15) / 16) * 4; }
+
 struct QConf {
   int dim, N, S, nn, nfc, nv, ni, n, nst, nlow, center;
   int coef_mode, rhs_mode, reaction_mode;
@@ -1846,7 +1851,10 @@ __device__ __forceinline__ void bk_stage(const BandArgs& a, int cell, int jbase,
 }
 
 // The 4 super-steps of tile row T (pivots j0 .. j0+15); the next 16 rows of
-// the band enter the freed slots.
+// the band enter the freed slots.  All 4 run even past the last row (rows
+// >= n are isolated unit diagonals, their panels zero): with no branch
+// between them, the waits for the row prefetch leave the panel stores of the
+// last super-steps in flight instead of waiting for their completion.
 template <typename R, int DIM, int T>
 __device__ __forceinline__ void bk_tile_row(const BandArgs& a, const BkAddr& ad, int cell, int j0,
                                             int my_off,
@@ -1855,20 +1863,19 @@ __device__ __forceinline__ void bk_tile_row(const BandArgs& a, const BkAddr& ad,
                                             typename Mf<R>::v4& Sacc, BandSmem& sh,
                                             BkSmem<R>& ms, RowPrefetch (&pf)[2], BkMon<R>& bad,
                                             const R (&ek)[4]) {
-  const int n = a.n;
   bk_stage(a, cell, j0 + kWin, sh, pf[0]);
   bk_superstep<R, DIM, T, 0>(a, ad, cell, j0, Tl, Yt, Sacc, ms, bad, ek);
   bk_insert<R, DIM, T, 0>(a, ad, j0 + kWin, 0, my_off, Tl, Yt, sh, ms);
-  if (j0 + 4 < n) {
+  {
     bk_superstep<R, DIM, T, 1>(a, ad, cell, j0 + 4, Tl, Yt, Sacc, ms, bad, ek);
     bk_insert<R, DIM, T, 1>(a, ad, j0 + 4 + kWin, 4, my_off, Tl, Yt, sh, ms);
   }
   bk_stage(a, cell, j0 + 8 + kWin, sh, pf[1]);
-  if (j0 + 8 < n) {
+  {
     bk_superstep<R, DIM, T, 2>(a, ad, cell, j0 + 8, Tl, Yt, Sacc, ms, bad, ek);
     bk_insert<R, DIM, T, 2>(a, ad, j0 + 8 + kWin, 0, my_off, Tl, Yt, sh, ms);
   }
-  if (j0 + 12 < n) {
+  {
     bk_superstep<R, DIM, T, 3>(a, ad, cell, j0 + 12, Tl, Yt, Sacc, ms, bad, ek);
     bk_insert<R, DIM, T, 3>(a, ad, j0 + 12 + kWin, 4, my_off, Tl, Yt, sh, ms);
   }
@@ -3235,7 +3242,7 @@ BandArgs band_args(const msfec_q_batch* h) {
   a.M = h->d_M;
   a.bvec = h->d_b;
   a.status = h->d_status;
-  a.nq = (c.n + 3) / 4;
+  a.nq = panel_quads(c.n);
   return a;
 }
 
@@ -3401,7 +3408,7 @@ msfec_status msfec_q_create(const msfec_q_params* params, int n_cells, const dou
   if (p.solver != MSFEC_SOLVER_CG) {
     // VALU variant: one column of 64 per pivot; MFMA variant: a 64 x 4 panel
     // per super-step of 4 pivots
-    const size_t nq = (size_t)(c.n + 3) / 4;
+    const size_t nq = (size_t)panel_quads(c.n);
     TRY(dalloc(h, &h->d_L, (size_t)n_cells * std::max((size_t)c.n * kWin, nq * 256)));
     TRY(dalloc(h, &h->d_W, (size_t)n_cells * std::max((size_t)c.n * kMaxNb, nq * 32)));
   }
