@@ -1597,6 +1597,8 @@ struct Mf<double> {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
   static __device__ __forceinline__ int cslot(int ll) { return ll; }
+  // lane holding (row slot t, column slot cs) of a register of a tile
+  static constexpr int lane_of(int t, int cs) { return 16 * t + cs; }
   // physical accumulator row of register g in lane group t
   static __device__ __forceinline__ int acc_row(int g, int t) { return 4 * g + t; }
 };
@@ -1607,6 +1609,7 @@ struct Mf<float> {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
   static __device__ __forceinline__ int cslot(int ll) { return 4 * (ll & 3) + (ll >> 2); }
+  static constexpr int lane_of(int t, int cs) { return 16 * t + 4 * (cs & 3) + (cs >> 2); }
   static __device__ __forceinline__ int acc_row(int g, int t) { return 4 * t + g; }
 };
 
@@ -1686,35 +1689,51 @@ __device__ __forceinline__ void bk_extract(const BkAddr& ad, const typename Mf<R
 }
 
 // Insert the rows jbase..jbase+3 (staged as rows rb..rb+3) into slots
-// 16T + 4G .. +3: scatter their nonzeros into LDS, then place rows and columns.
-template <typename R, int DIM, int T, int G>
-__device__ __forceinline__ void bk_insert(const BandArgs& a, const BkAddr& ad, int jbase, int rb,
-                                          int my_off, typename Mf<R>::v4 (&Tl)[10],
-                                          typename Mf<R>::v4 (&Yt)[4], BandSmem& sh,
-                                          BkSmem<R>& ms) {
-  using D = Dirs<DIM>;
-  constexpr int s0 = 16 * T + 4 * G;
+// 16T + 4G .. +3 in two parts.  The scatter writes their nonzeros into LDS
+// in the slot layout: lane (t = lh, e = ll) places lower entry e of row
+// jbase + t, and a coupling between two incoming rows also in the earlier
+// row.  It has no branch: a lane with nothing to place writes into a slot
+// of its own in the pool (unused during the elimination).  The placement
+// then copies rows and columns into the window tiles.
+template <typename R>
+__device__ __forceinline__ double bk_scatter_load(const BkAddr& ad, int rb, BandSmem& sh,
+                                                  BkSmem<R>& ms) {
 #pragma unroll
   for (int t = 0; t < 4; ++t) ms.nrow[ad.zr + 72 * t] = (R)0;
-  wave_sync();
-  {
-    const int t = ad.lh, e = ad.ll, jn = jbase + t;
-    if (jn < a.n) {
-      if (e < D::NLOW) {
-        const double v = sh.stg_low[rb + t][e];
-        if (v != 0.0) {
-          // slot of row jn - off; jbase = s0 mod 64
-          ms.nrow[t * 72 + pad8((bk_opaque(jbase) + t - my_off) & (kWin - 1))] = (R)v;
-          // couplings between two incoming rows are needed in both rows
-          // (pad8(s0 + t) = pad8(s0) + t as s0 is a multiple of 4)
-          if (e > 0 && my_off <= t) ms.nrow[(t - my_off) * 72 + pad8(s0) + t] = (R)v;
-        }
-      }
-    } else if (e == 0) {
-      ms.nrow[t * 72 + pad8(s0) + t] = (R)1;  // past the end: isolated unit diagonal
-    }
-  }
-  wave_sync();
+  return sh.stg_low[rb + ad.lh][ad.ll];
+}
+
+template <typename R, int DIM, int T, int G>
+__device__ __forceinline__ void bk_scatter_store(const BandArgs& a, const BkAddr& ad, int jbase,
+                                                 int my_off, double v, BandSmem& sh,
+                                                 BkSmem<R>& ms) {
+  using D = Dirs<DIM>;
+  constexpr int s0 = 16 * T + 4 * G;
+  // lane-dependent addresses from the opaque lane index: recomputed here,
+  // not hoisted out of the loop as one register per super-step
+  const int lane = bk_lane();
+  const int t = lane >> 4, e = lane & 15;
+  const bool live = jbase + t < a.n;
+  // zeros stay unwritten: on a grid with one interior node per direction
+  // some offsets coincide (or are 0), and their couplings are all zero
+  const bool w1 = live && e < D::NLOW && v != 0.0;
+  const bool w2 = w1 && e > 0 && my_off <= t;
+  const bool w3 = !live && e == 0;  // past the end: isolated unit diagonal
+  R* dummy = reinterpret_cast<R*>(SH_PART(sh)) + lane;
+  // slot of column jn - off (jbase = s0 mod 64); pad8(s0 + t) = pad8(s0) + t
+  const int i1 = w1 ? t * 72 + pad8((bk_opaque(jbase) + t - my_off) & (kWin - 1))
+                    : t * 72 + pad8(s0) + t;
+  R* p1 = (w1 || w3) ? &ms.nrow[i1] : dummy;
+  *p1 = w1 ? (R)v : (R)1;
+  R* p2 = w2 ? &ms.nrow[(t - my_off) * 72 + pad8(s0) + t] : dummy + 64;
+  *p2 = (R)v;
+}
+
+template <typename R, int DIM, int T, int G>
+__device__ __forceinline__ void bk_place(const BandArgs& a, const BkAddr& ad, int jbase, int rb,
+                                         typename Mf<R>::v4 (&Tl)[10],
+                                         typename Mf<R>::v4 (&Yt)[4], BandSmem& sh,
+                                         BkSmem<R>& ms) {
 #pragma unroll
   for (int Q = 0; Q <= T; ++Q) Tl[MF_TILE(T, Q)][G] = ms.nrow[ad.row + 18 * Q];
   if (ad.gcol[G]) {
@@ -1724,7 +1743,29 @@ __device__ __forceinline__ void bk_insert(const BandArgs& a, const BkAddr& ad, i
       for (int r = 0; r < 4; ++r) Tl[MF_TILE(P, T)][r] = ms.nrow[ad.col + 18 * P + kColR[r]];
   }
   Yt[T][G] = (ad.ll < 8 && jbase + ad.lh < a.n) ? (R)sh.stg_rhs[rb + ad.lh][ad.ll & 7] : (R)0;
+}
+
+template <typename R, int DIM, int T, int G>
+__device__ __forceinline__ void bk_insert(const BandArgs& a, const BkAddr& ad, int jbase, int rb,
+                                          int my_off, typename Mf<R>::v4 (&Tl)[10],
+                                          typename Mf<R>::v4 (&Yt)[4], BandSmem& sh,
+                                          BkSmem<R>& ms) {
+  const double v = bk_scatter_load<R>(ad, rb, sh, ms);
+  bk_scatter_store<R, DIM, T, G>(a, ad, jbase, my_off, v, sh, ms);
   wave_sync();
+  bk_place<R, DIM, T, G>(a, ad, jbase, rb, Tl, Yt, sh, ms);
+  wave_sync();
+}
+
+// A value of one lane, read into every lane.
+__device__ __forceinline__ double bk_readlane(double v, int l) {
+  const long long b = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_readlane((int)b, l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ float bk_readlane(float v, int l) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
 }
 
 // 1/sqrt: the hardware estimate, refined by two Newton steps in FP64 (one
@@ -1783,22 +1824,25 @@ __device__ __forceinline__ void bk_superstep(const BandArgs& a, const BkAddr& ad
   ms.yp[ad.lane] = Yt[T][G];
   wave_sync();
   const R* pv = ms.piv;
-  // C C^T = pivot block (every lane); ci_t = 1 / C[t][t]
-  const R a00 = pv[0 * 72 + pad8(s0 + 0)];
+  // C C^T = pivot block (every lane), read from the lanes of the diagonal
+  // tile that hold it; ci_t = 1 / C[t][t]
+  const R pk = Tl[MF_TILE(T, T)][G];
+  auto pe = [&](int t, int u) { return bk_readlane(pk, M::lane_of(t, 4 * G + u)); };
+  const R a00 = pe(0, 0);
   bk_check(a00, bad);
   const R ci0 = bk_rsqrt(a00);
-  const R c10 = pv[1 * 72 + pad8(s0 + 0)] * ci0, c20 = pv[2 * 72 + pad8(s0 + 0)] * ci0;
-  const R c30 = pv[3 * 72 + pad8(s0 + 0)] * ci0;
-  const R d1 = pv[1 * 72 + pad8(s0 + 1)] - c10 * c10;
+  const R c10 = pe(1, 0) * ci0, c20 = pe(2, 0) * ci0;
+  const R c30 = pe(3, 0) * ci0;
+  const R d1 = pe(1, 1) - c10 * c10;
   bk_check(d1, bad);
   const R ci1 = bk_rsqrt(d1);
-  const R c21 = (pv[2 * 72 + pad8(s0 + 1)] - c20 * c10) * ci1;
-  const R c31 = (pv[3 * 72 + pad8(s0 + 1)] - c30 * c10) * ci1;
-  const R d2 = pv[2 * 72 + pad8(s0 + 2)] - c20 * c20 - c21 * c21;
+  const R c21 = (pe(2, 1) - c20 * c10) * ci1;
+  const R c31 = (pe(3, 1) - c30 * c10) * ci1;
+  const R d2 = pe(2, 2) - c20 * c20 - c21 * c21;
   bk_check(d2, bad);
   const R ci2 = bk_rsqrt(d2);
-  const R c32 = (pv[3 * 72 + pad8(s0 + 2)] - c30 * c20 - c31 * c21) * ci2;
-  const R d3 = pv[3 * 72 + pad8(s0 + 3)] - c30 * c30 - c31 * c31 - c32 * c32;
+  const R c32 = (pe(3, 2) - c30 * c20 - c31 * c21) * ci2;
+  const R d3 = pe(3, 3) - c30 * c30 - c31 * c31 - c32 * c32;
   bk_check(d3, bad);
   const R ci3 = bk_rsqrt(d3);
   // row lh of C^-1: x C = e_lh solved from the last column backwards
@@ -1866,19 +1910,13 @@ __device__ __forceinline__ void bk_tile_row(const BandArgs& a, const BkAddr& ad,
   bk_stage(a, cell, j0 + kWin, sh, pf[0]);
   bk_superstep<R, DIM, T, 0>(a, ad, cell, j0, Tl, Yt, Sacc, ms, bad, ek);
   bk_insert<R, DIM, T, 0>(a, ad, j0 + kWin, 0, my_off, Tl, Yt, sh, ms);
-  {
-    bk_superstep<R, DIM, T, 1>(a, ad, cell, j0 + 4, Tl, Yt, Sacc, ms, bad, ek);
-    bk_insert<R, DIM, T, 1>(a, ad, j0 + 4 + kWin, 4, my_off, Tl, Yt, sh, ms);
-  }
+  bk_superstep<R, DIM, T, 1>(a, ad, cell, j0 + 4, Tl, Yt, Sacc, ms, bad, ek);
+  bk_insert<R, DIM, T, 1>(a, ad, j0 + 4 + kWin, 4, my_off, Tl, Yt, sh, ms);
   bk_stage(a, cell, j0 + 8 + kWin, sh, pf[1]);
-  {
-    bk_superstep<R, DIM, T, 2>(a, ad, cell, j0 + 8, Tl, Yt, Sacc, ms, bad, ek);
-    bk_insert<R, DIM, T, 2>(a, ad, j0 + 8 + kWin, 0, my_off, Tl, Yt, sh, ms);
-  }
-  {
-    bk_superstep<R, DIM, T, 3>(a, ad, cell, j0 + 12, Tl, Yt, Sacc, ms, bad, ek);
-    bk_insert<R, DIM, T, 3>(a, ad, j0 + 12 + kWin, 4, my_off, Tl, Yt, sh, ms);
-  }
+  bk_superstep<R, DIM, T, 2>(a, ad, cell, j0 + 8, Tl, Yt, Sacc, ms, bad, ek);
+  bk_insert<R, DIM, T, 2>(a, ad, j0 + 8 + kWin, 0, my_off, Tl, Yt, sh, ms);
+  bk_superstep<R, DIM, T, 3>(a, ad, cell, j0 + 12, Tl, Yt, Sacc, ms, bad, ek);
+  bk_insert<R, DIM, T, 3>(a, ad, j0 + 12 + kWin, 4, my_off, Tl, Yt, sh, ms);
 }
 
 // Initial fill of tile row T with the rows 16T .. 16T+15.
