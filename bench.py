@@ -165,13 +165,14 @@ def ned_config(steps: int, warmup: int, check: bool, cpu: bool):
     tflops = flops_cell * n_cells / (kt[1] * 1e-3) / 1e12
     out = {"workload": "2D Q1 x Nedelec basis, 32x32=1024 coarse cells x 16x16 fine cells "
                        "(833 DoFs, 705 interior, 4+4 RHS), symmetric indefinite band LDL^T FP64 "
-                       "on the vector ALUs, equilibrated, no pivoting (edge-after-node order)",
+                       "on the matrix cores (4x4 pivot blocks), equilibrated, no pivoting "
+                       "(edge-after-node order)",
            "cells_per_s": round(n_cells / (wall / steps), 1),
            "ms_per_step": round(wall * 1e3 / steps, 4),
            "kernel_ms": {"rows": round(kt[0], 4), "factor_solve": round(kt[1], 4),
-                         "element": round(kt[2], 4)},
+                         "psi_element": round(kt[2], 4)},
            "failed_cells": int((nb.get_cell_status() != 0).sum()),
-           "roofline": {"bound": "fp64-valu", "kernel": "ned_factor_kernel",
+           "roofline": {"bound": "mfma", "kernel": "q_blk_factor_kernel<double, 2, 1, true>",
                         "flops_per_cell": flops_cell, "achieved_tflops": round(tflops, 3),
                         "peak_tflops": FP64_PEAK_TFLOPS,
                         "frac": round(tflops / FP64_PEAK_TFLOPS, 4)}}

@@ -15,12 +15,14 @@
 //   ned_rows_kernel    one thread per interior row: the row of the interior
 //                      system with the sigma rows negated (which makes it
 //                      symmetric), in band order, its lower entries (at most
-//                      16) and the 8 condensed right-hand sides; the
-//                      boundary values of the 8 bases to Psi.
-//   ned_factor_kernel  one 256-thread workgroup per cell: LDL^T of the
-//                      equilibrated band matrix in a 64-row LDS window, the
-//                      forward elimination of the 8 right-hand sides with it,
-//                      then the backward sweep; the interior of Psi.
+//                      16) and the 8 condensed right-hand sides, equilibrated
+//                      (S K S, S = |diag K|^-1/2); the boundary values of the
+//                      8 bases to Psi.
+//   band_ldl_solve     (msfec_q.hip) one wave per cell: the matrix-core band
+//                      factorisation of the Q path in its L D L^T form, 4
+//                      pivots per step, with the forward elimination of the 8
+//                      right-hand sides and the backward sweep.
+//   ned_psi_kernel     the interior of Psi: the row scale times the solution.
 //   ned_element_kernel one workgroup per cell: the 8 x 8 element matrix and
 //                      the load vector from Psi staged in LDS.
 // Band order: node row j (nodes (i,j)), interleaved with the horizontal
@@ -28,8 +30,9 @@
 // after its interior end nodes.  With that order every leading principal
 // minor of the symmetric matrix [[-M, G^T], [G, C]] is non-singular (an
 // edge field that is a discrete gradient of nodes already eliminated and
-// orthogonal to their tests vanishes), so LDL^T needs no pivoting; the
-// half-bandwidth is 3m + 3 (51 at l = 4).
+// orthogonal to their tests vanishes), so LDL^T needs no pivoting, and its
+// 4 x 4 block form (the leading minors of every block are leading minors of
+// the matrix) needs none either; the half-bandwidth is 3m + 3 (51 at l = 4).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -43,33 +46,36 @@
 
 namespace msfec_detail {
 extern thread_local std::string g_last_error;
+// the matrix-core band LDL^T (msfec_q.hip)
+size_t band_ldl_panel_doubles(int n);
+size_t band_ldl_z_doubles(int n);
+hipError_t band_ldl_solve(int n_cells, int n, const double* rows, const int* loff, double* pan,
+                          double* zst, double* X, int* status, hipStream_t s);
 }
 
 namespace {
 
 constexpr int kNLow = 16;                // lower entries of a band row, diagonal first
-constexpr int kNW = 64;                  // window slots: the half-bandwidth stays <= 62
+constexpr int kMaxBand = 60;             // half-bandwidth of the matrix-core band solver
 constexpr int kNR = 8;                   // bases per cell
 constexpr int kRowLen = kNLow + kNR;     // row table: lower values, right-hand sides
-constexpr int kZLen = kNR + 2;           // z_k of the 8 bases, the row scale s_k, 1 / d_k
 constexpr int kNThreads = 256;
-constexpr int kMaxPairs = 63 * 64 / 2;
+constexpr int kMaxRows = 1024;           // interior dofs of a cell (705 at l = 4)
 
 struct NedArgs {
   int m, ns, nh, N;  // fine mesh: m cells per direction; nodes, horizontal edges, all dofs
-  int n, p, npairs, full_rhs;
+  int n, p, full_rhs;
   const int* order;  // [n] band row -> fine dof
   const int* pos;    // [N] fine dof -> band row, -1 on the boundary
   const int* loff;   // [n][kNLow] offsets row - column of the lower entries, -1 unused
-  const int* pairs;  // [npairs] (io << 8) | jo, 1 <= jo <= io <= p
   const double* verts;
   const double* a;
   const double* binv;
   const double* r;
   const double* f;
-  double* rows;  // [cells][n][kRowLen]
-  double* pan;   // [cells][n][kNW]: d_k at 0, l_{k+o,k} at o = 1..p
-  double* zs;    // [cells][n][kZLen]
+  double* rows;  // [cells][n][kRowLen], equilibrated: S K S with S = |diag K|^-1/2
+  double* scale; // [cells][n] the row scales s_r
+  double* X;     // [cells][n][8] solution of the equilibrated system
   double* psi;   // [cells][8][N]
   double* M;     // [cells][64]
   double* b;     // [cells][8]
@@ -197,6 +203,7 @@ __device__ __forceinline__ double ned_bval(const NedArgs& a, int d, int b) {
 
 __global__ __launch_bounds__(kNThreads) void ned_rows_kernel(NedArgs a) {
   const int cell = blockIdx.x;
+  __shared__ double ss[kMaxRows];  // row scales
   const NedBox bx = ned_box(a, cell);
   const int m = a.m, nf = m * m;
   const double* ca = a.a + (size_t)cell * nf;
@@ -256,10 +263,12 @@ __global__ __launch_bounds__(kNThreads) void ned_rows_kernel(NedArgs a) {
       for (int k = 0; k < 8; ++k) {
         const int pc = a.pos[dofs[k]];
         if (pc >= 0) {
+          // lower entries only: an upper coupling (o < 0) must not match the
+          // -1 that marks an unused slot
           const int o = r - pc;
 #pragma unroll
           for (int t = 0; t < kNLow; ++t)
-            if (lo[t] == o) low[t] += kr[k];
+            if (o >= 0 && lo[t] == o) low[t] += kr[k];
         } else {
 #pragma unroll
           for (int bb = 0; bb < kNR; ++bb) rhs[bb] -= kr[k] * ned_bval(a, dofs[k], bb);
@@ -274,6 +283,23 @@ __global__ __launch_bounds__(kNThreads) void ned_rows_kernel(NedArgs a) {
     for (int t = 0; t < kNLow; ++t) out[t] = low[t];
 #pragma unroll
     for (int t = 0; t < kNR; ++t) out[kNLow + t] = rhs[t];
+    const double dg = fabs(low[0]);
+    ss[r] = dg > 0.0 ? 1.0 / sqrt(dg) : 1.0;
+  }
+  __syncthreads();
+  // equilibration: entry (r, r - o) times s_r s_(r-o), right-hand sides times
+  // s_r (each thread rescales the rows it wrote)
+  for (int r = threadIdx.x; r < a.n; r += blockDim.x) {
+    double* row = a.rows + ((size_t)cell * a.n + r) * kRowLen;
+    const double sr = ss[r];
+#pragma unroll
+    for (int t = 0; t < kNLow; ++t) {
+      const int o = a.loff[(size_t)r * kNLow + t];
+      if (o >= 0) row[t] *= sr * ss[r - o];
+    }
+#pragma unroll
+    for (int t = 0; t < kNR; ++t) row[kNLow + t] *= sr;
+    a.scale[(size_t)cell * a.n + r] = sr;
   }
   for (int d = threadIdx.x; d < a.N; d += blockDim.x)
     if (a.pos[d] < 0)
@@ -281,279 +307,16 @@ __global__ __launch_bounds__(kNThreads) void ned_rows_kernel(NedArgs a) {
       for (int bb = 0; bb < kNR; ++bb) a.psi[((size_t)cell * kNR + bb) * a.N + d] = ned_bval(a, d, bb);
 }
 
-// Wait for every outstanding vector memory operation (vmcnt(0); expcnt and
-// lgkmcnt left at their maxima).
-__device__ __forceinline__ void ned_vm_wait() { __builtin_amdgcn_s_waitcnt(0x0F70); }
-
-// Entering rows of one group of 4 elimination steps, held by threads
-// 0..kRowLen+kNLow-1 (value or offset) between their load and their use.
-struct NedRowSet {
-  double v[4];
-  int o[4];
-};
-
-// Backward data of one group of 4 steps: per lane of wave 0 the panel entry
-// at offset lane, z of its basis, the row scale, 1 / pivot and the row's dof.
-struct NedBackSet {
-  double pv[4], z[4], s[4], di[4];
-  int ord[4];
-};
-
-__device__ __forceinline__ void ned_wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// one 64-bit value moved by a DPP lane pattern (32-bit halves)
-template <int CTRL>
-__device__ __forceinline__ double ned_dpp(double v) {
-  const long long b = __builtin_bit_cast(long long, v);
-  const int lo = __builtin_amdgcn_mov_dpp((int)b, CTRL, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, false);
-  return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
-}
-
-// sum over the 8 lanes of an aligned group: swap neighbours
-// (quad_perm [1,0,3,2]), swap pairs ([2,3,0,1]), mirror the half row
-__device__ __forceinline__ double ned_sum8(double v) {
-  v += ned_dpp<0xB1>(v);
-  v += ned_dpp<0x4E>(v);
-  v += ned_dpp<0x141>(v);
-  return v;
-}
-
-// LDL^T of the equilibrated band matrix S K S (S = |diag K|^-1/2), rows
-// entering a 64-row LDS window as earlier ones leave; the right-hand sides
-// are eliminated alongside; then x = S (L^-T D^-1 z).  The steps run in
-// groups of 4: one wait at the start of a group covers the loads issued a
-// group earlier (the entering rows, or the panels of the backward sweep) and
-// the stores of the group before; the group's own panels and solution rows
-// stay in LDS until the next group's start.  No step waits for memory.
-__global__ __launch_bounds__(kNThreads) void ned_factor_kernel(NedArgs a) {
-  const int cell = blockIdx.x, tid = threadIdx.x;
-  const int n = a.n, p = a.p;
-  __shared__ double W[kNW][kNW + 1];
-  __shared__ double Y[kNW][kNR];
-  __shared__ double srow[kNW];
-  __shared__ double lcol[kNW];
-  __shared__ double ring[32][kRowLen];  // entering rows: the first fill, then 4 per group
-  __shared__ int oring[32][kNLow];
-  __shared__ double pbuf[4][kNW];       // panels of the group (d at 0)
-  __shared__ double zbuf[4][kZLen];     // z and s of the group
-  __shared__ int ordbuf[4];
-  __shared__ double psib[4][kNR];       // backward: Psi rows of the group
-  __shared__ int pairs[kMaxPairs];
-  __shared__ int bad;
-  const double* rows = a.rows + (size_t)cell * n * kRowLen;
-  double* pan = a.pan + (size_t)cell * n * kNW;
-  double* zs = a.zs + (size_t)cell * n * kZLen;
-  for (int q = tid; q < a.npairs; q += kNThreads) pairs[q] = a.pairs[q];
-  if (tid == 0) bad = MSFEC_CELL_OK;
-
-  // row i staged at ring[u] into slot i mod 64: thread o <= p writes column
-  // i - o, scaled by s_i s_col
-  auto insert = [&](int i, int u) {
-    const int si = i & (kNW - 1);
-    const double dg = fabs(ring[u][0]);
-    const double s_i = dg > 0.0 ? 1.0 / sqrt(dg) : 1.0;
-    if (tid <= p && i - tid >= 0) {
-      double v = 0.0;
-#pragma unroll
-      for (int t = 0; t < kNLow; ++t)
-        if (oring[u][t] == tid) v = ring[u][t];
-      const int col = i - tid;
-      const double sc = tid == 0 ? s_i : srow[col & (kNW - 1)];
-      W[si][col & (kNW - 1)] = v * s_i * sc;
-    }
-    if (tid >= kNW && tid < kNW + kNR) Y[si][tid - kNW] = ring[u][kNLow + tid - kNW] * s_i;
-    // slot si is read by no other thread here (its columns are i - o, o >= 1)
-    if (tid == 0) srow[si] = s_i;
-  };
-  auto load_row = [&](int i, double& v, int& o) {
-    if (i < n) {
-      if (tid < kRowLen) v = rows[(size_t)i * kRowLen + tid];
-      else if (tid < kRowLen + kNLow) o = a.loff[(size_t)i * kNLow + tid - kRowLen];
-    }
-  };
-  auto stage_row = [&](int u, double v, int o) {
-    if (tid < kRowLen) ring[u][tid] = v;
-    else if (tid < kRowLen + kNLow) oring[u][tid - kRowLen] = o;
-  };
-
-  // first fill: rows 0..63, 32 at a time
-  for (int h = 0; h < 2; ++h) {
-    double v[32];
-    int o[32];
-#pragma unroll
-    for (int u = 0; u < 32; ++u) {
-      v[u] = 0.0;
-      o[u] = -1;
-      load_row(32 * h + u, v[u], o[u]);
-    }
-#pragma unroll
-    for (int u = 0; u < 32; ++u) stage_row(u, v[u], o[u]);
-    __syncthreads();
-    for (int u = 0; u < 32 && 32 * h + u < n; ++u) {
-      insert(32 * h + u, u);
-      __syncthreads();
-    }
-  }
-
-  // forward elimination in groups of 4 steps
-  // Step k: the row entering after step k - 1 (row k + 63, staged at ring
-  // slot rslot(k - 1)) is inserted while pivot k is read (they touch
-  // different slots since p <= 62), then the rank-1 update.
-  auto rslot = [](int j) { return ((j >> 2) & 1) * 4 + (j & 3); };
-  auto fstep = [&](int k, int u) {
-    const int sk = k & (kNW - 1);
-    if (k >= 1 && k - 1 + kNW < n) insert(k - 1 + kNW, rslot(k - 1));
-    const double dk = W[sk][sk];
-    if (tid >= 1 && tid <= p) {
-      const int i = k + tid;
-      const double l = (i < n) ? W[i & (kNW - 1)][sk] / dk : 0.0;
-      lcol[tid] = l;
-      pbuf[u][tid] = l;
-    }
-    if (tid == 0) {
-      pbuf[u][0] = dk;
-      if (!(dk != 0.0)) bad = MSFEC_CELL_NOT_SPD;
-      else if (!isfinite(dk) && bad == MSFEC_CELL_OK) bad = MSFEC_CELL_NONFINITE;
-    }
-    if (tid >= kNW && tid < kNW + kNR) zbuf[u][tid - kNW] = Y[sk][tid - kNW];
-    if (tid == kNW + kNR) zbuf[u][kNR] = srow[sk];
-    if (tid == kNW + kNR + 1) zbuf[u][kNR + 1] = 1.0 / dk;
-    __syncthreads();
-    for (int q = tid; q < a.npairs; q += kNThreads) {
-      const int pr = pairs[q], io = pr >> 8, jo = pr & 255;
-      if (k + io < n)
-        W[(k + io) & (kNW - 1)][(k + jo) & (kNW - 1)] -= lcol[io] * W[(k + jo) & (kNW - 1)][sk];
-    }
-    for (int q = tid; q < p * kNR; q += kNThreads) {
-      const int io = 1 + q / kNR, bb = q - (io - 1) * kNR;
-      if (k + io < n) Y[(k + io) & (kNW - 1)][bb] -= lcol[io] * Y[sk][bb];
-    }
-    __syncthreads();
-  };
-  // panels and z of steps kg .. kg+3 from LDS to HBM
-  auto fflush = [&](int kg) {
-    for (int t = tid; t < 4 * kNW; t += kNThreads) {
-      const int u = t / kNW, o = t - u * kNW, k = kg + u;
-      if (k < n && o <= p) pan[(size_t)k * kNW + o] = pbuf[u][o];
-    }
-    if (tid < 4 * kZLen) {
-      const int u = tid / kZLen, c = tid - u * kZLen, k = kg + u;
-      if (k < n) zs[(size_t)k * kZLen + c] = zbuf[u][c];
-    }
-  };
-  auto fgroup = [&](int kg, const NedRowSet& cur, NedRowSet& nxt) {
-    ned_vm_wait();
-    if (kg > 0) fflush(kg - 4);
-#pragma unroll
-    for (int u = 0; u < 4; ++u) stage_row(rslot(kg + u), cur.v[u], cur.o[u]);
-#pragma unroll
-    for (int u = 0; u < 4; ++u) load_row(kg + kNW + 4 + u, nxt.v[u], nxt.o[u]);
-    __syncthreads();
-#pragma unroll
-    for (int u = 0; u < 4; ++u)
-      if (kg + u < n) fstep(kg + u, u);
-  };
-  NedRowSet ra, rb;
-#pragma unroll
-  for (int u = 0; u < 4; ++u) {
-    ra.v[u] = rb.v[u] = 0.0;
-    ra.o[u] = rb.o[u] = -1;
-    load_row(kNW + u, ra.v[u], ra.o[u]);
-  }
-  int klast = 0;
-  for (int k0 = 0; k0 < n; k0 += 8) {
-    fgroup(k0, ra, rb);
-    klast = k0;
-    if (k0 + 4 >= n) break;
-    fgroup(k0 + 4, rb, ra);
-    klast = k0 + 4;
-  }
-  __syncthreads();
-  fflush(klast);
-
-  // backward sweep on wave 0: x'_k = z_k / d_k - sum_o l_{k+o,k} x'_{k+o};
-  // lane (bb = lane >> 3, c = lane & 7) sums the offsets o = 1 + c + 8t of
-  // basis bb, the 8 lanes of a basis add up by DPP; x' ring in Y.  Groups of
-  // 4 steps: the panels (lane o holds l_{k+o,k}) and z, s, 1/d of the next
-  // group load while the current one runs.
-  // the panels and z written above, visible to every thread of the block
-  ned_vm_wait();
-  __threadfence_block();
-  __syncthreads();
-  if (tid < kNW) {
-    for (int t = tid; t < kNW * kNR; t += kNW) Y[t / kNR][t % kNR] = 0.0;
-    const int bb = tid >> 3, c = tid & 7;
-    auto bload = [&](int k, NedBackSet& f, int u) {
-      if (k < 0) return;
-      f.pv[u] = pan[(size_t)k * kNW + tid];
-      f.z[u] = zs[(size_t)k * kZLen + bb];
-      f.s[u] = zs[(size_t)k * kZLen + kNR];
-      f.di[u] = zs[(size_t)k * kZLen + kNR + 1];
-      f.ord[u] = a.order[k];
-    };
-    auto bstep = [&](int k, int u, const NedBackSet& f) {
-      double part = 0.0;
-#pragma unroll
-      for (int t = 0; t < 8; ++t) {
-        const int o = 1 + c + 8 * t;
-        if (o <= p && k + o < n) part = fma(pbuf[u][o], Y[(k + o) & (kNW - 1)][bb], part);
-      }
-      part = ned_sum8(part);
-      if (c == 0) {
-        const double x = f.z[u] * f.di[u] - part;
-        Y[k & (kNW - 1)][bb] = x;
-        psib[u][bb] = x * f.s[u];
-      }
-      ned_wave_sync();
-    };
-    // Psi of steps kg, kg-1, kg-2, kg-3 (psib[u], u = kg - k) to HBM
-    auto bflush = [&](int kg) {
-      if (tid < 4 * kNR) {
-        const int u = tid / kNR, cc = tid - u * kNR;
-        if (kg - u >= 0) a.psi[((size_t)cell * kNR + cc) * a.N + ordbuf[u]] = psib[u][cc];
-      }
-    };
-    auto bgroup = [&](int kg, const NedBackSet& cur, NedBackSet& nxt, int kprev) {
-      ned_vm_wait();
-      if (kprev >= 0) bflush(kprev);
-      ned_wave_sync();
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        pbuf[u][tid] = cur.pv[u];
-        if (tid == 0) ordbuf[u] = cur.ord[u];
-      }
-#pragma unroll
-      for (int u = 0; u < 4; ++u) bload(kg - 4 - u, nxt, u);
-      ned_wave_sync();
-#pragma unroll
-      for (int u = 0; u < 4; ++u)
-        if (kg - u >= 0) bstep(kg - u, u, cur);
-    };
-    NedBackSet ba, bc;
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      ba.pv[u] = ba.z[u] = ba.s[u] = ba.di[u] = 0.0;
-      bc.pv[u] = bc.z[u] = bc.s[u] = bc.di[u] = 0.0;
-      ba.ord[u] = bc.ord[u] = 0;
-      bload(n - 1 - u, ba, u);
-    }
-    ned_wave_sync();
-    int kprev = -1;
-    for (int k0 = n - 1; k0 >= 0; k0 -= 8) {
-      bgroup(k0, ba, bc, kprev);
-      kprev = k0;
-      if (k0 - 4 < 0) break;
-      bgroup(k0 - 4, bc, ba, kprev);
-      kprev = k0 - 4;
-    }
-    ned_wave_sync();
-    bflush(kprev);
-    if (tid == 0) a.status[cell] = bad;
+// Interior of Psi: the solution of the equilibrated system times the row
+// scale, to the fine dof of each band row.
+__global__ __launch_bounds__(kNThreads) void ned_psi_kernel(NedArgs a) {
+  const int cell = blockIdx.x;
+  const double* x = a.X + (size_t)cell * a.n * kNR;
+  const double* sc = a.scale + (size_t)cell * a.n;
+  double* psi = a.psi + (size_t)cell * kNR * a.N;
+  for (int i = threadIdx.x; i < a.n * kNR; i += blockDim.x) {
+    const int r = i >> 3, bb = i & (kNR - 1);
+    psi[(size_t)bb * a.N + a.order[r]] = sc[r] * x[i];
   }
 }
 
@@ -637,7 +400,7 @@ msfec_status ned_fail(msfec_status st, const std::string& msg) {
 // band order, its inverse, lower offsets and the pair list of one refinement level
 struct NedConf {
   int l = 0, m = 0, ns = 0, nh = 0, nv = 0, N = 0, n = 0, p = 0;
-  std::vector<int> order, pos, loff, pairs;
+  std::vector<int> order, pos, loff;
 };
 
 bool ned_conf(int l, NedConf& c, std::string& why) {
@@ -685,13 +448,11 @@ bool ned_conf(int l, NedConf& c, std::string& why) {
     for (size_t t = 0; t < offs.size(); ++t) c.loff[(size_t)r * kNLow + t] = offs[t];
     c.p = std::max(c.p, offs.back());
   }
-  if (c.p > kNW - 2) {
-    why = "half-bandwidth " + std::to_string(c.p) + " exceeds 62 (l <= 4)";
+  if (c.p > kMaxBand || c.n > kMaxRows) {
+    why = "half-bandwidth " + std::to_string(c.p) + " exceeds " + std::to_string(kMaxBand) +
+          " (l <= 4)";
     return false;
   }
-  c.pairs.clear();
-  for (int io = 1; io <= c.p; ++io)
-    for (int jo = 1; jo <= io; ++jo) c.pairs.push_back((io << 8) | jo);
   return true;
 }
 
@@ -707,8 +468,10 @@ struct msfec_ned_batch {
   double* d_r = nullptr;
   double* d_f = nullptr;
   double* d_rows = nullptr;
-  double* d_pan = nullptr;
-  double* d_zs = nullptr;
+  double* d_pan = nullptr;    // band factor panels
+  double* d_zst = nullptr;    // forward-eliminated right-hand sides
+  double* d_X = nullptr;      // solution of the equilibrated system
+  double* d_scale = nullptr;  // row scales
   double* d_psi = nullptr;
   double* d_M = nullptr;
   double* d_b = nullptr;
@@ -716,7 +479,6 @@ struct msfec_ned_batch {
   int* d_order = nullptr;
   int* d_pos = nullptr;
   int* d_loff = nullptr;
-  int* d_pairs = nullptr;
   bool coef_set = false, computed = false;
   hipStream_t stream = nullptr;
   std::vector<std::array<hipEvent_t, 4>> ev_sets;
@@ -732,9 +494,9 @@ msfec_status ned_alloc(T** p, size_t count) {
 }
 
 void ned_free(msfec_ned_batch* h) {
-  void* ptrs[] = {h->d_verts, h->d_a,   h->d_binv,   h->d_r,     h->d_f,   h->d_rows,
-                  h->d_pan,   h->d_zs,  h->d_psi,    h->d_M,     h->d_b,   h->d_status,
-                  h->d_order, h->d_pos, h->d_loff,   h->d_pairs};
+  void* ptrs[] = {h->d_verts, h->d_a,   h->d_binv, h->d_r,   h->d_f,     h->d_rows,
+                  h->d_pan,   h->d_zst, h->d_X,    h->d_scale, h->d_psi, h->d_M,
+                  h->d_b,     h->d_status, h->d_order, h->d_pos, h->d_loff};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (auto& set : h->ev_sets)
@@ -766,20 +528,18 @@ NedArgs ned_args(const msfec_ned_batch* h) {
   a.N = c.N;
   a.n = c.n;
   a.p = c.p;
-  a.npairs = (int)c.pairs.size();
   a.full_rhs = h->prm.full_rhs ? 1 : 0;
   a.order = h->d_order;
   a.pos = h->d_pos;
   a.loff = h->d_loff;
-  a.pairs = h->d_pairs;
   a.verts = h->d_verts;
   a.a = h->d_a;
   a.binv = h->d_binv;
   a.r = h->d_r;
   a.f = h->d_f;
   a.rows = h->d_rows;
-  a.pan = h->d_pan;
-  a.zs = h->d_zs;
+  a.scale = h->d_scale;
+  a.X = h->d_X;
   a.psi = h->d_psi;
   a.M = h->d_M;
   a.b = h->d_b;
@@ -842,8 +602,10 @@ msfec_status msfec_ned_create(const msfec_ned_params* params, int n_cells, const
   NED_TRY(ned_alloc(&h->d_r, nc * nf));
   NED_TRY(ned_alloc(&h->d_f, nc * nf * 2));
   NED_TRY(ned_alloc(&h->d_rows, nc * c.n * kRowLen));
-  NED_TRY(ned_alloc(&h->d_pan, nc * c.n * kNW));
-  NED_TRY(ned_alloc(&h->d_zs, nc * c.n * kZLen));
+  NED_TRY(ned_alloc(&h->d_pan, nc * msfec_detail::band_ldl_panel_doubles(c.n)));
+  NED_TRY(ned_alloc(&h->d_zst, nc * msfec_detail::band_ldl_z_doubles(c.n)));
+  NED_TRY(ned_alloc(&h->d_X, nc * c.n * kNR));
+  NED_TRY(ned_alloc(&h->d_scale, nc * c.n));
   NED_TRY(ned_alloc(&h->d_psi, nc * kNR * c.N));
   NED_TRY(ned_alloc(&h->d_M, nc * 64));
   NED_TRY(ned_alloc(&h->d_b, nc * kNR));
@@ -851,7 +613,6 @@ msfec_status msfec_ned_create(const msfec_ned_params* params, int n_cells, const
   NED_TRY(ned_alloc(&h->d_order, c.order.size()));
   NED_TRY(ned_alloc(&h->d_pos, c.pos.size()));
   NED_TRY(ned_alloc(&h->d_loff, c.loff.size()));
-  NED_TRY(ned_alloc(&h->d_pairs, c.pairs.size()));
 #undef NED_TRY
   auto up = [&](void* dst, const void* src, size_t bytes) -> msfec_status {
     NED_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
@@ -862,8 +623,6 @@ msfec_status msfec_ned_create(const msfec_ned_params* params, int n_cells, const
   if (s2 == MSFEC_OK) s2 = up(h->d_order, c.order.data(), c.order.size() * sizeof(int));
   if (s2 == MSFEC_OK) s2 = up(h->d_pos, c.pos.data(), c.pos.size() * sizeof(int));
   if (s2 == MSFEC_OK) s2 = up(h->d_loff, c.loff.data(), c.loff.size() * sizeof(int));
-  if (s2 == MSFEC_OK && !c.pairs.empty())
-    s2 = up(h->d_pairs, c.pairs.data(), c.pairs.size() * sizeof(int));
   if (s2 != MSFEC_OK) {
     ned_free(h);
     delete h;
@@ -909,9 +668,11 @@ msfec_status msfec_ned_compute(msfec_ned_handle h, void* stream) {
   hipLaunchKernelGGL(ned_rows_kernel, dim3(h->n_cells), dim3(kNThreads), 0, s, a);
   NED_HIP(hipGetLastError());
   if (ev) NED_HIP(hipEventRecord((*ev)[1], s));
-  hipLaunchKernelGGL(ned_factor_kernel, dim3(h->n_cells), dim3(kNThreads), 0, s, a);
-  NED_HIP(hipGetLastError());
+  NED_HIP(msfec_detail::band_ldl_solve(h->n_cells, h->conf.n, h->d_rows, h->d_loff, h->d_pan,
+                                       h->d_zst, h->d_X, h->d_status, s));
   if (ev) NED_HIP(hipEventRecord((*ev)[2], s));
+  hipLaunchKernelGGL(ned_psi_kernel, dim3(h->n_cells), dim3(kNThreads), 0, s, a);
+  NED_HIP(hipGetLastError());
   hipLaunchKernelGGL(ned_element_kernel, dim3(h->n_cells), dim3(kNThreads), elem_lds, s, a);
   NED_HIP(hipGetLastError());
   if (ev) NED_HIP(hipEventRecord((*ev)[3], s));

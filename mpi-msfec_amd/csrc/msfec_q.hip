@@ -311,6 +311,9 @@ struct BandArgs {
   int n, ni, N, S, nn, nfc;
   int nq;  // super-steps of 4 pivots (MFMA variant)
   int off_int[kMaxLow];
+  // LDL^T rows (symmetric indefinite band): [n][16] row - column offsets of
+  // the lower entries of each row, -1 where unused
+  const int* loff;
   // general path tables (q_assemble_kernel)
   const double* Slo;      // [cell][nn][nlow]
   const double* Rhs;      // [cell][n][8]
@@ -360,6 +363,7 @@ struct BandSmem {
   double mv[8];
   double stg_low[8][16];
   double stg_rhs[8][8];
+  int stg_off[8][16];  // LDL^T rows: offsets of the staged lower entries
   // phase-dependent pool:
   //   forward : RHS partials [64][9]
   //   backward: w of the group [8][8] | L columns of the group [8][72] | x window [64][9]
@@ -782,10 +786,14 @@ __global__ __launch_bounds__(256) void q_rows_kernel(BandArgs a) {
 // Rows jbase..jbase+7 of the row table are 192 contiguous doubles: every
 // lane loads 3 of them into registers (issued one staging group ahead) and
 // later stores them to the LDS staging area.
+// LDL^T rows also carry the offsets of their lower entries (8 rows x 16 ints,
+// two per lane).
 struct RowPrefetch {
   double v[3];
+  long long o;
 };
 
+template <bool LDL = false>
 __device__ __forceinline__ void rows_load(const BandArgs& a, int cell, int jbase, RowPrefetch& pf) {
   const int lane = threadIdx.x;
   const double* src = a.Rows + ((size_t)cell * a.n + jbase) * kRowStride;
@@ -795,8 +803,13 @@ __device__ __forceinline__ void rows_load(const BandArgs& a, int cell, int jbase
     const int idx = lane + 64 * u;
     pf.v[u] = (idx < lim) ? src[idx] : 0.0;
   }
+  if (LDL) {
+    const long long* os = reinterpret_cast<const long long*>(a.loff + (size_t)jbase * 16);
+    pf.o = (2 * lane < (a.n - jbase) * 16) ? os[lane] : -1LL;  // -1: two unused (-1) entries
+  }
 }
 
+template <bool LDL = false>
 __device__ __forceinline__ void rows_store(const RowPrefetch& pf, BandSmem& sh) {
   const int lane = threadIdx.x;
 #pragma unroll
@@ -806,6 +819,7 @@ __device__ __forceinline__ void rows_store(const RowPrefetch& pf, BandSmem& sh) 
     if (t < 16) sh.stg_low[r][t] = pf.v[u];
     else sh.stg_rhs[r][t - 16] = pf.v[u];
   }
+  if (LDL) reinterpret_cast<long long*>(&sh.stg_off[0][0])[lane] = pf.o;
   wave_sync();
 }
 
@@ -1703,20 +1717,22 @@ __device__ __forceinline__ double bk_scatter_load(const BkAddr& ad, int rb, Band
   return sh.stg_low[rb + ad.lh][ad.ll];
 }
 
-template <typename R, int DIM, int T, int G>
+template <typename R, int DIM, int T, int G, bool LDL>
 __device__ __forceinline__ void bk_scatter_store(const BandArgs& a, const BkAddr& ad, int jbase,
-                                                 int my_off, double v, BandSmem& sh,
+                                                 int rb, int my_off, double v, BandSmem& sh,
                                                  BkSmem<R>& ms) {
   using D = Dirs<DIM>;
+  constexpr int nlow = LDL ? 16 : D::NLOW;
   constexpr int s0 = 16 * T + 4 * G;
   // lane-dependent addresses from the opaque lane index: recomputed here,
   // not hoisted out of the loop as one register per super-step
   const int lane = bk_lane();
   const int t = lane >> 4, e = lane & 15;
+  if (LDL) my_off = sh.stg_off[rb + t][e];  // the offset of this row's entry e
   const bool live = jbase + t < a.n;
   // zeros stay unwritten: on a grid with one interior node per direction
   // some offsets coincide (or are 0), and their couplings are all zero
-  const bool w1 = live && e < D::NLOW && v != 0.0;
+  const bool w1 = live && e < nlow && v != 0.0;
   const bool w2 = w1 && e > 0 && my_off <= t;
   const bool w3 = !live && e == 0;  // past the end: isolated unit diagonal
   R* dummy = reinterpret_cast<R*>(SH_PART(sh)) + lane;
@@ -1725,7 +1741,7 @@ __device__ __forceinline__ void bk_scatter_store(const BandArgs& a, const BkAddr
                     : t * 72 + pad8(s0) + t;
   R* p1 = (w1 || w3) ? &ms.nrow[i1] : dummy;
   *p1 = w1 ? (R)v : (R)1;
-  R* p2 = w2 ? &ms.nrow[(t - my_off) * 72 + pad8(s0) + t] : dummy + 64;
+  R* p2 = w2 ? &ms.nrow[max(t - my_off, 0) * 72 + pad8(s0) + t] : dummy + 64;
   *p2 = (R)v;
 }
 
@@ -1745,13 +1761,13 @@ __device__ __forceinline__ void bk_place(const BandArgs& a, const BkAddr& ad, in
   Yt[T][G] = (ad.ll < 8 && jbase + ad.lh < a.n) ? (R)sh.stg_rhs[rb + ad.lh][ad.ll & 7] : (R)0;
 }
 
-template <typename R, int DIM, int T, int G>
+template <typename R, int DIM, int T, int G, bool LDL>
 __device__ __forceinline__ void bk_insert(const BandArgs& a, const BkAddr& ad, int jbase, int rb,
                                           int my_off, typename Mf<R>::v4 (&Tl)[10],
                                           typename Mf<R>::v4 (&Yt)[4], BandSmem& sh,
                                           BkSmem<R>& ms) {
   const double v = bk_scatter_load<R>(ad, rb, sh, ms);
-  bk_scatter_store<R, DIM, T, G>(a, ad, jbase, my_off, v, sh, ms);
+  bk_scatter_store<R, DIM, T, G, LDL>(a, ad, jbase, rb, my_off, v, sh, ms);
   wave_sync();
   bk_place<R, DIM, T, G>(a, ad, jbase, rb, Tl, Yt, sh, ms);
   wave_sync();
@@ -1801,6 +1817,22 @@ __device__ __forceinline__ void bk_check(R d, BkMon<R>& mon) {
   mon.dsum += d;
 }
 
+// LDL^T pivots of an indefinite matrix may have either sign: a zero pivot
+// (singular leading block) is what marks the cell.
+template <typename R>
+__device__ __forceinline__ void bk_check_ldl(R d, BkMon<R>& mon) {
+  mon.dmin = fmin(mon.dmin, fabs(d));
+  mon.dsum += d;
+}
+
+// 1/x: the hardware estimate refined by two Newton steps.
+__device__ __forceinline__ double bk_rcp(double x) {
+  double y = __builtin_amdgcn_rcp(x);
+  y = fma(y, fma(-x, y, 1.0), y);
+  y = fma(y, fma(-x, y, 1.0), y);
+  return y;
+}
+
 // Order of the 15 updates of a super-step (tile row T, group G): first the
 // tiles and the right-hand-side tile that the insertion of the next rows
 // overwrites (row and column T), then, at the last group of the tile row,
@@ -1811,7 +1843,7 @@ __device__ constexpr int bk_prio(int P, int Q) {
   return (P == T || Q == T) ? 0 : (G == 3 && (P == ((T + 1) & 3) || Q == ((T + 1) & 3))) ? 1 : 2;
 }
 
-template <typename R, int DIM, int T, int G>
+template <typename R, int DIM, int T, int G, bool LDL>
 __device__ __forceinline__ void bk_superstep(const BandArgs& a, const BkAddr& ad, int cell, int j0,
                                              typename Mf<R>::v4 (&Tl)[10],
                                              typename Mf<R>::v4 (&Yt)[4],
@@ -1824,39 +1856,74 @@ __device__ __forceinline__ void bk_superstep(const BandArgs& a, const BkAddr& ad
   ms.yp[ad.lane] = Yt[T][G];
   wave_sync();
   const R* pv = ms.piv;
-  // C C^T = pivot block (every lane), read from the lanes of the diagonal
-  // tile that hold it; ci_t = 1 / C[t][t]
+  // the pivot block (every lane), read from the lanes of the diagonal tile
+  // that hold it
   const R pk = Tl[MF_TILE(T, T)][G];
   auto pe = [&](int t, int u) { return bk_readlane(pk, M::lane_of(t, 4 * G + u)); };
-  const R a00 = pe(0, 0);
-  bk_check(a00, bad);
-  const R ci0 = bk_rsqrt(a00);
-  const R c10 = pe(1, 0) * ci0, c20 = pe(2, 0) * ci0;
-  const R c30 = pe(3, 0) * ci0;
-  const R d1 = pe(1, 1) - c10 * c10;
-  bk_check(d1, bad);
-  const R ci1 = bk_rsqrt(d1);
-  const R c21 = (pe(2, 1) - c20 * c10) * ci1;
-  const R c31 = (pe(3, 1) - c30 * c10) * ci1;
-  const R d2 = pe(2, 2) - c20 * c20 - c21 * c21;
-  bk_check(d2, bad);
-  const R ci2 = bk_rsqrt(d2);
-  const R c32 = (pe(3, 2) - c30 * c20 - c31 * c21) * ci2;
-  const R d3 = pe(3, 3) - c30 * c30 - c31 * c31 - c32 * c32;
-  bk_check(d3, bad);
-  const R ci3 = bk_rsqrt(d3);
-  // row lh of C^-1: x C = e_lh solved from the last column backwards
-  const R x3 = ek[3] * ci3;
-  const R x2 = (ek[2] - x3 * c32) * ci2;
-  const R x1 = (ek[1] - x2 * c21 - x3 * c31) * ci1;
-  const R x0 = (ek[0] - x1 * c10 - x2 * c20 - x3 * c30) * ci0;
-  // panel in operand layout: lane (row slot 16P + cs, pivot lh)
+  R x0, x1, x2, x3;  // row lh of the inverse of the block's (unit) lower factor
+  R xd, sc;          // panel diagonal entry of pivot lh, scale of the left operand
+  if constexpr (!LDL) {
+    // C C^T = pivot block; ci_t = 1 / C[t][t]
+    const R a00 = pe(0, 0);
+    bk_check(a00, bad);
+    const R ci0 = bk_rsqrt(a00);
+    const R c10 = pe(1, 0) * ci0, c20 = pe(2, 0) * ci0;
+    const R c30 = pe(3, 0) * ci0;
+    const R d1 = pe(1, 1) - c10 * c10;
+    bk_check(d1, bad);
+    const R ci1 = bk_rsqrt(d1);
+    const R c21 = (pe(2, 1) - c20 * c10) * ci1;
+    const R c31 = (pe(3, 1) - c30 * c10) * ci1;
+    const R d2 = pe(2, 2) - c20 * c20 - c21 * c21;
+    bk_check(d2, bad);
+    const R ci2 = bk_rsqrt(d2);
+    const R c32 = (pe(3, 2) - c30 * c20 - c31 * c21) * ci2;
+    const R d3 = pe(3, 3) - c30 * c30 - c31 * c31 - c32 * c32;
+    bk_check(d3, bad);
+    const R ci3 = bk_rsqrt(d3);
+    // row lh of C^-1: x C = e_lh solved from the last column backwards
+    x3 = ek[3] * ci3;
+    x2 = (ek[2] - x3 * c32) * ci2;
+    x1 = (ek[1] - x2 * c21 - x3 * c31) * ci1;
+    x0 = (ek[0] - x1 * c10 - x2 * c20 - x3 * c30) * ci0;
+    xd = (lh == 0) ? x0 : (lh == 1) ? x1 : (lh == 2) ? x2 : x3;  // C^-1[lh][lh]
+    sc = (R)1;
+  } else {
+    // L1 D1 L1^T = pivot block; u_ij = l_ij d_j, di_t = 1 / d_t
+    const R d0 = pe(0, 0);
+    bk_check_ldl(d0, bad);
+    const R di0 = bk_rcp(d0);
+    const R u10 = pe(1, 0), u20 = pe(2, 0), u30 = pe(3, 0);
+    const R l10 = u10 * di0, l20 = u20 * di0, l30 = u30 * di0;
+    const R d1 = pe(1, 1) - l10 * u10;
+    bk_check_ldl(d1, bad);
+    const R di1 = bk_rcp(d1);
+    const R u21 = pe(2, 1) - l20 * u10, u31 = pe(3, 1) - l30 * u10;
+    const R l21 = u21 * di1, l31 = u31 * di1;
+    const R d2 = pe(2, 2) - l20 * u20 - l21 * u21;
+    bk_check_ldl(d2, bad);
+    const R di2 = bk_rcp(d2);
+    const R u32 = pe(3, 2) - l30 * u20 - l31 * u21;
+    const R l32 = u32 * di2;
+    const R d3 = pe(3, 3) - l30 * u30 - l31 * u31 - l32 * u32;
+    bk_check_ldl(d3, bad);
+    const R di3 = bk_rcp(d3);
+    // row lh of L1^-1: x L1 = e_lh
+    x3 = ek[3];
+    x2 = ek[2] - x3 * l32;
+    x1 = ek[1] - x2 * l21 - x3 * l31;
+    x0 = ek[0] - x1 * l10 - x2 * l20 - x3 * l30;
+    xd = (lh == 0) ? di0 : (lh == 1) ? di1 : (lh == 2) ? di2 : di3;  // 1 / d_lh
+    sc = xd;
+  }
+  // panel in operand layout: lane (row slot 16P + cs, pivot lh).  Cholesky:
+  // V = A21 C^-T, update -V V^T.  LDL^T: V = A21 L1^-T, update -V D1^-1 V^T.
   R V[4], nV[4];
 #pragma unroll
   for (int P = 0; P < 4; ++P) {
     const int x = ad.vrd + 18 * P;
     V[P] = pv[0 * 72 + x] * x0 + pv[1 * 72 + x] * x1 + pv[2 * 72 + x] * x2 + pv[3 * 72 + x] * x3;
-    nV[P] = -V[P];
+    nV[P] = LDL ? -V[P] * sc : -V[P];
   }
   const R Z = ms.yp[0 * 16 + ll] * x0 + ms.yp[1 * 16 + ll] * x1 + ms.yp[2 * 16 + ll] * x2 +
               ms.yp[3 * 16 + ll] * x3;
@@ -1870,12 +1937,12 @@ __device__ __forceinline__ void bk_superstep(const BandArgs& a, const BkAddr& ad
       if (bk_prio<T, G>(P, P) == pr) Yt[P] = M::mma(nV[P], Z, Yt[P]);
     }
   }
-  Sacc = M::mma(Z, Z, Sacc);
+  if (!LDL) Sacc = M::mma(Z, Z, Sacc);
   // panel [k][o], o = offset from the first pivot.  Offsets 0..3 hold C
-  // (lower part) with 1 / C[t][t] on the diagonal.
+  // (lower part) with 1 / C[t][t] on the diagonal; for LDL^T, L1 D1 with
+  // 1 / d_t on the diagonal: the backward sweep is the same for both.
   const int qs = j0 >> 2;
   R* pst = reinterpret_cast<R*>(a.pan) + ((size_t)cell * a.nq + qs) * 256;
-  const R xd = (lh == 0) ? x0 : (lh == 1) ? x1 : (lh == 2) ? x2 : x3;  // C^-1[lh][lh]
   const int jr = bk_opaque(j0);  // = s0 mod 64
 #pragma unroll
   for (int P = 0; P < 4; ++P) {
@@ -1888,10 +1955,11 @@ __device__ __forceinline__ void bk_superstep(const BandArgs& a, const BkAddr& ad
 
 // Staging for the matrix-core kernels: two prefetch sets used alternately,
 // so that rows are loaded 16 rows (4 super-steps) before they are stored.
+template <bool LDL>
 __device__ __forceinline__ void bk_stage(const BandArgs& a, int cell, int jbase, BandSmem& sh,
                                          RowPrefetch& pf) {
-  rows_store(pf, sh);
-  rows_load(a, cell, jbase + 16, pf);
+  rows_store<LDL>(pf, sh);
+  rows_load<LDL>(a, cell, jbase + 16, pf);
 }
 
 // The 4 super-steps of tile row T (pivots j0 .. j0+15); the next 16 rows of
@@ -1899,7 +1967,7 @@ __device__ __forceinline__ void bk_stage(const BandArgs& a, int cell, int jbase,
 // >= n are isolated unit diagonals, their panels zero): with no branch
 // between them, the waits for the row prefetch leave the panel stores of the
 // last super-steps in flight instead of waiting for their completion.
-template <typename R, int DIM, int T>
+template <typename R, int DIM, int T, bool LDL>
 __device__ __forceinline__ void bk_tile_row(const BandArgs& a, const BkAddr& ad, int cell, int j0,
                                             int my_off,
                                             typename Mf<R>::v4 (&Tl)[10],
@@ -1907,31 +1975,31 @@ __device__ __forceinline__ void bk_tile_row(const BandArgs& a, const BkAddr& ad,
                                             typename Mf<R>::v4& Sacc, BandSmem& sh,
                                             BkSmem<R>& ms, RowPrefetch (&pf)[2], BkMon<R>& bad,
                                             const R (&ek)[4]) {
-  bk_stage(a, cell, j0 + kWin, sh, pf[0]);
-  bk_superstep<R, DIM, T, 0>(a, ad, cell, j0, Tl, Yt, Sacc, ms, bad, ek);
-  bk_insert<R, DIM, T, 0>(a, ad, j0 + kWin, 0, my_off, Tl, Yt, sh, ms);
-  bk_superstep<R, DIM, T, 1>(a, ad, cell, j0 + 4, Tl, Yt, Sacc, ms, bad, ek);
-  bk_insert<R, DIM, T, 1>(a, ad, j0 + 4 + kWin, 4, my_off, Tl, Yt, sh, ms);
-  bk_stage(a, cell, j0 + 8 + kWin, sh, pf[1]);
-  bk_superstep<R, DIM, T, 2>(a, ad, cell, j0 + 8, Tl, Yt, Sacc, ms, bad, ek);
-  bk_insert<R, DIM, T, 2>(a, ad, j0 + 8 + kWin, 0, my_off, Tl, Yt, sh, ms);
-  bk_superstep<R, DIM, T, 3>(a, ad, cell, j0 + 12, Tl, Yt, Sacc, ms, bad, ek);
-  bk_insert<R, DIM, T, 3>(a, ad, j0 + 12 + kWin, 4, my_off, Tl, Yt, sh, ms);
+  bk_stage<LDL>(a, cell, j0 + kWin, sh, pf[0]);
+  bk_superstep<R, DIM, T, 0, LDL>(a, ad, cell, j0, Tl, Yt, Sacc, ms, bad, ek);
+  bk_insert<R, DIM, T, 0, LDL>(a, ad, j0 + kWin, 0, my_off, Tl, Yt, sh, ms);
+  bk_superstep<R, DIM, T, 1, LDL>(a, ad, cell, j0 + 4, Tl, Yt, Sacc, ms, bad, ek);
+  bk_insert<R, DIM, T, 1, LDL>(a, ad, j0 + 4 + kWin, 4, my_off, Tl, Yt, sh, ms);
+  bk_stage<LDL>(a, cell, j0 + 8 + kWin, sh, pf[1]);
+  bk_superstep<R, DIM, T, 2, LDL>(a, ad, cell, j0 + 8, Tl, Yt, Sacc, ms, bad, ek);
+  bk_insert<R, DIM, T, 2, LDL>(a, ad, j0 + 8 + kWin, 0, my_off, Tl, Yt, sh, ms);
+  bk_superstep<R, DIM, T, 3, LDL>(a, ad, cell, j0 + 12, Tl, Yt, Sacc, ms, bad, ek);
+  bk_insert<R, DIM, T, 3, LDL>(a, ad, j0 + 12 + kWin, 4, my_off, Tl, Yt, sh, ms);
 }
 
 // Initial fill of tile row T with the rows 16T .. 16T+15.
-template <typename R, int DIM, int T>
+template <typename R, int DIM, int T, bool LDL>
 __device__ __forceinline__ void bk_fill_row(const BandArgs& a, const BkAddr& ad, int cell,
                                             int my_off,
                                             typename Mf<R>::v4 (&Tl)[10],
                                             typename Mf<R>::v4 (&Yt)[4], BandSmem& sh,
                                             BkSmem<R>& ms, RowPrefetch (&pf)[2]) {
-  bk_stage(a, cell, 16 * T, sh, pf[0]);
-  bk_insert<R, DIM, T, 0>(a, ad, 16 * T + 0, 0, my_off, Tl, Yt, sh, ms);
-  bk_insert<R, DIM, T, 1>(a, ad, 16 * T + 4, 4, my_off, Tl, Yt, sh, ms);
-  bk_stage(a, cell, 16 * T + 8, sh, pf[1]);
-  bk_insert<R, DIM, T, 2>(a, ad, 16 * T + 8, 0, my_off, Tl, Yt, sh, ms);
-  bk_insert<R, DIM, T, 3>(a, ad, 16 * T + 12, 4, my_off, Tl, Yt, sh, ms);
+  bk_stage<LDL>(a, cell, 16 * T, sh, pf[0]);
+  bk_insert<R, DIM, T, 0, LDL>(a, ad, 16 * T + 0, 0, my_off, Tl, Yt, sh, ms);
+  bk_insert<R, DIM, T, 1, LDL>(a, ad, 16 * T + 4, 4, my_off, Tl, Yt, sh, ms);
+  bk_stage<LDL>(a, cell, 16 * T + 8, sh, pf[1]);
+  bk_insert<R, DIM, T, 2, LDL>(a, ad, 16 * T + 8, 0, my_off, Tl, Yt, sh, ms);
+  bk_insert<R, DIM, T, 3, LDL>(a, ad, 16 * T + 12, 4, my_off, Tl, Yt, sh, ms);
 }
 
 // Wait for every outstanding vector memory operation (vmcnt(0); expcnt and
@@ -2178,7 +2246,10 @@ struct BkOcc {
   static constexpr int waves = sizeof(R) == 8 ? 2 : 3;
 };
 
-template <typename R, int DIM, int MODE>
+// LDL: the rows are those of a symmetric indefinite band matrix with the
+// offsets of their lower entries in a.loff (factored as L D L^T, 16 lower
+// entries per row); only with MODE BK_X0.
+template <typename R, int DIM, int MODE, bool LDL = false>
 __global__ __launch_bounds__(64, BkOcc<R>::waves) void q_blk_factor_kernel(BandArgs a) {
   typedef typename Mf<R>::v4 V4;
   const int cell = blockIdx.x;
@@ -2200,20 +2271,23 @@ __global__ __launch_bounds__(64, BkOcc<R>::waves) void q_blk_factor_kernel(BandA
 #pragma unroll
   for (int t = 0; t < 4; ++t) ek[t] = ((lane >> 4) == t) ? (R)1 : (R)0;
   RowPrefetch pf[2];
-  rows_load(a, cell, 0, pf[0]);
-  rows_load(a, cell, 8, pf[1]);
-  bk_fill_row<R, DIM, 0>(a, ad, cell, my_off, Tl, Yt, sh, ms, pf);
-  bk_fill_row<R, DIM, 1>(a, ad, cell, my_off, Tl, Yt, sh, ms, pf);
-  bk_fill_row<R, DIM, 2>(a, ad, cell, my_off, Tl, Yt, sh, ms, pf);
-  bk_fill_row<R, DIM, 3>(a, ad, cell, my_off, Tl, Yt, sh, ms, pf);
+  rows_load<LDL>(a, cell, 0, pf[0]);
+  rows_load<LDL>(a, cell, 8, pf[1]);
+  bk_fill_row<R, DIM, 0, LDL>(a, ad, cell, my_off, Tl, Yt, sh, ms, pf);
+  bk_fill_row<R, DIM, 1, LDL>(a, ad, cell, my_off, Tl, Yt, sh, ms, pf);
+  bk_fill_row<R, DIM, 2, LDL>(a, ad, cell, my_off, Tl, Yt, sh, ms, pf);
+  bk_fill_row<R, DIM, 3, LDL>(a, ad, cell, my_off, Tl, Yt, sh, ms, pf);
   BkMon<R> bad;
   bad.init();
   const int n = a.n;
   for (int j0 = 0; j0 < n; j0 += 64) {
-    bk_tile_row<R, DIM, 0>(a, ad, cell, j0, my_off, Tl, Yt, Sacc, sh, ms, pf, bad, ek);
-    if (j0 + 16 < n) bk_tile_row<R, DIM, 1>(a, ad, cell, j0 + 16, my_off, Tl, Yt, Sacc, sh, ms, pf, bad, ek);
-    if (j0 + 32 < n) bk_tile_row<R, DIM, 2>(a, ad, cell, j0 + 32, my_off, Tl, Yt, Sacc, sh, ms, pf, bad, ek);
-    if (j0 + 48 < n) bk_tile_row<R, DIM, 3>(a, ad, cell, j0 + 48, my_off, Tl, Yt, Sacc, sh, ms, pf, bad, ek);
+    bk_tile_row<R, DIM, 0, LDL>(a, ad, cell, j0, my_off, Tl, Yt, Sacc, sh, ms, pf, bad, ek);
+    if (j0 + 16 < n)
+      bk_tile_row<R, DIM, 1, LDL>(a, ad, cell, j0 + 16, my_off, Tl, Yt, Sacc, sh, ms, pf, bad, ek);
+    if (j0 + 32 < n)
+      bk_tile_row<R, DIM, 2, LDL>(a, ad, cell, j0 + 32, my_off, Tl, Yt, Sacc, sh, ms, pf, bad, ek);
+    if (j0 + 48 < n)
+      bk_tile_row<R, DIM, 3, LDL>(a, ad, cell, j0 + 48, my_off, Tl, Yt, Sacc, sh, ms, pf, bad, ek);
   }
   double sacc = (MODE == BK_FINAL64) ? bk_sacc<R>(Sacc, SH_PART(sh)) : 0.0;
   // panels of this wave must be visible to its own loads in the backward sweep
@@ -2225,7 +2299,7 @@ __global__ __launch_bounds__(64, BkOcc<R>::waves) void q_blk_factor_kernel(BandA
   for (int o = 32; o > 0; o >>= 1) worst = max(worst, __shfl_xor(worst, o));
   if (lane == 0) {
     a.status[cell] = worst;
-    if (MODE == BK_X0) a.conv[cell] = (worst != 0) ? 1 : 0;  // failed cells do not refine
+    if (MODE == BK_X0 && !LDL) a.conv[cell] = (worst != 0) ? 1 : 0;  // failed cells do not refine
   }
   if (MODE == BK_FINAL64) fast_finish<DIM>(a, cell, sacc, bint);
 }
@@ -3133,7 +3207,31 @@ __global__ __launch_bounds__(256) void q_cg_finish_kernel(BandArgs a, CgArgs g) 
 // entry points (msfec_ned.hip)
 namespace msfec_detail {
 thread_local std::string g_last_error;
+
+// Panel storage per cell of band_ldl_solve, in doubles.
+size_t band_ldl_panel_doubles(int n) { return (size_t)panel_quads(n) * 256; }
+size_t band_ldl_z_doubles(int n) { return (size_t)panel_quads(n) * 32; }
+
+// The matrix-core band LDL^T of the Q_Ned interior systems: rows
+// [cells][n][24] (16 lower entries, diagonal first, with their offsets in
+// loff [n][16]; 8 right-hand sides), the solutions to X [cells][n][8], the
+// cell status (MSFEC_CELL_NOT_SPD for a zero pivot) to status.
+hipError_t band_ldl_solve(int n_cells, int n, const double* rows, const int* loff, double* pan,
+                          double* zst, double* X, int* status, hipStream_t s) {
+  BandArgs a{};
+  a.n = n;
+  a.nq = panel_quads(n);
+  a.Rows = rows;
+  a.loff = loff;
+  a.pan = pan;
+  a.zst = zst;
+  a.X = X;
+  a.status = status;
+  hipLaunchKernelGGL((q_blk_factor_kernel<double, 2, BK_X0, true>), dim3(n_cells), dim3(64), 0, s,
+                     a);
+  return hipGetLastError();
 }
+}  // namespace msfec_detail
 
 namespace {
 
