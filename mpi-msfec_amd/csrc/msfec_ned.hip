@@ -22,9 +22,9 @@
 //                      factorisation of the Q path in its L D L^T form, 4
 //                      pivots per step, with the forward elimination of the 8
 //                      right-hand sides and the backward sweep.
-//   ned_psi_kernel     the interior of Psi: the row scale times the solution.
-//   ned_element_kernel one workgroup per cell: the 8 x 8 element matrix and
-//                      the load vector from Psi staged in LDS.
+//   ned_element_kernel one workgroup per cell: the interior of Psi (the row
+//                      scale times the solution), then the 8 x 8 element
+//                      matrix and the load vector from Psi staged in LDS.
 // Band order: node row j (nodes (i,j)), interleaved with the horizontal
 // edges of row j and the vertical edges between rows j-1 and j, every edge
 // after its interior end nodes.  With that order every leading principal
@@ -307,76 +307,93 @@ __global__ __launch_bounds__(kNThreads) void ned_rows_kernel(NedArgs a) {
       for (int bb = 0; bb < kNR; ++bb) a.psi[((size_t)cell * kNR + bb) * a.N + d] = ned_bval(a, d, bb);
 }
 
-// Interior of Psi: the solution of the equilibrated system times the row
-// scale, to the fine dof of each band row.
-__global__ __launch_bounds__(kNThreads) void ned_psi_kernel(NedArgs a) {
-  const int cell = blockIdx.x;
-  const double* x = a.X + (size_t)cell * a.n * kNR;
-  const double* sc = a.scale + (size_t)cell * a.n;
-  double* psi = a.psi + (size_t)cell * kNR * a.N;
-  for (int i = threadIdx.x; i < a.n * kNR; i += blockDim.x) {
-    const int r = i >> 3, bb = i & (kNR - 1);
-    psi[(size_t)bb * a.N + a.order[r]] = sc[r] * x[i];
-  }
-}
-
-// Element matrix (q_ned_basis.cc:841-939): M[kt][kc] = psi_kt^T A psi_kc on
-// the sigma block for vertex bases and the u block for edge bases; b[kt] =
-// (psi_kt u, f) for the edge bases.  Psi of the cell staged in LDS; 3 groups
-// of 72 threads (64 matrix entries + 8 loads) share the fine cells.
+// Interior of Psi and the element matrix (q_ned_basis.cc:841-939):
+// M[kt][kc] = psi_kt^T A psi_kc on the sigma block for vertex bases and the u
+// block for edge bases; b[kt] = (psi_kt u, f) for the edge bases.  The
+// interior of Psi (the row scale times the solution of the equilibrated
+// system) goes to HBM and, with the boundary values, to LDS.  Thread
+// (kt = tid & 7, group tid >> 3) sums row kt of the fine-cell contributions
+// over every 32nd fine cell; the groups add up across lanes and waves.
 __global__ __launch_bounds__(kNThreads) void ned_element_kernel(NedArgs a) {
-  extern __shared__ double sm[];
+  extern __shared__ double ps[];  // [8][N]
+  __shared__ double part[kNThreads / 64][72];
   const int cell = blockIdx.x, tid = threadIdx.x;
-  const int N = a.N, m = a.m, nf = m * m;
-  double* ps = sm;               // [8][N]
-  double* part = sm + kNR * N;   // [3][72]
-  for (int t = tid; t < kNR * N; t += kNThreads) ps[t] = a.psi[(size_t)cell * kNR * N + t];
+  const int N = a.N, m = a.m, nf = m * m, n = a.n;
+  double* pg = a.psi + (size_t)cell * kNR * N;
+  for (int t = tid; t < kNR * N; t += kNThreads)
+    if (a.pos[t % N] < 0) ps[t] = pg[t];  // boundary values (row kernel)
+  const double* x = a.X + (size_t)cell * n * kNR;
+  const double* sc = a.scale + (size_t)cell * n;
+  for (int i = tid; i < n * kNR; i += kNThreads) {
+    const int r = i >> 3, bb = i & (kNR - 1);
+    const int t = bb * N + a.order[r];
+    const double v = sc[r] * x[i];
+    ps[t] = v;
+    pg[t] = v;
+  }
   __syncthreads();
   const NedBox bx = ned_box(a, cell);
-  const int e = tid % 72, grp = tid / 72;
-  double acc = 0.0;
-  if (grp < 3) {
-    const int kt = e < 64 ? e >> 3 : 4 + ((e - 64) & 3);
-    const int kc = e & 7;
-    const int r0 = kt < 4 ? 0 : 4, c0 = kc < 4 ? 0 : 4;
-    const bool isb = e >= 64;
-    for (int fc = grp; fc < nf; fc += 3) {
-      const int i = fc % m, j = fc / m;
-      int d[8];
-      ned_cell_dofs(a, i, j, d);
-      const size_t c = (size_t)cell * nf + fc;
-      if (!isb) {
-        double K[8][8];
-        ned_local(bx.hx, bx.hy, a.a[c], a.binv[c], a.r[c], K);
+  const int kt = tid & 7, grp = tid >> 3;
+  const bool edge_t = kt >= 4;
+  const int r0 = edge_t ? 4 : 0;
+  double acc[kNR], bacc = 0.0;
 #pragma unroll
-        for (int p = 0; p < 4; ++p) {
-          double t = 0.0;
+  for (int k = 0; k < kNR; ++k) acc[k] = 0.0;
+  for (int fc = grp; fc < nf; fc += kNThreads / 8) {
+    const int i = fc % m, j = fc / m;
+    int d[8];
+    ned_cell_dofs(a, i, j, d);
+    const size_t c = (size_t)cell * nf + fc;
+    double K[8][8];
+    ned_local(bx.hx, bx.hy, a.a[c], a.binv[c], a.r[c], K);
+    double u[4];
 #pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            // K[r0 + p][c0 + q] without dynamic register indexing
-            const double kv = r0 == 0 ? (c0 == 0 ? K[p][q] : K[p][4 + q])
-                                      : (c0 == 0 ? K[4 + p][q] : K[4 + p][4 + q]);
-            t += kv * ps[kc * N + d[c0 + q]];
-          }
-          acc += ps[kt * N + d[r0 + p]] * t;
-        }
-      } else if (e < 68) {
-        const double fx = a.f[2 * c], fy = a.f[2 * c + 1];
-        const double lf[4] = {fy * bx.hx / 2, fy * bx.hx / 2, fx * bx.hy / 2, fx * bx.hy / 2};
+    for (int p = 0; p < 4; ++p) u[p] = ps[kt * N + d[r0 + p]];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) acc += ps[kt * N + d[4 + q]] * lf[q];
+    for (int c0 = 0; c0 < 8; c0 += 4) {
+      double w[4];  // row kt of psi^T K on the (r0, c0) block
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        double t = 0.0;
+#pragma unroll
+        for (int p = 0; p < 4; ++p) t += u[p] * (edge_t ? K[4 + p][c0 + q] : K[p][c0 + q]);
+        w[q] = t;
+      }
+#pragma unroll
+      for (int kc = c0; kc < c0 + 4; ++kc) {
+        double t = 0.0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) t += w[q] * ps[kc * N + d[c0 + q]];
+        acc[kc] += t;
       }
     }
-    part[grp * 72 + e] = acc;
+    if (edge_t) {
+      const double fx = a.f[2 * c], fy = a.f[2 * c + 1];
+      const double lf[4] = {fy * bx.hx / 2, fy * bx.hx / 2, fx * bx.hy / 2, fx * bx.hy / 2};
+#pragma unroll
+      for (int q = 0; q < 4; ++q) bacc += u[q] * lf[q];
+    }
+  }
+  // the 8 groups of a wave (lane bits 3..5), then the waves
+#pragma unroll
+  for (int k = 0; k < kNR; ++k)
+#pragma unroll
+    for (int o = 8; o < 64; o <<= 1) acc[k] += __shfl_xor(acc[k], o);
+#pragma unroll
+  for (int o = 8; o < 64; o <<= 1) bacc += __shfl_xor(bacc, o);
+  const int wave = tid >> 6, lane = tid & 63;
+  if (lane < 8) {
+#pragma unroll
+    for (int k = 0; k < kNR; ++k) part[wave][lane * 8 + k] = acc[k];
+    part[wave][64 + lane] = bacc;
   }
   __syncthreads();
   if (tid < 72) {
-    const double v = part[tid] + part[72 + tid] + part[144 + tid];
+    double v = 0.0;
+#pragma unroll
+    for (int w = 0; w < kNThreads / 64; ++w) v += part[w][tid];
     if (tid < 64) a.M[(size_t)cell * 64 + tid] = v;
-    else if (tid < 68) {
-      a.b[(size_t)cell * kNR + tid - 64] = 0.0;  // vertex bases: no right-hand side
-      a.b[(size_t)cell * kNR + 4 + tid - 64] = v;
-    }
+    else a.b[(size_t)cell * kNR + tid - 64] = (tid - 64 < 4) ? 0.0 : v;  // vertex bases: no load
   }
 }
 
@@ -663,7 +680,7 @@ msfec_status msfec_ned_compute(msfec_ned_handle h, void* stream) {
     ev = &h->ev_sets[h->ev_used++];
     NED_HIP(hipEventRecord((*ev)[0], s));
   }
-  const size_t elem_lds = ((size_t)kNR * h->conf.N + 3 * 72) * sizeof(double);
+  const size_t elem_lds = (size_t)kNR * h->conf.N * sizeof(double);
   if (elem_lds > 160 * 1024) return ned_fail(MSFEC_ERR_UNSUPPORTED, "cell too large for LDS staging");
   hipLaunchKernelGGL(ned_rows_kernel, dim3(h->n_cells), dim3(kNThreads), 0, s, a);
   NED_HIP(hipGetLastError());
@@ -671,8 +688,6 @@ msfec_status msfec_ned_compute(msfec_ned_handle h, void* stream) {
   NED_HIP(msfec_detail::band_ldl_solve(h->n_cells, h->conf.n, h->d_rows, h->d_loff, h->d_pan,
                                        h->d_zst, h->d_X, h->d_status, s));
   if (ev) NED_HIP(hipEventRecord((*ev)[2], s));
-  hipLaunchKernelGGL(ned_psi_kernel, dim3(h->n_cells), dim3(kNThreads), 0, s, a);
-  NED_HIP(hipGetLastError());
   hipLaunchKernelGGL(ned_element_kernel, dim3(h->n_cells), dim3(kNThreads), elem_lds, s, a);
   NED_HIP(hipGetLastError());
   if (ev) NED_HIP(hipEventRecord((*ev)[3], s));
