@@ -68,6 +68,8 @@ struct NedArgs {
   const int* order;  // [n] band row -> fine dof
   const int* pos;    // [N] fine dof -> band row, -1 on the boundary
   const int* loff;   // [n][kNLow] offsets row - column of the lower entries, -1 unused
+  const signed char* cslot;  // [n][4][8] per fine cell of the row and local dof: the
+                             // lower slot of the coupling, -1 boundary dof, -2 upper
   const double* verts;
   const double* a;
   const double* binv;
@@ -204,6 +206,8 @@ __device__ __forceinline__ double ned_bval(const NedArgs& a, int d, int b) {
 __global__ __launch_bounds__(kNThreads) void ned_rows_kernel(NedArgs a) {
   const int cell = blockIdx.x;
   __shared__ double ss[kMaxRows];  // row scales
+  __shared__ double srow[kNThreads][kNLow + 1];  // the lower entries of the thread's row
+  double* sl = srow[threadIdx.x];
   const NedBox bx = ned_box(a, cell);
   const int m = a.m, nf = m * m;
   const double* ca = a.a + (size_t)cell * nf;
@@ -232,12 +236,13 @@ __global__ __launch_bounds__(kNThreads) void ned_rows_kernel(NedArgs a) {
       ci[2] = ci[3] = i; cj[2] = cj[3] = j; lp[2] = lp[3] = 4;
       nc = 2;
     }
-    int lo[kNLow];
+    // slots of the couplings: byte 8q + k of the row's 32
+    const int4* cw = reinterpret_cast<const int4*>(a.cslot + (size_t)r * 32);
+    const int4 w0 = cw[0], w1 = cw[1];
+    const int ws[8] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w};
+    double rhs[kNR];
 #pragma unroll
-    for (int t = 0; t < kNLow; ++t) lo[t] = a.loff[(size_t)r * kNLow + t];
-    double low[kNLow], rhs[kNR];
-#pragma unroll
-    for (int t = 0; t < kNLow; ++t) low[t] = 0.0;
+    for (int t = 0; t < kNLow; ++t) sl[t] = 0.0;
 #pragma unroll
     for (int t = 0; t < kNR; ++t) rhs[t] = 0.0;
 #pragma unroll
@@ -261,15 +266,10 @@ __global__ __launch_bounds__(kNThreads) void ned_rows_kernel(NedArgs a) {
       }
 #pragma unroll
       for (int k = 0; k < 8; ++k) {
-        const int pc = a.pos[dofs[k]];
-        if (pc >= 0) {
-          // lower entries only: an upper coupling (o < 0) must not match the
-          // -1 that marks an unused slot
-          const int o = r - pc;
-#pragma unroll
-          for (int t = 0; t < kNLow; ++t)
-            if (o >= 0 && lo[t] == o) low[t] += kr[k];
-        } else {
+        const int t = (signed char)(ws[(8 * q + k) >> 2] >> (8 * ((8 * q + k) & 3)));
+        if (t >= 0) {
+          sl[t] += kr[k];  // lower coupling in slot t
+        } else if (t == -1) {
 #pragma unroll
           for (int bb = 0; bb < kNR; ++bb) rhs[bb] -= kr[k] * ned_bval(a, dofs[k], bb);
         }
@@ -280,10 +280,10 @@ __global__ __launch_bounds__(kNThreads) void ned_rows_kernel(NedArgs a) {
     }
     double* out = a.rows + ((size_t)cell * a.n + r) * kRowLen;
 #pragma unroll
-    for (int t = 0; t < kNLow; ++t) out[t] = low[t];
+    for (int t = 0; t < kNLow; ++t) out[t] = sl[t];
 #pragma unroll
     for (int t = 0; t < kNR; ++t) out[kNLow + t] = rhs[t];
-    const double dg = fabs(low[0]);
+    const double dg = fabs(sl[0]);
     ss[r] = dg > 0.0 ? 1.0 / sqrt(dg) : 1.0;
   }
   __syncthreads();
@@ -418,6 +418,7 @@ msfec_status ned_fail(msfec_status st, const std::string& msg) {
 struct NedConf {
   int l = 0, m = 0, ns = 0, nh = 0, nv = 0, N = 0, n = 0, p = 0;
   std::vector<int> order, pos, loff;
+  std::vector<signed char> cslot;
 };
 
 bool ned_conf(int l, NedConf& c, std::string& why) {
@@ -465,6 +466,50 @@ bool ned_conf(int l, NedConf& c, std::string& why) {
     for (size_t t = 0; t < offs.size(); ++t) c.loff[(size_t)r * kNLow + t] = offs[t];
     c.p = std::max(c.p, offs.back());
   }
+  // per band row, fine cell of the row (in the order of ned_rows_kernel) and
+  // local dof (ned_cell_dofs): the lower slot of the coupling, -1 for a
+  // boundary dof, -2 for an upper coupling
+  c.cslot.assign((size_t)c.n * 32, (signed char)-2);
+  for (int r = 0; r < c.n; ++r) {
+    int d = c.order[r], type, i, j;
+    if (d < c.ns) {
+      type = 0, j = d / (m + 1), i = d % (m + 1);
+    } else if (d < c.ns + c.nh) {
+      d -= c.ns;
+      type = 1, j = d / m, i = d % m;
+    } else {
+      d -= c.ns + c.nh;
+      type = 2, j = d / (m + 1), i = d % (m + 1);
+    }
+    int ci[4], cj[4], nc;
+    if (type == 0) {
+      nc = 4;
+      ci[0] = i - 1, cj[0] = j - 1, ci[1] = i, cj[1] = j - 1;
+      ci[2] = i - 1, cj[2] = j, ci[3] = i, cj[3] = j;
+    } else if (type == 1) {
+      nc = 2;
+      ci[0] = i, cj[0] = j - 1, ci[1] = i, cj[1] = j;
+    } else {
+      nc = 2;
+      ci[0] = i - 1, cj[0] = j, ci[1] = i, cj[1] = j;
+    }
+    for (int q = 0; q < nc; ++q) {
+      const int x = ci[q], y = cj[q];
+      const int dd[8] = {node(x, y), node(x + 1, y), node(x, y + 1), node(x + 1, y + 1),
+                         ved(x, y),  ved(x + 1, y),  hed(x, y),      hed(x, y + 1)};
+      for (int k = 0; k < 8; ++k) {
+        signed char sl = -1;
+        const int pc = c.pos[dd[k]];
+        if (pc >= 0) {
+          sl = -2;
+          if (pc <= r)  // a lower coupling (an upper one could match the -1 of an unused slot)
+            for (int t = 0; t < kNLow; ++t)
+              if (c.loff[(size_t)r * kNLow + t] == r - pc) sl = (signed char)t;
+        }
+        c.cslot[(size_t)r * 32 + 8 * q + k] = sl;
+      }
+    }
+  }
   if (c.p > kMaxBand || c.n > kMaxRows) {
     why = "half-bandwidth " + std::to_string(c.p) + " exceeds " + std::to_string(kMaxBand) +
           " (l <= 4)";
@@ -496,6 +541,7 @@ struct msfec_ned_batch {
   int* d_order = nullptr;
   int* d_pos = nullptr;
   int* d_loff = nullptr;
+  signed char* d_cslot = nullptr;
   bool coef_set = false, computed = false;
   hipStream_t stream = nullptr;
   std::vector<std::array<hipEvent_t, 4>> ev_sets;
@@ -513,7 +559,7 @@ msfec_status ned_alloc(T** p, size_t count) {
 void ned_free(msfec_ned_batch* h) {
   void* ptrs[] = {h->d_verts, h->d_a,   h->d_binv, h->d_r,   h->d_f,     h->d_rows,
                   h->d_pan,   h->d_zst, h->d_X,    h->d_scale, h->d_psi, h->d_M,
-                  h->d_b,     h->d_status, h->d_order, h->d_pos, h->d_loff};
+                  h->d_b,     h->d_status, h->d_order, h->d_pos, h->d_loff, h->d_cslot};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (auto& set : h->ev_sets)
@@ -549,6 +595,7 @@ NedArgs ned_args(const msfec_ned_batch* h) {
   a.order = h->d_order;
   a.pos = h->d_pos;
   a.loff = h->d_loff;
+  a.cslot = h->d_cslot;
   a.verts = h->d_verts;
   a.a = h->d_a;
   a.binv = h->d_binv;
@@ -630,6 +677,7 @@ msfec_status msfec_ned_create(const msfec_ned_params* params, int n_cells, const
   NED_TRY(ned_alloc(&h->d_order, c.order.size()));
   NED_TRY(ned_alloc(&h->d_pos, c.pos.size()));
   NED_TRY(ned_alloc(&h->d_loff, c.loff.size()));
+  NED_TRY(ned_alloc(&h->d_cslot, c.cslot.size()));
 #undef NED_TRY
   auto up = [&](void* dst, const void* src, size_t bytes) -> msfec_status {
     NED_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
@@ -640,6 +688,7 @@ msfec_status msfec_ned_create(const msfec_ned_params* params, int n_cells, const
   if (s2 == MSFEC_OK) s2 = up(h->d_order, c.order.data(), c.order.size() * sizeof(int));
   if (s2 == MSFEC_OK) s2 = up(h->d_pos, c.pos.data(), c.pos.size() * sizeof(int));
   if (s2 == MSFEC_OK) s2 = up(h->d_loff, c.loff.data(), c.loff.size() * sizeof(int));
+  if (s2 == MSFEC_OK) s2 = up(h->d_cslot, c.cslot.data(), c.cslot.size());
   if (s2 != MSFEC_OK) {
     ned_free(h);
     delete h;
