@@ -68,6 +68,7 @@ struct NedArgs {
   const int* order;  // [n] band row -> fine dof
   const int* pos;    // [N] fine dof -> band row, -1 on the boundary
   const int* loff;   // [n][kNLow] offsets row - column of the lower entries, -1 unused
+  const double* gbnd;        // [N][8] boundary values of the 8 bases (0 inside)
   const signed char* cslot;  // [n][4][8] per fine cell of the row and local dof: the
                              // lower slot of the coupling, -1 boundary dof, -2 upper
   const double* verts;
@@ -95,22 +96,26 @@ __device__ __forceinline__ NedBox ned_box(const NedArgs& a, int cell) {
 
 // fine dof -> (type, i, j): 0 node (i,j); 1 horizontal edge (i,j)-(i+1,j);
 // 2 vertical edge (i,j)-(i,j+1)
-__device__ __forceinline__ void ned_decode(const NedArgs& a, int d, int& type, int& i, int& j) {
-  if (d < a.ns) {
+__host__ __device__ inline void ned_decode_dof(int ns, int nh, int m, int d, int& type, int& i,
+                                               int& j) {
+  if (d < ns) {
     type = 0;
-    j = d / (a.m + 1);
-    i = d - j * (a.m + 1);
-  } else if (d < a.ns + a.nh) {
+    j = d / (m + 1);
+    i = d - j * (m + 1);
+  } else if (d < ns + nh) {
     type = 1;
-    d -= a.ns;
-    j = d / a.m;
-    i = d - j * a.m;
+    d -= ns;
+    j = d / m;
+    i = d - j * m;
   } else {
     type = 2;
-    d -= a.ns + a.nh;
-    j = d / (a.m + 1);
-    i = d - j * (a.m + 1);
+    d -= ns + nh;
+    j = d / (m + 1);
+    i = d - j * (m + 1);
   }
+}
+__device__ __forceinline__ void ned_decode(const NedArgs& a, int d, int& type, int& i, int& j) {
+  ned_decode_dof(a.ns, a.nh, a.m, d, type, i, j);
 }
 
 // dofs of fine cell (i, j): nodes in deal.II vertex order, then the lines in
@@ -135,17 +140,25 @@ __device__ __forceinline__ void ned_cell_dofs(const NedArgs& a, int i, int j, in
 //   curl N_e          s_e / (hx hy), s = (-1, 1, 1, -1)
 //   Nedelec mass      hx / (3 hy), hx / (6 hy) for the vertical pair,
 //                     hy / (3 hx), hy / (6 hx) for the horizontal pair
-__device__ __forceinline__ void ned_local(double hx, double hy, double ac, double bc, double rc,
-                                          double (&K)[8][8]) {
-  const double mq = bc * hx * hy / 36.0;
+// The geometry factors of a fine box, formed once per coarse cell.
+struct NedGeo {
+  double m36, ihh, gx3, gx6, gy3, gy6;  // hx hy / 36, 1 / (hx hy), gx / 3, gx / 6, gy / 3, gy / 6
+};
+__device__ __forceinline__ NedGeo ned_geo(double hx, double hy) {
   const double gx = hx / hy, gy = hy / hx;
+  return {hx * hy / 36.0, 1.0 / (hx * hy), gx / 3, gx / 6, gy / 3, gy / 6};
+}
+
+__device__ __forceinline__ void ned_local(const NedGeo& g, double ac, double bc, double rc,
+                                          double (&K)[8][8]) {
+  const double mq = bc * g.m36;
   const double Mq[4][4] = {{4, 2, 2, 1}, {2, 4, 1, 2}, {2, 1, 4, 2}, {1, 2, 2, 4}};
-  const double G[4][4] = {{-gx / 3, -gx / 6, gx / 3, gx / 6},
-                          {-gx / 6, -gx / 3, gx / 6, gx / 3},
-                          {-gy / 3, gy / 3, -gy / 6, gy / 6},
-                          {-gy / 6, gy / 6, -gy / 3, gy / 3}};
+  const double G[4][4] = {{-g.gx3, -g.gx6, g.gx3, g.gx6},
+                          {-g.gx6, -g.gx3, g.gx6, g.gx3},
+                          {-g.gy3, g.gy3, -g.gy6, g.gy6},
+                          {-g.gy6, g.gy6, -g.gy3, g.gy3}};
   const double s[4] = {-1, 1, 1, -1};
-  const double cc = ac / (hx * hy);
+  const double cc = ac * g.ihh;
 #pragma unroll
   for (int p = 0; p < 4; ++p)
 #pragma unroll
@@ -162,45 +175,47 @@ __device__ __forceinline__ void ned_local(double hx, double hy, double ac, doubl
 #pragma unroll
     for (int h = 0; h < 4; ++h) {
       double mass = 0.0;
-      if ((e >> 1) == (h >> 1)) mass = ((e >> 1) == 0 ? gx : gy) * (e == h ? 1.0 / 3 : 1.0 / 6);
+      if ((e >> 1) == (h >> 1))
+        mass = (e >> 1) == 0 ? (e == h ? g.gx3 : g.gx6) : (e == h ? g.gy3 : g.gy6);
       K[4 + e][4 + h] = cc * s[e] * s[h] + rc * mass;
     }
 }
 
 // (N_e, grad phi_n) over fine cell (i, j) for the coarse bilinear phi_n:
 // d_x phi_n depends on y only, d_y phi_n on x only, both linearly.
-__device__ __forceinline__ double ned_source(int m, double hx, double hy, int i, int j, int n,
-                                             int e) {
-  const double Hx = hx * m, Hy = hy * m;
+__device__ __forceinline__ double ned_source(double im, double cx, double cy, int i, int j,
+                                             int n, int e) {
+  // cx = hx / (6 Hy), cy = hy / (6 Hx); im = 1 / m (exact: m is a power of 2)
   auto lin = [](int hi, double t) { return hi ? t : 1.0 - t; };
   if (e < 2) {  // vertical line: (0, lin/hy) . grad phi_n -> d_y phi_n(x)
-    const double dy = ((n >> 1) ? 1.0 : -1.0) / Hy;
-    const double h0 = lin(n & 1, (double)i / m) * dy, h1 = lin(n & 1, (double)(i + 1) / m) * dy;
-    return e == 0 ? hx * (h0 / 3 + h1 / 6) : hx * (h0 / 6 + h1 / 3);
+    const double c = (n >> 1) ? cx : -cx;
+    const double h0 = lin(n & 1, i * im), h1 = lin(n & 1, (i + 1) * im);
+    return e == 0 ? c * (2.0 * h0 + h1) : c * (h0 + 2.0 * h1);
   }
-  const double dx = ((n & 1) ? 1.0 : -1.0) / Hx;
-  const double g0 = lin(n >> 1, (double)j / m) * dx, g1 = lin(n >> 1, (double)(j + 1) / m) * dx;
-  return e == 2 ? hy * (g0 / 3 + g1 / 6) : hy * (g0 / 6 + g1 / 3);
+  const double c = (n & 1) ? cy : -cy;
+  const double g0 = lin(n >> 1, j * im), g1 = lin(n >> 1, (j + 1) * im);
+  return e == 2 ? c * (2.0 * g0 + g1) : c * (g0 + 2.0 * g1);
 }
 
 // Boundary value of basis b at boundary dof d (setup_basis_dofs_h1/_curl):
 // vertex basis: phi_n at nodes, phi_n(end) - phi_n(start) on edges (the
 // line integral of grad phi_n); edge basis E: 1/m on the fine edges of
-// coarse edge E (the line integral of N_E), 0 elsewhere.
-__device__ __forceinline__ double ned_bval(const NedArgs& a, int d, int b) {
+// coarse edge E (the line integral of N_E), 0 elsewhere.  The same for every
+// coarse cell of a refinement level: tabulated once per level on the host.
+inline double ned_bval(int ns, int nh, int mi, int d, int b) {
   int type, i, j;
-  ned_decode(a, d, type, i, j);
-  const double m = a.m;
+  ned_decode_dof(ns, nh, mi, d, type, i, j);
+  const double m = mi;
   auto phi = [](int n, double X, double Y) {
     return ((n & 1) ? X : 1.0 - X) * ((n >> 1) ? Y : 1.0 - Y);
   };
   if (type == 0) return b < 4 ? phi(b, i / m, j / m) : 0.0;
   if (type == 1) {
     if (b < 4) return phi(b, (i + 1) / m, j / m) - phi(b, i / m, j / m);
-    return ((b == 6 && j == 0) || (b == 7 && j == a.m)) ? 1.0 / m : 0.0;
+    return ((b == 6 && j == 0) || (b == 7 && j == mi)) ? 1.0 / m : 0.0;
   }
   if (b < 4) return phi(b, i / m, (j + 1) / m) - phi(b, i / m, j / m);
-  return ((b == 4 && i == 0) || (b == 5 && i == a.m)) ? 1.0 / m : 0.0;
+  return ((b == 4 && i == 0) || (b == 5 && i == mi)) ? 1.0 / m : 0.0;
 }
 
 __global__ __launch_bounds__(kNThreads) void ned_rows_kernel(NedArgs a) {
@@ -210,6 +225,8 @@ __global__ __launch_bounds__(kNThreads) void ned_rows_kernel(NedArgs a) {
   double* sl = srow[threadIdx.x];
   const NedBox bx = ned_box(a, cell);
   const int m = a.m, nf = m * m;
+  const double im = 1.0 / m, cx = bx.hx / (6.0 * bx.hy * m), cy = bx.hy / (6.0 * bx.hx * m);
+  const NedGeo geo = ned_geo(bx.hx, bx.hy);
   const double* ca = a.a + (size_t)cell * nf;
   const double* cb = a.binv + (size_t)cell * nf;
   const double* cr = a.r + (size_t)cell * nf;
@@ -250,7 +267,7 @@ __global__ __launch_bounds__(kNThreads) void ned_rows_kernel(NedArgs a) {
       if (q >= nc) break;
       const int fc = cj[q] * m + ci[q];
       double K[8][8];
-      ned_local(bx.hx, bx.hy, ca[fc], cb[fc], cr[fc], K);
+      ned_local(geo, ca[fc], cb[fc], cr[fc], K);
       int dofs[8];
       ned_cell_dofs(a, ci[q], cj[q], dofs);
       const int p = lp[q];
@@ -271,12 +288,12 @@ __global__ __launch_bounds__(kNThreads) void ned_rows_kernel(NedArgs a) {
           sl[t] += kr[k];  // lower coupling in slot t
         } else if (t == -1) {
 #pragma unroll
-          for (int bb = 0; bb < kNR; ++bb) rhs[bb] -= kr[k] * ned_bval(a, dofs[k], bb);
+          for (int bb = 0; bb < kNR; ++bb) rhs[bb] -= kr[k] * a.gbnd[(size_t)dofs[k] * kNR + bb];
         }
       }
       if (a.full_rhs && p >= 4)
 #pragma unroll
-        for (int n = 0; n < 4; ++n) rhs[n] += ned_source(m, bx.hx, bx.hy, ci[q], cj[q], n, p - 4);
+        for (int n = 0; n < 4; ++n) rhs[n] += ned_source(im, cx, cy, ci[q], cj[q], n, p - 4);
     }
     double* out = a.rows + ((size_t)cell * a.n + r) * kRowLen;
 #pragma unroll
@@ -304,7 +321,7 @@ __global__ __launch_bounds__(kNThreads) void ned_rows_kernel(NedArgs a) {
   for (int d = threadIdx.x; d < a.N; d += blockDim.x)
     if (a.pos[d] < 0)
 #pragma unroll
-      for (int bb = 0; bb < kNR; ++bb) a.psi[((size_t)cell * kNR + bb) * a.N + d] = ned_bval(a, d, bb);
+      for (int bb = 0; bb < kNR; ++bb) a.psi[((size_t)cell * kNR + bb) * a.N + d] = a.gbnd[(size_t)d * kNR + bb];
 }
 
 // Interior of Psi and the element matrix (q_ned_basis.cc:841-939):
@@ -333,6 +350,7 @@ __global__ __launch_bounds__(kNThreads) void ned_element_kernel(NedArgs a) {
   }
   __syncthreads();
   const NedBox bx = ned_box(a, cell);
+  const NedGeo geo = ned_geo(bx.hx, bx.hy);
   const int kt = tid & 7, grp = tid >> 3;
   const bool edge_t = kt >= 4;
   const int r0 = edge_t ? 4 : 0;
@@ -345,7 +363,7 @@ __global__ __launch_bounds__(kNThreads) void ned_element_kernel(NedArgs a) {
     ned_cell_dofs(a, i, j, d);
     const size_t c = (size_t)cell * nf + fc;
     double K[8][8];
-    ned_local(bx.hx, bx.hy, a.a[c], a.binv[c], a.r[c], K);
+    ned_local(geo, a.a[c], a.binv[c], a.r[c], K);
     double u[4];
 #pragma unroll
     for (int p = 0; p < 4; ++p) u[p] = ps[kt * N + d[r0 + p]];
@@ -419,6 +437,7 @@ struct NedConf {
   int l = 0, m = 0, ns = 0, nh = 0, nv = 0, N = 0, n = 0, p = 0;
   std::vector<int> order, pos, loff;
   std::vector<signed char> cslot;
+  std::vector<double> gbnd;
 };
 
 bool ned_conf(int l, NedConf& c, std::string& why) {
@@ -510,6 +529,10 @@ bool ned_conf(int l, NedConf& c, std::string& why) {
       }
     }
   }
+  c.gbnd.assign((size_t)c.N * kNR, 0.0);
+  for (int d = 0; d < c.N; ++d)
+    if (c.pos[d] < 0)
+      for (int b = 0; b < kNR; ++b) c.gbnd[(size_t)d * kNR + b] = ned_bval(c.ns, c.nh, m, d, b);
   if (c.p > kMaxBand || c.n > kMaxRows) {
     why = "half-bandwidth " + std::to_string(c.p) + " exceeds " + std::to_string(kMaxBand) +
           " (l <= 4)";
@@ -542,6 +565,7 @@ struct msfec_ned_batch {
   int* d_pos = nullptr;
   int* d_loff = nullptr;
   signed char* d_cslot = nullptr;
+  double* d_gbnd = nullptr;
   bool coef_set = false, computed = false;
   hipStream_t stream = nullptr;
   std::vector<std::array<hipEvent_t, 4>> ev_sets;
@@ -559,7 +583,7 @@ msfec_status ned_alloc(T** p, size_t count) {
 void ned_free(msfec_ned_batch* h) {
   void* ptrs[] = {h->d_verts, h->d_a,   h->d_binv, h->d_r,   h->d_f,     h->d_rows,
                   h->d_pan,   h->d_zst, h->d_X,    h->d_scale, h->d_psi, h->d_M,
-                  h->d_b,     h->d_status, h->d_order, h->d_pos, h->d_loff, h->d_cslot};
+                  h->d_b,     h->d_status, h->d_order, h->d_pos, h->d_loff, h->d_cslot, h->d_gbnd};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (auto& set : h->ev_sets)
@@ -596,6 +620,7 @@ NedArgs ned_args(const msfec_ned_batch* h) {
   a.pos = h->d_pos;
   a.loff = h->d_loff;
   a.cslot = h->d_cslot;
+  a.gbnd = h->d_gbnd;
   a.verts = h->d_verts;
   a.a = h->d_a;
   a.binv = h->d_binv;
@@ -678,6 +703,7 @@ msfec_status msfec_ned_create(const msfec_ned_params* params, int n_cells, const
   NED_TRY(ned_alloc(&h->d_pos, c.pos.size()));
   NED_TRY(ned_alloc(&h->d_loff, c.loff.size()));
   NED_TRY(ned_alloc(&h->d_cslot, c.cslot.size()));
+  NED_TRY(ned_alloc(&h->d_gbnd, c.gbnd.size()));
 #undef NED_TRY
   auto up = [&](void* dst, const void* src, size_t bytes) -> msfec_status {
     NED_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
@@ -689,6 +715,7 @@ msfec_status msfec_ned_create(const msfec_ned_params* params, int n_cells, const
   if (s2 == MSFEC_OK) s2 = up(h->d_pos, c.pos.data(), c.pos.size() * sizeof(int));
   if (s2 == MSFEC_OK) s2 = up(h->d_loff, c.loff.data(), c.loff.size() * sizeof(int));
   if (s2 == MSFEC_OK) s2 = up(h->d_cslot, c.cslot.data(), c.cslot.size());
+  if (s2 == MSFEC_OK) s2 = up(h->d_gbnd, c.gbnd.data(), c.gbnd.size() * sizeof(double));
   if (s2 != MSFEC_OK) {
     ned_free(h);
     delete h;
