@@ -2926,6 +2926,7 @@ constexpr int kCgWaves = kCgThreads / 64;
 struct CgArgs {
   double tol;
   int max_iter;
+  int seg;     // rows per thread: segments of grid lines in x (cg_segment)
   int* iters;  // [cells][NV][2]: iterations, state (0 converged, 1 not, 2 breakdown, 3 non-finite)
 };
 
@@ -2961,52 +2962,93 @@ __device__ __forceinline__ void cg_block_sum2(double& x, double& y, double (*red
   __syncthreads();
 }
 
-// (K v)(node c) for an interior node from the nodal vector v in LDS: the
-// 3^dim neighbour values, then the 2^dim fine cells around the node, each
-// a_e sum_v ks[al ^ v] v(vertex v) (+ the reaction mass when REAC), with
-// al the node's corner in the cell.  Every register index is static.
+// K v at the rows of one segment (nodes (x0 .. x0 + len - 1, y, z), all
+// interior) from the nodal vector v in LDS.  Per node: the fine cells
+// around it, each a_e sum_vt ks[al ^ vt] v(vertex vt) (+ the reaction mass
+// when REAC), al the node's corner in the cell.  Along the segment the
+// 3^dim neighbour values and the 2^dim cell coefficients slide in x: each
+// node loads one new plane of each (3^(dim-1) values and 2^(dim-1)
+// coefficients).  Every register index is static.
 template <int DIM, bool REAC>
-__device__ __forceinline__ double cg_apply(const double* cf, const double* rf, bool has_rf,
-                                           double rc, const double (&kr)[8], const double (&mr)[8],
-                                           const int* c, int N, int S, const double* v) {
-  constexpr int NV = 1 << DIM, NST = (DIM == 3) ? 27 : 9;
-  const int node = c[0] + S * c[1] + (DIM == 3 ? S * S * c[2] : 0);
-  double nb[NST];
+__device__ __forceinline__ void cg_line_apply(const double* cf, const double* rf, bool has_rf,
+                                              double rc, const double (&kr)[8],
+                                              const double (&mr)[8], int x0, int cy, int cz,
+                                              int len, int N, int S, const double* v,
+                                              double (&q)[kCgOwn]) {
+  constexpr int NV = 1 << DIM;
+  constexpr int NP = (DIM == 3) ? 9 : 3;  // neighbour values per x-plane
+  constexpr int NC = (DIM == 3) ? 4 : 2;  // cells per x-plane
+  auto plane = [&](int X, double (&w)[NP]) {
 #pragma unroll
-  for (int e = 0; e < NST; ++e) {
-    int off = 0, mul = 1;
+    for (int iz = 0; iz < (DIM == 3 ? 3 : 1); ++iz)
 #pragma unroll
-    for (int d = 0; d < DIM; ++d) {
-      off += dcomp<DIM>(e, d) * mul;
-      mul *= S;
-    }
-    nb[e] = v[node + off];
-  }
-  double acc = 0.0;
+      for (int iy = 0; iy < 3; ++iy)
+        w[iy + 3 * iz] = v[X + S * (cy - 1 + iy) + (DIM == 3 ? S * S * (cz - 1 + iz) : 0)];
+  };
+  auto cplane = [&](const double* src, int X, double (&c)[NC]) {
 #pragma unroll
-  for (int al = 0; al < NV; ++al) {
-    int z = 0, mul = 1;
+    for (int iz = 0; iz < (DIM == 3 ? 2 : 1); ++iz)
 #pragma unroll
-    for (int d = 0; d < DIM; ++d) {
-      z += (c[d] - ((al >> d) & 1)) * mul;
-      mul *= N;
-    }
-    double sk = 0.0, sm = 0.0;
+      for (int iy = 0; iy < 2; ++iy)
+        c[iy + 2 * iz] = src[X + N * (cy - 1 + iy) + (DIM == 3 ? N * N * (cz - 1 + iz) : 0)];
+  };
+  const bool crf = REAC && has_rf;
+  double w0[NP], w1[NP], w2[NP], c0[NC], c1[NC], e0[NC], e1[NC];
+  plane(x0 - 1, w0);
+  plane(x0, w1);
+  cplane(cf, x0 - 1, c0);
 #pragma unroll
-    for (int vt = 0; vt < NV; ++vt) {
-      int e = 0, p3 = 1;
+  for (int t = 0; t < NC; ++t) e0[t] = 0.0;
+  if (crf) cplane(rf, x0 - 1, e0);
 #pragma unroll
-      for (int d = 0; d < DIM; ++d) {
-        e += (((vt >> d) & 1) - ((al >> d) & 1) + 1) * p3;
-        p3 *= 3;
+  for (int sg = 0; sg < kCgOwn; ++sg) {
+    if (sg < len) {
+      const int X = x0 + sg;
+      plane(X + 1, w2);
+      cplane(cf, X, c1);
+#pragma unroll
+      for (int t = 0; t < NC; ++t) e1[t] = 0.0;
+      if (crf) cplane(rf, X, e1);
+      double acc = 0.0;
+#pragma unroll
+      for (int al = 0; al < NV; ++al) {
+        const int ax = al & 1, ay = (al >> 1) & 1, az = (DIM == 3) ? (al >> 2) & 1 : 0;
+        const int ci = (1 - ay) + (DIM == 3 ? 2 * (1 - az) : 0);  // cell (X - ax, y - ay, z - az)
+        double sk = 0.0, sm = 0.0;
+#pragma unroll
+        for (int vt = 0; vt < NV; ++vt) {
+          const int ex = (vt & 1) - ax + 1, ey = ((vt >> 1) & 1) - ay + 1;
+          const int ez = (DIM == 3) ? ((vt >> 2) & 1) - az + 1 : 0;
+          const double nbv = (ex == 0 ? w0 : ex == 1 ? w1 : w2)[ey + 3 * ez];
+          sk = fma(kr[al ^ vt], nbv, sk);
+          if (REAC) sm = fma(mr[al ^ vt], nbv, sm);
+        }
+        acc = fma((ax ? c0 : c1)[ci], sk, acc);
+        if (REAC) acc = fma((ax ? e0 : e1)[ci] + rc, sm, acc);
       }
-      sk = fma(kr[al ^ vt], nb[e], sk);
-      if (REAC) sm = fma(mr[al ^ vt], nb[e], sm);
+      q[sg] = acc;
+#pragma unroll
+      for (int t = 0; t < NP; ++t) {
+        w0[t] = w1[t];
+        w1[t] = w2[t];
+      }
+#pragma unroll
+      for (int t = 0; t < NC; ++t) {
+        c0[t] = c1[t];
+        e0[t] = e1[t];
+      }
     }
-    acc = fma(cf[z], sk, acc);
-    if (REAC) acc = fma((has_rf ? rf[z] : 0.0) + rc, sm, acc);
   }
-  return acc;
+}
+
+// Segment length of the CG workgroup: the shortest (at most kCgOwn rows)
+// with which the lines of the grid in x, cut into segments, need at most one
+// thread each; 0 if there is none.
+inline int cg_segment(int ni, int dim) {
+  const int lines = (dim == 3) ? ni * ni : ni;
+  for (int L = 1; L <= kCgOwn; ++L)
+    if ((long long)lines * ((ni + L - 1) / L) <= kCgThreads) return L;
+  return 0;
 }
 
 // interior row jn -> grid coordinates (1..N-1 per dimension)
@@ -3025,7 +3067,7 @@ __global__ __launch_bounds__(kCgThreads) void q_cg_kernel(BandArgs a, CgArgs g) 
   using D = Dirs<DIM>;
   const int cell = blockIdx.x / D::NV, j = blockIdx.x - cell * D::NV;
   const int tid = threadIdx.x;
-  // LDS: coefficient | reaction (per cell) | search direction over all nodes | K p
+  // LDS: coefficient | reaction (per cell) | search direction over all nodes
   extern __shared__ double dyn[];
   __shared__ double kv[8], mv[8];
   __shared__ double red[kCgWaves][2];
@@ -3033,7 +3075,6 @@ __global__ __launch_bounds__(kCgThreads) void q_cg_kernel(BandArgs a, CgArgs g) 
   double* cf = dyn;
   double* rf = dyn + a.nfc;
   double* pv = dyn + a.nfc * (has_rf ? 2 : 1);
-  double* qv = pv + a.nn;
   const double* cg = a.coef + (size_t)cell * a.nfc;
   for (int z = tid; z < a.nfc; z += kCgThreads) cf[z] = cg[z];
   if (has_rf) {
@@ -3053,22 +3094,31 @@ __global__ __launch_bounds__(kCgThreads) void q_cg_kernel(BandArgs a, CgArgs g) 
   }
   const double rc = (a.reaction_mode == MSFEC_REACTION_CONST) ? a.reaction_const : 0.0;
   const double* rows = a.Rows + (size_t)cell * a.n * kRowStride;
-  // owned rows jn = tid + t * kCgThreads: node, right-hand side, inverse diagonal
-  int node[kCgOwn];
-  double b[kCgOwn], dinv[kCgOwn], x[kCgOwn], r[kCgOwn];
+  // the owned segment: interior rows (x0 .. x0 + len - 1, cy, cz), consecutive
+  // in the row numbering (x fastest)
+  const int ni = a.ni, L = g.seg, spl = (ni + L - 1) / L;
+  const int lines = (DIM == 3) ? ni * ni : ni;
+  // threads past the last segment take line 0 with no rows (its loads stay
+  // inside the grid)
+  const int line0 = tid / spl, sx = tid - line0 * spl;
+  const int len = (line0 < lines) ? min(L, ni - sx * L) : 0;
+  const int line = (line0 < lines) ? line0 : 0;
+  const int x0 = 1 + sx * L, cy = 1 + line % ni, cz = (DIM == 3) ? 1 + line / ni : 0;
+  const int jn0 = (x0 - 1) + ni * ((cy - 1) + (DIM == 3 ? ni * (cz - 1) : 0));
+  const int node0 = x0 + a.S * cy + (DIM == 3 ? a.S * a.S * cz : 0);
+  // owned rows: right-hand side, inverse diagonal, iterate, residual, direction
+  double b[kCgOwn], dinv[kCgOwn], x[kCgOwn], r[kCgOwn], p[kCgOwn];
   double bb = 0.0, rz = 0.0;
 #pragma unroll
   for (int t = 0; t < kCgOwn; ++t) {
-    const int jn = tid + t * kCgThreads;
-    int c[3];
-    cg_coords<DIM>(jn < a.n ? jn : 0, a.ni, c);
-    node[t] = c[0] + a.S * c[1] + (DIM == 3 ? a.S * a.S * c[2] : 0);
-    b[t] = (jn < a.n) ? rows[(size_t)jn * kRowStride + 16 + j] : 0.0;
-    dinv[t] = (jn < a.n) ? 1.0 / rows[(size_t)jn * kRowStride] : 0.0;
+    const bool ok = t < len;
+    b[t] = ok ? rows[(size_t)(jn0 + t) * kRowStride + 16 + j] : 0.0;
+    dinv[t] = ok ? 1.0 / rows[(size_t)(jn0 + t) * kRowStride] : 0.0;
     x[t] = 0.0;
     r[t] = b[t];
+    p[t] = dinv[t] * r[t];
     bb = fma(b[t], b[t], bb);
-    rz = fma(r[t], dinv[t] * r[t], rz);
+    rz = fma(r[t], p[t], rz);
   }
   __syncthreads();  // coefficients, box tables and the zeroed vector
   double kr[8], mr[8];
@@ -3079,8 +3129,8 @@ __global__ __launch_bounds__(kCgThreads) void q_cg_kernel(BandArgs a, CgArgs g) 
   }
 #pragma unroll
   for (int t = 0; t < kCgOwn; ++t)
-    if (tid + t * kCgThreads < a.n) pv[node[t]] = dinv[t] * r[t];
-  cg_block_sum2(bb, rz, red);
+    if (t < len) pv[node0 + t] = p[t];
+  cg_block_sum2(bb, rz, red);  // also: p is in LDS
   double rr = bb;
   const double stop = g.tol * g.tol * bb;
   int it = 0, state = 0;
@@ -3089,17 +3139,12 @@ __global__ __launch_bounds__(kCgThreads) void q_cg_kernel(BandArgs a, CgArgs g) 
       state = 1;
       break;
     }
-    // q = K p into LDS (one row at a time: the stencil gather is register
-    // heavy), and p^T q
+    double q[kCgOwn];
+    cg_line_apply<DIM, REAC>(cf, rf, has_rf, rc, kr, mr, x0, cy, cz, len, a.N, a.S, pv, q);
     double pq = 0.0, dummy = 0.0;
-#pragma unroll 1
-    for (int jn = tid; jn < a.n; jn += kCgThreads) {
-      int c[3];
-      cg_coords<DIM>(jn, a.ni, c);
-      const double q = cg_apply<DIM, REAC>(cf, rf, has_rf, rc, kr, mr, c, a.N, a.S, pv);
-      qv[jn] = q;
-      pq = fma(pv[c[0] + a.S * c[1] + (DIM == 3 ? a.S * a.S * c[2] : 0)], q, pq);
-    }
+#pragma unroll
+    for (int t = 0; t < kCgOwn; ++t)
+      if (t < len) pq = fma(p[t], q[t], pq);
     cg_block_sum2(pq, dummy, red);
     if (!(pq > 0.0)) {
       state = 2;
@@ -3109,10 +3154,9 @@ __global__ __launch_bounds__(kCgThreads) void q_cg_kernel(BandArgs a, CgArgs g) 
     double rz1 = 0.0, rr1 = 0.0;
 #pragma unroll
     for (int t = 0; t < kCgOwn; ++t) {
-      const int jn = tid + t * kCgThreads;
-      if (jn < a.n) {
-        x[t] = fma(alpha, pv[node[t]], x[t]);
-        r[t] = fma(-alpha, qv[jn], r[t]);
+      if (t < len) {
+        x[t] = fma(alpha, p[t], x[t]);
+        r[t] = fma(-alpha, q[t], r[t]);
         rz1 = fma(r[t], dinv[t] * r[t], rz1);
         rr1 = fma(r[t], r[t], rr1);
       }
@@ -3122,31 +3166,32 @@ __global__ __launch_bounds__(kCgThreads) void q_cg_kernel(BandArgs a, CgArgs g) 
     rz = rz1;
     rr = rr1;
 #pragma unroll
-    for (int t = 0; t < kCgOwn; ++t)
-      if (tid + t * kCgThreads < a.n) pv[node[t]] = fma(beta, pv[node[t]], dinv[t] * r[t]);
+    for (int t = 0; t < kCgOwn; ++t) {
+      if (t < len) {
+        p[t] = fma(beta, p[t], dinv[t] * r[t]);
+        pv[node0 + t] = p[t];
+      }
+    }
     __syncthreads();
     ++it;
   }
   if (!isfinite(rr) || !isfinite(bb)) state = 3;
   // the true residual of the returned solution, for the energy form of S
+  __syncthreads();  // an early exit leaves threads that may still read p
 #pragma unroll
   for (int t = 0; t < kCgOwn; ++t)
-    if (tid + t * kCgThreads < a.n) pv[node[t]] = x[t];
+    if (t < len) pv[node0 + t] = x[t];
   __syncthreads();
-#pragma unroll 1
-  for (int jn = tid; jn < a.n; jn += kCgThreads) {
-    int c[3];
-    cg_coords<DIM>(jn, a.ni, c);
-    qv[jn] = cg_apply<DIM, REAC>(cf, rf, has_rf, rc, kr, mr, c, a.N, a.S, pv);
-  }
+  double q[kCgOwn];
+  cg_line_apply<DIM, REAC>(cf, rf, has_rf, rc, kr, mr, x0, cy, cz, len, a.N, a.S, pv, q);
   double* psi = a.Psi + ((size_t)cell * D::NV + j) * a.nn;
 #pragma unroll
   for (int t = 0; t < kCgOwn; ++t) {
-    const int jn = tid + t * kCgThreads;
-    if (jn < a.n) {
-      a.X[((size_t)cell * a.n + jn) * kMaxNb + j] = x[t];
-      a.Rres[((size_t)cell * a.n + jn) * kMaxNb + j] = b[t] - qv[jn];
-      psi[node[t]] = x[t];
+    if (t < len) {
+      const size_t jn = (size_t)cell * a.n + jn0 + t;
+      a.X[jn * kMaxNb + j] = x[t];
+      a.Rres[jn * kMaxNb + j] = b[t] - q[t];
+      psi[node0 + t] = x[t];
     }
   }
   if (tid == 0) {
@@ -3428,8 +3473,8 @@ msfec_status msfec_q_create(const msfec_q_params* params, int n_cells, const dou
     const long long nint = (long long)ipow(ni, p.dim), nnod = (long long)ipow(N + 1, p.dim);
     const long long lds = ((long long)ipow(N, p.dim) *
                                (1 + (p.reaction_mode == MSFEC_REACTION_CELL ? 1 : 0)) +
-                           nnod + nint) * (long long)sizeof(double);
-    if (nint > (long long)kCgThreads * kCgOwn || lds > 160 * 1024)
+                           nnod) * (long long)sizeof(double);
+    if (nint > (long long)kCgThreads * kCgOwn || lds > 160 * 1024 || cg_segment(ni, p.dim) == 0)
       return fail(MSFEC_ERR_UNSUPPORTED,
                   "cell too fine for the CG workgroup (3-D: n_refine_local <= 4, 2-D: <= 6)");
   } else if (bw + 1 > kWin)
@@ -3710,8 +3755,9 @@ msfec_status msfec_q_compute(msfec_q_handle h, void* stream) {
     g.tol = h->p.cg_tol > 0.0 ? h->p.cg_tol : 1e-13;
     g.max_iter = h->p.cg_max_iter > 0 ? h->p.cg_max_iter : 5000;
     g.iters = h->d_cgit;
+    g.seg = cg_segment(c.ni, c.dim);
     const size_t cdyn = ((size_t)c.nfc * (1 + (c.reaction_mode == MSFEC_REACTION_CELL ? 1 : 0)) +
-                         (size_t)c.nn + (size_t)c.n) * sizeof(double);
+                         (size_t)c.nn) * sizeof(double);
     const bool reac = c.reaction_mode != MSFEC_REACTION_NONE;
 #define LAUNCH_CG(D, R)                                                                   \
   do {                                                                                    \
