@@ -3203,34 +3203,50 @@ __global__ __launch_bounds__(kCgThreads) void q_cg_kernel(BandArgs a, CgArgs g) 
 // Element matrix and right-hand side of a CG cell: S in its energy form
 // S[i][k] = R_i^T x_k + x_i^T r_k (r = R - K x, exact up to the second order
 // in the error of x), symmetrised; M = P - S and b = b_B + sum_j x_j F_j as in
-// the band kernels.  Lane (i = lane >> 3, k = lane & 7) of each of 4 waves
-// sums every 4th interior row.
+// the band kernels.  Lane (i = lane >> 3, k = lane & 7) of each of 16 waves
+// sums every 16th interior row, 4 rows in flight.
+constexpr int kFinWaves = 16;
+
 template <int DIM>
-__global__ __launch_bounds__(256) void q_cg_finish_kernel(BandArgs a, CgArgs g) {
+__global__ __launch_bounds__(64 * kFinWaves) void q_cg_finish_kernel(BandArgs a, CgArgs g) {
   using D = Dirs<DIM>;
   const int cell = blockIdx.x, tid = threadIdx.x;
   const int lane = tid & 63, w = tid >> 6, i = lane >> 3, k = lane & 7;
-  __shared__ double part[4][64], spart[4][64];
+  __shared__ double part[kFinWaves][64], spart[kFinWaves][64];
   double s = 0.0, bi = 0.0;
   if (i < D::NV && k < D::NV) {
     const double* rows = a.Rows + (size_t)cell * a.n * kRowStride;
     const double* X = a.X + (size_t)cell * a.n * kMaxNb;
     const double* Rr = a.Rres + (size_t)cell * a.n * kMaxNb;
-    for (int jn = w; jn < a.n; jn += 4) {
+    auto term = [&](int jn, double& acc, double& bacc) {
       const double xi = X[(size_t)jn * kMaxNb + i], xk = X[(size_t)jn * kMaxNb + k];
-      s = fma(rows[(size_t)jn * kRowStride + 16 + i], xk, s);
-      s = fma(xi, Rr[(size_t)jn * kMaxNb + k], s);
-      if (i == 0) bi = fma(xk, rows[(size_t)jn * kRowStride + 14], bi);
+      acc = fma(rows[(size_t)jn * kRowStride + 16 + i], xk, acc);
+      acc = fma(xi, Rr[(size_t)jn * kMaxNb + k], acc);
+      if (i == 0) bacc = fma(xk, rows[(size_t)jn * kRowStride + 14], bacc);
+    };
+    // every 16th row per wave, 4 rows in flight
+    double s4[4] = {0.0, 0.0, 0.0, 0.0}, b4[4] = {0.0, 0.0, 0.0, 0.0};
+    int jn = w;
+    for (; jn + 3 * kFinWaves < a.n; jn += 4 * kFinWaves) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) term(jn + u * kFinWaves, s4[u], b4[u]);
     }
+    for (; jn < a.n; jn += kFinWaves) term(jn, s4[0], b4[0]);
+    s = (s4[0] + s4[1]) + (s4[2] + s4[3]);
+    bi = (b4[0] + b4[1]) + (b4[2] + b4[3]);
   }
   part[w][lane] = s;
   spart[w][lane] = bi;
   __syncthreads();
   if (w == 0) {
-    const double sik = part[0][lane] + part[1][lane] + part[2][lane] + part[3][lane];
     const int lt = k * 8 + i;  // lane of S[k][i]
-    const double ski = part[0][lt] + part[1][lt] + part[2][lt] + part[3][lt];
-    const double bint = spart[0][lane] + spart[1][lane] + spart[2][lane] + spart[3][lane];
+    double sik = 0.0, ski = 0.0, bint = 0.0;
+#pragma unroll
+    for (int v = 0; v < kFinWaves; ++v) {
+      sik += part[v][lane];
+      ski += part[v][lt];
+      bint += spart[v][lane];
+    }
     fast_finish<DIM>(a, cell, 0.5 * (sik + ski), bint);
     if (lane == 0) {
       int st = MSFEC_CELL_OK;
@@ -3777,9 +3793,9 @@ msfec_status msfec_q_compute(msfec_q_handle h, void* stream) {
     MSFEC_HIP(hipGetLastError());
     if (tm) MSFEC_HIP(hipEventRecord(ev[2], s));
     if (c.dim == 3)
-      hipLaunchKernelGGL((q_cg_finish_kernel<3>), dim3(h->n_cells), dim3(256), 0, s, a, g);
+      hipLaunchKernelGGL((q_cg_finish_kernel<3>), dim3(h->n_cells), dim3(64 * kFinWaves), 0, s, a, g);
     else
-      hipLaunchKernelGGL((q_cg_finish_kernel<2>), dim3(h->n_cells), dim3(256), 0, s, a, g);
+      hipLaunchKernelGGL((q_cg_finish_kernel<2>), dim3(h->n_cells), dim3(64 * kFinWaves), 0, s, a, g);
     MSFEC_HIP(hipGetLastError());
     if (tm) MSFEC_HIP(hipEventRecord(ev[3], s));
   } else if (mixed) {
