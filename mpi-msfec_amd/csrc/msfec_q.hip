@@ -2941,7 +2941,8 @@ __device__ __forceinline__ double cg_wave_sum(double v) {
   return v;
 }
 
-// block sums of two values; every thread gets both
+// block sums of two values; every thread gets both.  One barrier: a caller
+// reuses a buffer only after a later barrier (two buffers in turn).
 __device__ __forceinline__ void cg_block_sum2(double& x, double& y, double (*red)[2]) {
   x = cg_wave_sum(x);
   y = cg_wave_sum(y);
@@ -2959,7 +2960,6 @@ __device__ __forceinline__ void cg_block_sum2(double& x, double& y, double (*red
   }
   x = sx;
   y = sy;
-  __syncthreads();
 }
 
 // K v at the rows of one segment (nodes (x0 .. x0 + len - 1, y, z), all
@@ -3010,8 +3010,46 @@ __device__ __forceinline__ void cg_line_apply(const double* cf, const double* rf
       for (int t = 0; t < NC; ++t) e1[t] = 0.0;
       if (crf) cplane(rf, X, e1);
       double acc = 0.0;
+      if (!REAC) {
+        // K[n][n+e] = ks[mask(e)] * (sum of the coefficients of the cells
+        // shared by n and n+e): y-z sums of each cell plane, then x
+        double P0[NP], P1[NP];
+        auto ysums = [&](const double (&c)[NC], double (&P)[NP]) {
 #pragma unroll
-      for (int al = 0; al < NV; ++al) {
+          for (int ez = 0; ez < (DIM == 3 ? 3 : 1); ++ez)
+#pragma unroll
+            for (int ey = 0; ey < 3; ++ey) {
+              double v = 0.0;
+#pragma unroll
+              for (int iz = 0; iz < (DIM == 3 ? 2 : 1); ++iz)
+#pragma unroll
+                for (int iy = 0; iy < 2; ++iy) {
+                  // cell (y - 1 + iy) is shared with neighbour ey - 1 iff iy == ey / 2 or ey == 1
+                  const bool y_ok = (ey == 1) || (iy == ey / 2);
+                  const bool z_ok = (DIM == 2) || (ez == 1) || (iz == ez / 2);
+                  if (y_ok && z_ok) v += c[iy + 2 * iz];
+                }
+              P[ey + 3 * ez] = v;
+            }
+        };
+        ysums(c0, P0);
+        ysums(c1, P1);
+        double am[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) am[t] = 0.0;
+#pragma unroll
+        for (int e = 0; e < NP * 3; ++e) {
+          const int ex = e % 3, eyz = e / 3, ey = eyz % 3, ez = eyz / 3;
+          const int mask = (ex != 1 ? 1 : 0) | (ey != 1 ? 2 : 0) | (ez != 1 && DIM == 3 ? 4 : 0);
+          const double sa = ex == 0 ? P0[eyz] : ex == 2 ? P1[eyz] : P0[eyz] + P1[eyz];
+          const double nbv = (ex == 0 ? w0 : ex == 1 ? w1 : w2)[eyz];
+          am[mask] = fma(sa, nbv, am[mask]);
+        }
+#pragma unroll
+        for (int t = 0; t < (1 << DIM); ++t) acc = fma(kr[t], am[t], acc);
+      }
+#pragma unroll
+      for (int al = 0; al < (REAC ? NV : 0); ++al) {
         const int ax = al & 1, ay = (al >> 1) & 1, az = (DIM == 3) ? (al >> 2) & 1 : 0;
         const int ci = (1 - ay) + (DIM == 3 ? 2 * (1 - az) : 0);  // cell (X - ax, y - ay, z - az)
         double sk = 0.0, sm = 0.0;
@@ -3070,7 +3108,10 @@ __global__ __launch_bounds__(kCgThreads) void q_cg_kernel(BandArgs a, CgArgs g) 
   // LDS: coefficient | reaction (per cell) | search direction over all nodes
   extern __shared__ double dyn[];
   __shared__ double kv[8], mv[8];
-  __shared__ double red[kCgWaves][2];
+  // the two block sums of an iteration use one buffer each: the barrier of
+  // the other sum (or the one closing the iteration) separates the reads of
+  // a buffer from its next writes
+  __shared__ double red[2][kCgWaves][2];
   const bool has_rf = (a.reaction_mode == MSFEC_REACTION_CELL);
   double* cf = dyn;
   double* rf = dyn + a.nfc;
@@ -3130,7 +3171,7 @@ __global__ __launch_bounds__(kCgThreads) void q_cg_kernel(BandArgs a, CgArgs g) 
 #pragma unroll
   for (int t = 0; t < kCgOwn; ++t)
     if (t < len) pv[node0 + t] = p[t];
-  cg_block_sum2(bb, rz, red);  // also: p is in LDS
+  cg_block_sum2(bb, rz, red[1]);  // also: p is in LDS
   double rr = bb;
   const double stop = g.tol * g.tol * bb;
   int it = 0, state = 0;
@@ -3145,7 +3186,7 @@ __global__ __launch_bounds__(kCgThreads) void q_cg_kernel(BandArgs a, CgArgs g) 
 #pragma unroll
     for (int t = 0; t < kCgOwn; ++t)
       if (t < len) pq = fma(p[t], q[t], pq);
-    cg_block_sum2(pq, dummy, red);
+    cg_block_sum2(pq, dummy, red[0]);
     if (!(pq > 0.0)) {
       state = 2;
       break;
@@ -3161,7 +3202,7 @@ __global__ __launch_bounds__(kCgThreads) void q_cg_kernel(BandArgs a, CgArgs g) 
         rr1 = fma(r[t], r[t], rr1);
       }
     }
-    cg_block_sum2(rz1, rr1, red);  // also: every thread is done reading p
+    cg_block_sum2(rz1, rr1, red[1]);  // also: every thread is done reading p
     const double beta = rz1 / rz;
     rz = rz1;
     rr = rr1;
