@@ -356,14 +356,16 @@ struct Dirs {
 __device__ __forceinline__ int pad8(int x) { return 9 * (x >> 3) + (x & 7); }
 
 struct BandSmem {
-  double ubuf[72];   // pivot row, slot x at pad8(x)
+  union {
+    double ubuf[72];     // vector-ALU band kernel: pivot row, slot x at pad8(x)
+    int stg_off[8][16];  // matrix-core LDL^T rows: offsets of the staged lower entries
+  };
   double nbuf[72];   // incoming row, slot x at pad8(x)
   double ybuf[8];
   double kv[8];
   double mv[8];
   double stg_low[8][16];
   double stg_rhs[8][8];
-  int stg_off[8][16];  // LDL^T rows: offsets of the staged lower entries
   // phase-dependent pool:
   //   forward : RHS partials [64][9]
   //   backward: w of the group [8][8] | L columns of the group [8][72] | x window [64][9]
@@ -2017,9 +2019,12 @@ __device__ __forceinline__ double dpp_f64(double v, int ctrl_sel) {
   } else if (ctrl_sel == 1) {
     lo = __builtin_amdgcn_mov_dpp(lo, 0x4E, 0xF, 0xF, false);  // quad_perm [2,3,0,1]
     hi = __builtin_amdgcn_mov_dpp(hi, 0x4E, 0xF, 0xF, false);
-  } else {
+  } else if (ctrl_sel == 2) {
     lo = __builtin_amdgcn_mov_dpp(lo, 0x141, 0xF, 0xF, false);  // row_half_mirror
     hi = __builtin_amdgcn_mov_dpp(hi, 0x141, 0xF, 0xF, false);
+  } else {
+    lo = __builtin_amdgcn_mov_dpp(lo, 0x128, 0xF, 0xF, false);  // row_ror:8
+    hi = __builtin_amdgcn_mov_dpp(hi, 0x128, 0xF, 0xF, false);
   }
   return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
 }
@@ -2153,17 +2158,19 @@ __device__ __forceinline__ void bk_backward(const BandArgs& a, int cell, BandSme
     s01 = sum8(s01);
     s10 = sum8(s10);
     s11 = sum8(s11);
-    if (c == 0) {
-      tb[k0 * 8 + r0] = s00;
-      tb[k0 * 8 + r0 + 1] = s01;
-      tb[(k0 + 1) * 8 + r0] = s10;
-      tb[(k0 + 1) * 8 + r0 + 1] = s11;
-    }
-    wave_sync();
-    if (lane < 8) {
-      const int rr = lane;
-      const double w0 = zb[0 * 8 + rr] - tb[0 * 8 + rr], w1 = zb[1 * 8 + rr] - tb[1 * 8 + rr];
-      const double w2 = zb[2 * 8 + rr] - tb[2 * 8 + rr], w3 = zb[3 * 8 + rr] - tb[3 * 8 + rr];
+    // the other pair of k for the same pair of r sits in lanes l ^ 8 (a
+    // rotation of the row of 16 by 8): every lane then has (V^T x)[k][r]
+    // for all 4 k and solves for r = r0 + (c & 1), without a round trip
+    // through LDS; the lanes c < 2 of the groups with k0 = 0 write
+    const double q00 = dpp_f64(s00, 3), q01 = dpp_f64(s01, 3);
+    const double q10 = dpp_f64(s10, 3), q11 = dpp_f64(s11, 3);
+    const int rb = c & 1, rr = r0 + rb;
+    const bool lowk = k0 == 0;
+    const double oa = rb ? s01 : s00, ob = rb ? s11 : s10;  // own k0, k0 + 1
+    const double qa = rb ? q01 : q00, qb = rb ? q11 : q10;  // the partner's
+    {
+      const double w0 = zb[0 * 8 + rr] - (lowk ? oa : qa), w1 = zb[1 * 8 + rr] - (lowk ? ob : qb);
+      const double w2 = zb[2 * 8 + rr] - (lowk ? qa : oa), w3 = zb[3 * 8 + rr] - (lowk ? qb : ob);
       // C^T x = w with C (t,u) at pb[u * 68 + t] and 1 / C[t][t] at pb[t * 68 + t]
       const double x3 = w3 * (double)pb[3 * kPanRow + 3];
       const double x2 = (w2 - (double)pb[2 * kPanRow + 3] * x3) * (double)pb[2 * kPanRow + 2];
@@ -2173,10 +2180,11 @@ __device__ __forceinline__ void bk_backward(const BandArgs& a, int cell, BandSme
                          (double)pb[0 * kPanRow + 3] * x3) *
                         (double)pb[0 * kPanRow + 0];
       const double xs[4] = {x0, x1, x2, x3};
+      const bool wr = lowk && c < 2;
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
         const int j = j0 + t;
-        if (j >= n) continue;
+        if (!wr || j >= n) continue;
         const int p = j & (kWin - 1);
         xr[rr * kRing + p] = xs[t];
         xr[rr * kRing + p + 64] = xs[t];
@@ -2219,6 +2227,8 @@ __device__ __forceinline__ void bk_backward(const BandArgs& a, int cell, BandSme
   }
   if (jflush > 0) flush(0, jflush);
   wave_sync();
+  // the interior part of b for basis r was summed by lane 16 (r >> 1) + (r & 1)
+  bint = __shfl(bint, 16 * ((lane & 7) >> 1) + (lane & 1));
 }
 
 // Sacc (accumulator layout) -> lane (i = lane >> 3, k = lane & 7)
