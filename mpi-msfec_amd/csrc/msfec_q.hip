@@ -2070,7 +2070,11 @@ __device__ __forceinline__ void bk_backward(const BandArgs& a, int cell, BandSme
   double* xr = SH_PART(sh);           // [8][132]
   double* zb = xr + 8 * kRing;        // [4][8]
   double* tb = zb + 32;               // [4][8]
-  double* rw = tb + 32;               // [4][24] table rows of the step
+  double* rw = tb + 32;               // table rows of the step: [4][24] (REFINE) or [4][2]
+  // slot 14 (load) or 15 (node index) of row t of the step in rw
+  auto rwi = [](int t, int slot) {
+    return MODE == BK_REFINE ? t * kRowStride + slot : 2 * t + (slot - 14);
+  };
   double* xb = rw + 96;               // [4][8] first solution X of the step (BK_REFINE)
   double* r0b = xb + 32;              // [4][8] its residual (BK_REFINE)
   double* nodeb = &sh.stg_low[0][0];  // [64] node index of the ring rows
@@ -2085,8 +2089,13 @@ __device__ __forceinline__ void bk_backward(const BandArgs& a, int cell, BandSme
     for (int u = 0; u < 4; ++u) f.pp[u] = Pc[(size_t)q * 256 + u * 64 + lane];  // [k = u][o = lane]
     f.zp = (lane < 32) ? Zc[(size_t)q * 32 + lane] : (R)0;
     const int lim = (n - 4 * q) * kRowStride;  // valid doubles of the 4 rows
-    f.rp0 = (lane < lim) ? Rc[(size_t)4 * q * kRowStride + lane] : 0.0;
-    f.rp1 = (lane + 64 < min(lim, 96)) ? Rc[(size_t)4 * q * kRowStride + 64 + lane] : 0.0;
+    if (MODE == BK_REFINE) {  // the 4 whole rows (the lifting right-hand sides too)
+      f.rp0 = (lane < lim) ? Rc[(size_t)4 * q * kRowStride + lane] : 0.0;
+      f.rp1 = (lane + 64 < min(lim, 96)) ? Rc[(size_t)4 * q * kRowStride + 64 + lane] : 0.0;
+    } else if (MODE == BK_FINAL64) {  // load and node index of the 4 rows: slots 14, 15
+      const int rr = 4 * q + (lane >> 1);
+      f.rp0 = (lane < 8 && rr < n) ? Rc[(size_t)rr * kRowStride + 14 + (lane & 1)] : 0.0;
+    }
     if (MODE == BK_REFINE) {
       const bool ok = lane < 32 && 4 * q + (lane >> 3) < n;
       const size_t xi = ((size_t)cell * n + 4 * q) * kMaxNb + (ok ? lane : 0);
@@ -2123,15 +2132,19 @@ __device__ __forceinline__ void bk_backward(const BandArgs& a, int cell, BandSme
 #pragma unroll
     for (int v = 0; v < 4; ++v) pb[v * kPanRow + lane] = f.pp[v];
     if (lane < 32) zb[lane] = (double)f.zp;
-    rw[lane] = f.rp0;
-    if (lane < 32) rw[64 + lane] = f.rp1;
+    if (MODE == BK_REFINE) {
+      rw[lane] = f.rp0;
+      if (lane < 32) rw[64 + lane] = f.rp1;
+    } else if (MODE == BK_FINAL64 && lane < 8) {
+      rw[lane] = f.rp0;
+    }
     if (MODE == BK_REFINE && lane < 32) {
       xb[lane] = f.xp;
       r0b[lane] = f.qp;
     }
     wave_sync();
     if (MODE != BK_X0 && lane < 4 && j0 + lane < n)
-      nodeb[(j0 + lane) & (kWin - 1)] = rw[lane * kRowStride + 15];
+      nodeb[(j0 + lane) & (kWin - 1)] = rw[rwi(lane, 15)];
     if (MODE == BK_REFINE) {
       // energy-form Schur term: lane (i = lane >> 3, kk = lane & 7)
       const int i = lane >> 3, kk = lane & 7;
@@ -2195,7 +2208,7 @@ __device__ __forceinline__ void bk_backward(const BandArgs& a, int cell, BandSme
             dmax = fmax(dmax, fabs(xs[t]));
             xmax = fmax(xmax, fabs(xf));
           }
-          bint = fma(xf, rw[t * kRowStride + 14], bint);
+          bint = fma(xf, rw[rwi(t, 14)], bint);
         }
       }
     }
