@@ -2070,14 +2070,19 @@ __device__ __forceinline__ void bk_backward(const BandArgs& a, int cell, BandSme
   double* xr = SH_PART(sh);           // [8][132]
   double* zb = xr + 8 * kRing;        // [4][8]
   double* tb = zb + 32;               // [4][8]
-  double* rw = tb + 32;               // table rows of the step: [4][24] (REFINE) or [4][2]
-  // slot 14 (load) or 15 (node index) of row t of the step in rw
-  auto rwi = [](int t, int slot) {
-    return MODE == BK_REFINE ? t * kRowStride + slot : 2 * t + (slot - 14);
-  };
+  double* rw = tb + 32;               // table rows of the step: [4][24] (REFINE) or loads [4]
   double* xb = rw + 96;               // [4][8] first solution X of the step (BK_REFINE)
   double* r0b = xb + 32;              // [4][8] its residual (BK_REFINE)
-  double* nodeb = &sh.stg_low[0][0];  // [64] node index of the ring rows
+  // fine node of interior row j (the row table's slot 15)
+  auto node_of = [&](int j) {
+    const int c0 = j % a.ni + 1, t = j / a.ni;
+    const int c1 = (DIM == 3 ? t % a.ni : t) + 1, c2 = (DIM == 3) ? t / a.ni + 1 : 0;
+    return c0 + a.S * (c1 + a.S * c2);
+  };
+  // the load of row t of the step (slot 14): from the table for a load per
+  // fine cell, else the constant load times 2^dim
+  const bool cell_load = a.rhs_mode == MSFEC_RHS_CELL;
+  const double const_load = a.rhs_const * D::NV;
   for (int t = lane; t < 8 * kRing; t += 64) xr[t] = 0.0;
   for (int t = lane; t < 4 * kPanRow; t += 64) pb[t] = (R)0;
   const R* Pc = reinterpret_cast<const R*>(a.pan) + (size_t)cell * a.nq * 256;
@@ -2092,9 +2097,9 @@ __device__ __forceinline__ void bk_backward(const BandArgs& a, int cell, BandSme
     if (MODE == BK_REFINE) {  // the 4 whole rows (the lifting right-hand sides too)
       f.rp0 = (lane < lim) ? Rc[(size_t)4 * q * kRowStride + lane] : 0.0;
       f.rp1 = (lane + 64 < min(lim, 96)) ? Rc[(size_t)4 * q * kRowStride + 64 + lane] : 0.0;
-    } else if (MODE == BK_FINAL64) {  // load and node index of the 4 rows: slots 14, 15
-      const int rr = 4 * q + (lane >> 1);
-      f.rp0 = (lane < 8 && rr < n) ? Rc[(size_t)rr * kRowStride + 14 + (lane & 1)] : 0.0;
+    } else if (MODE == BK_FINAL64 && cell_load) {  // the loads of the 4 rows: slot 14
+      const int rr = 4 * q + lane;
+      f.rp0 = (lane < 4 && rr < n) ? Rc[(size_t)rr * kRowStride + 14] : 0.0;
     }
     if (MODE == BK_REFINE) {
       const bool ok = lane < 32 && 4 * q + (lane >> 3) < n;
@@ -2120,7 +2125,7 @@ __device__ __forceinline__ void bk_backward(const BandArgs& a, int cell, BandSme
         for (int rr = 0; rr < kMaxNb; rr += 2) xd[rr >> 1] = make_double2(v[rr], v[rr + 1]);
       }
       if (MODE != BK_X0) {
-        const int node = (int)nodeb[p];
+        const int node = node_of(j);
 #pragma unroll
         for (int rr = 0; rr < D::NV; ++rr) a.Psi[((size_t)cell * D::NV + rr) * a.nn + node] = v[rr];
       }
@@ -2135,7 +2140,7 @@ __device__ __forceinline__ void bk_backward(const BandArgs& a, int cell, BandSme
     if (MODE == BK_REFINE) {
       rw[lane] = f.rp0;
       if (lane < 32) rw[64 + lane] = f.rp1;
-    } else if (MODE == BK_FINAL64 && lane < 8) {
+    } else if (MODE == BK_FINAL64 && cell_load && lane < 4) {
       rw[lane] = f.rp0;
     }
     if (MODE == BK_REFINE && lane < 32) {
@@ -2143,8 +2148,6 @@ __device__ __forceinline__ void bk_backward(const BandArgs& a, int cell, BandSme
       r0b[lane] = f.qp;
     }
     wave_sync();
-    if (MODE != BK_X0 && lane < 4 && j0 + lane < n)
-      nodeb[(j0 + lane) & (kWin - 1)] = rw[rwi(lane, 15)];
     if (MODE == BK_REFINE) {
       // energy-form Schur term: lane (i = lane >> 3, kk = lane & 7)
       const int i = lane >> 3, kk = lane & 7;
@@ -2208,7 +2211,10 @@ __device__ __forceinline__ void bk_backward(const BandArgs& a, int cell, BandSme
             dmax = fmax(dmax, fabs(xs[t]));
             xmax = fmax(xmax, fabs(xf));
           }
-          bint = fma(xf, rw[rwi(t, 14)], bint);
+          const double fj = (MODE == BK_REFINE) ? rw[t * kRowStride + 14]
+                            : cell_load                ? rw[t]
+                                                       : const_load;
+          bint = fma(xf, fj, bint);
         }
       }
     }
