@@ -434,8 +434,8 @@ __device__ __forceinline__ double lift_k(int k, const int* m, double invN) {
 // all 3^dim directions e (dd_d = e's base-3 digit d minus 1) at once: the
 // coefficient (SA) and the reaction (SR, including the constant part).  The
 // 2^dim cells around the node are loaded once (cells outside the grid count
-// as zero) and combined one dimension at a time: dd_d = -1 keeps the cells
-// below the node in direction d, +1 those above, 0 both.
+// as zero); direction e sums the cells it shares with the node: dd_d = -1
+// keeps the cells below the node in direction d, +1 those above, 0 both.
 template <int DIM>
 __device__ __forceinline__ void node_sums(const double* cf, const double* rf, bool has_rf,
                                           double rc, const int* c, int N, double* SA,
