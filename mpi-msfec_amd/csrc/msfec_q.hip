@@ -1634,10 +1634,20 @@ struct Mf<float> {
 constexpr int kPanRow = 68;
 constexpr int kRing = 132;
 
+// Row stride of the pivot and incoming rows in LDS (slot x of row t at
+// t * stride + x).  A wave touches 16 consecutive slots of each of 4 rows
+// (row parts) or one slot of each of 4 rows (column parts): FP64 rows 68
+// apart (4 mod 16) and FP32 rows 80 apart (16 mod 32) keep both
+// conflict-free.
+template <typename R>
+constexpr int bk_row() {
+  return sizeof(R) == 8 ? 68 : 80;
+}
+
 template <typename R>
 struct BkSmem {
-  R piv[4 * 72];   // forward: pivot rows, slot x at pad8(x); backward: panel [4][68]
-  R nrow[4 * 72];  // forward: incoming rows
+  R piv[4 * 80];   // forward: pivot rows; backward: panel [4][68]
+  R nrow[4 * 80];  // forward: incoming rows
   R yp[64];        // forward: pivot right-hand sides [4][16]
 };
 
@@ -1662,18 +1672,16 @@ __device__ __forceinline__ int bk_lane() {
 }
 
 // Per-lane LDS offsets (in elements) of the super-step accesses, computed
-// once per kernel: every access is one of these plus a compile-time constant,
-// since pad8(16 Q + x) = 18 Q + pad8(x) and pad8(4 r + lh) = lh + {0,4,9,13}[r].
+// once per kernel: every access is one of these plus a compile-time constant
+// (W = bk_row<R>()).
 struct BkAddr {
   int lane, lh, ll, cs;
-  int row;  // lh * 72 + pad8(cs): row part of tile (T, Q) at + 18 Q
-  int col;  // (cs & 3) * 72 + lh: column part of tile (P, T) at + 18 P + kColR[r]
-  int vrd;  // pad8(cs): panel reads at + 72 t + 18 P
-  int zr;   // pad8(lane): zeroing of the incoming rows
+  int row;  // lh * W + cs: row part of tile (T, Q) at + 16 Q
+  int col;  // (cs & 3) * W + lh: column part of tile (P, T) at + 16 P + 4 r
+  int vrd;  // cs: panel reads at + W t + 16 P
+  int zr;   // lane: zeroing of the incoming rows
   bool gcol[4];  // (cs >> 2) == G: this lane holds pivot column cs & 3 of group G
 };
-
-__device__ constexpr int kColR[4] = {0, 4, 9, 13};
 
 template <typename R>
 __device__ __forceinline__ BkAddr bk_addr() {
@@ -1682,10 +1690,11 @@ __device__ __forceinline__ BkAddr bk_addr() {
   ad.lh = ad.lane >> 4;
   ad.ll = ad.lane & 15;
   ad.cs = Mf<R>::cslot(ad.ll);
-  ad.row = ad.lh * 72 + pad8(ad.cs);
-  ad.col = (ad.cs & 3) * 72 + ad.lh;
-  ad.vrd = pad8(ad.cs);
-  ad.zr = pad8(ad.lane);
+  constexpr int W = bk_row<R>();
+  ad.row = ad.lh * W + ad.cs;
+  ad.col = (ad.cs & 3) * W + ad.lh;
+  ad.vrd = ad.cs;
+  ad.zr = ad.lane;
 #pragma unroll
   for (int g = 0; g < 4; ++g) ad.gcol[g] = (ad.cs >> 2) == g;
   return ad;
@@ -1695,12 +1704,12 @@ template <typename R, int T, int G>
 __device__ __forceinline__ void bk_extract(const BkAddr& ad, const typename Mf<R>::v4 (&Tl)[10],
                                            R* piv) {
 #pragma unroll
-  for (int Q = 0; Q <= T; ++Q) piv[ad.row + 18 * Q] = Tl[MF_TILE(T, Q)][G];
+  for (int Q = 0; Q <= T; ++Q) piv[ad.row + 16 * Q] = Tl[MF_TILE(T, Q)][G];
   if (ad.gcol[G]) {
 #pragma unroll
     for (int P = T + 1; P < 4; ++P)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) piv[ad.col + 18 * P + kColR[r]] = Tl[MF_TILE(P, T)][r];
+      for (int r = 0; r < 4; ++r) piv[ad.col + 16 * P + 4 * r] = Tl[MF_TILE(P, T)][r];
   }
 }
 
@@ -1715,7 +1724,7 @@ template <typename R>
 __device__ __forceinline__ double bk_scatter_load(const BkAddr& ad, int rb, BandSmem& sh,
                                                   BkSmem<R>& ms) {
 #pragma unroll
-  for (int t = 0; t < 4; ++t) ms.nrow[ad.zr + 72 * t] = (R)0;
+  for (int t = 0; t < 4; ++t) ms.nrow[ad.zr + bk_row<R>() * t] = (R)0;
   return sh.stg_low[rb + ad.lh][ad.ll];
 }
 
@@ -1738,12 +1747,12 @@ __device__ __forceinline__ void bk_scatter_store(const BandArgs& a, const BkAddr
   const bool w2 = w1 && e > 0 && my_off <= t;
   const bool w3 = !live && e == 0;  // past the end: isolated unit diagonal
   R* dummy = reinterpret_cast<R*>(SH_PART(sh)) + lane;
-  // slot of column jn - off (jbase = s0 mod 64); pad8(s0 + t) = pad8(s0) + t
-  const int i1 = w1 ? t * 72 + pad8((bk_opaque(jbase) + t - my_off) & (kWin - 1))
-                    : t * 72 + pad8(s0) + t;
+  constexpr int W = bk_row<R>();
+  // slot of column jn - off (jbase = s0 mod 64)
+  const int i1 = w1 ? t * W + ((bk_opaque(jbase) + t - my_off) & (kWin - 1)) : t * W + s0 + t;
   R* p1 = (w1 || w3) ? &ms.nrow[i1] : dummy;
   *p1 = w1 ? (R)v : (R)1;
-  R* p2 = w2 ? &ms.nrow[max(t - my_off, 0) * 72 + pad8(s0) + t] : dummy + 64;
+  R* p2 = w2 ? &ms.nrow[max(t - my_off, 0) * W + s0 + t] : dummy + 64;
   *p2 = (R)v;
 }
 
@@ -1753,12 +1762,12 @@ __device__ __forceinline__ void bk_place(const BandArgs& a, const BkAddr& ad, in
                                          typename Mf<R>::v4 (&Yt)[4], BandSmem& sh,
                                          BkSmem<R>& ms) {
 #pragma unroll
-  for (int Q = 0; Q <= T; ++Q) Tl[MF_TILE(T, Q)][G] = ms.nrow[ad.row + 18 * Q];
+  for (int Q = 0; Q <= T; ++Q) Tl[MF_TILE(T, Q)][G] = ms.nrow[ad.row + 16 * Q];
   if (ad.gcol[G]) {
 #pragma unroll
     for (int P = T; P < 4; ++P)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) Tl[MF_TILE(P, T)][r] = ms.nrow[ad.col + 18 * P + kColR[r]];
+      for (int r = 0; r < 4; ++r) Tl[MF_TILE(P, T)][r] = ms.nrow[ad.col + 16 * P + 4 * r];
   }
   Yt[T][G] = (ad.ll < 8 && jbase + ad.lh < a.n) ? (R)sh.stg_rhs[rb + ad.lh][ad.ll & 7] : (R)0;
 }
@@ -1923,8 +1932,9 @@ __device__ __forceinline__ void bk_superstep(const BandArgs& a, const BkAddr& ad
   R V[4], nV[4];
 #pragma unroll
   for (int P = 0; P < 4; ++P) {
-    const int x = ad.vrd + 18 * P;
-    V[P] = pv[0 * 72 + x] * x0 + pv[1 * 72 + x] * x1 + pv[2 * 72 + x] * x2 + pv[3 * 72 + x] * x3;
+    constexpr int W = bk_row<R>();
+    const int x = ad.vrd + 16 * P;
+    V[P] = pv[0 * W + x] * x0 + pv[1 * W + x] * x1 + pv[2 * W + x] * x2 + pv[3 * W + x] * x3;
     nV[P] = LDL ? -V[P] * sc : -V[P];
   }
   const R Z = ms.yp[0 * 16 + ll] * x0 + ms.yp[1 * 16 + ll] * x1 + ms.yp[2 * 16 + ll] * x2 +
