@@ -1612,6 +1612,11 @@ struct Mf<double> {
   static __device__ __forceinline__ v4 mma(double a, double b, v4 c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
+  // c - a b: the FP64 instruction negates its first operand itself (the
+  // last modifier field is neg:[A,B,C] for this instruction)
+  static __device__ __forceinline__ v4 mma_neg(double a, double b, v4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 1);
+  }
   static __device__ __forceinline__ int cslot(int ll) { return ll; }
   // lane holding (row slot t, column slot cs) of a register of a tile
   static constexpr int lane_of(int t, int cs) { return 16 * t + cs; }
@@ -1624,6 +1629,7 @@ struct Mf<float> {
   static __device__ __forceinline__ v4 mma(float a, float b, v4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
+  static __device__ __forceinline__ v4 mma_neg(float a, float b, v4 c) { return mma(-a, b, c); }
   static __device__ __forceinline__ int cslot(int ll) { return 4 * (ll & 3) + (ll >> 2); }
   static constexpr int lane_of(int t, int cs) { return 16 * t + 4 * (cs & 3) + (cs >> 2); }
   static __device__ __forceinline__ int acc_row(int g, int t) { return 4 * t + g; }
@@ -1929,13 +1935,13 @@ __device__ __forceinline__ void bk_superstep(const BandArgs& a, const BkAddr& ad
   }
   // panel in operand layout: lane (row slot 16P + cs, pivot lh).  Cholesky:
   // V = A21 C^-T, update -V V^T.  LDL^T: V = A21 L1^-T, update -V D1^-1 V^T.
-  R V[4], nV[4];
+  R V[4], aV[4];  // aV: the left operand, V D1^-1 for LDL^T (negated by the instruction)
 #pragma unroll
   for (int P = 0; P < 4; ++P) {
     constexpr int W = bk_row<R>();
     const int x = ad.vrd + 16 * P;
     V[P] = pv[0 * W + x] * x0 + pv[1 * W + x] * x1 + pv[2 * W + x] * x2 + pv[3 * W + x] * x3;
-    nV[P] = LDL ? -V[P] * sc : -V[P];
+    aV[P] = LDL ? V[P] * sc : V[P];
   }
   const R Z = ms.yp[0 * 16 + ll] * x0 + ms.yp[1 * 16 + ll] * x1 + ms.yp[2 * 16 + ll] * x2 +
               ms.yp[3 * 16 + ll] * x3;
@@ -1945,8 +1951,9 @@ __device__ __forceinline__ void bk_superstep(const BandArgs& a, const BkAddr& ad
     for (int P = 0; P < 4; ++P) {
 #pragma unroll
       for (int Q = 0; Q <= P; ++Q)
-        if (bk_prio<T, G>(P, Q) == pr) Tl[MF_TILE(P, Q)] = M::mma(nV[P], V[Q], Tl[MF_TILE(P, Q)]);
-      if (bk_prio<T, G>(P, P) == pr) Yt[P] = M::mma(nV[P], Z, Yt[P]);
+        if (bk_prio<T, G>(P, Q) == pr)
+          Tl[MF_TILE(P, Q)] = M::mma_neg(aV[P], V[Q], Tl[MF_TILE(P, Q)]);
+      if (bk_prio<T, G>(P, P) == pr) Yt[P] = M::mma_neg(aV[P], Z, Yt[P]);
     }
   }
   if (!LDL) Sacc = M::mma(Z, Z, Sacc);
