@@ -2381,15 +2381,18 @@ __global__ __launch_bounds__(256) void q_resid_kernel(BandArgs a, int stage_lo) 
       return stage_lo ? los[jj * D::NLOW + t] : rows[(size_t)jj * kRowStride + t];
     };
     double acc = rj[16 + r] - lo(j, 0) * xs[i];
+    // no branches, so that the reads issue together: a lower coupling to a
+    // row below 0 is zero in the table (its neighbour is on the boundary),
+    // and an upper one beyond n - 1 is masked; indices are clamped
 #pragma unroll
     for (int t = 1; t < D::NLOW; ++t) {
       const int off = a.off_int[t];
-      const double l = lo(j, t);
-      if (l != 0.0 && j >= off) acc = fma(-l, xs[(j - off) * kMaxNb + r], acc);
-      if (j + off < n) {
-        const double up = lo(j + off, t);
-        if (up != 0.0) acc = fma(-up, xs[(j + off) * kMaxNb + r], acc);
-      }
+      const int jm = max(j - off, 0), jp = j + off;
+      const bool pin = jp < n;
+      const int jq = pin ? jp : n - 1;
+      acc = fma(-lo(j, t), xs[jm * kMaxNb + r], acc);
+      const double up = lo(jq, t);
+      acc = fma(pin ? -up : 0.0, xs[jq * kMaxNb + r], acc);
     }
     a.Rres[((size_t)cell * n + j) * kMaxNb + r] = acc;
   }
