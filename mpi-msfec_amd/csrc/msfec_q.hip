@@ -370,7 +370,7 @@ struct BandSmem {
   //   forward : RHS partials [64][9]
   //   backward: w of the group [8][8] | L columns of the group [8][72] | x window [64][9]
   //   epilogue: psi of 64 nodes [64][9] | (K psi)_B of 64 nodes [64][9] | F of 64 nodes [64]
-  //   matrix-core backward: x ring [8][132] | z [4][8] | partial sums [4][8] | table rows [4][24] | x [4][8]
+  //   matrix-core backward: x ring [8][132] | z [4][8] | table rows [4][24] | x [4][8] | residual [4][8]
   double pool[1280];
 };
 #define SH_PART(sh) ((sh).pool)
@@ -2054,12 +2054,14 @@ __device__ __forceinline__ double sum8(double v) {
 
 // Backward sweep.  (V^T x)[k][r] over the 60 offsets of a panel: lane
 // (k0 = 2 ((lane >> 3) & 1), r0 = 2 (lane >> 4), c = lane & 7) sums a 2 x 2
-// block of it over every 8th offset, the 8 lanes of the block add up by DPP;
-// lanes 0..7 (r = lane) then apply C^-T.  The steps go in groups of 4: at the start of
-// a group one wait covers the data of the group (the panels, z, the 4 table
-// rows of each step: lifting right-hand sides, interior load, node index),
-// loaded during the previous group into the other of two register sets, and
-// the loads of the next group are issued.  The steps issue no global stores:
+// block of it over every 8th offset, the 8 lanes of the block add up by DPP,
+// the other pair of k comes from lanes l ^ 8, and every lane applies C^-T
+// for its column r = r0 + (c & 1).  The steps go in groups of 4: at the
+// start of a group one wait covers the data of the group (the panels, z;
+// for the refinement sweep the 4 table rows of each step; for a load per
+// fine cell the loads of the 4 rows), loaded during the previous group into
+// the other of two register sets, and the loads of the next group are
+// issued.  The steps issue no global stores:
 // the solution rows stay in the LDS ring (64 rows) and are written out 64
 // rows at a time, one row per lane, right after a group's wait, so no wait
 // ever covers a store or load issued less than a group earlier.
@@ -2086,8 +2088,7 @@ __device__ __forceinline__ void bk_backward(const BandArgs& a, int cell, BandSme
   R* pb = ms.piv;                     // [4][68]
   double* xr = SH_PART(sh);           // [8][132]
   double* zb = xr + 8 * kRing;        // [4][8]
-  double* tb = zb + 32;               // [4][8]
-  double* rw = tb + 32;               // table rows of the step: [4][24] (REFINE) or loads [4]
+  double* rw = zb + 32;               // table rows of the step: [4][24] (REFINE) or loads [4]
   double* xb = rw + 96;               // [4][8] first solution X of the step (BK_REFINE)
   double* r0b = xb + 32;              // [4][8] its residual (BK_REFINE)
   // fine node of interior row j (the row table's slot 15)
