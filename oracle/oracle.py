@@ -89,6 +89,10 @@ def lib():
         L.oq_ned_basis_cell.restype = ctypes.c_int
         L.oq_ned_assemble.argtypes = [ctypes.c_int, dp, dp, dp, dp, dp, dp, dp]
         L.oq_ned_assemble.restype = ctypes.c_int
+        L.oq_rt_basis_cell.argtypes = [ctypes.c_int, dp, dp, dp, dp, dp, dp, dp]
+        L.oq_rt_basis_cell.restype = ctypes.c_int
+        L.oq_rt_assemble.argtypes = [ctypes.c_int, dp, dp, dp, dp, dp]
+        L.oq_rt_assemble.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -293,5 +297,67 @@ def ned_assemble(l, verts, a, binv=1.0, r=0.0, f=(1.0, 1.0)):
     A = np.zeros((N, N))
     F = np.zeros(N)
     st = lib().oq_ned_assemble(l, _dp(v), _dp(a), _dp(binv), _dp(r), _dp(f), _dp(A), _dp(F))
+    assert st == 0
+    return A, F
+
+
+# ---------------------------------------------------------------------------
+# 2-D Raviart-Thomas x DQ0 basis (rt_oracle.c)
+# ---------------------------------------------------------------------------
+def rt_sizes(l):
+    """(nv, ne, nc): vertical edges, all edges, fine cells."""
+    m = 1 << l
+    nv = m * (m + 1)
+    return nv, 2 * nv, m * m
+
+
+def _rt_coefs(l, a, f):
+    nf = (1 << l) ** 2
+    a = np.ascontiguousarray(np.broadcast_to(np.asarray(a, dtype=np.float64), (nf,)))
+    f = np.ascontiguousarray(np.broadcast_to(np.asarray(f, dtype=np.float64), (nf,)))
+    return a, f
+
+
+def rt_basis_cell(l, verts, a, f=1.0):
+    """One coarse box cell: returns (M 5x5, b 5, Sigma 4 x ne, U 4 x nc, status)."""
+    a, f = _rt_coefs(l, a, f)
+    v = np.ascontiguousarray(verts, dtype=np.float64).reshape(4, 2)
+    _, ne, nc = rt_sizes(l)
+    M = np.zeros((5, 5))
+    b = np.zeros(5)
+    Sig = np.zeros((4, ne))
+    U = np.zeros((4, nc))
+    st = lib().oq_rt_basis_cell(l, _dp(v), _dp(a), _dp(f), _dp(M), _dp(b), _dp(Sig), _dp(U))
+    return M, b, Sig, U, st
+
+
+def rt_basis_batch(l, verts, a, f=1.0):
+    """Cells of verts (n, 4, 2) with per-cell coefficient blocks a, f (n, m*m)."""
+    verts = np.asarray(verts, dtype=np.float64)
+    n = verts.shape[0]
+    nf = (1 << l) ** 2
+
+    def per_cell(x):
+        x = np.asarray(x, dtype=np.float64)
+        return np.broadcast_to(x, (n, nf)) if x.ndim <= 1 else x.reshape(n, nf)
+
+    a_, f_ = per_cell(a), per_cell(f)
+    out = {"M": [], "b": [], "Sigma": [], "U": [], "status": []}
+    for c in range(n):
+        M, b, S, U, st = rt_basis_cell(l, verts[c], a_[c], f_[c])
+        for k, v in zip(("M", "b", "Sigma", "U", "status"), (M, b, S, U, st)):
+            out[k].append(v)
+    return {k: np.array(v) for k, v in out.items()}
+
+
+def rt_assemble(l, verts, a, f=1.0):
+    """The assembled mixed system over all fine dofs (row = test) and the load."""
+    a, f = _rt_coefs(l, a, f)
+    v = np.ascontiguousarray(verts, dtype=np.float64).reshape(4, 2)
+    _, ne, nc = rt_sizes(l)
+    N = ne + nc
+    A = np.zeros((N, N))
+    F = np.zeros(N)
+    st = lib().oq_rt_assemble(l, _dp(v), _dp(a), _dp(f), _dp(A), _dp(F))
     assert st == 0
     return A, F
