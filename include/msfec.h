@@ -274,6 +274,74 @@ msfec_status msfec_ned_get_cell_status(msfec_ned_handle h, int *status);
 msfec_status msfec_ned_kernel_times(msfec_ned_handle h, float *ms, int max, int *n);
 msfec_status msfec_ned_destroy(msfec_ned_handle h);
 
+/* ---------------------------------------------------------------------------
+ * RT_DQ: the mixed lowest-order Raviart-Thomas x DQ0 multiscale basis,
+ * two-dimensional (the reference's RTDQBasis, source/RT_DQ/rt_dq_basis.cc, is
+ * its 3-D form).  Per coarse box cell and fine cell (2^l x 2^l,
+ * lexicographic, x fastest): a (scalar diffusion; the solve uses 1/a like
+ * DiffusionInverse_A) and f (load).  4 bases, one per coarse face in deal.II
+ * face order (x = x0, x = x1, y = y0, y = y1): the boundary flux is that of
+ * the coarse Raviart-Thomas function of the face and the divergence -+1/|K|
+ * (is_laplace, rt_dq_basis.cc:516-538).  Fine degrees of freedom of a basis
+ * (ne = 2 m (m+1), m = 2^l): the flux through each fine edge in the +x / +y
+ * direction, the m (m+1) vertical edges first (edge (P, q), x-node P and cell
+ * row q, at q (m+1) + P), then the horizontal edges (edge (p, Q) at
+ * m (m+1) + Q m + p).  The u part of every basis is the constant 1
+ * (set_u_to_std, rt_dq_basis.cc:1039-1043) and is not stored.  The element
+ * matrix is 5 x 5 with index 4 the DQ0 basis.
+ * ------------------------------------------------------------------------- */
+typedef struct {
+  int n_refine_local; /* l, 1..5 (the reduced system's half-bandwidth 2^l <= 60) */
+  int flags;          /* MSFEC_FLAG_*                                          */
+} msfec_rt_params;
+
+typedef struct msfec_rt_batch *msfec_rt_handle;
+
+/* Create a batch.  vertices: n_cells x 4 x 2 doubles, axis-aligned boxes.
+ * Replaces RTDQBasis::RTDQBasis + setup_grid/setup_system_matrix
+ * (source/RT_DQ/rt_dq_basis.cc:7-265, include/RT_DQ/rt_dq_basis.h:96-111). */
+msfec_status msfec_rt_create(const msfec_rt_params *params, int n_cells,
+                             const double *vertices, int vertices_loc,
+                             msfec_rt_handle *out);
+
+/* Per-fine-cell data, n_cells x 4^l doubles each; f may be NULL (f = 1).
+ * Stands for the DiffusionInverse_A and RightHandSideParsed evaluations of
+ * RTDQBasis::assemble_system (rt_dq_basis.cc:423-461). */
+msfec_status msfec_rt_set_coefficients(msfec_rt_handle h, const double *a, const double *f,
+                                       int loc);
+
+/* All cells: the flux solves of the 4 bases and the element matrix, enqueued
+ * on stream.  Replaces RTDQBasis::run (rt_dq_basis.cc:1114-1170) with
+ * solve_direct (:582-612). */
+msfec_status msfec_rt_compute(msfec_rt_handle h, void *stream);
+msfec_status msfec_rt_synchronize(msfec_rt_handle h);
+
+/* M: n_cells x 5 x 5 (row = test basis), b: n_cells x 5, as built by
+ * RTDQBasis::assemble_global_element_matrix (rt_dq_basis.cc:822-914) and
+ * returned by get_global_element_matrix/_rhs (rt_dq_basis.cc:1096-1106). */
+msfec_status msfec_rt_get_element(msfec_rt_handle h, double *M, double *b, int loc);
+
+/* sigma: n_cells x 4 x ne, the fine fluxes of the bases (the sigma blocks of
+ * basis_div_v of RTDQBasis, renumbered as above). */
+msfec_status msfec_rt_get_basis(msfec_rt_handle h, double *sigma, int loc);
+
+/* The fine solution from the coarse weights w (n_cells x 5: 4 face weights,
+ * then the cell weight): sigma = sum_i w_i sigma_i (n_cells x ne) and
+ * u = w_4 on every fine cell (n_cells x 4^l); either output may be NULL.
+ * Replaces RTDQBasis::set_global_weights (rt_dq_basis.cc:1011-1036).
+ * Synchronizes. */
+msfec_status msfec_rt_reconstruct(msfec_rt_handle h, const double *w, int w_loc, double *sigma,
+                                  double *u, int out_loc);
+
+/* Per-cell status: MSFEC_CELL_OK, MSFEC_CELL_NOT_SPD for a zero pivot,
+ * MSFEC_CELL_NONFINITE for a non-finite element matrix. */
+msfec_status msfec_rt_get_cell_status(msfec_rt_handle h, int *status);
+
+/* Mean per-kernel times in ms (rows, factor + solves, element) of the computes
+ * made with MSFEC_FLAG_TIMING since the previous call. */
+msfec_status msfec_rt_kernel_times(msfec_rt_handle h, float *ms, int max, int *n);
+msfec_status msfec_rt_destroy(msfec_rt_handle h);
+
 /* CG iterations of the last compute, n_cells x 2^dim ints on the host
  * (basis-minor), for a handle created with MSFEC_SOLVER_CG.  The reference
  * reports CG iterations per basis solve in verbose mode

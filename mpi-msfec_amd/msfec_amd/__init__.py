@@ -1,5 +1,5 @@
-"""MI355X multiscale basis construction (MsFEC Q1 and 2-D Q1 x Nedelec pairings),
-Python face.
+"""MI355X multiscale basis construction (MsFEC Q1, and the 2-D Q1 x Nedelec and
+Raviart-Thomas x DQ0 pairings), Python face.
 
 The compute path is libmsfec_hip.so (hand-written HIP for gfx950) behind the
 C-ABI in include/msfec.h; this package only binds it (ctypes) and generates
@@ -9,6 +9,7 @@ from . import _lib
 from ._lib import MsfecError
 from .ned import NedBasisBatch
 from .qbasis import QBasisBatch
+from .rt import RTDQBasisBatch
 from .uneven import UnevenBatch
 
-__all__ = ["QBasisBatch", "NedBasisBatch", "UnevenBatch", "MsfecError", "_lib"]
+__all__ = ["QBasisBatch", "NedBasisBatch", "RTDQBasisBatch", "UnevenBatch", "MsfecError", "_lib"]

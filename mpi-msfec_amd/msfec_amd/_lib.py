@@ -33,6 +33,9 @@ EXPORTED = (
     "msfec_ned_create", "msfec_ned_set_coefficients", "msfec_ned_compute",
     "msfec_ned_synchronize", "msfec_ned_get_element", "msfec_ned_get_basis",
     "msfec_ned_get_cell_status", "msfec_ned_kernel_times", "msfec_ned_destroy",
+    "msfec_rt_create", "msfec_rt_set_coefficients", "msfec_rt_compute", "msfec_rt_synchronize",
+    "msfec_rt_get_element", "msfec_rt_get_basis", "msfec_rt_reconstruct",
+    "msfec_rt_get_cell_status", "msfec_rt_kernel_times", "msfec_rt_destroy",
     "msfec_last_error", "msfec_version",
 )
 
@@ -58,6 +61,13 @@ class NedParams(ctypes.Structure):
     _fields_ = [
         ("n_refine_local", ctypes.c_int),
         ("full_rhs", ctypes.c_int),
+        ("flags", ctypes.c_int),
+    ]
+
+
+class RtParams(ctypes.Structure):
+    _fields_ = [
+        ("n_refine_local", ctypes.c_int),
         ("flags", ctypes.c_int),
     ]
 
@@ -130,6 +140,17 @@ def load(path: str | None = None):
     L.msfec_ned_get_cell_status.argtypes = [vp, ip]
     L.msfec_ned_kernel_times.argtypes = [vp, ctypes.POINTER(ctypes.c_float), ctypes.c_int, ip]
     L.msfec_ned_destroy.argtypes = [vp]
+    L.msfec_rt_create.argtypes = [ctypes.POINTER(RtParams), ctypes.c_int, vp, ctypes.c_int,
+                                  ctypes.POINTER(vp)]
+    L.msfec_rt_set_coefficients.argtypes = [vp, vp, vp, ctypes.c_int]
+    L.msfec_rt_compute.argtypes = [vp, vp]
+    L.msfec_rt_synchronize.argtypes = [vp]
+    L.msfec_rt_get_element.argtypes = [vp, vp, vp, ctypes.c_int]
+    L.msfec_rt_get_basis.argtypes = [vp, vp, ctypes.c_int]
+    L.msfec_rt_reconstruct.argtypes = [vp, vp, ctypes.c_int, vp, vp, ctypes.c_int]
+    L.msfec_rt_get_cell_status.argtypes = [vp, ip]
+    L.msfec_rt_kernel_times.argtypes = [vp, ctypes.POINTER(ctypes.c_float), ctypes.c_int, ip]
+    L.msfec_rt_destroy.argtypes = [vp]
     L.msfec_last_error.restype = ctypes.c_char_p
     L.msfec_version.restype = ctypes.c_char_p
     for name in EXPORTED:

@@ -144,3 +144,31 @@ def test_cg_solver_limits(lib):
     assert st == lib.ERR_UNSUPPORTED and "CG" in msg
     st, msg = _create(lib, dim=3, l=5, solver=lib.SOLVER_CG)
     assert st == lib.ERR_UNSUPPORTED and "CG" in msg
+
+
+def _rt_create(lib, l=2, verts=None):
+    L = lib.load()
+    p = lib.RtParams()
+    p.n_refine_local, p.flags = l, 0
+    if verts is None:
+        verts = box_vertices(2, [0.0, 0.0], [1.0, 1.0])[None]
+    verts = np.ascontiguousarray(verts, dtype=np.float64)
+    h = ctypes.c_void_p()
+    st = L.msfec_rt_create(ctypes.byref(p), verts.shape[0], ctypes.c_void_p(verts.ctypes.data),
+                           lib.HOST, ctypes.byref(h))
+    return st, L.msfec_last_error().decode()
+
+
+def test_rt_rejects_non_box_and_bad_level(lib):
+    v = box_vertices(2, [0, 0], [1, 1])
+    v[2, 0] -= 0.1
+    st, msg = _rt_create(lib, verts=v[None])
+    assert st == lib.ERR_INVALID and "axis-aligned" in msg
+    st, msg = _rt_create(lib, l=0)
+    assert st == lib.ERR_INVALID
+    st, msg = _rt_create(lib, l=6)
+    assert st == lib.ERR_UNSUPPORTED and "band" in msg
+    L = lib.load()
+    assert L.msfec_rt_create(None, 1, None, 0, None) == lib.ERR_INVALID
+    assert L.msfec_rt_compute(None, None) == lib.ERR_INVALID
+    assert L.msfec_rt_reconstruct(None, None, 0, None, None, 0) == lib.ERR_INVALID
