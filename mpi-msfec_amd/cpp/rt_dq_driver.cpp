@@ -1,0 +1,70 @@
+// The RT_DQ cell loop over the RTDQBasis shim: a std::map of bases filled by
+// copy-insertion, run() on every entry, then the getters and
+// set_global_weights (the per-cell calls the reference's RT_DQ driver makes).
+//
+// Usage: rt_dq_driver <in.bin> <out.bin>
+//   in.bin : int32 n_refine_local, int32 nc, then a (nc^2 cells x 4^l fine
+//            cells, FP64, per-cell lexicographic); f = 1 on the unit square
+//   out.bin: M (cells x 5 x 5), b (cells x 5), the fluxes (cells x 4 x ne), and
+//            the cell solution for the weights w_i = i + 1 (cells x (ne + 4^l)),
+//            FP64
+#include <cstdint>
+#include <cstdio>
+#include <map>
+#include <memory>
+#include <vector>
+
+#include "msfec_rt_dq_basis.hpp"
+
+int main(int argc, char** argv) {
+  if (argc != 3) {
+    std::fprintf(stderr, "usage: %s in.bin out.bin\n", argv[0]);
+    return 2;
+  }
+  std::FILE* in = std::fopen(argv[1], "rb");
+  if (!in) return 2;
+  int32_t hdr[2];
+  if (std::fread(hdr, sizeof(int32_t), 2, in) != 2) return 2;
+  const int l = hdr[0], nc = hdr[1], m = 1 << l, nf = m * m, ncell = nc * nc;
+  std::vector<double> a((size_t)ncell * nf);
+  if (std::fread(a.data(), sizeof(double), a.size(), in) != a.size()) return 2;
+  std::fclose(in);
+
+  msfec::ParametersRTDQ prm;
+  prm.n_refine_local = (unsigned)l;
+  auto batch = std::make_shared<msfec::RTDQBatch>(prm);
+  std::map<int, msfec::RTDQBasis> bases;
+  const double h = 1.0 / nc;
+  const std::vector<double> f(nf, 1.0);
+  try {
+    for (int cy = 0; cy < nc; ++cy)
+      for (int cx = 0; cx < nc; ++cx) {
+        const int c = cy * nc + cx;
+        std::vector<double> corners = {cx * h, cy * h, (cx + 1) * h, cy * h,
+                                       cx * h, (cy + 1) * h, (cx + 1) * h, (cy + 1) * h};
+        std::vector<double> ac(a.begin() + (size_t)c * nf, a.begin() + (size_t)(c + 1) * nf);
+        bases.insert(std::make_pair(c, msfec::RTDQBasis(batch, corners, ac, f)));
+      }
+    for (auto& e : bases) e.second.run();
+    std::FILE* out = std::fopen(argv[2], "wb");
+    if (!out) return 2;
+    for (auto& e : bases) std::fwrite(e.second.get_global_element_matrix().data(), 8, 25, out);
+    for (auto& e : bases) std::fwrite(e.second.get_global_element_rhs().data(), 8, 5, out);
+    for (auto& e : bases)
+      for (int i = 0; i < 4; ++i) {
+        const std::vector<double> p = e.second.get_flux(i);
+        std::fwrite(p.data(), 8, p.size(), out);
+      }
+    const std::vector<double> w = {1, 2, 3, 4, 5};
+    for (auto& e : bases) {
+      e.second.set_global_weights(w);
+      const std::vector<double>& s = e.second.get_global_solution();
+      std::fwrite(s.data(), 8, s.size(), out);
+    }
+    std::fclose(out);
+  } catch (const std::exception& ex) {
+    std::fprintf(stderr, "error: %s\n", ex.what());
+    return 1;
+  }
+  return 0;
+}

@@ -83,3 +83,41 @@ def test_ned_driver(oracle_mod):
     expect = np.concatenate([np.einsum("i,cid->cd", w[:4], P[:, :4, :ns]),
                              np.einsum("i,cid->cd", w[4:], P[:, 4:, ns:])], axis=1)
     assert np.abs(sol - expect).max() < 1e-13 * np.abs(expect).max()
+
+
+RT_DRIVER = os.path.join(REPO, "mpi-msfec_amd", "cpp", "rt_dq_driver")
+
+
+def test_rt_dq_driver(oracle_mod):
+    """The C++ RTDQBasis shim: map of cells, run(), getters and
+    set_global_weights (fluxes sum_i w_i sigma_i, cell values w_4,
+    rt_dq_basis.cc:1011-1036) against the oracle."""
+    print("case rt_dq_shim_driver l=3 nc=4", flush=True)
+    from msfec_amd import fields as F
+    assert os.path.exists(RT_DRIVER), "driver not built"
+    l, nc = 3, 4
+    a = F.lognormal_coefficient(2, nc, l, 1.0, 0.0, seed=17)
+    ncell = nc * nc
+    nv, ne, nfc = oracle_mod.rt_sizes(l)
+    with tempfile.TemporaryDirectory() as td:
+        fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+        with open(fin, "wb") as fh:
+            fh.write(np.array([l, nc], dtype=np.int32).tobytes())
+            fh.write(np.ascontiguousarray(a, dtype=np.float64).tobytes())
+        r = subprocess.run([RT_DRIVER, fin, fout], capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        d = np.frombuffer(open(fout, "rb").read(), dtype=np.float64)
+    o = 0
+    M = d[o:o + ncell * 25].reshape(ncell, 5, 5); o += ncell * 25
+    b = d[o:o + ncell * 5].reshape(ncell, 5); o += ncell * 5
+    S = d[o:o + ncell * 4 * ne].reshape(ncell, 4, ne); o += ncell * 4 * ne
+    sol = d[o:o + ncell * (ne + nfc)].reshape(ncell, ne + nfc)
+    verts = F.coarse_vertices(2, nc)
+    ref = oracle_mod.rt_basis_batch(l, verts, a)
+    assert np.abs(M - ref["M"]).max() < 1e-11 * np.abs(ref["M"]).max()
+    assert np.abs(b - ref["b"]).max() < 1e-13 * np.abs(ref["b"]).max()
+    assert np.abs(S - ref["Sigma"]).max() < 1e-11 * np.abs(ref["Sigma"]).max()
+    w = np.arange(1.0, 6.0)
+    expect = np.concatenate([np.einsum("i,cie->ce", w[:4], S), np.full((ncell, nfc), w[4])],
+                            axis=1)
+    assert np.abs(sol - expect).max() < 1e-13 * np.abs(expect).max()
