@@ -1,6 +1,8 @@
 // QBasis / QBatch shim over the C-ABI (see msfec_q_basis.hpp).
 #include "msfec_q_basis.hpp"
 
+#include "msfec_vtu.hpp"
+
 #include <string>
 
 namespace msfec {
@@ -83,6 +85,9 @@ const double* QBatch::basis(std::size_t slot, int i) const {
   return psi_.data() + (slot * n_basis() + i) * (std::size_t)n_fine_nodes_;
 }
 int QBatch::cell_status(std::size_t slot) const { return status_[slot]; }
+const double* QBatch::vertices(std::size_t slot) const {
+  return verts_.data() + slot * (std::size_t)(n_basis() * prm_.dim);
+}
 
 std::vector<double> QBatch::coarse_solve(int nc, double abs_tol, double rel_tol, int max_iter,
                                          int* iterations, double* residual) {
@@ -96,13 +101,41 @@ std::vector<double> QBatch::coarse_solve(int nc, double abs_tol, double rel_tol,
 
 QBasis::QBasis(std::shared_ptr<QBatch> batch, const std::vector<double>& corner_points,
                const std::vector<double>& coef, const std::vector<double>& f,
-               const std::vector<double>& reac)
+               const std::vector<double>& reac, const std::string& cell_id)
     : batch_(std::move(batch)) {
   if (!batch_) throw std::invalid_argument("null batch");
   slot_ = batch_->add_cell(corner_points, coef, f, reac);
+  cell_id_ = cell_id.empty() ? std::to_string(slot_) : cell_id;
 }
 
-void QBasis::run() { run_called_ = true; }
+void QBasis::run() {
+  run_called_ = true;
+  const ParametersBasis& prm = batch_->parameters();
+  filename_global_ = prm.filename_output + "." + int_to_string(prm.local_subdomain, 5) +
+                     ".cell-" + cell_id_ + ".vtu";
+}
+
+void QBasis::set_output_flag() {
+  if (!batch_->parameters().prevent_output) output_flag_ = (slot_ == 0);
+}
+
+void QBasis::output_basis() const {
+  ensure();
+  const ParametersBasis& prm = batch_->parameters();
+  for (int i = 0; i < batch_->n_basis(); ++i) {
+    const std::string name = "basis_q." + int_to_string(prm.local_subdomain, 5) + ".cell-" +
+                             cell_id_ + ".index-" + int_to_string((unsigned)i, 2) + ".vtu";
+    write_vtu_box(prm.dirname_output + "/" + name, prm.dim, batch_->vertices(slot_),
+                  (int)prm.n_refine_local, {"u"}, {batch_->basis(slot_, i)});
+  }
+}
+
+void QBasis::output_global_solution_in_cell() const {
+  if (solution_.empty()) throw std::logic_error("global weights must be set first");
+  const ParametersBasis& prm = batch_->parameters();
+  write_vtu_box(prm.dirname_output + "/" + filename_global_, prm.dim, batch_->vertices(slot_),
+                (int)prm.n_refine_local, {"u"}, {solution_.data()});
+}
 
 void QBasis::ensure() const {
   if (!run_called_) throw std::logic_error("QBasis::run() must be called first");

@@ -16,6 +16,7 @@
 #include <cstddef>
 #include <memory>
 #include <stdexcept>
+#include <string>
 #include <vector>
 
 #include "msfec.h"
@@ -38,6 +39,11 @@ struct ParametersBasis {
   int max_refine = 0;  // BAND_MIXED: refinement steps at most (0 = library default)
   double cg_tol = 0.0;  // CG: relative residual (0 = library default)
   int cg_max_iter = 0;  // CG: iterations at most (0 = library default)
+  // output (ParametersBasis, include/Q/q_parameters.h:78-101)
+  std::string dirname_output = "Q";            // directory of the VTU files
+  std::string filename_output = "solution";    // stem of filename_global
+  bool prevent_output = false;
+  unsigned local_subdomain = 0;                // the rank in the file names
 };
 
 class QBatch {
@@ -65,6 +71,7 @@ class QBatch {
   const double* element_matrix(std::size_t slot) const;   // nb x nb
   const double* element_rhs(std::size_t slot) const;      // nb
   const double* basis(std::size_t slot, int i) const;     // n_fine_nodes
+  const double* vertices(std::size_t slot) const;         // 2^dim x dim
   int cell_status(std::size_t slot) const;
 
   // Coarse problem on an nc^dim structured grid whose cells were added in
@@ -91,9 +98,11 @@ class QBasis {
   // Construct from a coarse cell (corner points + its coefficient block);
   // replaces QBasis(parameters_ms, prm_file, cell, first_cell, subdomain, comm)
   // (q_basis.cc:7-56).
+  // cell_id names the cell in output file names (CellId::to_string in the
+  // reference); empty means the batch slot.
   QBasis(std::shared_ptr<QBatch> batch, const std::vector<double>& corner_points,
          const std::vector<double>& coef, const std::vector<double>& f = {},
-         const std::vector<double>& reac = {});
+         const std::vector<double>& reac = {}, const std::string& cell_id = "");
   QBasis(const QBasis& other) = default;
   QBasis& operator=(const QBasis& other) = default;
 
@@ -112,6 +121,22 @@ class QBasis {
   void set_global_weights(const std::vector<double>& weights);
   const std::vector<double>& get_global_solution() const;
 
+  // q_basis.cc:377-389, 420-426: filename_global = filename_output
+  // + "." + subdomain (5 digits) + ".cell-" + cell id + ".vtu", set by run();
+  // the output flag is raised on the batch's first cell unless
+  // prevent_output (ParametersBasis::set_output_flag, q_parameters.cc:276-281).
+  const std::string& get_filename_global() const { return filename_global_; }
+  void set_output_flag();
+  bool output_flag() const { return output_flag_; }
+  // q_basis.cc:431-476: basis_q.<subdomain>.cell-<id>.index-<i>.vtu, one file
+  // per basis, in dirname_output (the reference writes them from run() when
+  // the output flag is set; here run() precedes the batch computation, so the
+  // caller asks for them).
+  void output_basis() const;
+  // q_basis.cc:481-513: the global solution of set_global_weights as the
+  // point field "u" in dirname_output / filename_global.
+  void output_global_solution_in_cell() const;
+
   int status() const;
   std::size_t slot() const { return slot_; }
 
@@ -122,6 +147,8 @@ class QBasis {
   bool run_called_ = false;
   mutable std::vector<double> M_, b_;
   std::vector<double> weights_, solution_;
+  std::string cell_id_, filename_global_;
+  bool output_flag_ = false;
 };
 
 }  // namespace msfec

@@ -4,13 +4,17 @@
 //   by copy-insertion, then run() on every entry) -> assemble_system +
 //   solve_iterative -> send_global_weights_to_cell (set_global_weights).
 //
-// Usage: q_multiscale_driver <in.bin> <out.bin>
+// Usage: q_multiscale_driver <in.bin> <out.bin> [output directory]
 //   in.bin : int32 dim, int32 n_refine_local, int32 nc, then the coefficient
 //            blocks (nc^dim cells x (2^l)^dim fine cells, FP64, per-cell
 //            lexicographic as in msfec_q_set_coefficient)
 //   out.bin: M (cells x nb x nb), b (cells x nb), weights (cells x nb),
 //            fine solution (cells x fine nodes), all FP64, then int32 CG
 //            iterations
+//   output directory (optional): the VTU files of output_basis (first cell)
+//            and output_global_solution_in_cell (every cell), as
+//            QMultiscale::output_results has the cells write them
+//            (q_global.cc:397-428)
 #include <cstdint>
 #include <cstdio>
 #include <map>
@@ -20,8 +24,8 @@
 #include "msfec_q_basis.hpp"
 
 int main(int argc, char** argv) {
-  if (argc != 3) {
-    std::fprintf(stderr, "usage: %s in.bin out.bin\n", argv[0]);
+  if (argc != 3 && argc != 4) {
+    std::fprintf(stderr, "usage: %s in.bin out.bin [output directory]\n", argv[0]);
     return 2;
   }
   std::FILE* in = std::fopen(argv[1], "rb");
@@ -43,6 +47,7 @@ int main(int argc, char** argv) {
     msfec::ParametersBasis prm;
     prm.dim = dim;
     prm.n_refine_local = (unsigned)l;
+    if (argc == 4) prm.dirname_output = argv[3];
     auto batch = std::make_shared<msfec::QBatch>(prm);
 
     // make_grid + initialize_and_compute_basis: one QBasis per coarse cell,
@@ -92,6 +97,12 @@ int main(int argc, char** argv) {
     int32_t it32 = its;
     std::fwrite(&it32, sizeof(int32_t), 1, out);
     std::fclose(out);
+    if (argc == 4)
+      for (auto& kv : cell_basis_map) {
+        kv.second.set_output_flag();
+        if (kv.second.output_flag()) kv.second.output_basis();
+        kv.second.output_global_solution_in_cell();
+      }
     std::printf("cells %d, coarse CG iterations %d, residual %.3e\n", ncell, its, resid);
   } catch (const std::exception& e) {
     std::fprintf(stderr, "error: %s\n", e.what());

@@ -43,7 +43,7 @@ def build_driver(verbose: bool = False) -> str:
                             ("msfec_q_ned_basis.cpp", "q_ned_driver.cpp", NED_DRIVER),
                             ("msfec_rt_dq_basis.cpp", "rt_dq_driver.cpp", RT_DRIVER)):
         cmd = ["g++", "-O2", "-std=c++17", "-Wall", f"-I{INCLUDE}", f"-I{CPP}",
-               os.path.join(CPP, shim), os.path.join(CPP, main),
+               os.path.join(CPP, shim), os.path.join(CPP, "msfec_vtu.cpp"), os.path.join(CPP, main),
                f"-L{HERE}", "-l:libmsfec_hip.so", "-Wl,-rpath,$ORIGIN/../msfec_amd",
                "-o", exe + ".tmp"]
         if verbose:

@@ -44,6 +44,71 @@ def test_multiscale_driver(oracle_mod, dim, l, nc):
     assert np.abs(uf - ufo).max() < 1e-9 * np.abs(ufo).max()
 
 
+def read_vtu(path):
+    """(points (n, 3), connectivity (cells, k), types, {name: values}) of an
+    ASCII VTU file as the shim writes it."""
+    import xml.etree.ElementTree as ET
+    piece = ET.parse(path).getroot().find("UnstructuredGrid").find("Piece")
+    arr = lambda e, t: np.array(e.text.split(), dtype=t)
+    pts = arr(piece.find("Points").find("DataArray"), np.float64).reshape(-1, 3)
+    cells = {e.get("Name"): e for e in piece.find("Cells").findall("DataArray")}
+    types = arr(cells["types"], np.int64)
+    conn = arr(cells["connectivity"], np.int64).reshape(len(types), -1)
+    assert int(piece.get("NumberOfPoints")) == len(pts)
+    assert int(piece.get("NumberOfCells")) == len(types)
+    fields = {e.get("Name"): arr(e, np.float64) for e in piece.find("PointData").findall("DataArray")}
+    return pts, conn, types, fields
+
+
+@pytest.mark.parametrize("dim,l,nc", [(2, 2, 3), (3, 1, 2)])
+def test_multiscale_driver_vtu_output(oracle_mod, dim, l, nc):
+    """output_basis (first cell) and output_global_solution_in_cell (every
+    cell): file names, fine grid and the written values."""
+    print(f"case shim_driver_vtu dim={dim} l={l} nc={nc}", flush=True)
+    from msfec_amd import fields as F
+    coef = F.lognormal_coefficient(dim, nc, l, 1.0, 0.0, seed=21)
+    ncell, nv, S = nc ** dim, 1 << dim, (1 << l) + 1
+    nn = S ** dim
+    with tempfile.TemporaryDirectory() as td:
+        fin, fout, od = os.path.join(td, "in.bin"), os.path.join(td, "out.bin"), os.path.join(td, "vtu")
+        os.mkdir(od)
+        with open(fin, "wb") as fh:
+            fh.write(np.array([dim, l, nc], dtype=np.int32).tobytes())
+            fh.write(np.ascontiguousarray(coef, dtype=np.float64).tobytes())
+        r = subprocess.run([DRIVER, fin, fout, od], capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        d = np.frombuffer(open(fout, "rb").read()[:-4], dtype=np.float64)
+        uf = d[ncell * (nv * nv + 2 * nv):].reshape(ncell, nn)
+        names = sorted(os.listdir(od))
+        expect = sorted([f"basis_q.00000.cell-0.index-{i:02d}.vtu" for i in range(nv)] +
+                        [f"solution.00000.cell-{c}.vtu" for c in range(ncell)])
+        assert names == expect
+        verts = F.coarse_vertices(dim, nc)
+        ref = oracle_mod.basis_batch(oracle_mod.make_problem(dim, l), verts, coef=coef)
+        for i in range(nv):
+            pts, conn, types, fl = read_vtu(os.path.join(od, f"basis_q.00000.cell-0.index-{i:02d}.vtu"))
+            assert len(pts) == nn and np.all(types == (9 if dim == 2 else 12))
+            assert np.abs(fl["u"] - ref["Psi"][0, i]).max() < 1e-11
+            # basis i is 1 at coarse vertex i (delta property)
+            corner = verts[0, i]
+            k = np.argmin(np.abs(pts[:, :dim] - corner).sum(axis=1))
+            assert abs(fl["u"][k] - 1.0) < 1e-14
+        c = ncell - 1
+        pts, conn, types, fl = read_vtu(os.path.join(od, f"solution.00000.cell-{c}.vtu"))
+        np.testing.assert_allclose(fl["u"], uf[c], rtol=0, atol=1e-15 * np.abs(uf[c]).max())
+        lo, hi = verts[c].min(axis=0), verts[c].max(axis=0)
+        assert np.allclose(pts[:, :dim].min(axis=0), lo) and np.allclose(pts[:, :dim].max(axis=0), hi)
+        # every cell of the connectivity is an axis-aligned fine box of the right size
+        h = (hi - lo) / (S - 1)
+        ext = pts[conn][:, :, :dim].max(axis=1) - pts[conn][:, :, :dim].min(axis=1)
+        assert np.allclose(ext, h)
+        # VTK vertex order: the bottom face counter-clockwise (shoelace area > 0)
+        q = pts[conn[:, :4]][:, :, :2]
+        area = 0.5 * (q[:, :, 0] * np.roll(q[:, :, 1], -1, axis=1)
+                      - np.roll(q[:, :, 0], -1, axis=1) * q[:, :, 1]).sum(axis=1)
+        assert np.allclose(area, h[0] * h[1])
+
+
 NED_DRIVER = os.path.join(REPO, "mpi-msfec_amd", "cpp", "q_ned_driver")
 
 
