@@ -218,6 +218,37 @@ inline double ned_bval(int ns, int nh, int mi, int d, int b) {
   return ((b == 4 && i == 0) || (b == 5 && i == mi)) ? 1.0 / m : 0.0;
 }
 
+// The fine cells around fine dof d of type/(i, j) and the dof's local index
+// in each (ned_cell_dofs order); returns their number.
+__device__ __forceinline__ int ned_row_cells(int type, int i, int j, int (&ci)[4], int (&cj)[4],
+                                             int (&lp)[4]) {
+  if (type == 0) {
+    ci[0] = i - 1; cj[0] = j - 1; lp[0] = 3;
+    ci[1] = i;     cj[1] = j - 1; lp[1] = 2;
+    ci[2] = i - 1; cj[2] = j;     lp[2] = 1;
+    ci[3] = i;     cj[3] = j;     lp[3] = 0;
+    return 4;
+  }
+  if (type == 1) {
+    ci[0] = i; cj[0] = j - 1; lp[0] = 7;
+    ci[1] = i; cj[1] = j;     lp[1] = 6;
+    ci[2] = ci[3] = i; cj[2] = cj[3] = j; lp[2] = lp[3] = 6;
+    return 2;
+  }
+  ci[0] = i - 1; cj[0] = j; lp[0] = 5;
+  ci[1] = i;     cj[1] = j; lp[1] = 4;
+  ci[2] = ci[3] = i; cj[2] = cj[3] = j; lp[2] = lp[3] = 4;
+  return 2;
+}
+
+// Entry (p, k) of the local matrix, selected without dynamic register indexing.
+__device__ __forceinline__ double ned_pick(const double (&K)[8][8], int p, int k) {
+  double v = 0.0;
+#pragma unroll
+  for (int pp = 0; pp < 8; ++pp) v = (pp == p) ? K[pp][k] : v;
+  return v;
+}
+
 __global__ __launch_bounds__(kNThreads) void ned_rows_kernel(NedArgs a) {
   const int cell = blockIdx.x;
   __shared__ double ss[kMaxRows];  // row scales
@@ -230,29 +261,35 @@ __global__ __launch_bounds__(kNThreads) void ned_rows_kernel(NedArgs a) {
   const double* ca = a.a + (size_t)cell * nf;
   const double* cb = a.binv + (size_t)cell * nf;
   const double* cr = a.r + (size_t)cell * nf;
+  // the row scales s_r = |K_rr|^-1/2 first (the diagonal summed in the order
+  // the rows below accumulate it), so that every row is written once,
+  // equilibrated: entry (r, r - o) times s_r s_(r-o), right-hand sides times s_r
   for (int r = threadIdx.x; r < a.n; r += blockDim.x) {
-    const int d = a.order[r];
-    int type, i, j;
-    ned_decode(a, d, type, i, j);
-    // fine cells around the dof and its local index in each
-    int ci[4], cj[4], lp[4], nc;
-    if (type == 0) {
-      ci[0] = i - 1; cj[0] = j - 1; lp[0] = 3;
-      ci[1] = i;     cj[1] = j - 1; lp[1] = 2;
-      ci[2] = i - 1; cj[2] = j;     lp[2] = 1;
-      ci[3] = i;     cj[3] = j;     lp[3] = 0;
-      nc = 4;
-    } else if (type == 1) {
-      ci[0] = i; cj[0] = j - 1; lp[0] = 7;
-      ci[1] = i; cj[1] = j;     lp[1] = 6;
-      ci[2] = ci[3] = i; cj[2] = cj[3] = j; lp[2] = lp[3] = 6;
-      nc = 2;
-    } else {
-      ci[0] = i - 1; cj[0] = j; lp[0] = 5;
-      ci[1] = i;     cj[1] = j; lp[1] = 4;
-      ci[2] = ci[3] = i; cj[2] = cj[3] = j; lp[2] = lp[3] = 4;
-      nc = 2;
+    int type, i, j, ci[4], cj[4], lp[4];
+    ned_decode(a, a.order[r], type, i, j);
+    const int nc = ned_row_cells(type, i, j, ci, cj, lp);
+    double dg = 0.0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if (q >= nc) break;
+      const int fc = cj[q] * m + ci[q];
+      double K[8][8];
+      ned_local(geo, ca[fc], cb[fc], cr[fc], K);
+      const int p = lp[q];
+      const double sg = p < 4 ? -1.0 : 1.0;  // sigma rows negated
+      double v = 0.0;
+#pragma unroll
+      for (int pp = 0; pp < 8; ++pp) v = (pp == p) ? K[pp][pp] : v;
+      dg += sg * v;
     }
+    dg = fabs(dg);
+    ss[r] = dg > 0.0 ? 1.0 / sqrt(dg) : 1.0;
+  }
+  __syncthreads();
+  for (int r = threadIdx.x; r < a.n; r += blockDim.x) {
+    int type, i, j, ci[4], cj[4], lp[4];
+    ned_decode(a, a.order[r], type, i, j);
+    const int nc = ned_row_cells(type, i, j, ci, cj, lp);
     // slots of the couplings: byte 8q + k of the row's 32
     const int4* cw = reinterpret_cast<const int4*>(a.cslot + (size_t)r * 32);
     const int4 w0 = cw[0], w1 = cw[1];
@@ -274,13 +311,7 @@ __global__ __launch_bounds__(kNThreads) void ned_rows_kernel(NedArgs a) {
       const double sg = p < 4 ? -1.0 : 1.0;  // sigma rows negated
       double kr[8];
 #pragma unroll
-      for (int k = 0; k < 8; ++k) {
-        // row p of K, selected without dynamic register indexing
-        double v = 0.0;
-#pragma unroll
-        for (int pp = 0; pp < 8; ++pp) v = (pp == p) ? K[pp][k] : v;
-        kr[k] = sg * v;
-      }
+      for (int k = 0; k < 8; ++k) kr[k] = sg * ned_pick(K, p, k);
 #pragma unroll
       for (int k = 0; k < 8; ++k) {
         const int t = (signed char)(ws[(8 * q + k) >> 2] >> (8 * ((8 * q + k) & 3)));
@@ -295,27 +326,16 @@ __global__ __launch_bounds__(kNThreads) void ned_rows_kernel(NedArgs a) {
 #pragma unroll
         for (int n = 0; n < 4; ++n) rhs[n] += ned_source(im, cx, cy, ci[q], cj[q], n, p - 4);
     }
-    double* out = a.rows + ((size_t)cell * a.n + r) * kRowLen;
-#pragma unroll
-    for (int t = 0; t < kNLow; ++t) out[t] = sl[t];
-#pragma unroll
-    for (int t = 0; t < kNR; ++t) out[kNLow + t] = rhs[t];
-    const double dg = fabs(sl[0]);
-    ss[r] = dg > 0.0 ? 1.0 / sqrt(dg) : 1.0;
-  }
-  __syncthreads();
-  // equilibration: entry (r, r - o) times s_r s_(r-o), right-hand sides times
-  // s_r (each thread rescales the rows it wrote)
-  for (int r = threadIdx.x; r < a.n; r += blockDim.x) {
-    double* row = a.rows + ((size_t)cell * a.n + r) * kRowLen;
     const double sr = ss[r];
+    double* out = a.rows + ((size_t)cell * a.n + r) * kRowLen;
+    const int* lo = a.loff + (size_t)r * kNLow;
 #pragma unroll
     for (int t = 0; t < kNLow; ++t) {
-      const int o = a.loff[(size_t)r * kNLow + t];
-      if (o >= 0) row[t] *= sr * ss[r - o];
+      const int o = lo[t];
+      out[t] = o >= 0 ? sl[t] * (sr * ss[r - o]) : sl[t];
     }
 #pragma unroll
-    for (int t = 0; t < kNR; ++t) row[kNLow + t] *= sr;
+    for (int t = 0; t < kNR; ++t) out[kNLow + t] = rhs[t] * sr;
     a.scale[(size_t)cell * a.n + r] = sr;
   }
   for (int d = threadIdx.x; d < a.N; d += blockDim.x)
