@@ -565,6 +565,11 @@ __global__ __launch_bounds__(256) void q_rows_kernel(BandArgs a) {
   const int cell = blockIdx.x;
   extern __shared__ double dyn[];  // coefficient | reaction | load
   __shared__ double kv[8], mv[8];
+  // rows of 128 threads at a time staged for coalesced stores (row stride
+  // 25 keeps the 64-bit LDS writes of a wave at 2 lanes per bank); later the
+  // chunk buffers of the boundary sums
+  constexpr int kStage = 25;
+  __shared__ double rbuf[256 * 19];
   double* cf = dyn;
   double* rf = dyn + a.nfc;
   const bool has_rf = (a.reaction_mode == MSFEC_REACTION_CELL);
@@ -592,7 +597,11 @@ __global__ __launch_bounds__(256) void q_rows_kernel(BandArgs a) {
   __syncthreads();
   const double rc = (a.reaction_mode == MSFEC_REACTION_CONST) ? a.reaction_const : 0.0;
   const double invN = 1.0 / (double)a.N;
-  for (int jn = threadIdx.x; jn < a.n; jn += blockDim.x) {
+  const int tid = threadIdx.x;
+  for (int jn0 = 0; jn0 < a.n; jn0 += 256) {
+    const int jn = jn0 + tid;
+    double lo[16], rhs[8];
+    if (jn < a.n) {
     int c[3] = {0, 0, 0};
     {
       int t = jn;
@@ -602,7 +611,6 @@ __global__ __launch_bounds__(256) void q_rows_kernel(BandArgs a) {
         t /= a.ni;
       }
     }
-    double lo[16], rhs[8];
 #pragma unroll
     for (int t = 0; t < 16; ++t) lo[t] = 0.0;
 #pragma unroll
@@ -656,11 +664,30 @@ __global__ __launch_bounds__(256) void q_rows_kernel(BandArgs a) {
     }
     lo[14] = fj;
     lo[15] = (double)(c[0] + a.S * (c[1] + a.S * c[2]));
-    double* dst = const_cast<double*>(a.Rows) + ((size_t)cell * a.n + jn) * kRowStride;
+    }
+    // the chunk's rows are contiguous in the row table: staged by halves,
+    // then written with 16-byte stores
 #pragma unroll
-    for (int t = 0; t < 16; ++t) dst[t] = lo[t];
+    for (int half = 0; half < 2; ++half) {
+      const int r0 = jn0 + 128 * half;
+      if (r0 >= a.n) break;
+      if ((tid >> 7) == half && jn < a.n) {
+        double* st = rbuf + (tid & 127) * kStage;
 #pragma unroll
-    for (int k = 0; k < 8; ++k) dst[16 + k] = rhs[k];
+        for (int t = 0; t < 16; ++t) st[t] = lo[t];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) st[16 + k] = rhs[k];
+      }
+      __syncthreads();
+      const int cnt = min(128, a.n - r0);
+      double2* dst = reinterpret_cast<double2*>(const_cast<double*>(a.Rows) +
+                                                ((size_t)cell * a.n + r0) * kRowStride);
+      for (int idx = tid; idx < cnt * (kRowStride / 2); idx += 256) {
+        const int r = idx / (kRowStride / 2), k = 2 * (idx - r * (kRowStride / 2));
+        dst[idx] = make_double2(rbuf[r * kStage + k], rbuf[r * kStage + k + 1]);
+      }
+      __syncthreads();
+    }
   }
 
   // Boundary part of the element matrix and right-hand side, and the
@@ -669,7 +696,9 @@ __global__ __launch_bounds__(256) void q_rows_kernel(BandArgs a) {
   //   P[i][k] = sum_{m in B} g_i(m) (K_BB g_k)(m),   b_B[i] = sum_{m in B} g_i(m) F(m).
   // Nodes are taken 256 at a time; threads (i,k) < (2^dim)^2 then sum their
   // entry over the chunk.
-  __shared__ double gs[256 * 9], kps[256 * 9], fsh[256];
+  double* gs = rbuf;
+  double* kps = rbuf + 256 * 9;
+  double* fsh = rbuf + 256 * 18;
   double* psi_out = a.Psi + (size_t)cell * D::NV * a.nn;
   double vol = 1.0;
   {
@@ -677,7 +706,6 @@ __global__ __launch_bounds__(256) void q_rows_kernel(BandArgs a) {
 #pragma unroll
     for (int d = 0; d < DIM; ++d) vol *= (v[(D::NV - 1) * DIM + d] - v[d]) / a.N;
   }
-  const int tid = threadIdx.x;
   // thread (quarter qt, pair (pi, pk)) sums its quarter of every chunk:
   // 4 independent chains over 64 nodes instead of one over 256
   const int qt = tid >> 6, pi = (tid >> 3) & 7, pk = tid & 7;
