@@ -140,7 +140,11 @@ msfec_status msfec_q_set_reaction(msfec_q_handle h, const double *data,
 
 /* Compute every basis of every cell: fine assembly, Dirichlet lifting of the
  * coarse Q1 shape functions, the 2^dim local solves and the coarse element
- * matrix/RHS.  Asynchronous on `stream`.
+ * matrix/RHS.  Asynchronous on `stream`: kernel launches only (no memsets, no
+ * host synchronisation, no allocation), so one call can be captured into a
+ * HIP graph and replayed; replays read the handle's current coefficient,
+ * load and reaction buffers.  The same holds for msfec_ned_compute and
+ * msfec_rt_compute.
  * Replaces QBasis::run (source/Q/q_basis.cc:518-603), i.e. make_grid,
  * setup_system, assemble_system, the condense + solve_direct/solve_iterative
  * loop and assemble_global_element_matrix (:95-357). */

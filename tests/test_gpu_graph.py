@@ -1,0 +1,114 @@
+"""The compute entry points under HIP graph capture: msfec_q_compute,
+msfec_ned_compute and msfec_rt_compute enqueue kernels only (no memsets, no
+host synchronisation), so a stream capture of one call replays the whole
+batch.  Each graph is replayed three times with new coefficients uploaded in
+between (same device buffers), with eager work and synchronisations between
+the replays, and every replay is checked against an eager compute of the
+same data (bit for bit: the same kernels on the same inputs)."""
+import ctypes
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+class Hip:
+    def __init__(self):
+        self.L = ctypes.CDLL("libamdhip64.so")
+        vp = ctypes.c_void_p
+        for name, args in (("hipStreamCreate", [ctypes.POINTER(vp)]),
+                           ("hipStreamDestroy", [vp]),
+                           ("hipStreamSynchronize", [vp]),
+                           ("hipStreamBeginCapture", [vp, ctypes.c_int]),
+                           ("hipStreamEndCapture", [vp, ctypes.POINTER(vp)]),
+                           ("hipGraphInstantiate", [ctypes.POINTER(vp), vp, vp, vp, ctypes.c_size_t]),
+                           ("hipGraphLaunch", [vp, vp]),
+                           ("hipGraphExecDestroy", [vp]),
+                           ("hipGraphDestroy", [vp])):
+            fn = getattr(self.L, name)
+            fn.argtypes = args
+            fn.restype = ctypes.c_int
+
+    def ok(self, st, what):
+        assert st == 0, f"{what} returned {st}"
+
+    def capture(self, stream, enqueue):
+        """Graph exec of what enqueue() puts on stream."""
+        self.ok(self.L.hipStreamBeginCapture(stream, 1), "hipStreamBeginCapture")  # thread-local mode
+        enqueue()
+        g = ctypes.c_void_p()
+        self.ok(self.L.hipStreamEndCapture(stream, ctypes.byref(g)), "hipStreamEndCapture")
+        ex = ctypes.c_void_p()
+        self.ok(self.L.hipGraphInstantiate(ctypes.byref(ex), g, None, None, 0), "hipGraphInstantiate")
+        return g, ex
+
+
+def replay_check(batch, stream, set_data, results, datasets):
+    """Eager results per dataset, then a captured compute replayed per dataset."""
+    hip = Hip()
+    eager = []
+    for d in datasets:
+        set_data(d)
+        batch.run(stream.value)
+        eager.append(results())
+    set_data(datasets[0])
+    g, ex = hip.capture(stream, lambda: batch.run(stream.value))
+    try:
+        for rep, k in enumerate([0, 1, 0]):
+            set_data(datasets[k])
+            hip.ok(hip.L.hipGraphLaunch(ex, stream), "hipGraphLaunch")
+            hip.ok(hip.L.hipStreamSynchronize(stream), "hipStreamSynchronize")
+            got = results()
+            for x, y in zip(got, eager[k]):
+                np.testing.assert_array_equal(x, y, err_msg=f"replay {rep} (data {k})")
+            batch.run(stream.value)  # eager work between the replays
+            hip.ok(hip.L.hipStreamSynchronize(stream), "hipStreamSynchronize")
+    finally:
+        hip.L.hipGraphExecDestroy(ex)
+        hip.L.hipGraphDestroy(g)
+
+
+@pytest.fixture()
+def stream():
+    hip = Hip()
+    s = ctypes.c_void_p()
+    hip.ok(hip.L.hipStreamCreate(ctypes.byref(s)), "hipStreamCreate")
+    yield s
+    hip.L.hipStreamSynchronize(s)
+    hip.L.hipStreamDestroy(s)
+
+
+@pytest.mark.parametrize("solver", [0, 1, 2, 3])
+def test_q_compute_graph_replay(stream, solver):
+    print(f"case graph q solver={solver}", flush=True)
+    from msfec_amd import QBasisBatch, fields as F
+    dim, l, nc = 3, 2, 3
+    verts = F.coarse_vertices(dim, nc)
+    data = [F.lognormal_coefficient(dim, nc, l, 1.0, 0.0, seed=s) for s in (5, 6)]
+    with QBasisBatch(verts, dim=dim, n_refine_local=l, solver=solver) as qb:
+        replay_check(qb, stream, qb.set_coefficient,
+                     lambda: (qb.get_global_element_matrix(), qb.get_global_element_rhs(),
+                              qb.get_basis(), qb.get_cell_status()), data)
+
+
+def test_ned_compute_graph_replay(stream):
+    print("case graph ned", flush=True)
+    from msfec_amd import NedBasisBatch, fields as F
+    verts = F.coarse_vertices(2, 4)
+    data = [F.lognormal_coefficient(2, 4, 3, 1.0, 0.0, seed=s) for s in (7, 8)]
+    with NedBasisBatch(verts, n_refine_local=3) as nb:
+        replay_check(nb, stream, lambda a: nb.set_coefficients(a, 1.0, 0.0, (1.0, 1.0)),
+                     lambda: (nb.get_global_element_matrix(), nb.get_global_element_rhs(),
+                              nb.get_basis(), nb.get_cell_status()), data)
+
+
+def test_rt_compute_graph_replay(stream):
+    print("case graph rt", flush=True)
+    from msfec_amd import RTDQBasisBatch, fields as F
+    verts = F.coarse_vertices(2, 4)
+    data = [F.lognormal_coefficient(2, 4, 3, 1.0, 0.0, seed=s) for s in (9, 10)]
+    with RTDQBasisBatch(verts, n_refine_local=3) as rb:
+        replay_check(rb, stream, rb.set_coefficients,
+                     lambda: (rb.get_global_element_matrix(), rb.get_global_element_rhs(),
+                              rb.get_basis(), rb.get_cell_status()), data)
