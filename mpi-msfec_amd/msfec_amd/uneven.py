@@ -3,9 +3,10 @@
 One handle per refinement level present.  The local solver of a level is the
 FP64 band solver on the matrix cores where the 64-row window holds the
 interior band (3-D l <= 3, 2-D l <= 5) and matrix-free CG beyond it.  With
-``autotune=True`` the levels that both solvers can take are timed once on a
-sample of their cells and the faster solver is kept (cached per dimension and
-level for the process).  All level handles run on the same stream.
+``autotune=True`` the levels the band solvers can take are timed once on a
+sample of their cells with each FP64 solver (matrix-core band, vector-ALU
+band, CG) and the fastest is kept (cached per dimension and level for the
+process).  All level handles run on the same stream.
 """
 from __future__ import annotations
 
@@ -37,19 +38,26 @@ def _time_solver(verts, blocks, dim, level, solver, reps=3):
         return (time.perf_counter() - t0) / reps
 
 
+#: the FP64 local solvers a level may use, in the order ties are broken
+CANDIDATES = (("band_mfma", L.SOLVER_BAND_FP64_MFMA), ("band_valu", L.SOLVER_BAND_FP64),
+              ("cg", L.SOLVER_CG))
+
+
 def choose_solver(dim, level, verts=None, blocks=None, autotune=True):
-    """The solver for one level: band where it fits, CG otherwise; when both
-    fit and autotune is on, the faster of the two on the given cells."""
+    """The solver for one level: the matrix-core band solver where the band
+    fits, CG otherwise; when the band fits and autotune is on, the fastest of
+    the FP64 solvers (matrix-core band, vector-ALU band, CG) on the given
+    cells."""
     if not band_fits(dim, level):
         return L.SOLVER_CG
     if not autotune or verts is None:
         return L.SOLVER_BAND_FP64_MFMA
     key = (dim, level)
     if key not in _TUNED:
-        t_band = _time_solver(verts, blocks, dim, level, L.SOLVER_BAND_FP64_MFMA)
-        t_cg = _time_solver(verts, blocks, dim, level, L.SOLVER_CG)
-        _TUNED[key] = {"solver": L.SOLVER_BAND_FP64_MFMA if t_band <= t_cg else L.SOLVER_CG,
-                       "band_s": t_band, "cg_s": t_cg, "sample_cells": int(verts.shape[0])}
+        times = {name: _time_solver(verts, blocks, dim, level, sv) for name, sv in CANDIDATES}
+        best = min(CANDIDATES, key=lambda c: times[c[0]])
+        _TUNED[key] = {"solver": best[1], "sample_cells": int(verts.shape[0]),
+                       **{f"{name}_s": t for name, t in times.items()}}
     return _TUNED[key]["solver"]
 
 
