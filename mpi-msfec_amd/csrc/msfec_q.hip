@@ -4050,24 +4050,30 @@ msfec_status msfec_q_reconstruct(msfec_q_handle h, const double* weights, int we
   if (!h || !weights || !u_fine) return fail(MSFEC_ERR_INVALID, "null argument");
   if (!h->computed) return fail(MSFEC_ERR_STATE, "compute not called");
   const size_t nw = (size_t)h->n_cells * h->conf.nv, nu = (size_t)h->n_cells * h->conf.nn;
+  // host weights and outputs go through staging buffers that are freed on
+  // every return path
   double* dw = nullptr;
   double* du = nullptr;
   const double* wsrc = weights;
+  double* dst = u_fine;
+  hipError_t e = hipSuccess;
   if (weights_loc != MSFEC_DEVICE) {
-    MSFEC_HIP(hipMalloc((void**)&dw, nw * sizeof(double)));
-    MSFEC_HIP(hipMemcpyAsync(dw, weights, nw * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    e = hipMalloc((void**)&dw, nw * sizeof(double));
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(dw, weights, nw * sizeof(double), hipMemcpyHostToDevice, h->stream);
     wsrc = dw;
   }
-  double* dst = u_fine;
-  if (out_loc != MSFEC_DEVICE) {
-    MSFEC_HIP(hipMalloc((void**)&du, nu * sizeof(double)));
+  if (e == hipSuccess && out_loc != MSFEC_DEVICE) {
+    e = hipMalloc((void**)&du, nu * sizeof(double));
     dst = du;
   }
-  const int threads = 256;
-  const unsigned blocks = (unsigned)((nu + threads - 1) / threads);
-  hipLaunchKernelGGL(q_reconstruct_kernel, dim3(blocks), dim3(threads), 0, h->stream, h->conf.nv,
-                     h->conf.nn, h->n_cells, h->d_psi, wsrc, dst);
-  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) {
+    const int threads = 256;
+    const unsigned blocks = (unsigned)((nu + threads - 1) / threads);
+    hipLaunchKernelGGL(q_reconstruct_kernel, dim3(blocks), dim3(threads), 0, h->stream,
+                       h->conf.nv, h->conf.nn, h->n_cells, h->d_psi, wsrc, dst);
+    e = hipGetLastError();
+  }
   if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
   if (e == hipSuccess && du) e = hipMemcpy(u_fine, du, nu * sizeof(double), hipMemcpyDeviceToHost);
   if (dw) (void)hipFree(dw);
