@@ -2,7 +2,7 @@
  * FP64 CPU oracle for the 2-D multiscale Q1 x Nedelec (Q_Ned) basis.
  *
  * TEST INFRASTRUCTURE ONLY, like msfec_oracle.h: only tests/, smoke() and the
- * cpu_baseline leg of bench.py load it, as the checker.
+ * cpu_baseline leg of bench.py load it, to check results against.
  *
  * The reference (source/Q_Ned/q_ned_basis.cc) is three-dimensional.  This is
  * its two-dimensional restatement on one axis-aligned coarse box:

@@ -1,7 +1,7 @@
 """ctypes front end of the FP64 CPU oracle (oracle/msfec_oracle.c).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, ``__graft_entry__.smoke()`` and
-the cpu_baseline leg of bench.py, always as the checker (or, for the CPU
+the cpu_baseline leg of bench.py, always as the results to compare with (or, for the CPU
 baseline, as the thing timed beside the GPU path).  The product library never
 imports it.
 """
