@@ -93,6 +93,10 @@ def lib():
         L.oq_rt_basis_cell.restype = ctypes.c_int
         L.oq_rt_assemble.argtypes = [ctypes.c_int, dp, dp, dp, dp, dp]
         L.oq_rt_assemble.restype = ctypes.c_int
+        L.oq_rt_basis_cell_qp.argtypes = [ctypes.c_int, dp, dp, dp, dp, dp, dp, dp]
+        L.oq_rt_basis_cell_qp.restype = ctypes.c_int
+        L.oq_rt_assemble_qp.argtypes = [ctypes.c_int, dp, dp, dp, dp, dp]
+        L.oq_rt_assemble_qp.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -311,53 +315,60 @@ def rt_sizes(l):
     return nv, 2 * nv, m * m
 
 
-def _rt_coefs(l, a, f):
+def _rt_coefs(l, a, f, tensor=False):
     nf = (1 << l) ** 2
-    a = np.ascontiguousarray(np.broadcast_to(np.asarray(a, dtype=np.float64), (nf,)))
+    shp = (nf, 4, 3) if tensor else (nf,)
+    a = np.ascontiguousarray(np.broadcast_to(np.asarray(a, dtype=np.float64), shp))
     f = np.ascontiguousarray(np.broadcast_to(np.asarray(f, dtype=np.float64), (nf,)))
     return a, f
 
 
-def rt_basis_cell(l, verts, a, f=1.0):
-    """One coarse box cell: returns (M 5x5, b 5, Sigma 4 x ne, U 4 x nc, status)."""
-    a, f = _rt_coefs(l, a, f)
+def rt_basis_cell(l, verts, a, f=1.0, tensor=False):
+    """One coarse box cell: returns (M 5x5, b 5, Sigma 4 x ne, U 4 x nc, status).
+    a: one value per fine cell, or with tensor=True (xx, xy, yy) at the 4
+    Gauss points of every fine cell (m*m x 4 x 3)."""
+    a, f = _rt_coefs(l, a, f, tensor)
     v = np.ascontiguousarray(verts, dtype=np.float64).reshape(4, 2)
     _, ne, nc = rt_sizes(l)
     M = np.zeros((5, 5))
     b = np.zeros(5)
     Sig = np.zeros((4, ne))
     U = np.zeros((4, nc))
-    st = lib().oq_rt_basis_cell(l, _dp(v), _dp(a), _dp(f), _dp(M), _dp(b), _dp(Sig), _dp(U))
+    fn = lib().oq_rt_basis_cell_qp if tensor else lib().oq_rt_basis_cell
+    st = fn(l, _dp(v), _dp(a), _dp(f), _dp(M), _dp(b), _dp(Sig), _dp(U))
     return M, b, Sig, U, st
 
 
-def rt_basis_batch(l, verts, a, f=1.0):
-    """Cells of verts (n, 4, 2) with per-cell coefficient blocks a, f (n, m*m)."""
+def rt_basis_batch(l, verts, a, f=1.0, tensor=False):
+    """Cells of verts (n, 4, 2) with per-cell coefficient blocks a (n, m*m, or
+    n, m*m, 4, 3 with tensor=True) and f (n, m*m)."""
     verts = np.asarray(verts, dtype=np.float64)
     n = verts.shape[0]
     nf = (1 << l) ** 2
 
-    def per_cell(x):
+    def per_cell(x, shp):
         x = np.asarray(x, dtype=np.float64)
-        return np.broadcast_to(x, (n, nf)) if x.ndim <= 1 else x.reshape(n, nf)
+        return np.broadcast_to(x, (n,) + shp) if x.ndim <= len(shp) else x.reshape((n,) + shp)
 
-    a_, f_ = per_cell(a), per_cell(f)
+    a_ = per_cell(a, (nf, 4, 3) if tensor else (nf,))
+    f_ = per_cell(f, (nf,))
     out = {"M": [], "b": [], "Sigma": [], "U": [], "status": []}
     for c in range(n):
-        M, b, S, U, st = rt_basis_cell(l, verts[c], a_[c], f_[c])
+        M, b, S, U, st = rt_basis_cell(l, verts[c], a_[c], f_[c], tensor)
         for k, v in zip(("M", "b", "Sigma", "U", "status"), (M, b, S, U, st)):
             out[k].append(v)
     return {k: np.array(v) for k, v in out.items()}
 
 
-def rt_assemble(l, verts, a, f=1.0):
+def rt_assemble(l, verts, a, f=1.0, tensor=False):
     """The assembled mixed system over all fine dofs (row = test) and the load."""
-    a, f = _rt_coefs(l, a, f)
+    a, f = _rt_coefs(l, a, f, tensor)
     v = np.ascontiguousarray(verts, dtype=np.float64).reshape(4, 2)
     _, ne, nc = rt_sizes(l)
     N = ne + nc
     A = np.zeros((N, N))
     F = np.zeros(N)
-    st = lib().oq_rt_assemble(l, _dp(v), _dp(a), _dp(f), _dp(A), _dp(F))
+    fn = lib().oq_rt_assemble_qp if tensor else lib().oq_rt_assemble
+    st = fn(l, _dp(v), _dp(a), _dp(f), _dp(A), _dp(F))
     assert st == 0
     return A, F

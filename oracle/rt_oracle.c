@@ -18,7 +18,11 @@
  *     solve satisfies the first equation with the recomputed assembled
  *     matrix; the flux block equals the one of an independent stream-function
  *     solve written in the test; the complementary energy of each basis is at
- *     most that of the coarse function.
+ *     most that of the coarse function;
+ *   - tensors at the Gauss points: a I given as a tensor reproduces the scalar
+ *     mode, and random rotated anisotropic tensors match the test's
+ *     stream-function solve with the same point values and satisfy the
+ *     saddle-point equations.
  */
 #include "rt_oracle.h"
 
@@ -85,16 +89,37 @@ static void rt_shape(double s, double t, double hx, double hy, double (*phi)[2],
   }
 }
 
-/* the local matrix (test, trial) of assemble_system on one fine cell with
- * inverse diffusion ainv (rt_dq_basis.cc:495-503) and the local load
- * (rt_dq_basis.cc:507) */
-static void rt_local(const rt_mesh *g, double ainv, double f, double K[5][5], double Fl[5]) {
+/* A^-1 at Gauss point qp of fine cell c: a I per fine cell (tensor == 0, a
+ * holds one value per cell) or the inverse of the symmetric tensor
+ * (xx, xy, yy) given at the point (tensor == 1, a holds 4 x 3 per cell),
+ * which is what DiffusionInverse_A::value_list returns there
+ * (source/equation_data/eqn_coeff_A.cc:194-242) */
+static void rt_ainv(const double *a, int tensor, int c, int qp, double B[2][2]) {
+  if (!tensor) {
+    const double v = 1.0 / a[c];
+    B[0][0] = v;
+    B[1][1] = v;
+    B[0][1] = B[1][0] = 0.0;
+    return;
+  }
+  const double *t = a + ((size_t)c * 4 + qp) * 3;
+  const double det = t[0] * t[2] - t[1] * t[1];
+  B[0][0] = t[2] / det;
+  B[1][1] = t[0] / det;
+  B[0][1] = B[1][0] = -t[1] / det;
+}
+
+/* the local matrix (test, trial) of assemble_system on one fine cell
+ * (rt_dq_basis.cc:495-503) and the local load (rt_dq_basis.cc:507) */
+static void rt_local(const rt_mesh *g, const double *a, int tensor, int c, double f,
+                     double K[5][5], double Fl[5]) {
   memset(K, 0, sizeof(double) * 25);
   memset(Fl, 0, sizeof(double) * 5);
   for (int qy = 0; qy < 2; ++qy)
     for (int qx = 0; qx < 2; ++qx) {
-      double phi[4][2], div[4];
+      double phi[4][2], div[4], B[2][2];
       rt_shape(kRtGx[qx], kRtGx[qy], g->hx, g->hy, phi, div);
+      rt_ainv(a, tensor, c, qx + 2 * qy, B);
       const double jxw = kRtGw[qx] * kRtGw[qy] * g->hx * g->hy;
       /* local dof k < 4: sigma shape k, u value 0; k = 4: sigma 0, u 1 */
       for (int i = 0; i < 5; ++i) {
@@ -103,15 +128,16 @@ static void rt_local(const rt_mesh *g, double ainv, double f, double K[5][5], do
         for (int j = 0; j < 5; ++j) {
           const double sj0 = j < 4 ? phi[j][0] : 0.0, sj1 = j < 4 ? phi[j][1] : 0.0;
           const double dj = j < 4 ? div[j] : 0.0, uj = j == 4 ? 1.0 : 0.0;
-          K[i][j] += (ainv * (si0 * sj0 + si1 * sj1) - di * uj + ui * dj) * jxw;
+          const double aa = si0 * (B[0][0] * sj0 + B[0][1] * sj1) + si1 * (B[1][0] * sj0 + B[1][1] * sj1);
+          K[i][j] += (aa - di * uj + ui * dj) * jxw;
         }
         Fl[i] += ui * f * jxw;
       }
     }
 }
 
-int oq_rt_assemble(int l, const double *verts, const double *a, const double *f, double *A,
-                   double *F) {
+static int rt_assemble(int l, const double *verts, const double *a, int tensor, const double *f,
+                       double *A, double *F) {
   rt_mesh g;
   if (rt_mesh_init(l, verts, &g)) return 1;
   const int N = g.N;
@@ -121,7 +147,7 @@ int oq_rt_assemble(int l, const double *verts, const double *a, const double *f,
     for (int p = 0; p < g.m; ++p) {
       const int c = q * g.m + p;
       double K[5][5], Fl[5];
-      rt_local(&g, 1.0 / a[c], f ? f[c] : 1.0, K, Fl);
+      rt_local(&g, a, tensor, c, f ? f[c] : 1.0, K, Fl);
       int d[5];
       rt_cell_dofs(&g, p, q, d);
       for (int i = 0; i < 5; ++i) {
@@ -208,8 +234,18 @@ static int rt_on_boundary(const rt_mesh *g, int e) {
   return q == 0 || q == g->m;
 }
 
-int oq_rt_basis_cell(int l, const double *verts, const double *a, const double *f, double *M,
-                     double *b, double *Sigma, double *U) {
+int oq_rt_assemble(int l, const double *verts, const double *a, const double *f, double *A,
+                   double *F) {
+  return rt_assemble(l, verts, a, 0, f, A, F);
+}
+
+int oq_rt_assemble_qp(int l, const double *verts, const double *a_qp, const double *f, double *A,
+                      double *F) {
+  return rt_assemble(l, verts, a_qp, 1, f, A, F);
+}
+
+static int rt_basis_cell(int l, const double *verts, const double *a, int tensor, const double *f,
+                         double *M, double *b, double *Sigma, double *U) {
   rt_mesh g;
   if (rt_mesh_init(l, verts, &g)) return 1;
   const int N = g.N, ne = g.ne, nc = g.nc;
@@ -224,7 +260,7 @@ int oq_rt_basis_cell(int l, const double *verts, const double *a, const double *
     free(sol);
     return 3;
   }
-  oq_rt_assemble(l, verts, a, f, A, F);
+  rt_assemble(l, verts, a, tensor, f, A, F);
   /* free unknowns: interior edges, then every cell but the last, whose
    * balance equation is the sum of the others (the pure Neumann problem
    * fixes u up to a constant; its value is pinned to 0 and the mean removed
@@ -302,4 +338,14 @@ int oq_rt_basis_cell(int l, const double *verts, const double *a, const double *
   free(pos);
   free(sol);
   return st;
+}
+
+int oq_rt_basis_cell(int l, const double *verts, const double *a, const double *f, double *M,
+                     double *b, double *Sigma, double *U) {
+  return rt_basis_cell(l, verts, a, 0, f, M, b, Sigma, U);
+}
+
+int oq_rt_basis_cell_qp(int l, const double *verts, const double *a_qp, const double *f,
+                        double *M, double *b, double *Sigma, double *U) {
+  return rt_basis_cell(l, verts, a_qp, 1, f, M, b, Sigma, U);
 }

@@ -18,8 +18,9 @@
  *   solve_direct          rt_dq_basis.cc:582-612
  *   set_u_to_std          rt_dq_basis.cc:1039-1043 u block of every basis := 1
  *   assemble_global_element_matrix  rt_dq_basis.cc:822-914
- * ReactionRate is zero (source/equation_data/eqn_coeff_R.cc:8-14) and A is
- * a I on each fine cell, so DiffusionInverse_A is 1/a.
+ * ReactionRate is zero (source/equation_data/eqn_coeff_R.cc:8-14).  A is
+ * either a I on each fine cell (DiffusionInverse_A is then 1/a) or a
+ * symmetric tensor given at the quadrature points.
  */
 #ifndef RT_ORACLE_H
 #define RT_ORACLE_H
@@ -49,10 +50,18 @@ extern "C" {
 int oq_rt_basis_cell(int l, const double *verts, const double *a, const double *f, double *M,
                      double *b, double *Sigma, double *U);
 
+/* The same with the diffusion tensor A = (xx, xy, yy) at the 4 QGauss(2)
+ * points of every fine cell (a_qp: m^2 x 4 x 3, points x fastest); the
+ * solve uses its inverse at each point. */
+int oq_rt_basis_cell_qp(int l, const double *verts, const double *a_qp, const double *f,
+                        double *M, double *b, double *Sigma, double *U);
+
 /* The assembled matrix of assemble_system over all fine dofs (ne + m^2 rows,
  * row = test function) and the load (v, f) (zero on the edges). */
 int oq_rt_assemble(int l, const double *verts, const double *a, const double *f, double *A,
                    double *F);
+int oq_rt_assemble_qp(int l, const double *verts, const double *a_qp, const double *f, double *A,
+                      double *F);
 
 #ifdef __cplusplus
 }

@@ -39,51 +39,62 @@ def coarse_fluxes(l):
     return F
 
 
+GAUSS = np.array([0.5 - 0.5 / np.sqrt(3), 0.5 + 0.5 / np.sqrt(3)])
+
+
+def ainv_at(a, c, qx, qy):
+    """A^-1 at Gauss point (qx, qy) of fine cell c: a scalar per cell, or the
+    inverse of the tensor (xx, xy, yy) given per point (a of shape (nf, 4, 3))."""
+    a = np.asarray(a)
+    if a.ndim == 1:
+        return np.eye(2) / a[c]
+    xx, xy, yy = a[c, qx + 2 * qy]
+    return np.linalg.inv(np.array([[xx, xy], [xy, yy]]))
+
+
 def rt_mass(l, a, S):
-    """sum over fine cells of int sigma_i . sigma_j / a, by 2-point Gauss."""
+    """sum over fine cells of int sigma_i . A^-1 sigma_j, by 2-point Gauss."""
     m = 1 << l
     nv = m * (m + 1)
     hx, hy = HX / m, HY / m
-    g = np.array([0.5 - 0.5 / np.sqrt(3), 0.5 + 0.5 / np.sqrt(3)])
     M = np.zeros((S.shape[0], S.shape[0]))
     for q in range(m):
         for p in range(m):
             fl, fr = S[:, q * (m + 1) + p], S[:, q * (m + 1) + p + 1]
             fb, ft = S[:, nv + q * m + p], S[:, nv + (q + 1) * m + p]
-            for s in g:
-                for t in g:
-                    sx = (fl * (1 - s) + fr * s) / hy
-                    sy = (fb * (1 - t) + ft * t) / hx
-                    M += 0.25 * hx * hy * (np.outer(sx, sx) + np.outer(sy, sy)) / a[q * m + p]
+            for qy, t in enumerate(GAUSS):
+                for qx, s in enumerate(GAUSS):
+                    sig = np.stack([(fl * (1 - s) + fr * s) / hy, (fb * (1 - t) + ft * t) / hx], 1)
+                    M += 0.25 * hx * hy * sig @ ainv_at(a, q * m + p, qx, qy) @ sig.T
     return M
 
 
 def stream_function_fluxes(l, a):
     """sigma_n = coarse_n + curl psi, psi in Q1 with zero boundary values minimising
-    int |coarse_n + curl psi|^2 / a (the divergence-free correction)."""
+    int (coarse_n + curl psi) . A^-1 (coarse_n + curl psi) (the divergence-free
+    correction)."""
     m = 1 << l
     hx, hy = HX / m, HY / m
     nn = (m + 1) ** 2
-    g = np.array([0.5 - 0.5 / np.sqrt(3), 0.5 + 0.5 / np.sqrt(3)])
     K = np.zeros((nn, nn))
     R = np.zeros((nn, 4))
     for q in range(m):
         for p in range(m):
-            kap = 1.0 / a[q * m + p]
             nodes = [q * (m + 1) + p, q * (m + 1) + p + 1, (q + 1) * (m + 1) + p,
                      (q + 1) * (m + 1) + p + 1]
-            for s in g:
-                for t in g:
-                    w = 0.25 * hx * hy * kap
+            for qy, t in enumerate(GAUSS):
+                for qx, s in enumerate(GAUSS):
+                    w = 0.25 * hx * hy
+                    B = ainv_at(a, q * m + p, qx, qy)
                     phx = np.array([-(1 - t), 1 - t, -t, t]) / hx
                     phy = np.array([-(1 - s), -s, 1 - s, s]) / hy
                     curl = np.stack([phy, -phx], axis=1)  # (d_y phi, -d_x phi)
-                    K[np.ix_(nodes, nodes)] += w * curl @ curl.T
+                    K[np.ix_(nodes, nodes)] += w * curl @ B @ curl.T
                     X = (p + s) / m
                     Y = (q + t) / m
                     sc = np.array([[(1 - X) / HY, 0], [X / HY, 0], [0, (1 - Y) / HX],
                                    [0, Y / HX]])
-                    R[nodes, :] -= w * curl @ sc.T
+                    R[nodes, :] -= w * curl @ B @ sc.T
     inner = [j * (m + 1) + i for j in range(1, m) for i in range(1, m)]
     psi = np.zeros((nn, 4))
     if inner:
@@ -200,3 +211,55 @@ def test_rejects_non_box():
     v = VERTS.copy()
     v[3, 0] += 0.01
     assert O.rt_basis_cell(2, v, 1.0)[4] == 1
+
+
+def random_tensors(l, seed, sigma=1.0):
+    """Symmetric positive definite (xx, xy, yy) at every Gauss point: lognormal
+    eigenvalues, random rotation per point."""
+    rng = np.random.default_rng(seed)
+    nf = (1 << l) ** 2
+    lam = np.exp(sigma * rng.normal(size=(nf, 4, 2)))
+    th = rng.uniform(0, np.pi, size=(nf, 4))
+    c, s = np.cos(th), np.sin(th)
+    xx = lam[..., 0] * c * c + lam[..., 1] * s * s
+    yy = lam[..., 0] * s * s + lam[..., 1] * c * c
+    xy = (lam[..., 0] - lam[..., 1]) * c * s
+    return np.stack([xx, xy, yy], axis=-1)
+
+
+@pytest.mark.parametrize("l", [1, 3])
+def test_tensor_mode_with_isotropic_tensor_equals_scalar(l):
+    a = lognormal(l, 20 + l, 1.5)
+    t = np.zeros(((1 << l) ** 2, 4, 3))
+    t[:, :, 0] = a[:, None]
+    t[:, :, 2] = a[:, None]
+    Ms, bs, Ss, Us, st0 = O.rt_basis_cell(l, VERTS, a)
+    Mt, bt, St, Ut, st1 = O.rt_basis_cell(l, VERTS, t, tensor=True)
+    assert st0 == 0 and st1 == 0
+    np.testing.assert_allclose(St, Ss, rtol=0, atol=1e-13 * np.abs(Ss).max())
+    np.testing.assert_allclose(Mt, Ms, rtol=1e-13, atol=1e-15)
+
+
+@pytest.mark.parametrize("l,seed", [(2, 30), (3, 31), (4, 32)])
+def test_tensor_mode_matches_stream_function_solve(l, seed):
+    t = random_tensors(l, seed)
+    M, b, S, U, st = O.rt_basis_cell(l, VERTS, t, tensor=True)
+    assert st == 0
+    Sref = stream_function_fluxes(l, t)
+    np.testing.assert_allclose(S, Sref, rtol=0, atol=1e-11 * np.abs(Sref).max())
+    np.testing.assert_allclose(M[:4, :4], rt_mass(l, t, Sref), rtol=1e-11, atol=1e-14)
+    np.testing.assert_allclose(M[:4, :4], M[:4, :4].T, rtol=1e-12, atol=1e-15)
+    Mc = rt_mass(l, t, coarse_fluxes(l))
+    assert np.all(np.diag(M[:4, :4]) <= np.diag(Mc) * (1 + 1e-12))
+    # the saddle-point equations with the recomputed assembled matrix
+    m = 1 << l
+    nv, ne, nc = O.rt_sizes(l)
+    A, Fl = O.rt_assemble(l, VERTS, t, tensor=True)
+    bnd = set([q * (m + 1) + p for q in range(m) for p in (0, m)] +
+              [nv + q * m + p for q in (0, m) for p in range(m)])
+    inner = [e for e in range(ne) if e not in bnd]
+    sgn = np.array([-1.0, 1.0, -1.0, 1.0])
+    for n in range(4):
+        r = A @ np.concatenate([S[n], U[n]])
+        assert np.abs(r[inner]).max() < 1e-12 * np.abs(A).max()
+        np.testing.assert_allclose(r[ne:], sgn[n] / m ** 2, atol=1e-12)
