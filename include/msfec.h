@@ -282,8 +282,9 @@ msfec_status msfec_ned_destroy(msfec_ned_handle h);
  * RT_DQ: the mixed lowest-order Raviart-Thomas x DQ0 multiscale basis,
  * two-dimensional (the reference's RTDQBasis, source/RT_DQ/rt_dq_basis.cc, is
  * its 3-D form).  Per coarse box cell and fine cell (2^l x 2^l,
- * lexicographic, x fastest): a (scalar diffusion; the solve uses 1/a like
- * DiffusionInverse_A) and f (load).  4 bases, one per coarse face in deal.II
+ * lexicographic, x fastest): a (scalar diffusion, or the diffusion tensor at
+ * the Gauss points; the solve uses its inverse like DiffusionInverse_A) and
+ * f (load).  4 bases, one per coarse face in deal.II
  * face order (x = x0, x = x1, y = y0, y = y1): the boundary flux is that of
  * the coarse Raviart-Thomas function of the face and the divergence -+1/|K|
  * (is_laplace, rt_dq_basis.cc:516-538).  Fine degrees of freedom of a basis
@@ -297,6 +298,11 @@ msfec_status msfec_ned_destroy(msfec_ned_handle h);
 typedef struct {
   int n_refine_local; /* l, 1..5 (the reduced system's half-bandwidth 2^l <= 60) */
   int flags;          /* MSFEC_FLAG_*                                          */
+  int coef_mode;      /* MSFEC_COEF_SCALAR_CELL: a per fine cell (A = a I);
+                         MSFEC_COEF_TENSOR_QP: the symmetric A (xx, xy, yy) at
+                         the 4 QGauss(2) points of each fine cell, points
+                         x fastest (what Diffusion_A gives there); the solve
+                         inverts it at each point like DiffusionInverse_A */
 } msfec_rt_params;
 
 typedef struct msfec_rt_batch *msfec_rt_handle;
@@ -308,9 +314,10 @@ msfec_status msfec_rt_create(const msfec_rt_params *params, int n_cells,
                              const double *vertices, int vertices_loc,
                              msfec_rt_handle *out);
 
-/* Per-fine-cell data, n_cells x 4^l doubles each; f may be NULL (f = 1).
- * Stands for the DiffusionInverse_A and RightHandSideParsed evaluations of
- * RTDQBasis::assemble_system (rt_dq_basis.cc:423-461). */
+/* Per-fine-cell data: a, n_cells x 4^l doubles (MSFEC_COEF_SCALAR_CELL) or
+ * n_cells x 4^l x 4 x 3 (MSFEC_COEF_TENSOR_QP); f, n_cells x 4^l, may be NULL
+ * (f = 1).  Stands for the DiffusionInverse_A and RightHandSideParsed
+ * evaluations of RTDQBasis::assemble_system (rt_dq_basis.cc:423-461). */
 msfec_status msfec_rt_set_coefficients(msfec_rt_handle h, const double *a, const double *f,
                                        int loc);
 

@@ -25,8 +25,9 @@ RTDQBatch::~RTDQBatch() {
 std::size_t RTDQBatch::add_cell(const std::vector<double>& vertices, const std::vector<double>& a,
                                 const std::vector<double>& f) {
   const std::size_t nf = (std::size_t)n_fine_cells_;
+  const std::size_t na = nf * (prm_.coef_mode == MSFEC_COEF_TENSOR_QP ? 12 : 1);
   if (vertices.size() != 8) throw std::invalid_argument("a cell has 4 vertices x 2 coordinates");
-  if (a.size() != nf || f.size() != nf) throw std::invalid_argument("per-fine-cell data of the wrong size");
+  if (a.size() != na || f.size() != nf) throw std::invalid_argument("per-fine-cell data of the wrong size");
   verts_.insert(verts_.end(), vertices.begin(), vertices.end());
   a_.insert(a_.end(), a.begin(), a.end());
   f_.insert(f_.end(), f.begin(), f.end());
@@ -44,6 +45,7 @@ void RTDQBatch::compute() {
   msfec_rt_params p{};
   p.n_refine_local = (int)prm_.n_refine_local;
   p.flags = 0;
+  p.coef_mode = prm_.coef_mode;
   check(msfec_rt_create(&p, (int)n_cells_, verts_.data(), MSFEC_HOST, &handle_), "msfec_rt_create");
   check(msfec_rt_set_coefficients(handle_, a_.data(), f_.data(), MSFEC_HOST),
         "msfec_rt_set_coefficients");

@@ -17,6 +17,9 @@ namespace msfec {
 
 struct ParametersRTDQ {
   unsigned n_refine_local = 4;  // ParametersMs::n_refine_local
+  // MSFEC_COEF_SCALAR_CELL (a per fine cell) or MSFEC_COEF_TENSOR_QP
+  // ((xx, xy, yy) of A at the 4 Gauss points of every fine cell)
+  int coef_mode = MSFEC_COEF_SCALAR_CELL;
 };
 
 class RTDQBatch {
@@ -27,7 +30,8 @@ class RTDQBatch {
   RTDQBatch& operator=(const RTDQBatch&) = delete;
 
   // Register a cell: 4 vertices x 2 coordinates (deal.II order), its
-  // diffusion a and load f per fine cell (4^l values each).
+  // diffusion a (4^l values, or 4^l x 4 x 3 in tensor mode) and load f per
+  // fine cell (4^l values).
   std::size_t add_cell(const std::vector<double>& vertices, const std::vector<double>& a,
                        const std::vector<double>& f);
   void compute();

@@ -28,13 +28,18 @@
 //   rt_rows_kernel     one workgroup per cell, one thread per interior node:
 //                      its row of the Q1 system (diagonal, then the lower
 //                      neighbours left, bottom-right, bottom, bottom-left) and
-//                      the 4 right-hand sides, from 1/a staged in LDS.
+//                      the 4 right-hand sides, from exact integrals with 1/a
+//                      staged in LDS, or, for a tensor A given at the
+//                      QGauss(2) points, by that rule with A^-1 at each point.
 //   band_ldl_solve     (msfec_q.hip) one wave per cell: the matrix-core band
 //                      L D L^T, 4 pivots per step, with the forward
 //                      elimination and the backward sweep.
 //   rt_element_kernel  one workgroup per cell: psi in LDS, the fine fluxes of
 //                      the 4 bases to HBM and the 5 x 5 element matrix and
-//                      load from exact integrals per fine cell.
+//                      load from exact integrals per fine cell (QGauss(2)
+//                      with A^-1 at each point for a tensor A).
+// With a tensor A the curl form becomes (curl phi, A^-1 curl psi), still
+// symmetric positive definite; both sides use the reference's quadrature.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -68,9 +73,11 @@ struct RtArgs {
   int m;     // fine cells per direction
   int n;     // interior nodes (m - 1)^2
   int ne;    // fine edges 2 m (m + 1)
+  int tensor;  // 0: a I per fine cell; 1: the tensor (xx, xy, yy) at the 4 Gauss points
+               // (the kernels are instantiated per mode)
   const int* loff;      // [n][kNLow] row - column offsets of the lower entries, -1 unused
   const double* verts;  // [cells][4][2], deal.II vertex order
-  const double* a;      // [cells][m * m] diffusion per fine cell
+  const double* a;      // [cells][m * m] diffusion per fine cell, or [cells][m * m][4][3]
   const double* f;      // [cells][m * m] load per fine cell
   double* rows;         // [cells][n][kRowLen]
   const double* X;      // [cells][n][kNR] psi of the 4 bases at the interior nodes
@@ -104,6 +111,32 @@ __device__ __forceinline__ double rt_cflux_h(int nb, int Q, int m) {
   return nb == 2 ? (1.0 - Y) / m : (nb == 3 ? Y / m : 0.0);
 }
 
+// QGauss(2) on [0, 1]: the points; each of the 4 points of a fine box
+// carries the weight hx hy / 4.
+__constant__ double kRtGauss[2] = {0.21132486540518711775, 0.78867513459481288225};
+
+// A^-1 at Gauss point qp of fine cell fc (tensor mode): the inverse of the
+// symmetric (xx, xy, yy), as DiffusionInverse_A gives it there
+// (source/equation_data/eqn_coeff_A.cc:194-242).  B = (b00, b01, b11).
+__device__ __forceinline__ void rt_ainv_qp(const double* ac, int fc, int qp, double& b00,
+                                           double& b01, double& b11) {
+  const double* t = ac + ((size_t)fc * 4 + qp) * 3;
+  const double xx = t[0], xy = t[1], yy = t[2];
+  const double id = 1.0 / (xx * yy - xy * xy);
+  b00 = yy * id;
+  b01 = -xy * id;
+  b11 = xx * id;
+}
+
+// curl (d_y phi, -d_x phi) of the bilinear function of corner (ex, ey) at
+// (s, t) of a fine box (hx, hy)
+__device__ __forceinline__ void rt_corner_curl(int ex, int ey, double s, double t, double hx,
+                                               double hy, double& cx, double& cy) {
+  const double lx = ex ? s : 1.0 - s, ly = ey ? t : 1.0 - t;
+  cx = (ey ? 1.0 : -1.0) * lx / hy;
+  cy = -(ex ? 1.0 : -1.0) * ly / hx;
+}
+
 // Row of interior node (i, j) in the Q1 system with coefficient k = 1/a.
 // Exact integrals on a fine box (hx, hy), al = hy / hx, be = hx / hy:
 //   diagonal          sum of the 4 cells  k (al + be) / 3
@@ -114,16 +147,81 @@ __device__ __forceinline__ double rt_cflux_h(int nb, int Q, int m) {
 // above-right of the node.  Right-hand side of basis nb:
 // -(curl phi, k c_nb) summed over the 4 cells, with the 1-D exact integrals
 // of a linear function against the hat function's two halves.
+template <bool TENSOR>
 __global__ __launch_bounds__(kNThreads) void rt_rows_kernel(RtArgs a) {
-  extern __shared__ double kap[];  // [m * m] 1 / a
+  extern __shared__ double kap[];  // [m * m] 1 / a (scalar mode)
   const int cell = blockIdx.x, m = a.m, nf = m * m;
-  const double* ac = a.a + (size_t)cell * nf;
-  for (int c = threadIdx.x; c < nf; c += blockDim.x) kap[c] = 1.0 / ac[c];
+  if (!TENSOR) {
+    const double* ac = a.a + (size_t)cell * nf;
+    for (int c = threadIdx.x; c < nf; c += blockDim.x) kap[c] = 1.0 / ac[c];
+  }
   __syncthreads();
   const RtBox bx = rt_box(a, cell);
   const double al = bx.hy / bx.hx, be = bx.hx / bx.hy;
   const double Hx = bx.hx * m, Hy = bx.hy * m;
   const int mi = m - 1;
+  if (TENSOR) {
+    // the same row by QGauss(2) with A^-1 at each point: per cell around the
+    // node and point, w curl(phi_node)^T B curl(phi_other) for the lower
+    // neighbours and -w curl(phi_node)^T B c_n for the right-hand sides
+    const double* ac = a.a + (size_t)cell * nf * 12;
+    const double w = 0.25 * bx.hx * bx.hy;
+    for (int r = threadIdx.x; r < a.n; r += blockDim.x) {
+      const int i = r % mi + 1, j = r / mi + 1;
+      double vd = 0.0, vl = 0.0, vbr = 0.0, vb = 0.0, vbl = 0.0;
+      double rh[kNB] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int dy = 0; dy < 2; ++dy)
+#pragma unroll
+        for (int dx = 0; dx < 2; ++dx) {
+          const int ci = i - 1 + dx, cj = j - 1 + dy, fc = cj * m + ci;
+          const int bxn = 1 - dx, byn = 1 - dy;  // the node's corner in this cell
+#pragma unroll
+          for (int qp = 0; qp < 4; ++qp) {
+            const double sq = kRtGauss[qp & 1], tq = kRtGauss[qp >> 1];
+            double b00, b01, b11;
+            rt_ainv_qp(ac, fc, qp, b00, b01, b11);
+            double cx, cy;
+            rt_corner_curl(bxn, byn, sq, tq, bx.hx, bx.hy, cx, cy);
+            const double ux = w * (b00 * cx + b01 * cy), uy = w * (b01 * cx + b11 * cy);
+#pragma unroll
+            for (int ey = 0; ey < 2; ++ey)
+#pragma unroll
+              for (int ex = 0; ex < 2; ++ex) {
+                double ox, oy;
+                rt_corner_curl(ex, ey, sq, tq, bx.hx, bx.hy, ox, oy);
+                const double v = ux * ox + uy * oy;
+                const int di = ex - bxn, dj = ey - byn;
+                if (di == 0 && dj == 0) vd += v;
+                else if (di == -1 && dj == 0) vl += v;
+                else if (di == 1 && dj == -1) vbr += v;
+                else if (di == 0 && dj == -1) vb += v;
+                else if (di == -1 && dj == -1) vbl += v;
+              }
+            const double X = (ci + sq) / m, Y = (cj + tq) / m;
+            rh[0] -= ux * (1.0 - X) / Hy;
+            rh[1] -= ux * X / Hy;
+            rh[2] -= uy * (1.0 - Y) / Hx;
+            rh[3] -= uy * Y / Hx;
+          }
+        }
+      double out[kRowLen];
+#pragma unroll
+      for (int t = 0; t < kRowLen; ++t) out[t] = 0.0;
+      out[0] = vd;
+      int t = 1;
+      if (i > 1) out[t++] = vl;
+      if (j > 1 && i < mi) out[t++] = vbr;
+      if (j > 1) out[t++] = vb;
+      if (j > 1 && i > 1) out[t++] = vbl;
+#pragma unroll
+      for (int nb = 0; nb < kNB; ++nb) out[kNLow + nb] = rh[nb];
+      double* dst = a.rows + ((size_t)cell * a.n + r) * kRowLen;
+#pragma unroll
+      for (int q = 0; q < kRowLen; ++q) dst[q] = out[q];
+    }
+    return;
+  }
   for (int r = threadIdx.x; r < a.n; r += blockDim.x) {
     const int i = r % mi + 1, j = r / mi + 1;
     const double k00 = kap[(j - 1) * m + i - 1], k10 = kap[(j - 1) * m + i];
@@ -165,6 +263,7 @@ __global__ __launch_bounds__(kNThreads) void rt_rows_kernel(RtArgs a) {
 // Each thread takes every 256th fine cell: its 4 edge fluxes per basis from
 // psi in LDS, the exact mass integral of the two linear flux components and
 // the out-flux; the sums go across lanes and waves.
+template <bool TENSOR>
 __global__ __launch_bounds__(kNThreads) void rt_element_kernel(RtArgs a) {
   extern __shared__ double ps[];  // [4][(m + 1)^2] psi on every node, 0 on the boundary
   __shared__ double part[kNThreads / 64][16];
@@ -202,7 +301,7 @@ __global__ __launch_bounds__(kNThreads) void rt_element_kernel(RtArgs a) {
   const int nf = m * m;
   for (int c = tid; c < nf; c += kNThreads) {
     const int p = c % m, q = c / m;
-    const double kap = 1.0 / a.a[(size_t)cell * nf + c];
+    const double kap = TENSOR ? 0.0 : 1.0 / a.a[(size_t)cell * nf + c];
     double FL[kNB], FR[kNB], FB[kNB], FT[kNB];
 #pragma unroll
     for (int nb = 0; nb < kNB; ++nb) {
@@ -214,15 +313,41 @@ __global__ __launch_bounds__(kNThreads) void rt_element_kernel(RtArgs a) {
       FB[nb] = rt_cflux_h(nb, q, m) - (s10 - s00);
       FT[nb] = rt_cflux_h(nb, q + 1, m) - (s11 - s01);
     }
-    int k = 0;
+    if (TENSOR) {
+      // QGauss(2) with A^-1 at each point: sigma_i there is
+      // ((FL (1 - s) + FR s) / hy, (FB (1 - t) + FT t) / hx)
+      const double* ac = a.a + (size_t)cell * nf * 12;
+      const double w = 0.25 * bx.hx * bx.hy;
 #pragma unroll
-    for (int i = 0; i < kNB; ++i)
+      for (int qp = 0; qp < 4; ++qp) {
+        const double sq = kRtGauss[qp & 1], tq = kRtGauss[qp >> 1];
+        double b00, b01, b11;
+        rt_ainv_qp(ac, c, qp, b00, b01, b11);
+        double sx[kNB], sy[kNB];
 #pragma unroll
-      for (int j = i; j < kNB; ++j) {
-        const double xx = (FL[i] * FL[j] + FR[i] * FR[j]) / 3.0 + (FL[i] * FR[j] + FR[i] * FL[j]) / 6.0;
-        const double yy = (FB[i] * FB[j] + FT[i] * FT[j]) / 3.0 + (FB[i] * FT[j] + FT[i] * FB[j]) / 6.0;
-        acc[k++] += kap * (rx * xx + ry * yy);
+        for (int nb = 0; nb < kNB; ++nb) {
+          sx[nb] = (FL[nb] * (1.0 - sq) + FR[nb] * sq) / bx.hy;
+          sy[nb] = (FB[nb] * (1.0 - tq) + FT[nb] * tq) / bx.hx;
+        }
+        int k = 0;
+#pragma unroll
+        for (int i = 0; i < kNB; ++i) {
+          const double ux = w * (b00 * sx[i] + b01 * sy[i]), uy = w * (b01 * sx[i] + b11 * sy[i]);
+#pragma unroll
+          for (int j = i; j < kNB; ++j) acc[k++] += ux * sx[j] + uy * sy[j];
+        }
       }
+    } else {
+      int k = 0;
+#pragma unroll
+      for (int i = 0; i < kNB; ++i)
+#pragma unroll
+        for (int j = i; j < kNB; ++j) {
+          const double xx = (FL[i] * FL[j] + FR[i] * FR[j]) / 3.0 + (FL[i] * FR[j] + FR[i] * FL[j]) / 6.0;
+          const double yy = (FB[i] * FB[j] + FT[i] * FT[j]) / 3.0 + (FB[i] * FT[j] + FT[i] * FB[j]) / 6.0;
+          acc[k++] += kap * (rx * xx + ry * yy);
+        }
+    }
 #pragma unroll
     for (int nb = 0; nb < kNB; ++nb) acc[10 + nb] += (FR[nb] - FL[nb]) + (FT[nb] - FB[nb]);
     acc[14] += a.f[(size_t)cell * nf + c] * bx.hx * bx.hy;
@@ -364,6 +489,7 @@ RtArgs rt_args(const msfec_rt_batch* h) {
   a.m = h->m;
   a.n = h->n;
   a.ne = h->ne;
+  a.tensor = h->prm.coef_mode == MSFEC_COEF_TENSOR_QP ? 1 : 0;
   a.loff = h->d_loff;
   a.verts = h->d_verts;
   a.a = h->d_a;
@@ -410,6 +536,8 @@ msfec_status msfec_rt_create(const msfec_rt_params* params, int n_cells, const d
                    "n_refine_local above 5: the band of the reduced system exceeds the band window");
   if (vertices_loc != MSFEC_HOST && vertices_loc != MSFEC_DEVICE)
     return rt_fail(MSFEC_ERR_INVALID, "bad vertices location");
+  if (params->coef_mode != MSFEC_COEF_SCALAR_CELL && params->coef_mode != MSFEC_COEF_TENSOR_QP)
+    return rt_fail(MSFEC_ERR_INVALID, "coef_mode must be MSFEC_COEF_SCALAR_CELL or MSFEC_COEF_TENSOR_QP");
   std::vector<double> hv((size_t)n_cells * 8);
   if (vertices_loc == MSFEC_DEVICE) {
     RT_HIP(hipMemcpy(hv.data(), vertices, hv.size() * sizeof(double), hipMemcpyDeviceToHost));
@@ -446,7 +574,7 @@ msfec_status msfec_rt_create(const msfec_rt_params* params, int n_cells, const d
     }                     \
   } while (0)
   RT_TRY(rt_alloc(&h->d_verts, nc * 8));
-  RT_TRY(rt_alloc(&h->d_a, nc * nf));
+  RT_TRY(rt_alloc(&h->d_a, nc * nf * (params->coef_mode == MSFEC_COEF_TENSOR_QP ? 12 : 1)));
   RT_TRY(rt_alloc(&h->d_f, nc * nf));
   RT_TRY(rt_alloc(&h->d_rows, nc * h->n * kRowLen));
   RT_TRY(rt_alloc(&h->d_pan, nc * msfec_detail::band_ldl_panel_doubles(h->n)));
@@ -473,7 +601,8 @@ msfec_status msfec_rt_set_coefficients(msfec_rt_handle h, const double* a, const
   if (!h || !a) return rt_fail(MSFEC_ERR_INVALID, "null argument");
   if (loc != MSFEC_HOST && loc != MSFEC_DEVICE) return rt_fail(MSFEC_ERR_INVALID, "bad location");
   const size_t cnt = (size_t)h->n_cells * h->m * h->m;
-  msfec_status st = rt_in(h->d_a, a, cnt, loc);
+  const size_t acnt = cnt * (h->prm.coef_mode == MSFEC_COEF_TENSOR_QP ? 12 : 1);
+  msfec_status st = rt_in(h->d_a, a, acnt, loc);
   if (st != MSFEC_OK) return st;
   if (f) {
     st = rt_in(h->d_f, f, cnt, loc);
@@ -506,14 +635,20 @@ msfec_status msfec_rt_compute(msfec_rt_handle h, void* stream) {
   const size_t rows_lds = (size_t)h->m * h->m * sizeof(double);
   const size_t elem_lds = (size_t)kNB * (h->m + 1) * (h->m + 1) * sizeof(double);
   if (h->n > 0) {
-    hipLaunchKernelGGL(rt_rows_kernel, dim3(h->n_cells), dim3(kNThreads), rows_lds, s, a);
+    if (a.tensor)
+      hipLaunchKernelGGL(rt_rows_kernel<true>, dim3(h->n_cells), dim3(kNThreads), 0, s, a);
+    else
+      hipLaunchKernelGGL(rt_rows_kernel<false>, dim3(h->n_cells), dim3(kNThreads), rows_lds, s, a);
     RT_HIP(hipGetLastError());
   }
   if (ev) RT_HIP(hipEventRecord((*ev)[1], s));
   RT_HIP(msfec_detail::band_ldl_solve(h->n_cells, h->n, h->d_rows, h->d_loff, h->d_pan, h->d_zst,
                                       h->d_X, h->d_status, s));
   if (ev) RT_HIP(hipEventRecord((*ev)[2], s));
-  hipLaunchKernelGGL(rt_element_kernel, dim3(h->n_cells), dim3(kNThreads), elem_lds, s, a);
+  if (a.tensor)
+    hipLaunchKernelGGL(rt_element_kernel<true>, dim3(h->n_cells), dim3(kNThreads), elem_lds, s, a);
+  else
+    hipLaunchKernelGGL(rt_element_kernel<false>, dim3(h->n_cells), dim3(kNThreads), elem_lds, s, a);
   RT_HIP(hipGetLastError());
   if (ev) RT_HIP(hipEventRecord((*ev)[3], s));
   h->computed = true;

@@ -69,6 +69,7 @@ class RtParams(ctypes.Structure):
     _fields_ = [
         ("n_refine_local", ctypes.c_int),
         ("flags", ctypes.c_int),
+        ("coef_mode", ctypes.c_int),
     ]
 
 

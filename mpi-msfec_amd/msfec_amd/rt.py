@@ -26,12 +26,16 @@ def rt_sizes(n_refine_local):
 
 
 class RTDQBasisBatch:
-    def __init__(self, vertices, n_refine_local=4, timing=False):
+    def __init__(self, vertices, n_refine_local=4, timing=False, coef_mode=L.COEF_SCALAR_CELL):
+        """coef_mode: COEF_SCALAR_CELL (a per fine cell) or COEF_TENSOR_QP
+        ((xx, xy, yy) of A at the 4 Gauss points of every fine cell)."""
         self._lib = L.load()
         self._h = ctypes.c_void_p()
         p = L.RtParams()
         p.n_refine_local = n_refine_local
         p.flags = L.FLAG_TIMING if timing else 0
+        p.coef_mode = coef_mode
+        self.coef_mode = coef_mode
         vptr, vloc, keep = _ptr(vertices)
         n_cells = int(np.prod(np.shape(vertices))) // 8
         L.check(self._lib.msfec_rt_create(ctypes.byref(p), n_cells, vptr, vloc,
@@ -41,20 +45,22 @@ class RTDQBasisBatch:
         _, self.ne, self.nc = rt_sizes(n_refine_local)
 
     def set_coefficients(self, a, f=None):
-        """Diffusion a and load f per fine cell (n_cells x 4^l; scalars broadcast,
-        f None means 1)."""
+        """Diffusion a (n_cells x 4^l, or n_cells x 4^l x 4 x 3 in tensor mode)
+        and load f per fine cell (n_cells x 4^l); scalars broadcast, f None
+        means 1."""
         shp = (self.n_cells, self.m * self.m)
+        ashp = shp + ((4, 3) if self.coef_mode == L.COEF_TENSOR_QP else ())
 
-        def full(x):
+        def full(x, s):
             if hasattr(x, "is_cuda") and x.is_cuda:
                 return x
-            return np.ascontiguousarray(np.broadcast_to(np.asarray(x, dtype=np.float64), shp))
+            return np.ascontiguousarray(np.broadcast_to(np.asarray(x, dtype=np.float64), s))
 
-        aptr, aloc, akeep = _ptr(full(a))
+        aptr, aloc, akeep = _ptr(full(a, ashp))
         if f is None:
             L.check(self._lib.msfec_rt_set_coefficients(self._h, aptr, None, aloc))
             return
-        fptr, floc, fkeep = _ptr(full(f))
+        fptr, floc, fkeep = _ptr(full(f, shp))
         if aloc != floc:
             raise ValueError("a and f must live on the same side")
         L.check(self._lib.msfec_rt_set_coefficients(self._h, aptr, fptr, aloc))

@@ -146,10 +146,10 @@ def test_cg_solver_limits(lib):
     assert st == lib.ERR_UNSUPPORTED and "CG" in msg
 
 
-def _rt_create(lib, l=2, verts=None):
+def _rt_create(lib, l=2, verts=None, coef_mode=0):
     L = lib.load()
     p = lib.RtParams()
-    p.n_refine_local, p.flags = l, 0
+    p.n_refine_local, p.flags, p.coef_mode = l, 0, coef_mode
     if verts is None:
         verts = box_vertices(2, [0.0, 0.0], [1.0, 1.0])[None]
     verts = np.ascontiguousarray(verts, dtype=np.float64)
@@ -168,6 +168,8 @@ def test_rt_rejects_non_box_and_bad_level(lib):
     assert st == lib.ERR_INVALID
     st, msg = _rt_create(lib, l=6)
     assert st == lib.ERR_UNSUPPORTED and "band" in msg
+    st, msg = _rt_create(lib, coef_mode=7)
+    assert st == lib.ERR_INVALID and "coef_mode" in msg
     L = lib.load()
     assert L.msfec_rt_create(None, 1, None, 0, None) == lib.ERR_INVALID
     assert L.msfec_rt_compute(None, None) == lib.ERR_INVALID

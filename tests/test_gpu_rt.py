@@ -29,9 +29,10 @@ def boxes(rng, n):
     return v
 
 
-def run_gpu(verts, l, a, f=None):
-    from msfec_amd import RTDQBasisBatch
-    with RTDQBasisBatch(verts, n_refine_local=l) as rb:
+def run_gpu(verts, l, a, f=None, tensor=False):
+    from msfec_amd import RTDQBasisBatch, _lib as L
+    mode = L.COEF_TENSOR_QP if tensor else L.COEF_SCALAR_CELL
+    with RTDQBasisBatch(verts, n_refine_local=l, coef_mode=mode) as rb:
         rb.set_coefficients(a, f)
         rb.run()
         return (rb.get_global_element_matrix(), rb.get_global_element_rhs(), rb.get_basis(),
@@ -158,3 +159,36 @@ def test_rt_nonfinite_coefficient_reported():
     a[1, 5] = np.nan
     M, b, S, st = run_gpu(verts, 2, a)
     assert st[0] == 0 and st[2] == 0 and st[1] != 0
+
+
+@pytest.mark.parametrize("l", [1, 2, 3, 4])
+def test_rt_tensor_matches_oracle(oracle_mod, l):
+    """A rotated anisotropic tensor at every Gauss point (lognormal eigenvalues)."""
+    print(f"case rt tensor l={l}", flush=True)
+    rng = np.random.default_rng(400 + l)
+    n = 5
+    verts = boxes(rng, n)
+    t = np.stack([R.random_tensors(l, 500 + 10 * l + c) for c in range(n)])
+    f = rng.uniform(0.5, 2.0, size=(n, (1 << l) ** 2))
+    M, b, S, st = run_gpu(verts, l, t, f, tensor=True)
+    assert np.all(st == 0), st
+    o = oracle_mod.rt_basis_batch(l, verts, t, f, tensor=True)
+    assert np.all(o["status"] == 0)
+    compare(M, b, S, o)
+
+
+def test_rt_tensor_isotropic_equals_scalar():
+    print("case rt tensor isotropic l=3", flush=True)
+    l = 3
+    rng = np.random.default_rng(77)
+    verts = boxes(rng, 4)
+    a = np.exp(1.5 * rng.normal(size=(4, 64)))
+    t = np.zeros((4, 64, 4, 3))
+    t[..., 0] = a[..., None]
+    t[..., 2] = a[..., None]
+    Ms, bs, Ss, sts = run_gpu(verts, l, a)
+    Mt, bt, St, stt = run_gpu(verts, l, t, tensor=True)
+    assert np.all(sts == 0) and np.all(stt == 0)
+    np.testing.assert_allclose(St, Ss, rtol=0, atol=1e-13 * np.abs(Ss).max())
+    np.testing.assert_allclose(Mt, Ms, rtol=1e-12, atol=1e-14)
+    np.testing.assert_array_equal(bt, bs)
