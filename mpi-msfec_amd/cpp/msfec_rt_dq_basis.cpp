@@ -3,6 +3,8 @@
 
 #include <string>
 
+#include "msfec_vtu.hpp"
+
 namespace msfec {
 
 namespace {
@@ -66,15 +68,67 @@ const double* RTDQBatch::flux(std::size_t slot, int i) const {
   return &sigma_[(slot * 4 + (std::size_t)i) * n_edges_];
 }
 int RTDQBatch::cell_status(std::size_t slot) const { return status_[slot]; }
+const double* RTDQBatch::vertices(std::size_t slot) const { return &verts_[slot * 8]; }
+
+namespace {
+// The flux field at the fine cell centres (3 components, z = 0) from the
+// fine edge fluxes: the mean of the two opposite fluxes over the edge length.
+std::vector<double> cell_centre_flux(const double* flux, const double* verts, int l) {
+  const int m = 1 << l, nv = m * (m + 1);
+  const double hx = (verts[2] - verts[0]) / m, hy = (verts[5] - verts[1]) / m;
+  std::vector<double> out((size_t)m * m * 3, 0.0);
+  for (int q = 0; q < m; ++q)
+    for (int p = 0; p < m; ++p) {
+      double* o = &out[((size_t)q * m + p) * 3];
+      o[0] = 0.5 * (flux[q * (m + 1) + p] + flux[q * (m + 1) + p + 1]) / hy;
+      o[1] = 0.5 * (flux[nv + q * m + p] + flux[nv + (q + 1) * m + p]) / hx;
+    }
+  return out;
+}
+}  // namespace
 
 RTDQBasis::RTDQBasis(std::shared_ptr<RTDQBatch> batch, const std::vector<double>& corner_points,
-                     const std::vector<double>& a, const std::vector<double>& f)
+                     const std::vector<double>& a, const std::vector<double>& f,
+                     const std::string& cell_id)
     : batch_(std::move(batch)) {
   if (!batch_) throw std::invalid_argument("null batch");
   slot_ = batch_->add_cell(corner_points, a, f);
+  cell_id_ = cell_id.empty() ? std::to_string(slot_) : cell_id;
 }
 
-void RTDQBasis::run() { run_called_ = true; }
+void RTDQBasis::run() {
+  run_called_ = true;
+  const ParametersRTDQ& prm = batch_->parameters();
+  filename_global_ = prm.filename_output + "." + int_to_string(prm.local_subdomain, 5) +
+                     ".cell-" + cell_id_ + ".vtu";
+}
+
+void RTDQBasis::set_output_flag() {
+  if (!batch_->parameters().prevent_output) output_flag_ = (slot_ == 0);
+}
+
+void RTDQBasis::output_basis() const {
+  ensure();
+  const ParametersRTDQ& prm = batch_->parameters();
+  const int l = (int)prm.n_refine_local;
+  const std::vector<double> ones((size_t)batch_->n_fine_cells(), 1.0);
+  for (int i = 0; i < 4; ++i) {
+    const std::vector<double> sig = cell_centre_flux(batch_->flux(slot_, i), batch_->vertices(slot_), l);
+    const std::string name = "basis_rt-dq.div." + int_to_string(prm.local_subdomain, 5) + ".cell-" +
+                             cell_id_ + ".index-" + int_to_string((unsigned)i, 2) + ".vtu";
+    write_vtu_box(prm.dirname_output + "/" + name, 2, batch_->vertices(slot_), l, {}, {},
+                  {{"sigma", 3, sig.data()}, {"u", 1, ones.data()}});
+  }
+}
+
+void RTDQBasis::output_global_solution_in_cell() const {
+  if (solution_.empty()) throw std::logic_error("global weights must be set first");
+  const ParametersRTDQ& prm = batch_->parameters();
+  const int l = (int)prm.n_refine_local;
+  const std::vector<double> sig = cell_centre_flux(solution_.data(), batch_->vertices(slot_), l);
+  write_vtu_box(prm.dirname_output + "/" + filename_global_, 2, batch_->vertices(slot_), l, {}, {},
+                {{"sigma", 3, sig.data()}, {"u", 1, solution_.data() + batch_->n_fluxes()}});
+}
 
 void RTDQBasis::ensure() const {
   if (!run_called_) throw std::logic_error("run() was not called on this cell");

@@ -9,6 +9,7 @@
 #include <cstddef>
 #include <memory>
 #include <stdexcept>
+#include <string>
 #include <vector>
 
 #include "msfec.h"
@@ -20,6 +21,11 @@ struct ParametersRTDQ {
   // MSFEC_COEF_SCALAR_CELL (a per fine cell) or MSFEC_COEF_TENSOR_QP
   // ((xx, xy, yy) of A at the 4 Gauss points of every fine cell)
   int coef_mode = MSFEC_COEF_SCALAR_CELL;
+  // output (the ParametersBasis fields of include/RT_DQ/rt_dq_parameters.h)
+  std::string dirname_output = "RT_DQ";
+  std::string filename_output = "solution";
+  bool prevent_output = false;
+  unsigned local_subdomain = 0;
 };
 
 class RTDQBatch {
@@ -42,6 +48,8 @@ class RTDQBatch {
   const double* element_matrix(std::size_t slot) const;  // 5 x 5
   const double* element_rhs(std::size_t slot) const;     // 5
   const double* flux(std::size_t slot, int i) const;     // n_fluxes, basis i < 4
+  const double* vertices(std::size_t slot) const;        // 4 x 2
+  const ParametersRTDQ& parameters() const { return prm_; }
   int cell_status(std::size_t slot) const;
 
  private:
@@ -60,8 +68,11 @@ class RTDQBasis {
   RTDQBasis() = delete;
   // Replaces RTDQBasis(parameters_ms, prm_file, cell, first_cell, subdomain,
   // comm) (rt_dq_basis.cc:7-68): the cell's data is handed over explicitly.
+  // cell_id names the cell in output file names (CellId::to_string in the
+  // reference); empty means the batch slot.
   RTDQBasis(std::shared_ptr<RTDQBatch> batch, const std::vector<double>& corner_points,
-            const std::vector<double>& a, const std::vector<double>& f);
+            const std::vector<double>& a, const std::vector<double>& f,
+            const std::string& cell_id = "");
   RTDQBasis(const RTDQBasis& other) = default;
   RTDQBasis& operator=(const RTDQBasis& other) = default;
 
@@ -78,6 +89,20 @@ class RTDQBasis {
   const std::vector<double>& get_global_solution() const;
   int status() const;
 
+  // rt_dq_basis.cc:1088-1110: filename_global = filename_output + "." +
+  // subdomain (5 digits) + ".cell-" + cell id + ".vtu", set by run(); the
+  // output flag is raised on the batch's first cell unless prevent_output.
+  const std::string& get_filename_global() const { return filename_global_; }
+  void set_output_flag();
+  bool output_flag() const { return output_flag_; }
+  // rt_dq_basis.cc:916-970: basis_rt-dq.div.<subdomain>.cell-<id>.index-<i>.vtu
+  // per face basis in dirname_output, with the cell fields "sigma" (the flux
+  // field at the fine cell centres) and "u" (1, set_u_to_std).
+  void output_basis() const;
+  // rt_dq_basis.cc:973-1002: the global solution of set_global_weights
+  // ("sigma", "u") in dirname_output / filename_global.
+  void output_global_solution_in_cell() const;
+
  private:
   void ensure() const;
   std::shared_ptr<RTDQBatch> batch_;
@@ -85,6 +110,8 @@ class RTDQBasis {
   bool run_called_ = false;
   mutable std::vector<double> M_, b_;
   std::vector<double> solution_;
+  std::string cell_id_, filename_global_;
+  bool output_flag_ = false;
 };
 
 }  // namespace msfec

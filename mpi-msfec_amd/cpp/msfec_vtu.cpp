@@ -14,6 +14,12 @@ std::string int_to_string(unsigned value, unsigned digits) {
 
 void write_vtu_box(const std::string& path, int dim, const double* corners, int n_refine,
                    const std::vector<std::string>& names, const std::vector<const double*>& fields) {
+  write_vtu_box(path, dim, corners, n_refine, names, fields, {});
+}
+
+void write_vtu_box(const std::string& path, int dim, const double* corners, int n_refine,
+                   const std::vector<std::string>& names, const std::vector<const double*>& fields,
+                   const std::vector<CellField>& cell_fields) {
   if (dim != 2 && dim != 3) throw std::invalid_argument("dim must be 2 or 3");
   if (names.size() != fields.size()) throw std::invalid_argument("one name per field");
   const int m = 1 << n_refine, S = m + 1;
@@ -64,7 +70,25 @@ void write_vtu_box(const std::string& path, int dim, const double* corners, int 
     for (long p = 0; p < np; ++p) std::fprintf(f, "%.17g\n", fields[a][p]);
     std::fprintf(f, "</DataArray>\n");
   }
-  std::fprintf(f, "</PointData>\n</Piece>\n</UnstructuredGrid>\n</VTKFile>\n");
+  std::fprintf(f, "</PointData>\n");
+  if (!cell_fields.empty()) {
+    std::fprintf(f, "<CellData>\n");
+    for (const CellField& cf : cell_fields) {
+      if (cf.ncomp != 1 && cf.ncomp != 3) {
+        std::fclose(f);
+        throw std::invalid_argument("cell fields have 1 or 3 components");
+      }
+      std::fprintf(f, "<DataArray type=\"Float64\" Name=\"%s\" NumberOfComponents=\"%d\" format=\"ascii\">\n",
+                   cf.name.c_str(), cf.ncomp);
+      for (long c = 0; c < nc; ++c) {
+        for (int k = 0; k < cf.ncomp; ++k) std::fprintf(f, k ? " %.17g" : "%.17g", cf.values[c * cf.ncomp + k]);
+        std::fprintf(f, "\n");
+      }
+      std::fprintf(f, "</DataArray>\n");
+    }
+    std::fprintf(f, "</CellData>\n");
+  }
+  std::fprintf(f, "</Piece>\n</UnstructuredGrid>\n</VTKFile>\n");
   const bool ok = std::ferror(f) == 0;
   if (std::fclose(f) != 0 || !ok) throw std::runtime_error("error writing " + path);
 }

@@ -18,6 +18,19 @@ namespace msfec {
 void write_vtu_box(const std::string& path, int dim, const double* corners, int n_refine,
                    const std::vector<std::string>& names, const std::vector<const double*>& fields);
 
+// A field given per fine cell (lexicographic, x fastest) with ncomp
+// components per cell (1: scalar, 3: vector).
+struct CellField {
+  std::string name;
+  int ncomp;
+  const double* values;
+};
+
+// The same with cell fields beside (or instead of) the point fields.
+void write_vtu_box(const std::string& path, int dim, const double* corners, int n_refine,
+                   const std::vector<std::string>& names, const std::vector<const double*>& fields,
+                   const std::vector<CellField>& cell_fields);
+
 // deal.II's Utilities::int_to_string(value, digits): zero-padded decimal.
 std::string int_to_string(unsigned value, unsigned digits);
 

@@ -186,3 +186,51 @@ def test_rt_dq_driver(oracle_mod):
     expect = np.concatenate([np.einsum("i,cie->ce", w[:4], S), np.full((ncell, nfc), w[4])],
                             axis=1)
     assert np.abs(sol - expect).max() < 1e-13 * np.abs(expect).max()
+
+
+def read_vtu_cells(path):
+    """{name: (cells, ncomp) values} of the CellData of an ASCII VTU file."""
+    import xml.etree.ElementTree as ET
+    piece = ET.parse(path).getroot().find("UnstructuredGrid").find("Piece")
+    out = {}
+    for e in piece.find("CellData").findall("DataArray"):
+        k = int(e.get("NumberOfComponents", "1"))
+        out[e.get("Name")] = np.array(e.text.split(), dtype=np.float64).reshape(-1, k)
+    return out
+
+
+def test_rt_dq_driver_vtu_output(oracle_mod):
+    """RTDQBasis output_basis (first cell) and output_global_solution_in_cell:
+    file names, the flux field at the fine cell centres and u."""
+    print("case rt_dq_shim_driver_vtu l=2 nc=3", flush=True)
+    from msfec_amd import fields as F
+    l, nc = 2, 3
+    m = 1 << l
+    a = F.lognormal_coefficient(2, nc, l, 1.0, 0.0, seed=23)
+    ncell = nc * nc
+    nv, ne, nfc = oracle_mod.rt_sizes(l)
+    with tempfile.TemporaryDirectory() as td:
+        fin, fout, od = os.path.join(td, "in.bin"), os.path.join(td, "out.bin"), os.path.join(td, "vtu")
+        os.mkdir(od)
+        with open(fin, "wb") as fh:
+            fh.write(np.array([l, nc], dtype=np.int32).tobytes())
+            fh.write(np.ascontiguousarray(a, dtype=np.float64).tobytes())
+        r = subprocess.run([RT_DRIVER, fin, fout, od], capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        names = sorted(os.listdir(od))
+        expect = sorted([f"basis_rt-dq.div.00000.cell-0.index-{i:02d}.vtu" for i in range(4)] +
+                        [f"solution.00000.cell-{c}.vtu" for c in range(ncell)])
+        assert names == expect
+        verts = F.coarse_vertices(2, nc)
+        ref = oracle_mod.rt_basis_batch(l, verts, a)
+        h = 1.0 / nc / m
+        for i in range(4):
+            cf = read_vtu_cells(os.path.join(od, f"basis_rt-dq.div.00000.cell-0.index-{i:02d}.vtu"))
+            S = ref["Sigma"][0, i]
+            sx = 0.5 * (S[:nv].reshape(m, m + 1)[:, :-1] + S[:nv].reshape(m, m + 1)[:, 1:]) / h
+            sy = 0.5 * (S[nv:].reshape(m + 1, m)[:-1] + S[nv:].reshape(m + 1, m)[1:]) / h
+            np.testing.assert_allclose(cf["sigma"][:, 0], sx.ravel(), rtol=1e-11, atol=1e-11)
+            np.testing.assert_allclose(cf["sigma"][:, 1], sy.ravel(), rtol=1e-11, atol=1e-11)
+            assert np.all(cf["sigma"][:, 2] == 0.0) and np.all(cf["u"] == 1.0)
+        cf = read_vtu_cells(os.path.join(od, f"solution.00000.cell-{ncell - 1}.vtu"))
+        np.testing.assert_array_equal(cf["u"][:, 0], np.full(nfc, 5.0))  # w_4 = 5
