@@ -3,6 +3,8 @@
 
 #include <string>
 
+#include "msfec_vtu.hpp"
+
 namespace msfec {
 
 namespace {
@@ -72,16 +74,71 @@ const double* QNedBatch::basis(std::size_t slot, int i) const {
   return &psi_[(slot * 8 + (std::size_t)i) * n_dofs_];
 }
 int QNedBatch::cell_status(std::size_t slot) const { return status_[slot]; }
+const double* QNedBatch::vertices(std::size_t slot) const { return &verts_[slot * 8]; }
+
+namespace {
+// The Nedelec field at the fine cell centres (3 components, z = 0) from the
+// tangential line integrals: the mean of the two parallel edges over their
+// length.  Edge layout: horizontal edge (i, j) at ns + j m + i, vertical edge
+// (i, j) at ns + nh + j (m + 1) + i.
+std::vector<double> cell_centre_u(const double* dofs, const double* verts, int l) {
+  const int m = 1 << l, ns = (m + 1) * (m + 1), nh = m * (m + 1);
+  const double hx = (verts[2] - verts[0]) / m, hy = (verts[5] - verts[1]) / m;
+  std::vector<double> out((size_t)m * m * 3, 0.0);
+  for (int j = 0; j < m; ++j)
+    for (int i = 0; i < m; ++i) {
+      double* o = &out[((size_t)j * m + i) * 3];
+      o[0] = 0.5 * (dofs[ns + j * m + i] + dofs[ns + (j + 1) * m + i]) / hx;
+      o[1] = 0.5 * (dofs[ns + nh + j * (m + 1) + i] + dofs[ns + nh + j * (m + 1) + i + 1]) / hy;
+    }
+  return out;
+}
+}  // namespace
 
 QNedBasis::QNedBasis(std::shared_ptr<QNedBatch> batch, const std::vector<double>& corner_points,
                      const std::vector<double>& a, const std::vector<double>& binv,
-                     const std::vector<double>& r, const std::vector<double>& f)
+                     const std::vector<double>& r, const std::vector<double>& f,
+                     const std::string& cell_id)
     : batch_(std::move(batch)) {
   if (!batch_) throw std::invalid_argument("null batch");
   slot_ = batch_->add_cell(corner_points, a, binv, r, f);
+  cell_id_ = cell_id.empty() ? std::to_string(slot_) : cell_id;
 }
 
-void QNedBasis::run() { run_called_ = true; }
+void QNedBasis::run() {
+  run_called_ = true;
+  const ParametersNed& prm = batch_->parameters();
+  filename_global_ = prm.filename_output + "." + int_to_string(prm.local_subdomain, 5) +
+                     ".cell-" + cell_id_ + ".vtu";
+}
+
+void QNedBasis::set_output_flag() {
+  if (!batch_->parameters().prevent_output) output_flag_ = (slot_ == 0);
+}
+
+void QNedBasis::output_basis() const {
+  ensure();
+  const ParametersNed& prm = batch_->parameters();
+  const int l = (int)prm.n_refine_local;
+  for (int i = 0; i < 8; ++i) {
+    const double* p = batch_->basis(slot_, i);
+    const std::vector<double> u = cell_centre_u(p, batch_->vertices(slot_), l);
+    const std::string name = std::string("basis_q-ned") + (i < 4 ? ".h1." : ".curl.") +
+                             int_to_string(prm.local_subdomain, 5) + ".cell-" + cell_id_ +
+                             ".index-" + int_to_string((unsigned)(i % 4), 2) + ".vtu";
+    write_vtu_box(prm.dirname_output + "/" + name, 2, batch_->vertices(slot_), l, {"sigma"}, {p},
+                  {{"u", 3, u.data()}});
+  }
+}
+
+void QNedBasis::output_global_solution_in_cell() const {
+  if (solution_.empty()) throw std::logic_error("global weights must be set first");
+  const ParametersNed& prm = batch_->parameters();
+  const int l = (int)prm.n_refine_local;
+  const std::vector<double> u = cell_centre_u(solution_.data(), batch_->vertices(slot_), l);
+  write_vtu_box(prm.dirname_output + "/" + filename_global_, 2, batch_->vertices(slot_), l,
+                {"sigma"}, {solution_.data()}, {{"u", 3, u.data()}});
+}
 
 void QNedBasis::ensure() const {
   if (!run_called_) throw std::logic_error("run() was not called on this cell");

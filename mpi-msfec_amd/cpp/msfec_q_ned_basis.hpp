@@ -9,6 +9,7 @@
 #include <cstddef>
 #include <memory>
 #include <stdexcept>
+#include <string>
 #include <vector>
 
 #include "msfec.h"
@@ -18,6 +19,11 @@ namespace msfec {
 struct ParametersNed {
   unsigned n_refine_local = 4;  // ParametersMs::n_refine_local
   bool full_rhs = true;         // ParametersMs::full_rhs (q_ned_basis.cc:432-444)
+  // output (the ParametersBasis fields of include/Q_Ned/q_ned_parameters.h)
+  std::string dirname_output = "Q_Ned";
+  std::string filename_output = "solution";
+  bool prevent_output = false;
+  unsigned local_subdomain = 0;
 };
 
 class QNedBatch {
@@ -41,6 +47,8 @@ class QNedBatch {
   const double* element_matrix(std::size_t slot) const;  // 8 x 8
   const double* element_rhs(std::size_t slot) const;     // 8
   const double* basis(std::size_t slot, int i) const;    // n_dofs
+  const double* vertices(std::size_t slot) const;        // 4 x 2
+  const ParametersNed& parameters() const { return prm_; }
   int cell_status(std::size_t slot) const;
 
  private:
@@ -59,9 +67,12 @@ class QNedBasis {
   QNedBasis() = delete;
   // Replaces QNedBasis(parameters_ms, prm_file, cell, first_cell, subdomain,
   // comm) (q_ned_basis.cc:27-90): the cell's data is handed over explicitly.
+  // cell_id names the cell in output file names (CellId::to_string in the
+  // reference); empty means the batch slot.
   QNedBasis(std::shared_ptr<QNedBatch> batch, const std::vector<double>& corner_points,
             const std::vector<double>& a, const std::vector<double>& binv,
-            const std::vector<double>& r, const std::vector<double>& f);
+            const std::vector<double>& r, const std::vector<double>& f,
+            const std::string& cell_id = "");
   QNedBasis(const QNedBasis& other) = default;
   QNedBasis& operator=(const QNedBasis& other) = default;
 
@@ -76,6 +87,21 @@ class QNedBasis {
   const std::vector<double>& get_global_solution() const;
   int status() const;
 
+  // q_ned_basis.cc:1152-1158, 1255-1280: filename_global = filename_output +
+  // "." + subdomain (5 digits) + ".cell-" + cell id + ".vtu", set by run();
+  // the output flag is raised on the batch's first cell unless prevent_output.
+  const std::string& get_filename_global() const { return filename_global_; }
+  void set_output_flag();
+  bool output_flag() const { return output_flag_; }
+  // q_ned_basis.cc:942-1010: basis_q-ned.h1.<subdomain>.cell-<id>.index-<i>.vtu
+  // for the vertex bases and basis_q-ned.curl.<...>.index-<i>.vtu for the edge
+  // bases, in dirname_output: sigma at the fine nodes, u (the Nedelec field
+  // at the fine cell centres) as cell data.
+  void output_basis() const;
+  // q_ned_basis.cc:1082-1149: the global solution of set_global_weights in
+  // dirname_output / filename_global.
+  void output_global_solution_in_cell() const;
+
  private:
   void ensure() const;
   std::shared_ptr<QNedBatch> batch_;
@@ -83,6 +109,8 @@ class QNedBasis {
   bool run_called_ = false;
   mutable std::vector<double> M_, b_;
   std::vector<double> solution_;
+  std::string cell_id_, filename_global_;
+  bool output_flag_ = false;
 };
 
 }  // namespace msfec

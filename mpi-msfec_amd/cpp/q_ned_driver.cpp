@@ -2,12 +2,14 @@
 // copy-insertion, run() on every entry, then the getters and
 // set_global_weights (the per-cell calls the reference's Q_Ned driver makes).
 //
-// Usage: q_ned_driver <in.bin> <out.bin>
+// Usage: q_ned_driver <in.bin> <out.bin> [output directory]
 //   in.bin : int32 n_refine_local, int32 nc, then a (nc^2 cells x 4^l fine
 //            cells, FP64, per-cell lexicographic); B = 1, R = 0, f = (1, 1)
 //            on the unit square
 //   out.bin: M (cells x 8 x 8), b (cells x 8), Psi (cells x 8 x dofs), and the
 //            cell solution for the weights w_i = i + 1 (cells x dofs), FP64
+//   output directory (optional): the VTU files of output_basis (first cell)
+//            and output_global_solution_in_cell (every cell)
 #include <cstdint>
 #include <cstdio>
 #include <map>
@@ -17,8 +19,8 @@
 #include "msfec_q_ned_basis.hpp"
 
 int main(int argc, char** argv) {
-  if (argc != 3) {
-    std::fprintf(stderr, "usage: %s in.bin out.bin\n", argv[0]);
+  if (argc != 3 && argc != 4) {
+    std::fprintf(stderr, "usage: %s in.bin out.bin [output directory]\n", argv[0]);
     return 2;
   }
   std::FILE* in = std::fopen(argv[1], "rb");
@@ -32,6 +34,7 @@ int main(int argc, char** argv) {
 
   msfec::ParametersNed prm;
   prm.n_refine_local = (unsigned)l;
+  if (argc == 4) prm.dirname_output = argv[3];
   auto batch = std::make_shared<msfec::QNedBatch>(prm);
   std::map<int, msfec::QNedBasis> bases;
   const double h = 1.0 / nc;
@@ -62,6 +65,12 @@ int main(int argc, char** argv) {
       std::fwrite(s.data(), 8, s.size(), out);
     }
     std::fclose(out);
+    if (argc == 4)
+      for (auto& e : bases) {
+        e.second.set_output_flag();
+        if (e.second.output_flag()) e.second.output_basis();
+        e.second.output_global_solution_in_cell();
+      }
   } catch (const std::exception& ex) {
     std::fprintf(stderr, "error: %s\n", ex.what());
     return 1;

@@ -234,3 +234,42 @@ def test_rt_dq_driver_vtu_output(oracle_mod):
             assert np.all(cf["sigma"][:, 2] == 0.0) and np.all(cf["u"] == 1.0)
         cf = read_vtu_cells(os.path.join(od, f"solution.00000.cell-{ncell - 1}.vtu"))
         np.testing.assert_array_equal(cf["u"][:, 0], np.full(nfc, 5.0))  # w_4 = 5
+
+
+def test_ned_driver_vtu_output(oracle_mod):
+    """QNedBasis output_basis (first cell: 4 .h1 and 4 .curl files) and
+    output_global_solution_in_cell: sigma at the fine nodes, u at the fine cell
+    centres."""
+    print("case ned_shim_driver_vtu l=2 nc=2", flush=True)
+    from msfec_amd import fields as F
+    l, nc = 2, 2
+    m = 1 << l
+    a = F.lognormal_coefficient(2, nc, l, 1.0, 0.0, seed=29)
+    ncell = nc * nc
+    ns, nh, nv, N = oracle_mod.ned_sizes(l)
+    with tempfile.TemporaryDirectory() as td:
+        fin, fout, od = os.path.join(td, "in.bin"), os.path.join(td, "out.bin"), os.path.join(td, "vtu")
+        os.mkdir(od)
+        with open(fin, "wb") as fh:
+            fh.write(np.array([l, nc], dtype=np.int32).tobytes())
+            fh.write(np.ascontiguousarray(a, dtype=np.float64).tobytes())
+        r = subprocess.run([NED_DRIVER, fin, fout, od], capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        d = np.frombuffer(open(fout, "rb").read(), dtype=np.float64)
+        P = d[ncell * 72:ncell * 72 + ncell * 8 * N].reshape(ncell, 8, N)
+        names = sorted(os.listdir(od))
+        expect = sorted([f"basis_q-ned.{k}.00000.cell-0.index-{i:02d}.vtu"
+                         for k in ("h1", "curl") for i in range(4)] +
+                        [f"solution.00000.cell-{c}.vtu" for c in range(ncell)])
+        assert names == expect
+        h = 1.0 / nc / m
+        for b in range(8):
+            kind, i = ("h1", b) if b < 4 else ("curl", b - 4)
+            path = os.path.join(od, f"basis_q-ned.{kind}.00000.cell-0.index-{i:02d}.vtu")
+            pts, conn, types, fl = read_vtu(path)
+            np.testing.assert_array_equal(fl["sigma"], P[0, b, :ns])
+            u = read_vtu_cells(path)["u"]
+            uh = P[0, b, ns:ns + nh].reshape(m + 1, m)
+            uv = P[0, b, ns + nh:].reshape(m, m + 1)
+            np.testing.assert_allclose(u[:, 0], (0.5 * (uh[:-1] + uh[1:]) / h).ravel(), rtol=1e-14, atol=1e-15)
+            np.testing.assert_allclose(u[:, 1], (0.5 * (uv[:, :-1] + uv[:, 1:]) / h).ravel(), rtol=1e-14, atol=1e-15)
