@@ -11,6 +11,17 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
+
+def hip_runtime():
+    """The HIP runtime libmsfec_hip.so itself uses: the library is loaded
+    first, then its runtime is opened by soname, which returns the instance
+    already in the process (torch, when imported earlier, brings its own copy
+    of the runtime, and a second instance opened by another name sees no
+    device)."""
+    from msfec_amd import _lib
+    _lib.load()
+    return ctypes.CDLL("libamdhip64.so.7")
+
 DEV = 1
 
 
@@ -20,7 +31,7 @@ class DevBuf:
 
     def __init__(self, n):
         if DevBuf.hip is None:
-            h = ctypes.CDLL("libamdhip64.so")
+            h = hip_runtime()
             h.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
             h.hipFree.argtypes = [ctypes.c_void_p]
             h.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]

@@ -13,9 +13,20 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
+def hip_runtime():
+    """The HIP runtime libmsfec_hip.so itself uses: the library is loaded
+    first, then its runtime is opened by soname, which returns the instance
+    already in the process (torch, when imported earlier, brings its own copy
+    of the runtime, and a second instance opened by another name sees no
+    device)."""
+    from msfec_amd import _lib
+    _lib.load()
+    return ctypes.CDLL("libamdhip64.so.7")
+
+
 class Hip:
     def __init__(self):
-        self.L = ctypes.CDLL("libamdhip64.so")
+        self.L = hip_runtime()
         vp = ctypes.c_void_p
         for name, args in (("hipStreamCreate", [ctypes.POINTER(vp)]),
                            ("hipStreamDestroy", [vp]),
