@@ -3,12 +3,14 @@
  *
  * The library replaces, for a whole batch of coarse cells at once, the
  * per-cell object Q::QBasis of MsFEC and the parts of Q::QMultiscale that
- * consume it.  Every entry point below cites the reference member it stands
- * in for (paths relative to the MsFEC repository).  Plain C types only: no
+ * consume it (msfec_q_*, msfec_coarse_solve), and the two-dimensional forms
+ * of QNed::QNedBasis (msfec_ned_*) and RTDQ::RTDQBasis (msfec_rt_*).  Every
+ * entry point below cites the reference member it stands in for (paths
+ * relative to the MsFEC repository).  Plain C types only: no
  * C++ exceptions cross this boundary, pointers are host or device memory as
  * the accompanying *_loc argument says, streams are hipStream_t passed as
- * void*.  All GPU work of one handle runs on the stream given to
- * msfec_q_compute (0 = the default stream).  A handle is not thread-safe;
+ * void*.  All GPU work of one handle runs on the stream given to its
+ * compute call (0 = the default stream).  A handle is not thread-safe;
  * distinct handles may be used from distinct threads.
  *
  * Conventions shared with the reference (deal.II):
