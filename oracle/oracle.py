@@ -270,7 +270,17 @@ def ned_basis_cell(l, verts, a, binv=1.0, r=0.0, f=(1.0, 1.0), full_rhs=True):
     return M, b, Psi, st
 
 
-def ned_basis_batch(l, verts, a, binv=1.0, r=0.0, f=(1.0, 1.0), full_rhs=True):
+def _map_cells(fn, n, n_threads):
+    """fn(c) for every cell, on n_threads host threads (the C calls release
+    the interpreter lock; the oracle keeps no state between calls)."""
+    if n_threads <= 1:
+        return [fn(c) for c in range(n)]
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(n_threads) as ex:
+        return list(ex.map(fn, range(n)))
+
+
+def ned_basis_batch(l, verts, a, binv=1.0, r=0.0, f=(1.0, 1.0), full_rhs=True, n_threads=1):
     """Cells of verts (n, 4, 2) with per-cell coefficient blocks a (n, m*m) etc."""
     verts = np.asarray(verts, dtype=np.float64)
     n = verts.shape[0]
@@ -283,14 +293,9 @@ def ned_basis_batch(l, verts, a, binv=1.0, r=0.0, f=(1.0, 1.0), full_rhs=True):
     a_, b_, r_ = per_cell(a, (m * m,)), per_cell(binv, (m * m,)), per_cell(r, (m * m,))
     f_ = per_cell(f, (m * m, 2)) if np.asarray(f).ndim >= 2 else np.broadcast_to(
         np.asarray(f, dtype=np.float64), (n, m * m, 2))
-    out = {"M": [], "b": [], "Psi": [], "status": []}
-    for c in range(n):
-        M, b, P, st = ned_basis_cell(l, verts[c], a_[c], b_[c], r_[c], f_[c], full_rhs)
-        out["M"].append(M)
-        out["b"].append(b)
-        out["Psi"].append(P)
-        out["status"].append(st)
-    return {k: np.array(v) for k, v in out.items()}
+    res = _map_cells(lambda c: ned_basis_cell(l, verts[c], a_[c], b_[c], r_[c], f_[c], full_rhs),
+                     n, n_threads)
+    return {k: np.array([x[i] for x in res]) for i, k in enumerate(("M", "b", "Psi", "status"))}
 
 
 def ned_assemble(l, verts, a, binv=1.0, r=0.0, f=(1.0, 1.0)):
@@ -339,7 +344,7 @@ def rt_basis_cell(l, verts, a, f=1.0, tensor=False):
     return M, b, Sig, U, st
 
 
-def rt_basis_batch(l, verts, a, f=1.0, tensor=False):
+def rt_basis_batch(l, verts, a, f=1.0, tensor=False, n_threads=1):
     """Cells of verts (n, 4, 2) with per-cell coefficient blocks a (n, m*m, or
     n, m*m, 4, 3 with tensor=True) and f (n, m*m)."""
     verts = np.asarray(verts, dtype=np.float64)
@@ -352,12 +357,9 @@ def rt_basis_batch(l, verts, a, f=1.0, tensor=False):
 
     a_ = per_cell(a, (nf, 4, 3) if tensor else (nf,))
     f_ = per_cell(f, (nf,))
-    out = {"M": [], "b": [], "Sigma": [], "U": [], "status": []}
-    for c in range(n):
-        M, b, S, U, st = rt_basis_cell(l, verts[c], a_[c], f_[c], tensor)
-        for k, v in zip(("M", "b", "Sigma", "U", "status"), (M, b, S, U, st)):
-            out[k].append(v)
-    return {k: np.array(v) for k, v in out.items()}
+    res = _map_cells(lambda c: rt_basis_cell(l, verts[c], a_[c], f_[c], tensor), n, n_threads)
+    return {k: np.array([x[i] for x in res])
+            for i, k in enumerate(("M", "b", "Sigma", "U", "status"))}
 
 
 def rt_assemble(l, verts, a, f=1.0, tensor=False):
