@@ -358,7 +358,10 @@ msfec_status msfec_rt_destroy(msfec_rt_handle h);
 /* CG iterations of the last compute, n_cells x 2^dim ints on the host
  * (basis-minor), for a handle created with MSFEC_SOLVER_CG.  The reference
  * reports CG iterations per basis solve in verbose mode
- * (source/Q/q_basis.cc:349-353). */
+ * (source/Q/q_basis.cc:349-353).  Without a reaction term the last basis is
+ * not iterated: it is 1 minus the sum of the others (the bases form a
+ * partition of unity), and its entry repeats the largest count of the
+ * others. */
 msfec_status msfec_q_get_cg_iterations(msfec_q_handle h, int *iterations);
 
 /* Thread-local description of the last error of this thread. */

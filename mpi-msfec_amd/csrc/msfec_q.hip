@@ -3005,6 +3005,8 @@ struct CgArgs {
   double tol;
   int max_iter;
   int seg;     // rows per thread: segments of grid lines in x (cg_segment)
+  int nsolve;  // bases solved per cell: 2^dim, or 2^dim - 1 when the last one
+               // follows from the partition of unity (q_cg_unity_kernel)
   int* iters;  // [cells][NV][2]: iterations, state (0 converged, 1 not, 2 breakdown, 3 non-finite)
 };
 
@@ -3181,7 +3183,7 @@ __device__ __forceinline__ void cg_coords(int jn, int ni, int* c) {
 template <int DIM, bool REAC>
 __global__ __launch_bounds__(kCgThreads) void q_cg_kernel(BandArgs a, CgArgs g) {
   using D = Dirs<DIM>;
-  const int cell = blockIdx.x / D::NV, j = blockIdx.x - cell * D::NV;
+  const int cell = blockIdx.x / g.nsolve, j = blockIdx.x - cell * g.nsolve;
   const int tid = threadIdx.x;
   // LDS: coefficient | reaction (per cell) | search direction over all nodes
   extern __shared__ double dyn[];
@@ -3316,6 +3318,44 @@ __global__ __launch_bounds__(kCgThreads) void q_cg_kernel(BandArgs a, CgArgs g) 
   if (tid == 0) {
     g.iters[((size_t)cell * D::NV + j) * 2] = it;
     g.iters[((size_t)cell * D::NV + j) * 2 + 1] = state;
+  }
+}
+
+// The last basis of a cell from the others (no reaction term): the lifted
+// values sum to 1 on the boundary and K 1 = 0, so sum_i psi_i = 1 and
+//   x_last = 1 - sum_i x_i,   r_last = R_last - K x_last = -sum_i r_i
+// (the right-hand sides sum to K_II 1).  One thread per interior row; the
+// row's first thread also records the largest iteration count and the worst
+// state of the solved bases for the last one.
+template <int DIM>
+__global__ __launch_bounds__(256) void q_cg_unity_kernel(BandArgs a, CgArgs g, int n_cells) {
+  using D = Dirs<DIM>;
+  constexpr int L = D::NV - 1;
+  const long long idx = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= (long long)n_cells * a.n) return;
+  const int cell = (int)(idx / a.n), jn = (int)(idx - (long long)cell * a.n);
+  const size_t base = ((size_t)cell * a.n + jn) * kMaxNb;
+  double xs = 0.0, rs = 0.0;
+#pragma unroll
+  for (int i = 0; i < L; ++i) {
+    xs += a.X[base + i];
+    rs += a.Rres[base + i];
+  }
+  a.X[base + L] = 1.0 - xs;
+  a.Rres[base + L] = -rs;
+  int c[3];
+  cg_coords<DIM>(jn, a.ni, c);
+  const int node = c[0] + a.S * (c[1] + (DIM == 3 ? a.S * c[2] : 0));
+  a.Psi[((size_t)cell * D::NV + L) * a.nn + node] = 1.0 - xs;
+  if (jn == 0) {
+    int it = 0, st = 0;
+#pragma unroll
+    for (int i = 0; i < L; ++i) {
+      it = max(it, g.iters[((size_t)cell * D::NV + i) * 2]);
+      st = max(st, g.iters[((size_t)cell * D::NV + i) * 2 + 1]);
+    }
+    g.iters[((size_t)cell * D::NV + L) * 2] = it;
+    g.iters[((size_t)cell * D::NV + L) * 2 + 1] = st;
   }
 }
 
@@ -3894,11 +3934,12 @@ msfec_status msfec_q_compute(msfec_q_handle h, void* stream) {
     const size_t cdyn = ((size_t)c.nfc * (1 + (c.reaction_mode == MSFEC_REACTION_CELL ? 1 : 0)) +
                          (size_t)c.nn) * sizeof(double);
     const bool reac = c.reaction_mode != MSFEC_REACTION_NONE;
+    g.nsolve = reac ? c.nv : c.nv - 1;
 #define LAUNCH_CG(D, R)                                                                   \
   do {                                                                                    \
     MSFEC_HIP(hipFuncSetAttribute((const void*)q_cg_kernel<D, R>,                         \
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)cdyn)); \
-    hipLaunchKernelGGL((q_cg_kernel<D, R>), dim3(h->n_cells * c.nv), dim3(kCgThreads), cdyn, \
+    hipLaunchKernelGGL((q_cg_kernel<D, R>), dim3(h->n_cells * g.nsolve), dim3(kCgThreads), cdyn, \
                        s, a, g);                                                          \
   } while (0)
     if (c.dim == 3) {
@@ -3910,6 +3951,14 @@ msfec_status msfec_q_compute(msfec_q_handle h, void* stream) {
     }
 #undef LAUNCH_CG
     MSFEC_HIP(hipGetLastError());
+    if (!reac) {
+      const unsigned ub = (unsigned)(((size_t)h->n_cells * c.n + 255) / 256);
+      if (c.dim == 3)
+        hipLaunchKernelGGL((q_cg_unity_kernel<3>), dim3(ub), dim3(256), 0, s, a, g, h->n_cells);
+      else
+        hipLaunchKernelGGL((q_cg_unity_kernel<2>), dim3(ub), dim3(256), 0, s, a, g, h->n_cells);
+      MSFEC_HIP(hipGetLastError());
+    }
     if (tm) MSFEC_HIP(hipEventRecord(ev[2], s));
     if (c.dim == 3)
       hipLaunchKernelGGL((q_cg_finish_kernel<3>), dim3(h->n_cells), dim3(64 * kFinWaves), 0, s, a, g);
