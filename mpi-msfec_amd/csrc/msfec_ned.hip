@@ -351,22 +351,35 @@ __global__ __launch_bounds__(kNThreads) void ned_rows_kernel(NedArgs a) {
 // system) goes to HBM and, with the boundary values, to LDS.  Thread
 // (kt = tid & 7, group tid >> 3) sums row kt of the fine-cell contributions
 // over every 32nd fine cell; the groups add up across lanes and waves.
+// LDS slot of value d of basis bb in the element kernel, or -1: the element
+// matrix reads only the sigma (node) values of the vertex bases and the u
+// (edge) values of the edge bases, so LDS holds [4][ns] node values, then
+// [4][N - ns] edge values (half of Psi).
+__device__ __forceinline__ int ned_lds_slot(int bb, int d, int ns, int N) {
+  if (bb < 4) return d < ns ? bb * ns + d : -1;
+  return d >= ns ? 4 * ns + (bb - 4) * (N - ns) + (d - ns) : -1;
+}
+
 __global__ __launch_bounds__(kNThreads) void ned_element_kernel(NedArgs a) {
-  extern __shared__ double ps[];  // [8][N]
+  extern __shared__ double ps[];  // [4][ns] node values | [4][N - ns] edge values
   __shared__ double part[kNThreads / 64][72];
   const int cell = blockIdx.x, tid = threadIdx.x;
-  const int N = a.N, m = a.m, nf = m * m, n = a.n;
+  const int N = a.N, m = a.m, nf = m * m, n = a.n, ns = a.ns, nu = N - ns;
   double* pg = a.psi + (size_t)cell * kNR * N;
-  for (int t = tid; t < kNR * N; t += kNThreads)
-    if (a.pos[t % N] < 0) ps[t] = pg[t];  // boundary values (row kernel)
+  for (int t = tid; t < kNR * N; t += kNThreads) {
+    const int bb = t / N, d = t - bb * N;
+    const int sl = ned_lds_slot(bb, d, ns, N);
+    if (sl >= 0 && a.pos[d] < 0) ps[sl] = pg[t];  // boundary values (row kernel)
+  }
   const double* x = a.X + (size_t)cell * n * kNR;
   const double* sc = a.scale + (size_t)cell * n;
   for (int i = tid; i < n * kNR; i += kNThreads) {
     const int r = i >> 3, bb = i & (kNR - 1);
-    const int t = bb * N + a.order[r];
+    const int d = a.order[r];
     const double v = sc[r] * x[i];
-    ps[t] = v;
-    pg[t] = v;
+    const int sl = ned_lds_slot(bb, d, ns, N);
+    if (sl >= 0) ps[sl] = v;
+    pg[bb * N + d] = v;
   }
   __syncthreads();
   const NedBox bx = ned_box(a, cell);
@@ -386,7 +399,8 @@ __global__ __launch_bounds__(kNThreads) void ned_element_kernel(NedArgs a) {
     ned_local(geo, a.a[c], a.binv[c], a.r[c], K);
     double u[4];
 #pragma unroll
-    for (int p = 0; p < 4; ++p) u[p] = ps[kt * N + d[r0 + p]];
+    for (int p = 0; p < 4; ++p)
+      u[p] = edge_t ? ps[4 * ns + (kt - 4) * nu + d[4 + p] - ns] : ps[kt * ns + d[p]];
 #pragma unroll
     for (int c0 = 0; c0 < 8; c0 += 4) {
       double w[4];  // row kt of psi^T K on the (r0, c0) block
@@ -401,7 +415,8 @@ __global__ __launch_bounds__(kNThreads) void ned_element_kernel(NedArgs a) {
       for (int kc = c0; kc < c0 + 4; ++kc) {
         double t = 0.0;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) t += w[q] * ps[kc * N + d[c0 + q]];
+        for (int q = 0; q < 4; ++q)
+          t += w[q] * (c0 == 0 ? ps[kc * ns + d[q]] : ps[4 * ns + (kc - 4) * nu + d[4 + q] - ns]);
         acc[kc] += t;
       }
     }
@@ -776,7 +791,7 @@ msfec_status msfec_ned_compute(msfec_ned_handle h, void* stream) {
     ev = &h->ev_sets[h->ev_used++];
     NED_HIP(hipEventRecord((*ev)[0], s));
   }
-  const size_t elem_lds = (size_t)kNR * h->conf.N * sizeof(double);
+  const size_t elem_lds = (size_t)4 * h->conf.N * sizeof(double);
   if (elem_lds > 160 * 1024) return ned_fail(MSFEC_ERR_UNSUPPORTED, "cell too large for LDS staging");
   hipLaunchKernelGGL(ned_rows_kernel, dim3(h->n_cells), dim3(kNThreads), 0, s, a);
   NED_HIP(hipGetLastError());
