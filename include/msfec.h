@@ -10,7 +10,11 @@
  * C++ exceptions cross this boundary, pointers are host or device memory as
  * the accompanying *_loc argument says, streams are hipStream_t passed as
  * void*.  All GPU work of one handle runs on the stream given to its
- * compute call (0 = the default stream).  A handle is not thread-safe;
+ * compute call (0 = the default stream).  The set_* calls copy their input
+ * on the stream of the handle's last compute call (the default stream
+ * before the first) and return once the copy is done, so a compute or graph
+ * replay still in flight there finishes with the old input; device-resident
+ * inputs must be ready on that stream.  A handle is not thread-safe;
  * distinct handles may be used from distinct threads.
  *
  * Conventions shared with the reference (deal.II):

@@ -627,10 +627,13 @@ void ned_free(msfec_ned_batch* h) {
   h->ev_sets.clear();
 }
 
-msfec_status ned_in(double* dst, const double* src, size_t count, int loc) {
+// Copied on the handle's stream (that of its last compute call), waited for:
+// work still in flight there reads the old buffer to the end.
+msfec_status ned_in(hipStream_t s, double* dst, const double* src, size_t count, int loc) {
   if (!src) return ned_fail(MSFEC_ERR_INVALID, "null data pointer");
-  NED_HIP(hipMemcpy(dst, src, count * sizeof(double),
-                    loc == MSFEC_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice));
+  NED_HIP(hipMemcpyAsync(dst, src, count * sizeof(double),
+                         loc == MSFEC_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
+  NED_HIP(hipStreamSynchronize(s));
   return MSFEC_OK;
 }
 
@@ -765,10 +768,10 @@ msfec_status msfec_ned_set_coefficients(msfec_ned_handle h, const double* a, con
   if (!h) return ned_fail(MSFEC_ERR_INVALID, "null handle");
   if (loc != MSFEC_HOST && loc != MSFEC_DEVICE) return ned_fail(MSFEC_ERR_INVALID, "bad location");
   const size_t cnt = (size_t)h->n_cells * h->conf.m * h->conf.m;
-  msfec_status st = ned_in(h->d_a, a, cnt, loc);
-  if (st == MSFEC_OK) st = ned_in(h->d_binv, binv, cnt, loc);
-  if (st == MSFEC_OK) st = ned_in(h->d_r, r, cnt, loc);
-  if (st == MSFEC_OK) st = ned_in(h->d_f, f, 2 * cnt, loc);
+  msfec_status st = ned_in(h->stream, h->d_a, a, cnt, loc);
+  if (st == MSFEC_OK) st = ned_in(h->stream, h->d_binv, binv, cnt, loc);
+  if (st == MSFEC_OK) st = ned_in(h->stream, h->d_r, r, cnt, loc);
+  if (st == MSFEC_OK) st = ned_in(h->stream, h->d_f, f, 2 * cnt, loc);
   if (st != MSFEC_OK) return st;
   h->coef_set = true;
   return MSFEC_OK;

@@ -3546,10 +3546,16 @@ void free_all(msfec_q_batch* h) {
   h->ev_sets.clear();
 }
 
-msfec_status copy_in(double* dst, const double* src, size_t count, int loc) {
+// Inputs are copied on the handle's stream (that of its last compute call,
+// the null stream before the first) and the call returns once the copy is
+// done: a compute or graph replay still in flight on that stream reads the
+// old buffer to the end, and device-resident data produced on that stream is
+// read after its producer.
+msfec_status copy_in(hipStream_t s, double* dst, const double* src, size_t count, int loc) {
   if (!src) return fail(MSFEC_ERR_INVALID, "null data pointer");
-  MSFEC_HIP(hipMemcpy(dst, src, count * sizeof(double),
-                      loc == MSFEC_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice));
+  MSFEC_HIP(hipMemcpyAsync(dst, src, count * sizeof(double),
+                           loc == MSFEC_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
+  MSFEC_HIP(hipStreamSynchronize(s));
   return MSFEC_OK;
 }
 
@@ -3602,6 +3608,49 @@ BandArgs band_args(const msfec_q_batch* h) {
   return a;
 }
 
+}  // namespace
+
+namespace {
+// The per-level constants of a batch: grid sizes and the stencil offsets of
+// the lower neighbours on the full and the interior grid.
+void fill_qconf(const msfec_q_params& p, QConf& c) {
+  const int N = 1 << p.n_refine_local;
+  const int ni = N - 1;
+  const int nv = 1 << p.dim;
+  c.dim = p.dim;
+  c.N = N;
+  c.S = N + 1;
+  c.nn = ipow(c.S, p.dim);
+  c.nfc = ipow(N, p.dim);
+  c.nv = nv;
+  c.ni = ni;
+  c.n = ipow(ni, p.dim);
+  c.nst = ipow(3, p.dim);
+  c.center = (c.nst - 1) / 2;
+  c.nlow = c.center + 1;
+  c.coef_mode = p.coef_mode;
+  c.rhs_mode = p.rhs_mode;
+  c.reaction_mode = p.reaction_mode;
+  c.rhs_const = p.rhs_const;
+  c.reaction_const = p.reaction_const;
+  c.off_full[0] = 0;
+  c.off_int[0] = 0;
+  c.dir_low[0] = -1;
+  for (int e = 0; e < c.center; ++e) {
+    int r = e, of = 0, oi = 0, mf = 1, mi = 1;
+    for (int d = 0; d < p.dim; ++d) {
+      const int dd = r % 3 - 1;
+      r /= 3;
+      of += dd * mf;
+      oi += dd * mi;
+      mf *= c.S;
+      mi *= ni;
+    }
+    c.off_full[1 + e] = -of;
+    c.off_int[1 + e] = -oi;
+    c.dir_low[1 + e] = e;
+  }
+}
 }  // namespace
 
 extern "C" {
@@ -3687,39 +3736,7 @@ msfec_status msfec_q_create(const msfec_q_params* params, int n_cells, const dou
   h->n_cells = n_cells;
   h->bw = bw;
   QConf& c = h->conf;
-  c.dim = p.dim;
-  c.N = N;
-  c.S = N + 1;
-  c.nn = ipow(c.S, p.dim);
-  c.nfc = ipow(N, p.dim);
-  c.nv = nv;
-  c.ni = ni;
-  c.n = ipow(ni, p.dim);
-  c.nst = ipow(3, p.dim);
-  c.center = (c.nst - 1) / 2;
-  c.nlow = c.center + 1;
-  c.coef_mode = p.coef_mode;
-  c.rhs_mode = p.rhs_mode;
-  c.reaction_mode = p.reaction_mode;
-  c.rhs_const = p.rhs_const;
-  c.reaction_const = p.reaction_const;
-  c.off_full[0] = 0;
-  c.off_int[0] = 0;
-  c.dir_low[0] = -1;
-  for (int e = 0; e < c.center; ++e) {
-    int r = e, of = 0, oi = 0, mf = 1, mi = 1;
-    for (int d = 0; d < p.dim; ++d) {
-      const int dd = r % 3 - 1;
-      r /= 3;
-      of += dd * mf;
-      oi += dd * mi;
-      mf *= c.S;
-      mi *= ni;
-    }
-    c.off_full[1 + e] = -of;
-    c.off_int[1 + e] = -oi;
-    c.dir_low[1 + e] = e;
-  }
+  fill_qconf(p, c);
   const int nq = nv;
   h->coef_count = (p.coef_mode == MSFEC_COEF_SCALAR_CELL)
                       ? (size_t)n_cells * c.nfc
@@ -3837,7 +3854,7 @@ msfec_status msfec_q_set_coefficient(msfec_q_handle h, const double* data, size_
   if (count != h->coef_count)
     return fail(MSFEC_ERR_INVALID, "coefficient count " + std::to_string(count) + " != expected " +
                                        std::to_string(h->coef_count));
-  msfec_status st = copy_in(h->d_coef, data, count, loc);
+  msfec_status st = copy_in(h->stream, h->d_coef, data, count, loc);
   if (st == MSFEC_OK) h->coef_set = true;
   return st;
 }
@@ -3847,7 +3864,7 @@ msfec_status msfec_q_set_rhs(msfec_q_handle h, const double* data, size_t count,
   if (h->rhs_count == 0) return fail(MSFEC_ERR_STATE, "rhs_mode is constant; no data expected");
   if (count != h->rhs_count)
     return fail(MSFEC_ERR_INVALID, "rhs count mismatch (expected " + std::to_string(h->rhs_count) + ")");
-  msfec_status st = copy_in(h->d_f, data, count, loc);
+  msfec_status st = copy_in(h->stream, h->d_f, data, count, loc);
   if (st == MSFEC_OK) h->rhs_set = true;
   return st;
 }
@@ -3857,7 +3874,7 @@ msfec_status msfec_q_set_reaction(msfec_q_handle h, const double* data, size_t c
   if (h->reac_count == 0) return fail(MSFEC_ERR_STATE, "reaction_mode takes no per-cell data");
   if (count != h->reac_count)
     return fail(MSFEC_ERR_INVALID, "reaction count mismatch (expected " + std::to_string(h->reac_count) + ")");
-  msfec_status st = copy_in(h->d_reac, data, count, loc);
+  msfec_status st = copy_in(h->stream, h->d_reac, data, count, loc);
   if (st == MSFEC_OK) h->reac_set = true;
   return st;
 }

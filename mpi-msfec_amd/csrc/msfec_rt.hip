@@ -471,9 +471,12 @@ void rt_free(msfec_rt_batch* h) {
   h->ev_sets.clear();
 }
 
-msfec_status rt_in(double* dst, const double* src, size_t count, int loc) {
-  RT_HIP(hipMemcpy(dst, src, count * sizeof(double),
-                   loc == MSFEC_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice));
+// Copied on the handle's stream (that of its last compute call), waited for:
+// work still in flight there reads the old buffer to the end.
+msfec_status rt_in(hipStream_t s, double* dst, const double* src, size_t count, int loc) {
+  RT_HIP(hipMemcpyAsync(dst, src, count * sizeof(double),
+                        loc == MSFEC_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
+  RT_HIP(hipStreamSynchronize(s));
   return MSFEC_OK;
 }
 
@@ -586,7 +589,7 @@ msfec_status msfec_rt_create(const msfec_rt_params* params, int n_cells, const d
   RT_TRY(rt_alloc(&h->d_w, nc * 5));
   RT_TRY(rt_alloc(&h->d_status, nc));
   RT_TRY(rt_alloc(&h->d_loff, loff.size()));
-  RT_TRY(rt_in(h->d_verts, hv.data(), hv.size(), MSFEC_HOST));
+  RT_TRY(rt_in(h->stream, h->d_verts, hv.data(), hv.size(), MSFEC_HOST));
   {
     hipError_t e = hipMemcpy(h->d_loff, loff.data(), loff.size() * sizeof(int), hipMemcpyHostToDevice);
     if (e != hipSuccess) RT_TRY(rt_fail(MSFEC_ERR_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e)));
@@ -602,13 +605,13 @@ msfec_status msfec_rt_set_coefficients(msfec_rt_handle h, const double* a, const
   if (loc != MSFEC_HOST && loc != MSFEC_DEVICE) return rt_fail(MSFEC_ERR_INVALID, "bad location");
   const size_t cnt = (size_t)h->n_cells * h->m * h->m;
   const size_t acnt = cnt * (h->prm.coef_mode == MSFEC_COEF_TENSOR_QP ? 12 : 1);
-  msfec_status st = rt_in(h->d_a, a, acnt, loc);
+  msfec_status st = rt_in(h->stream, h->d_a, a, acnt, loc);
   if (st != MSFEC_OK) return st;
   if (f) {
-    st = rt_in(h->d_f, f, cnt, loc);
+    st = rt_in(h->stream, h->d_f, f, cnt, loc);
   } else {
     const std::vector<double> ones(cnt, 1.0);
-    st = rt_in(h->d_f, ones.data(), cnt, MSFEC_HOST);
+    st = rt_in(h->stream, h->d_f, ones.data(), cnt, MSFEC_HOST);
   }
   if (st != MSFEC_OK) return st;
   h->coef_set = true;
@@ -687,7 +690,7 @@ msfec_status msfec_rt_reconstruct(msfec_rt_handle h, const double* w, int w_loc,
     return rt_fail(MSFEC_ERR_INVALID, "bad location");
   if (!h->computed) return rt_fail(MSFEC_ERR_STATE, "compute not called");
   RT_HIP(hipStreamSynchronize(h->stream));
-  msfec_status st = rt_in(h->d_w, w, (size_t)h->n_cells * 5, w_loc);
+  msfec_status st = rt_in(h->stream, h->d_w, w, (size_t)h->n_cells * 5, w_loc);
   if (st != MSFEC_OK) return st;
   const size_t ns = (size_t)h->n_cells * h->ne, nu = (size_t)h->n_cells * h->m * h->m;
   double* ds = nullptr;

@@ -29,6 +29,23 @@ def _ptr(a):
     return ctypes.c_void_p(arr.ctypes.data), L.HOST, arr
 
 
+def _out_ptr(a, shape):
+    """(pointer, loc, keepalive) for an output buffer the library writes into.
+    No copy is ever made: a host array must be a C-contiguous float64 numpy
+    array (or CPU torch tensor) of the expected size, a device tensor a
+    contiguous float64 one; anything else is an error, not a silent copy."""
+    n = int(np.prod(shape))
+    if hasattr(a, "data_ptr") and hasattr(a, "is_cuda"):
+        import torch
+        if a.dtype != torch.float64 or not a.is_contiguous() or a.numel() != n:
+            raise ValueError(f"out must be a contiguous float64 tensor of {n} elements")
+        return ctypes.c_void_p(a.data_ptr()), (L.DEVICE if a.is_cuda else L.HOST), a
+    if (not isinstance(a, np.ndarray) or a.dtype != np.float64 or not a.flags.c_contiguous
+            or not a.flags.writeable or a.size != n):
+        raise ValueError(f"out must be a writeable C-contiguous float64 array of {n} elements")
+    return ctypes.c_void_p(a.ctypes.data), L.HOST, a
+
+
 class QBasisBatch:
     def __init__(self, vertices, dim=3, n_refine_local=3, coef_mode=L.COEF_SCALAR_CELL,
                  rhs_mode=L.RHS_CONST, rhs_const=1.0, reaction_mode=L.REACTION_NONE,
@@ -80,19 +97,19 @@ class QBasisBatch:
     # -- getters --------------------------------------------------------------
     def get_global_element_matrix(self, out=None):
         out = np.empty((self.n_cells, self.nb, self.nb)) if out is None else out
-        ptr, loc, keep = _ptr(out)
+        ptr, loc, keep = _out_ptr(out, (self.n_cells, self.nb, self.nb))
         L.check(self._lib.msfec_q_get_element(self._h, ptr, None, loc))
         return out
 
     def get_global_element_rhs(self, out=None):
         out = np.empty((self.n_cells, self.nb)) if out is None else out
-        ptr, loc, keep = _ptr(out)
+        ptr, loc, keep = _out_ptr(out, (self.n_cells, self.nb))
         L.check(self._lib.msfec_q_get_element(self._h, None, ptr, loc))
         return out
 
     def get_basis(self, out=None):
         out = np.empty((self.n_cells, self.nb, self.nn)) if out is None else out
-        ptr, loc, keep = _ptr(out)
+        ptr, loc, keep = _out_ptr(out, (self.n_cells, self.nb, self.nn))
         L.check(self._lib.msfec_q_get_basis(self._h, ptr, loc))
         return out
 
@@ -105,7 +122,7 @@ class QBasisBatch:
         """Fine solution u|_K = sum_k w_k psi_k for every cell."""
         out = np.empty((self.n_cells, self.nn)) if out is None else out
         wp, wl, wk = _ptr(weights)
-        op, ol, ok = _ptr(out)
+        op, ol, ok = _out_ptr(out, (self.n_cells, self.nn))
         L.check(self._lib.msfec_q_reconstruct(self._h, wp, wl, op, ol))
         return out
 

@@ -174,3 +174,23 @@ def test_rt_rejects_non_box_and_bad_level(lib):
     assert L.msfec_rt_create(None, 1, None, 0, None) == lib.ERR_INVALID
     assert L.msfec_rt_compute(None, None) == lib.ERR_INVALID
     assert L.msfec_rt_reconstruct(None, None, 0, None, None, 0) == lib.ERR_INVALID
+
+
+def test_output_buffers_are_never_silently_copied():
+    """A getter's out= buffer that the library could not write in place (wrong
+    dtype, not C-contiguous, wrong size, read-only) is rejected up front
+    instead of being filled through a temporary copy."""
+    from msfec_amd.qbasis import _out_ptr
+    shape = (3, 8, 8)
+    ok = np.empty(shape)
+    ptr, loc, keep = _out_ptr(ok, shape)
+    assert ptr.value == ok.ctypes.data and keep is ok
+    bad = [np.empty(shape, dtype=np.float32), np.empty((8, 8, 3)).transpose(2, 0, 1),
+           np.empty((3, 8, 7)), np.empty(shape).tolist()]
+    ro = np.empty(shape)
+    ro.flags.writeable = False
+    bad.append(ro)
+    for b in bad:
+        with pytest.raises(ValueError):
+            _out_ptr(b, shape)
+

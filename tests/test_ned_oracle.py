@@ -194,3 +194,53 @@ def test_no_full_rhs_vertex_bases_see_only_the_lifting():
     tot = Psi[:4].sum(axis=0)
     assert np.abs(tot[:ns][~inner[:ns]] - 1.0).max() < 1e-15
     assert np.abs(tot[ns:][~inner[ns:]]).max() < 1e-15
+
+
+@pytest.mark.parametrize("l", [1, 2, 3, 4])
+def test_boundary_traces_of_all_bases(l):
+    """The lifting data of q_ned_basis.cc:223-357 on every boundary dof, for
+    rough coefficients (the traces do not depend on them): a vertex basis n
+    carries sigma = phi_n at the boundary nodes and, on every boundary edge,
+    the line integral of grad phi_n, i.e. phi_n(end) - phi_n(start) along the
+    edge's +x / +y tangent; an edge basis E carries sigma = 0 and the line
+    integral of the coarse Nedelec function N_E (deal.II line order
+    x = x0, x = x1, y = y0, y = y1)."""
+    rng = np.random.default_rng(60 + l)
+    x0, y0, Hx, Hy = -0.3, 0.45, 0.7, 0.4
+    v = box(x0, y0, Hx, Hy)
+    a, binv, r, f = rand_coefs(rng, l)
+    M, b, Psi, st = O.ned_basis_cell(l, v, a, binv, r, f)
+    assert st == 0
+    m = 1 << l
+    ns, nh, nv, N = O.ned_sizes(l)
+    bnd = ~interior_mask(l)
+    x, y = node_xy(l, v)
+    hx, hy = Hx / m, Hy / m
+    for n in range(4):
+        phi = coarse_phi(n, x, y, v)
+        want = np.zeros(N)
+        want[:ns] = phi
+        for j in range(m + 1):          # horizontal edges (i,j)-(i+1,j)
+            for i in range(m):
+                want[ns + j * m + i] = phi[j * (m + 1) + i + 1] - phi[j * (m + 1) + i]
+        for j in range(m):              # vertical edges (i,j)-(i,j+1)
+            for i in range(m + 1):
+                want[ns + nh + j * (m + 1) + i] = phi[(j + 1) * (m + 1) + i] - phi[j * (m + 1) + i]
+        assert np.abs(Psi[n][bnd] - want[bnd]).max() < 1e-15, n
+        # the nodal part of the trace is grad phi_n's line integral, not its value:
+        # a sign or end swap breaks this
+        assert np.abs(want[ns:][bnd[ns:]]).max() > 0.1 / m
+    # edge bases: sigma = 0, and the tangential line integrals of N_E
+    ii, jj = np.meshgrid(np.arange(m), np.arange(m + 1))          # horizontal edges
+    Yh = (jj.ravel() * hy) / Hy
+    ii, jj = np.meshgrid(np.arange(m + 1), np.arange(m))          # vertical edges
+    Xv = (ii.ravel() * hx) / Hx
+    want = np.zeros((4, N))
+    want[0, ns + nh:] = (1 - Xv) / Hy * hy
+    want[1, ns + nh:] = Xv / Hy * hy
+    want[2, ns:ns + nh] = (1 - Yh) / Hx * hx
+    want[3, ns:ns + nh] = Yh / Hx * hx
+    for e in range(4):
+        assert np.abs(Psi[4 + e][bnd] - want[e][bnd]).max() < 1e-15, e
+        # the trace is non-zero on exactly one side of the coarse box
+        assert np.count_nonzero(np.abs(want[e][bnd]) > 1e-14) == m

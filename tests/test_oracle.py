@@ -108,6 +108,33 @@ def test_diffusion_A_rotation_invariants(oracle_mod):
         assert np.allclose(np.sort(np.linalg.eigvalsh(A)), np.sort(want), rtol=1e-12)
 
 
+def test_diffusion_A_is_the_zxz_euler_rotation(oracle_mod):
+    """eqn_coeff_A.cc:25-39 builds rot from the fixed angles alpha = pi/3,
+    beta = pi/6, gamma = pi/4 as R_z(alpha) R_x(beta) R_z(gamma) (intrinsic
+    z-x'-z'' Euler angles) and :185-190 sets value = rot * D * transpose(rot).
+    scipy's Rotation is an independent construction of that matrix; the
+    transposed similarity rot^T D rot (a plausible slip) must fail."""
+    from scipy.spatial.transform import Rotation
+    R = Rotation.from_euler("ZXZ", [np.pi / 3, np.pi / 6, np.pi / 4]).as_matrix()
+    # the first row as eqn_coeff_A.cc:27-30 writes it
+    a, b, g = np.pi / 3, np.pi / 6, np.pi / 4
+    assert np.allclose(R[0], [np.cos(a) * np.cos(g) - np.sin(a) * np.cos(b) * np.sin(g),
+                              -np.cos(a) * np.sin(g) - np.sin(a) * np.cos(b) * np.cos(g),
+                              np.sin(a) * np.sin(b)], atol=1e-15)
+    scale, alpha, freq = (1.0, 2.0, 3.0), (0.9, 0.5, 0.1), (17, 16, 15)
+    p = oracle_mod.make_problem(3, 1, coef_mode=oracle_mod.COEF_ANALYTIC, freq=freq,
+                                scale=scale, alpha=alpha, rotate=True)
+    rng = np.random.default_rng(21)
+    worst_transposed = 0.0
+    for x in np.vstack([[0.3, 0.7, 0.1], rng.random((40, 3))]):
+        D = np.diag([s * (1 - al * np.sin(2 * np.pi * k * xx))
+                     for s, al, k, xx in zip(scale, alpha, freq, x)])
+        A = oracle_mod.diffusion_A(p, x)
+        assert np.abs(A - R @ D @ R.T).max() < 1e-14 * np.abs(D).max()
+        worst_transposed = max(worst_transposed, np.abs(A - R.T @ D @ R).max())
+    assert worst_transposed > 0.5
+
+
 def test_diffusion_A_matches_field_generator(oracle_mod):
     """The host-side quadrature-point tensor used for the GPU path agrees with
     the oracle's inline evaluation."""
