@@ -161,7 +161,7 @@ def cpu_baseline(verts, coef, budget_s: float = 20.0):
         "cores": cores,
         "kind": "port",
         "sample": f"{n_sample} of the 4096 cells (seeded), CG+SSOR(1.6) abs tol 1e-12 per "
-                  f"basis, OpenMP {cores} threads, max CG iterations {int(r['iters'].max())}, "
+                  f"basis, OpenMP on {host_cores_text(cores)}, max CG iterations {int(r['iters'].max())}, "
                   f"wall {wall:.2f} s",
     }
 
@@ -174,10 +174,33 @@ def ned_config(steps: int, warmup: int, check: bool, cpu: bool):
     coarse cells of the unit square, 16 x 16 fine cells each (289 nodes + 544
     edges, 705 interior unknowns, 4 + 4 bases), a = exp(g) with g iid
     N(0, 1) per fine cell (seed 20261202), B = 1, R = 0, f = (1, 1)."""
-    from msfec_amd import NedBasisBatch, fields as F
+    from msfec_amd import NedBasisBatch, _lib as L, fields as F
     verts = F.coarse_vertices(2, NED_NC)
     a = F.lognormal_coefficient(2, NED_NC, NED_NREF, NED_SIGMA, 0.0, SEED)
     n_cells = verts.shape[0]
+    # the configuration's precision recipe: FP32 LDL^T + FP64 refinement,
+    # timed beside the FP64 LDL^T
+    mixed = {}
+    mb = NedBasisBatch(verts, n_refine_local=NED_NREF, timing=True, solver=L.SOLVER_BAND_MIXED)
+    mb.set_coefficients(a, 1.0, 0.0, (1.0, 1.0))
+    for _ in range(max(warmup, 1)):
+        mb.run()
+    mb.synchronize()
+    mb.kernel_times()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        mb.run()
+    mb.synchronize()
+    wm = time.perf_counter() - t0
+    km = mb.kernel_times()
+    mixed = {"solver": "FP32 LDL^T on the matrix cores + refinement with FP64 residuals",
+             "cells_per_s": round(n_cells / (wm / steps), 1),
+             "ms_per_step": round(wm * 1e3 / steps, 4),
+             "kernel_ms": {"rows": round(km[0], 4), "factor_refine": round(km[1], 4),
+                           "psi_element": round(km[2], 4)},
+             "failed_cells": int((mb.get_cell_status() != 0).sum())}
+    mixed_out = (mb.get_global_element_matrix(), mb.get_global_element_rhs(), mb.get_basis())
+    mb.close()
     nb = NedBasisBatch(verts, n_refine_local=NED_NREF, timing=True)
     nb.set_coefficients(a, 1.0, 0.0, (1.0, 1.0))
     for _ in range(warmup):
@@ -217,22 +240,26 @@ def ned_config(steps: int, warmup: int, check: bool, cpu: bool):
                               n_threads=cores if cpu else 1)
         t_cpu = time.perf_counter() - t0
         if check:
-            M, b, P = nb.get_global_element_matrix(), nb.get_global_element_rhs(), nb.get_basis()
             ns = (2 ** NED_NREF + 1) ** 2
-            out["rel_err_vs_fp64_oracle"] = {
-                "M": rel_err(M[samp], o["M"]), "b": rel_err(b[samp, 4:], o["b"][:, 4:]),
-                "Psi_sigma": rel_err(P[samp, :, :ns], o["Psi"][:, :, :ns]),
-                "Psi_u": rel_err(P[samp, :, ns:], o["Psi"][:, :, ns:]),
-                "cells_checked": len(samp)}
+            for dst, (M, b, P) in ((out, (nb.get_global_element_matrix(),
+                                          nb.get_global_element_rhs(), nb.get_basis())),
+                                   (mixed, mixed_out)):
+                dst["rel_err_vs_fp64_oracle"] = {
+                    "M": rel_err(M[samp], o["M"]), "b": rel_err(b[samp, 4:], o["b"][:, 4:]),
+                    "Psi_sigma": rel_err(P[samp, :, :ns], o["Psi"][:, :, :ns]),
+                    "Psi_u": rel_err(P[samp, :, ns:], o["Psi"][:, :, ns:]),
+                    "cells_checked": len(samp)}
         if cpu:
             out["cpu_baseline"] = {"value": round(len(samp) / t_cpu, 3), "unit": "cells/s",
                                    "cores": cores, "kind": "port",
                                    "sample": f"{len(samp)} of the 1024 cells (seeded): dense "
                                              "assembly, equilibrated LU with partial pivoting "
                                              "+ 3 refinement steps (the UMFPACK direct solve "
-                                             f"restated), one cell per thread on {cores} "
-                                             f"threads, wall {t_cpu:.2f} s"}
+                                             f"restated), one cell per thread on "
+                                             f"{host_cores_text(cores)}, wall {t_cpu:.2f} s"}
     nb.close()
+    out["mixed"] = mixed
+    out["mixed_vs_fp64"] = round(mixed["cells_per_s"] / out["cells_per_s"], 3)
     return out
 
 
@@ -307,9 +334,18 @@ def rt_config(steps: int, warmup: int, check: bool, cpu: bool):
                                              "assembly of the mixed system, LU with partial "
                                              "pivoting of the saddle-point system (the UMFPACK "
                                              f"direct solve restated), one cell per thread on "
-                                             f"{cores} threads, wall {t_cpu:.2f} s"}
+                                             f"{host_cores_text(cores)}, wall {t_cpu:.2f} s"}
     rb.close()
     return out
+
+
+def cg_flops_per_iteration(dim: int, level: int) -> float:
+    """Algorithmic FP64 flops of one Jacobi-PCG iteration of one basis on one
+    cell: the matrix-free 3^dim-point product over the interior rows (a
+    multiply-add per coupling) and the 3 dot products, 3 updates and the
+    preconditioner (2 flops each per row)."""
+    n = ((1 << level) - 1) ** dim
+    return n * (2.0 * 3 ** dim + 2.0 * 7)
 
 
 def uneven_config(steps: int, warmup: int, check: bool, cpu: bool):
@@ -317,78 +353,214 @@ def uneven_config(steps: int, warmup: int, check: bool, cpu: bool):
     cells; 410 seeded inclusions (A = 1e4, radius 0.02) in a background A = 1;
     the cells holding an inclusion centre refined 4 levels (3375 interior
     unknowns, matrix-free CG), the others 2 or 3 levels by a seeded coin
-    (band solver or CG, whichever the one-off timing on 256 cells of the
-    level picks); f = 1."""
-    from msfec_amd import UnevenBatch, fields as F, uneven as U
+    (band solver or CG, whichever the library's timing on up to 256 cells of
+    the level picks); f = 1.  One handle (msfec_qu_*), levels overlapped on
+    streams of their own."""
+    from msfec_amd import UnevenBatch, fields as F
     nc = 16
     cen = F.inclusion_centres(3, 410, SEED)
     lv = F.uneven_levels(3, nc, cen, SEED + 1)
     blocks = F.inclusion_blocks(3, nc, lv, cen, 0.02, 1e4)
     verts = F.coarse_vertices(3, nc)
     ub = UnevenBatch(verts, lv, dim=3, coef_blocks=blocks, timing=True)
+    t0 = time.perf_counter()
+    ub.tune()
+    t_tune = time.perf_counter() - t0
     for _ in range(warmup):
         ub.run()
     ub.synchronize()
-    for g in ub.groups.values():
+    for l in ub.level_set:
         if warmup > 0:
-            g["qb"].kernel_times()
+            ub.level(l).kernel_times()
     t0 = time.perf_counter()
     for _ in range(steps):
         ub.run()
     ub.synchronize()
     wall = time.perf_counter() - t0
+    summ = ub.summary()
     per_level = {}
-    for lvl, g in ub.groups.items():
-        kt = g["qb"].kernel_times()
-        per_level[str(lvl)] = {"cells": int(g["cells"].shape[0]),
-                               "solver": ub.summary()[lvl]["solver"],
-                               "kernel_ms": [round(t, 4) for t in kt[:4]]}
+    for l in ub.level_set:
+        kt = ub.level(l).kernel_times()
+        per_level[str(l)] = {"cells": summ[l]["cells"], "solver": summ[l]["solver"],
+                             "tuned_ms": summ[l]["tuned_ms"],
+                             "kernel_ms": [round(t, 4) for t in kt[:4]]}
     out = {"workload": "uneven 3D Q1 batch: 16^3 cells, levels 2/3 (band or CG, timed choice) "
                        "and 4 (3375 interior DoFs, matrix-free Jacobi-PCG FP64) for the cells "
-                       "holding one of 410 inclusions (A=1e4, r=0.02)",
+                       "holding one of 410 inclusions (A=1e4, r=0.02); one msfec_qu handle, "
+                       "levels forked onto their own streams",
            "cells_per_s": round(ub.n_cells / (wall / steps), 1),
            "ms_per_step": round(wall * 1e3 / steps, 4),
            "levels": per_level,
-           "autotune": {f"{k[0]}d_l{k[1]}": v for k, v in U.tuning_table().items()},
+           "tune_s": round(t_tune, 3),
            "failed_cells": int((ub.get_cell_status() != 0).sum())}
-    g4 = ub.groups[4]
-    it = g4["qb"].cg_iterations()
+    it = ub.level(4).cg_iterations()
     out["cg_iterations_l4"] = {"mean": round(float(it.mean()), 1), "max": int(it.max())}
+    cg_ms = per_level["4"]["kernel_ms"][1]
+    if summ[4]["solver"] == "cg" and cg_ms > 0:
+        fl = float(it[:, :7].sum()) * cg_flops_per_iteration(3, 4)
+        tf = fl / (cg_ms * 1e-3) / 1e12
+        out["cg_roofline"] = {"bound": "fp64 vector ALU (operands LDS-resident)",
+                              "kernel": "q_cg_kernel<3, false>", "kernel_ms": cg_ms,
+                              "flops_per_iteration_basis": cg_flops_per_iteration(3, 4),
+                              "iterations_total": int(it[:, :7].sum()),
+                              "achieved_tflops": round(tf, 3), "peak_tflops": FP64_PEAK_TFLOPS,
+                              "frac": round(tf / FP64_PEAK_TFLOPS, 4)}
     if check or cpu:
         from oracle import oracle as O
         O.build()
         rng = np.random.default_rng(11)
-        samp = {2: rng.choice(ub.groups[2]["cells"], 8, replace=False),
-                3: rng.choice(ub.groups[3]["cells"], 8, replace=False),
-                4: rng.choice(g4["cells"], 1, replace=False)}
+        cells = {l: np.nonzero(lv == l)[0] for l in (2, 3, 4)}
+        samp = {2: rng.choice(cells[2], 8, replace=False), 3: rng.choice(cells[3], 8, replace=False),
+                4: rng.choice(cells[4], 1, replace=False)}
         if check:
             M, b = ub.get_global_element_matrix(), ub.get_global_element_rhs()
+            P = ub.get_basis_flat()
             errs = {}
-            for lvl, idx in samp.items():
-                o = O.basis_batch(O.make_problem(3, lvl), verts[idx],
+            for l, idx in samp.items():
+                o = O.basis_batch(O.make_problem(3, l), verts[idx],
                                   coef=np.stack([blocks[c] for c in idx]), solver=O.SOLVER_DIRECT)
-                errs[f"l{lvl}"] = {"M": rel_err(M[idx], o["M"]), "b": rel_err(b[idx], o["b"]),
-                                   "cells": len(idx)}
+                nn = ((1 << l) + 1) ** 3
+                Pl = np.stack([P[int(ub.basis_offsets[c]):int(ub.basis_offsets[c]) + 8 * nn]
+                               for c in idx]).reshape(len(idx), 8, nn)
+                errs[f"l{l}"] = {"M": rel_err(M[idx], o["M"]), "b": rel_err(b[idx], o["b"]),
+                                 "Psi": rel_err(Pl, o["Psi"]), "cells": len(idx)}
             out["rel_err_vs_fp64_oracle"] = errs
         if cpu:
             # the reference's default CG + SSOR on a sample of each level,
             # scaled to the batch's level mix
             cores = host_cores()
             t_cell = {}
-            for lvl in (2, 3, 4):
-                idx = rng.choice(ub.groups[lvl]["cells"], cores if lvl < 4 else cores, replace=False)
+            for l in (2, 3, 4):
+                idx = rng.choice(cells[l], cores, replace=False)
                 t0 = time.perf_counter()
-                O.basis_batch(O.make_problem(3, lvl), verts[idx],
+                O.basis_batch(O.make_problem(3, l), verts[idx],
                               coef=np.stack([blocks[c] for c in idx]), solver=O.SOLVER_CG_SSOR,
                               n_threads=cores, want_psi=False)
-                t_cell[lvl] = (time.perf_counter() - t0) / len(idx)
-            total = sum(t_cell[lvl] * ub.groups[lvl]["cells"].shape[0] for lvl in (2, 3, 4))
+                t_cell[l] = (time.perf_counter() - t0) / len(idx)
+            total = sum(t_cell[l] * len(cells[l]) for l in (2, 3, 4))
             out["cpu_baseline"] = {"value": round(ub.n_cells / total, 1), "unit": "cells/s",
                                    "cores": cores, "kind": "port",
                                    "sample": f"{cores} cells per level, CG+SSOR(1.6) abs tol 1e-12, "
-                                             f"OpenMP {cores} threads, per-level times scaled to "
-                                             "the batch's level counts"}
+                                             f"OpenMP on {host_cores_text(cores)}, per-level "
+                                             "times scaled to the batch's level counts"}
     ub.close()
+    return out
+
+
+def plain_config(steps: int, warmup: int, check: bool):
+    """The plain-method configuration: 2-D -div(A grad u) + u = f on the unit
+    square, 8 x 8 coarse cells refined 4 levels (16 x 16 fine cells, 289
+    nodes, 4 bases each), A = exp(g) with g iid N(0, 1) per fine cell (seed
+    20261202), reaction 1, f = 1.  Timed: the basis construction of the 64
+    cells, and the whole method (bases, coarse assembly, coarse CG to 1e-12,
+    weights).  Checked: M, b and Psi of all 64 cells and the coarse solution
+    against the oracle."""
+    from msfec_amd import QBasisBatch, _lib as L, fields as F
+    dim, nc, l = 2, 8, 4
+    verts = F.coarse_vertices(dim, nc)
+    coef = F.lognormal_coefficient(dim, nc, l, 1.0, 0.0, SEED)
+    qb = QBasisBatch(verts, dim=dim, n_refine_local=l, solver=L.SOLVER_BAND_FP64_MFMA,
+                     reaction_mode=L.REACTION_CONST, reaction_const=1.0, timing=True)
+    qb.set_coefficient(coef)
+    for _ in range(warmup):
+        qb.run()
+    qb.synchronize()
+    if warmup > 0:
+        qb.kernel_times()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        qb.run()
+    qb.synchronize()
+    wall = time.perf_counter() - t0
+    kt = qb.kernel_times()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        qb.run()
+        u, w, it, res = qb.coarse_solve(nc, abs_tol=1e-12, rel_tol=1e-12)
+    wall_all = time.perf_counter() - t0
+    out = {"workload": "2D Q1, -div(A grad u) + u = f, 8x8 coarse cells x 16x16 fine cells "
+                       "(225 interior DoFs, 4 RHS), A=exp(g) g iid N(0,1), reaction 1, f=1; "
+                       "banded Cholesky FP64 on the matrix cores + coarse Jacobi-PCG",
+           "cells_per_s": round(64 / (wall / steps), 1),
+           "ms_per_step": round(wall * 1e3 / steps, 4),
+           "kernel_ms": {"rows": round(kt[0], 4), "factor_solve": round(kt[1], 4)},
+           "method_ms_per_step": round(wall_all * 1e3 / steps, 4),
+           "coarse_cg_iterations": it,
+           "failed_cells": int((qb.get_cell_status() != 0).sum())}
+    if check:
+        from oracle import oracle as O
+        O.build()
+        o = O.basis_batch(O.make_problem(dim, l, reaction_mode=O.REACTION_CONST, reaction_const=1.0),
+                          verts, coef=coef, solver=O.SOLVER_DIRECT)
+        M, b, P = qb.get_global_element_matrix(), qb.get_global_element_rhs(), qb.get_basis()
+        uo, _ = O.coarse_solve(dim, nc, o["M"], o["b"], rtol=1e-14)
+        out["rel_err_vs_fp64_oracle"] = {"M": rel_err(M, o["M"]), "b": rel_err(b, o["b"]),
+                                         "Psi": rel_err(P, o["Psi"]),
+                                         "u_coarse": float(np.abs(u - uo).max() / np.abs(uo).max()),
+                                         "cells_checked": 64}
+    qb.close()
+    return out
+
+
+def adjoint_config(steps: int, check: bool):
+    """The adjoint configuration: the headline batch (16^3 x 8^3, 2.1 M fine
+    log-coefficients g, a = exp(g)), J = 1/2 |u_H - u_target|^2 over the free
+    coarse nodes with u_target the coarse solution of a second seeded field
+    (sigma 1.0, iid).  One step = the forward basis construction with FP32
+    factors + FP64 refinement and the backward pass (coarse solve, adjoint
+    coarse solve, w = K_II^-1 F_I from the stored panels, the per-fine-cell
+    gradient); the gradient lands in host memory.  Checked against a central
+    difference of the forward map along a seeded direction."""
+    from msfec_amd import QBasisBatch, _lib as L, fields as F
+    nc = NC
+    verts = F.coarse_vertices(DIM, nc)
+    g = np.log(F.lognormal_coefficient(DIM, nc, NREF, SIGMA, CORR, SEED))
+    g2 = np.log(F.lognormal_coefficient(DIM, nc, NREF, 1.0, 0.0, SEED + 1))
+
+    def coarse(gv, solver):
+        q = QBasisBatch(verts, dim=DIM, n_refine_local=NREF, solver=solver)
+        q.set_coefficient(np.exp(gv))
+        q.run()
+        u, _, _, _ = q.coarse_solve(nc, abs_tol=0.0, rel_tol=1e-14)
+        q.close()
+        return u
+
+    ut = coarse(g2, L.SOLVER_BAND_FP64_MFMA)
+    qb = QBasisBatch(verts, dim=DIM, n_refine_local=NREF, solver=L.SOLVER_BAND_MIXED)
+    qb.set_coefficient(np.exp(g))
+    qb.run()
+    J, grad = qb.adjoint_gradient(nc, ut)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        qb.run()
+        J, grad = qb.adjoint_gradient(nc, ut)
+    wall = time.perf_counter() - t0
+    out = {"workload": "adjoint of J = 1/2 |u_H - u_target|^2 through the basis solves and the "
+                       "coarse CG, 16^3 coarse x 8^3 fine cells (2,097,152 log-coefficients), "
+                       "FP32 factors + FP64 refinement",
+           "ms_per_step": round(wall * 1e3 / steps, 4),
+           "gradients_per_s": round(steps / wall, 2),
+           "J": J}
+    if check:
+        free = np.ones((nc + 1,) * DIM, dtype=bool)
+        free[0] = free[-1] = False
+        free[:, 0] = free[:, -1] = False
+        free[:, :, 0] = free[:, :, -1] = False
+        free = free.ravel()
+
+        def misfit(gv):
+            u = coarse(gv, L.SOLVER_BAND_MIXED)
+            return 0.5 * float(((u - ut)[free] ** 2).sum())
+
+        dg = np.random.default_rng(4).standard_normal(g.shape)
+        eps = 1e-4
+        fd = (misfit(g + eps * dg) - misfit(g - eps * dg)) / (2 * eps)
+        ad = float((grad * dg).sum())
+        out["fd_check"] = {"directional_fd": fd, "directional_adjoint": ad,
+                           "abs_diff_over_grad_norm_dg_norm":
+                               abs(fd - ad) / (np.linalg.norm(grad) * np.linalg.norm(dg)),
+                           "rel_diff": abs(fd - ad) / abs(fd), "eps": eps}
+    qb.close()
     return out
 
 
@@ -430,6 +602,10 @@ def main():
                     help="skip the 2-D Raviart-Thomas x DQ0 configuration timed beside the headline")
     ap.add_argument("--no-uneven", action="store_true",
                     help="skip the uneven 3-D configuration timed beside the headline")
+    ap.add_argument("--no-plain", action="store_true",
+                    help="skip the 2-D 8x8 plain-method configuration timed beside the headline")
+    ap.add_argument("--no-adjoint", action="store_true",
+                    help="skip the adjoint configuration timed beside the headline")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run MIB-sized device copies (counter calibration)")
     args = ap.parse_args()
@@ -598,6 +774,12 @@ def main():
         others_cfg["q_uneven_3d_16x16x16"] = uneven_config(
             max(2, args.steps // 4), min(args.warmup, 1), not args.no_check,
             not args.no_cpu_baseline)
+    if not args.no_plain:
+        others_cfg["q_plain_2d_8x8_l4"] = plain_config(max(2, args.steps // 2),
+                                                       min(args.warmup, 2), not args.no_check)
+    if not args.no_adjoint:
+        others_cfg["q_adjoint_3d_16x16x16"] = adjoint_config(max(2, args.steps // 10),
+                                                             not args.no_check)
     if others_cfg:
         out["other_configs"] = others_cfg
     print(json.dumps(out))

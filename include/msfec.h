@@ -76,10 +76,12 @@ enum { MSFEC_REACTION_NONE = 0, MSFEC_REACTION_CONST = 1, MSFEC_REACTION_CELL = 
  *                  one workgroup per (cell, basis), no bandwidth limit (3-D up
  *                  to n_refine_local = 4, 2-D up to 6), stops at
  *                  ||r|| <= cg_tol ||b|| (default 1e-13) or after cg_max_iter
- *                  iterations (default 5000); the element matrix uses the
- *                  energy form of the Schur term, second order in the solve's
- *                  error (scalar coefficient per fine cell and constant or
- *                  per-cell load only). */
+ *                  iterations (default 5000); the element matrix is second
+ *                  order in the solve's error (the energy form of the Schur
+ *                  term for a scalar coefficient per fine cell; Psi^T K Psi
+ *                  with the assembled matrix for tensors at the Gauss points
+ *                  or loads per Gauss point, where K p is applied from the
+ *                  assembled stencil). */
 enum { MSFEC_SOLVER_BAND_FP64 = 0, MSFEC_SOLVER_BAND_FP64_MFMA = 1, MSFEC_SOLVER_BAND_MIXED = 2,
        MSFEC_SOLVER_CG = 3 };
 /* per-cell status codes.  NOT_CONVERGED: BAND_MIXED stopped after
@@ -106,6 +108,10 @@ typedef struct {
   int max_refine;      /* BAND_MIXED: refinement steps at most (0 = 3)     */
   double cg_tol;       /* CG: relative residual to reach (0 = 1e-13)       */
   int cg_max_iter;     /* CG: iterations at most (0 = 5000)                */
+  int band_window;     /* BAND_FP64_MFMA, scalar coefficient: slots of the
+                          band window, 16, 32 or 64 (it must hold the
+                          half-bandwidth + 4); 0 = the smallest that does.
+                          The other solvers keep 64.                       */
 } msfec_q_params;
 
 typedef struct {
@@ -115,6 +121,7 @@ typedef struct {
   int n_fine_cells;    /* 2^(l dim)                                        */
   int n_interior;      /* (2^l - 1)^dim                                    */
   int bandwidth;       /* half-bandwidth of the interior system            */
+  int band_window;     /* slots of the matrix-core band window in use      */
   size_t coef_count;   /* doubles expected by msfec_q_set_coefficient      */
   size_t rhs_count;    /* doubles expected by msfec_q_set_rhs (0: none)    */
   size_t reaction_count;
@@ -225,6 +232,87 @@ msfec_status msfec_q_adjoint_gradient(msfec_q_handle h, int nc, const double *u_
                                       double *J, double *grad, int grad_loc);
 
 /* ---------------------------------------------------------------------------
+ * Uneven batches: coarse cells with their own refinement level.
+ *
+ * The reference refines every cell by the one n_refine_local of the
+ * parameter file (source/Q/q_parameters.cc:229, q_basis.cc:101); here each
+ * cell of one std::map<CellId, QBasis> (q_global.cc:38-87) may carry its own
+ * level.  One handle holds one batch per level present, each with its own
+ * local solver:
+ *   - solver MSFEC_SOLVER_AUTO: where the interior band fits the band window
+ *     (3-D l <= 3, 2-D l <= 5) the fastest of the FP64 solvers and tile
+ *     shapes (matrix-core band with a 16-, 32- or 64-slot window, vector-ALU
+ *     band, CG), timed on up to 256 of the level's own cells
+ *     with the current coefficients (a candidate whose cells do not all come
+ *     out MSFEC_CELL_OK is rejected); matrix-free CG beyond the window.  The
+ *     choice is made by msfec_qu_tune, or by the first compute when no tuning
+ *     was done (that compute synchronizes, so a graph capture comes after an
+ *     eager compute); it is kept per handle until msfec_qu_tune is called
+ *     again (for instance after a coefficient field of different contrast);
+ *   - any other solver: that solver where it applies, CG beyond the window.
+ * compute forks the levels onto streams of their own (default priority)
+ * after the caller's stream and joins them back, so the levels overlap; the
+ * fork and join are event waits, so a compute can be captured into a graph.
+ * Layouts, cells in batch order:
+ *   coefficient (and per-cell load / reaction): cell c holds its N_c^dim
+ *     values (N_c = 2^levels[c]) at coef_offsets[c];
+ *   Psi: cell c holds nb x (N_c + 1)^dim values at basis_offsets[c];
+ *   M, b, status: n_cells x nb x nb, n_cells x nb, n_cells as for msfec_q.
+ * Only MSFEC_COEF_SCALAR_CELL and MSFEC_RHS_CONST / _CELL are taken. */
+enum { MSFEC_SOLVER_AUTO = 4 };
+
+typedef struct msfec_qu_batch *msfec_qu_handle;
+
+/* params as for msfec_q_create (n_refine_local is ignored), levels: n_cells
+ * ints on the host.  Replaces QBasis::QBasis for every cell of the map
+ * (source/Q/q_basis.cc:7-56, q_global.cc:38-71) with a level per cell. */
+msfec_status msfec_qu_create(const msfec_q_params *params, int n_cells,
+                             const double *vertices, int vertices_loc,
+                             const int *levels, msfec_qu_handle *out);
+/* n_cells + 1 prefix offsets (host) of the coefficient and Psi layouts;
+ * either pointer may be NULL. */
+msfec_status msfec_qu_offsets(msfec_qu_handle h, size_t *coef_offsets,
+                              size_t *basis_offsets);
+/* count = coef_offsets[n_cells]. */
+msfec_status msfec_qu_set_coefficient(msfec_qu_handle h, const double *data,
+                                      size_t count, int loc);
+msfec_status msfec_qu_set_rhs(msfec_qu_handle h, const double *data, size_t count,
+                              int loc);
+msfec_status msfec_qu_set_reaction(msfec_qu_handle h, const double *data,
+                                   size_t count, int loc);
+/* Time the candidate solvers of every level (MSFEC_SOLVER_AUTO) on the
+ * current coefficients and keep the fastest; synchronous. */
+msfec_status msfec_qu_tune(msfec_qu_handle h);
+/* The solver of a level present in the batch, its band window (slots; 0
+ * for the solvers without one), its cell count and, when tuned, the timed
+ * milliseconds of the 5 candidates (matrix-core band with a 16-, 32- and
+ * 64-slot window, vector-ALU band, CG; -1 where not timed, not applicable or
+ * rejected).  Any pointer may be NULL. */
+msfec_status msfec_qu_level_info(msfec_qu_handle h, int level, int *solver, int *window,
+                                 int *n_cells, float *ms5);
+/* The batch of one level, for read-only queries through msfec_q_get_info,
+ * msfec_q_kernel_times (params.flags MSFEC_FLAG_TIMING) and
+ * msfec_q_get_cg_iterations; owned by h (do not destroy or set data). */
+msfec_status msfec_qu_level_handle(msfec_qu_handle h, int level, msfec_q_handle *out);
+/* QBasis::run (source/Q/q_basis.cc:518-603) for every cell of every level;
+ * asynchronous on stream. */
+msfec_status msfec_qu_compute(msfec_qu_handle h, void *stream);
+msfec_status msfec_qu_synchronize(msfec_qu_handle h);
+/* As msfec_q_get_element / _get_basis / _get_cell_status, cells in batch
+ * order (Psi in the ragged layout above).  Synchronize. */
+msfec_status msfec_qu_get_element(msfec_qu_handle h, double *M, double *b, int loc);
+msfec_status msfec_qu_get_basis(msfec_qu_handle h, double *Psi, int loc);
+msfec_status msfec_qu_get_cell_status(msfec_qu_handle h, int *status);
+/* As msfec_coarse_solve for a batch whose cells are the nc^dim coarse
+ * cells in lexicographic order, whatever their levels (the coarse problem
+ * sees only M and b; q_global.cc:198-392). */
+msfec_status msfec_qu_coarse_solve(msfec_qu_handle h, int nc, double abs_tol,
+                                   double rel_tol, int max_iter, double *u, int u_loc,
+                                   double *weights, int weights_loc, int *iterations,
+                                   double *residual);
+void msfec_qu_destroy(msfec_qu_handle h);
+
+/* ---------------------------------------------------------------------------
  * Q_Ned: the mixed Q1 x lowest-order Nedelec multiscale basis, two-dimensional
  * (the reference's QNedBasis, source/Q_Ned/q_ned_basis.cc, is its 3-D form).
  * Per coarse box cell and fine cell (2^l x 2^l, lexicographic, x fastest):
@@ -239,9 +327,18 @@ msfec_status msfec_q_adjoint_gradient(msfec_q_handle h, int nc, const double *u_
  * vertical edges (edge (i,j) at j (m+1) + i).
  * ------------------------------------------------------------------------- */
 typedef struct {
-  int n_refine_local; /* l, 1..4 (the interior half-bandwidth must stay <= 63) */
+  int n_refine_local; /* l, 1..4 (the interior half-bandwidth must stay <= 60) */
   int full_rhs;       /* QNedBasis' full_rhs parameter (q_ned_basis.cc:432-444) */
   int flags;          /* MSFEC_FLAG_*                                          */
+  int solver;         /* 0 or MSFEC_SOLVER_BAND_FP64_MFMA: FP64 LDL^T on the
+                         matrix cores; MSFEC_SOLVER_BAND_MIXED: FP32 LDL^T on
+                         the matrix cores and refinement with FP64 residuals
+                         until the predicted error |dx| / |x| times the
+                         contraction (|dx| / |x| over its previous value) is
+                         below 1e-14 per cell, at most max_refine passes
+                         (0 = 5), the cells still above 1e-12 flagged
+                         MSFEC_CELL_NOT_CONVERGED                          */
+  int max_refine;
 } msfec_ned_params;
 
 typedef struct msfec_ned_batch *msfec_ned_handle;

@@ -35,7 +35,7 @@ struct ParametersBasis {
   double rhs_const = 1.0;
   int reaction_mode = MSFEC_REACTION_NONE;
   double reaction_const = 0.0;
-  int solver = MSFEC_SOLVER_BAND_FP64;
+  int solver = MSFEC_SOLVER_BAND_FP64;  // MSFEC_SOLVER_AUTO: per-level timing (uneven batches)
   int max_refine = 0;  // BAND_MIXED: refinement steps at most (0 = library default)
   double cg_tol = 0.0;  // CG: relative residual (0 = library default)
   int cg_max_iter = 0;  // CG: iterations at most (0 = library default)
@@ -55,9 +55,13 @@ class QBatch {
 
   // Register a cell: 2^dim vertices (deal.II order) and its coefficient
   // block (layout of msfec_q_set_coefficient), optional load / reaction
-  // blocks.  Returns the batch slot.
+  // blocks, and its refinement level (-1: prm.n_refine_local).  Cells of
+  // different levels make an uneven batch (msfec_qu_*), whose per-level
+  // solver is chosen by timing unless prm.solver fixes it.  Returns the
+  // batch slot.
   std::size_t add_cell(const std::vector<double>& vertices, const std::vector<double>& coef,
-                       const std::vector<double>& f = {}, const std::vector<double>& reac = {});
+                       const std::vector<double>& f = {}, const std::vector<double>& reac = {},
+                       int level = -1);
 
   // Compute every registered cell (idempotent until a new cell is added).
   void compute();
@@ -65,7 +69,10 @@ class QBatch {
 
   std::size_t n_cells() const { return n_cells_; }
   int n_basis() const { return 1 << prm_.dim; }
-  int n_fine_nodes() const { return n_fine_nodes_; }
+  // fine nodes of the cell in a slot ((2^level + 1)^dim)
+  int n_fine_nodes(std::size_t slot = 0) const;
+  int level(std::size_t slot) const { return levels_[slot]; }
+  bool uneven() const;
   const ParametersBasis& parameters() const { return prm_; }
 
   const double* element_matrix(std::size_t slot) const;   // nb x nb
@@ -84,11 +91,13 @@ class QBatch {
  private:
   ParametersBasis prm_;
   std::size_t n_cells_ = 0;
-  int n_fine_nodes_ = 0;
+  std::vector<int> levels_;
   std::vector<double> verts_, coef_, f_, reac_;
   std::vector<double> M_, b_, psi_;
+  std::vector<std::size_t> psi_off_;  // per slot: offset of its Psi block
   std::vector<int> status_;
-  msfec_q_handle handle_ = nullptr;
+  msfec_q_handle handle_ = nullptr;   // one level
+  msfec_qu_handle uhandle_ = nullptr; // mixed levels
   bool computed_ = false;
 };
 
@@ -99,10 +108,11 @@ class QBasis {
   // replaces QBasis(parameters_ms, prm_file, cell, first_cell, subdomain, comm)
   // (q_basis.cc:7-56).
   // cell_id names the cell in output file names (CellId::to_string in the
-  // reference); empty means the batch slot.
+  // reference); empty means the batch slot.  level: the cell's own
+  // n_refine_local (-1: the batch parameter's).
   QBasis(std::shared_ptr<QBatch> batch, const std::vector<double>& corner_points,
          const std::vector<double>& coef, const std::vector<double>& f = {},
-         const std::vector<double>& reac = {}, const std::string& cell_id = "");
+         const std::vector<double>& reac = {}, const std::string& cell_id = "", int level = -1);
   QBasis(const QBasis& other) = default;
   QBasis& operator=(const QBasis& other) = default;
 

@@ -52,6 +52,8 @@ void QNedBatch::compute() {
   p.n_refine_local = (int)prm_.n_refine_local;
   p.full_rhs = prm_.full_rhs ? 1 : 0;
   p.flags = 0;
+  p.solver = prm_.solver;
+  p.max_refine = prm_.max_refine;
   check(msfec_ned_create(&p, (int)n_cells_, verts_.data(), MSFEC_HOST, &handle_),
         "msfec_ned_create");
   check(msfec_ned_set_coefficients(handle_, a_.data(), binv_.data(), r_.data(), f_.data(),

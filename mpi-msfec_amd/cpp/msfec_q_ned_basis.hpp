@@ -19,6 +19,11 @@ namespace msfec {
 struct ParametersNed {
   unsigned n_refine_local = 4;  // ParametersMs::n_refine_local
   bool full_rhs = true;         // ParametersMs::full_rhs (q_ned_basis.cc:432-444)
+  // local solver: MSFEC_SOLVER_BAND_FP64_MFMA (FP64 LDL^T) or
+  // MSFEC_SOLVER_BAND_MIXED (FP32 LDL^T + FP64 refinement, at most
+  // max_refine passes, 0 = 3); both replace solve_direct (q_ned_basis.cc:570-625)
+  int solver = MSFEC_SOLVER_BAND_FP64_MFMA;
+  int max_refine = 0;
   // output (the ParametersBasis fields of include/Q_Ned/q_ned_parameters.h)
   std::string dirname_output = "Q_Ned";
   std::string filename_output = "solution";

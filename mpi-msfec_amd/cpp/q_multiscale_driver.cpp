@@ -7,10 +7,13 @@
 // Usage: q_multiscale_driver <in.bin> <out.bin> [output directory]
 //   in.bin : int32 dim, int32 n_refine_local, int32 nc, then the coefficient
 //            blocks (nc^dim cells x (2^l)^dim fine cells, FP64, per-cell
-//            lexicographic as in msfec_q_set_coefficient)
+//            lexicographic as in msfec_q_set_coefficient).  n_refine_local
+//            = 0 marks an uneven batch: int32 levels of the nc^dim cells
+//            follow the header, and cell c's block has (2^levels[c])^dim
+//            values (the solver of each level is then chosen by timing)
 //   out.bin: M (cells x nb x nb), b (cells x nb), weights (cells x nb),
-//            fine solution (cells x fine nodes), all FP64, then int32 CG
-//            iterations
+//            fine solution (cells x their fine nodes), all FP64, then int32
+//            CG iterations
 //   output directory (optional): the VTU files of output_basis (first cell)
 //            and output_global_solution_in_cell (every cell), as
 //            QMultiscale::output_results has the cells write them
@@ -33,20 +36,26 @@ int main(int argc, char** argv) {
   int32_t hdr[3];
   if (std::fread(hdr, sizeof(int32_t), 3, in) != 3) return 2;
   const int dim = hdr[0], l = hdr[1], nc = hdr[2];
-  const int nv = 1 << dim, N = 1 << l;
-  int ncell = 1, nfc = 1;
-  for (int d = 0; d < dim; ++d) {
-    ncell *= nc;
-    nfc *= N;
+  const int nv = 1 << dim;
+  int ncell = 1;
+  for (int d = 0; d < dim; ++d) ncell *= nc;
+  std::vector<int32_t> levels(ncell, l);
+  if (l <= 0 && std::fread(levels.data(), sizeof(int32_t), ncell, in) != (size_t)ncell) return 2;
+  std::vector<size_t> coff(ncell + 1, 0);
+  for (int c = 0; c < ncell; ++c) {
+    size_t nfc = 1;
+    for (int d = 0; d < dim; ++d) nfc <<= levels[c];
+    coff[c + 1] = coff[c] + nfc;
   }
-  std::vector<double> coef((size_t)ncell * nfc);
+  std::vector<double> coef(coff[ncell]);
   if (std::fread(coef.data(), sizeof(double), coef.size(), in) != coef.size()) return 2;
   std::fclose(in);
 
   try {
     msfec::ParametersBasis prm;
     prm.dim = dim;
-    prm.n_refine_local = (unsigned)l;
+    prm.n_refine_local = (unsigned)(l > 0 ? l : levels[0]);
+    if (l <= 0) prm.solver = MSFEC_SOLVER_AUTO;
     if (argc == 4) prm.dirname_output = argv[3];
     auto batch = std::make_shared<msfec::QBatch>(prm);
 
@@ -63,8 +72,8 @@ int main(int argc, char** argv) {
       std::vector<double> corners((size_t)nv * dim);
       for (int v = 0; v < nv; ++v)
         for (int d = 0; d < dim; ++d) corners[v * dim + d] = (cc[d] + ((v >> d) & 1)) * h;
-      std::vector<double> cblock(coef.begin() + (size_t)c * nfc, coef.begin() + (size_t)(c + 1) * nfc);
-      msfec::QBasis current_cell_problem(batch, corners, cblock);
+      std::vector<double> cblock(coef.begin() + coff[c], coef.begin() + coff[c + 1]);
+      msfec::QBasis current_cell_problem(batch, corners, cblock, {}, {}, "", levels[c]);
       auto res = cell_basis_map.insert(std::make_pair(c, current_cell_problem));
       if (!res.second) throw std::runtime_error("insertion into the cell map failed");
     }

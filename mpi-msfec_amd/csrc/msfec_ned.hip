@@ -49,8 +49,14 @@ extern thread_local std::string g_last_error;
 // the matrix-core band LDL^T (msfec_q.hip)
 size_t band_ldl_panel_doubles(int n);
 size_t band_ldl_z_doubles(int n);
-hipError_t band_ldl_solve(int n_cells, int n, const double* rows, const int* loff, double* pan,
-                          double* zst, double* X, int* status, hipStream_t s);
+hipError_t band_ldl_solve(int n_cells, int n, int bw, const double* rows, const int* loff,
+                          double* pan, double* zst, double* X, int* status, hipStream_t s);
+// the same with an FP32 factor and FP64 refinement (msfec_q.hip)
+size_t band_ldl_mixed_doubles(int n);
+std::vector<int> band_upper_table(const std::vector<int>& loff, int n);
+hipError_t band_ldl_mixed_solve(int n_cells, int n, int bw, const double* rows, const int* loff,
+                                const int* utab, double* pan, double* zst, double* X,
+                                double* scratch, int* status, int max_refine, hipStream_t s);
 }
 
 namespace {
@@ -601,6 +607,8 @@ struct msfec_ned_batch {
   int* d_loff = nullptr;
   signed char* d_cslot = nullptr;
   double* d_gbnd = nullptr;
+  int* d_utab = nullptr;       // mixed solver: transposed offsets of the band rows
+  double* d_mixed = nullptr;   // mixed solver: residual, correction, refinement flags
   bool coef_set = false, computed = false;
   hipStream_t stream = nullptr;
   std::vector<std::array<hipEvent_t, 4>> ev_sets;
@@ -618,7 +626,8 @@ msfec_status ned_alloc(T** p, size_t count) {
 void ned_free(msfec_ned_batch* h) {
   void* ptrs[] = {h->d_verts, h->d_a,   h->d_binv, h->d_r,   h->d_f,     h->d_rows,
                   h->d_pan,   h->d_zst, h->d_X,    h->d_scale, h->d_psi, h->d_M,
-                  h->d_b,     h->d_status, h->d_order, h->d_pos, h->d_loff, h->d_cslot, h->d_gbnd};
+                  h->d_b,     h->d_status, h->d_order, h->d_pos, h->d_loff, h->d_cslot, h->d_gbnd,
+                  h->d_utab,  h->d_mixed};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (auto& set : h->ev_sets)
@@ -637,10 +646,13 @@ msfec_status ned_in(hipStream_t s, double* dst, const double* src, size_t count,
   return MSFEC_OK;
 }
 
-msfec_status ned_out(double* dst, const double* src, size_t count, int loc) {
+// Copied on the handle's stream and waited for: a device destination holds
+// the data when the call returns (a device-to-device hipMemcpy may return first).
+msfec_status ned_out(hipStream_t s, double* dst, const double* src, size_t count, int loc) {
   if (!dst) return ned_fail(MSFEC_ERR_INVALID, "null output pointer");
-  NED_HIP(hipMemcpy(dst, src, count * sizeof(double),
-                    loc == MSFEC_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost));
+  NED_HIP(hipMemcpyAsync(dst, src, count * sizeof(double),
+                         loc == MSFEC_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
+  NED_HIP(hipStreamSynchronize(s));
   return MSFEC_OK;
 }
 
@@ -687,6 +699,11 @@ msfec_status msfec_ned_create(const msfec_ned_params* params, int n_cells, const
   if (l < 1 || l > 6) return ned_fail(MSFEC_ERR_INVALID, "n_refine_local must be in 1..6");
   if (vertices_loc != MSFEC_HOST && vertices_loc != MSFEC_DEVICE)
     return ned_fail(MSFEC_ERR_INVALID, "bad vertices location");
+  if (params->solver != 0 && params->solver != MSFEC_SOLVER_BAND_FP64_MFMA &&
+      params->solver != MSFEC_SOLVER_BAND_MIXED)
+    return ned_fail(MSFEC_ERR_INVALID, "solver must be BAND_FP64_MFMA or BAND_MIXED");
+  if (params->max_refine < 0 || params->max_refine > 16)
+    return ned_fail(MSFEC_ERR_INVALID, "max_refine must be in 0..16");
   std::vector<double> hv((size_t)n_cells * 8);
   if (vertices_loc == MSFEC_DEVICE) {
     NED_HIP(hipMemcpy(hv.data(), vertices, hv.size() * sizeof(double), hipMemcpyDeviceToHost));
@@ -742,6 +759,17 @@ msfec_status msfec_ned_create(const msfec_ned_params* params, int n_cells, const
   NED_TRY(ned_alloc(&h->d_loff, c.loff.size()));
   NED_TRY(ned_alloc(&h->d_cslot, c.cslot.size()));
   NED_TRY(ned_alloc(&h->d_gbnd, c.gbnd.size()));
+  std::vector<int> utab;
+  if (h->prm.solver == MSFEC_SOLVER_BAND_MIXED) {
+    utab = msfec_detail::band_upper_table(c.loff, c.n);
+    if (utab.empty()) {
+      ned_free(h);
+      delete h;
+      return ned_fail(MSFEC_ERR_UNSUPPORTED, "band rows with more than 16 upper entries");
+    }
+    NED_TRY(ned_alloc(&h->d_utab, utab.size()));
+    NED_TRY(ned_alloc(&h->d_mixed, nc * msfec_detail::band_ldl_mixed_doubles(c.n)));
+  }
 #undef NED_TRY
   auto up = [&](void* dst, const void* src, size_t bytes) -> msfec_status {
     NED_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
@@ -754,6 +782,7 @@ msfec_status msfec_ned_create(const msfec_ned_params* params, int n_cells, const
   if (s2 == MSFEC_OK) s2 = up(h->d_loff, c.loff.data(), c.loff.size() * sizeof(int));
   if (s2 == MSFEC_OK) s2 = up(h->d_cslot, c.cslot.data(), c.cslot.size());
   if (s2 == MSFEC_OK) s2 = up(h->d_gbnd, c.gbnd.data(), c.gbnd.size() * sizeof(double));
+  if (s2 == MSFEC_OK && !utab.empty()) s2 = up(h->d_utab, utab.data(), utab.size() * sizeof(int));
   if (s2 != MSFEC_OK) {
     ned_free(h);
     delete h;
@@ -799,8 +828,13 @@ msfec_status msfec_ned_compute(msfec_ned_handle h, void* stream) {
   hipLaunchKernelGGL(ned_rows_kernel, dim3(h->n_cells), dim3(kNThreads), 0, s, a);
   NED_HIP(hipGetLastError());
   if (ev) NED_HIP(hipEventRecord((*ev)[1], s));
-  NED_HIP(msfec_detail::band_ldl_solve(h->n_cells, h->conf.n, h->d_rows, h->d_loff, h->d_pan,
-                                       h->d_zst, h->d_X, h->d_status, s));
+  if (h->prm.solver == MSFEC_SOLVER_BAND_MIXED)
+    NED_HIP(msfec_detail::band_ldl_mixed_solve(h->n_cells, h->conf.n, h->conf.p, h->d_rows,
+                                               h->d_loff, h->d_utab, h->d_pan, h->d_zst, h->d_X,
+                                               h->d_mixed, h->d_status, h->prm.max_refine, s));
+  else
+    NED_HIP(msfec_detail::band_ldl_solve(h->n_cells, h->conf.n, h->conf.p, h->d_rows, h->d_loff,
+                                         h->d_pan, h->d_zst, h->d_X, h->d_status, s));
   if (ev) NED_HIP(hipEventRecord((*ev)[2], s));
   hipLaunchKernelGGL(ned_element_kernel, dim3(h->n_cells), dim3(kNThreads), elem_lds, s, a);
   NED_HIP(hipGetLastError());
@@ -820,8 +854,8 @@ msfec_status msfec_ned_get_element(msfec_ned_handle h, double* M, double* b, int
   if (!h->computed) return ned_fail(MSFEC_ERR_STATE, "compute not called");
   NED_HIP(hipStreamSynchronize(h->stream));
   msfec_status st = MSFEC_OK;
-  if (M) st = ned_out(M, h->d_M, (size_t)h->n_cells * 64, loc);
-  if (st == MSFEC_OK && b) st = ned_out(b, h->d_b, (size_t)h->n_cells * kNR, loc);
+  if (M) st = ned_out(h->stream, M, h->d_M, (size_t)h->n_cells * 64, loc);
+  if (st == MSFEC_OK && b) st = ned_out(h->stream, b, h->d_b, (size_t)h->n_cells * kNR, loc);
   return st;
 }
 
@@ -829,7 +863,7 @@ msfec_status msfec_ned_get_basis(msfec_ned_handle h, double* Psi, int loc) {
   if (!h) return ned_fail(MSFEC_ERR_INVALID, "null handle");
   if (!h->computed) return ned_fail(MSFEC_ERR_STATE, "compute not called");
   NED_HIP(hipStreamSynchronize(h->stream));
-  return ned_out(Psi, h->d_psi, (size_t)h->n_cells * kNR * h->conf.N, loc);
+  return ned_out(h->stream, Psi, h->d_psi, (size_t)h->n_cells * kNR * h->conf.N, loc);
 }
 
 msfec_status msfec_ned_get_cell_status(msfec_ned_handle h, int* status) {

@@ -47,10 +47,23 @@ constexpr int kPStride = 80;   // per cell: P (8 x 8) | b_B (8) | |cell| / 2^dim
 // runs every super-step of its last tile row).
 inline int panel_quads(int n) { return ((n + 15) / 16) * 4; }
 
+// Tile rows (16 slots each) of the band window: the incoming rows enter the
+// slots of the 4 pivots just eliminated, so the half-bandwidth must stay 4
+// below the window (bw + 4 <= 16 NT).  requested: 16, 32, 64 slots, or 0 for
+// the smallest window that holds the band; 0 is returned when none does.
+inline int band_tile_rows(int bw, int requested) {
+  for (int nt : {1, 2, 4}) {
+    if (requested != 0 && requested != 16 * nt) continue;
+    if (bw + 4 <= 16 * nt) return nt;
+  }
+  return 0;
+}
+
 struct QConf {
   int dim, N, S, nn, nfc, nv, ni, n, nst, nlow, center;
   int coef_mode, rhs_mode, reaction_mode;
   double rhs_const, reaction_const;
+  int nt;                  // band window of the matrix-core kernels: 16 nt slots
   int off_full[kMaxLow];   // full-grid offset (>0) of lower neighbour t
   int off_int[kMaxLow];    // interior-grid offset (>0) of lower neighbour t
   int dir_low[kMaxLow];    // direction index e of lower neighbour t (t>0)
@@ -1734,14 +1747,14 @@ __device__ __forceinline__ BkAddr bk_addr() {
   return ad;
 }
 
-template <typename R, int T, int G>
+template <typename R, int NT, int T, int G>
 __device__ __forceinline__ void bk_extract(const BkAddr& ad, const typename Mf<R>::v4 (&Tl)[10],
                                            R* piv) {
 #pragma unroll
   for (int Q = 0; Q <= T; ++Q) piv[ad.row + 16 * Q] = Tl[MF_TILE(T, Q)][G];
   if (ad.gcol[G]) {
 #pragma unroll
-    for (int P = T + 1; P < 4; ++P)
+    for (int P = T + 1; P < NT; ++P)
 #pragma unroll
       for (int r = 0; r < 4; ++r) piv[ad.col + 16 * P + 4 * r] = Tl[MF_TILE(P, T)][r];
   }
@@ -1762,7 +1775,7 @@ __device__ __forceinline__ double bk_scatter_load(const BkAddr& ad, int rb, Band
   return sh.stg_low[rb + ad.lh][ad.ll];
 }
 
-template <typename R, int DIM, int T, int G, bool LDL>
+template <typename R, int DIM, int NT, int T, int G, bool LDL>
 __device__ __forceinline__ void bk_scatter_store(const BandArgs& a, const BkAddr& ad, int jbase,
                                                  int rb, int my_off, double v, BandSmem& sh,
                                                  BkSmem<R>& ms) {
@@ -1783,14 +1796,14 @@ __device__ __forceinline__ void bk_scatter_store(const BandArgs& a, const BkAddr
   R* dummy = reinterpret_cast<R*>(SH_PART(sh)) + lane;
   constexpr int W = bk_row<R>();
   // slot of column jn - off (jbase = s0 mod 64)
-  const int i1 = w1 ? t * W + ((bk_opaque(jbase) + t - my_off) & (kWin - 1)) : t * W + s0 + t;
+  const int i1 = w1 ? t * W + ((bk_opaque(jbase) + t - my_off) & (16 * NT - 1)) : t * W + s0 + t;
   R* p1 = (w1 || w3) ? &ms.nrow[i1] : dummy;
   *p1 = w1 ? (R)v : (R)1;
   R* p2 = w2 ? &ms.nrow[max(t - my_off, 0) * W + s0 + t] : dummy + 64;
   *p2 = (R)v;
 }
 
-template <typename R, int DIM, int T, int G>
+template <typename R, int DIM, int NT, int T, int G>
 __device__ __forceinline__ void bk_place(const BandArgs& a, const BkAddr& ad, int jbase, int rb,
                                          typename Mf<R>::v4 (&Tl)[10],
                                          typename Mf<R>::v4 (&Yt)[4], BandSmem& sh,
@@ -1799,22 +1812,22 @@ __device__ __forceinline__ void bk_place(const BandArgs& a, const BkAddr& ad, in
   for (int Q = 0; Q <= T; ++Q) Tl[MF_TILE(T, Q)][G] = ms.nrow[ad.row + 16 * Q];
   if (ad.gcol[G]) {
 #pragma unroll
-    for (int P = T; P < 4; ++P)
+    for (int P = T; P < NT; ++P)
 #pragma unroll
       for (int r = 0; r < 4; ++r) Tl[MF_TILE(P, T)][r] = ms.nrow[ad.col + 16 * P + 4 * r];
   }
   Yt[T][G] = (ad.ll < 8 && jbase + ad.lh < a.n) ? (R)sh.stg_rhs[rb + ad.lh][ad.ll & 7] : (R)0;
 }
 
-template <typename R, int DIM, int T, int G, bool LDL>
+template <typename R, int DIM, int NT, int T, int G, bool LDL>
 __device__ __forceinline__ void bk_insert(const BandArgs& a, const BkAddr& ad, int jbase, int rb,
                                           int my_off, typename Mf<R>::v4 (&Tl)[10],
                                           typename Mf<R>::v4 (&Yt)[4], BandSmem& sh,
                                           BkSmem<R>& ms) {
   const double v = bk_scatter_load<R>(ad, rb, sh, ms);
-  bk_scatter_store<R, DIM, T, G, LDL>(a, ad, jbase, rb, my_off, v, sh, ms);
+  bk_scatter_store<R, DIM, NT, T, G, LDL>(a, ad, jbase, rb, my_off, v, sh, ms);
   wave_sync();
-  bk_place<R, DIM, T, G>(a, ad, jbase, rb, Tl, Yt, sh, ms);
+  bk_place<R, DIM, NT, T, G>(a, ad, jbase, rb, Tl, Yt, sh, ms);
   wave_sync();
 }
 
@@ -1883,12 +1896,12 @@ __device__ __forceinline__ double bk_rcp(double x) {
 // overwrites (row and column T), then, at the last group of the tile row,
 // those that the next super-step extracts (row and column T + 1), then the
 // rest, so that the next steps wait for as few matrix-core results as possible.
-template <int T, int G>
+template <int NT, int T, int G>
 __device__ constexpr int bk_prio(int P, int Q) {
-  return (P == T || Q == T) ? 0 : (G == 3 && (P == ((T + 1) & 3) || Q == ((T + 1) & 3))) ? 1 : 2;
+  return (P == T || Q == T) ? 0 : (G == 3 && (P == ((T + 1) % NT) || Q == ((T + 1) % NT))) ? 1 : 2;
 }
 
-template <typename R, int DIM, int T, int G, bool LDL>
+template <typename R, int DIM, int NT, int T, int G, bool LDL>
 __device__ __forceinline__ void bk_superstep(const BandArgs& a, const BkAddr& ad, int cell, int j0,
                                              typename Mf<R>::v4 (&Tl)[10],
                                              typename Mf<R>::v4 (&Yt)[4],
@@ -1897,7 +1910,7 @@ __device__ __forceinline__ void bk_superstep(const BandArgs& a, const BkAddr& ad
   typedef Mf<R> M;
   const int lh = ad.lh, ll = ad.ll, cs = ad.cs;
   constexpr int s0 = 16 * T + 4 * G;
-  bk_extract<R, T, G>(ad, Tl, ms.piv);
+  bk_extract<R, NT, T, G>(ad, Tl, ms.piv);
   ms.yp[ad.lane] = Yt[T][G];
   wave_sync();
   const R* pv = ms.piv;
@@ -1965,7 +1978,7 @@ __device__ __forceinline__ void bk_superstep(const BandArgs& a, const BkAddr& ad
   // V = A21 C^-T, update -V V^T.  LDL^T: V = A21 L1^-T, update -V D1^-1 V^T.
   R V[4], aV[4];  // aV: the left operand, V D1^-1 for LDL^T (negated by the instruction)
 #pragma unroll
-  for (int P = 0; P < 4; ++P) {
+  for (int P = 0; P < NT; ++P) {
     constexpr int W = bk_row<R>();
     const int x = ad.vrd + 16 * P;
     V[P] = pv[0 * W + x] * x0 + pv[1 * W + x] * x1 + pv[2 * W + x] * x2 + pv[3 * W + x] * x3;
@@ -1976,12 +1989,12 @@ __device__ __forceinline__ void bk_superstep(const BandArgs& a, const BkAddr& ad
 #pragma unroll
   for (int pr = 0; pr < 3; ++pr) {
 #pragma unroll
-    for (int P = 0; P < 4; ++P) {
+    for (int P = 0; P < NT; ++P) {
 #pragma unroll
       for (int Q = 0; Q <= P; ++Q)
-        if (bk_prio<T, G>(P, Q) == pr)
+        if (bk_prio<NT, T, G>(P, Q) == pr)
           Tl[MF_TILE(P, Q)] = M::mma_neg(aV[P], V[Q], Tl[MF_TILE(P, Q)]);
-      if (bk_prio<T, G>(P, P) == pr) Yt[P] = M::mma_neg(aV[P], Z, Yt[P]);
+      if (bk_prio<NT, T, G>(P, P) == pr) Yt[P] = M::mma_neg(aV[P], Z, Yt[P]);
     }
   }
   if (!LDL) Sacc = M::mma(Z, Z, Sacc);
@@ -1992,8 +2005,8 @@ __device__ __forceinline__ void bk_superstep(const BandArgs& a, const BkAddr& ad
   R* pst = reinterpret_cast<R*>(a.pan) + ((size_t)cell * a.nq + qs) * 256;
   const int jr = bk_opaque(j0);  // = s0 mod 64
 #pragma unroll
-  for (int P = 0; P < 4; ++P) {
-    const int o = (16 * P + cs - jr) & (kWin - 1);
+  for (int P = 0; P < NT; ++P) {
+    const int o = (16 * P + cs - jr) & (16 * NT - 1);
     pst[lh * 64 + o] = (P == T && o == lh) ? xd : V[P];
   }
   if (ll < 8) reinterpret_cast<R*>(a.zst)[((size_t)cell * a.nq + qs) * 32 + lh * 8 + ll] = Z;
@@ -2014,7 +2027,7 @@ __device__ __forceinline__ void bk_stage(const BandArgs& a, int cell, int jbase,
 // >= n are isolated unit diagonals, their panels zero): with no branch
 // between them, the waits for the row prefetch leave the panel stores of the
 // last super-steps in flight instead of waiting for their completion.
-template <typename R, int DIM, int T, bool LDL>
+template <typename R, int DIM, int NT, int T, bool LDL>
 __device__ __forceinline__ void bk_tile_row(const BandArgs& a, const BkAddr& ad, int cell, int j0,
                                             int my_off,
                                             typename Mf<R>::v4 (&Tl)[10],
@@ -2022,31 +2035,32 @@ __device__ __forceinline__ void bk_tile_row(const BandArgs& a, const BkAddr& ad,
                                             typename Mf<R>::v4& Sacc, BandSmem& sh,
                                             BkSmem<R>& ms, RowPrefetch (&pf)[2], BkMon<R>& bad,
                                             const R (&ek)[4]) {
-  bk_stage<LDL>(a, cell, j0 + kWin, sh, pf[0]);
-  bk_superstep<R, DIM, T, 0, LDL>(a, ad, cell, j0, Tl, Yt, Sacc, ms, bad, ek);
-  bk_insert<R, DIM, T, 0, LDL>(a, ad, j0 + kWin, 0, my_off, Tl, Yt, sh, ms);
-  bk_superstep<R, DIM, T, 1, LDL>(a, ad, cell, j0 + 4, Tl, Yt, Sacc, ms, bad, ek);
-  bk_insert<R, DIM, T, 1, LDL>(a, ad, j0 + 4 + kWin, 4, my_off, Tl, Yt, sh, ms);
-  bk_stage<LDL>(a, cell, j0 + 8 + kWin, sh, pf[1]);
-  bk_superstep<R, DIM, T, 2, LDL>(a, ad, cell, j0 + 8, Tl, Yt, Sacc, ms, bad, ek);
-  bk_insert<R, DIM, T, 2, LDL>(a, ad, j0 + 8 + kWin, 0, my_off, Tl, Yt, sh, ms);
-  bk_superstep<R, DIM, T, 3, LDL>(a, ad, cell, j0 + 12, Tl, Yt, Sacc, ms, bad, ek);
-  bk_insert<R, DIM, T, 3, LDL>(a, ad, j0 + 12 + kWin, 4, my_off, Tl, Yt, sh, ms);
+  constexpr int W = 16 * NT;
+  bk_stage<LDL>(a, cell, j0 + W, sh, pf[0]);
+  bk_superstep<R, DIM, NT, T, 0, LDL>(a, ad, cell, j0, Tl, Yt, Sacc, ms, bad, ek);
+  bk_insert<R, DIM, NT, T, 0, LDL>(a, ad, j0 + W, 0, my_off, Tl, Yt, sh, ms);
+  bk_superstep<R, DIM, NT, T, 1, LDL>(a, ad, cell, j0 + 4, Tl, Yt, Sacc, ms, bad, ek);
+  bk_insert<R, DIM, NT, T, 1, LDL>(a, ad, j0 + 4 + W, 4, my_off, Tl, Yt, sh, ms);
+  bk_stage<LDL>(a, cell, j0 + 8 + W, sh, pf[1]);
+  bk_superstep<R, DIM, NT, T, 2, LDL>(a, ad, cell, j0 + 8, Tl, Yt, Sacc, ms, bad, ek);
+  bk_insert<R, DIM, NT, T, 2, LDL>(a, ad, j0 + 8 + W, 0, my_off, Tl, Yt, sh, ms);
+  bk_superstep<R, DIM, NT, T, 3, LDL>(a, ad, cell, j0 + 12, Tl, Yt, Sacc, ms, bad, ek);
+  bk_insert<R, DIM, NT, T, 3, LDL>(a, ad, j0 + 12 + W, 4, my_off, Tl, Yt, sh, ms);
 }
 
 // Initial fill of tile row T with the rows 16T .. 16T+15.
-template <typename R, int DIM, int T, bool LDL>
+template <typename R, int DIM, int NT, int T, bool LDL>
 __device__ __forceinline__ void bk_fill_row(const BandArgs& a, const BkAddr& ad, int cell,
                                             int my_off,
                                             typename Mf<R>::v4 (&Tl)[10],
                                             typename Mf<R>::v4 (&Yt)[4], BandSmem& sh,
                                             BkSmem<R>& ms, RowPrefetch (&pf)[2]) {
   bk_stage<LDL>(a, cell, 16 * T, sh, pf[0]);
-  bk_insert<R, DIM, T, 0, LDL>(a, ad, 16 * T + 0, 0, my_off, Tl, Yt, sh, ms);
-  bk_insert<R, DIM, T, 1, LDL>(a, ad, 16 * T + 4, 4, my_off, Tl, Yt, sh, ms);
+  bk_insert<R, DIM, NT, T, 0, LDL>(a, ad, 16 * T + 0, 0, my_off, Tl, Yt, sh, ms);
+  bk_insert<R, DIM, NT, T, 1, LDL>(a, ad, 16 * T + 4, 4, my_off, Tl, Yt, sh, ms);
   bk_stage<LDL>(a, cell, 16 * T + 8, sh, pf[1]);
-  bk_insert<R, DIM, T, 2, LDL>(a, ad, 16 * T + 8, 0, my_off, Tl, Yt, sh, ms);
-  bk_insert<R, DIM, T, 3, LDL>(a, ad, 16 * T + 12, 4, my_off, Tl, Yt, sh, ms);
+  bk_insert<R, DIM, NT, T, 2, LDL>(a, ad, 16 * T + 8, 0, my_off, Tl, Yt, sh, ms);
+  bk_insert<R, DIM, NT, T, 3, LDL>(a, ad, 16 * T + 12, 4, my_off, Tl, Yt, sh, ms);
 }
 
 // Wait for every outstanding vector memory operation (vmcnt(0); expcnt and
@@ -2105,7 +2119,7 @@ struct BkPre {
   double rp0, rp1, xp, qp;
 };
 
-template <typename R, int DIM, int MODE>
+template <typename R, int DIM, int MODE, int NT = 4>
 __device__ __forceinline__ void bk_backward(const BandArgs& a, int cell, BandSmem& sh,
                                             BkSmem<R>& ms, double* xfr, double& bint,
                                             double& sacc, double& dmax, double& xmax) {
@@ -2137,7 +2151,8 @@ __device__ __forceinline__ void bk_backward(const BandArgs& a, int cell, BandSme
   auto load = [&](int q, BkPre<R>& f) {
     if (q < 0) return;
 #pragma unroll
-    for (int u = 0; u < 4; ++u) f.pp[u] = Pc[(size_t)q * 256 + u * 64 + lane];  // [k = u][o = lane]
+    for (int u = 0; u < 4; ++u)  // [k = u][o = lane]; offsets beyond the window are zero
+      f.pp[u] = (lane < 16 * NT) ? Pc[(size_t)q * 256 + u * 64 + lane] : (R)0;
     f.zp = (lane < 32) ? Zc[(size_t)q * 32 + lane] : (R)0;
     const int lim = (n - 4 * q) * kRowStride;  // valid doubles of the 4 rows
     if (MODE == BK_REFINE) {  // the 4 whole rows (the lifting right-hand sides too)
@@ -2208,7 +2223,7 @@ __device__ __forceinline__ void bk_backward(const BandArgs& a, int cell, BandSme
     const R* vw = pb + k0 * kPanRow + 4 + c;
     double s00 = 0.0, s01 = 0.0, s10 = 0.0, s11 = 0.0;
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
+    for (int i = 0; i < 2 * NT; ++i) {
       const double v0 = (double)vw[8 * i], v1 = (double)vw[kPanRow + 8 * i];
       const double x0 = xw[8 * i], x1 = xw[kRing + 8 * i];
       s00 = fma(v0, x0, s00);
@@ -2324,7 +2339,7 @@ struct BkOcc {
 // LDL: the rows are those of a symmetric indefinite band matrix with the
 // offsets of their lower entries in a.loff (factored as L D L^T, 16 lower
 // entries per row); only with MODE BK_X0.
-template <typename R, int DIM, int MODE, bool LDL = false>
+template <typename R, int DIM, int MODE, bool LDL = false, int NT = 4>
 __global__ __launch_bounds__(64, BkOcc<R>::waves) void q_blk_factor_kernel(BandArgs a) {
   typedef typename Mf<R>::v4 V4;
   const int cell = blockIdx.x;
@@ -2348,28 +2363,31 @@ __global__ __launch_bounds__(64, BkOcc<R>::waves) void q_blk_factor_kernel(BandA
   RowPrefetch pf[2];
   rows_load<LDL>(a, cell, 0, pf[0]);
   rows_load<LDL>(a, cell, 8, pf[1]);
-  bk_fill_row<R, DIM, 0, LDL>(a, ad, cell, my_off, Tl, Yt, sh, ms, pf);
-  bk_fill_row<R, DIM, 1, LDL>(a, ad, cell, my_off, Tl, Yt, sh, ms, pf);
-  bk_fill_row<R, DIM, 2, LDL>(a, ad, cell, my_off, Tl, Yt, sh, ms, pf);
-  bk_fill_row<R, DIM, 3, LDL>(a, ad, cell, my_off, Tl, Yt, sh, ms, pf);
+  // the tile rows beyond NT are never instantiated (their template index
+  // folds onto 0 and the branch is a compile-time false)
+  constexpr int T1 = NT > 1 ? 1 : 0, T2 = NT > 2 ? 2 : 0, T3 = NT > 2 ? 3 : 0;
+  bk_fill_row<R, DIM, NT, 0, LDL>(a, ad, cell, my_off, Tl, Yt, sh, ms, pf);
+  if (NT > 1) bk_fill_row<R, DIM, NT, T1, LDL>(a, ad, cell, my_off, Tl, Yt, sh, ms, pf);
+  if (NT > 2) bk_fill_row<R, DIM, NT, T2, LDL>(a, ad, cell, my_off, Tl, Yt, sh, ms, pf);
+  if (NT > 2) bk_fill_row<R, DIM, NT, T3, LDL>(a, ad, cell, my_off, Tl, Yt, sh, ms, pf);
   BkMon<R> bad;
   bad.init();
   const int n = a.n;
-  for (int j0 = 0; j0 < n; j0 += 64) {
-    bk_tile_row<R, DIM, 0, LDL>(a, ad, cell, j0, my_off, Tl, Yt, Sacc, sh, ms, pf, bad, ek);
-    if (j0 + 16 < n)
-      bk_tile_row<R, DIM, 1, LDL>(a, ad, cell, j0 + 16, my_off, Tl, Yt, Sacc, sh, ms, pf, bad, ek);
-    if (j0 + 32 < n)
-      bk_tile_row<R, DIM, 2, LDL>(a, ad, cell, j0 + 32, my_off, Tl, Yt, Sacc, sh, ms, pf, bad, ek);
-    if (j0 + 48 < n)
-      bk_tile_row<R, DIM, 3, LDL>(a, ad, cell, j0 + 48, my_off, Tl, Yt, Sacc, sh, ms, pf, bad, ek);
+  for (int j0 = 0; j0 < n; j0 += 16 * NT) {
+    bk_tile_row<R, DIM, NT, 0, LDL>(a, ad, cell, j0, my_off, Tl, Yt, Sacc, sh, ms, pf, bad, ek);
+    if (NT > 1 && j0 + 16 < n)
+      bk_tile_row<R, DIM, NT, T1, LDL>(a, ad, cell, j0 + 16, my_off, Tl, Yt, Sacc, sh, ms, pf, bad, ek);
+    if (NT > 2 && j0 + 32 < n)
+      bk_tile_row<R, DIM, NT, T2, LDL>(a, ad, cell, j0 + 32, my_off, Tl, Yt, Sacc, sh, ms, pf, bad, ek);
+    if (NT > 2 && j0 + 48 < n)
+      bk_tile_row<R, DIM, NT, T3, LDL>(a, ad, cell, j0 + 48, my_off, Tl, Yt, Sacc, sh, ms, pf, bad, ek);
   }
   double sacc = (MODE == BK_FINAL64) ? bk_sacc<R>(Sacc, SH_PART(sh)) : 0.0;
   // panels of this wave must be visible to its own loads in the backward sweep
   __threadfence_block();
   __syncthreads();
   double bint = 0.0, dmax = 0.0, xmax = 0.0;
-  bk_backward<R, DIM, MODE>(a, cell, sh, ms, nullptr, bint, sacc, dmax, xmax);
+  bk_backward<R, DIM, MODE, NT>(a, cell, sh, ms, nullptr, bint, sacc, dmax, xmax);
   int worst = bad.status();
   for (int o = 32; o > 0; o >>= 1) worst = max(worst, __shfl_xor(worst, o));
   if (lane == 0) {
@@ -2441,22 +2459,23 @@ struct RfPre {
   R rq;
 };
 
-template <typename R>
+template <typename R, int NT>
 __device__ __forceinline__ void rf_load(const BandArgs& a, int cell, int q, RfPre<R>& f) {
   const int lane = threadIdx.x, lh = lane >> 4, ll = lane & 15;
   const int n = a.n;
   if (4 * q >= n) return;
   const R* Pn = reinterpret_cast<const R*>(a.pan) + ((size_t)cell * a.nq + q) * 256;
 #pragma unroll
-  for (int u = 0; u < 4; ++u) f.pp[u] = Pn[u * 64 + lane];  // [k = u][o = lane]
-  const int jn = 4 * q + kWin + lh;
+  for (int u = 0; u < 4; ++u)  // [k = u][o = lane]
+    f.pp[u] = (lane < 16 * NT) ? Pn[u * 64 + lane] : (R)0;
+  const int jn = 4 * q + 16 * NT + lh;
   const R rv = (R)a.Rres[((size_t)cell * n + min(jn, n - 1)) * kMaxNb + (ll & 7)];
   f.rq = (ll < 8 && jn < n) ? rv : (R)0;
 }
 
 // One super-step of the forward elimination of a.Rres with the stored
 // panels: Z = C^-1 y_pivot, Y_P -= V_P Z (4 MFMAs), Z to HBM, next rows in.
-template <typename R, int DIM, int T, int G>
+template <typename R, int DIM, int NT, int T, int G, bool LDL>
 __device__ __forceinline__ void rf_step(const BandArgs& a, int cell, int j0,
                                         typename Mf<R>::v4 (&Yt)[4], BkSmem<R>& ms,
                                         RfPre<R>& f) {
@@ -2470,7 +2489,7 @@ __device__ __forceinline__ void rf_step(const BandArgs& a, int cell, int j0,
   ms.yp[lh * 16 + ll] = Yt[T][G];
   const R rin = f.rq;
   wave_sync();
-  rf_load<R>(a, cell, qs + 4, f);  // 4 super-steps ahead
+  rf_load<R, NT>(a, cell, qs + 4, f);  // 4 super-steps ahead
   // Z = C^-1 y_pivot by forward substitution (C (t,u) at pb[u * 68 + t],
   // 1 / C[t][t] at pb[t * 68 + t]); lane (k = lh, rhs ll) keeps row lh
   const R y0 = ms.yp[0 * 16 + ll], y1 = ms.yp[1 * 16 + ll];
@@ -2482,42 +2501,45 @@ __device__ __forceinline__ void rf_step(const BandArgs& a, int cell, int j0,
                 pb[2 * kPanRow + 3] * z2) *
                pb[3 * kPanRow + 3];
   const R Z = (lh == 0) ? z0 : (lh == 1) ? z1 : (lh == 2) ? z2 : z3;
+  // LDL^T panels hold C = L1 D1: the update takes D1^-1 L1^-1 y (= Z), the
+  // backward sweep the unit-triangular L1^-1 y = D1 Z (1 / d_lh on the diagonal)
+  const R Zst = LDL ? Z / pb[lh * kPanRow + lh] : Z;
   R nV[4];
   const int jr = bk_opaque(j0);  // = s0 mod 64
 #pragma unroll
-  for (int P = 0; P < 4; ++P) nV[P] = -pb[lh * kPanRow + ((16 * P + cs - jr) & (kWin - 1))];
+  for (int P = 0; P < NT; ++P) nV[P] = -pb[lh * kPanRow + ((16 * P + cs - jr) & (16 * NT - 1))];
 #pragma unroll
-  for (int P = 0; P < 4; ++P) Yt[P] = M::mma(nV[P], Z, Yt[P]);
-  if (ll < 8) reinterpret_cast<R*>(a.zst)[((size_t)cell * a.nq + qs) * 32 + lh * 8 + ll] = Z;
+  for (int P = 0; P < NT; ++P) Yt[P] = M::mma(nV[P], Z, Yt[P]);
+  if (ll < 8) reinterpret_cast<R*>(a.zst)[((size_t)cell * a.nq + qs) * 32 + lh * 8 + ll] = Zst;
   // the next rows of the band enter the pivot slots
   Yt[T][G] = rin;
   wave_sync();
 }
 
-template <typename R, int DIM, int T>
+template <typename R, int DIM, int NT, int T, bool LDL>
 __device__ __forceinline__ void rf_tile_row(const BandArgs& a, int cell, int j0,
                                             typename Mf<R>::v4 (&Yt)[4], BkSmem<R>& ms,
                                             RfPre<R> (&f)[4]) {
   const int n = a.n;
-  rf_step<R, DIM, T, 0>(a, cell, j0, Yt, ms, f[0]);
-  if (j0 + 4 < n) rf_step<R, DIM, T, 1>(a, cell, j0 + 4, Yt, ms, f[1]);
-  if (j0 + 8 < n) rf_step<R, DIM, T, 2>(a, cell, j0 + 8, Yt, ms, f[2]);
-  if (j0 + 12 < n) rf_step<R, DIM, T, 3>(a, cell, j0 + 12, Yt, ms, f[3]);
+  rf_step<R, DIM, NT, T, 0, LDL>(a, cell, j0, Yt, ms, f[0]);
+  if (j0 + 4 < n) rf_step<R, DIM, NT, T, 1, LDL>(a, cell, j0 + 4, Yt, ms, f[1]);
+  if (j0 + 8 < n) rf_step<R, DIM, NT, T, 2, LDL>(a, cell, j0 + 8, Yt, ms, f[2]);
+  if (j0 + 12 < n) rf_step<R, DIM, NT, T, 3, LDL>(a, cell, j0 + 12, Yt, ms, f[3]);
 }
 
 // Forward sweep of a.Rres over all super-steps of the cell.
-template <typename R, int DIM>
+template <typename R, int DIM, int NT = 4, bool LDL = false>
 __device__ __forceinline__ void rf_forward(const BandArgs& a, int cell, BkSmem<R>& ms) {
   typedef typename Mf<R>::v4 V4;
   const int lane = threadIdx.x, lh = lane >> 4, ll = lane & 15;
   const int n = a.n;
   RfPre<R> pre[4];
 #pragma unroll
-  for (int u = 0; u < 4; ++u) rf_load<R>(a, cell, u, pre[u]);
+  for (int u = 0; u < 4; ++u) rf_load<R, NT>(a, cell, u, pre[u]);
   V4 Yt[4];
-  // rows 0..63: slot 4g + lh of tile row P is register g
+  // rows 0..16 NT - 1: slot 4g + lh of tile row P is register g
 #pragma unroll
-  for (int P = 0; P < 4; ++P)
+  for (int P = 0; P < NT; ++P)
 #pragma unroll
     for (int g = 0; g < 4; ++g) {
       const int j = 16 * P + 4 * g + lh;
@@ -2526,11 +2548,14 @@ __device__ __forceinline__ void rf_forward(const BandArgs& a, int cell, BkSmem<R
     }
   for (int t = lane; t < 4 * kPanRow; t += 64) ms.piv[t] = (R)0;
   wave_sync();
-  for (int j0 = 0; j0 < n; j0 += 64) {
-    rf_tile_row<R, DIM, 0>(a, cell, j0, Yt, ms, pre);
-    if (j0 + 16 < n) rf_tile_row<R, DIM, 1>(a, cell, j0 + 16, Yt, ms, pre);
-    if (j0 + 32 < n) rf_tile_row<R, DIM, 2>(a, cell, j0 + 32, Yt, ms, pre);
-    if (j0 + 48 < n) rf_tile_row<R, DIM, 3>(a, cell, j0 + 48, Yt, ms, pre);
+  for (int j0 = 0; j0 < n; j0 += 16 * NT) {
+    rf_tile_row<R, DIM, NT, 0, LDL>(a, cell, j0, Yt, ms, pre);
+    if (NT > 1 && j0 + 16 < n)
+      rf_tile_row<R, DIM, NT, (NT > 1 ? 1 : 0), LDL>(a, cell, j0 + 16, Yt, ms, pre);
+    if (NT > 2 && j0 + 32 < n)
+      rf_tile_row<R, DIM, NT, (NT > 2 ? 2 : 0), LDL>(a, cell, j0 + 32, Yt, ms, pre);
+    if (NT > 2 && j0 + 48 < n)
+      rf_tile_row<R, DIM, NT, (NT > 2 ? 3 : 0), LDL>(a, cell, j0 + 48, Yt, ms, pre);
   }
 }
 
@@ -2542,7 +2567,7 @@ __global__ __launch_bounds__(64, 2) void q_blk_refine_kernel(BandArgs a) {
   __shared__ BandSmem sh;
   __shared__ BkSmem<float> ms;
   __shared__ double xfr[kMaxNb * 64];  // refined solution of the ring rows
-  rf_forward<float, DIM>(a, cell, ms);
+  rf_forward<float, DIM, 4>(a, cell, ms);
   __threadfence_block();
   __syncthreads();
   double bint = 0.0, sacc = 0.0, dmax = 0.0, xmax = 0.0;
@@ -2571,16 +2596,133 @@ __global__ __launch_bounds__(64, 2) void q_blk_refine_kernel(BandArgs a) {
 // factorisation (FP64 or FP32), x to a.X.  Used by the adjoint for
 // w = K_II^{-1} F_I.
 // ---------------------------------------------------------------------------
-template <typename R, int DIM>
+template <typename R, int DIM, int NT>
 __global__ __launch_bounds__(64, 2) void q_blk_solve_kernel(BandArgs a) {
   const int cell = blockIdx.x;
   __shared__ BandSmem sh;
   __shared__ BkSmem<R> ms;
-  rf_forward<R, DIM>(a, cell, ms);
+  rf_forward<R, DIM, NT>(a, cell, ms);
   __threadfence_block();
   __syncthreads();
   double bint = 0.0, sacc = 0.0, dmax = 0.0, xmax = 0.0;
-  bk_backward<R, DIM, BK_X0>(a, cell, sh, ms, nullptr, bint, sacc, dmax, xmax);
+  bk_backward<R, DIM, BK_X0, NT>(a, cell, sh, ms, nullptr, bint, sacc, dmax, xmax);
+}
+
+// ---------------------------------------------------------------------------
+// Mixed precision for the symmetric indefinite band systems (Q_Ned): FP32
+// LDL^T on the matrix cores (q_blk_factor_kernel<float, 2, BK_X0, true>),
+// then refinement passes, each an FP64 residual (ldl_resid_kernel), a
+// correction solved with the stored FP32 panels (ldl_refine_kernel) and the
+// update x += correction with the convergence test (ldl_update_kernel).
+// ---------------------------------------------------------------------------
+// Rres = R - K X in FP64 for rows [n][24] (16 lower entries with their
+// offsets in loff, diagonal first; 8 right-hand sides).  The upper part of
+// row j is read through utab [n][16]: (i << 4 | t) for the lower entry t of
+// row i > j that couples to j, -1 unused.  One block per cell, X staged in LDS.
+__global__ __launch_bounds__(256) void ldl_resid_kernel(BandArgs a, const int* __restrict__ utab) {
+  const int cell = blockIdx.x;
+  if (a.conv[cell]) return;  // uniform over the block
+  extern __shared__ double xs[];  // [n][8]
+  const int n = a.n;
+  const double* rows = a.Rows + (size_t)cell * n * kRowStride;
+  const double* X = a.X + (size_t)cell * n * kMaxNb;
+  for (int i = threadIdx.x; i < n * kMaxNb; i += blockDim.x) xs[i] = X[i];
+  __syncthreads();
+  for (int i = threadIdx.x; i < n * kMaxNb; i += blockDim.x) {
+    const int j = i >> 3, r = i & (kMaxNb - 1);
+    const double* rj = rows + (size_t)j * kRowStride;
+    const int* lo = a.loff + (size_t)j * 16;
+    const int* up = utab + (size_t)j * 16;
+    double acc = rj[16 + r];
+#pragma unroll
+    for (int t = 0; t < 16; ++t) {
+      const int o = lo[t];
+      const int jm = (o >= 0) ? j - o : j;
+      acc = fma(o >= 0 ? -rj[t] : 0.0, xs[jm * kMaxNb + r], acc);
+    }
+#pragma unroll
+    for (int t = 0; t < 16; ++t) {
+      const int u = up[t];
+      const int ri = (u >= 0) ? (u >> 4) : j, ti = (u >= 0) ? (u & 15) : 0;
+      acc = fma(u >= 0 ? -rows[(size_t)ri * kRowStride + ti] : 0.0, xs[ri * kMaxNb + r], acc);
+    }
+    a.Rres[((size_t)cell * n + j) * kMaxNb + r] = acc;
+  }
+}
+
+// The correction K d = Rres with the stored FP32 LDL^T panels, to a.X
+// (which the host points at the correction buffer).
+template <int NT>
+__global__ __launch_bounds__(64, 2) void ldl_refine_kernel(BandArgs a) {
+  const int cell = blockIdx.x;
+  if (a.conv[cell]) return;  // converged or failed: the whole wave leaves
+  __shared__ BandSmem sh;
+  __shared__ BkSmem<float> ms;
+  rf_forward<float, 2, NT, true>(a, cell, ms);
+  __threadfence_block();
+  __syncthreads();
+  double bint = 0.0, sacc = 0.0, dmax = 0.0, xmax = 0.0;
+  bk_backward<float, 2, BK_X0, NT>(a, cell, sh, ms, nullptr, bint, sacc, dmax, xmax);
+}
+
+// x += d for the cells still refining.  With q = |d| / |x| and q' its value
+// in the pass before (1 in the first), the error left after the step is
+// about q rho with the contraction rho = q / q' (the first pass: q itself,
+// since d then is the first solution's error).  A cell stops once that is
+// below 1e-14 of the largest unknown (the blocks of a mixed system differ in
+// scale: sigma of Q_Ned is ~1/40 of the largest entry), or after the last
+// pass, flagged MSFEC_CELL_NOT_CONVERGED above 1e-12.  One block per cell.
+__global__ __launch_bounds__(256) void ldl_update_kernel(BandArgs a, const double* __restrict__ dX,
+                                                         double* __restrict__ qprev) {
+  const int cell = blockIdx.x;
+  if (a.conv[cell]) return;
+  __shared__ double red[2][4];
+  const size_t base = (size_t)cell * a.n * kMaxNb;
+  double dm = 0.0, xm = 0.0;
+  for (int i = threadIdx.x; i < a.n * kMaxNb; i += blockDim.x) {
+    const double x = a.X[base + i] + dX[base + i];
+    a.X[base + i] = x;
+    // a NaN turns the measure infinite (fmax alone would skip it)
+    const double ad = fabs(dX[base + i]), ax = fabs(x);
+    dm = (ad == ad) ? fmax(dm, ad) : INFINITY;
+    xm = (ax == ax) ? fmax(xm, ax) : INFINITY;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    dm = fmax(dm, __shfl_xor(dm, o));
+    xm = fmax(xm, __shfl_xor(xm, o));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = dm;
+    red[1][threadIdx.x >> 6] = xm;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    dm = fmax(fmax(red[0][0], red[0][1]), fmax(red[0][2], red[0][3]));
+    xm = fmax(fmax(red[1][0], red[1][1]), fmax(red[1][2], red[1][3]));
+    const double q = dm / fmax(xm, 1e-300);
+    const double est = q * fmin(1.0, q / fmax(qprev[cell], 1e-300));
+    qprev[cell] = q;
+    if (!isfinite(est)) {
+      a.conv[cell] = 1;
+      if (a.status[cell] == MSFEC_CELL_OK) a.status[cell] = MSFEC_CELL_NONFINITE;
+    } else if (est <= 1e-14) {
+      a.conv[cell] = 1;
+    } else if (a.last_pass) {
+      a.conv[cell] = 1;
+      if (est > 1e-12 && a.status[cell] == MSFEC_CELL_OK) a.status[cell] = MSFEC_CELL_NOT_CONVERGED;
+    }
+  }
+}
+
+// conv = 1 for the cells whose factorisation failed (they do not refine);
+// the previous correction ratio starts at 1
+__global__ void ldl_conv_init_kernel(const int* __restrict__ status, int* __restrict__ conv,
+                                     double* __restrict__ qprev, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    conv[i] = (status[i] != MSFEC_CELL_OK) ? 1 : 0;
+    qprev[i] = 1.0;
+  }
 }
 
 // Right-hand side of w: column 0 = F_I (interior load), columns 1..7 = 0.
@@ -3008,6 +3150,7 @@ struct CgArgs {
   int nsolve;  // bases solved per cell: 2^dim, or 2^dim - 1 when the last one
                // follows from the partition of unity (q_cg_unity_kernel)
   int* iters;  // [cells][NV][2]: iterations, state (0 converged, 1 not, 2 breakdown, 3 non-finite)
+  const double* fnode;  // assembled-stencil path: nodal load [cells][nn]
 };
 
 // sum over the 64 lanes of the wave: DPP inside rows of 16, then across rows
@@ -3420,11 +3563,276 @@ __global__ __launch_bounds__(64 * kFinWaves) void q_cg_finish_kernel(BandArgs a,
   }
 }
 
+// ---------------------------------------------------------------------------
+// CG on the assembled stencil: tensor coefficients at the Gauss points and /
+// or loads per Gauss point (the reference's Diffusion_A and
+// RightHandSideParsed evaluated at the QGauss(2) points, q_basis.cc:196-220),
+// solved matrix-free beyond the band window.  q_assemble_kernel has written
+// the lower stencil Slo of every node (diagonal first, then the lower
+// directions t with full-grid offsets off_full[t]), the nodal load F and the
+// lifting right-hand sides Rhs.  One workgroup per (cell, basis) as in
+// q_cg_kernel: the search direction over all nodes in LDS (zero on the
+// boundary), every thread a segment of a grid line in x; K p at node n is
+// Slo[n][0] p[n] + sum_t Slo[n][t] p[n - off_t] + Slo[n + off_t][t] p[n + off_t]
+// with the stencil rows read from global memory (L2) each iteration.
+// ---------------------------------------------------------------------------
+template <int DIM>
+__device__ __forceinline__ double cg_stencil_row(const double* __restrict__ slo, const int* off,
+                                                 int node, const double* v) {
+  using D = Dirs<DIM>;
+  double acc = slo[(size_t)node * D::NLOW] * v[node];
+#pragma unroll
+  for (int t = 1; t < D::NLOW; ++t) {
+    const int o = off[t];
+    acc = fma(slo[(size_t)node * D::NLOW + t], v[node - o], acc);
+    acc = fma(slo[(size_t)(node + o) * D::NLOW + t], v[node + o], acc);
+  }
+  return acc;
+}
+
+template <int DIM>
+__global__ __launch_bounds__(kCgThreads) void q_cg_stencil_kernel(BandArgs a, CgArgs g, QConf c) {
+  using D = Dirs<DIM>;
+  const int cell = blockIdx.x / g.nsolve, j = blockIdx.x - cell * g.nsolve;
+  const int tid = threadIdx.x;
+  extern __shared__ double dyn[];  // search direction over all nodes
+  __shared__ double red[2][kCgWaves][2];
+  __shared__ int off[D::NLOW];
+  double* pv = dyn;
+  if (tid < D::NLOW) off[tid] = c.off_full[tid];
+  for (int z = tid; z < a.nn; z += kCgThreads) pv[z] = 0.0;
+  const double* slo = a.Slo + (size_t)cell * a.nn * D::NLOW;
+  const int ni = a.ni, L = g.seg, spl = (ni + L - 1) / L;
+  const int lines = (DIM == 3) ? ni * ni : ni;
+  const int line0 = tid / spl, sx = tid - line0 * spl;
+  const int len = (line0 < lines) ? min(L, ni - sx * L) : 0;
+  const int line = (line0 < lines) ? line0 : 0;
+  const int x0 = 1 + sx * L, cy = 1 + line % ni, cz = (DIM == 3) ? 1 + line / ni : 0;
+  const int jn0 = (x0 - 1) + ni * ((cy - 1) + (DIM == 3 ? ni * (cz - 1) : 0));
+  const int node0 = x0 + a.S * cy + (DIM == 3 ? a.S * a.S * cz : 0);
+  double b[kCgOwn], dinv[kCgOwn], x[kCgOwn], r[kCgOwn], p[kCgOwn];
+  double bb = 0.0, rz = 0.0;
+#pragma unroll
+  for (int t = 0; t < kCgOwn; ++t) {
+    const bool ok = t < len;
+    b[t] = ok ? a.Rhs[((size_t)cell * a.n + jn0 + t) * kMaxNb + j] : 0.0;
+    dinv[t] = ok ? 1.0 / slo[(size_t)(node0 + t) * D::NLOW] : 0.0;
+    x[t] = 0.0;
+    r[t] = b[t];
+    p[t] = dinv[t] * r[t];
+    bb = fma(b[t], b[t], bb);
+    rz = fma(r[t], p[t], rz);
+  }
+  __syncthreads();  // zeroed vector, offsets
+#pragma unroll
+  for (int t = 0; t < kCgOwn; ++t)
+    if (t < len) pv[node0 + t] = p[t];
+  cg_block_sum2(bb, rz, red[1]);  // also: p is in LDS
+  double rr = bb;
+  const double stop = g.tol * g.tol * bb;
+  int it = 0, state = 0;
+  while (rr > stop) {
+    if (it >= g.max_iter) {
+      state = 1;
+      break;
+    }
+    double q[kCgOwn];
+#pragma unroll
+    for (int t = 0; t < kCgOwn; ++t) q[t] = (t < len) ? cg_stencil_row<DIM>(slo, off, node0 + t, pv) : 0.0;
+    double pq = 0.0, dummy = 0.0;
+#pragma unroll
+    for (int t = 0; t < kCgOwn; ++t)
+      if (t < len) pq = fma(p[t], q[t], pq);
+    cg_block_sum2(pq, dummy, red[0]);
+    if (!(pq > 0.0)) {
+      state = 2;
+      break;
+    }
+    const double alpha = rz / pq;
+    double rz1 = 0.0, rr1 = 0.0;
+#pragma unroll
+    for (int t = 0; t < kCgOwn; ++t) {
+      if (t < len) {
+        x[t] = fma(alpha, p[t], x[t]);
+        r[t] = fma(-alpha, q[t], r[t]);
+        rz1 = fma(r[t], dinv[t] * r[t], rz1);
+        rr1 = fma(r[t], r[t], rr1);
+      }
+    }
+    cg_block_sum2(rz1, rr1, red[1]);  // also: every thread is done reading p
+    const double beta = rz1 / rz;
+    rz = rz1;
+    rr = rr1;
+#pragma unroll
+    for (int t = 0; t < kCgOwn; ++t) {
+      if (t < len) {
+        p[t] = fma(beta, p[t], dinv[t] * r[t]);
+        pv[node0 + t] = p[t];
+      }
+    }
+    __syncthreads();
+    ++it;
+  }
+  if (!isfinite(rr) || !isfinite(bb)) state = 3;
+  // the true residual (the last basis of a cell without reaction follows
+  // from the partition of unity, q_cg_unity_kernel, which sums it)
+  __syncthreads();  // an early exit leaves threads that may still read p
+#pragma unroll
+  for (int t = 0; t < kCgOwn; ++t)
+    if (t < len) pv[node0 + t] = x[t];
+  __syncthreads();
+  double* psi = a.Psi + ((size_t)cell * D::NV + j) * a.nn;
+#pragma unroll
+  for (int t = 0; t < kCgOwn; ++t) {
+    if (t < len) {
+      const size_t jn = (size_t)cell * a.n + jn0 + t;
+      a.X[jn * kMaxNb + j] = x[t];
+      a.Rres[jn * kMaxNb + j] = b[t] - cg_stencil_row<DIM>(slo, off, node0 + t, pv);
+      psi[node0 + t] = x[t];
+    }
+  }
+  if (tid == 0) {
+    g.iters[((size_t)cell * D::NV + j) * 2] = it;
+    g.iters[((size_t)cell * D::NV + j) * 2 + 1] = state;
+  }
+}
+
+// Element matrix of a cell solved on the assembled stencil, with the whole
+// assembled matrix as the reference forms it (q_basis.cc:262-300):
+// M[i][k] = psi_i^T K psi_k, b[i] = psi_i^T F.  Second order in the error of
+// the interior solution: psi_k is K-harmonic at the interior rows and the
+// error vanishes on the boundary.  Psi is completed on the boundary with the
+// lifted coarse shape functions first; then every thread takes every 1024th
+// node, forms (K psi_k)(node) for the 2^dim bases and accumulates the 64
+// products and the 8 loads, summed over the workgroup at the end.
+constexpr int kTfThreads = 1024;
+
+template <int DIM>
+__global__ __launch_bounds__(kTfThreads) void q_cg_tensor_finish_kernel(BandArgs a, CgArgs g,
+                                                                        QConf c) {
+  using D = Dirs<DIM>;
+  const int cell = blockIdx.x, tid = threadIdx.x;
+  __shared__ double part[kTfThreads / 64][72];
+  __shared__ int off[D::NLOW];
+  if (tid < D::NLOW) off[tid] = c.off_full[tid];
+  double* psi = a.Psi + (size_t)cell * D::NV * a.nn;
+  const double* slo = a.Slo + (size_t)cell * a.nn * D::NLOW;
+  // boundary values of every basis
+  for (int node = tid; node < a.nn; node += kTfThreads) {
+    int m[3] = {0, 0, 0}, rr = node;
+    bool bnd = false;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+      m[d] = rr % a.S;
+      rr /= a.S;
+      bnd = bnd || m[d] == 0 || m[d] == a.N;
+    }
+    if (bnd)
+#pragma unroll
+      for (int k = 0; k < D::NV; ++k) psi[(size_t)k * a.nn + node] = lift_value(c, k, m);
+  }
+  __threadfence_block();
+  __syncthreads();
+  double acc[D::NV * D::NV + D::NV];
+#pragma unroll
+  for (int t = 0; t < D::NV * D::NV + D::NV; ++t) acc[t] = 0.0;
+  const double* F = g.fnode + (size_t)cell * a.nn;
+  for (int node = tid; node < a.nn; node += kTfThreads) {
+    int m[3] = {0, 0, 0}, rr = node;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+      m[d] = rr % a.S;
+      rr /= a.S;
+    }
+    double kp[D::NV];
+#pragma unroll
+    for (int k = 0; k < D::NV; ++k) kp[k] = slo[(size_t)node * D::NLOW] * psi[(size_t)k * a.nn + node];
+    // the 3^dim - 1 neighbours inside the grid: lower ones from the own row,
+    // upper ones from the neighbour's row
+#pragma unroll
+    for (int t = 1; t < D::NLOW; ++t) {
+      const int o = off[t];
+      const int e = c.dir_low[t];  // lower direction e: node - o is its neighbour
+      bool lo_ok = true, up_ok = true;
+      int ee = e;
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) {
+        const int dd = ee % 3 - 1;  // offset of the lower neighbour in direction d is dd
+        ee /= 3;
+        if (m[d] + dd < 0 || m[d] + dd > a.N) lo_ok = false;
+        if (m[d] - dd < 0 || m[d] - dd > a.N) up_ok = false;
+      }
+      if (lo_ok) {
+        const double kl = slo[(size_t)node * D::NLOW + t];
+#pragma unroll
+        for (int k = 0; k < D::NV; ++k) kp[k] = fma(kl, psi[(size_t)k * a.nn + node - o], kp[k]);
+      }
+      if (up_ok) {
+        const double ku = slo[(size_t)(node + o) * D::NLOW + t];
+#pragma unroll
+        for (int k = 0; k < D::NV; ++k) kp[k] = fma(ku, psi[(size_t)k * a.nn + node + o], kp[k]);
+      }
+    }
+    double pn[D::NV];
+#pragma unroll
+    for (int i = 0; i < D::NV; ++i) pn[i] = psi[(size_t)i * a.nn + node];
+    const double fn = F[node];
+#pragma unroll
+    for (int i = 0; i < D::NV; ++i) {
+#pragma unroll
+      for (int k = 0; k < D::NV; ++k) acc[i * D::NV + k] = fma(pn[i], kp[k], acc[i * D::NV + k]);
+      acc[D::NV * D::NV + i] = fma(pn[i], fn, acc[D::NV * D::NV + i]);
+    }
+  }
+  const int w = tid >> 6, lane = tid & 63;
+#pragma unroll
+  for (int t = 0; t < D::NV * D::NV + D::NV; ++t) {
+    const double v = cg_wave_sum(acc[t]);
+    if (lane == 0) part[w][t] = v;
+  }
+  __syncthreads();
+  if (tid < D::NV * D::NV + D::NV) {
+    double v = 0.0;
+#pragma unroll
+    for (int u = 0; u < kTfThreads / 64; ++u) v += part[u][tid];
+    if (tid < D::NV * D::NV) a.M[(size_t)cell * D::NV * D::NV + tid] = v;
+    else a.bvec[(size_t)cell * D::NV + tid - D::NV * D::NV] = v;
+  }
+  if (tid == 0) {
+    int st = MSFEC_CELL_OK;
+    for (int jj = 0; jj < D::NV; ++jj) {
+      const int state = g.iters[((size_t)cell * D::NV + jj) * 2 + 1];
+      if (state == 3) st = MSFEC_CELL_NONFINITE;
+      else if (state == 2 && st != MSFEC_CELL_NONFINITE) st = MSFEC_CELL_NOT_SPD;
+      else if (state == 1 && st == MSFEC_CELL_OK) st = MSFEC_CELL_NOT_CONVERGED;
+    }
+    a.status[cell] = st;
+  }
+}
+
 // ===========================================================================
 // Host side: handle, error reporting, launches
 // ===========================================================================
 // the last error message of the calling thread, shared with the Q_Ned
 // entry points (msfec_ned.hip)
+namespace {
+// q_blk_factor_kernel<double, DIM, BK_FINAL64> with a window of 16 nt slots
+void launch_factor64(int dim, int nt, int n_cells, hipStream_t s, const BandArgs& a) {
+#define F64(D, NT) \
+  hipLaunchKernelGGL((q_blk_factor_kernel<double, D, BK_FINAL64, false, NT>), dim3(n_cells), dim3(64), 0, s, a)
+  if (dim == 3) {
+    if (nt == 1) F64(3, 1);
+    else if (nt == 2) F64(3, 2);
+    else F64(3, 4);
+  } else {
+    if (nt == 1) F64(2, 1);
+    else if (nt == 2) F64(2, 2);
+    else F64(2, 4);
+  }
+#undef F64
+}
+}  // namespace
+
 namespace msfec_detail {
 thread_local std::string g_last_error;
 
@@ -3436,8 +3844,10 @@ size_t band_ldl_z_doubles(int n) { return (size_t)panel_quads(n) * 32; }
 // [cells][n][24] (16 lower entries, diagonal first, with their offsets in
 // loff [n][16]; 8 right-hand sides), the solutions to X [cells][n][8], the
 // cell status (MSFEC_CELL_NOT_SPD for a zero pivot) to status.
-hipError_t band_ldl_solve(int n_cells, int n, const double* rows, const int* loff, double* pan,
-                          double* zst, double* X, int* status, hipStream_t s) {
+hipError_t band_ldl_solve(int n_cells, int n, int bw, const double* rows, const int* loff,
+                          double* pan, double* zst, double* X, int* status, hipStream_t s) {
+  const int nt = band_tile_rows(bw, 0);
+  if (nt == 0) return hipErrorInvalidValue;
   BandArgs a{};
   a.n = n;
   a.nq = panel_quads(n);
@@ -3447,8 +3857,124 @@ hipError_t band_ldl_solve(int n_cells, int n, const double* rows, const int* lof
   a.zst = zst;
   a.X = X;
   a.status = status;
-  hipLaunchKernelGGL((q_blk_factor_kernel<double, 2, BK_X0, true>), dim3(n_cells), dim3(64), 0, s,
-                     a);
+#define LDL(NT) \
+  hipLaunchKernelGGL((q_blk_factor_kernel<double, 2, BK_X0, true, NT>), dim3(n_cells), dim3(64), 0, s, a)
+  if (nt == 1) LDL(1);
+  else if (nt == 2) LDL(2);
+  else LDL(4);
+#undef LDL
+  return hipGetLastError();
+}
+
+// Scratch per cell of band_ldl_mixed_solve beyond band_ldl_solve's, in
+// doubles: the residual and the correction ([n][8] each), the last correction
+// ratio and the refinement flag.
+size_t band_ldl_mixed_doubles(int n) { return (size_t)n * kMaxNb * 2 + 2; }
+
+// The transposed offset table of ldl_resid_kernel from the lower offsets
+// loff [n][16] (host side): entry (j, k) = (i << 4 | t) for the k-th lower
+// entry t of a row i > j with i - loff[i][t] = j, -1 unused.  Empty if a row
+// has more than 16 upper entries.
+std::vector<int> band_upper_table(const std::vector<int>& loff, int n) {
+  std::vector<int> ut((size_t)std::max(n, 1) * 16, -1);
+  std::vector<int> cnt(std::max(n, 1), 0);
+  for (int i = 0; i < n; ++i)
+    for (int t = 0; t < 16; ++t) {
+      const int o = loff[(size_t)i * 16 + t];
+      if (o <= 0) continue;
+      const int j = i - o;
+      if (j < 0 || cnt[j] >= 16) return {};
+      ut[(size_t)j * 16 + cnt[j]++] = (i << 4) | t;
+    }
+  return ut;
+}
+
+// band_ldl_solve with the factor in FP32 (panels pan and zst as FP32) and
+// FP64 refinement: X to FP64 accuracy, cells that do not converge in
+// max_refine passes flagged MSFEC_CELL_NOT_CONVERGED.  scratch: n_cells x
+// band_ldl_mixed_doubles(n) doubles.
+hipError_t band_ldl_mixed_solve(int n_cells, int n, int bw, const double* rows, const int* loff,
+                                const int* utab, double* pan, double* zst, double* X,
+                                double* scratch, int* status, int max_refine, hipStream_t s) {
+  const int nt = band_tile_rows(bw, 0);
+  if (nt == 0) return hipErrorInvalidValue;
+  double* Rres = scratch;
+  double* dX = Rres + (size_t)n_cells * n * kMaxNb;
+  double* qprev = dX + (size_t)n_cells * n * kMaxNb;
+  int* conv = reinterpret_cast<int*>(qprev + n_cells);
+  BandArgs a{};
+  a.n = n;
+  a.nq = panel_quads(n);
+  a.Rows = rows;
+  a.loff = loff;
+  a.pan = pan;
+  a.zst = zst;
+  a.X = X;
+  a.Rres = Rres;
+  a.conv = conv;
+  a.status = status;
+#define LDLF(NT) \
+  hipLaunchKernelGGL((q_blk_factor_kernel<float, 2, BK_X0, true, NT>), dim3(n_cells), dim3(64), 0, s, a)
+  if (nt == 1) LDLF(1);
+  else if (nt == 2) LDLF(2);
+  else LDLF(4);
+#undef LDLF
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(ldl_conv_init_kernel, dim3((n_cells + 255) / 256), dim3(256), 0, s, status,
+                     conv, qprev, n_cells);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const size_t rsm = (size_t)n * kMaxNb * sizeof(double);
+  const int passes = max_refine > 0 ? max_refine : 5;
+  for (int pass = 1; pass <= passes; ++pass) {
+    a.last_pass = (pass == passes) ? 1 : 0;
+    a.X = X;
+    hipLaunchKernelGGL(ldl_resid_kernel, dim3(n_cells), dim3(256), rsm, s, a, utab);
+    BandArgs ar = a;
+    ar.X = dX;  // the correction
+#define LDLR(NT) hipLaunchKernelGGL((ldl_refine_kernel<NT>), dim3(n_cells), dim3(64), 0, s, ar)
+    if (nt == 1) LDLR(1);
+    else if (nt == 2) LDLR(2);
+    else LDLR(4);
+#undef LDLR
+    hipLaunchKernelGGL(ldl_update_kernel, dim3(n_cells), dim3(256), 0, s, a, dX, qprev);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+// Coarse assembly, Jacobi-PCG and the per-cell weights on the device for the
+// element data of the nc^dim structured coarse grid (cells lexicographic):
+// dM [cells][2^dim][2^dim], db [cells][2^dim].  Buffers: cK [nodes][3^dim],
+// crhs [nodes], cvec [5 nodes + 1 + cells 2^dim] (u | r | z | p | q |
+// final residual | weights), cit [1].  Enqueued on s.
+hipError_t coarse_solve_device(int dim, int nc, const double* dM, const double* db, double* cK,
+                               double* crhs, double* cvec, int* cit, double abs_tol,
+                               double rel_tol, int max_iter, hipStream_t s) {
+  int nnode = 1, ncell = 1;
+  for (int d = 0; d < dim; ++d) {
+    nnode *= nc + 1;
+    ncell *= nc;
+  }
+  double* du = cvec;
+  double* r = du + nnode;
+  double* z = r + nnode;
+  double* pv = z + nnode;
+  double* qv = pv + nnode;
+  double* res = qv + nnode;
+  double* dw = res + 1;
+  hipLaunchKernelGGL(coarse_assemble_kernel, dim3((nnode + 255) / 256), dim3(256), 0, s, dim, nc, dM,
+                     db, cK, crhs);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(coarse_cg_kernel, dim3(1), dim3(1024), 0, s, dim, nc, cK, crhs, du, r, z, pv,
+                     qv, abs_tol, rel_tol, max_iter, cit, res);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const int nw = ncell << dim;
+  hipLaunchKernelGGL(coarse_weights_kernel, dim3((nw + 255) / 256), dim3(256), 0, s, dim, nc, du, dw);
   return hipGetLastError();
 }
 }  // namespace msfec_detail
@@ -3559,9 +4085,13 @@ msfec_status copy_in(hipStream_t s, double* dst, const double* src, size_t count
   return MSFEC_OK;
 }
 
-msfec_status copy_out(double* dst, const double* src, size_t count, int loc) {
-  MSFEC_HIP(hipMemcpy(dst, src, count * sizeof(double),
-                      loc == MSFEC_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost));
+// Outputs are copied on the handle's stream and waited for: a device
+// destination holds the data when the call returns, whatever stream the
+// caller reads it on next (a device-to-device hipMemcpy may return first).
+msfec_status copy_out(hipStream_t s, double* dst, const double* src, size_t count, int loc) {
+  MSFEC_HIP(hipMemcpyAsync(dst, src, count * sizeof(double),
+                           loc == MSFEC_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
+  MSFEC_HIP(hipStreamSynchronize(s));
   return MSFEC_OK;
 }
 
@@ -3676,11 +4206,6 @@ msfec_status msfec_q_create(const msfec_q_params* params, int n_cells, const dou
   if (p.solver != MSFEC_SOLVER_BAND_FP64 && p.solver != MSFEC_SOLVER_BAND_FP64_MFMA &&
       p.solver != MSFEC_SOLVER_BAND_MIXED && p.solver != MSFEC_SOLVER_CG)
     return fail(MSFEC_ERR_INVALID, "unknown solver");
-  if (p.solver == MSFEC_SOLVER_CG &&
-      (p.coef_mode != MSFEC_COEF_SCALAR_CELL || p.rhs_mode == MSFEC_RHS_QP))
-    return fail(MSFEC_ERR_UNSUPPORTED,
-                "the CG solver needs a scalar coefficient per fine cell and a constant or "
-                "per-cell load");
   if (p.solver == MSFEC_SOLVER_CG && (p.cg_tol < 0.0 || p.cg_max_iter < 0))
     return fail(MSFEC_ERR_INVALID, "cg_tol and cg_max_iter must not be negative");
   if (p.max_refine < 0 || p.max_refine > 16)
@@ -3701,10 +4226,16 @@ msfec_status msfec_q_create(const msfec_q_params* params, int n_cells, const dou
     if (nint > (long long)kCgThreads * kCgOwn || lds > 160 * 1024 || cg_segment(ni, p.dim) == 0)
       return fail(MSFEC_ERR_UNSUPPORTED,
                   "cell too fine for the CG workgroup (3-D: n_refine_local <= 4, 2-D: <= 6)");
-  } else if (bw + 1 > kWin)
+  } else if (band_tile_rows(bw, 0) == 0)
     return fail(MSFEC_ERR_UNSUPPORTED,
                 "interior half-bandwidth " + std::to_string(bw) +
-                    " exceeds the 63 of the band solver (3-D: n_refine_local <= 3, 2-D: <= 5)");
+                    " exceeds the 60 of the band solver (3-D: n_refine_local <= 3, 2-D: <= 5)");
+  if (p.band_window != 0 && p.band_window != 16 && p.band_window != 32 && p.band_window != 64)
+    return fail(MSFEC_ERR_INVALID, "band_window must be 0, 16, 32 or 64");
+  if (p.solver != MSFEC_SOLVER_CG && band_tile_rows(bw, p.band_window) == 0)
+    return fail(MSFEC_ERR_UNSUPPORTED, "a band window of " + std::to_string(p.band_window) +
+                                           " slots does not hold the half-bandwidth " +
+                                           std::to_string(bw) + " (it must be >= bw + 4)");
   // validate cells on the host (axis-aligned boxes)
   const int nv = 1 << p.dim;
   std::vector<double> hv((size_t)n_cells * nv * p.dim);
@@ -3737,6 +4268,9 @@ msfec_status msfec_q_create(const msfec_q_params* params, int n_cells, const dou
   h->bw = bw;
   QConf& c = h->conf;
   fill_qconf(p, c);
+  h->fast = (p.coef_mode == MSFEC_COEF_SCALAR_CELL) && (p.rhs_mode != MSFEC_RHS_QP);
+  // the mixed solver's refinement sweep and the general path keep the 64-slot window
+  c.nt = (p.solver == MSFEC_SOLVER_BAND_FP64_MFMA && h->fast) ? band_tile_rows(bw, p.band_window) : 4;
   const int nq = nv;
   h->coef_count = (p.coef_mode == MSFEC_COEF_SCALAR_CELL)
                       ? (size_t)n_cells * c.nfc
@@ -3745,7 +4279,6 @@ msfec_status msfec_q_create(const msfec_q_params* params, int n_cells, const dou
                  : (p.rhs_mode == MSFEC_RHS_CELL) ? (size_t)n_cells * c.nfc
                                                   : (size_t)n_cells * c.nfc * nq;
   h->reac_count = (p.reaction_mode == MSFEC_REACTION_CELL) ? (size_t)n_cells * c.nfc : 0;
-  h->fast = (p.coef_mode == MSFEC_COEF_SCALAR_CELL) && (p.rhs_mode != MSFEC_RHS_QP);
 
   msfec_status st = MSFEC_OK;
 #define TRY(x)              \
@@ -3842,6 +4375,7 @@ msfec_status msfec_q_get_info(msfec_q_handle h, msfec_q_info* info) {
   info->n_fine_cells = h->conf.nfc;
   info->n_interior = h->conf.n;
   info->bandwidth = h->bw;
+  info->band_window = 16 * h->conf.nt;
   info->coef_count = h->coef_count;
   info->rhs_count = h->rhs_count;
   info->reaction_count = h->reac_count;
@@ -3942,7 +4476,44 @@ msfec_status msfec_q_compute(msfec_q_handle h, void* stream) {
                                 sizeof(double)
                           : 0;
 #define LAUNCH_BAND(K, D, F) hipLaunchKernelGGL((K<D, F>), dim3(h->n_cells), dim3(64), dyn, s, a)
-  if (h->p.solver == MSFEC_SOLVER_CG) {
+  if (h->p.solver == MSFEC_SOLVER_CG && !fast) {
+    // tensor coefficients / Gauss-point loads: CG on the assembled stencil
+    CgArgs g{};
+    g.tol = h->p.cg_tol > 0.0 ? h->p.cg_tol : 1e-13;
+    g.max_iter = h->p.cg_max_iter > 0 ? h->p.cg_max_iter : 5000;
+    g.iters = h->d_cgit;
+    g.seg = cg_segment(c.ni, c.dim);
+    g.fnode = h->d_F;
+    const bool reac = c.reaction_mode != MSFEC_REACTION_NONE;
+    g.nsolve = reac ? c.nv : c.nv - 1;
+    const size_t cdyn = (size_t)c.nn * sizeof(double);
+#define LAUNCH_CGS(D)                                                                     \
+  do {                                                                                    \
+    MSFEC_HIP(hipFuncSetAttribute((const void*)q_cg_stencil_kernel<D>,                    \
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)cdyn)); \
+    hipLaunchKernelGGL((q_cg_stencil_kernel<D>), dim3(h->n_cells * g.nsolve), dim3(kCgThreads), \
+                       cdyn, s, a, g, c);                                                 \
+  } while (0)
+    if (c.dim == 3) LAUNCH_CGS(3);
+    else LAUNCH_CGS(2);
+#undef LAUNCH_CGS
+    MSFEC_HIP(hipGetLastError());
+    if (!reac) {
+      const unsigned ub = (unsigned)(((size_t)h->n_cells * c.n + 255) / 256);
+      if (c.dim == 3)
+        hipLaunchKernelGGL((q_cg_unity_kernel<3>), dim3(ub), dim3(256), 0, s, a, g, h->n_cells);
+      else
+        hipLaunchKernelGGL((q_cg_unity_kernel<2>), dim3(ub), dim3(256), 0, s, a, g, h->n_cells);
+      MSFEC_HIP(hipGetLastError());
+    }
+    if (tm) MSFEC_HIP(hipEventRecord(ev[2], s));
+    if (c.dim == 3)
+      hipLaunchKernelGGL((q_cg_tensor_finish_kernel<3>), dim3(h->n_cells), dim3(kTfThreads), 0, s, a, g, c);
+    else
+      hipLaunchKernelGGL((q_cg_tensor_finish_kernel<2>), dim3(h->n_cells), dim3(kTfThreads), 0, s, a, g, c);
+    MSFEC_HIP(hipGetLastError());
+    if (tm) MSFEC_HIP(hipEventRecord(ev[3], s));
+  } else if (h->p.solver == MSFEC_SOLVER_CG) {
     CgArgs g{};
     g.tol = h->p.cg_tol > 0.0 ? h->p.cg_tol : 1e-13;
     g.max_iter = h->p.cg_max_iter > 0 ? h->p.cg_max_iter : 5000;
@@ -4008,10 +4579,7 @@ msfec_status msfec_q_compute(msfec_q_handle h, void* stream) {
         hipLaunchKernelGGL((q_blk_refine_kernel<2>), dim3(h->n_cells), dim3(64), 0, s, a);
     }
   } else if (mfma && fast) {
-    if (c.dim == 3)
-      hipLaunchKernelGGL((q_blk_factor_kernel<double, 3, BK_FINAL64>), dim3(h->n_cells), dim3(64), 0, s, a);
-    else
-      hipLaunchKernelGGL((q_blk_factor_kernel<double, 2, BK_FINAL64>), dim3(h->n_cells), dim3(64), 0, s, a);
+    launch_factor64(c.dim, c.nt, h->n_cells, s, a);
   } else if (mfma) {
     // tensor coefficients: stencil tables of q_assemble_kernel
     if (c.dim == 3) LAUNCH_BAND(q_band_mfma_kernel, 3, false);
@@ -4032,7 +4600,7 @@ msfec_status msfec_q_compute(msfec_q_handle h, void* stream) {
     MSFEC_HIP(hipEventRecord(ev[3], s));
   }
   if (tm) MSFEC_HIP(hipEventRecord(ev[4], s));
-  if (!fast) {
+  if (!fast && h->p.solver != MSFEC_SOLVER_CG) {
     const size_t shm = (size_t)(2 * c.nv * c.nn + 4 * 72) * sizeof(double);
     if (shm > 160 * 1024) return fail(MSFEC_ERR_UNSUPPORTED, "element kernel LDS too large");
     hipLaunchKernelGGL(q_element_kernel, dim3(h->n_cells), dim3(256), shm, s, c, h->d_slo,
@@ -4086,11 +4654,11 @@ msfec_status msfec_q_get_element(msfec_q_handle h, double* M, double* b, int loc
   MSFEC_HIP(hipStreamSynchronize(h->stream));
   const size_t nv = h->conf.nv;
   if (M) {
-    msfec_status st = copy_out(M, h->d_M, (size_t)h->n_cells * nv * nv, loc);
+    msfec_status st = copy_out(h->stream, M, h->d_M, (size_t)h->n_cells * nv * nv, loc);
     if (st) return st;
   }
   if (b) {
-    msfec_status st = copy_out(b, h->d_b, (size_t)h->n_cells * nv, loc);
+    msfec_status st = copy_out(h->stream, b, h->d_b, (size_t)h->n_cells * nv, loc);
     if (st) return st;
   }
   return MSFEC_OK;
@@ -4100,7 +4668,7 @@ msfec_status msfec_q_get_basis(msfec_q_handle h, double* Psi, int loc) {
   if (!h || !Psi) return fail(MSFEC_ERR_INVALID, "null argument");
   if (!h->computed) return fail(MSFEC_ERR_STATE, "compute not called");
   MSFEC_HIP(hipStreamSynchronize(h->stream));
-  return copy_out(Psi, h->d_psi, (size_t)h->n_cells * h->conf.nv * h->conf.nn, loc);
+  return copy_out(h->stream, Psi, h->d_psi, (size_t)h->n_cells * h->conf.nv * h->conf.nn, loc);
 }
 
 msfec_status msfec_q_get_cell_status(msfec_q_handle h, int* status) {
@@ -4171,22 +4739,12 @@ msfec_status msfec_coarse_solve(msfec_q_handle h, int nc, double abs_tol, double
     h->coarse_nc = nc;
   }
   double* du = h->d_cvec;
-  double* r = du + nnode;
-  double* z = r + nnode;
-  double* pv = z + nnode;
-  double* qv = pv + nnode;
-  double* res = qv + nnode;
+  double* res = du + 5 * (size_t)nnode;
   double* dw = res + 1;
-  hipLaunchKernelGGL(coarse_assemble_kernel, dim3((nnode + 255) / 256), dim3(256), 0, h->stream, dim,
-                     nc, h->d_M, h->d_b, h->d_cK, h->d_crhs);
-  MSFEC_HIP(hipGetLastError());
-  hipLaunchKernelGGL(coarse_cg_kernel, dim3(1), dim3(1024), 0, h->stream, dim, nc, h->d_cK,
-                     h->d_crhs, du, r, z, pv, qv, abs_tol, rel_tol, max_iter, h->d_cit, res);
-  MSFEC_HIP(hipGetLastError());
+  MSFEC_HIP(msfec_detail::coarse_solve_device(dim, nc, h->d_M, h->d_b, h->d_cK, h->d_crhs,
+                                              h->d_cvec, h->d_cit, abs_tol, rel_tol, max_iter,
+                                              h->stream));
   const int nw = h->n_cells * nv;
-  hipLaunchKernelGGL(coarse_weights_kernel, dim3((nw + 255) / 256), dim3(256), 0, h->stream, dim, nc,
-                     du, dw);
-  MSFEC_HIP(hipGetLastError());
   MSFEC_HIP(hipStreamSynchronize(h->stream));
   int it = 0;
   double rr = 0.0;
@@ -4195,11 +4753,11 @@ msfec_status msfec_coarse_solve(msfec_q_handle h, int nc, double abs_tol, double
   if (iterations) *iterations = it;
   if (residual) *residual = rr;
   if (u) {
-    msfec_status st = copy_out(u, du, nnode, u_loc);
+    msfec_status st = copy_out(h->stream, u, du, nnode, u_loc);
     if (st) return st;
   }
   if (weights) {
-    msfec_status st = copy_out(weights, dw, (size_t)nw, weights_loc);
+    msfec_status st = copy_out(h->stream, weights, dw, (size_t)nw, weights_loc);
     if (st) return st;
   }
   if (it < 0) return fail(MSFEC_ERR_CELL, "coarse CG did not converge");
@@ -4326,12 +4884,21 @@ msfec_status msfec_q_adjoint_gradient(msfec_q_handle h, int nc, const double* u_
     ADJ_HIP(hipGetLastError());
   }
   const bool f64 = (h->p.solver == MSFEC_SOLVER_BAND_FP64_MFMA);
+  // the panels were stored with the handle's window (FP32 panels: always 64 slots)
+#define SOLVE(R, D, NT) \
+  hipLaunchKernelGGL((q_blk_solve_kernel<R, D, NT>), dim3(h->n_cells), dim3(64), 0, s, a)
+  const int nt = f64 ? c.nt : 4;
   if (dim == 3) {
-    if (f64) hipLaunchKernelGGL((q_blk_solve_kernel<double, 3>), dim3(h->n_cells), dim3(64), 0, s, a);
-    else hipLaunchKernelGGL((q_blk_solve_kernel<float, 3>), dim3(h->n_cells), dim3(64), 0, s, a);
+    if (!f64) SOLVE(float, 3, 4);
+    else if (nt == 1) SOLVE(double, 3, 1);
+    else if (nt == 2) SOLVE(double, 3, 2);
+    else SOLVE(double, 3, 4);
   } else {
-    if (f64) hipLaunchKernelGGL((q_blk_solve_kernel<double, 2>), dim3(h->n_cells), dim3(64), 0, s, a);
-    else hipLaunchKernelGGL((q_blk_solve_kernel<float, 2>), dim3(h->n_cells), dim3(64), 0, s, a);
+    if (!f64) SOLVE(float, 2, 4);
+    else if (nt == 1) SOLVE(double, 2, 1);
+    else if (nt == 2) SOLVE(double, 2, 2);
+    else SOLVE(double, 2, 4);
+#undef SOLVE
   }
   ADJ_HIP(hipGetLastError());
   const size_t gsm = (size_t)2 * c.nn * sizeof(double);

@@ -55,8 +55,8 @@ extern thread_local std::string g_last_error;
 // the matrix-core band LDL^T (msfec_q.hip)
 size_t band_ldl_panel_doubles(int n);
 size_t band_ldl_z_doubles(int n);
-hipError_t band_ldl_solve(int n_cells, int n, const double* rows, const int* loff, double* pan,
-                          double* zst, double* X, int* status, hipStream_t s);
+hipError_t band_ldl_solve(int n_cells, int n, int bw, const double* rows, const int* loff,
+                          double* pan, double* zst, double* X, int* status, hipStream_t s);
 }
 
 namespace {
@@ -446,6 +446,7 @@ struct msfec_rt_batch {
   double* d_w = nullptr;
   int* d_status = nullptr;
   int* d_loff = nullptr;
+  int bw = 0;  // half-bandwidth of the reduced system (largest lower offset)
   bool coef_set = false, computed = false;
   hipStream_t stream = nullptr;
   std::vector<std::array<hipEvent_t, 4>> ev_sets;
@@ -480,10 +481,13 @@ msfec_status rt_in(hipStream_t s, double* dst, const double* src, size_t count, 
   return MSFEC_OK;
 }
 
-msfec_status rt_out(double* dst, const double* src, size_t count, int loc) {
+// Copied on the handle's stream and waited for: a device destination holds
+// the data when the call returns (a device-to-device hipMemcpy may return first).
+msfec_status rt_out(hipStream_t s, double* dst, const double* src, size_t count, int loc) {
   if (!dst) return rt_fail(MSFEC_ERR_INVALID, "null output pointer");
-  RT_HIP(hipMemcpy(dst, src, count * sizeof(double),
-                   loc == MSFEC_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost));
+  RT_HIP(hipMemcpyAsync(dst, src, count * sizeof(double),
+                        loc == MSFEC_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
+  RT_HIP(hipStreamSynchronize(s));
   return MSFEC_OK;
 }
 
@@ -566,6 +570,7 @@ msfec_status msfec_rt_create(const msfec_rt_params* params, int n_cells, const d
   h->ne = 2 * h->m * (h->m + 1);
   const size_t nc = (size_t)n_cells, nf = (size_t)h->m * h->m;
   const std::vector<int> loff = rt_loff(h->m);
+  h->bw = loff.empty() ? 0 : *std::max_element(loff.begin(), loff.end());
   msfec_status st = MSFEC_OK;
 #define RT_TRY(x)         \
   do {                    \
@@ -645,7 +650,7 @@ msfec_status msfec_rt_compute(msfec_rt_handle h, void* stream) {
     RT_HIP(hipGetLastError());
   }
   if (ev) RT_HIP(hipEventRecord((*ev)[1], s));
-  RT_HIP(msfec_detail::band_ldl_solve(h->n_cells, h->n, h->d_rows, h->d_loff, h->d_pan, h->d_zst,
+  RT_HIP(msfec_detail::band_ldl_solve(h->n_cells, h->n, h->bw, h->d_rows, h->d_loff, h->d_pan, h->d_zst,
                                       h->d_X, h->d_status, s));
   if (ev) RT_HIP(hipEventRecord((*ev)[2], s));
   if (a.tensor)
@@ -669,8 +674,8 @@ msfec_status msfec_rt_get_element(msfec_rt_handle h, double* M, double* b, int l
   if (!h->computed) return rt_fail(MSFEC_ERR_STATE, "compute not called");
   RT_HIP(hipStreamSynchronize(h->stream));
   msfec_status st = MSFEC_OK;
-  if (M) st = rt_out(M, h->d_M, (size_t)h->n_cells * kNE, loc);
-  if (st == MSFEC_OK && b) st = rt_out(b, h->d_b, (size_t)h->n_cells * 5, loc);
+  if (M) st = rt_out(h->stream, M, h->d_M, (size_t)h->n_cells * kNE, loc);
+  if (st == MSFEC_OK && b) st = rt_out(h->stream, b, h->d_b, (size_t)h->n_cells * 5, loc);
   return st;
 }
 
@@ -678,7 +683,7 @@ msfec_status msfec_rt_get_basis(msfec_rt_handle h, double* sigma, int loc) {
   if (!h) return rt_fail(MSFEC_ERR_INVALID, "null handle");
   if (!h->computed) return rt_fail(MSFEC_ERR_STATE, "compute not called");
   RT_HIP(hipStreamSynchronize(h->stream));
-  return rt_out(sigma, h->d_sigma, (size_t)h->n_cells * kNB * h->ne, loc);
+  return rt_out(h->stream, sigma, h->d_sigma, (size_t)h->n_cells * kNB * h->ne, loc);
 }
 
 msfec_status msfec_rt_reconstruct(msfec_rt_handle h, const double* w, int w_loc, double* sigma,

@@ -19,7 +19,7 @@ HOST, DEVICE = 0, 1
 COEF_SCALAR_CELL, COEF_TENSOR_QP = 0, 1
 RHS_CONST, RHS_CELL, RHS_QP = 0, 1, 2
 REACTION_NONE, REACTION_CONST, REACTION_CELL = 0, 1, 2
-SOLVER_BAND_FP64, SOLVER_BAND_FP64_MFMA, SOLVER_BAND_MIXED, SOLVER_CG = 0, 1, 2, 3
+SOLVER_BAND_FP64, SOLVER_BAND_FP64_MFMA, SOLVER_BAND_MIXED, SOLVER_CG, SOLVER_AUTO = 0, 1, 2, 3, 4
 CELL_OK, CELL_NOT_CONVERGED, CELL_NOT_SPD, CELL_NONFINITE = 0, 1, 2, 3
 FLAG_TIMING = 1
 
@@ -36,6 +36,10 @@ EXPORTED = (
     "msfec_rt_create", "msfec_rt_set_coefficients", "msfec_rt_compute", "msfec_rt_synchronize",
     "msfec_rt_get_element", "msfec_rt_get_basis", "msfec_rt_reconstruct",
     "msfec_rt_get_cell_status", "msfec_rt_kernel_times", "msfec_rt_destroy",
+    "msfec_qu_create", "msfec_qu_offsets", "msfec_qu_set_coefficient", "msfec_qu_set_rhs",
+    "msfec_qu_set_reaction", "msfec_qu_tune", "msfec_qu_level_info", "msfec_qu_level_handle",
+    "msfec_qu_compute", "msfec_qu_synchronize", "msfec_qu_get_element", "msfec_qu_get_basis",
+    "msfec_qu_get_cell_status", "msfec_qu_coarse_solve", "msfec_qu_destroy",
     "msfec_last_error", "msfec_version",
 )
 
@@ -54,6 +58,7 @@ class Params(ctypes.Structure):
         ("max_refine", ctypes.c_int),
         ("cg_tol", ctypes.c_double),
         ("cg_max_iter", ctypes.c_int),
+        ("band_window", ctypes.c_int),
     ]
 
 
@@ -62,6 +67,8 @@ class NedParams(ctypes.Structure):
         ("n_refine_local", ctypes.c_int),
         ("full_rhs", ctypes.c_int),
         ("flags", ctypes.c_int),
+        ("solver", ctypes.c_int),
+        ("max_refine", ctypes.c_int),
     ]
 
 
@@ -83,6 +90,7 @@ class Info(ctypes.Structure):
         ("n_fine_cells", ctypes.c_int),
         ("n_interior", ctypes.c_int),
         ("bandwidth", ctypes.c_int),
+        ("band_window", ctypes.c_int),
         ("coef_count", ctypes.c_size_t),
         ("rhs_count", ctypes.c_size_t),
         ("reaction_count", ctypes.c_size_t),
@@ -152,11 +160,31 @@ def load(path: str | None = None):
     L.msfec_rt_get_cell_status.argtypes = [vp, ip]
     L.msfec_rt_kernel_times.argtypes = [vp, ctypes.POINTER(ctypes.c_float), ctypes.c_int, ip]
     L.msfec_rt_destroy.argtypes = [vp]
+    sp = ctypes.POINTER(ctypes.c_size_t)
+    L.msfec_qu_create.argtypes = [ctypes.POINTER(Params), ctypes.c_int, vp, ctypes.c_int, ip,
+                                  ctypes.POINTER(vp)]
+    L.msfec_qu_offsets.argtypes = [vp, sp, sp]
+    L.msfec_qu_set_coefficient.argtypes = [vp, vp, ctypes.c_size_t, ctypes.c_int]
+    L.msfec_qu_set_rhs.argtypes = [vp, vp, ctypes.c_size_t, ctypes.c_int]
+    L.msfec_qu_set_reaction.argtypes = [vp, vp, ctypes.c_size_t, ctypes.c_int]
+    L.msfec_qu_tune.argtypes = [vp]
+    L.msfec_qu_level_info.argtypes = [vp, ctypes.c_int, ip, ip, ip, ctypes.POINTER(ctypes.c_float)]
+    L.msfec_qu_level_handle.argtypes = [vp, ctypes.c_int, ctypes.POINTER(vp)]
+    L.msfec_qu_compute.argtypes = [vp, vp]
+    L.msfec_qu_synchronize.argtypes = [vp]
+    L.msfec_qu_get_element.argtypes = [vp, vp, vp, ctypes.c_int]
+    L.msfec_qu_get_basis.argtypes = [vp, vp, ctypes.c_int]
+    L.msfec_qu_get_cell_status.argtypes = [vp, ip]
+    L.msfec_qu_coarse_solve.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                        ctypes.c_int, vp, ctypes.c_int, vp, ctypes.c_int, ip,
+                                        ctypes.POINTER(ctypes.c_double)]
+    L.msfec_qu_destroy.argtypes = [vp]
+    L.msfec_qu_destroy.restype = None
     L.msfec_last_error.restype = ctypes.c_char_p
     L.msfec_version.restype = ctypes.c_char_p
     for name in EXPORTED:
         fn = getattr(L, name)
-        if name not in ("msfec_q_destroy", "msfec_last_error", "msfec_version"):
+        if name not in ("msfec_q_destroy", "msfec_qu_destroy", "msfec_last_error", "msfec_version"):
             fn.restype = ctypes.c_int
     _lib = L
     return L

@@ -12,7 +12,7 @@ CSRC = os.path.join(PKG_ROOT, "csrc")
 INCLUDE = os.path.join(REPO, "include")
 LIB = os.path.join(HERE, "libmsfec_hip.so")
 
-SOURCES = ["msfec_q.hip", "msfec_ned.hip", "msfec_rt.hip"]
+SOURCES = ["msfec_q.hip", "msfec_ned.hip", "msfec_rt.hip", "msfec_uneven.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
@@ -54,8 +54,23 @@ def build_driver(verbose: bool = False) -> str:
 
 
 def build(verbose: bool = False) -> str:
-    cmd = [hipcc(), *HIPCC_FLAGS, f"-I{INCLUDE}",
-           *[os.path.join(CSRC, s) for s in SOURCES], "-o", LIB + ".tmp"]
+    """Every source compiled to an object in parallel, then linked."""
+    from concurrent.futures import ThreadPoolExecutor
+    objdir = os.path.join(PKG_ROOT, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    cc_flags = [f for f in HIPCC_FLAGS if f != "-shared"]
+
+    def compile_one(src):
+        obj = os.path.join(objdir, os.path.splitext(src)[0] + ".o")
+        cmd = [hipcc(), *cc_flags, f"-I{INCLUDE}", "-c", os.path.join(CSRC, src), "-o", obj]
+        if verbose:
+            print(" ".join(cmd))
+        subprocess.run(cmd, check=True)
+        return obj
+
+    with ThreadPoolExecutor(len(SOURCES)) as ex:
+        objs = list(ex.map(compile_one, SOURCES))
+    cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", LIB + ".tmp"]
     if verbose:
         print(" ".join(cmd))
     subprocess.run(cmd, check=True)

@@ -26,12 +26,16 @@ def ned_sizes(n_refine_local):
 
 
 class NedBasisBatch:
-    def __init__(self, vertices, n_refine_local=4, full_rhs=True, timing=False):
+    def __init__(self, vertices, n_refine_local=4, full_rhs=True, timing=False,
+                 solver=L.SOLVER_BAND_FP64_MFMA, max_refine=0):
+        """solver: SOLVER_BAND_FP64_MFMA (FP64 LDL^T) or SOLVER_BAND_MIXED
+        (FP32 LDL^T + FP64 refinement, at most max_refine passes, 0 = 3)."""
         self._lib = L.load()
         self._h = ctypes.c_void_p()
         p = L.NedParams()
         p.n_refine_local, p.full_rhs = n_refine_local, int(bool(full_rhs))
         p.flags = L.FLAG_TIMING if timing else 0
+        p.solver, p.max_refine = solver, max_refine
         vptr, vloc, keep = _ptr(vertices)
         n_cells = int(np.prod(np.shape(vertices))) // 8
         L.check(self._lib.msfec_ned_create(ctypes.byref(p), n_cells, vptr, vloc,

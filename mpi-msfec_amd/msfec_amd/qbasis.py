@@ -50,7 +50,7 @@ class QBasisBatch:
     def __init__(self, vertices, dim=3, n_refine_local=3, coef_mode=L.COEF_SCALAR_CELL,
                  rhs_mode=L.RHS_CONST, rhs_const=1.0, reaction_mode=L.REACTION_NONE,
                  reaction_const=0.0, solver=L.SOLVER_BAND_FP64, timing=False, max_refine=0,
-                 cg_tol=0.0, cg_max_iter=0):
+                 cg_tol=0.0, cg_max_iter=0, band_window=0):
         self._lib = L.load()
         self._h = ctypes.c_void_p()
         p = L.Params()
@@ -59,6 +59,7 @@ class QBasisBatch:
         p.solver, p.flags = solver, (L.FLAG_TIMING if timing else 0)
         p.max_refine = max_refine
         p.cg_tol, p.cg_max_iter = cg_tol, cg_max_iter
+        p.band_window = band_window
         vptr, vloc, keep = _ptr(vertices)
         nv = 1 << dim
         n_cells = int(np.prod(vertices.shape)) // (nv * dim)

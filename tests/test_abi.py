@@ -138,10 +138,8 @@ def test_ned_rejects_non_box_and_bad_level(lib):
 
 
 def test_cg_solver_limits(lib):
-    """CG lifts the band limit (3-D l = 4 accepted up to the allocation) but
-    refuses tensor coefficients and cells beyond its workgroup."""
-    st, msg = _create(lib, coef_mode=1, solver=lib.SOLVER_CG)
-    assert st == lib.ERR_UNSUPPORTED and "CG" in msg
+    """CG lifts the band limit (3-D l = 4 accepted up to the allocation, with
+    scalar or tensor coefficients) but refuses cells beyond its workgroup."""
     st, msg = _create(lib, dim=3, l=5, solver=lib.SOLVER_CG)
     assert st == lib.ERR_UNSUPPORTED and "CG" in msg
 
@@ -194,3 +192,41 @@ def test_output_buffers_are_never_silently_copied():
         with pytest.raises(ValueError):
             _out_ptr(b, shape)
 
+
+def test_band_window_validated_on_the_host(lib):
+    """band_window: 0, 16, 32 or 64 slots, and large enough for the half-
+    bandwidth + 4; checked before any device call."""
+    L = lib.load()
+    verts = np.ascontiguousarray(box_vertices(3, [0.0] * 3, [1.0] * 3)[None])
+    for window, l, want in ((48, 2, lib.ERR_INVALID), (16, 2, lib.ERR_UNSUPPORTED),
+                            (32, 3, lib.ERR_UNSUPPORTED), (16, 3, lib.ERR_UNSUPPORTED)):
+        p = lib.Params()
+        p.dim, p.n_refine_local, p.rhs_const, p.solver = 3, l, 1.0, lib.SOLVER_BAND_FP64_MFMA
+        p.band_window = window
+        h = ctypes.c_void_p()
+        st = L.msfec_q_create(ctypes.byref(p), 1, ctypes.c_void_p(verts.ctypes.data), lib.HOST,
+                              ctypes.byref(h))
+        assert st == want, (window, l, st, L.msfec_last_error())
+        assert "window" in L.msfec_last_error().decode()
+
+
+def test_uneven_create_rejects_on_the_host(lib):
+    """msfec_qu_create: levels outside the supported range, tensor
+    coefficients and unknown solvers are refused before any device call."""
+    L = lib.load()
+    verts = np.ascontiguousarray(np.stack([box_vertices(3, [0.0] * 3, [1.0] * 3)] * 2))
+    cases = (({"solver": lib.SOLVER_AUTO}, [2, 5], lib.ERR_UNSUPPORTED),
+             ({"solver": lib.SOLVER_AUTO}, [0, 2], lib.ERR_UNSUPPORTED),
+             ({"solver": 9}, [2, 3], lib.ERR_INVALID),
+             ({"solver": lib.SOLVER_AUTO, "coef_mode": 1}, [2, 3], lib.ERR_UNSUPPORTED),
+             ({"solver": lib.SOLVER_AUTO, "rhs_mode": 2}, [2, 3], lib.ERR_UNSUPPORTED))
+    for fields, levels, want in cases:
+        p = lib.Params()
+        p.dim, p.rhs_const = 3, 1.0
+        for k, v in fields.items():
+            setattr(p, k, v)
+        lv = np.array(levels, dtype=np.int32)
+        h = ctypes.c_void_p()
+        st = L.msfec_qu_create(ctypes.byref(p), 2, ctypes.c_void_p(verts.ctypes.data), lib.HOST,
+                               lv.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), ctypes.byref(h))
+        assert st == want, (fields, levels, st, L.msfec_last_error())

@@ -29,6 +29,7 @@ def test_cg_lognormal(oracle_mod, dim, l):
     coef = np.exp(1.5 * rng.standard_normal((n_cells, (1 << l) ** dim)))
     g = run_gpu(verts, dim, l, coef, solver=CG)
     o = run_oracle(oracle_mod, verts, dim, l, coef)
+    print("errors", {k: rel(g[k], o[k]) for k in ("M", "b", "Psi")}, flush=True)
     assert (g["status"] == 0).all(), g["status"]
     assert rel(g["M"], o["M"]) < TOL_M
     assert rel(g["b"], o["b"]) < TOL_M
@@ -64,6 +65,7 @@ def test_cg_inclusions_l4(oracle_mod):
     pick = np.argsort(-(coef_all > 10).mean(axis=1))[:2]
     g = run_gpu(verts_all[pick], 3, 4, coef_all[pick], solver=CG)
     o = run_oracle(oracle_mod, verts_all[pick], 3, 4, coef_all[pick])
+    print("errors", {k: rel(g[k], o[k]) for k in ("M", "b", "Psi")}, flush=True)
     assert (g["status"] == 0).all()
     assert rel(g["M"], o["M"]) < TOL_M
     assert rel(g["b"], o["b"]) < TOL_M
@@ -107,3 +109,67 @@ def test_cg_bad_cells_and_iteration_cap():
         it = qb.cg_iterations()
     assert (st == L.CELL_NOT_CONVERGED).all()
     assert (it == 2).all()
+
+
+def random_spd_tensors(rng, n_cells, nf, nq, dim):
+    """Random SPD tensors per Gauss point, B B^T + 0.5 I, as the symmetric
+    3- or 6-vectors of MSFEC_COEF_TENSOR_QP."""
+    B = rng.standard_normal((n_cells, nf, nq, dim, dim))
+    A = np.einsum("...ij,...kj->...ik", B, B) + 0.5 * np.eye(dim)
+    if dim == 2:
+        return np.stack([A[..., 0, 0], A[..., 0, 1], A[..., 1, 1]], -1)
+    return np.stack([A[..., 0, 0], A[..., 0, 1], A[..., 0, 2], A[..., 1, 1], A[..., 1, 2],
+                     A[..., 2, 2]], -1)
+
+
+@pytest.mark.parametrize("dim,l", [(2, 3), (2, 5), (3, 2), (3, 3)])
+def test_cg_tensor_qp_and_qp_load(oracle_mod, dim, l):
+    """CG on the assembled stencil: anisotropic tensors and loads at the
+    Gauss points, with and without a reaction (all bases solved, or the last
+    one from the partition of unity), against the oracle's direct solve and
+    the band solver of the same assembled system."""
+    print(f"case cg_tensor dim={dim} l={l}", flush=True)
+    rng = np.random.default_rng(400 + 10 * dim + l)
+    n_cells, N, nq = 4, 1 << l, 1 << dim
+    verts = random_boxes(rng, dim, n_cells)
+    T = random_spd_tensors(rng, n_cells, N ** dim, nq, dim)
+    f = rng.standard_normal((n_cells, N ** dim, nq))
+    for reaction in (0.0, 2.0):
+        kw = dict(coef_mode=1, f=f, rhs_mode=2, reaction_mode=1 if reaction else 0,
+                  reaction_const=reaction)
+        g = run_gpu(verts, dim, l, T, solver=CG, **kw)
+        o = run_oracle(oracle_mod, verts, dim, l, T, **kw)
+        assert (g["status"] == 0).all(), g["status"]
+        assert rel(g["M"], o["M"]) < TOL_M, reaction
+        assert rel(g["b"], o["b"]) < TOL_M, reaction
+        assert rel(g["Psi"], o["Psi"]) < TOL_PSI, reaction
+        if l <= 3:
+            gb = run_gpu(verts, dim, l, T, solver=0, **kw)
+            assert rel(g["M"], gb["M"]) < TOL_M
+
+
+def test_cg_reference_example_l4(oracle_mod):
+    """The reference's example (example_parameters/prm_q_test-01.prm): coarse
+    mesh of 4^3 cells (global refinements 2), local refinements 4 (16^3 fine
+    cells, 3375 interior unknowns), the rotated oscillatory Diffusion_A
+    (frequencies 17, 16, 15, alpha 0.9, scale 1, rotate) and f = sin(2 pi x)
+    sin(2 pi y) sin(2 pi z), both at the QGauss(2) points.  Two cells against
+    the oracle, which evaluates Diffusion_A inline (q_basis.cc:196-220)."""
+    print("case cg_reference_example dim=3 l=4", flush=True)
+    from msfec_amd import fields as F
+    dim, l = 3, 4
+    verts = F.coarse_vertices(dim, 4)[[5, 42]]
+    pts = F.qp_coordinates(verts, l)
+    T = F.diffusion_A_tensor(pts, freq=(17, 16, 15), scale=(1.0, 1.0, 1.0),
+                             alpha=(0.9, 0.9, 0.9), rotate=True)
+    f = np.prod(np.sin(2 * np.pi * pts), axis=-1)
+    g = run_gpu(verts, dim, l, T, coef_mode=1, f=f, rhs_mode=2, solver=CG)
+    o = oracle_mod.basis_batch(
+        oracle_mod.make_problem(dim, l, coef_mode=oracle_mod.COEF_ANALYTIC, rhs_mode=2,
+                                freq=(17, 16, 15), scale=(1.0, 1.0, 1.0), alpha=(0.9, 0.9, 0.9),
+                                rotate=True),
+        verts, f=f, solver=oracle_mod.SOLVER_DIRECT, n_threads=2)
+    assert (g["status"] == 0).all(), g["status"]
+    e = {k: rel(g[k], o[k]) for k in ("M", "b", "Psi")}
+    print("reference example errors", e, flush=True)
+    assert e["M"] < TOL_M and e["b"] < TOL_M and e["Psi"] < TOL_PSI, e

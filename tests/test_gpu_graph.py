@@ -123,3 +123,20 @@ def test_rt_compute_graph_replay(stream):
         replay_check(rb, stream, rb.set_coefficients,
                      lambda: (rb.get_global_element_matrix(), rb.get_global_element_rhs(),
                               rb.get_basis(), rb.get_cell_status()), data)
+
+
+def test_uneven_compute_graph_replay(stream):
+    """msfec_qu_compute forks the levels onto streams of their own and joins
+    them back with event waits: captured from the caller's stream, the graph
+    holds every level's kernels.  The first eager compute does the timing."""
+    print("case graph uneven", flush=True)
+    from msfec_amd import UnevenBatch, fields as F
+    nc = 3
+    cen = F.inclusion_centres(3, 2, seed=71)
+    lv = F.uneven_levels(3, nc, cen, seed=72)
+    verts = F.coarse_vertices(3, nc)
+    data = [F.inclusion_blocks(3, nc, lv, cen, r, a) for r, a in ((0.1, 1e3), (0.15, 30.0))]
+    with UnevenBatch(verts, lv, dim=3) as ub:
+        replay_check(ub, stream, ub.set_coefficients,
+                     lambda: (ub.get_global_element_matrix(), ub.get_global_element_rhs(),
+                              ub.get_basis_flat(), ub.get_cell_status()), data)

@@ -205,18 +205,34 @@ def test_headline_batch_invariants(oracle_mod, solver):
     assert rel(Psi[sample], o["Psi"]) < tol(solver)
 
 
+@pytest.mark.parametrize("reaction", [0.0, 1.0])
 @pytest.mark.parametrize("dim,nc,l", [(2, 8, 4), (3, 4, 3)])
-def test_coarse_solve_and_reconstruct(oracle_mod, dim, nc, l):
-    print(f"case coarse dim={dim} nc={nc} l={l}", flush=True)
-    from msfec_amd import QBasisBatch, fields as F
+def test_coarse_solve_and_reconstruct(oracle_mod, dim, nc, l, reaction):
+    """The whole method on a structured coarse grid: 2-D 8 x 8 cells refined 4
+    levels, a = exp(g) with g iid N(0, 1) (the plain-method configuration,
+    with the reaction 1 of -div(A grad u) + u = f and with the reference's
+    reaction 0), and 3-D 4^3 cells refined 3 levels.  M, b and Psi of every
+    cell against the oracle's direct solve; the coarse solve against the
+    oracle's coarse solve of the same M and b, and against that of the
+    oracle's own M and b; the reconstruction."""
+    print(f"case coarse dim={dim} nc={nc} l={l} reaction={reaction}", flush=True)
+    from msfec_amd import QBasisBatch, _lib as L, fields as F
     verts = F.coarse_vertices(dim, nc)
-    coef = F.lognormal_coefficient(dim, nc, l, 1.0, 0.0, seed=3)
-    with QBasisBatch(verts, dim=dim, n_refine_local=l) as qb:
+    coef = F.lognormal_coefficient(dim, nc, l, 1.0, 0.0, seed=3 if dim == 3 else 20261202)
+    rmode = L.REACTION_CONST if reaction else L.REACTION_NONE
+    with QBasisBatch(verts, dim=dim, n_refine_local=l, reaction_mode=rmode,
+                     reaction_const=reaction) as qb:
         qb.set_coefficient(coef)
         qb.run()
         M, b, Psi = qb.get_global_element_matrix(), qb.get_global_element_rhs(), qb.get_basis()
         u, w, it, res = qb.coarse_solve(nc, abs_tol=0.0, rel_tol=1e-13)
         uf = qb.set_global_weights(w)
+    o = run_oracle(oracle_mod, verts, dim, l, coef, reaction_mode=rmode, reaction_const=reaction)
+    assert rel(M, o["M"]) < TOL
+    assert rel(b, o["b"]) < TOL
+    assert rel(Psi, o["Psi"]) < TOL
+    uoo, _ = oracle_mod.coarse_solve(dim, nc, o["M"], o["b"], rtol=1e-14)
+    assert np.abs(u - uoo).max() <= 1e-10 * np.abs(uoo).max()
     uo, ito = oracle_mod.coarse_solve(dim, nc, M, b, rtol=1e-14)
     assert it > 0 and ito > 0
     assert np.abs(u - uo).max() <= 1e-10 * np.abs(uo).max()
@@ -278,3 +294,63 @@ def test_solvers_agree(dim, l):
         assert rel(g0["b"], g1["b"]) < tol(solver)
         assert rel(g0["Psi"], g1["Psi"]) < tol(solver)
 
+
+
+@pytest.mark.parametrize("dim,l,window", [(2, 1, 16), (2, 2, 16), (2, 3, 16), (2, 3, 32),
+                                          (2, 4, 32), (2, 4, 64), (2, 5, 64), (3, 1, 16),
+                                          (3, 1, 32), (3, 2, 32), (3, 2, 64)])
+def test_band_windows(oracle_mod, dim, l, window):
+    """The matrix-core band factorisation with a 16-, 32- or 64-slot window
+    (1, 2 or 4 tile rows of 16; the window must hold the half-bandwidth + 4):
+    every window that holds the band gives the oracle's basis, and a window
+    too small for it is refused."""
+    print(f"case band_window dim={dim} l={l} window={window}", flush=True)
+    from msfec_amd import QBasisBatch, MsfecError, _lib as L
+    rng = np.random.default_rng(70 + 10 * dim + l)
+    n_cells, N = 7, 1 << l
+    verts = random_boxes(rng, dim, n_cells)
+    coef = np.exp(1.5 * rng.standard_normal((n_cells, N ** dim)))
+    with QBasisBatch(verts, dim=dim, n_refine_local=l, solver=L.SOLVER_BAND_FP64_MFMA,
+                     band_window=window) as qb:
+        assert qb.info.band_window == window
+        qb.set_coefficient(coef)
+        qb.run()
+        g = {"M": qb.get_global_element_matrix(), "b": qb.get_global_element_rhs(),
+             "Psi": qb.get_basis(), "status": qb.get_cell_status()}
+    o = run_oracle(oracle_mod, verts, dim, l, coef)
+    assert (g["status"] == 0).all()
+    # b and Psi against the oracle: with this contrast (exp(1.5 N(0,1)) iid
+    # per fine cell) the oracle's own dense solve leaves ~1e-12 absolute on
+    # entries that are exactly zero here, 1.1e-11 relative on Psi and 1.8e-11
+    # on b (3-D l = 2, for every window alike): the metric's 1e-10
+    for k, t in (("M", TOL), ("b", TOL_MIXED), ("Psi", TOL_MIXED)):
+        assert rel(g[k], o[k]) < t, k
+    if window > 16:
+        ni = N - 1
+        bw = ni + 1 if dim == 2 else ni * ni + ni + 1
+        if bw + 4 > window // 2:
+            with pytest.raises(MsfecError):
+                QBasisBatch(verts, dim=dim, n_refine_local=l, solver=L.SOLVER_BAND_FP64_MFMA,
+                            band_window=window // 2)
+
+
+@pytest.mark.parametrize("dim,l,w1,w2", [(2, 4, 32, 64), (3, 2, 32, 64), (2, 3, 16, 32)])
+def test_band_windows_agree(dim, l, w1, w2):
+    """Two windows that both hold the band do the same eliminations (an
+    entry's updates arrive in the same order; rows outside the band only add
+    exact zeros), so the bases agree to a few ulps."""
+    print(f"case band_windows_agree dim={dim} l={l} {w1} vs {w2}", flush=True)
+    from msfec_amd import QBasisBatch, _lib as L
+    rng = np.random.default_rng(90 + dim + l)
+    n_cells, N = 9, 1 << l
+    verts = random_boxes(rng, dim, n_cells)
+    coef = np.exp(2.0 * rng.standard_normal((n_cells, N ** dim)))
+    out = []
+    for w in (w1, w2):
+        with QBasisBatch(verts, dim=dim, n_refine_local=l, solver=L.SOLVER_BAND_FP64_MFMA,
+                         band_window=w) as qb:
+            qb.set_coefficient(coef)
+            qb.run()
+            out.append((qb.get_global_element_matrix(), qb.get_basis()))
+    assert rel(out[0][0], out[1][0]) < 1e-13
+    assert rel(out[0][1], out[1][1]) < 1e-13

@@ -273,3 +273,57 @@ def test_ned_driver_vtu_output(oracle_mod):
             uv = P[0, b, ns + nh:].reshape(m, m + 1)
             np.testing.assert_allclose(u[:, 0], (0.5 * (uh[:-1] + uh[1:]) / h).ravel(), rtol=1e-14, atol=1e-15)
             np.testing.assert_allclose(u[:, 1], (0.5 * (uv[:, :-1] + uv[:, 1:]) / h).ravel(), rtol=1e-14, atol=1e-15)
+
+
+def test_multiscale_driver_uneven_levels(oracle_mod):
+    """The C++ map loop with cells of mixed refinement (QBasis(..., level),
+    QBatch on msfec_qu_*): element data per level against the oracle, and the
+    fine solution of every cell against the oracle's coarse solve and
+    reconstruction."""
+    print("case shim_driver uneven", flush=True)
+    from msfec_amd import fields as F
+    assert os.path.exists(DRIVER), "driver not built"
+    dim, nc = 3, 4
+    cen = F.inclusion_centres(3, 2, seed=81)
+    lv = F.uneven_levels(3, nc, cen, seed=82, coarse=(2, 3))
+    blocks = F.inclusion_blocks(3, nc, lv, cen, 0.08, 1e3)
+    ncell, nv = nc ** dim, 1 << dim
+    nns = [((1 << int(l)) + 1) ** dim for l in lv]
+    with tempfile.TemporaryDirectory() as td:
+        fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+        with open(fin, "wb") as fh:
+            fh.write(np.array([dim, 0, nc], dtype=np.int32).tobytes())
+            fh.write(np.asarray(lv, dtype=np.int32).tobytes())
+            fh.write(np.concatenate([np.ravel(x) for x in blocks]).astype(np.float64).tobytes())
+        r = subprocess.run([DRIVER, fin, fout], capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, r.stderr
+        raw = open(fout, "rb").read()
+    d = np.frombuffer(raw[:-4], dtype=np.float64)
+    o = 0
+    M = d[o:o + ncell * nv * nv].reshape(ncell, nv, nv); o += ncell * nv * nv
+    b = d[o:o + ncell * nv].reshape(ncell, nv); o += ncell * nv
+    w = d[o:o + ncell * nv].reshape(ncell, nv); o += ncell * nv
+    uf = []
+    for nn in nns:
+        uf.append(d[o:o + nn]); o += nn
+    assert o == d.size
+    verts = F.coarse_vertices(dim, nc)
+    Mo, bo, Po = np.zeros_like(M), np.zeros_like(b), [None] * ncell
+    for l in sorted(set(lv.tolist())):
+        idx = np.nonzero(lv == l)[0]
+        ref = oracle_mod.basis_batch(oracle_mod.make_problem(dim, int(l)), verts[idx],
+                                     coef=np.stack([blocks[c] for c in idx]),
+                                     solver=oracle_mod.SOLVER_DIRECT)
+        Mo[idx], bo[idx] = ref["M"], ref["b"]
+        for k, c in enumerate(idx):
+            Po[c] = ref["Psi"][k]
+    assert np.abs(M - Mo).max() < 1e-10 * np.abs(Mo).max()
+    assert np.abs(b - bo).max() < 1e-10 * np.abs(bo).max()
+    u, _ = oracle_mod.coarse_solve(dim, nc, Mo, bo, rtol=1e-14)
+    for c in range(ncell):
+        ix, iy, iz = c % nc, (c // nc) % nc, c // (nc * nc)
+        wc = np.array([u[(ix + (k & 1)) + (nc + 1) * ((iy + ((k >> 1) & 1)) + (nc + 1) * (iz + (k >> 2)))]
+                       for k in range(nv)])
+        assert np.abs(w[c] - wc).max() < 1e-9 * np.abs(u).max()
+        ufo = wc @ Po[c]
+        assert np.abs(uf[c] - ufo).max() < 1e-9 * np.abs(ufo).max(), c
