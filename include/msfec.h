@@ -163,6 +163,32 @@ msfec_status msfec_q_set_reaction(msfec_q_handle h, const double *data,
  * loop and assemble_global_element_matrix (:95-357). */
 msfec_status msfec_q_compute(msfec_q_handle h, void *stream);
 
+/* msfec_q_compute without a handle and without any allocation: no
+ * counterpart in the reference.  It is for callers that cannot allocate
+ * device memory -- code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc.  It runs the kernels of msfec_q_compute
+ * with MSFEC_SOLVER_BAND_FP64_MFMA (bit-identical results) and takes the
+ * params of msfec_q_create restricted to that solver, a scalar coefficient
+ * per fine cell, MSFEC_RHS_CONST and MSFEC_REACTION_NONE / _CONST.  Every
+ * pointer is device memory owned by the caller: vertices (n_cells x 2^dim x
+ * dim, axis-aligned boxes, not checked here), coef (n_cells x 2^(l dim)),
+ * Psi (n_cells x nb x n_fine_nodes), M (n_cells x nb x nb), b (n_cells x
+ * nb), status (n_cells ints).  Asynchronous on `stream`: kernel launches
+ * only (no allocation, no memset, no host synchronisation).
+ * Scratch (row table, panels, forward results) lives in one 128 MiB device
+ * array per GPU, held from the library's first use on that GPU until the
+ * process ends; a call processes its cells in chunks that fit it (3-D l = 3:
+ * 498 cells per chunk, one after another on the stream).  Every scratch
+ * word a chunk reads is written earlier in the same chunk, so no clearing is
+ * needed; calls that overlap on one GPU (on different streams or from
+ * different threads) share the array and race.  Rejected with
+ * MSFEC_ERR_UNSUPPORTED: other solvers or modes, and levels beyond the band
+ * window (3-D l > 3, 2-D l > 5). */
+msfec_status msfec_q_compute_noalloc(const msfec_q_params *params, int n_cells,
+                                     const double *vertices, const double *coef,
+                                     double *Psi, double *M, double *b, int *status,
+                                     void *stream);
+
 /* Wait for the work of the last compute. */
 msfec_status msfec_q_synchronize(msfec_q_handle h);
 

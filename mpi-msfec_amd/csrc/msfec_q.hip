@@ -30,6 +30,7 @@
 #include <array>
 #include <new>
 #include <string>
+#include <mutex>
 #include <vector>
 
 #include "msfec.h"
@@ -4183,6 +4184,45 @@ void fill_qconf(const msfec_q_params& p, QConf& c) {
 }
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// The allocation-free entry point (msfec_q_compute_noalloc).  Its kernels
+// (q_rows_kernel + q_blk_factor_kernel<double, DIM, BK_FINAL64>, those of
+// msfec_q_compute with MSFEC_SOLVER_BAND_FP64_MFMA) take their scratch -- the
+// row table, P | b_B, the panels and the forward results -- from one
+// file-scope device array instead of a handle's hipMalloc'd buffers.  Each
+// GPU the library runs on holds its own copy of the array until the process
+// ends.  Its host-side name is not a device address: the address is looked
+// up once per device with hipGetSymbolAddress and only that is passed on.
+// ---------------------------------------------------------------------------
+constexpr size_t kNoAllocScratchBytes = (size_t)128 << 20;  // 128 MiB per GPU
+constexpr int kNoAllocMaxDevices = 64;
+static __device__ __align__(256) double scratch[kNoAllocScratchBytes / sizeof(double)];
+
+namespace {
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kNoAllocMaxDevices] = {};
+
+// the device address of `scratch` on the current device, looked up once
+msfec_status scratch_on_device(double** out) {
+  int dev = -1;
+  MSFEC_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kNoAllocMaxDevices)
+    return fail(MSFEC_ERR_UNSUPPORTED, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(MSFEC_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return MSFEC_OK;
+}
+
+inline size_t round_up32(size_t x) { return (x + 31) & ~(size_t)31; }
+}  // namespace
+
 extern "C" {
 
 const char* msfec_last_error(void) { return g_last_error.c_str(); }
@@ -4914,6 +4954,97 @@ msfec_status msfec_q_adjoint_gradient(msfec_q_handle h, int nc, const double* u_
 #undef ADJ_HIP
   cleanup();
   if (it < 0) return fail(MSFEC_ERR_CELL, "adjoint coarse CG did not converge");
+  return MSFEC_OK;
+}
+
+msfec_status msfec_q_compute_noalloc(const msfec_q_params* params, int n_cells,
+                                     const double* vertices, const double* coef, double* Psi,
+                                     double* M, double* b, int* status, void* stream) {
+  if (!params || !vertices || !coef || !Psi || !M || !b || !status)
+    return fail(MSFEC_ERR_INVALID, "null argument");
+  const msfec_q_params& p = *params;
+  if (p.dim != 2 && p.dim != 3) return fail(MSFEC_ERR_INVALID, "dim must be 2 or 3");
+  if (n_cells <= 0) return fail(MSFEC_ERR_INVALID, "n_cells must be positive");
+  if (p.n_refine_local < 1) return fail(MSFEC_ERR_INVALID, "n_refine_local must be >= 1");
+  if (p.solver != MSFEC_SOLVER_BAND_FP64_MFMA || p.coef_mode != MSFEC_COEF_SCALAR_CELL ||
+      p.rhs_mode != MSFEC_RHS_CONST ||
+      (p.reaction_mode != MSFEC_REACTION_NONE && p.reaction_mode != MSFEC_REACTION_CONST))
+    return fail(MSFEC_ERR_UNSUPPORTED,
+                "msfec_q_compute_noalloc takes MSFEC_SOLVER_BAND_FP64_MFMA with a scalar "
+                "coefficient per fine cell, a constant load and no or a constant reaction");
+  const int ni = (1 << p.n_refine_local) - 1;
+  const int bw = (p.dim == 2) ? ni + 1 : ni * ni + ni + 1;
+  const int nt = band_tile_rows(bw, 0);
+  if (nt == 0)
+    return fail(MSFEC_ERR_UNSUPPORTED,
+                "interior half-bandwidth " + std::to_string(bw) +
+                    " exceeds the 60 of the band solver (3-D: n_refine_local <= 3, 2-D: <= 5)");
+  QConf c{};
+  fill_qconf(p, c);
+  // scratch per cell: row table | P b_B | panels | forward results
+  const size_t nq = (size_t)panel_quads(c.n);
+  const size_t rows_d = (size_t)c.n * kRowStride, pb_d = kPStride, pan_d = nq * 256, z_d = nq * 32;
+  const size_t per_cell = rows_d + pb_d + pan_d + z_d;
+  const size_t cap_d = kNoAllocScratchBytes / sizeof(double);
+  if (4 * 32 + per_cell > cap_d)
+    return fail(MSFEC_ERR_UNSUPPORTED, "one cell needs more scratch than the 128 MiB array holds");
+  const int chunk = (int)std::min<size_t>((size_t)n_cells, (cap_d - 4 * 32) / per_cell);
+  double* scr = nullptr;
+  msfec_status st = scratch_on_device(&scr);
+  if (st != MSFEC_OK) return st;
+  double* d_rows = scr;
+  double* d_pb = d_rows + round_up32((size_t)chunk * rows_d);
+  double* d_pan = d_pb + round_up32((size_t)chunk * pb_d);
+  double* d_z = d_pan + round_up32((size_t)chunk * pan_d);
+  hipStream_t s = (hipStream_t)stream;
+  const int nv = c.nv;
+  const size_t rdyn = (size_t)c.nfc * sizeof(double);
+  const bool reac = p.reaction_mode != MSFEC_REACTION_NONE;
+  // chunks of cells run one after another on the stream, each through the
+  // same scratch: the kernels of a chunk are stream-ordered after those of
+  // the chunk before
+  for (int c0 = 0; c0 < n_cells; c0 += chunk) {
+    const int nc = std::min(chunk, n_cells - c0);
+    BandArgs a{};
+    a.n = c.n;
+    a.ni = c.ni;
+    a.N = c.N;
+    a.S = c.S;
+    a.nn = c.nn;
+    a.nfc = c.nfc;
+    for (int t = 0; t < kMaxLow; ++t) a.off_int[t] = c.off_int[t];
+    a.Rows = d_rows;
+    a.verts = vertices + (size_t)c0 * nv * c.dim;
+    a.coef = coef + (size_t)c0 * c.nfc;
+    a.reac = nullptr;
+    a.fcell = nullptr;
+    a.reaction_mode = c.reaction_mode;
+    a.rhs_mode = c.rhs_mode;
+    a.reaction_const = c.reaction_const;
+    a.rhs_const = c.rhs_const;
+    a.pan = d_pan;
+    a.zst = d_z;
+    a.Pb = d_pb;
+    a.Psi = Psi + (size_t)c0 * nv * c.nn;
+    a.M = M + (size_t)c0 * nv * nv;
+    a.bvec = b + (size_t)c0 * nv;
+    a.status = status + c0;
+    a.nq = (int)nq;
+    if (c.dim == 3) {
+      if (reac)
+        hipLaunchKernelGGL((q_rows_kernel<3, true>), dim3(nc), dim3(256), rdyn, s, a);
+      else
+        hipLaunchKernelGGL((q_rows_kernel<3, false>), dim3(nc), dim3(256), rdyn, s, a);
+      launch_factor64(3, nt, nc, s, a);
+    } else {
+      if (reac)
+        hipLaunchKernelGGL((q_rows_kernel<2, true>), dim3(nc), dim3(256), rdyn, s, a);
+      else
+        hipLaunchKernelGGL((q_rows_kernel<2, false>), dim3(nc), dim3(256), rdyn, s, a);
+      launch_factor64(2, nt, nc, s, a);
+    }
+    MSFEC_HIP(hipGetLastError());
+  }
   return MSFEC_OK;
 }
 

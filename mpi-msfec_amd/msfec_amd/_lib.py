@@ -29,7 +29,7 @@ EXPORTED = (
     "msfec_q_compute", "msfec_q_synchronize", "msfec_q_get_element", "msfec_q_get_basis",
     "msfec_q_get_cell_status", "msfec_q_reconstruct", "msfec_q_get_info",
     "msfec_q_kernel_times", "msfec_q_destroy", "msfec_coarse_solve", "msfec_q_adjoint_gradient",
-    "msfec_q_get_cg_iterations",
+    "msfec_q_get_cg_iterations", "msfec_q_compute_noalloc",
     "msfec_ned_create", "msfec_ned_set_coefficients", "msfec_ned_compute",
     "msfec_ned_synchronize", "msfec_ned_get_element", "msfec_ned_get_basis",
     "msfec_ned_get_cell_status", "msfec_ned_kernel_times", "msfec_ned_destroy",

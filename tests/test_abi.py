@@ -193,6 +193,33 @@ def test_output_buffers_are_never_silently_copied():
             _out_ptr(b, shape)
 
 
+def test_noalloc_rejects_before_touching_the_gpu(lib):
+    """The allocation-free entry point validates its request on the host:
+    null pointers and unsupported solvers / modes / levels return an error
+    before any device call."""
+    L = lib.load()
+    p = lib.Params()
+    p.dim, p.n_refine_local, p.coef_mode, p.rhs_mode, p.rhs_const = 3, 3, 0, 0, 1.0
+    p.reaction_mode, p.reaction_const, p.solver = 0, 0.0, lib.SOLVER_BAND_FP64_MFMA
+    fake = ctypes.c_void_p(0x1000)  # never dereferenced: every case fails validation first
+    args = [fake] * 7 + [ctypes.c_void_p(0)]
+    assert L.msfec_q_compute_noalloc(ctypes.byref(p), 4, None, *args[1:]) == lib.ERR_INVALID
+    for field, value in (("solver", lib.SOLVER_BAND_FP64), ("solver", lib.SOLVER_CG),
+                         ("coef_mode", 1), ("rhs_mode", 1), ("reaction_mode", 2),
+                         ("n_refine_local", 4)):
+        q = lib.Params()
+        ctypes.pointer(q)[0] = p
+        setattr(q, field, value)
+        st = L.msfec_q_compute_noalloc(ctypes.byref(q), 4, *args)
+        assert st == lib.ERR_UNSUPPORTED, (field, value, st)
+        assert "noalloc" in L.msfec_last_error().decode() or "band" in L.msfec_last_error().decode()
+    q = lib.Params()
+    ctypes.pointer(q)[0] = p
+    q.dim = 4
+    assert L.msfec_q_compute_noalloc(ctypes.byref(q), 4, *args) == lib.ERR_INVALID
+    assert L.msfec_q_compute_noalloc(ctypes.byref(p), 0, *args) == lib.ERR_INVALID
+
+
 def test_band_window_validated_on_the_host(lib):
     """band_window: 0, 16, 32 or 64 slots, and large enough for the half-
     bandwidth + 4; checked before any device call."""
