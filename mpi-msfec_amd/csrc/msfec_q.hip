@@ -1695,7 +1695,7 @@ constexpr int bk_row() {
 template <typename R>
 struct BkSmem {
   R piv[4 * 80];   // forward: pivot rows; backward: panel [4][68]
-  R nrow[4 * 80];  // forward: incoming rows
+  R nrow[4 * 80 + 64];  // forward: incoming rows, then one spare slot per lane
   R yp[64];        // forward: pivot right-hand sides [4][16]
 };
 
@@ -1794,14 +1794,17 @@ __device__ __forceinline__ void bk_scatter_store(const BandArgs& a, const BkAddr
   const bool w1 = live && e < nlow && v != 0.0;
   const bool w2 = w1 && e > 0 && my_off <= t;
   const bool w3 = !live && e == 0;  // past the end: isolated unit diagonal
-  R* dummy = reinterpret_cast<R*>(SH_PART(sh)) + lane;
   constexpr int W = bk_row<R>();
+  // Both stores go to ms.nrow at a selected index, a lane with nothing to
+  // place writing its own spare slot past the rows: one address select and
+  // one LDS write each, where pointers into two arrays became exec-mask
+  // branches around every write.
+  const int spare = 4 * 80 + lane;
   // slot of column jn - off (jbase = s0 mod 64)
-  const int i1 = w1 ? t * W + ((bk_opaque(jbase) + t - my_off) & (16 * NT - 1)) : t * W + s0 + t;
-  R* p1 = (w1 || w3) ? &ms.nrow[i1] : dummy;
-  *p1 = w1 ? (R)v : (R)1;
-  R* p2 = w2 ? &ms.nrow[max(t - my_off, 0) * W + s0 + t] : dummy + 64;
-  *p2 = (R)v;
+  const int i1 = w1 ? t * W + ((bk_opaque(jbase) + t - my_off) & (16 * NT - 1))
+                    : (w3 ? t * W + s0 + t : spare);
+  ms.nrow[i1] = w1 ? (R)v : (R)1;
+  ms.nrow[w2 ? max(t - my_off, 0) * W + s0 + t : spare] = (R)v;
 }
 
 template <typename R, int DIM, int NT, int T, int G>
@@ -2153,7 +2156,7 @@ __device__ __forceinline__ void bk_backward(const BandArgs& a, int cell, BandSme
     if (q < 0) return;
 #pragma unroll
     for (int u = 0; u < 4; ++u)  // [k = u][o = lane]; offsets beyond the window are zero
-      f.pp[u] = (lane < 16 * NT) ? Pc[(size_t)q * 256 + u * 64 + lane] : (R)0;
+      f.pp[u] = (NT == 4 || lane < 16 * NT) ? Pc[(size_t)q * 256 + u * 64 + lane] : (R)0;
     f.zp = (lane < 32) ? Zc[(size_t)q * 32 + lane] : (R)0;
     const int lim = (n - 4 * q) * kRowStride;  // valid doubles of the 4 rows
     if (MODE == BK_REFINE) {  // the 4 whole rows (the lifting right-hand sides too)
@@ -2468,7 +2471,7 @@ __device__ __forceinline__ void rf_load(const BandArgs& a, int cell, int q, RfPr
   const R* Pn = reinterpret_cast<const R*>(a.pan) + ((size_t)cell * a.nq + q) * 256;
 #pragma unroll
   for (int u = 0; u < 4; ++u)  // [k = u][o = lane]
-    f.pp[u] = (lane < 16 * NT) ? Pn[u * 64 + lane] : (R)0;
+    f.pp[u] = (NT == 4 || lane < 16 * NT) ? Pn[u * 64 + lane] : (R)0;
   const int jn = 4 * q + 16 * NT + lh;
   const R rv = (R)a.Rres[((size_t)cell * n + min(jn, n - 1)) * kMaxNb + (ll & 7)];
   f.rq = (ll < 8 && jn < n) ? rv : (R)0;
@@ -3216,11 +3219,34 @@ __device__ __forceinline__ void cg_line_apply(const double* cf, const double* rf
       for (int iy = 0; iy < 2; ++iy)
         c[iy + 2 * iz] = src[X + N * (cy - 1 + iy) + (DIM == 3 ? N * N * (cz - 1 + iz) : 0)];
   };
+  // K[n][n+e] = ks[mask(e)] * (sum of the coefficients of the cells shared
+  // by n and n+e): the y-z sums P of each x-plane of cells, then x.  The
+  // sums of the plane behind a node are those of the plane ahead of the
+  // node before it, so each node forms one plane's sums.
+  auto ysums = [&](const double (&c)[NC], double (&P)[NP]) {
+#pragma unroll
+    for (int ez = 0; ez < (DIM == 3 ? 3 : 1); ++ez)
+#pragma unroll
+      for (int ey = 0; ey < 3; ++ey) {
+        double v = 0.0;
+#pragma unroll
+        for (int iz = 0; iz < (DIM == 3 ? 2 : 1); ++iz)
+#pragma unroll
+          for (int iy = 0; iy < 2; ++iy) {
+            // cell (y - 1 + iy) is shared with neighbour ey - 1 iff iy == ey / 2 or ey == 1
+            const bool y_ok = (ey == 1) || (iy == ey / 2);
+            const bool z_ok = (DIM == 2) || (ez == 1) || (iz == ez / 2);
+            if (y_ok && z_ok) v += c[iy + 2 * iz];
+          }
+        P[ey + 3 * ez] = v;
+      }
+  };
   const bool crf = REAC && has_rf;
-  double w0[NP], w1[NP], w2[NP], c0[NC], c1[NC], e0[NC], e1[NC];
+  double w0[NP], w1[NP], w2[NP], c0[NC], c1[NC], e0[NC], e1[NC], P0[NP], P1[NP];
   plane(x0 - 1, w0);
   plane(x0, w1);
   cplane(cf, x0 - 1, c0);
+  if (!REAC) ysums(c0, P0);
 #pragma unroll
   for (int t = 0; t < NC; ++t) e0[t] = 0.0;
   if (crf) cplane(rf, x0 - 1, e0);
@@ -3235,28 +3261,6 @@ __device__ __forceinline__ void cg_line_apply(const double* cf, const double* rf
       if (crf) cplane(rf, X, e1);
       double acc = 0.0;
       if (!REAC) {
-        // K[n][n+e] = ks[mask(e)] * (sum of the coefficients of the cells
-        // shared by n and n+e): y-z sums of each cell plane, then x
-        double P0[NP], P1[NP];
-        auto ysums = [&](const double (&c)[NC], double (&P)[NP]) {
-#pragma unroll
-          for (int ez = 0; ez < (DIM == 3 ? 3 : 1); ++ez)
-#pragma unroll
-            for (int ey = 0; ey < 3; ++ey) {
-              double v = 0.0;
-#pragma unroll
-              for (int iz = 0; iz < (DIM == 3 ? 2 : 1); ++iz)
-#pragma unroll
-                for (int iy = 0; iy < 2; ++iy) {
-                  // cell (y - 1 + iy) is shared with neighbour ey - 1 iff iy == ey / 2 or ey == 1
-                  const bool y_ok = (ey == 1) || (iy == ey / 2);
-                  const bool z_ok = (DIM == 2) || (ez == 1) || (iz == ez / 2);
-                  if (y_ok && z_ok) v += c[iy + 2 * iz];
-                }
-              P[ey + 3 * ez] = v;
-            }
-        };
-        ysums(c0, P0);
         ysums(c1, P1);
         double am[8];
 #pragma unroll
@@ -3299,6 +3303,8 @@ __device__ __forceinline__ void cg_line_apply(const double* cf, const double* rf
         c0[t] = c1[t];
         e0[t] = e1[t];
       }
+#pragma unroll
+      for (int t = 0; t < NP; ++t) P0[t] = P1[t];
     }
   }
 }
@@ -3371,19 +3377,27 @@ __global__ __launch_bounds__(kCgThreads) void q_cg_kernel(BandArgs a, CgArgs g) 
   const int x0 = 1 + sx * L, cy = 1 + line % ni, cz = (DIM == 3) ? 1 + line / ni : 0;
   const int jn0 = (x0 - 1) + ni * ((cy - 1) + (DIM == 3 ? ni * (cz - 1) : 0));
   const int node0 = x0 + a.S * cy + (DIM == 3 ? a.S * a.S * cz : 0);
-  // owned rows: right-hand side, inverse diagonal, iterate, residual, direction
+  // owned rows: right-hand side, inverse diagonal, iterate, residual, direction.
+  // The iteration starts from the coarse Q1 function of basis j at the
+  // interior nodes, which is the solution for a constant coefficient: only
+  // the coefficient's variation is left for CG to resolve.
   double b[kCgOwn], dinv[kCgOwn], x[kCgOwn], r[kCgOwn], p[kCgOwn];
-  double bb = 0.0, rz = 0.0;
+  double bb = 0.0, rz = 0.0, rr = 0.0, bad = 0.0;
+  double phi_yz = 1.0;  // the factors of phi_j in the other dimensions
+  {
+    const double invN = 1.0 / (double)a.N;
+    const double ty = cy * invN, tz = cz * invN;
+    phi_yz = ((j >> 1) & 1) ? ty : 1.0 - ty;
+    if (DIM == 3) phi_yz *= ((j >> 2) & 1) ? tz : 1.0 - tz;
 #pragma unroll
-  for (int t = 0; t < kCgOwn; ++t) {
-    const bool ok = t < len;
-    b[t] = ok ? rows[(size_t)(jn0 + t) * kRowStride + 16 + j] : 0.0;
-    dinv[t] = ok ? 1.0 / rows[(size_t)(jn0 + t) * kRowStride] : 0.0;
-    x[t] = 0.0;
-    r[t] = b[t];
-    p[t] = dinv[t] * r[t];
-    bb = fma(b[t], b[t], bb);
-    rz = fma(r[t], p[t], rz);
+    for (int t = 0; t < kCgOwn; ++t) {
+      const bool ok = t < len;
+      const double tx = (x0 + t) * invN;
+      b[t] = ok ? rows[(size_t)(jn0 + t) * kRowStride + 16 + j] : 0.0;
+      dinv[t] = ok ? 1.0 / rows[(size_t)(jn0 + t) * kRowStride] : 0.0;
+      x[t] = ok ? ((j & 1) ? tx : 1.0 - tx) * phi_yz : 0.0;
+      if (ok && !(dinv[t] > 0.0)) bad = 1.0;  // a diagonal entry <= 0: not SPD
+    }
   }
   __syncthreads();  // coefficients, box tables and the zeroed vector
   double kr[8], mr[8];
@@ -3394,12 +3408,31 @@ __global__ __launch_bounds__(kCgThreads) void q_cg_kernel(BandArgs a, CgArgs g) 
   }
 #pragma unroll
   for (int t = 0; t < kCgOwn; ++t)
+    if (t < len) pv[node0 + t] = x[t];
+  __syncthreads();
+  {
+    double q[kCgOwn];
+    cg_line_apply<DIM, REAC>(cf, rf, has_rf, rc, kr, mr, x0, cy, cz, len, a.N, a.S, pv, q);
+#pragma unroll
+    for (int t = 0; t < kCgOwn; ++t) {
+      r[t] = (t < len) ? b[t] - q[t] : 0.0;
+      p[t] = dinv[t] * r[t];
+      bb = fma(b[t], b[t], bb);
+      rr = fma(r[t], r[t], rr);
+      rz = fma(r[t], p[t], rz);
+    }
+  }
+  __syncthreads();  // every thread is done reading the initial iterate
+#pragma unroll
+  for (int t = 0; t < kCgOwn; ++t)
     if (t < len) pv[node0 + t] = p[t];
-  cg_block_sum2(bb, rz, red[1]);  // also: p is in LDS
-  double rr = bb;
+  // red[0] first: the loop's first sum writes red[0] again, which the
+  // barrier of the red[1] sum orders after every read of this one
+  cg_block_sum2(rr, bad, red[0]);  // also: p is in LDS
+  cg_block_sum2(bb, rz, red[1]);
   const double stop = g.tol * g.tol * bb;
-  int it = 0, state = 0;
-  while (rr > stop) {
+  int it = 0, state = (bad > 0.0) ? 2 : 0;
+  while (state == 0 && rr > stop) {
     if (it >= g.max_iter) {
       state = 1;
       break;
