@@ -3,10 +3,11 @@ oracle's direct solve, and the uneven batch that routes each refinement
 level to its solver.
 
 CG stops at ||r|| <= 1e-13 ||b||.  With the Jacobi-scaled condition numbers
-of these cells (up to ~1e4 with the 1e4 inclusions) the solution error is at
-most ~1e-9 relative, and the element matrix, formed with the energy form of
-the Schur term, is second order in it: TOL_PSI = 1e-9, TOL_M = 1e-10
-(relative, max over cells of the norm ratio).
+of these cells (up to ~1e4 with the 1e4 inclusions) that bounds the solution
+error by ~1e-9 relative; measured, it is at most 3.3e-12 on every case here
+(printed by each test), so Psi is held to the metric's TOL_PSI = 1e-10.  The
+element matrix, formed with the energy form of the Schur term, is second
+order in it: TOL_M = 1e-10 (relative, max over cells of the norm ratio).
 """
 import numpy as np
 import pytest
@@ -16,7 +17,7 @@ from test_gpu_parity import random_boxes, rel, run_gpu, run_oracle
 pytestmark = pytest.mark.gpu
 
 CG = 3
-TOL_PSI = 1e-9
+TOL_PSI = 1e-10
 TOL_M = 1e-10
 
 
@@ -48,6 +49,7 @@ def test_cg_reaction_and_cell_load(oracle_mod):
     kw = dict(f=f, rhs_mode=1, reac=reac, reaction_mode=2)
     g = run_gpu(verts, dim, l, coef, solver=CG, **kw)
     o = run_oracle(oracle_mod, verts, dim, l, coef, **kw)
+    print("errors", {k: rel(g[k], o[k]) for k in ("M", "b", "Psi")}, flush=True)
     assert (g["status"] == 0).all()
     assert rel(g["M"], o["M"]) < TOL_M
     assert rel(g["b"], o["b"]) < TOL_M
@@ -139,6 +141,8 @@ def test_cg_tensor_qp_and_qp_load(oracle_mod, dim, l):
                   reaction_const=reaction)
         g = run_gpu(verts, dim, l, T, solver=CG, **kw)
         o = run_oracle(oracle_mod, verts, dim, l, T, **kw)
+        print("reaction", reaction, "errors", {k: rel(g[k], o[k]) for k in ("M", "b", "Psi")},
+              flush=True)
         assert (g["status"] == 0).all(), g["status"]
         assert rel(g["M"], o["M"]) < TOL_M, reaction
         assert rel(g["b"], o["b"]) < TOL_M, reaction
