@@ -3006,7 +3006,7 @@ __global__ void coarse_assemble_kernel(int dim, int nc, const double* __restrict
     } else if (bnd && e == (nst - 1) / 2) {
       s = 1.0;  // constrained node: unit row, zero right-hand side
     }
-    K[(size_t)node * nst + e] = s;
+    K[(size_t)e * nnode + node] = s;  // [entry][node]: coalesced across nodes
   }
   if (!bnd) {
     for (int al = 0; al < nv; ++al) {
@@ -3062,7 +3062,7 @@ __global__ __launch_bounds__(1024) void coarse_cg_kernel(int dim, int nc, const 
   while (rnorm2 > stop && it < max_iter) {
     double rz = 0.0;
     for (int i = threadIdx.x; i < nnode; i += blockDim.x) {
-      const double zi = r[i] / K[(size_t)i * nst + center];
+      const double zi = r[i] / K[(size_t)center * nnode + i];
       z[i] = zi;
       rz += r[i] * zi;
     }
@@ -3090,7 +3090,7 @@ __global__ __launch_bounds__(1024) void coarse_cg_kernel(int dim, int nc, const 
           off += dd * mul;
           mul *= S;
         }
-        if (ok) s = fma(K[(size_t)i * nst + e], pv[i + off], s);
+        if (ok) s = fma(K[(size_t)e * nnode + i], pv[i + off], s);
       }
       qv[i] = s;
       pq += pv[i] * s;
@@ -3111,6 +3111,165 @@ __global__ __launch_bounds__(1024) void coarse_cg_kernel(int dim, int nc, const 
     *iters_out = (rnorm2 > stop) ? -it : it;
     *res_out = sqrt(rnorm2);
   }
+}
+
+// Block sums of two values over the 16 waves of a 1024-thread workgroup;
+// every thread gets both.  One barrier: a caller reuses a buffer only after a
+// later barrier (two buffers in turn).
+__device__ __forceinline__ void coarse_block_sum2(double& x, double& y, double (*red)[2]) {
+  for (int o = 32; o > 0; o >>= 1) {
+    x += __shfl_xor(x, o);
+    y += __shfl_xor(y, o);
+  }
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    red[w][0] = x;
+    red[w][1] = y;
+  }
+  __syncthreads();
+  double sx = 0.0, sy = 0.0;
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    sx += red[i][0];
+    sy += red[i][1];
+  }
+  x = sx;
+  y = sy;
+}
+
+constexpr int kCoarseThreads = 1024;
+
+// LDS bytes of coarse_pcg_kernel: the search direction over all nodes and the
+// node ids of the interior nodes
+inline size_t coarse_pcg_lds(int dim, int nc) {
+  size_t nnode = 1, nint = 1;
+  for (int d = 0; d < dim; ++d) {
+    nnode *= (size_t)(nc + 1);
+    nint *= (size_t)(nc - 1);
+  }
+  return nnode * sizeof(double) + nint * sizeof(int);
+}
+
+// Jacobi-PCG on the interior coarse nodes.  The boundary nodes carry unit
+// rows and zero right-hand sides and no interior row couples to them, so
+// their values stay 0 and only the interior is iterated.  One workgroup;
+// thread t owns the interior nodes t, t + 1024, ...  The search direction
+// over all nodes (zero on the boundary) and the node ids of the interior
+// nodes sit in LDS; K is read in its [entry][node] layout, so the loads of
+// one stencil entry by a wave are coalesced; the neighbour offsets are
+// formed once.  u, r, z, q and the inverse diagonal (in pv) stay in global
+// memory, indexed by node.  Two block sums per iteration.
+template <int DIM>
+__global__ __launch_bounds__(kCoarseThreads) void coarse_pcg_kernel(
+    int nc, const double* __restrict__ K, const double* __restrict__ rhs, double* __restrict__ u,
+    double* __restrict__ r, double* __restrict__ z, double* __restrict__ dinv,
+    double* __restrict__ qv, double abs_tol, double rel_tol, int max_iter, int* iters_out,
+    double* res_out) {
+  constexpr int NST = (DIM == 3) ? 27 : 9, CEN = (NST - 1) / 2;
+  extern __shared__ double dyn[];
+  __shared__ double red[2][16][2];
+  const int S = nc + 1, ni = nc - 1;
+  const int nnode = (DIM == 3) ? S * S * S : S * S;
+  const int nint = (DIM == 3) ? ni * ni * ni : ni * ni;
+  double* P = dyn;
+  int* nid = reinterpret_cast<int*>(dyn + nnode);
+  const int tid = threadIdx.x;
+  for (int i = tid; i < nnode; i += kCoarseThreads) {
+    P[i] = 0.0;
+    u[i] = 0.0;
+  }
+  for (int j = tid; j < nint; j += kCoarseThreads) {
+    const int jx = j % ni, jr = j / ni;
+    const int jy = (DIM == 3) ? jr % ni : jr, jz = (DIM == 3) ? jr / ni : 0;
+    nid[j] = (jx + 1) + S * (jy + 1) + (DIM == 3 ? S * S * (jz + 1) : 0);
+  }
+  int off[NST];
+#pragma unroll
+  for (int e = 0; e < NST; ++e)
+    off[e] = (e % 3 - 1) + S * ((e / 3) % 3 - 1) + (DIM == 3 ? S * S * (e / 9 - 1) : 0);
+  __syncthreads();  // P zeroed, node ids
+  // u = 0: r = rhs, p = z = D^-1 r
+  double bb = 0.0, rz = 0.0;
+  for (int j = tid; j < nint; j += kCoarseThreads) {
+    const int i = nid[j];
+    const double ri = rhs[i], di = 1.0 / K[(size_t)CEN * nnode + i];
+    const double zi = di * ri;
+    dinv[i] = di;
+    r[i] = ri;
+    P[i] = zi;
+    bb = fma(ri, ri, bb);
+    rz = fma(ri, zi, rz);
+  }
+  coarse_block_sum2(bb, rz, red[0]);  // also: p is in LDS
+  const double stop = fmax(abs_tol * abs_tol, rel_tol * rel_tol * bb);
+  double rr = bb;
+  int it = 0;
+  while (rr > stop && it < max_iter) {
+    double pq = 0.0, dummy = 0.0;
+    for (int j = tid; j < nint; j += kCoarseThreads) {
+      const int i = nid[j];
+      double q = 0.0;
+#pragma unroll
+      for (int e = 0; e < NST; ++e) q = fma(K[(size_t)e * nnode + i], P[i + off[e]], q);
+      qv[i] = q;
+      pq = fma(P[i], q, pq);
+    }
+    coarse_block_sum2(pq, dummy, red[1]);
+    if (!(pq > 0.0)) break;  // not SPD: reported as not converged
+    const double alpha = rz / pq;
+    double rz1 = 0.0, rr1 = 0.0;
+    for (int j = tid; j < nint; j += kCoarseThreads) {
+      const int i = nid[j];
+      u[i] = fma(alpha, P[i], u[i]);
+      const double ri = fma(-alpha, qv[i], r[i]);
+      const double zi = dinv[i] * ri;
+      r[i] = ri;
+      z[i] = zi;
+      rz1 = fma(ri, zi, rz1);
+      rr1 = fma(ri, ri, rr1);
+    }
+    coarse_block_sum2(rz1, rr1, red[0]);  // also: every thread is done reading p
+    const double beta = rz1 / rz;
+    rz = rz1;
+    rr = rr1;
+    for (int j = tid; j < nint; j += kCoarseThreads) {
+      const int i = nid[j];
+      P[i] = fma(beta, P[i], z[i]);
+    }
+    __syncthreads();
+    ++it;
+  }
+  if (tid == 0) {
+    *iters_out = (rr > stop) ? -it : it;
+    *res_out = sqrt(rr);
+  }
+}
+
+// The coarse Jacobi-PCG: coarse_pcg_kernel where its LDS holds the grid,
+// the general one-workgroup kernel beyond (its vectors in global memory).
+// Buffers as coarse_cg_kernel's (pv holds the inverse diagonal in the former).
+hipError_t launch_coarse_cg(int dim, int nc, const double* K, const double* rhs, double* u,
+                            double* r, double* z, double* pv, double* qv, double abs_tol,
+                            double rel_tol, int max_iter, int* iters, double* res,
+                            hipStream_t s) {
+  const size_t lds = coarse_pcg_lds(dim, nc);
+  if (nc >= 2 && lds <= 150 * 1024) {
+    const void* fn = (dim == 3) ? (const void*)coarse_pcg_kernel<3> : (const void*)coarse_pcg_kernel<2>;
+    if (lds > 48 * 1024) {
+      const hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+      if (e != hipSuccess) return e;
+    }
+    if (dim == 3)
+      hipLaunchKernelGGL(coarse_pcg_kernel<3>, dim3(1), dim3(kCoarseThreads), lds, s, nc, K, rhs, u,
+                         r, z, pv, qv, abs_tol, rel_tol, max_iter, iters, res);
+    else
+      hipLaunchKernelGGL(coarse_pcg_kernel<2>, dim3(1), dim3(kCoarseThreads), lds, s, nc, K, rhs, u,
+                         r, z, pv, qv, abs_tol, rel_tol, max_iter, iters, res);
+  } else {
+    hipLaunchKernelGGL(coarse_cg_kernel, dim3(1), dim3(1024), 0, s, dim, nc, K, rhs, u, r, z, pv,
+                       qv, abs_tol, rel_tol, max_iter, iters, res);
+  }
+  return hipGetLastError();
 }
 
 __global__ void coarse_weights_kernel(int dim, int nc, const double* __restrict__ u,
@@ -3981,7 +4140,7 @@ hipError_t band_ldl_mixed_solve(int n_cells, int n, int bw, const double* rows, 
 
 // Coarse assembly, Jacobi-PCG and the per-cell weights on the device for the
 // element data of the nc^dim structured coarse grid (cells lexicographic):
-// dM [cells][2^dim][2^dim], db [cells][2^dim].  Buffers: cK [nodes][3^dim],
+// dM [cells][2^dim][2^dim], db [cells][2^dim].  Buffers: cK [3^dim][nodes],
 // crhs [nodes], cvec [5 nodes + 1 + cells 2^dim] (u | r | z | p | q |
 // final residual | weights), cit [1].  Enqueued on s.
 hipError_t coarse_solve_device(int dim, int nc, const double* dM, const double* db, double* cK,
@@ -4003,9 +4162,7 @@ hipError_t coarse_solve_device(int dim, int nc, const double* dM, const double* 
                      db, cK, crhs);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(coarse_cg_kernel, dim3(1), dim3(1024), 0, s, dim, nc, cK, crhs, du, r, z, pv,
-                     qv, abs_tol, rel_tol, max_iter, cit, res);
-  e = hipGetLastError();
+  e = launch_coarse_cg(dim, nc, cK, crhs, du, r, z, pv, qv, abs_tol, rel_tol, max_iter, cit, res, s);
   if (e != hipSuccess) return e;
   const int nw = ncell << dim;
   hipLaunchKernelGGL(coarse_weights_kernel, dim3((nw + 255) / 256), dim3(256), 0, s, dim, nc, du, dw);
@@ -4912,9 +5069,8 @@ msfec_status msfec_q_adjoint_gradient(msfec_q_handle h, int nc, const double* u_
   hipLaunchKernelGGL(coarse_assemble_kernel, dim3((nnode + 255) / 256), dim3(256), 0, s, dim, nc,
                      h->d_M, h->d_b, h->d_cK, h->d_crhs);
   ADJ_HIP(hipGetLastError());
-  hipLaunchKernelGGL(coarse_cg_kernel, dim3(1), dim3(1024), 0, s, dim, nc, h->d_cK, h->d_crhs, du,
-                     r, z, pv, qv, 0.0, rel_tol, max_iter, h->d_cit, res);
-  ADJ_HIP(hipGetLastError());
+  ADJ_HIP(launch_coarse_cg(dim, nc, h->d_cK, h->d_crhs, du, r, z, pv, qv, 0.0, rel_tol, max_iter,
+                           h->d_cit, res, s));
   int it = 0;
   ADJ_HIP(hipMemcpyAsync(&it, h->d_cit, sizeof(int), hipMemcpyDeviceToHost, s));
   std::vector<double> hu(nnode), ht(nnode), hd(nnode);
@@ -4944,9 +5100,8 @@ msfec_status msfec_q_adjoint_gradient(msfec_q_handle h, int nc, const double* u_
   }
   *J = jsum;
   ADJ_HIP(hipMemcpyAsync(h->d_crhs, hd.data(), nnode * sizeof(double), hipMemcpyHostToDevice, s));
-  hipLaunchKernelGGL(coarse_cg_kernel, dim3(1), dim3(1024), 0, s, dim, nc, h->d_cK, h->d_crhs, dv,
-                     r, z, pv, qv, 0.0, rel_tol, max_iter, h->d_cit, res);
-  ADJ_HIP(hipGetLastError());
+  ADJ_HIP(launch_coarse_cg(dim, nc, h->d_cK, h->d_crhs, dv, r, z, pv, qv, 0.0, rel_tol, max_iter,
+                           h->d_cit, res, s));
   // w = K_II^{-1} F_I with the stored panels
   BandArgs a = band_args(h);
   a.Rres = h->d_rres;

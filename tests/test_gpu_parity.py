@@ -240,6 +240,31 @@ def test_coarse_solve_and_reconstruct(oracle_mod, dim, nc, l, reaction):
     assert np.abs(uf - ufo).max() <= 1e-10 * np.abs(ufo).max()
 
 
+@pytest.mark.parametrize("dim,nc,l", [(3, 16, 1), (3, 24, 1), (2, 128, 1)])
+def test_coarse_solve_grid_sizes(oracle_mod, dim, nc, l):
+    """The coarse Jacobi-PCG on grids whose search direction fits the
+    workgroup's LDS (3-D 16^3: the adjoint and uneven configurations' coarse
+    grid) and on grids beyond it (3-D 24^3, 2-D 128^2: the general kernel),
+    against the oracle's coarse solve of the same M and b; a = exp(g), g iid
+    N(0, 1.5^2) per fine cell."""
+    print(f"case coarse sizes dim={dim} nc={nc} l={l}", flush=True)
+    from msfec_amd import QBasisBatch, fields as F
+    verts = F.coarse_vertices(dim, nc)
+    coef = F.lognormal_coefficient(dim, nc, l, 1.5, 0.0, seed=31 + nc)
+    with QBasisBatch(verts, dim=dim, n_refine_local=l) as qb:
+        qb.set_coefficient(coef)
+        qb.run()
+        M, b = qb.get_global_element_matrix(), qb.get_global_element_rhs()
+        u, w, it, res = qb.coarse_solve(nc, abs_tol=0.0, rel_tol=1e-13)
+    uo, ito = oracle_mod.coarse_solve(dim, nc, M, b, rtol=1e-14)
+    print("iterations", it, "oracle", ito, "difference", np.abs(u - uo).max() / np.abs(uo).max(),
+          flush=True)
+    assert it > 0 and ito > 0
+    assert np.abs(u - uo).max() <= 1e-10 * np.abs(uo).max()
+    # the weights are the coarse values at each cell's vertices
+    assert w.shape == (nc ** dim, 1 << dim) and np.isfinite(w).all()
+
+
 @pytest.mark.parametrize("solver", SOLVERS)
 @pytest.mark.parametrize("dim,l", [(2, 4), (3, 3)])
 def test_fused_path_rhs_cell_and_reaction(oracle_mod, dim, l, solver):
