@@ -2967,8 +2967,10 @@ __global__ void coarse_assemble_kernel(int dim, int nc, const double* __restrict
   const int nst = (dim == 2) ? 9 : 27, nv = 1 << dim;
   int nnode = 1;
   for (int d = 0; d < dim; ++d) nnode *= nc + 1;
-  const int node = blockIdx.x * blockDim.x + threadIdx.x;
-  if (node >= nnode) return;
+  // one thread per (stencil entry, node), nodes fastest: coalesced stores
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)nnode * nst) return;
+  const int e = (int)(t / nnode), node = (int)(t - (long long)e * nnode);
   int cn[3] = {0, 0, 0}, r = node;
   bool bnd = false;
   for (int d = 0; d < dim; ++d) {
@@ -2977,7 +2979,7 @@ __global__ void coarse_assemble_kernel(int dim, int nc, const double* __restrict
     if (cn[d] == 0 || cn[d] == nc) bnd = true;
   }
   double rs = 0.0;
-  for (int e = 0; e < nst; ++e) {
+  {
     int dd[3] = {0, 0, 0}, rr = e;
     bool nbnd = false, inside = true;
     for (int d = 0; d < dim; ++d) {
@@ -3006,9 +3008,9 @@ __global__ void coarse_assemble_kernel(int dim, int nc, const double* __restrict
     } else if (bnd && e == (nst - 1) / 2) {
       s = 1.0;  // constrained node: unit row, zero right-hand side
     }
-    K[(size_t)e * nnode + node] = s;  // [entry][node]: coalesced across nodes
+    K[(size_t)e * nnode + node] = s;  // [entry][node]
   }
-  if (!bnd) {
+  if (e == 0 && !bnd) {
     for (int al = 0; al < nv; ++al) {
       bool ok = true;
       int z = 0, mul = 1;
@@ -3021,7 +3023,7 @@ __global__ void coarse_assemble_kernel(int dim, int nc, const double* __restrict
       if (ok) rs += b[(size_t)z * nv + al];
     }
   }
-  rhs[node] = rs;
+  if (e == 0) rhs[node] = rs;
 }
 
 __device__ double block_sum(double v, double* red) {
@@ -4158,7 +4160,8 @@ hipError_t coarse_solve_device(int dim, int nc, const double* dM, const double* 
   double* qv = pv + nnode;
   double* res = qv + nnode;
   double* dw = res + 1;
-  hipLaunchKernelGGL(coarse_assemble_kernel, dim3((nnode + 255) / 256), dim3(256), 0, s, dim, nc, dM,
+  hipLaunchKernelGGL(coarse_assemble_kernel, dim3((nnode * (dim == 3 ? 27 : 9) + 255) / 256),
+                     dim3(256), 0, s, dim, nc, dM,
                      db, cK, crhs);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
@@ -5066,7 +5069,8 @@ msfec_status msfec_q_adjoint_gradient(msfec_q_handle h, int nc, const double* u_
     }                                                 \
   } while (0)
   // forward coarse problem
-  hipLaunchKernelGGL(coarse_assemble_kernel, dim3((nnode + 255) / 256), dim3(256), 0, s, dim, nc,
+  hipLaunchKernelGGL(coarse_assemble_kernel, dim3((nnode * (dim == 3 ? 27 : 9) + 255) / 256),
+                     dim3(256), 0, s, dim, nc,
                      h->d_M, h->d_b, h->d_cK, h->d_crhs);
   ADJ_HIP(hipGetLastError());
   ADJ_HIP(launch_coarse_cg(dim, nc, h->d_cK, h->d_crhs, du, r, z, pv, qv, 0.0, rel_tol, max_iter,

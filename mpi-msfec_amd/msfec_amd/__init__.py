@@ -7,9 +7,10 @@ seeded synthetic inputs.
 """
 from . import _lib
 from ._lib import MsfecError
+from .device import DeviceBuffer
 from .ned import NedBasisBatch
 from .qbasis import QBasisBatch
 from .rt import RTDQBasisBatch
 from .uneven import UnevenBatch
 
-__all__ = ["QBasisBatch", "NedBasisBatch", "RTDQBasisBatch", "UnevenBatch", "MsfecError", "_lib"]
+__all__ = ["DeviceBuffer", "QBasisBatch", "NedBasisBatch", "RTDQBasisBatch", "UnevenBatch", "MsfecError", "_lib"]

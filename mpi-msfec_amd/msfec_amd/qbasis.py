@@ -35,6 +35,11 @@ def _out_ptr(a, shape):
     array (or CPU torch tensor) of the expected size, a device tensor a
     contiguous float64 one; anything else is an error, not a silent copy."""
     n = int(np.prod(shape))
+    from .device import DeviceBuffer
+    if isinstance(a, DeviceBuffer):
+        if a.nbytes != 8 * n:
+            raise ValueError(f"out must hold {n} float64 values")
+        return a.ptr, L.DEVICE, a
     if hasattr(a, "data_ptr") and hasattr(a, "is_cuda"):
         import torch
         if a.dtype != torch.float64 or not a.is_contiguous() or a.numel() != n:
@@ -141,17 +146,21 @@ class QBasisBatch:
                                              ctypes.byref(it), ctypes.byref(res)))
         return u, w, it.value, res.value
 
-    def adjoint_gradient(self, nc, u_target, rel_tol=1e-14, max_iter=100000):
+    def adjoint_gradient(self, nc, u_target, rel_tol=1e-14, max_iter=100000, out=None):
         """(J, dJ/dg) for J = 1/2 |u_H - u_target|^2 over the free coarse nodes
         and the log-coefficient g of every fine cell (a = exp(g)); the batch
         must have been computed (run()) with solver BAND_FP64_MFMA or
-        BAND_MIXED on the nc^dim structured grid of coarse_solve."""
+        BAND_MIXED on the nc^dim structured grid of coarse_solve.  ``out``
+        (n_cells x fine cells float64: a host array, a DeviceBuffer or a
+        device tensor) receives the gradient in place."""
         ut = np.ascontiguousarray(u_target, dtype=np.float64)
-        grad = np.empty((self.n_cells, self.info.n_fine_cells))
+        shape = (self.n_cells, self.info.n_fine_cells)
+        grad = np.empty(shape) if out is None else out
+        gptr, gloc, keep = _out_ptr(grad, shape)
         J = ctypes.c_double(0.0)
         L.check(self._lib.msfec_q_adjoint_gradient(
             self._h, nc, ctypes.c_void_p(ut.ctypes.data), L.HOST, rel_tol, max_iter,
-            ctypes.byref(J), ctypes.c_void_p(grad.ctypes.data), L.HOST))
+            ctypes.byref(J), gptr, gloc))
         return J.value, grad
 
     def cg_iterations(self):

@@ -59,6 +59,27 @@ def test_adjoint_matches_oracle(oracle_mod, dim, nc, l, solver):
     assert np.linalg.norm(grad - go) <= tol * np.linalg.norm(go)
 
 
+def test_adjoint_gradient_into_device_memory():
+    """The gradient written into a DeviceBuffer (MSFEC_DEVICE) equals the
+    host-landing one bit for bit, and a wrongly sized buffer is refused."""
+    print("case adjoint_device_out", flush=True)
+    from msfec_amd import DeviceBuffer, fields as F
+    dim, nc, l = 3, 3, 3
+    verts = F.coarse_vertices(dim, nc)
+    g = np.log(F.lognormal_coefficient(dim, nc, l, 1.0, 0.0, seed=13))
+    ut = target(verts, dim, l, nc, 14)
+    with forward(verts, dim, l, nc, g, 1) as qb:
+        J, grad = qb.adjoint_gradient(nc, ut)
+        with DeviceBuffer(grad.shape) as d:
+            Jd, _ = qb.adjoint_gradient(nc, ut, out=d)
+            gd = d.numpy()
+        with DeviceBuffer(grad.size - 1) as bad:
+            with pytest.raises(ValueError):
+                qb.adjoint_gradient(nc, ut, out=bad)
+    assert Jd == J
+    np.testing.assert_array_equal(gd, grad)
+
+
 @pytest.mark.parametrize("solver", [1, 2])
 def test_adjoint_matches_finite_differences(solver):
     print(f"case adjoint_fd solver={solver}", flush=True)
