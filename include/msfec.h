@@ -361,7 +361,7 @@ typedef struct {
                          the matrix cores and refinement with FP64 residuals
                          until the predicted error |dx| / |x| times the
                          contraction (|dx| / |x| over its previous value) is
-                         below 1e-14 per cell, at most max_refine passes
+                         below 1e-13 per cell, at most max_refine passes
                          (0 = 5), the cells still above 1e-12 flagged
                          MSFEC_CELL_NOT_CONVERGED                          */
   int max_refine;

@@ -2632,25 +2632,35 @@ __global__ __launch_bounds__(256) void ldl_resid_kernel(BandArgs a, const int* _
   const double* X = a.X + (size_t)cell * n * kMaxNb;
   for (int i = threadIdx.x; i < n * kMaxNb; i += blockDim.x) xs[i] = X[i];
   __syncthreads();
-  for (int i = threadIdx.x; i < n * kMaxNb; i += blockDim.x) {
-    const int j = i >> 3, r = i & (kMaxNb - 1);
+  // one thread per row and all 8 right-hand sides: each matrix entry is
+  // loaded once and used 8 times
+  for (int j = threadIdx.x; j < n; j += blockDim.x) {
     const double* rj = rows + (size_t)j * kRowStride;
     const int* lo = a.loff + (size_t)j * 16;
     const int* up = utab + (size_t)j * 16;
-    double acc = rj[16 + r];
+    double acc[kMaxNb];
+#pragma unroll
+    for (int r = 0; r < kMaxNb; ++r) acc[r] = rj[16 + r];
 #pragma unroll
     for (int t = 0; t < 16; ++t) {
       const int o = lo[t];
-      const int jm = (o >= 0) ? j - o : j;
-      acc = fma(o >= 0 ? -rj[t] : 0.0, xs[jm * kMaxNb + r], acc);
+      const double v = (o >= 0) ? -rj[t] : 0.0;
+      const double* xr = xs + ((o >= 0) ? j - o : j) * kMaxNb;
+#pragma unroll
+      for (int r = 0; r < kMaxNb; ++r) acc[r] = fma(v, xr[r], acc[r]);
     }
 #pragma unroll
     for (int t = 0; t < 16; ++t) {
       const int u = up[t];
       const int ri = (u >= 0) ? (u >> 4) : j, ti = (u >= 0) ? (u & 15) : 0;
-      acc = fma(u >= 0 ? -rows[(size_t)ri * kRowStride + ti] : 0.0, xs[ri * kMaxNb + r], acc);
+      const double v = (u >= 0) ? -rows[(size_t)ri * kRowStride + ti] : 0.0;
+      const double* xr = xs + ri * kMaxNb;
+#pragma unroll
+      for (int r = 0; r < kMaxNb; ++r) acc[r] = fma(v, xr[r], acc[r]);
     }
-    a.Rres[((size_t)cell * n + j) * kMaxNb + r] = acc;
+    double* out = a.Rres + ((size_t)cell * n + j) * kMaxNb;
+#pragma unroll
+    for (int r = 0; r < kMaxNb; ++r) out[r] = acc[r];
   }
 }
 
@@ -2673,9 +2683,11 @@ __global__ __launch_bounds__(64, 2) void ldl_refine_kernel(BandArgs a) {
 // in the pass before (1 in the first), the error left after the step is
 // about q rho with the contraction rho = q / q' (the first pass: q itself,
 // since d then is the first solution's error).  A cell stops once that is
-// below 1e-14 of the largest unknown (the blocks of a mixed system differ in
-// scale: sigma of Q_Ned is ~1/40 of the largest entry), or after the last
-// pass, flagged MSFEC_CELL_NOT_CONVERGED above 1e-12.  One block per cell.
+// below 1e-13 of the largest unknown, or after the last pass, flagged
+// MSFEC_CELL_NOT_CONVERGED above 1e-12.  The blocks of a mixed system differ
+// in scale (sigma of Q_Ned is ~1/40 of the largest entry): 1e-13 of the
+// largest entry keeps the refinement's share of sigma's error below 1e-11 of
+// sigma, its FP64 floor.  One block per cell.
 __global__ __launch_bounds__(256) void ldl_update_kernel(BandArgs a, const double* __restrict__ dX,
                                                          double* __restrict__ qprev) {
   const int cell = blockIdx.x;
@@ -2709,7 +2721,7 @@ __global__ __launch_bounds__(256) void ldl_update_kernel(BandArgs a, const doubl
     if (!isfinite(est)) {
       a.conv[cell] = 1;
       if (a.status[cell] == MSFEC_CELL_OK) a.status[cell] = MSFEC_CELL_NONFINITE;
-    } else if (est <= 1e-14) {
+    } else if (est <= 1e-13) {
       a.conv[cell] = 1;
     } else if (a.last_pass) {
       a.conv[cell] = 1;
