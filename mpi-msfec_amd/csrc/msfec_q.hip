@@ -2641,7 +2641,9 @@ __global__ __launch_bounds__(256) void ldl_resid_kernel(BandArgs a, const int* _
     double acc[kMaxNb];
 #pragma unroll
     for (int r = 0; r < kMaxNb; ++r) acc[r] = rj[16 + r];
-#pragma unroll
+    // 4 entries in flight: unrolled over all 32 the loads of every entry and
+    // its 8 x values stay live at once and spill
+#pragma unroll 4
     for (int t = 0; t < 16; ++t) {
       const int o = lo[t];
       const double v = (o >= 0) ? -rj[t] : 0.0;
@@ -2649,7 +2651,7 @@ __global__ __launch_bounds__(256) void ldl_resid_kernel(BandArgs a, const int* _
 #pragma unroll
       for (int r = 0; r < kMaxNb; ++r) acc[r] = fma(v, xr[r], acc[r]);
     }
-#pragma unroll
+#pragma unroll 4
     for (int t = 0; t < 16; ++t) {
       const int u = up[t];
       const int ri = (u >= 0) ? (u >> 4) : j, ti = (u >= 0) ? (u & 15) : 0;

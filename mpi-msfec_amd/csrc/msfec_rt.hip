@@ -205,20 +205,21 @@ __global__ __launch_bounds__(kNThreads) void rt_rows_kernel(RtArgs a) {
             rh[3] -= uy * Y / Hx;
           }
         }
-      double out[kRowLen];
-#pragma unroll
-      for (int t = 0; t < kRowLen; ++t) out[t] = 0.0;
-      out[0] = vd;
-      int t = 1;
-      if (i > 1) out[t++] = vl;
-      if (j > 1 && i < mi) out[t++] = vbr;
-      if (j > 1) out[t++] = vb;
-      if (j > 1 && i > 1) out[t++] = vbl;
-#pragma unroll
-      for (int nb = 0; nb < kNB; ++nb) out[kNLow + nb] = rh[nb];
+      // the present couplings packed to the front of the row, written
+      // straight to the table (a register array indexed by the running
+      // count would live in scratch memory)
       double* dst = a.rows + ((size_t)cell * a.n + r) * kRowLen;
+      dst[0] = vd;
+      int t = 1;
+      if (i > 1) dst[t++] = vl;
+      if (j > 1 && i < mi) dst[t++] = vbr;
+      if (j > 1) dst[t++] = vb;
+      if (j > 1 && i > 1) dst[t++] = vbl;
+      for (; t < kNLow; ++t) dst[t] = 0.0;
 #pragma unroll
-      for (int q = 0; q < kRowLen; ++q) dst[q] = out[q];
+      for (int nb = 0; nb < kNB; ++nb) dst[kNLow + nb] = rh[nb];
+#pragma unroll
+      for (int q = kNLow + kNB; q < kRowLen; ++q) dst[q] = 0.0;
     }
     return;
   }
@@ -226,33 +227,31 @@ __global__ __launch_bounds__(kNThreads) void rt_rows_kernel(RtArgs a) {
     const int i = r % mi + 1, j = r / mi + 1;
     const double k00 = kap[(j - 1) * m + i - 1], k10 = kap[(j - 1) * m + i];
     const double k01 = kap[j * m + i - 1], k11 = kap[j * m + i];
-    double out[kRowLen];
-#pragma unroll
-    for (int t = 0; t < kRowLen; ++t) out[t] = 0.0;
-    out[0] = (k00 + k10 + k01 + k11) * (al + be) / 3.0;
+    double* dst = a.rows + ((size_t)cell * a.n + r) * kRowLen;
+    dst[0] = (k00 + k10 + k01 + k11) * (al + be) / 3.0;
     int t = 1;
-    if (i > 1) out[t++] = (k00 + k01) * (be / 6.0 - al / 3.0);          // left
-    if (j > 1 && i < mi) out[t++] = -k10 * (al + be) / 6.0;               // below-right
-    if (j > 1) out[t++] = (k00 + k10) * (al / 6.0 - be / 3.0);          // below
-    if (j > 1 && i > 1) out[t++] = -k00 * (al + be) / 6.0;                // below-left
+    if (i > 1) dst[t++] = (k00 + k01) * (be / 6.0 - al / 3.0);          // left
+    if (j > 1 && i < mi) dst[t++] = -k10 * (al + be) / 6.0;               // below-right
+    if (j > 1) dst[t++] = (k00 + k10) * (al / 6.0 - be / 3.0);          // below
+    if (j > 1 && i > 1) dst[t++] = -k00 * (al + be) / 6.0;                // below-left
+    for (; t < kNLow; ++t) dst[t] = 0.0;
     // x bases: c = (g(X), 0), int g d_y phi over the 4 cells
     {
       const double gm = (double)(i - 1) / m, g0 = (double)i / m, gp = (double)(i + 1) / m;
       const double wl = bx.hx / 6.0 * (k00 - k01), wr = bx.hx / 6.0 * (k10 - k11);
       // basis 0: g = (1 - X) / Hy; basis 1: g = X / Hy
-      out[kNLow + 0] = -(wl * ((1.0 - gm) + 2.0 * (1.0 - g0)) + wr * (2.0 * (1.0 - g0) + (1.0 - gp))) / Hy;
-      out[kNLow + 1] = -(wl * (gm + 2.0 * g0) + wr * (2.0 * g0 + gp)) / Hy;
+      dst[kNLow + 0] = -(wl * ((1.0 - gm) + 2.0 * (1.0 - g0)) + wr * (2.0 * (1.0 - g0) + (1.0 - gp))) / Hy;
+      dst[kNLow + 1] = -(wl * (gm + 2.0 * g0) + wr * (2.0 * g0 + gp)) / Hy;
     }
     // y bases: c = (0, g(Y)), int -g d_x phi over the 4 cells
     {
       const double gm = (double)(j - 1) / m, g0 = (double)j / m, gp = (double)(j + 1) / m;
       const double wb = bx.hy / 6.0 * (k10 - k00), wt = bx.hy / 6.0 * (k11 - k01);
-      out[kNLow + 2] = -(wb * ((1.0 - gm) + 2.0 * (1.0 - g0)) + wt * (2.0 * (1.0 - g0) + (1.0 - gp))) / Hx;
-      out[kNLow + 3] = -(wb * (gm + 2.0 * g0) + wt * (2.0 * g0 + gp)) / Hx;
+      dst[kNLow + 2] = -(wb * ((1.0 - gm) + 2.0 * (1.0 - g0)) + wt * (2.0 * (1.0 - g0) + (1.0 - gp))) / Hx;
+      dst[kNLow + 3] = -(wb * (gm + 2.0 * g0) + wt * (2.0 * g0 + gp)) / Hx;
     }
-    double* dst = a.rows + ((size_t)cell * a.n + r) * kRowLen;
 #pragma unroll
-    for (int q = 0; q < kRowLen; ++q) dst[q] = out[q];
+    for (int q = kNLow + kNB; q < kRowLen; ++q) dst[q] = 0.0;
   }
 }
 
