@@ -3911,10 +3911,12 @@ __global__ __launch_bounds__(kCgThreads) void q_cg_stencil_kernel(BandArgs a, Cg
 // M[i][k] = psi_i^T K psi_k, b[i] = psi_i^T F.  Second order in the error of
 // the interior solution: psi_k is K-harmonic at the interior rows and the
 // error vanishes on the boundary.  Psi is completed on the boundary with the
-// lifted coarse shape functions first; then every thread takes every 1024th
+// lifted coarse shape functions first; then every thread takes every 512th
 // node, forms (K psi_k)(node) for the 2^dim bases and accumulates the 64
-// products and the 8 loads, summed over the workgroup at the end.
-constexpr int kTfThreads = 1024;
+// products and the 8 loads, summed over the workgroup at the end.  512
+// threads: the 72 accumulators need more than the 128 registers a thread of
+// a 1024-thread workgroup may hold.
+constexpr int kTfThreads = 512;
 
 template <int DIM>
 __global__ __launch_bounds__(kTfThreads) void q_cg_tensor_finish_kernel(BandArgs a, CgArgs g,
