@@ -205,6 +205,34 @@ def test_headline_batch_invariants(oracle_mod, solver):
     assert rel(Psi[sample], o["Psi"]) < tol(solver)
 
 
+def test_batch_beyond_32bit_indices(oracle_mod):
+    """64^3 = 262 144 cells at l = 3 (~85 GB of device buffers): the panel
+    store holds 5.9e9 doubles and the row table 2.2e9, beyond 32-bit element
+    indices.  a = exp(g), g iid N(0, 1) per fine cell.  Status, symmetry,
+    M 1 = 0 and sum b = |K| on every cell; the last cells and one in the
+    middle against the oracle."""
+    print("case batch 64^3 cells l=3", flush=True)
+    from msfec_amd import QBasisBatch, fields as F
+    dim, l, nc = 3, 3, 64
+    verts = F.coarse_vertices(dim, nc)
+    n = verts.shape[0]
+    coef = np.exp(np.random.default_rng(64).standard_normal((n, 512)))
+    with QBasisBatch(verts, dim=dim, n_refine_local=l, solver=1) as qb:
+        qb.set_coefficient(coef)
+        qb.run()
+        M, b, st = qb.get_global_element_matrix(), qb.get_global_element_rhs(), qb.get_cell_status()
+    assert (st == 0).all()
+    scale = np.abs(M).max(axis=(1, 2))
+    assert np.max(np.abs(M - M.transpose(0, 2, 1)).max(axis=(1, 2)) / scale) < 1e-12
+    assert np.max(np.abs(M.sum(axis=2)).max(axis=1) / scale) < TOL
+    vol = (1.0 / nc) ** 3
+    assert np.abs(b.sum(axis=1) - vol).max() / vol < TOL
+    pick = np.array([n // 2, n - 2, n - 1])
+    o = run_oracle(oracle_mod, verts[pick], dim, l, coef[pick])
+    assert rel(M[pick], o["M"]) < TOL
+    assert rel(b[pick], o["b"]) < TOL
+
+
 @pytest.mark.parametrize("reaction", [0.0, 1.0])
 @pytest.mark.parametrize("dim,nc,l", [(2, 8, 4), (3, 4, 3)])
 def test_coarse_solve_and_reconstruct(oracle_mod, dim, nc, l, reaction):
