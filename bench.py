@@ -382,6 +382,7 @@ def uneven_config(steps: int, warmup: int, check: bool, cpu: bool):
     for l in ub.level_set:
         kt = ub.level(l).kernel_times()
         per_level[str(l)] = {"cells": summ[l]["cells"], "solver": summ[l]["solver"],
+                             "band_window": summ[l]["window"] or None,
                              "tuned_ms": summ[l]["tuned_ms"],
                              "kernel_ms": [round(t, 4) for t in kt[:4]]}
     out = {"workload": "uneven 3D Q1 batch: 16^3 cells, levels 2/3 (band or CG, timed choice) "
