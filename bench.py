@@ -131,38 +131,64 @@ def rel_err(a, b):
     return float(np.max(num / np.maximum(den, 1e-300)))
 
 
+def timed_reps(fn, reps: int):
+    """Walls of `reps` calls of fn after one untimed warm-up call."""
+    fn()
+    walls = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        out = fn()
+        walls.append(time.perf_counter() - t0)
+    return walls, out
+
+
 def cpu_baseline(verts, coef, budget_s: float = 20.0):
     """The reference algorithm on the host cores: the oracle's CG + SSOR(1.6)
     path (abs tol 1e-12, maxit 1000 -- QBasis::solve_iterative, the default of
-    every example parameter file), OpenMP over cells like mpirun ranks."""
+    every example parameter file), OpenMP over cells like mpirun ranks; one
+    warm-up, then the best and the median of 5 runs over a seeded sample of
+    about budget_s / 2 CPU-seconds.  Beside it, the direct variant (dense FP64
+    Cholesky of the interior system, the 'use direct solver basis' analogue)
+    on a smaller sample, best of 3."""
     from oracle import oracle as O
     O.build()
     cores = host_cores()
     p = O.make_problem(DIM, NREF)
     rng = np.random.default_rng(7)
     order = rng.permutation(verts.shape[0])
-    # calibrate on a few cells, then size the sample to ~budget_s CPU-seconds
-    cal = order[:cores]
-    t0 = time.perf_counter()
-    O.basis_batch(p, verts[cal], coef=coef[cal], solver=O.SOLVER_CG_SSOR, n_threads=cores,
-                  want_psi=False)
-    t_cal = time.perf_counter() - t0
-    per_cell_cpu = t_cal * cores / len(cal)
-    n_sample = int(min(verts.shape[0], max(cores, budget_s / max(per_cell_cpu, 1e-6))))
-    n_sample = max(cores, (n_sample // cores) * cores)
-    samp = order[:n_sample]
-    t0 = time.perf_counter()
-    r = O.basis_batch(p, verts[samp], coef=coef[samp], solver=O.SOLVER_CG_SSOR,
-                      n_threads=cores, want_psi=False)
-    wall = time.perf_counter() - t0
+
+    def sized(solver, budget):
+        # calibrate on a few cells, then size the sample to ~budget CPU-seconds
+        cal = order[:cores]
+        t0 = time.perf_counter()
+        O.basis_batch(p, verts[cal], coef=coef[cal], solver=solver, n_threads=cores,
+                      want_psi=False)
+        per_cell_cpu = (time.perf_counter() - t0) * cores / len(cal)
+        n = int(min(verts.shape[0], max(cores, budget / max(per_cell_cpu, 1e-6))))
+        return order[:max(cores, (n // cores) * cores)]
+
+    samp = sized(O.SOLVER_CG_SSOR, budget_s / 2)
+    walls, r = timed_reps(lambda: O.basis_batch(p, verts[samp], coef=coef[samp],
+                                                solver=O.SOLVER_CG_SSOR, n_threads=cores,
+                                                want_psi=False), 5)
+    dsamp = sized(O.SOLVER_DIRECT, budget_s / 6)
+    dwalls, _ = timed_reps(lambda: O.basis_batch(p, verts[dsamp], coef=coef[dsamp],
+                                                 solver=O.SOLVER_DIRECT, n_threads=cores,
+                                                 want_psi=False), 3)
+    n_sample = len(samp)
     return {
-        "value": n_sample / wall,
+        "value": n_sample / min(walls),
         "unit": "cells/s",
         "cores": cores,
         "kind": "port",
         "sample": f"{n_sample} of the 4096 cells (seeded), CG+SSOR(1.6) abs tol 1e-12 per "
-                  f"basis, OpenMP on {host_cores_text(cores)}, max CG iterations {int(r['iters'].max())}, "
-                  f"wall {wall:.2f} s",
+                  f"basis, OpenMP on {host_cores_text(cores)}, max CG iterations "
+                  f"{int(r['iters'].max())}; best of 5 runs after a warm-up, walls "
+                  + ", ".join(f"{w:.2f}" for w in walls) + " s",
+        "median_value": n_sample / float(np.median(walls)),
+        "direct_value": len(dsamp) / min(dwalls),
+        "direct_sample": f"{len(dsamp)} cells, dense FP64 Cholesky of K_II (the direct "
+                         f"solver), best of 3 after a warm-up",
     }
 
 
@@ -448,7 +474,7 @@ def uneven_config(steps: int, warmup: int, check: bool, cpu: bool):
     return out
 
 
-def plain_config(steps: int, warmup: int, check: bool):
+def plain_config(steps: int, warmup: int, check: bool, cpu: bool):
     """The plain-method configuration: 2-D -div(A grad u) + u = f on the unit
     square, 8 x 8 coarse cells refined 4 levels (16 x 16 fine cells, 289
     nodes, 4 bases each), A = exp(g) with g iid N(0, 1) per fine cell (seed
@@ -460,20 +486,32 @@ def plain_config(steps: int, warmup: int, check: bool):
     dim, nc, l = 2, 8, 4
     verts = F.coarse_vertices(dim, nc)
     coef = F.lognormal_coefficient(dim, nc, l, 1.0, 0.0, SEED)
-    qb = QBasisBatch(verts, dim=dim, n_refine_local=l, solver=L.SOLVER_BAND_FP64_MFMA,
+    # kernel times from a handle with HIP events between the kernels, the
+    # rates from one without them
+    qt = QBasisBatch(verts, dim=dim, n_refine_local=l, solver=L.SOLVER_BAND_FP64_MFMA,
                      reaction_mode=L.REACTION_CONST, reaction_const=1.0, timing=True)
+    qt.set_coefficient(coef)
+    for _ in range(warmup):
+        qt.run()
+    qt.synchronize()
+    if warmup > 0:
+        qt.kernel_times()
+    for _ in range(steps):
+        qt.run()
+    qt.synchronize()
+    kt = qt.kernel_times()
+    qt.close()
+    qb = QBasisBatch(verts, dim=dim, n_refine_local=l, solver=L.SOLVER_BAND_FP64_MFMA,
+                     reaction_mode=L.REACTION_CONST, reaction_const=1.0)
     qb.set_coefficient(coef)
     for _ in range(warmup):
         qb.run()
     qb.synchronize()
-    if warmup > 0:
-        qb.kernel_times()
     t0 = time.perf_counter()
     for _ in range(steps):
         qb.run()
     qb.synchronize()
     wall = time.perf_counter() - t0
-    kt = qb.kernel_times()
     t0 = time.perf_counter()
     for _ in range(steps):
         qb.run()
@@ -499,6 +537,18 @@ def plain_config(steps: int, warmup: int, check: bool):
                                          "Psi": rel_err(P, o["Psi"]),
                                          "u_coarse": float(np.abs(u - uo).max() / np.abs(uo).max()),
                                          "cells_checked": 64}
+    if cpu:
+        from oracle import oracle as O
+        O.build()
+        cores = host_cores()
+        pp = O.make_problem(dim, l, reaction_mode=O.REACTION_CONST, reaction_const=1.0)
+        walls, r = timed_reps(lambda: O.basis_batch(pp, verts, coef=coef, solver=O.SOLVER_CG_SSOR,
+                                                    n_threads=cores, want_psi=False), 5)
+        out["cpu_baseline"] = {
+            "value": round(64 / min(walls), 1), "unit": "cells/s", "cores": cores, "kind": "port",
+            "sample": f"all 64 cells, CG+SSOR(1.6) abs tol 1e-12 per basis, OpenMP on "
+                      f"{host_cores_text(cores)}, best of 5 runs after a warm-up (median "
+                      f"{64 / float(np.median(walls)):.0f} cells/s)"}
     qb.close()
     return out
 
@@ -799,7 +849,8 @@ def main():
             not args.no_cpu_baseline)
     if not args.no_plain:
         others_cfg["q_plain_2d_8x8_l4"] = plain_config(max(2, args.steps // 2),
-                                                       min(args.warmup, 2), not args.no_check)
+                                                       min(args.warmup, 2), not args.no_check,
+                                                       not args.no_cpu_baseline)
     if not args.no_adjoint:
         others_cfg["q_adjoint_3d_16x16x16"] = adjoint_config(max(2, args.steps // 10),
                                                              not args.no_check)
