@@ -74,7 +74,10 @@ enum { MSFEC_REACTION_NONE = 0, MSFEC_REACTION_CONST = 1, MSFEC_REACTION_CELL = 
  *                  preconditioner, the analogue of use_direct_solver_basis =
  *                  false (solve_iterative, q_basis.cc:335-357, which uses SSOR):
  *                  one workgroup per (cell, basis), no bandwidth limit (3-D up
- *                  to n_refine_local = 4, 2-D up to 6), stops at
+ *                  to n_refine_local = 4, 2-D up to 6), starts from the coarse
+ *                  Q1 function of the basis (scalar coefficient per fine
+ *                  cell: the solution when the coefficient is constant; zero
+ *                  otherwise), stops at
  *                  ||r|| <= cg_tol ||b|| (default 1e-13) or after cg_max_iter
  *                  iterations (default 5000); the element matrix is second
  *                  order in the solve's error (the energy form of the Schur
@@ -86,7 +89,8 @@ enum { MSFEC_SOLVER_BAND_FP64 = 0, MSFEC_SOLVER_BAND_FP64_MFMA = 1, MSFEC_SOLVER
        MSFEC_SOLVER_CG = 3 };
 /* per-cell status codes.  NOT_CONVERGED: BAND_MIXED stopped after
  * max_refine steps with a predicted error above 1e-10 relative, or CG
- * reached cg_max_iter.  NOT_SPD also reports a CG breakdown (p^T K p <= 0). */
+ * reached cg_max_iter.  NOT_SPD also reports a CG breakdown (a diagonal
+ * entry or p^T K p <= 0). */
 enum { MSFEC_CELL_OK = 0, MSFEC_CELL_NOT_CONVERGED = 1, MSFEC_CELL_NOT_SPD = 2, MSFEC_CELL_NONFINITE = 3 };
 /* flags */
 enum { MSFEC_FLAG_TIMING = 1 /* record per-kernel HIP events in compute */ };

@@ -66,10 +66,12 @@ __global__ void qu_scatter_kernel(const double* __restrict__ src, const size_t* 
   dst[off[i] * mul + k] = src[t];
 }
 
+// the band solvers' condition: the half-bandwidth and the 4 rows entering at
+// each super-step fit the 64-slot window
 bool band_fits(int dim, int level) {
   const int ni = (1 << level) - 1;
   const int bw = (dim == 2) ? ni + 1 : ni * ni + ni + 1;
-  return bw + 1 <= 64;
+  return bw + 4 <= 64;
 }
 
 int ipow_qu(int b, int e) {

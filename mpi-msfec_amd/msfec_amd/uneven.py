@@ -31,9 +31,10 @@ CANDIDATES = ("band_mfma_w16", "band_mfma_w32", "band_mfma_w64", "band_valu", "c
 
 
 def band_fits(dim: int, level: int) -> bool:
+    """Whether the band solvers take this level: half-bandwidth + 4 <= 64."""
     ni = (1 << level) - 1
     bw = ni + 1 if dim == 2 else ni * ni + ni + 1
-    return bw + 1 <= 64
+    return bw + 4 <= 64
 
 
 class _LevelView:
