@@ -569,9 +569,18 @@ __device__ __forceinline__ void box_tables(int x, const double* hh, double& ks, 
 // t = 1+e lower direction e, zero where the neighbour is on the boundary),
 // Rows[c][j][16..23] = lifting right-hand sides -sum_{m in B} K[j][m] g_k(m).
 // K[n][n+dd] = kv[mask(dd)] * (sum of the coefficients of the fine cells
-// shared by n and n+dd) + the mass analogue for the reaction term.  One block
-// per cell, one thread per interior row; all branching stays out of the
-// serial factorisation.
+// shared by n and n+dd) + the mass analogue for the reaction term.
+//
+// The same pass gives the boundary part of the element matrix and
+// right-hand side and the boundary values of Psi: with g_k the lifting
+// (coarse Q1 basis k at the boundary nodes, zero inside),
+//   P[i][k] = sum_{m in B} g_i(m) (K_BB g_k)(m),   b_B[i] = sum_{m in B} g_i(m) F(m).
+// One block per cell; the fine nodes are taken 256 at a time, one per thread,
+// interior and boundary nodes alike: both need the node's coupling sums and
+// one contraction of the couplings to boundary neighbours with the lifting
+// (-K_IB g_k for an interior row, K_BB g_k at a boundary node), so a wave
+// does that work once for whatever mix of nodes it holds.  Threads
+// (i, k) < (2^dim)^2 then sum their entry of P over the chunk.
 // ---------------------------------------------------------------------------
 template <int DIM, bool REAC>
 __global__ __launch_bounds__(256) void q_rows_kernel(BandArgs a) {
@@ -579,10 +588,7 @@ __global__ __launch_bounds__(256) void q_rows_kernel(BandArgs a) {
   const int cell = blockIdx.x;
   extern __shared__ double dyn[];  // coefficient | reaction | load
   __shared__ double kv[8], mv[8];
-  // rows of 128 threads at a time staged for coalesced stores (row stride
-  // 25 keeps the 64-bit LDS writes of a wave at 2 lanes per bank); later the
-  // chunk buffers of the boundary sums
-  constexpr int kStage = 25;
+  // chunk buffers of the boundary sums: g | K_BB g of 256 nodes (stride 9) | F
   __shared__ double rbuf[256 * 19];
   double* cf = dyn;
   double* rf = dyn + a.nfc;
@@ -598,128 +604,31 @@ __global__ __launch_bounds__(256) void q_rows_kernel(BandArgs a) {
     const double* rg = a.reac + (size_t)cell * a.nfc;
     for (int z = threadIdx.x; z < a.nfc; z += blockDim.x) rf[z] = rg[z];
   }
-  if (threadIdx.x < 8) {
+  double vol = 1.0;
+  {
     const double* v = a.verts + (size_t)cell * D::NV * DIM;
     double hh[3] = {1.0, 1.0, 1.0};
 #pragma unroll
-    for (int d = 0; d < DIM; ++d) hh[d] = (v[(D::NV - 1) * DIM + d] - v[d]) / a.N;
-    double ks, ms;
-    box_tables<DIM>(threadIdx.x, hh, ks, ms);
-    kv[threadIdx.x] = ks;
-    mv[threadIdx.x] = ms;
+    for (int d = 0; d < DIM; ++d) {
+      hh[d] = (v[(D::NV - 1) * DIM + d] - v[d]) / a.N;
+      vol *= hh[d];
+    }
+    if (threadIdx.x < 8) {
+      double ks, ms;
+      box_tables<DIM>(threadIdx.x, hh, ks, ms);
+      kv[threadIdx.x] = ks;
+      mv[threadIdx.x] = ms;
+    }
   }
   __syncthreads();
   const double rc = (a.reaction_mode == MSFEC_REACTION_CONST) ? a.reaction_const : 0.0;
   const double invN = 1.0 / (double)a.N;
   const int tid = threadIdx.x;
-  for (int jn0 = 0; jn0 < a.n; jn0 += 256) {
-    const int jn = jn0 + tid;
-    double lo[16], rhs[8];
-    if (jn < a.n) {
-    int c[3] = {0, 0, 0};
-    {
-      int t = jn;
-#pragma unroll
-      for (int d = 0; d < DIM; ++d) {
-        c[d] = t % a.ni + 1;
-        t /= a.ni;
-      }
-    }
-#pragma unroll
-    for (int t = 0; t < 16; ++t) lo[t] = 0.0;
-#pragma unroll
-    for (int k = 0; k < 8; ++k) rhs[k] = 0.0;
-    double SA[D::NST], SR[D::NST];
-    node_sums<DIM>(cf, rf, has_rf, rc, c, a.N, SA, SR);
-    double fl[3][3][2];
-#pragma unroll
-    for (int d = 0; d < DIM; ++d)
-#pragma unroll
-      for (int q = 0; q < 3; ++q) {
-        const double t = (double)(c[d] + q - 1) * invN;
-        fl[d][q][0] = 1.0 - t;
-        fl[d][q][1] = t;
-      }
-    double wb[D::NST];  // couplings to boundary neighbours
-#pragma unroll
-    for (int e = 0; e < D::NST; ++e) {
-      int mask = 0;
-      bool bnd = false;
-#pragma unroll
-      for (int d = 0; d < DIM; ++d) {
-        const int dd = dcomp<DIM>(e, d);
-        if (dd != 0) mask |= 1 << d;
-        const int m = c[d] + dd;
-        if (m == 0 || m == a.N) bnd = true;
-      }
-      const double kval = kv[mask] * SA[e] + (REAC ? mv[mask] * SR[e] : 0.0);
-      wb[e] = (bnd && e != D::CENTER) ? -kval : 0.0;
-      if (e == D::CENTER) lo[0] = kval;
-      else if (!bnd && e < D::CENTER) lo[1 + e] = kval;
-    }
-    lift_contract<DIM>(wb, fl, rhs);
-    // slot 14: sum of the loads of the fine cells around the node (the
-    // interior load is this times |cell| / 2^dim); slot 15: the node index
-    double fj;
-    if (a.rhs_mode == MSFEC_RHS_CELL) {
-      fj = 0.0;
-#pragma unroll
-      for (int al = 0; al < D::NV; ++al) {
-        int z = 0, mul = 1;
-#pragma unroll
-        for (int d = 0; d < DIM; ++d) {
-          z += (c[d] - ((al >> d) & 1)) * mul;
-          mul *= a.N;
-        }
-        fj += ff[z];
-      }
-    } else {
-      fj = a.rhs_const * D::NV;
-    }
-    lo[14] = fj;
-    lo[15] = (double)(c[0] + a.S * (c[1] + a.S * c[2]));
-    }
-    // the chunk's rows are contiguous in the row table: staged by halves,
-    // then written with 16-byte stores
-#pragma unroll
-    for (int half = 0; half < 2; ++half) {
-      const int r0 = jn0 + 128 * half;
-      if (r0 >= a.n) break;
-      if ((tid >> 7) == half && jn < a.n) {
-        double* st = rbuf + (tid & 127) * kStage;
-#pragma unroll
-        for (int t = 0; t < 16; ++t) st[t] = lo[t];
-#pragma unroll
-        for (int k = 0; k < 8; ++k) st[16 + k] = rhs[k];
-      }
-      __syncthreads();
-      const int cnt = min(128, a.n - r0);
-      double2* dst = reinterpret_cast<double2*>(const_cast<double*>(a.Rows) +
-                                                ((size_t)cell * a.n + r0) * kRowStride);
-      for (int idx = tid; idx < cnt * (kRowStride / 2); idx += 256) {
-        const int r = idx / (kRowStride / 2), k = 2 * (idx - r * (kRowStride / 2));
-        dst[idx] = make_double2(rbuf[r * kStage + k], rbuf[r * kStage + k + 1]);
-      }
-      __syncthreads();
-    }
-  }
-
-  // Boundary part of the element matrix and right-hand side, and the
-  // boundary values of Psi: with g_k the lifting (coarse Q1 basis k at the
-  // boundary nodes, zero inside),
-  //   P[i][k] = sum_{m in B} g_i(m) (K_BB g_k)(m),   b_B[i] = sum_{m in B} g_i(m) F(m).
-  // Nodes are taken 256 at a time; threads (i,k) < (2^dim)^2 then sum their
-  // entry over the chunk.
   double* gs = rbuf;
   double* kps = rbuf + 256 * 9;
   double* fsh = rbuf + 256 * 18;
   double* psi_out = a.Psi + (size_t)cell * D::NV * a.nn;
-  double vol = 1.0;
-  {
-    const double* v = a.verts + (size_t)cell * D::NV * DIM;
-#pragma unroll
-    for (int d = 0; d < DIM; ++d) vol *= (v[(D::NV - 1) * DIM + d] - v[d]) / a.N;
-  }
+  double* rows_out = const_cast<double*>(a.Rows) + (size_t)cell * a.n * kRowStride;
   // thread (quarter qt, pair (pi, pk)) sums its quarter of every chunk:
   // 4 independent chains over 64 nodes instead of one over 256
   const int qt = tid >> 6, pi = (tid >> 3) & 7, pk = tid & 7;
@@ -728,10 +637,11 @@ __global__ __launch_bounds__(256) void q_rows_kernel(BandArgs a) {
   double pacc = 0.0, bacc = 0.0;
   for (int base = 0; base < a.nn; base += 256) {
     const int node = base + tid;
+    const bool live = node < a.nn;
     int c[3] = {0, 0, 0};
     bool bnd = false;
     {
-      int t = (node < a.nn) ? node : 0;
+      int t = live ? node : 0;
 #pragma unroll
       for (int d = 0; d < DIM; ++d) {
         c[d] = t % a.S;
@@ -739,34 +649,14 @@ __global__ __launch_bounds__(256) void q_rows_kernel(BandArgs a) {
         if (c[d] == 0 || c[d] == a.N) bnd = true;
       }
     }
-    double g[D::NV], kp[D::NV], fsum = 0.0;
+    // out: the contraction with the lifting (8 slots: the row table's
+    // right-hand sides, zero beyond 2^dim)
+    double g[D::NV], out[8], fsum = 0.0;
 #pragma unroll
-    for (int k = 0; k < D::NV; ++k) {
-      g[k] = 0.0;
-      kp[k] = 0.0;
-    }
-    if (node < a.nn && bnd) {
+    for (int k = 0; k < D::NV; ++k) g[k] = 0.0;
 #pragma unroll
-      for (int k = 0; k < D::NV; ++k) {
-        g[k] = lift_k<DIM>(k, c, invN);
-        psi_out[(size_t)k * a.nn + node] = g[k];
-      }
-      // F[node] = sum over the fine cells touching the node of f_z |z| / 2^dim
-#pragma unroll
-      for (int al = 0; al < D::NV; ++al) {
-        bool ok = true;
-        int z = 0, mul = 1;
-#pragma unroll
-        for (int d = 0; d < DIM; ++d) {
-          const int zd = c[d] - ((al >> d) & 1);
-          if (zd < 0 || zd >= a.N) ok = false;
-          z += zd * mul;
-          mul *= a.N;
-        }
-        if (ok) fsum += (a.rhs_mode == MSFEC_RHS_CELL) ? ff[z] : a.rhs_const;
-      }
-      fsum *= vol / D::NV;
-      // (K_BB g_k)[node]: boundary neighbours only
+    for (int k = 0; k < 8; ++k) out[k] = 0.0;
+    if (live) {
       double SA[D::NST], SR[D::NST];
       node_sums<DIM>(cf, rf, has_rf, rc, c, a.N, SA, SR);
       double fl[3][3][2];
@@ -778,7 +668,12 @@ __global__ __launch_bounds__(256) void q_rows_kernel(BandArgs a) {
           fl[d][q][0] = 1.0 - t;
           fl[d][q][1] = t;
         }
-      double wb[D::NST];  // couplings to boundary neighbours inside the grid
+      // couplings to boundary neighbours inside the grid (the diagonal too at
+      // a boundary node): -K_IB at an interior row, K_BB at a boundary node;
+      // the lower stencil of an interior row from its interior neighbours
+      double lo[16], wb[D::NST];
+#pragma unroll
+      for (int t = 0; t < 16; ++t) lo[t] = 0.0;
 #pragma unroll
       for (int e = 0; e < D::NST; ++e) {
         int mask = 0;
@@ -791,15 +686,72 @@ __global__ __launch_bounds__(256) void q_rows_kernel(BandArgs a) {
           if (m < 0 || m > a.N) inside = false;
           if (m == 0 || m == a.N) mb = true;
         }
-        wb[e] = (inside && mb) ? kv[mask] * SA[e] + (REAC ? mv[mask] * SR[e] : 0.0) : 0.0;
+        const double kval = kv[mask] * SA[e] + (REAC ? mv[mask] * SR[e] : 0.0);
+        const bool take = inside && mb && (bnd || e != D::CENTER);
+        wb[e] = take ? (bnd ? kval : -kval) : 0.0;
+        if (e == D::CENTER) lo[0] = kval;
+        else if (!mb && e < D::CENTER) lo[1 + e] = kval;
       }
-      lift_contract<DIM>(wb, fl, kp);
+      lift_contract<DIM>(wb, fl, out);
+      if (!bnd) {
+        // slot 14: sum of the loads of the fine cells around the node (the
+        // interior load is this times |cell| / 2^dim); slot 15: the node index
+        double fj;
+        if (a.rhs_mode == MSFEC_RHS_CELL) {
+          fj = 0.0;
+#pragma unroll
+          for (int al = 0; al < D::NV; ++al) {
+            int z = 0, mul = 1;
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) {
+              z += (c[d] - ((al >> d) & 1)) * mul;
+              mul *= a.N;
+            }
+            fj += ff[z];
+          }
+        } else {
+          fj = a.rhs_const * D::NV;
+        }
+        lo[14] = fj;
+        lo[15] = (double)node;
+        int jn = 0;
+#pragma unroll
+        for (int d = DIM - 1; d >= 0; --d) jn = jn * a.ni + (c[d] - 1);
+        double2* dst = reinterpret_cast<double2*>(rows_out + (size_t)jn * kRowStride);
+#pragma unroll
+        for (int t = 0; t < 8; ++t) dst[t] = make_double2(lo[2 * t], lo[2 * t + 1]);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) dst[8 + k] = make_double2(out[2 * k], out[2 * k + 1]);
+#pragma unroll
+        for (int k = 0; k < D::NV; ++k) out[k] = 0.0;  // no boundary contribution
+      } else {
+#pragma unroll
+        for (int k = 0; k < D::NV; ++k) {
+          g[k] = lift_k<DIM>(k, c, invN);
+          psi_out[(size_t)k * a.nn + node] = g[k];
+        }
+        // F[node] = sum over the fine cells touching the node of f_z |z| / 2^dim
+#pragma unroll
+        for (int al = 0; al < D::NV; ++al) {
+          bool ok = true;
+          int z = 0, mul = 1;
+#pragma unroll
+          for (int d = 0; d < DIM; ++d) {
+            const int zd = c[d] - ((al >> d) & 1);
+            if (zd < 0 || zd >= a.N) ok = false;
+            z += zd * mul;
+            mul *= a.N;
+          }
+          if (ok) fsum += (a.rhs_mode == MSFEC_RHS_CELL) ? ff[z] : a.rhs_const;
+        }
+        fsum *= vol / D::NV;
+      }
     }
     __syncthreads();  // the previous chunk has been summed
 #pragma unroll
     for (int k = 0; k < D::NV; ++k) {
       gs[tid * 9 + k] = g[k];
-      kps[tid * 9 + k] = kp[k];
+      kps[tid * 9 + k] = out[k];
     }
     fsh[tid] = fsum;
     __syncthreads();
