@@ -3666,49 +3666,59 @@ __global__ __launch_bounds__(256) void q_cg_unity_kernel(BandArgs a, CgArgs g, i
 // Element matrix and right-hand side of a CG cell: S in its energy form
 // S[i][k] = R_i^T x_k + x_i^T r_k (r = R - K x, exact up to the second order
 // in the error of x), symmetrised; M = P - S and b = b_B + sum_j x_j F_j as in
-// the band kernels.  Lane (i = lane >> 3, k = lane & 7) of each of 16 waves
-// sums every 16th interior row, 4 rows in flight.
+// the band kernels.  Thread (i = tid & 7, row group tid >> 3) sums row i of S
+// and b_i over every 128th interior row: the 8 threads of a row read its
+// 8-wide blocks of x, r and R once between them, and a wave covers 8 rows a
+// step.  The row groups add up across lanes (shuffles) and waves (LDS).
 constexpr int kFinWaves = 16;
 
 template <int DIM>
 __global__ __launch_bounds__(64 * kFinWaves) void q_cg_finish_kernel(BandArgs a, CgArgs g) {
   using D = Dirs<DIM>;
+  constexpr int kGroups = 8 * kFinWaves;
   const int cell = blockIdx.x, tid = threadIdx.x;
-  const int lane = tid & 63, w = tid >> 6, i = lane >> 3, k = lane & 7;
-  __shared__ double part[kFinWaves][64], spart[kFinWaves][64];
-  double s = 0.0, bi = 0.0;
-  if (i < D::NV && k < D::NV) {
+  const int lane = tid & 63, w = tid >> 6, i = tid & 7;
+  __shared__ double part[kFinWaves][8][9];  // per wave: S[i][0..7] | b_i
+  double s[8], bi = 0.0;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) s[k] = 0.0;
+  if (i < D::NV) {
     const double* rows = a.Rows + (size_t)cell * a.n * kRowStride;
     const double* X = a.X + (size_t)cell * a.n * kMaxNb;
     const double* Rr = a.Rres + (size_t)cell * a.n * kMaxNb;
-    auto term = [&](int jn, double& acc, double& bacc) {
-      const double xi = X[(size_t)jn * kMaxNb + i], xk = X[(size_t)jn * kMaxNb + k];
-      acc = fma(rows[(size_t)jn * kRowStride + 16 + i], xk, acc);
-      acc = fma(xi, Rr[(size_t)jn * kMaxNb + k], acc);
-      if (i == 0) bacc = fma(xk, rows[(size_t)jn * kRowStride + 14], bacc);
-    };
-    // every 16th row per wave, 4 rows in flight
-    double s4[4] = {0.0, 0.0, 0.0, 0.0}, b4[4] = {0.0, 0.0, 0.0, 0.0};
-    int jn = w;
-    for (; jn + 3 * kFinWaves < a.n; jn += 4 * kFinWaves) {
+#pragma unroll 2
+    for (int jn = tid >> 3; jn < a.n; jn += kGroups) {
+      const double Ri = rows[(size_t)jn * kRowStride + 16 + i];
+      const double Fj = rows[(size_t)jn * kRowStride + 14];
+      const double* xr = X + (size_t)jn * kMaxNb;
+      const double* rr = Rr + (size_t)jn * kMaxNb;
+      const double xi = xr[i];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) term(jn + u * kFinWaves, s4[u], b4[u]);
+      for (int k = 0; k < D::NV; ++k) s[k] = fma(xi, rr[k], fma(Ri, xr[k], s[k]));
+      bi = fma(xi, Fj, bi);
     }
-    for (; jn < a.n; jn += kFinWaves) term(jn, s4[0], b4[0]);
-    s = (s4[0] + s4[1]) + (s4[2] + s4[3]);
-    bi = (b4[0] + b4[1]) + (b4[2] + b4[3]);
   }
-  part[w][lane] = s;
-  spart[w][lane] = bi;
+  // the 8 row groups of a wave (lanes i, i + 8, ..., i + 56)
+#pragma unroll
+  for (int o = 8; o < 64; o <<= 1) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) s[k] += __shfl_xor(s[k], o);
+    bi += __shfl_xor(bi, o);
+  }
+  if (lane < 8) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) part[w][lane][k] = s[k];
+    part[w][lane][8] = bi;
+  }
   __syncthreads();
   if (w == 0) {
-    const int lt = k * 8 + i;  // lane of S[k][i]
+    const int mi = lane >> 3, mk = lane & 7;  // lane of S[mi][mk] in fast_finish
     double sik = 0.0, ski = 0.0, bint = 0.0;
 #pragma unroll
     for (int v = 0; v < kFinWaves; ++v) {
-      sik += part[v][lane];
-      ski += part[v][lt];
-      bint += spart[v][lane];
+      sik += part[v][mi][mk];
+      ski += part[v][mk][mi];
+      bint += part[v][lane & 7][8];  // b_lane for the lanes < 2^dim
     }
     fast_finish<DIM>(a, cell, 0.5 * (sik + ski), bint);
     if (lane == 0) {
