@@ -97,6 +97,12 @@ def lib():
         L.oq_rt_basis_cell_qp.restype = ctypes.c_int
         L.oq_rt_assemble_qp.argtypes = [ctypes.c_int, dp, dp, dp, dp, dp]
         L.oq_rt_assemble_qp.restype = ctypes.c_int
+        L.oq_nrt_basis_cell.argtypes = [ctypes.c_int, dp, dp, dp, dp, dp, dp, dp, dp]
+        L.oq_nrt_basis_cell.restype = ctypes.c_int
+        L.oq_nrt_assemble.argtypes = [ctypes.c_int, dp, dp, dp, dp, dp, dp, dp]
+        L.oq_nrt_assemble.restype = ctypes.c_int
+        L.oq_nrt_curl_loads.argtypes = [ctypes.c_int, dp, dp]
+        L.oq_nrt_curl_loads.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -374,3 +380,89 @@ def rt_assemble(l, verts, a, f=1.0, tensor=False):
     st = fn(l, _dp(v), _dp(a), _dp(f), _dp(A), _dp(F))
     assert st == 0
     return A, F
+
+
+# ---------------------------------------------------------------------------
+# 3-D Nedelec x Raviart-Thomas basis (nrt_oracle.c)
+# ---------------------------------------------------------------------------
+def nrt_sizes(l):
+    """(m, nx, ne, nf1, nf, N): fine cells per direction, x-edges, edges,
+    x-faces, faces, all fine dofs."""
+    m = 1 << l
+    nx = m * (m + 1) ** 2
+    nf1 = m * m * (m + 1)
+    return m, nx, 3 * nx, nf1, 3 * nf1, 3 * nx + 3 * nf1
+
+
+def _nrt_coefs(l, ainv, bq, rq, fq):
+    nc = (1 << l) ** 3
+    ainv = np.ascontiguousarray(np.broadcast_to(np.asarray(ainv, dtype=np.float64), (nc, 8, 6)))
+    bq = np.ascontiguousarray(np.broadcast_to(np.asarray(bq, dtype=np.float64), (nc, 8)))
+    rq = None if rq is None else np.ascontiguousarray(
+        np.broadcast_to(np.asarray(rq, dtype=np.float64), (nc, 8)))
+    fq = None if fq is None else np.ascontiguousarray(
+        np.broadcast_to(np.asarray(fq, dtype=np.float64), (nc, 8, 3)))
+    return ainv, bq, rq, fq
+
+
+def nrt_basis_cell(l, verts, ainv, bq=1.0, rq=None, fq=None):
+    """One coarse box cell (verts 8 x 3, deal.II vertex order): returns
+    (M 18x18, b 18, X 18 x N, status).  ainv: (xx, xy, xz, yy, yz, zz) of
+    A^-1 at the 8 Gauss points of every fine cell (m^3 x 8 x 6, or
+    broadcastable); bq, rq: m^3 x 8; fq: m^3 x 8 x 3."""
+    ainv, bq, rq, fq = _nrt_coefs(l, ainv, bq, rq, fq)
+    v = np.ascontiguousarray(verts, dtype=np.float64).reshape(8, 3)
+    N = nrt_sizes(l)[5]
+    M = np.zeros((18, 18))
+    b = np.zeros(18)
+    X = np.zeros((18, N))
+    st = lib().oq_nrt_basis_cell(l, _dp(v), _dp(ainv), _dp(bq), _dp(rq), _dp(fq), _dp(M),
+                                 _dp(b), _dp(X))
+    return M, b, X, st
+
+
+def nrt_basis_batch(l, verts, ainv, bq=1.0, rq=None, fq=None, n_threads=1):
+    """Cells of verts (n, 8, 3) with per-cell coefficient blocks (n, m^3, 8, ...)."""
+    verts = np.asarray(verts, dtype=np.float64)
+    n = verts.shape[0]
+    nc = (1 << l) ** 3
+
+    def per_cell(x, shp):
+        if x is None:
+            return [None] * n
+        x = np.asarray(x, dtype=np.float64)
+        return np.broadcast_to(x, (n,) + shp) if x.ndim <= len(shp) else x.reshape((n,) + shp)
+
+    a_, b_ = per_cell(ainv, (nc, 8, 6)), per_cell(bq, (nc, 8))
+    r_, f_ = per_cell(rq, (nc, 8)), per_cell(fq, (nc, 8, 3))
+    res = _map_cells(lambda c: nrt_basis_cell(l, verts[c], a_[c], b_[c], r_[c], f_[c]), n,
+                     n_threads)
+    return {k: np.array([x[i] for x in res]) for i, k in enumerate(("M", "b", "X", "status"))}
+
+
+def nrt_assemble(l, verts, ainv, bq=1.0, rq=None, fq=None):
+    """The assembled mixed system over all fine dofs (row = test) and the load (v, f)."""
+    ainv, bq, rq, fq = _nrt_coefs(l, ainv, bq, rq, fq)
+    v = np.ascontiguousarray(verts, dtype=np.float64).reshape(8, 3)
+    N = nrt_sizes(l)[5]
+    A = np.zeros((N, N))
+    F = np.zeros(N)
+    st = lib().oq_nrt_assemble(l, _dp(v), _dp(ainv), _dp(bq), _dp(rq), _dp(fq), _dp(A), _dp(F))
+    assert st == 0
+    return A, F
+
+
+def nrt_curl_loads(l, verts):
+    """(v, curl N_n) of the 12 edge bases over all fine dofs (12 x N)."""
+    v = np.ascontiguousarray(verts, dtype=np.float64).reshape(8, 3)
+    Fc = np.zeros((12, nrt_sizes(l)[5]))
+    st = lib().oq_nrt_curl_loads(l, _dp(v), _dp(Fc))
+    assert st == 0
+    return Fc
+
+
+def box_vertices_3d(x0, x1):
+    """The 8 vertices of the box [x0, x1] (3-vectors) in deal.II order."""
+    x0 = np.asarray(x0, dtype=np.float64)
+    x1 = np.asarray(x1, dtype=np.float64)
+    return np.array([[(x1 if (n >> d) & 1 else x0)[d] for d in range(3)] for n in range(8)])
