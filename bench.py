@@ -365,6 +365,95 @@ def rt_config(steps: int, warmup: int, check: bool, cpu: bool):
     return out
 
 
+NRT_NREF, NRT_NC = 2, 16
+
+
+def nrt_flops_per_cell(n: int, nrhs: int) -> float:
+    """Algorithmic FP64 flops of the dense symmetric indefinite L D L^T of an
+    n x n system (n^3 / 3) and its forward and backward sweeps for nrhs
+    right-hand sides (2 n^2 each)."""
+    return n ** 3 / 3.0 + 2.0 * n * n * nrhs
+
+
+def nrt_config(steps: int, warmup: int, check: bool, cpu: bool):
+    """The 3-D Nedelec x Raviart-Thomas configuration: 16^3 coarse cells of the
+    unit cube, 4^3 fine cells each (300 edges + 240 faces; 108 + 144 = 252
+    interior unknowns), 12 edge and 6 face bases, the coefficients of
+    example_parameters/prm_ned_rt_test-01.prm at the Gauss points (rotated
+    DiffusionInverse_A with frequencies 17/16/15 and contrast 0.9, Diffusion_B
+    with frequency 14, the gradient right-hand side, no reaction)."""
+    from msfec_amd import NedRTBasisBatch, fields as F
+    verts = F.box_grid_3d(NRT_NC)
+    pts = F.gauss_points_3d(verts, NRT_NREF)
+    ainv = F.diffusion_inverse_A_tensor(pts)
+    bq = F.diffusion_B(pts)
+    fq = F.ned_rt_rhs(pts)
+    n_cells = verts.shape[0]
+    nb = NedRTBasisBatch(verts, n_refine_local=NRT_NREF, timing=True)
+    nb.set_coefficients(ainv, bq, None, fq)
+    for _ in range(warmup):
+        nb.run()
+    nb.synchronize()
+    if warmup > 0:
+        nb.kernel_times()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        nb.run()
+    nb.synchronize()
+    wall = time.perf_counter() - t0
+    kt = nb.kernel_times()
+    m = 1 << NRT_NREF
+    n_int = 3 * m * (m - 1) ** 2 + 3 * (m - 1) * m * m
+    flops_cell = nrt_flops_per_cell(n_int, 18)
+    tflops = flops_cell * n_cells / (kt[0] * 1e-3) / 1e12
+    # compulsory HBM bytes per cell: the Gauss-point coefficients in (6 + 1 + 3
+    # values at 8 points of 64 fine cells), the 18 fine vectors, M and b out
+    hbm_cell = 8.0 * (m ** 3 * 8 * 10 + 18 * nb.N + 18 * 18 + 18)
+    out = {"workload": "3D Nedelec x Raviart-Thomas basis, 16^3=4096 coarse cells x 4^3 fine "
+                       "cells (300 edges + 240 faces, 252 interior unknowns), 12 edge + 6 face "
+                       "bases, coefficients of prm_ned_rt_test-01 at the Gauss points; one "
+                       "workgroup per cell: dense symmetric indefinite L D L^T FP64 on the "
+                       "matrix cores with the 18 right-hand sides, backward sweep, element matrix",
+           "cells_per_s": round(n_cells / (wall / steps), 1),
+           "ms_per_step": round(wall * 1e3 / steps, 4),
+           "kernel_ms": {"basis": round(kt[0], 4)},
+           "failed_cells": int((nb.get_cell_status() != 0).sum()),
+           "roofline": {"bound": "mfma", "kernel": "nrt_basis_kernel<2>",
+                        "flops_per_cell": flops_cell, "achieved_tflops": round(tflops, 4),
+                        "peak_tflops": FP64_PEAK_TFLOPS,
+                        "frac": round(tflops / FP64_PEAK_TFLOPS, 5),
+                        "hbm_bytes_per_cell": hbm_cell,
+                        "hbm_frac": round(hbm_cell * n_cells / (kt[0] * 1e-3) / 1e9
+                                          / HBM_PEAK_GBS, 5)}}
+    if check or cpu:
+        from oracle import oracle as O
+        O.build()
+        cores = host_cores()
+        samp = np.random.default_rng(13).choice(n_cells, min(n_cells, 16 * cores) if cpu else 8,
+                                                replace=False)
+        t0 = time.perf_counter()
+        o = O.nrt_basis_batch(NRT_NREF, verts[samp], ainv[samp], bq[samp], None, fq[samp],
+                              n_threads=cores if cpu else 1)
+        t_cpu = time.perf_counter() - t0
+        if check:
+            M, b, X = nb.get_global_element_matrix(), nb.get_global_element_rhs(), nb.get_basis()
+            out["rel_err_vs_fp64_oracle"] = {
+                "M": rel_err(M[samp], o["M"]), "b": rel_err(b[samp], o["b"]),
+                "X": rel_err(X[samp], o["X"]), "cells_checked": len(samp)}
+        if cpu:
+            out["cpu_baseline"] = {"value": round(len(samp) / t_cpu, 3), "unit": "cells/s",
+                                   "cores": cores, "kind": "port",
+                                   "sample": f"{len(samp)} of the 4096 cells (seeded): dense "
+                                             "assembly of the mixed system over all fine dofs, "
+                                             "LU with partial pivoting and iterative refinement "
+                                             "of the interior system (the UMFPACK direct solve "
+                                             "restated), 18 right-hand sides, one cell per "
+                                             f"thread on {host_cores_text(cores)}, wall "
+                                             f"{t_cpu:.2f} s"}
+    nb.close()
+    return out
+
+
 def cg_flops_per_iteration(dim: int, level: int) -> float:
     """Algorithmic FP64 flops of one Jacobi-PCG iteration of one basis on one
     cell: the matrix-free 3^dim-point product over the interior rows (a
@@ -664,6 +753,9 @@ def main():
                     help="skip the 2-D Q1 x Nedelec configuration timed beside the headline")
     ap.add_argument("--no-rt", action="store_true",
                     help="skip the 2-D Raviart-Thomas x DQ0 configuration timed beside the headline")
+    ap.add_argument("--no-nrt", action="store_true",
+                    help="skip the 3-D Nedelec x Raviart-Thomas configuration timed beside the "
+                         "headline")
     ap.add_argument("--no-uneven", action="store_true",
                     help="skip the uneven 3-D configuration timed beside the headline")
     ap.add_argument("--no-plain", action="store_true",
@@ -841,6 +933,10 @@ def main():
             not args.no_cpu_baseline)
     if not args.no_rt:
         others_cfg["rt_dq_2d_32x32_l4"] = rt_config(
+            max(2, args.steps // 2), min(args.warmup, 2), not args.no_check,
+            not args.no_cpu_baseline)
+    if not args.no_nrt:
+        others_cfg["ned_rt_3d_16x16x16_l2"] = nrt_config(
             max(2, args.steps // 2), min(args.warmup, 2), not args.no_check,
             not args.no_cpu_baseline)
     if not args.no_uneven:

@@ -486,6 +486,86 @@ msfec_status msfec_rt_get_cell_status(msfec_rt_handle h, int *status);
 msfec_status msfec_rt_kernel_times(msfec_rt_handle h, float *ms, int max, int *n);
 msfec_status msfec_rt_destroy(msfec_rt_handle h);
 
+/* ---------------------------------------------------------------------------
+ * Ned_RT: the mixed lowest-order Nedelec x Raviart-Thomas multiscale basis in
+ * three dimensions (the reference's NedRTBasis, source/Ned_RT/
+ * ned_rt_basis.cc).  Per coarse box cell, on the 2^l x 2^l x 2^l fine grid:
+ * sigma on the fine edges, u on the fine faces,
+ *   (A^-1 sigma, tau) - (u, curl tau) = 0,
+ *   (v, curl sigma) + (B div u, div v) + (R u, v) = (v, curl N_n),
+ * 12 edge bases (deal.II line order; the tangential trace of the coarse
+ * Nedelec function N_n on the boundary, the load only for them) and 6 face
+ * bases (deal.II face order; the normal trace of the coarse Raviart-Thomas
+ * function).  Fine degrees of freedom (m = 2^l, nx = m (m+1)^2, ne = 3 nx,
+ * nf1 = m^2 (m+1), N = ne + 3 nf1):
+ *   x-edge (i, j, k) [nodes (i,j,k)-(i+1,j,k)] at (k (m+1) + j) m + i,
+ *   y-edge at nx + (k m + j) (m+1) + i, z-edge at 2 nx + (k (m+1) + j) (m+1) + i
+ *   (the line integral of the tangential component along +x, +y, +z);
+ *   x-face (i, j, k) [x = x0 + i hx] at ne + (k m + j) (m+1) + i,
+ *   y-face at ne + nf1 + (k (m+1) + j) m + i, z-face at ne + 2 nf1 + (k m + j) m + i
+ *   (the flux in the +x, +y, +z direction).
+ * The element matrix is 18 x 18 (row = test basis; edge bases first).
+ * ------------------------------------------------------------------------- */
+typedef struct {
+  int n_refine_local; /* l, 1..2 (the dense interior system holds up to 256 unknowns) */
+  int flags;          /* MSFEC_FLAG_*                                                 */
+} msfec_nrt_params;
+
+typedef struct msfec_nrt_batch *msfec_nrt_handle;
+
+/* Create a batch.  vertices: n_cells x 8 x 3 doubles, axis-aligned boxes in
+ * deal.II vertex order.  Replaces NedRTBasis::NedRTBasis + setup_grid /
+ * setup_system_matrix / setup_basis_dofs_curl / setup_basis_dofs_div
+ * (source/Ned_RT/ned_rt_basis.cc:7-359, include/Ned_RT/ned_rt_basis.h).
+ * MSFEC_ERR_UNSUPPORTED for l > 2. */
+msfec_status msfec_nrt_create(const msfec_nrt_params *params, int n_cells,
+                              const double *vertices, int vertices_loc,
+                              msfec_nrt_handle *out);
+
+/* Values at the 8 QGauss(2) points of every fine cell (fine cells x fastest,
+ * then y, z; points x fastest): ainv, n_cells x 8^l x 8 x 6 (xx, xy, xz, yy,
+ * yz, zz of DiffusionInverse_A); b, n_cells x 8^l x 8 (Diffusion_B); r, the
+ * reaction rate, same shape, NULL for zero (the reference's ReactionRate,
+ * eqn_coeff_R.cc:8-14); f, n_cells x 8^l x 8 x 3 (RightHandSideParsed), NULL
+ * for zero.  Stands for the value_list calls of NedRTBasis::assemble_system
+ * (ned_rt_basis.cc:453-461). */
+msfec_status msfec_nrt_set_coefficients(msfec_nrt_handle h, const double *ainv, const double *b,
+                                        const double *r, const double *f, int loc);
+
+/* All cells: the 18 basis solves and the element matrix, enqueued on stream.
+ * Replaces NedRTBasis::run (ned_rt_basis.cc:1280-1352) with solve_direct
+ * (:579-634) and assemble_global_element_matrix (:850-948). */
+msfec_status msfec_nrt_compute(msfec_nrt_handle h, void *stream);
+msfec_status msfec_nrt_synchronize(msfec_nrt_handle h);
+
+/* M: n_cells x 18 x 18, b: n_cells x 18 (zero for the edge bases), as built
+ * by assemble_global_element_matrix and returned by
+ * get_global_element_matrix/_rhs (ned_rt_basis.cc:1262-1272). */
+msfec_status msfec_nrt_get_element(msfec_nrt_handle h, double *M, double *b, int loc);
+
+/* X: n_cells x 18 x N, the full fine vectors of the bases (basis_curl_v and
+ * basis_div_v of NedRTBasis, renumbered as above). */
+msfec_status msfec_nrt_get_basis(msfec_nrt_handle h, double *X, int loc);
+
+/* The fine solution from the coarse weights w (n_cells x 18): sigma =
+ * sum_{i<12} w_i sigma_i (edge part of the edge bases, n_cells x ne) and u =
+ * sum_{i>=12} w_i u_i (face part of the face bases, n_cells x (N - ne)); either
+ * output may be NULL.  Replaces NedRTBasis::set_global_weights
+ * (ned_rt_basis.cc:1156-1181).  Synchronizes. */
+msfec_status msfec_nrt_reconstruct(msfec_nrt_handle h, const double *w, int w_loc, double *sigma,
+                                   double *u, int out_loc);
+
+/* Per-cell status: MSFEC_CELL_OK, MSFEC_CELL_NOT_SPD when a pivot of the
+ * symmetric indefinite factorisation has the wrong sign (edge pivots must be
+ * positive, face pivots negative), MSFEC_CELL_NONFINITE for a non-finite
+ * element matrix. */
+msfec_status msfec_nrt_get_cell_status(msfec_nrt_handle h, int *status);
+
+/* Mean time in ms of the basis kernel over the computes made with
+ * MSFEC_FLAG_TIMING since the previous call. */
+msfec_status msfec_nrt_kernel_times(msfec_nrt_handle h, float *ms, int max, int *n);
+msfec_status msfec_nrt_destroy(msfec_nrt_handle h);
+
 /* CG iterations of the last compute, n_cells x 2^dim ints on the host
  * (basis-minor), for a handle created with MSFEC_SOLVER_CG.  The reference
  * reports CG iterations per basis solve in verbose mode

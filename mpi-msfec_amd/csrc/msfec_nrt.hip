@@ -1,0 +1,1192 @@
+// Ned_RT: the three-dimensional lowest-order Nedelec x Raviart-Thomas
+// multiscale basis on MI355X (gfx950), behind the msfec_nrt_* entry points of
+// include/msfec.h.
+//
+// Reference: source/Ned_RT/ned_rt_basis.cc (NedRTBasis).  Per coarse box K,
+// on the 2^l x 2^l x 2^l fine grid, unknowns sigma on the fine edges and u on
+// the fine faces:
+//   (A^-1 sigma, tau) - (u, curl tau) = 0
+//   (v, curl sigma) + (B div u, div v) + (R u, v) = (v, curl N_n)   (n < 12)
+// (ned_rt_basis.cc:488-511) with the tangential trace of the coarse Nedelec
+// function N_n (edge bases n < 12) or the normal trace of the coarse
+// Raviart-Thomas function of face n - 12 (face bases) on the boundary
+// (:225-359), one solve per basis (:1280-1352), then the 18 x 18 element
+// matrix of :850-948.
+//
+// One workgroup per coarse cell does all of it (nrt_basis_kernel):
+//   local matrices  the fine-cell blocks (edge mass with A^-1 at the 8 Gauss
+//                   points, face div-div + reaction, the coefficient-free
+//                   curl coupling) and the curl loads, in LDS;
+//   assembly        the interior system in the symmetric form
+//                   K' = [Me, -C^T; -C, -Du] (face rows negated), edge dofs
+//                   first, held in registers as 16 x 16 FP64 matrix-core tiles
+//                   (lower triangle), with the 18 right-hand sides (loads
+//                   minus the boundary lift) appended as two more tile rows;
+//   factorisation   L D L^T without pivoting, 4 pivots per step: the edge
+//                   block Me is symmetric positive definite and its Schur
+//                   complement -(Du + C Me^-1 C^T) negative definite, so every
+//                   pivot has a known sign (a pivot of the wrong sign flags
+//                   the cell).  Each step factors the 4 x 4 pivot block in
+//                   every thread, writes the panel V = A21 L1^-T and
+//                   V D1^-1 to LDS and updates all trailing tiles with one
+//                   FP64 MFMA each; the right-hand-side rows ride along, so
+//                   the forward elimination is free;
+//   backward sweep  per tile column, right to left: the owners of the tiles
+//                   below the diagonal form L^T x products on the matrix
+//                   cores (their accumulator registers are the transposed A
+//                   operand), the diagonal tile's owner solves 16 rows;
+//   outputs         the full fine vectors of the 18 bases, then the element
+//                   matrix by the reference's block rule from the local
+//                   matrices (edge bases through their edge part, face bases
+//                   through their face part) and the load (v, f) paired with
+//                   the face part of the face bases.
+// Tiles are spread over TN / 2 waves, wave w owning tile rows w and
+// TN - 1 - w, which keeps the trailing work of every step balanced.
+// Dense storage limits the interior system to 256 unknowns: l = 1 (18) and
+// l = 2 (252).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "msfec.h"
+
+namespace msfec_detail {
+extern thread_local std::string g_last_error;
+}
+
+namespace {
+
+constexpr int kNB = 18;   // 12 edge bases + 6 face bases
+constexpr int kMaxL = 2;  // dense interior systems up to 256 unknowns
+
+typedef double nrt_d4 __attribute__((ext_vector_type(4)));
+
+template <int L>
+struct Nd {
+  static constexpr int m = 1 << L;
+  static constexpr int nfc = m * m * m;                    // fine cells
+  static constexpr int ns = 3 * m * (m - 1) * (m - 1);     // interior edges
+  static constexpr int nu = 3 * (m - 1) * m * m;           // interior faces
+  static constexpr int n = ns + nu;
+  static constexpr int TN = ((n + 31) / 32) * 2;           // tile rows of the matrix
+  static constexpr int NP = 16 * TN;                       // padded size
+  static constexpr int NW = TN / 2;                        // waves
+  static constexpr int NT = 64 * NW;                       // threads
+  static constexpr int NR = NP + 32;                       // + two right-hand-side tile rows
+  static constexpr int NS = TN + 5;                        // tile slots per wave
+  static constexpr int XS = kNB;                           // row stride of x in LDS
+  static constexpr int CH = nfc < 8 ? nfc : 8;             // fine cells per element chunk
+  // LDS, in doubles.  Region A lives through the factorisation (pivot
+  // columns, V, V D^-1), later the backward partial sums and the element
+  // chunks; region B holds the coefficients, curl loads and dof table, later
+  // x and the element chunk's basis values; region C the local matrices.
+  static constexpr int cmax(int a, int b) { return a > b ? a : b; }
+  static constexpr int oStage = 0, oV = NR * 4, oVD = 2 * NR * 4;
+  static constexpr int oPart = 0;                          // NW x 16 x 18
+  static constexpr int oLs = NW * 16 * 18;                 // 16 x 16
+  static constexpr int oY = 0;                             // CH x 18 x 18
+  static constexpr int szA = cmax(cmax(3 * NR * 4, NW * 16 * 18 + 256), CH * 324);
+  static constexpr int oB = szA;
+  static constexpr int oCf = oB;                           // nfc x 8 x 8
+  static constexpr int oFc = oCf + nfc * 64;               // 12 x nu
+  static constexpr int oIc = oFc + 12 * nu;                // n x 4 ints
+  static constexpr int oX = oB;                            // NP x XS
+  static constexpr int oXL = oB + NP * XS;                 // CH x 180
+  static constexpr int szB = cmax(nfc * 64 + 12 * nu + 2 * n, NP * XS + CH * 180);
+  static constexpr int oC = oB + szB;
+  static constexpr int oMl = oC;                           // nfc x 78
+  static constexpr int oDl = oMl + nfc * 78;               // nfc x 21
+  static constexpr int oCc = oDl + nfc * 21;               // 6 x 12
+  static constexpr int oFl = oCc + 72;                     // nfc x 6
+  static constexpr int total = oFl + nfc * 6;
+  static_assert(total * 8 <= 160 * 1024, "LDS budget");
+};
+
+// deal.II's line order on the unit cube: axis of line L and the values of
+// the two other coordinates (lower axis first)
+__constant__ int kLineAx[12] = {1, 1, 0, 0, 1, 1, 0, 0, 2, 2, 2, 2};
+__constant__ int kLineF1[12] = {0, 1, 0, 1, 0, 1, 0, 1, 0, 1, 0, 1};
+__constant__ int kLineF2[12] = {0, 0, 0, 0, 1, 1, 1, 1, 0, 0, 1, 1};
+
+__device__ __forceinline__ double gauss_pt(int b) {
+  return b ? 0.78867513459481288225 : 0.21132486540518711775;
+}
+__device__ __forceinline__ double lag(int s, double t) { return s ? t : 1.0 - t; }
+__device__ __forceinline__ double dlag(int s) { return s ? 1.0 : -1.0; }
+__device__ __forceinline__ int oth1(int d) { return d == 0 ? 1 : 0; }
+__device__ __forceinline__ int oth2(int d) { return d == 2 ? 1 : 2; }
+// (e_a x e_b)_c
+__device__ __forceinline__ double levi(int a, int b, int c) {
+  if (a == b || b == c || a == c) return 0.0;
+  return ((b - a + 3) % 3 == 1) ? 1.0 : -1.0;
+}
+// index of the (a, b) entry of a symmetric 3 x 3 stored (xx, xy, xz, yy, yz, zz)
+__device__ __forceinline__ int sym6(int a, int b) {
+  const int lo = min(a, b), hi = max(a, b);
+  return lo == 0 ? hi : (lo == 1 ? 2 + hi : 5);
+}
+// packed lower triangle (i >= j)
+__device__ __forceinline__ int tri(int i, int j) {
+  const int hi = max(i, j), lo = min(i, j);
+  return hi * (hi + 1) / 2 + lo;
+}
+__device__ __forceinline__ void tri_ij(int e, int& i, int& j) {
+  i = 0;
+  while ((i + 1) * (i + 2) / 2 <= e) ++i;
+  j = e - i * (i + 1) / 2;
+}
+
+// a 3-vector of scalars: selecting a component by a run-time index stays a
+// register select (an array indexed at run time would live in scratch)
+struct V3 {
+  double x, y, z;
+};
+__device__ __forceinline__ double sel3(V3 v, int i) {
+  const double xy = i == 1 ? v.y : v.x;
+  return i == 2 ? v.z : xy;
+}
+
+// Nedelec function of line L on a box of sizes h at reference point X (the
+// covariant map of l_f1(X_o1) l_f2(X_o2) e_ax): its axis value
+__device__ __forceinline__ double ned_val(int L, V3 X, V3 h) {
+  const int ax = kLineAx[L];
+  return lag(kLineF1[L], sel3(X, oth1(ax))) * lag(kLineF2[L], sel3(X, oth2(ax))) / sel3(h, ax);
+}
+// component c of its curl, grad(g / h_ax) x e_ax
+__device__ __forceinline__ double ned_curl(int L, V3 X, V3 h, int c) {
+  const int ax = kLineAx[L], o1 = oth1(ax), o2 = oth2(ax), f1 = kLineF1[L], f2 = kLineF2[L];
+  const double ha = sel3(h, ax);
+  const double g1 = dlag(f1) * lag(f2, sel3(X, o2)) / (ha * sel3(h, o1));
+  const double g2 = lag(f1, sel3(X, o1)) * dlag(f2) / (ha * sel3(h, o2));
+  return g1 * levi(o1, ax, c) + g2 * levi(o2, ax, c);
+}
+// Raviart-Thomas function of face F (axis F / 2, side F % 2): its axis
+// value (contravariant map of l_s(X_ax) e_ax) and divergence
+__device__ __forceinline__ double rt_val(int F, V3 X, V3 h) {
+  const int ax = F >> 1;
+  return lag(F & 1, sel3(X, ax)) / (sel3(h, oth1(ax)) * sel3(h, oth2(ax)));
+}
+__device__ __forceinline__ double rt_div(int F, V3 h) {
+  return dlag(F & 1) / (h.x * h.y * h.z);
+}
+__device__ __forceinline__ V3 gauss_xyz(int q) {
+  return V3{gauss_pt(q & 1), gauss_pt((q >> 1) & 1), gauss_pt(q >> 2)};
+}
+
+// f(integral_constant<int, s>) for s = B .. E - 1
+template <int B, int E>
+struct SlotLoop {
+  template <typename F>
+  __device__ __forceinline__ static void run(F&& f) {
+    if constexpr (B < E) {
+      f(std::integral_constant<int, B>{});
+      SlotLoop<B + 1, E>::run(f);
+    }
+  }
+};
+
+// Opaque copies of the lane and wave index: the per-slot LDS addresses of a
+// factorisation step are recomputed from them (a few integer operations)
+// instead of being hoisted out of the loop as dozens of live registers.
+__device__ __forceinline__ int opaque_v(int x) {
+  asm volatile("" : "+v"(x));
+  return x;
+}
+__device__ __forceinline__ int opaque_s(int x) {
+  asm volatile("" : "+s"(x));
+  return x;
+}
+
+__device__ __forceinline__ void wave_fence() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+struct NrtArgs {
+  const double* verts;  // n_cells x 8 x 3
+  const double* ainv;   // n_cells x nfc x 8 x 6
+  const double* bq;     // n_cells x nfc x 8
+  const double* rq;     // n_cells x nfc x 8, or null (no reaction)
+  const double* fq;     // n_cells x nfc x 8 x 3, or null (no load)
+  const int* icell;     // n x 4: fine cell * 32 + local dof of each cell holding the interior dof, -1
+  const int* cdof;      // nfc x 18: full dof of every local dof
+  const int* f2i;       // N: interior index of a full dof, -1 on the boundary
+  const double* bval;   // N x 18: the fixed boundary values of the bases (0 inside)
+  double* X;            // n_cells x 18 x N
+  double* M;            // n_cells x 18 x 18
+  double* b;            // n_cells x 18
+  int* status;          // n_cells
+  int N;
+};
+
+// entry (a, b) of the local matrix of fine cell c: K' (face rows negated,
+// symmetric) or, with orig, the assembled local matrix of the reference
+// (row = test); Me, Du packed lower, C[F][L] = (psi_F, curl phi_L)
+template <bool Orig>
+__device__ __forceinline__ double kloc(const double* Ml, const double* Dl, const double* Cc, int c,
+                                       int a, int b) {
+  if (a < 12 && b < 12) return Ml[c * 78 + tri(a, b)];
+  if (a >= 12 && b >= 12) return Orig ? Dl[c * 21 + tri(a - 12, b - 12)] : -Dl[c * 21 + tri(a - 12, b - 12)];
+  if (a < 12) return -Cc[(b - 12) * 12 + a];
+  return Orig ? Cc[(a - 12) * 12 + b] : -Cc[(a - 12) * 12 + b];
+}
+
+// The entries of tile (P, Q) held by this lane (column 16 Q + ll, rows
+// 16 P + 4 gg + lh): K' for P < TN (identity on the padding), else the
+// right-hand sides r = 16 (P - TN) + 4 gg + lh: the curl load (face rows,
+// negated in K') minus the lift of the boundary values.
+template <int L>
+__device__ __forceinline__ nrt_d4 nrt_tile(int P, int Q, int ll, int lh, const int* ic,
+                                        const double* Ml, const double* Dl, const double* Cc,
+                                        const double* Fc, const int* cdof, const int* f2i,
+                                        const double* bval) {
+  typedef Nd<L> D;
+  nrt_d4 t = {0.0, 0.0, 0.0, 0.0};
+  const int J = 16 * Q + ll;
+  if (J >= D::n) {
+    if (P < D::TN)
+      for (int gg = 0; gg < 4; ++gg) t[gg] = (16 * P + 4 * gg + lh == J) ? 1.0 : 0.0;
+    return t;
+  }
+  int cj[4];
+  for (int k = 0; k < 4; ++k) cj[k] = ic[J * 4 + k];
+  if (P < D::TN) {
+    for (int gg = 0; gg < 4; ++gg) {
+      const int I = 16 * P + 4 * gg + lh;
+      double v = 0.0;
+      if (I < D::n) {
+        for (int a = 0; a < 4; ++a) {
+          const int ei = ic[I * 4 + a];
+          if (ei < 0) continue;
+          for (int bb = 0; bb < 4; ++bb) {
+            const int ej = cj[bb];
+            if (ej >= 0 && (ej >> 5) == (ei >> 5))
+              v += kloc<false>(Ml, Dl, Cc, ei >> 5, ei & 31, ej & 31);
+          }
+        }
+      }
+      t[gg] = v;
+    }
+    return t;
+  }
+  double v[4];
+  int r[4];
+  for (int gg = 0; gg < 4; ++gg) {
+    r[gg] = 16 * (P - D::TN) + 4 * gg + lh;
+    v[gg] = (r[gg] < 12 && J >= D::ns) ? -Fc[r[gg] * D::nu + (J - D::ns)] : 0.0;
+  }
+  for (int bb = 0; bb < 4; ++bb) {
+    const int ej = cj[bb];
+    if (ej < 0) continue;
+    const int c = ej >> 5, lj = ej & 31;
+    for (int lb = 0; lb < 18; ++lb) {
+      const int full = cdof[c * 18 + lb];
+      if (f2i[full] >= 0) continue;
+      const double k = kloc<false>(Ml, Dl, Cc, c, lj, lb);
+      for (int gg = 0; gg < 4; ++gg)
+        if (r[gg] < kNB) v[gg] -= k * bval[(size_t)full * kNB + r[gg]];
+    }
+  }
+  for (int gg = 0; gg < 4; ++gg) t[gg] = r[gg] < kNB ? v[gg] : 0.0;
+  return t;
+}
+
+template <int L>
+__global__ __launch_bounds__(Nd<L>::NT) void nrt_basis_kernel(NrtArgs g) {
+  typedef Nd<L> D;
+  extern __shared__ double sm[];
+  __shared__ int s_flag;
+  __shared__ double s_pblk[16];
+  const int cell = blockIdx.x, tid = threadIdx.x;
+  const int lane = tid & 63, w = tid >> 6, ll = lane & 15, lh = lane >> 4;
+  const int N = g.N;
+
+  // ---- coarse and fine box sizes
+  V3 Hc, h;
+  {
+    const double* v = g.verts + (size_t)cell * 24;
+    Hc = V3{v[3] - v[0], v[7] - v[1], v[14] - v[2]};
+    h = V3{Hc.x / D::m, Hc.y / D::m, Hc.z / D::m};
+  }
+  const double vol = h.x * h.y * h.z;
+  if (tid == 0) s_flag = MSFEC_CELL_OK;
+
+  // ---- coefficients to LDS: per fine cell and point, A^-1 (6), B, R
+  double* cf = sm + D::oCf;
+  for (int idx = tid; idx < D::nfc * 8; idx += D::NT) {
+    const size_t gi = (size_t)cell * D::nfc * 8 + idx;
+    double* o = cf + idx * 8;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) o[k] = g.ainv[gi * 6 + k];
+    o[6] = g.bq[gi];
+    o[7] = g.rq ? g.rq[gi] : 0.0;
+  }
+  int* ic = reinterpret_cast<int*>(sm + D::oIc);
+  for (int idx = tid; idx < D::n * 4; idx += D::NT) ic[idx] = g.icell[idx];
+  __syncthreads();
+
+  // ---- local matrices, curl coupling, loads
+  double* Ml = sm + D::oMl;
+  double* Dl = sm + D::oDl;
+  double* Cc = sm + D::oCc;
+  double* Fl = sm + D::oFl;
+  double* Fc = sm + D::oFc;
+  for (int idx = tid; idx < D::nfc * 99; idx += D::NT) {
+    const int c = idx / 99, e = idx % 99;
+    const double* co = cf + c * 64;
+    double acc = 0.0;
+    if (e < 78) {
+      int i, j;
+      tri_ij(e, i, j);
+      const int ci = sym6(kLineAx[i], kLineAx[j]);
+      for (int q = 0; q < 8; ++q) {
+        const V3 X = gauss_xyz(q);
+        acc += ned_val(i, X, h) * ned_val(j, X, h) * co[q * 8 + ci];
+      }
+      Ml[c * 78 + e] = acc * vol * 0.125;
+    } else {
+      int i, j;
+      tri_ij(e - 78, i, j);
+      const double dd = rt_div(i, h) * rt_div(j, h);
+      const bool same = (i >> 1) == (j >> 1);
+      for (int q = 0; q < 8; ++q) {
+        const V3 X = gauss_xyz(q);
+        double v = co[q * 8 + 6] * dd;
+        if (same) v += co[q * 8 + 7] * rt_val(i, X, h) * rt_val(j, X, h);
+        acc += v;
+      }
+      Dl[c * 21 + (e - 78)] = acc * vol * 0.125;
+    }
+  }
+  for (int idx = tid; idx < 72 + D::nfc * 6 + 12 * D::nu; idx += D::NT) {
+    if (idx < 72) {
+      const int F = idx / 12, Ln = idx % 12, ax = F >> 1;
+      double acc = 0.0;
+      for (int q = 0; q < 8; ++q) {
+        const V3 X = gauss_xyz(q);
+        acc += rt_val(F, X, h) * ned_curl(Ln, X, h, ax);
+      }
+      Cc[idx] = acc * vol * 0.125;
+    } else if (idx < 72 + D::nfc * 6) {
+      const int c = (idx - 72) / 6, F = (idx - 72) % 6, ax = F >> 1;
+      double acc = 0.0;
+      if (g.fq) {
+        const double* f = g.fq + ((size_t)cell * D::nfc + c) * 24;
+        for (int q = 0; q < 8; ++q) {
+          const V3 X = gauss_xyz(q);
+          acc += rt_val(F, X, h) * f[q * 3 + ax];
+        }
+      }
+      Fl[c * 6 + F] = acc * vol * 0.125;
+    } else {
+      // (psi_I, curl N_n) over the two fine cells of interior face I
+      const int t = idx - 72 - D::nfc * 6, nb = t / D::nu, J = t % D::nu;
+      double acc = 0.0;
+      for (int k = 0; k < 2; ++k) {
+        const int e = ic[(D::ns + J) * 4 + k];
+        const int c = e >> 5, F = (e & 31) - 12, ax = F >> 1;
+        const int c0 = c % D::m, c1 = (c / D::m) % D::m, c2 = c / (D::m * D::m);
+        for (int q = 0; q < 8; ++q) {
+          const V3 X = gauss_xyz(q);
+          const V3 Xc{(c0 + X.x) / D::m, (c1 + X.y) / D::m, (c2 + X.z) / D::m};
+          acc += rt_val(F, X, h) * ned_curl(nb, Xc, Hc, ax);
+        }
+      }
+      Fc[nb * D::nu + J] = acc * vol * 0.125;
+    }
+  }
+  __syncthreads();
+
+  // ---- assembly into the tile registers
+  // slot s of wave w: for s <= TN, tile (TN-1-w, s) while s <= TN-1-w, else
+  // tile (w, TN-s) -- the tile column is a compile-time constant per branch;
+  // slots TN+1..TN+4 are the right-hand-side tiles (TN, w), (TN, TN-1-w),
+  // (TN+1, w), (TN+1, TN-1-w)
+  const int Pb = D::TN - 1 - w;
+  auto slot_pq = [&](int s, int& P, int& Q) {
+    if (s <= D::TN) {
+      const bool lo = s <= Pb;
+      P = lo ? Pb : w;
+      Q = lo ? s : D::TN - s;
+    } else {
+      const int r = s - D::TN - 1;
+      P = D::TN + (r >> 1);
+      Q = (r & 1) ? Pb : w;
+    }
+  };
+  // every loop over the tile slots below runs through SlotLoop, so the slot
+  // index is a compile-time constant and the tiles stay in registers
+  nrt_d4 T[D::NS];
+  SlotLoop<0, D::NS>::run([&](auto sc) {
+    constexpr int s = decltype(sc)::value;
+    int P, Q;
+    slot_pq(s, P, Q);
+    T[s] = nrt_tile<L>(P, Q, ll, lh, ic, Ml, Dl, Cc, Fc, g.cdof, g.f2i, g.bval);
+  });
+  // first pivot columns (0..3) to LDS
+  double* stg = sm + D::oStage;
+  double* vb = sm + D::oV;
+  double* vdb = sm + D::oVD;
+  SlotLoop<0, D::NS>::run([&](auto sc) {
+    constexpr int s = decltype(sc)::value;
+    int P, Q;
+    slot_pq(s, P, Q);
+    if (Q == 0 && ll < 4)
+#pragma unroll
+      for (int gg = 0; gg < 4; ++gg) stg[(16 * P + 4 * gg + lh) * 4 + ll] = T[s][gg];
+  });
+
+  // ---- L D L^T, 4 pivots per step, right-hand-side rows alongside
+  for (int k = 0; k < D::NP; k += 4) {
+    const int Qk = k >> 4, G = (k >> 2) & 3;
+    const int ll = opaque_v(tid) & 15, lh = (opaque_v(tid) >> 4) & 3;
+    const int w = opaque_s(__builtin_amdgcn_readfirstlane(tid >> 6)), Pb = D::TN - 1 - w;
+    auto slot_pq = [&](int s, int& P, int& Q) {
+      if (s <= D::TN) {
+        const bool lo = s <= Pb;
+        P = lo ? Pb : w;
+        Q = lo ? s : D::TN - s;
+      } else {
+        const int r = s - D::TN - 1;
+        P = D::TN + (r >> 1);
+        Q = (r & 1) ? Pb : w;
+      }
+    };
+    __syncthreads();
+    {
+      // pivot block = L1 D1 L1^T, factored by every thread
+      double p[4][4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c <= r; ++c) p[r][c] = stg[(k + r) * 4 + c];
+      double d[4], l1[4][4], u[4][4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        double dc = p[c][c];
+#pragma unroll
+        for (int t = 0; t < c; ++t) dc -= l1[c][t] * u[c][t];
+        d[c] = dc;
+#pragma unroll
+        for (int r = c + 1; r < 4; ++r) {
+          double v = p[r][c];
+#pragma unroll
+          for (int t = 0; t < c; ++t) v -= l1[r][t] * u[c][t];
+          u[r][c] = v;
+          l1[r][c] = v / dc;
+        }
+      }
+      // rows of L1^-1 (unit lower)
+      double li[4][4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) li[r][c] = (r == c) ? 1.0 : 0.0;
+#pragma unroll
+        for (int c = 0; c < r; ++c) {
+          double v = 0.0;
+#pragma unroll
+          for (int t = c; t < r; ++t) v -= l1[r][t] * li[t][c];
+          li[r][c] = v;
+        }
+      }
+      if (tid == 0) {
+        // L1 below the diagonal, D1 on it, for the pivot columns; the sign check
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+          for (int c = 0; c <= r; ++c) s_pblk[r * 4 + c] = r == c ? d[c] : l1[r][c];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          const int idx = k + c;
+          const double sg = (idx < D::ns || idx >= D::n) ? 1.0 : -1.0;
+          if (!(sg * d[c] > 0.0) || !isfinite(d[c])) s_flag = MSFEC_CELL_NOT_SPD;
+        }
+      }
+      for (int row = tid; row < D::NR; row += D::NT) {
+        const bool act = row > k + 3;
+        double a[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) a[c] = act ? stg[row * 4 + c] : 0.0;
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) {
+          double v = 0.0;
+#pragma unroll
+          for (int c = 0; c <= kk; ++c) v += a[c] * li[kk][c];
+          vb[row * 4 + kk] = v;
+          vdb[row * 4 + kk] = act ? v / d[kk] : 0.0;
+        }
+      }
+    }
+    __syncthreads();
+    {
+      // operands: V D^-1 of the wave's 4 tile rows, V of every active tile column
+      const double aPb = vdb[(16 * Pb + ll) * 4 + lh], aPa = vdb[(16 * w + ll) * 4 + lh];
+      const double aR0 = vdb[(16 * D::TN + ll) * 4 + lh];
+      const double aR1 = vdb[(16 * (D::TN + 1) + ll) * 4 + lh];
+      double bq[D::TN];
+#pragma unroll
+      for (int q = 0; q < D::TN; ++q) bq[q] = q >= Qk ? vb[(16 * q + ll) * 4 + lh] : 0.0;
+      SlotLoop<0, D::NS>::run([&](auto sc) {
+        constexpr int s = decltype(sc)::value;
+        if constexpr (s <= D::TN) {
+          const bool lo = s <= Pb;
+          const int Q = lo ? s : D::TN - s;
+          constexpr int qh = (D::TN - s) < D::TN ? D::TN - s : 0;
+          if (Q >= Qk)
+            T[s] = __builtin_amdgcn_mfma_f64_16x16x4f64(lo ? aPb : aPa, lo ? bq[s < D::TN ? s : 0] : bq[qh],
+                                                         T[s], 0, 0, 1);
+        } else {
+          constexpr int r = s - D::TN - 1;
+          const int Q = (r & 1) ? Pb : w;
+          if (Q >= Qk)
+            T[s] = __builtin_amdgcn_mfma_f64_16x16x4f64((r >> 1) ? aR1 : aR0,
+                                                         vb[(16 * Q + ll) * 4 + lh], T[s], 0, 0, 1);
+        }
+      });
+    }
+    // the pivot columns become L (unit lower, D on the diagonal), then the
+    // next pivot columns go to LDS (read after the barrier at the top)
+    const int k2 = k + 4, Q2 = k2 >> 4, G2 = (k2 >> 2) & 3;
+    SlotLoop<0, D::NS>::run([&](auto sc) {
+      constexpr int s = decltype(sc)::value;
+      int P, Q;
+      slot_pq(s, P, Q);
+      if (Q == Qk && (ll >> 2) == G) {
+        const int c = ll & 3;
+#pragma unroll
+        for (int gg = 0; gg < 4; ++gg) {
+          const int row = 16 * P + 4 * gg + lh;
+          if (row > k + 3) T[s][gg] = vdb[row * 4 + c];
+          else if (row >= k + c) T[s][gg] = s_pblk[(row - k) * 4 + c];
+        }
+      }
+      if (k2 < D::NP && Q == Q2 && (ll >> 2) == G2)
+#pragma unroll
+        for (int gg = 0; gg < 4; ++gg) stg[(16 * P + 4 * gg + lh) * 4 + (ll & 3)] = T[s][gg];
+    });
+  }
+  __syncthreads();
+
+  // ---- backward sweep L^T x = z; z = D^-1 L^-1 b sits in the rhs tiles
+  double* xb = sm + D::oX;
+  double* part = sm + D::oPart;
+  double* Ls = sm + D::oLs;
+  SlotLoop<D::TN + 1, D::NS>::run([&](auto sc) {
+    constexpr int s = decltype(sc)::value;
+    int P, Q;
+    slot_pq(s, P, Q);
+#pragma unroll
+    for (int gg = 0; gg < 4; ++gg) {
+      const int r = 16 * (P - D::TN) + 4 * gg + lh;
+      if (r < kNB) xb[(16 * Q + ll) * D::XS + r] = T[s][gg];
+    }
+  });
+  __syncthreads();
+  for (int Qb = D::TN - 1; Qb >= 0; --Qb) {
+    const int ll = opaque_v(tid) & 15, lh = (opaque_v(tid) >> 4) & 3;
+    const int w = opaque_s(__builtin_amdgcn_readfirstlane(tid >> 6)), Pb = D::TN - 1 - w;
+    auto slot_pq = [&](int s, int& P, int& Q) {
+      if (s <= D::TN) {
+        const bool lo = s <= Pb;
+        P = lo ? Pb : w;
+        Q = lo ? s : D::TN - s;
+      } else {
+        const int r = s - D::TN - 1;
+        P = D::TN + (r >> 1);
+        Q = (r & 1) ? Pb : w;
+      }
+    };
+    nrt_d4 acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
+    SlotLoop<0, D::TN + 1>::run([&](auto sc) {
+      constexpr int s = decltype(sc)::value;
+      int P, Q;
+      slot_pq(s, P, Q);
+      if (Q == Qb && P > Qb) {
+        // (L_PQ^T x_P): the accumulator layout of the tile is the A operand
+        // of its transpose, slice sp = register sp
+#pragma unroll
+        for (int sp = 0; sp < 4; ++sp) {
+          const int rx = 16 * P + 4 * sp + lh;
+          const double b0 = xb[rx * D::XS + ll];
+          const double b1 = ll < 2 ? xb[rx * D::XS + 16 + ll] : 0.0;
+          acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(T[s][sp], b0, acc0, 0, 0, 0);
+          acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(T[s][sp], b1, acc1, 0, 0, 0);
+        }
+      }
+    });
+#pragma unroll
+    for (int gg = 0; gg < 4; ++gg) {
+      part[(w * 16 + 4 * gg + lh) * kNB + ll] = acc0[gg];
+      if (ll < 2) part[(w * 16 + 4 * gg + lh) * kNB + 16 + ll] = acc1[gg];
+    }
+    __syncthreads();
+    if (Qb == w || Qb == D::TN - 1 - w) {
+      SlotLoop<0, D::TN + 1>::run([&](auto sc) {
+        constexpr int s = decltype(sc)::value;
+        int P, Q;
+        slot_pq(s, P, Q);
+        if (P == Qb && Q == Qb)
+#pragma unroll
+          for (int gg = 0; gg < 4; ++gg) Ls[(4 * gg + lh) * 16 + ll] = T[s][gg];
+      });
+      wave_fence();
+      if (lane < kNB) {
+        double x[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+          double v = xb[(16 * Qb + i) * D::XS + lane];
+          for (int ww = 0; ww < D::NW; ++ww) v -= part[(ww * 16 + i) * kNB + lane];
+          x[i] = v;
+        }
+#pragma unroll
+        for (int i = 15; i >= 0; --i) {
+#pragma unroll
+          for (int r = i + 1; r < 16; ++r) x[i] -= Ls[r * 16 + i] * x[r];
+        }
+#pragma unroll
+        for (int i = 0; i < 16; ++i) xb[(16 * Qb + i) * D::XS + lane] = x[i];
+      }
+    }
+    __syncthreads();
+  }
+
+  // ---- the fine vectors of the bases: interior from x, boundary values fixed
+  {
+    double* Xo = g.X + (size_t)cell * kNB * N;
+    for (int idx = tid; idx < kNB * N; idx += D::NT) {
+      const int r = idx / N, f = idx - r * N;
+      const int I = g.f2i[f];
+      Xo[idx] = I >= 0 ? xb[I * D::XS + r] : g.bval[(size_t)f * kNB + r];
+    }
+  }
+
+  // ---- element matrix (ned_rt_basis.cc:850-948) and load, by fine-cell chunks
+  constexpr int KP = (kNB * kNB + 6 + D::NT - 1) / D::NT;
+  double macc[KP];
+#pragma unroll
+  for (int k = 0; k < KP; ++k) macc[k] = 0.0;
+  double* XL = sm + D::oXL;  // [cl][144 edge values: local edge a, edge basis j | 36 face values]
+  double* Yb = sm + D::oY;   // [cl][local row a][basis j]
+  for (int c0 = 0; c0 < D::nfc; c0 += D::CH) {
+    for (int idx = tid; idx < D::CH * 180; idx += D::NT) {
+      const int cl = idx / 180, e = idx % 180, c = c0 + cl;
+      int a, j;
+      if (e < 144) {
+        a = e / 12;
+        j = e % 12;
+      } else {
+        a = 12 + (e - 144) / 6;
+        j = 12 + (e - 144) % 6;
+      }
+      const int full = g.cdof[c * 18 + a];
+      const int I = g.f2i[full];
+      XL[idx] = I >= 0 ? xb[I * D::XS + j] : g.bval[(size_t)full * kNB + j];
+    }
+    __syncthreads();
+    // Y[cl][a][j] = (K_loc x_j)[a] over the block of basis j
+    for (int idx = tid; idx < D::CH * 324; idx += D::NT) {
+      const int cl = idx / 324, a = (idx % 324) / 18, j = idx % 18, c = c0 + cl;
+      const double* xl = XL + cl * 180;
+      double v = 0.0;
+      if (j < 12) {
+        for (int bb = 0; bb < 12; ++bb) v += kloc<true>(Ml, Dl, Cc, c, a, bb) * xl[bb * 12 + j];
+      } else {
+        for (int bb = 0; bb < 6; ++bb)
+          v += kloc<true>(Ml, Dl, Cc, c, a, 12 + bb) * xl[144 + bb * 6 + (j - 12)];
+      }
+      Yb[idx] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < KP; ++k) {
+      const int pr = tid + k * D::NT;
+      if (pr < kNB * kNB) {
+        const int i = pr / kNB, j = pr % kNB;
+        double v = 0.0;
+        for (int cl = 0; cl < D::CH; ++cl) {
+          const double* xl = XL + cl * 180;
+          const double* y = Yb + cl * 324;
+          if (i < 12) {
+            for (int a = 0; a < 12; ++a) v += xl[a * 12 + i] * y[a * 18 + j];
+          } else {
+            for (int a = 0; a < 6; ++a) v += xl[144 + a * 6 + (i - 12)] * y[(12 + a) * 18 + j];
+          }
+        }
+        macc[k] += v;
+      } else if (pr < kNB * kNB + 6) {
+        const int i = pr - kNB * kNB;
+        double v = 0.0;
+        for (int cl = 0; cl < D::CH; ++cl)
+          for (int a = 0; a < 6; ++a) v += XL[cl * 180 + 144 + a * 6 + i] * Fl[(c0 + cl) * 6 + a];
+        macc[k] += v;
+      }
+    }
+    __syncthreads();
+  }
+  bool bad = false;
+#pragma unroll
+  for (int k = 0; k < KP; ++k) {
+    const int pr = tid + k * D::NT;
+    if (pr < kNB * kNB) {
+      g.M[(size_t)cell * kNB * kNB + pr] = macc[k];
+      bad |= !isfinite(macc[k]);
+    } else if (pr < kNB * kNB + 6) {
+      g.b[(size_t)cell * kNB + 12 + (pr - kNB * kNB)] = macc[k];
+    }
+    if (pr < 12) g.b[(size_t)cell * kNB + pr] = 0.0;
+  }
+  if (bad) atomicCAS(&s_flag, (int)MSFEC_CELL_OK, (int)MSFEC_CELL_NONFINITE);
+  __syncthreads();
+  if (tid == 0) g.status[cell] = s_flag;
+}
+
+// set_global_weights (ned_rt_basis.cc:1156-1181): sigma from the edge part of
+// the edge bases, u from the face part of the face bases
+__global__ __launch_bounds__(256) void nrt_reconstruct_kernel(const double* X, const double* w,
+                                                               int N, int ne, double* sig,
+                                                               double* u) {
+  const int cell = blockIdx.x;
+  const double* wc = w + (size_t)cell * kNB;
+  const double* xc = X + (size_t)cell * kNB * N;
+  if (sig)
+    for (int e = threadIdx.x; e < ne; e += blockDim.x) {
+      double v = 0.0;
+#pragma unroll
+      for (int k = 0; k < 12; ++k) v += wc[k] * xc[(size_t)k * N + e];
+      sig[(size_t)cell * ne + e] = v;
+    }
+  if (u)
+    for (int f = threadIdx.x; f < N - ne; f += blockDim.x) {
+      double v = 0.0;
+#pragma unroll
+      for (int k = 12; k < kNB; ++k) v += wc[k] * xc[(size_t)k * N + ne + f];
+      u[(size_t)cell * (N - ne) + f] = v;
+    }
+}
+
+msfec_status nrt_fail(msfec_status st, const std::string& msg) {
+  msfec_detail::g_last_error = msg;
+  return st;
+}
+
+#define NRT_HIP(call)                                                                     \
+  do {                                                                                    \
+    hipError_t e_ = (call);                                                               \
+    if (e_ != hipSuccess)                                                                 \
+      return nrt_fail(MSFEC_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// The fine dof tables of one coarse cell (they depend on l only).
+struct NrtTables {
+  int m = 0, nx = 0, ne = 0, nf1 = 0, N = 0, n = 0, ns = 0;
+  std::vector<int> icell, cdof, f2i;
+  std::vector<double> bval;
+};
+
+NrtTables nrt_tables(int l) {
+  NrtTables t;
+  const int m = 1 << l;
+  t.m = m;
+  t.nx = m * (m + 1) * (m + 1);
+  t.ne = 3 * t.nx;
+  t.nf1 = m * m * (m + 1);
+  t.N = t.ne + 3 * t.nf1;
+  auto ex = [&](int i, int j, int k) { return (k * (m + 1) + j) * m + i; };
+  auto ey = [&](int i, int j, int k) { return t.nx + (k * m + j) * (m + 1) + i; };
+  auto ez = [&](int i, int j, int k) { return 2 * t.nx + (k * (m + 1) + j) * (m + 1) + i; };
+  auto fx = [&](int i, int j, int k) { return t.ne + (k * m + j) * (m + 1) + i; };
+  auto fy = [&](int i, int j, int k) { return t.ne + t.nf1 + (k * (m + 1) + j) * m + i; };
+  auto fz = [&](int i, int j, int k) { return t.ne + 2 * t.nf1 + (k * m + j) * m + i; };
+  static const int ax[12] = {1, 1, 0, 0, 1, 1, 0, 0, 2, 2, 2, 2};
+  static const int f1[12] = {0, 1, 0, 1, 0, 1, 0, 1, 0, 1, 0, 1};
+  static const int f2[12] = {0, 0, 0, 0, 1, 1, 1, 1, 0, 0, 1, 1};
+  const int nfc = m * m * m;
+  t.cdof.assign((size_t)nfc * 18, 0);
+  for (int c3 = 0; c3 < m; ++c3)
+    for (int b = 0; b < m; ++b)
+      for (int a = 0; a < m; ++a) {
+        int* d = &t.cdof[(size_t)((c3 * m + b) * m + a) * 18];
+        for (int L = 0; L < 12; ++L) {
+          if (ax[L] == 0) d[L] = ex(a, b + f1[L], c3 + f2[L]);
+          else if (ax[L] == 1) d[L] = ey(a + f1[L], b, c3 + f2[L]);
+          else d[L] = ez(a + f1[L], b + f2[L], c3);
+        }
+        d[12] = fx(a, b, c3);
+        d[13] = fx(a + 1, b, c3);
+        d[14] = fy(a, b, c3);
+        d[15] = fy(a, b + 1, c3);
+        d[16] = fz(a, b, c3);
+        d[17] = fz(a, b, c3 + 1);
+      }
+  // position of every full dof: type (0..2 edges along x/y/z, 3..5 faces
+  // normal to x/y/z) and its integer coordinates
+  t.f2i.assign(t.N, -1);
+  t.bval.assign((size_t)t.N * kNB, 0.0);
+  auto lagf = [](int s, double x) { return s ? x : 1.0 - x; };
+  for (int f = 0; f < t.N; ++f) {
+    int ty, p[3];
+    if (f < t.ne) {
+      ty = f / t.nx;
+      int r = f % t.nx;
+      int ext[3] = {m + 1, m + 1, m + 1};
+      ext[ty] = m;
+      p[0] = r % ext[0];
+      r /= ext[0];
+      p[1] = r % ext[1];
+      p[2] = r / ext[1];
+    } else {
+      ty = 3 + (f - t.ne) / t.nf1;
+      int r = (f - t.ne) % t.nf1;
+      int ext[3] = {m, m, m};
+      ext[ty - 3] = m + 1;
+      p[0] = r % ext[0];
+      r /= ext[0];
+      p[1] = r % ext[1];
+      p[2] = r / ext[1];
+    }
+    bool bnd;
+    if (ty < 3) {
+      bnd = false;
+      for (int d = 0; d < 3; ++d)
+        if (d != ty && (p[d] == 0 || p[d] == m)) bnd = true;
+    } else {
+      bnd = p[ty - 3] == 0 || p[ty - 3] == m;
+    }
+    if (!bnd) {
+      t.f2i[f] = t.n++;
+      continue;
+    }
+    // fixed values: line integral of coarse Nedelec function L along a
+    // boundary edge, flux of coarse Raviart-Thomas function F through a
+    // boundary face
+    for (int L = 0; L < 12; ++L) {
+      if (ty != ax[L]) continue;
+      const int o1 = ax[L] == 0 ? 1 : 0, o2 = ax[L] == 2 ? 1 : 2;
+      t.bval[(size_t)f * kNB + L] = lagf(f1[L], (double)p[o1] / m) * lagf(f2[L], (double)p[o2] / m) / m;
+    }
+    if (ty >= 3) {
+      const int axf = ty - 3;
+      const int F = 2 * axf + (p[axf] == m ? 1 : 0);
+      t.bval[(size_t)f * kNB + 12 + F] = 1.0 / ((double)m * m);
+    }
+  }
+  t.ns = 0;
+  for (int f = 0; f < t.ne; ++f)
+    if (t.f2i[f] >= 0) ++t.ns;
+  t.icell.assign((size_t)std::max(t.n, 1) * 4, -1);
+  std::vector<int> cnt(t.n, 0);
+  for (int c = 0; c < nfc; ++c)
+    for (int a = 0; a < 18; ++a) {
+      const int I = t.f2i[t.cdof[(size_t)c * 18 + a]];
+      if (I < 0) continue;
+      t.icell[(size_t)I * 4 + cnt[I]++] = c * 32 + a;
+    }
+  return t;
+}
+
+}  // namespace
+
+struct msfec_nrt_batch {
+  msfec_nrt_params prm{};
+  int n_cells = 0, l = 0;
+  NrtTables tab;
+  double* d_verts = nullptr;
+  double* d_ainv = nullptr;
+  double* d_b = nullptr;
+  double* d_r = nullptr;
+  double* d_f = nullptr;
+  int* d_icell = nullptr;
+  int* d_cdof = nullptr;
+  int* d_f2i = nullptr;
+  double* d_bval = nullptr;
+  double* d_X = nullptr;
+  double* d_M = nullptr;
+  double* d_rhs = nullptr;
+  double* d_w = nullptr;
+  int* d_status = nullptr;
+  bool coef_set = false, has_r = false, has_f = false, computed = false;
+  hipStream_t stream = nullptr;
+  std::vector<std::array<hipEvent_t, 2>> ev_sets;
+  size_t ev_used = 0;
+};
+
+namespace {
+
+template <typename T>
+msfec_status nrt_alloc(T** p, size_t count) {
+  NRT_HIP(hipMalloc((void**)p, std::max<size_t>(count, 1) * sizeof(T)));
+  return MSFEC_OK;
+}
+
+void nrt_free(msfec_nrt_batch* h) {
+  void* ptrs[] = {h->d_verts, h->d_ainv, h->d_b, h->d_r,   h->d_f,   h->d_icell,  h->d_cdof,
+                  h->d_f2i,   h->d_bval, h->d_X, h->d_M,   h->d_rhs, h->d_w,      h->d_status};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  for (auto& set : h->ev_sets)
+    for (auto& e : set)
+      if (e) (void)hipEventDestroy(e);
+  h->ev_sets.clear();
+}
+
+// Copied on the handle's stream (that of its last compute call), waited for.
+msfec_status nrt_in(hipStream_t s, void* dst, const void* src, size_t bytes, int loc) {
+  NRT_HIP(hipMemcpyAsync(dst, src, bytes,
+                         loc == MSFEC_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
+  NRT_HIP(hipStreamSynchronize(s));
+  return MSFEC_OK;
+}
+
+msfec_status nrt_out(hipStream_t s, double* dst, const double* src, size_t count, int loc) {
+  if (!dst) return nrt_fail(MSFEC_ERR_INVALID, "null output pointer");
+  if (loc != MSFEC_HOST && loc != MSFEC_DEVICE) return nrt_fail(MSFEC_ERR_INVALID, "bad location");
+  NRT_HIP(hipMemcpyAsync(dst, src, count * sizeof(double),
+                         loc == MSFEC_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
+  NRT_HIP(hipStreamSynchronize(s));
+  return MSFEC_OK;
+}
+
+template <int L>
+msfec_status nrt_launch(msfec_nrt_batch* h, hipStream_t s) {
+  typedef Nd<L> D;
+  const size_t lds = (size_t)D::total * sizeof(double);
+  NRT_HIP(hipFuncSetAttribute((const void*)nrt_basis_kernel<L>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  NrtArgs a{};
+  a.verts = h->d_verts;
+  a.ainv = h->d_ainv;
+  a.bq = h->d_b;
+  a.rq = h->has_r ? h->d_r : nullptr;
+  a.fq = h->has_f ? h->d_f : nullptr;
+  a.icell = h->d_icell;
+  a.cdof = h->d_cdof;
+  a.f2i = h->d_f2i;
+  a.bval = h->d_bval;
+  a.X = h->d_X;
+  a.M = h->d_M;
+  a.b = h->d_rhs;
+  a.status = h->d_status;
+  a.N = h->tab.N;
+  hipLaunchKernelGGL(nrt_basis_kernel<L>, dim3(h->n_cells), dim3(D::NT), lds, s, a);
+  NRT_HIP(hipGetLastError());
+  return MSFEC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+msfec_status msfec_nrt_create(const msfec_nrt_params* params, int n_cells, const double* vertices,
+                              int vertices_loc, msfec_nrt_handle* out) {
+  if (!params || !vertices || !out) return nrt_fail(MSFEC_ERR_INVALID, "null argument");
+  *out = nullptr;
+  if (n_cells <= 0) return nrt_fail(MSFEC_ERR_INVALID, "n_cells must be positive");
+  const int l = params->n_refine_local;
+  if (l < 1) return nrt_fail(MSFEC_ERR_INVALID, "n_refine_local must be at least 1");
+  if (l > kMaxL)
+    return nrt_fail(MSFEC_ERR_UNSUPPORTED,
+                    "n_refine_local above 2: the dense interior system exceeds 256 unknowns");
+  if (vertices_loc != MSFEC_HOST && vertices_loc != MSFEC_DEVICE)
+    return nrt_fail(MSFEC_ERR_INVALID, "bad vertices location");
+  std::vector<double> hv((size_t)n_cells * 24);
+  if (vertices_loc == MSFEC_DEVICE) {
+    NRT_HIP(hipMemcpy(hv.data(), vertices, hv.size() * sizeof(double), hipMemcpyDeviceToHost));
+  } else {
+    std::copy(vertices, vertices + hv.size(), hv.begin());
+  }
+  for (int c = 0; c < n_cells; ++c) {
+    const double* v = &hv[(size_t)c * 24];
+    bool box = true;
+    for (int d = 0; d < 3; ++d) {
+      const double Hd = v[3 * (1 << d) + d] - v[d];
+      if (!(Hd > 0)) box = false;
+      for (int k = 0; k < 8 && box; ++k)
+        if (std::fabs(v[3 * k + d] - (v[d] + ((k >> d) & 1) * Hd)) > 1e-14 * Hd) box = false;
+    }
+    if (!box)
+      return nrt_fail(MSFEC_ERR_INVALID,
+                      "cell " + std::to_string(c) + " is not an axis-aligned box in deal.II vertex order");
+  }
+  msfec_nrt_batch* h = new msfec_nrt_batch();
+  h->prm = *params;
+  h->n_cells = n_cells;
+  h->l = l;
+  h->tab = nrt_tables(l);
+  const size_t nc = (size_t)n_cells, nfc = (size_t)1 << (3 * l);
+  msfec_status st = MSFEC_OK;
+#define NRT_TRY(x)        \
+  do {                    \
+    st = (x);             \
+    if (st != MSFEC_OK) { \
+      nrt_free(h);        \
+      delete h;           \
+      return st;          \
+    }                     \
+  } while (0)
+  NRT_TRY(nrt_alloc(&h->d_verts, nc * 24));
+  NRT_TRY(nrt_alloc(&h->d_ainv, nc * nfc * 8 * 6));
+  NRT_TRY(nrt_alloc(&h->d_b, nc * nfc * 8));
+  NRT_TRY(nrt_alloc(&h->d_r, nc * nfc * 8));
+  NRT_TRY(nrt_alloc(&h->d_f, nc * nfc * 8 * 3));
+  NRT_TRY(nrt_alloc(&h->d_icell, h->tab.icell.size()));
+  NRT_TRY(nrt_alloc(&h->d_cdof, h->tab.cdof.size()));
+  NRT_TRY(nrt_alloc(&h->d_f2i, h->tab.f2i.size()));
+  NRT_TRY(nrt_alloc(&h->d_bval, h->tab.bval.size()));
+  NRT_TRY(nrt_alloc(&h->d_X, nc * kNB * h->tab.N));
+  NRT_TRY(nrt_alloc(&h->d_M, nc * kNB * kNB));
+  NRT_TRY(nrt_alloc(&h->d_rhs, nc * kNB));
+  NRT_TRY(nrt_alloc(&h->d_w, nc * kNB));
+  NRT_TRY(nrt_alloc(&h->d_status, nc));
+  NRT_TRY(nrt_in(h->stream, h->d_verts, hv.data(), hv.size() * sizeof(double), MSFEC_HOST));
+  NRT_TRY(nrt_in(h->stream, h->d_icell, h->tab.icell.data(), h->tab.icell.size() * sizeof(int), MSFEC_HOST));
+  NRT_TRY(nrt_in(h->stream, h->d_cdof, h->tab.cdof.data(), h->tab.cdof.size() * sizeof(int), MSFEC_HOST));
+  NRT_TRY(nrt_in(h->stream, h->d_f2i, h->tab.f2i.data(), h->tab.f2i.size() * sizeof(int), MSFEC_HOST));
+  NRT_TRY(nrt_in(h->stream, h->d_bval, h->tab.bval.data(), h->tab.bval.size() * sizeof(double), MSFEC_HOST));
+#undef NRT_TRY
+  *out = h;
+  return MSFEC_OK;
+}
+
+msfec_status msfec_nrt_set_coefficients(msfec_nrt_handle h, const double* ainv, const double* b,
+                                        const double* r, const double* f, int loc) {
+  if (!h || !ainv || !b) return nrt_fail(MSFEC_ERR_INVALID, "null argument");
+  if (loc != MSFEC_HOST && loc != MSFEC_DEVICE) return nrt_fail(MSFEC_ERR_INVALID, "bad location");
+  const size_t pts = (size_t)h->n_cells * ((size_t)1 << (3 * h->l)) * 8;
+  msfec_status st = nrt_in(h->stream, h->d_ainv, ainv, pts * 6 * sizeof(double), loc);
+  if (st == MSFEC_OK) st = nrt_in(h->stream, h->d_b, b, pts * sizeof(double), loc);
+  if (st == MSFEC_OK && r) st = nrt_in(h->stream, h->d_r, r, pts * sizeof(double), loc);
+  if (st == MSFEC_OK && f) st = nrt_in(h->stream, h->d_f, f, pts * 3 * sizeof(double), loc);
+  if (st != MSFEC_OK) return st;
+  h->has_r = r != nullptr;
+  h->has_f = f != nullptr;
+  h->coef_set = true;
+  return MSFEC_OK;
+}
+
+msfec_status msfec_nrt_compute(msfec_nrt_handle h, void* stream) {
+  if (!h) return nrt_fail(MSFEC_ERR_INVALID, "null handle");
+  if (!h->coef_set) return nrt_fail(MSFEC_ERR_STATE, "coefficients not set");
+  hipStream_t s = (hipStream_t)stream;
+  h->stream = s;
+  std::array<hipEvent_t, 2>* ev = nullptr;
+  if (h->prm.flags & MSFEC_FLAG_TIMING) {
+    if (h->ev_used == h->ev_sets.size()) {
+      std::array<hipEvent_t, 2> set{};
+      for (auto& e : set) NRT_HIP(hipEventCreate(&e));
+      h->ev_sets.push_back(set);
+    }
+    ev = &h->ev_sets[h->ev_used++];
+    NRT_HIP(hipEventRecord((*ev)[0], s));
+  }
+  const msfec_status st = h->l == 1 ? nrt_launch<1>(h, s) : nrt_launch<2>(h, s);
+  if (st != MSFEC_OK) return st;
+  if (ev) NRT_HIP(hipEventRecord((*ev)[1], s));
+  h->computed = true;
+  return MSFEC_OK;
+}
+
+msfec_status msfec_nrt_synchronize(msfec_nrt_handle h) {
+  if (!h) return nrt_fail(MSFEC_ERR_INVALID, "null handle");
+  NRT_HIP(hipStreamSynchronize(h->stream));
+  return MSFEC_OK;
+}
+
+msfec_status msfec_nrt_get_element(msfec_nrt_handle h, double* M, double* b, int loc) {
+  if (!h) return nrt_fail(MSFEC_ERR_INVALID, "null handle");
+  if (!h->computed) return nrt_fail(MSFEC_ERR_STATE, "compute not called");
+  msfec_status st = MSFEC_OK;
+  if (M) st = nrt_out(h->stream, M, h->d_M, (size_t)h->n_cells * kNB * kNB, loc);
+  if (st == MSFEC_OK && b) st = nrt_out(h->stream, b, h->d_rhs, (size_t)h->n_cells * kNB, loc);
+  return st;
+}
+
+msfec_status msfec_nrt_get_basis(msfec_nrt_handle h, double* X, int loc) {
+  if (!h) return nrt_fail(MSFEC_ERR_INVALID, "null handle");
+  if (!h->computed) return nrt_fail(MSFEC_ERR_STATE, "compute not called");
+  return nrt_out(h->stream, X, h->d_X, (size_t)h->n_cells * kNB * h->tab.N, loc);
+}
+
+msfec_status msfec_nrt_reconstruct(msfec_nrt_handle h, const double* w, int w_loc, double* sigma,
+                                   double* u, int out_loc) {
+  if (!h || !w) return nrt_fail(MSFEC_ERR_INVALID, "null argument");
+  if (!sigma && !u) return nrt_fail(MSFEC_ERR_INVALID, "no output requested");
+  if (w_loc != MSFEC_HOST && w_loc != MSFEC_DEVICE) return nrt_fail(MSFEC_ERR_INVALID, "bad location");
+  if (out_loc != MSFEC_HOST && out_loc != MSFEC_DEVICE)
+    return nrt_fail(MSFEC_ERR_INVALID, "bad location");
+  if (!h->computed) return nrt_fail(MSFEC_ERR_STATE, "compute not called");
+  msfec_status st = nrt_in(h->stream, h->d_w, w, (size_t)h->n_cells * kNB * sizeof(double), w_loc);
+  if (st != MSFEC_OK) return st;
+  const int ne = h->tab.ne, nf = h->tab.N - h->tab.ne;
+  const size_t ns = (size_t)h->n_cells * ne, nu = (size_t)h->n_cells * nf;
+  double* ds = nullptr;
+  double* du = nullptr;
+  // device outputs are written in place; host outputs go through staging
+  // buffers freed before returning
+  if (sigma) {
+    if (out_loc == MSFEC_DEVICE) ds = sigma;
+    else NRT_HIP(hipMalloc((void**)&ds, ns * sizeof(double)));
+  }
+  if (u) {
+    if (out_loc == MSFEC_DEVICE) {
+      du = u;
+    } else if (hipMalloc((void**)&du, nu * sizeof(double)) != hipSuccess) {
+      if (ds && out_loc != MSFEC_DEVICE) (void)hipFree(ds);
+      return nrt_fail(MSFEC_ERR_HIP, "hipMalloc of the reconstruction staging buffer failed");
+    }
+  }
+  hipLaunchKernelGGL(nrt_reconstruct_kernel, dim3(h->n_cells), dim3(256), 0, h->stream, h->d_X,
+                     h->d_w, h->tab.N, ne, ds, du);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  if (e == hipSuccess && out_loc == MSFEC_HOST) {
+    if (sigma) e = hipMemcpy(sigma, ds, ns * sizeof(double), hipMemcpyDeviceToHost);
+    if (e == hipSuccess && u) e = hipMemcpy(u, du, nu * sizeof(double), hipMemcpyDeviceToHost);
+  }
+  if (out_loc == MSFEC_HOST) {
+    if (ds) (void)hipFree(ds);
+    if (du) (void)hipFree(du);
+  }
+  if (e != hipSuccess) return nrt_fail(MSFEC_ERR_HIP, std::string("reconstruct: ") + hipGetErrorString(e));
+  return MSFEC_OK;
+}
+
+msfec_status msfec_nrt_get_cell_status(msfec_nrt_handle h, int* status) {
+  if (!h || !status) return nrt_fail(MSFEC_ERR_INVALID, "null argument");
+  if (!h->computed) return nrt_fail(MSFEC_ERR_STATE, "compute not called");
+  NRT_HIP(hipStreamSynchronize(h->stream));
+  NRT_HIP(hipMemcpy(status, h->d_status, (size_t)h->n_cells * sizeof(int), hipMemcpyDeviceToHost));
+  return MSFEC_OK;
+}
+
+msfec_status msfec_nrt_kernel_times(msfec_nrt_handle h, float* ms, int max, int* n) {
+  if (!h || !ms) return nrt_fail(MSFEC_ERR_INVALID, "null argument");
+  if (h->ev_used == 0) return nrt_fail(MSFEC_ERR_STATE, "no timed compute (set MSFEC_FLAG_TIMING)");
+  double acc = 0.0;
+  for (size_t c = 0; c < h->ev_used; ++c) {
+    NRT_HIP(hipEventSynchronize(h->ev_sets[c][1]));
+    float t = 0.f;
+    NRT_HIP(hipEventElapsedTime(&t, h->ev_sets[c][0], h->ev_sets[c][1]));
+    acc += t;
+  }
+  if (max > 0) ms[0] = (float)(acc / (double)h->ev_used);
+  if (n) *n = max > 0 ? 1 : 0;
+  h->ev_used = 0;
+  return MSFEC_OK;
+}
+
+msfec_status msfec_nrt_destroy(msfec_nrt_handle h) {
+  if (!h) return MSFEC_OK;
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  nrt_free(h);
+  delete h;
+  return MSFEC_OK;
+}
+
+}  // extern "C"

@@ -1,5 +1,6 @@
-"""MI355X multiscale basis construction (MsFEC Q1, and the 2-D Q1 x Nedelec and
-Raviart-Thomas x DQ0 pairings), Python face.
+"""MI355X multiscale basis construction (MsFEC Q1, the 2-D Q1 x Nedelec and
+Raviart-Thomas x DQ0 pairings and the 3-D Nedelec x Raviart-Thomas pairing),
+Python face.
 
 The compute path is libmsfec_hip.so (hand-written HIP for gfx950) behind the
 C-ABI in include/msfec.h; this package only binds it (ctypes) and generates
@@ -9,8 +10,10 @@ from . import _lib
 from ._lib import MsfecError
 from .device import DeviceBuffer
 from .ned import NedBasisBatch
+from .nrt import NedRTBasisBatch
 from .qbasis import QBasisBatch
 from .rt import RTDQBasisBatch
 from .uneven import UnevenBatch
 
-__all__ = ["DeviceBuffer", "QBasisBatch", "NedBasisBatch", "RTDQBasisBatch", "UnevenBatch", "MsfecError", "_lib"]
+__all__ = ["DeviceBuffer", "QBasisBatch", "NedBasisBatch", "RTDQBasisBatch", "NedRTBasisBatch",
+           "UnevenBatch", "MsfecError", "_lib"]

@@ -12,7 +12,7 @@ CSRC = os.path.join(PKG_ROOT, "csrc")
 INCLUDE = os.path.join(REPO, "include")
 LIB = os.path.join(HERE, "libmsfec_hip.so")
 
-SOURCES = ["msfec_q.hip", "msfec_ned.hip", "msfec_rt.hip", "msfec_uneven.hip"]
+SOURCES = ["msfec_q.hip", "msfec_ned.hip", "msfec_rt.hip", "msfec_uneven.hip", "msfec_nrt.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

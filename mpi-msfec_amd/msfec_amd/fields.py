@@ -106,25 +106,85 @@ def qp_coordinates(vertices: np.ndarray, n_refine: int) -> np.ndarray:
     return pts
 
 
+def _euler_rotation(rotate=True) -> np.ndarray:
+    """The fixed ZXZ Euler rotation of eqn_coeff_A.cc:25-41 (pi/3, pi/6, pi/4)."""
+    a, b, g = np.pi / 3, np.pi / 6, np.pi / 4
+    if not rotate:
+        return np.eye(3)
+    return np.array([
+        [np.cos(a) * np.cos(g) - np.sin(a) * np.cos(b) * np.sin(g),
+         -np.cos(a) * np.sin(g) - np.sin(a) * np.cos(b) * np.cos(g), np.sin(a) * np.sin(b)],
+        [np.sin(a) * np.cos(g) + np.cos(a) * np.cos(b) * np.sin(g),
+         -np.sin(a) * np.sin(g) + np.cos(a) * np.cos(b) * np.cos(g), -np.cos(a) * np.sin(b)],
+        [np.sin(b) * np.sin(g), np.sin(b) * np.cos(g), np.cos(b)]])
+
+
+def _sym6(A: np.ndarray) -> np.ndarray:
+    return np.stack([A[..., 0, 0], A[..., 0, 1], A[..., 0, 2], A[..., 1, 1], A[..., 1, 2],
+                     A[..., 2, 2]], -1)
+
+
 def diffusion_A_tensor(points: np.ndarray, freq=(17, 16, 15), scale=(1.0, 1.0, 1.0),
                        alpha=(0.9, 0.9, 0.9), rotate=True) -> np.ndarray:
     """Diffusion_A of source/equation_data/eqn_coeff_A.cc:163-191 at 3-D points,
     returned as the symmetric 6-vector (xx,xy,xz,yy,yz,zz) per point."""
-    a, b, g = np.pi / 3, np.pi / 6, np.pi / 4
-    if rotate:
-        R = np.array([
-            [np.cos(a) * np.cos(g) - np.sin(a) * np.cos(b) * np.sin(g),
-             -np.cos(a) * np.sin(g) - np.sin(a) * np.cos(b) * np.cos(g), np.sin(a) * np.sin(b)],
-            [np.sin(a) * np.cos(g) + np.cos(a) * np.cos(b) * np.sin(g),
-             -np.sin(a) * np.sin(g) + np.cos(a) * np.cos(b) * np.cos(g), -np.cos(a) * np.sin(b)],
-            [np.sin(b) * np.sin(g), np.sin(b) * np.cos(g), np.cos(b)]])
-    else:
-        R = np.eye(3)
+    R = _euler_rotation(rotate)
     D = np.stack([scale[d] * (1.0 - alpha[d] * np.sin(2 * np.pi * freq[d] * points[..., d]))
                   for d in range(3)], -1)
-    A = np.einsum("ik,...k,jk->...ij", R, D, R)
-    return np.stack([A[..., 0, 0], A[..., 0, 1], A[..., 0, 2], A[..., 1, 1], A[..., 1, 2],
-                     A[..., 2, 2]], -1)
+    return _sym6(np.einsum("ik,...k,jk->...ij", R, D, R))
+
+
+def diffusion_inverse_A_tensor(points: np.ndarray, freq=(17, 16, 15), scale=(1.0, 1.0, 1.0),
+                               alpha=(0.9, 0.9, 0.9), rotate=True) -> np.ndarray:
+    """DiffusionInverse_A (eqn_coeff_A.cc:194-242): the rotated diagonal of
+    reciprocals, as the symmetric 6-vector per point."""
+    R = _euler_rotation(rotate)
+    D = np.stack([1.0 / (scale[d] * (1.0 - alpha[d] * np.sin(2 * np.pi * freq[d] * points[..., d])))
+                  for d in range(3)], -1)
+    return _sym6(np.einsum("ik,...k,jk->...ij", R, D, R))
+
+
+def diffusion_B(points: np.ndarray, freq=14, scale=1.0, alpha=0.9) -> np.ndarray:
+    """Diffusion_B with the expression of the Ned_RT examples
+    (example_parameters/prm_ned_rt_test-01.prm: scale (1 - alpha sin(2 pi f x)))."""
+    return scale * (1.0 - alpha * np.sin(2 * np.pi * freq * points[..., 0]))
+
+
+def ned_rt_rhs(points: np.ndarray, scale=100.0) -> np.ndarray:
+    """The right-hand side of prm_ned_rt_test-01.prm: scale times the gradient
+    of x y z (x-1) (y-1) (z-1)."""
+    x, y, z = points[..., 0], points[..., 1], points[..., 2]
+    return scale * np.stack([(2 * x - 1) * (y * y - y) * (z * z - z),
+                             (2 * y - 1) * (x * x - x) * (z * z - z),
+                             (2 * z - 1) * (x * x - x) * (y * y - y)], -1)
+
+
+def gauss_points_3d(verts: np.ndarray, l: int) -> np.ndarray:
+    """Physical QGauss(2) points of every fine cell of every box: verts
+    (n, 8, 3) in deal.II vertex order -> (n, 8^l, 8, 3), fine cells x fastest,
+    points x fastest."""
+    verts = np.asarray(verts, dtype=np.float64).reshape(-1, 8, 3)
+    m = 1 << l
+    g = np.array([0.5 - 0.5 / np.sqrt(3), 0.5 + 0.5 / np.sqrt(3)])
+    c = np.arange(m ** 3)
+    cell = np.stack([c % m, (c // m) % m, c // (m * m)], -1).astype(np.float64)   # (m^3, 3)
+    q = np.arange(8)
+    qp = np.stack([g[q & 1], g[(q >> 1) & 1], g[q >> 2]], -1)                    # (8, 3)
+    ref = (cell[:, None, :] + qp[None, :, :]) / m                                   # (m^3, 8, 3)
+    lo = verts[:, 0, :]
+    H = np.stack([verts[:, 1, 0] - lo[:, 0], verts[:, 2, 1] - lo[:, 1], verts[:, 4, 2] - lo[:, 2]], -1)
+    return lo[:, None, None, :] + ref[None] * H[:, None, None, :]
+
+
+def box_grid_3d(n: int) -> np.ndarray:
+    """The n^3 boxes of the unit cube (x fastest), vertices (n^3, 8, 3) in deal.II order."""
+    h = 1.0 / n
+    out = np.empty((n ** 3, 8, 3))
+    for c in range(n ** 3):
+        i, j, k = c % n, (c // n) % n, c // (n * n)
+        for v in range(8):
+            out[c, v] = [(i + (v & 1)) * h, (j + ((v >> 1) & 1)) * h, (k + ((v >> 2) & 1)) * h]
+    return out
 
 
 def inclusion_centres(dim: int, n_inclusions: int, seed: int) -> np.ndarray:

@@ -18,7 +18,11 @@ from . import _lib as L
 
 
 def _ptr(a):
-    """(pointer, loc, keepalive) for a numpy array or a torch tensor."""
+    """(pointer, loc, keepalive) for a numpy array, a torch tensor or a
+    DeviceBuffer."""
+    from .device import DeviceBuffer
+    if isinstance(a, DeviceBuffer):
+        return a.ptr, L.DEVICE, a
     if hasattr(a, "data_ptr") and hasattr(a, "is_cuda"):
         if a.is_cuda:
             if not a.is_contiguous():

@@ -5,7 +5,8 @@
  * This follows the reference's algorithm: assemble the mixed system over all
  * fine dofs with QGauss(2) and the deal.II lowest-order shape functions, fix
  * the boundary dofs to the traces of the coarse function of each basis,
- * condense, solve directly (LU with partial pivoting here, UMFPACK there) and
+ * condense, solve directly (LU with partial pivoting and two steps of
+ * iterative refinement here, UMFPACK there) and
  * form the element matrix from the assembled matrix by the reference's block
  * rule.  The GPU library solves the same systems with a different
  * factorisation (symmetric indefinite, tangential dofs first) and builds the
@@ -316,51 +317,98 @@ static double nrt_boundary_value(const nrt_mesh *g, int t, const int *p, int n) 
   return 1.0 / ((double)m * m);
 }
 
-/* LU with partial pivoting of the n x n row-major matrix A, solving for nr
- * right-hand sides stored as B[n][nr]; returns 2 on a zero pivot */
-static int nrt_lu_solve(int n, double *A, int nr, double *B) {
+/* LU with partial pivoting of the n x n row-major matrix A in place (the
+ * multipliers below the diagonal, the row exchanges in piv); returns 2 on a
+ * zero pivot */
+static int nrt_lu(int n, double *A, int *piv) {
   for (int k = 0; k < n; ++k) {
-    int piv = k;
+    int p = k;
     double best = fabs(A[(size_t)k * n + k]);
     for (int i = k + 1; i < n; ++i) {
       const double v = fabs(A[(size_t)i * n + k]);
       if (v > best) {
         best = v;
-        piv = i;
+        p = i;
       }
     }
     if (!(best > 0.0)) return 2;
-    if (piv != k) {
+    piv[k] = p;
+    if (p != k)
       for (int j = 0; j < n; ++j) {
         const double t = A[(size_t)k * n + j];
-        A[(size_t)k * n + j] = A[(size_t)piv * n + j];
-        A[(size_t)piv * n + j] = t;
+        A[(size_t)k * n + j] = A[(size_t)p * n + j];
+        A[(size_t)p * n + j] = t;
       }
-      for (int r = 0; r < nr; ++r) {
-        const double t = B[(size_t)k * nr + r];
-        B[(size_t)k * nr + r] = B[(size_t)piv * nr + r];
-        B[(size_t)piv * nr + r] = t;
-      }
-    }
     const double inv = 1.0 / A[(size_t)k * n + k];
     const double *rk = A + (size_t)k * n;
     for (int i = k + 1; i < n; ++i) {
-      const double lik = A[(size_t)i * n + k] * inv;
-      if (lik == 0.0) continue;
       double *ri = A + (size_t)i * n;
+      const double lik = ri[k] * inv;
+      ri[k] = lik;
+      if (lik == 0.0) continue;
       for (int j = k + 1; j < n; ++j) ri[j] -= lik * rk[j];
-      for (int r = 0; r < nr; ++r) B[(size_t)i * nr + r] -= lik * B[(size_t)k * nr + r];
-    }
-  }
-  for (int k = n - 1; k >= 0; --k) {
-    const double inv = 1.0 / A[(size_t)k * n + k];
-    for (int r = 0; r < nr; ++r) {
-      double s = B[(size_t)k * nr + r];
-      for (int j = k + 1; j < n; ++j) s -= A[(size_t)k * n + j] * B[(size_t)j * nr + r];
-      B[(size_t)k * nr + r] = s * inv;
     }
   }
   return 0;
+}
+
+/* x = A^-1 b with the factors of nrt_lu, in place on b (n values): the row
+ * exchanges (whole rows, multipliers included, were swapped), then the unit
+ * lower and the upper triangular solves */
+static void nrt_lu_apply(int n, const double *LU, const int *piv, double *b) {
+  for (int k = 0; k < n; ++k)
+    if (piv[k] != k) {
+      const double t = b[k];
+      b[k] = b[piv[k]];
+      b[piv[k]] = t;
+    }
+  for (int k = 0; k < n; ++k) {
+    const double bk = b[k];
+    if (bk != 0.0)
+      for (int i = k + 1; i < n; ++i) b[i] -= LU[(size_t)i * n + k] * bk;
+  }
+  for (int k = n - 1; k >= 0; --k) {
+    double s = b[k];
+    for (int j = k + 1; j < n; ++j) s -= LU[(size_t)k * n + j] * b[j];
+    b[k] = s / LU[(size_t)k * n + k];
+  }
+}
+
+/* Solves S x = b for nr right-hand sides stored as B[n][nr]: LU with partial
+ * pivoting, then two steps of iterative refinement with FP64 residuals of the
+ * unfactored S (the pivot growth of the indefinite mixed systems otherwise
+ * costs a digit or two on strongly varying coefficients); returns 2 on a zero
+ * pivot, 3 when memory runs out */
+static int nrt_solve(int n, const double *S, int nr, double *B) {
+  double *LU = (double *)malloc(sizeof(double) * (size_t)n * n);
+  int *piv = (int *)malloc(sizeof(int) * n);
+  double *x = (double *)malloc(sizeof(double) * n);
+  double *r = (double *)malloc(sizeof(double) * n);
+  int st = (LU && piv && x && r) ? 0 : 3;
+  if (!st) {
+    memcpy(LU, S, sizeof(double) * (size_t)n * n);
+    st = nrt_lu(n, LU, piv);
+  }
+  for (int k = 0; !st && k < nr; ++k) {
+    for (int i = 0; i < n; ++i) x[i] = B[(size_t)i * nr + k];
+    nrt_lu_apply(n, LU, piv, x);
+    for (int it = 0; it < 2; ++it) {
+      for (int i = 0; i < n; ++i) {
+        double s = B[(size_t)i * nr + k];
+        const double *si = S + (size_t)i * n;
+        for (int j = 0; j < n; ++j) s -= si[j] * x[j];
+        r[i] = s;
+      }
+      nrt_lu_apply(n, LU, piv, r);
+      for (int i = 0; i < n; ++i) x[i] += r[i];
+    }
+    for (int i = 0; i < n; ++i) B[(size_t)i * nr + k] = x[i];
+  }
+  free(LU);
+  free(piv);
+  free(x);
+  free(r);
+  return st;
 }
 
 int oq_nrt_assemble(int l, const double *verts, const double *ainv, const double *bq,
@@ -423,7 +471,7 @@ int oq_nrt_basis_cell(int l, const double *verts, const double *ainv, const doub
         }
       }
     }
-    st = nrt_lu_solve(n, S, 18, B);
+    st = nrt_solve(n, S, 18, B);
   }
   if (!st) {
     for (int d = 0; d < N; ++d)

@@ -257,3 +257,30 @@ def test_uneven_create_rejects_on_the_host(lib):
         st = L.msfec_qu_create(ctypes.byref(p), 2, ctypes.c_void_p(verts.ctypes.data), lib.HOST,
                                lv.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), ctypes.byref(h))
         assert st == want, (fields, levels, st, L.msfec_last_error())
+
+
+def _nrt_create(lib, l=2, verts=None):
+    L = lib.load()
+    p = lib.NrtParams()
+    p.n_refine_local, p.flags = l, 0
+    if verts is None:
+        verts = box_vertices(3, [0.0] * 3, [1.0] * 3)[None]
+    verts = np.ascontiguousarray(verts, dtype=np.float64)
+    h = ctypes.c_void_p()
+    st = L.msfec_nrt_create(ctypes.byref(p), verts.shape[0], ctypes.c_void_p(verts.ctypes.data),
+                            lib.HOST, ctypes.byref(h))
+    return st, L.msfec_last_error().decode()
+
+
+def test_nrt_rejects_non_box_and_bad_level(lib):
+    v = box_vertices(3, [0, 0, 0], [1, 1, 1])
+    v[6, 2] += 0.05
+    st, msg = _nrt_create(lib, verts=v[None])
+    assert st == lib.ERR_INVALID and "axis-aligned" in msg
+    st, msg = _nrt_create(lib, l=0)
+    assert st == lib.ERR_INVALID
+    st, msg = _nrt_create(lib, l=3)
+    assert st == lib.ERR_UNSUPPORTED and "256 unknowns" in msg
+    L = lib.load()
+    assert L.msfec_nrt_create(None, 1, None, 0, None) == lib.ERR_INVALID
+    assert L.msfec_nrt_destroy(None) == lib.OK
