@@ -92,18 +92,26 @@ struct Nd {
   static constexpr int oY = 0;                             // CH x 18 x 18
   static constexpr int szA = cmax(cmax(3 * NR * 4, NW * 16 * 18 + 256), CH * 324);
   static constexpr int oB = szA;
-  static constexpr int oCf = oB;                           // nfc x 8 x 8
-  static constexpr int oFc = oCf + nfc * 64;               // 12 x nu
+  static constexpr int oFc = oB;                           // 12 x nu
   static constexpr int oIc = oFc + 12 * nu;                // n x 4 ints
+  static constexpr int oCf = oIc + 2 * n;                  // nfc x 8 x 8
+  static constexpr int oRb = oCf;                          // 18 x n (after the local matrices)
   static constexpr int oX = oB;                            // NP x XS
   static constexpr int oXL = oB + NP * XS;                 // CH x 180
-  static constexpr int szB = cmax(nfc * 64 + 12 * nu + 2 * n, NP * XS + CH * 180);
+  static constexpr int szB = cmax(12 * nu + 2 * n + cmax(nfc * 64, kNB * n), NP * XS + CH * 180);
   static constexpr int oC = oB + szB;
   static constexpr int oMl = oC;                           // nfc x 78
   static constexpr int oDl = oMl + nfc * 78;               // nfc x 21
   static constexpr int oCc = oDl + nfc * 21;               // 6 x 12
   static constexpr int oFl = oCc + 72;                     // nfc x 6
-  static constexpr int total = oFl + nfc * 6;
+  static constexpr int oPi = oFl + nfc * 6;                // NP / 4 pivot-block inverses, 4 x 4
+  static constexpr int oSc = oPi + NP * 4;                 // n row scales
+  static constexpr int oCd = oSc + n;                      // ints: nfc x 18 local dofs, then N f2i
+  static constexpr int N = 3 * m * (m + 1) * (m + 1) + 3 * m * m * (m + 1);
+  static constexpr int total = oCd + (nfc * 18 + N + 1) / 2;
+  // phase-0 tables in region A: W[78][8], PP[21][8], DD[21], comp[78] (ints)
+  static constexpr int oW = 0, oPP = 624, oDD = 624 + 168, oCmp = 624 + 168 + 21;
+  static_assert(oCmp + 39 <= szA, "phase-0 tables exceed region A");
   static_assert(total * 8 <= 160 * 1024, "LDS budget");
 };
 
@@ -208,6 +216,56 @@ __device__ __forceinline__ void wave_fence() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// deal.II's line order as compile-time tables (for loops unrolled over the
+// bases)
+constexpr int kLAx[12] = {1, 1, 0, 0, 1, 1, 0, 0, 2, 2, 2, 2};
+constexpr int kLF1[12] = {0, 1, 0, 1, 0, 1, 0, 1, 0, 1, 0, 1};
+constexpr int kLF2[12] = {0, 0, 0, 0, 1, 1, 1, 1, 0, 0, 1, 1};
+
+// Type (0..2 edges along x/y/z, 3..5 faces normal to x/y/z) and integer
+// position of the full dof f: the start node of an edge, the level along
+// its normal and the cell indices along the other axes of a face.
+template <int M>
+__device__ __forceinline__ void dof_pos(int f, int& t, int& p0, int& p1, int& p2) {
+  constexpr int nx = M * (M + 1) * (M + 1), ne = 3 * nx, nf1 = M * M * (M + 1);
+  int q, e0, e1;
+  if (f < ne) {
+    t = f / nx;
+    q = f - t * nx;
+    e0 = t == 0 ? M : M + 1;
+    e1 = t == 1 ? M : M + 1;
+  } else {
+    const int tf = (f - ne) / nf1;
+    q = (f - ne) - tf * nf1;
+    e0 = tf == 0 ? M + 1 : M;
+    e1 = tf == 1 ? M + 1 : M;
+    t = 3 + tf;
+  }
+  p0 = q % e0;
+  q /= e0;
+  p1 = q % e1;
+  p2 = q / e1;
+}
+
+// The fixed boundary value of basis r on the boundary dof f (full
+// numbering): the line integral of the coarse Nedelec function r along a
+// boundary edge, (1/m) l_f1(p_o1 / m) l_f2(p_o2 / m) for an edge along the
+// line's axis, and the flux 1/m^2 of the coarse Raviart-Thomas function of
+// face r - 12 through a fine face of that face.  Only called for dofs on the
+// coarse boundary.
+template <int M>
+__device__ __forceinline__ double bnd_value(int f, int r) {
+  int t, p0, p1, p2;
+  dof_pos<M>(f, t, p0, p1, p2);
+  if (t < 3) {
+    if (r >= 12 || kLineAx[r] != t) return 0.0;
+    const int po1 = t == 0 ? p1 : p0, po2 = t == 2 ? p1 : p2;
+    return lag(kLineF1[r], (double)po1 / M) * lag(kLineF2[r], (double)po2 / M) / M;
+  }
+  const int d = t - 3, pd = d == 0 ? p0 : (d == 1 ? p1 : p2);
+  return r == 12 + 2 * d + (pd == M ? 1 : 0) ? 1.0 / (M * M) : 0.0;
+}
+
 struct NrtArgs {
   const double* verts;  // n_cells x 8 x 3
   const double* ainv;   // n_cells x nfc x 8 x 6
@@ -217,7 +275,6 @@ struct NrtArgs {
   const int* icell;     // n x 4: fine cell * 32 + local dof of each cell holding the interior dof, -1
   const int* cdof;      // nfc x 18: full dof of every local dof
   const int* f2i;       // N: interior index of a full dof, -1 on the boundary
-  const double* bval;   // N x 18: the fixed boundary values of the bases (0 inside)
   double* X;            // n_cells x 18 x N
   double* M;            // n_cells x 18 x 18
   double* b;            // n_cells x 18
@@ -231,21 +288,23 @@ struct NrtArgs {
 template <bool Orig>
 __device__ __forceinline__ double kloc(const double* Ml, const double* Dl, const double* Cc, int c,
                                        int a, int b) {
-  if (a < 12 && b < 12) return Ml[c * 78 + tri(a, b)];
-  if (a >= 12 && b >= 12) return Orig ? Dl[c * 21 + tri(a - 12, b - 12)] : -Dl[c * 21 + tri(a - 12, b - 12)];
-  if (a < 12) return -Cc[(b - 12) * 12 + a];
-  return Orig ? Cc[(a - 12) * 12 + b] : -Cc[(a - 12) * 12 + b];
+  // one address and one sign, no branches: Me, Du packed lower, C[F][L]
+  const bool ea = a < 12, eb = b < 12;
+  const int hi = max(a, b), lo = min(a, b);
+  const double* base = (ea && eb) ? Ml + c * 78 : ((!ea && !eb) ? Dl + c * 21 : Cc);
+  const int off = (ea && eb) ? hi * (hi + 1) / 2 + lo
+                             : ((!ea && !eb) ? (hi - 12) * (hi - 11) / 2 + (lo - 12) : (hi - 12) * 12 + lo);
+  const double v = base[off];
+  // K': every entry but the edge block negated; the assembled matrix: only
+  // the (edge test, face trial) block, -C^T
+  const bool neg = Orig ? (ea && !eb) : !(ea && eb);
+  return neg ? -v : v;
 }
 
-// The entries of tile (P, Q) held by this lane (column 16 Q + ll, rows
-// 16 P + 4 gg + lh): K' for P < TN (identity on the padding), else the
-// right-hand sides r = 16 (P - TN) + 4 gg + lh: the curl load (face rows,
-// negated in K') minus the lift of the boundary values.
 template <int L>
 __device__ __forceinline__ nrt_d4 nrt_tile(int P, int Q, int ll, int lh, const int* ic,
                                         const double* Ml, const double* Dl, const double* Cc,
-                                        const double* Fc, const int* cdof, const int* f2i,
-                                        const double* bval) {
+                                        const double* rb, const double* sc) {
   typedef Nd<L> D;
   nrt_d4 t = {0.0, 0.0, 0.0, 0.0};
   const int J = 16 * Q + ll;
@@ -270,30 +329,16 @@ __device__ __forceinline__ nrt_d4 nrt_tile(int P, int Q, int ll, int lh, const i
               v += kloc<false>(Ml, Dl, Cc, ei >> 5, ei & 31, ej & 31);
           }
         }
+        v *= sc[I] * sc[J];
       }
       t[gg] = v;
     }
     return t;
   }
-  double v[4];
-  int r[4];
   for (int gg = 0; gg < 4; ++gg) {
-    r[gg] = 16 * (P - D::TN) + 4 * gg + lh;
-    v[gg] = (r[gg] < 12 && J >= D::ns) ? -Fc[r[gg] * D::nu + (J - D::ns)] : 0.0;
+    const int r = 16 * (P - D::TN) + 4 * gg + lh;
+    t[gg] = r < kNB ? rb[r * D::n + J] : 0.0;
   }
-  for (int bb = 0; bb < 4; ++bb) {
-    const int ej = cj[bb];
-    if (ej < 0) continue;
-    const int c = ej >> 5, lj = ej & 31;
-    for (int lb = 0; lb < 18; ++lb) {
-      const int full = cdof[c * 18 + lb];
-      if (f2i[full] >= 0) continue;
-      const double k = kloc<false>(Ml, Dl, Cc, c, lj, lb);
-      for (int gg = 0; gg < 4; ++gg)
-        if (r[gg] < kNB) v[gg] -= k * bval[(size_t)full * kNB + r[gg]];
-    }
-  }
-  for (int gg = 0; gg < 4; ++gg) t[gg] = r[gg] < kNB ? v[gg] : 0.0;
   return t;
 }
 
@@ -302,7 +347,6 @@ __global__ __launch_bounds__(Nd<L>::NT) void nrt_basis_kernel(NrtArgs g) {
   typedef Nd<L> D;
   extern __shared__ double sm[];
   __shared__ int s_flag;
-  __shared__ double s_pblk[16];
   const int cell = blockIdx.x, tid = threadIdx.x;
   const int lane = tid & 63, w = tid >> 6, ll = lane & 15, lh = lane >> 4;
   const int N = g.N;
@@ -317,7 +361,8 @@ __global__ __launch_bounds__(Nd<L>::NT) void nrt_basis_kernel(NrtArgs g) {
   const double vol = h.x * h.y * h.z;
   if (tid == 0) s_flag = MSFEC_CELL_OK;
 
-  // ---- coefficients to LDS: per fine cell and point, A^-1 (6), B, R
+  // ---- coefficients to LDS: per fine cell and point, A^-1 (6), B, R; the
+  // dof tables; the point products of the shape functions
   double* cf = sm + D::oCf;
   for (int idx = tid; idx < D::nfc * 8; idx += D::NT) {
     const size_t gi = (size_t)cell * D::nfc * 8 + idx;
@@ -329,6 +374,39 @@ __global__ __launch_bounds__(Nd<L>::NT) void nrt_basis_kernel(NrtArgs g) {
   }
   int* ic = reinterpret_cast<int*>(sm + D::oIc);
   for (int idx = tid; idx < D::n * 4; idx += D::NT) ic[idx] = g.icell[idx];
+  int* cdl = reinterpret_cast<int*>(sm + D::oCd);
+  int* f2l = cdl + D::nfc * 18;
+  for (int idx = tid; idx < D::nfc * 18; idx += D::NT) cdl[idx] = g.cdof[idx];
+  for (int idx = tid; idx < D::N; idx += D::NT) f2l[idx] = g.f2i[idx];
+  double* Wt = sm + D::oW;
+  double* PP = sm + D::oPP;
+  double* DD = sm + D::oDD;
+  int* cmp = reinterpret_cast<int*>(sm + D::oCmp);
+  const double w8 = vol * 0.125;
+  for (int idx = tid; idx < 78 * 8 + 21 * 8 + 21 + 78; idx += D::NT) {
+    if (idx < 78 * 8) {
+      const int e = idx >> 3, q = idx & 7;
+      int i, j;
+      tri_ij(e, i, j);
+      const V3 X = gauss_xyz(q);
+      Wt[idx] = w8 * ned_val(i, X, h) * ned_val(j, X, h);
+    } else if (idx < 78 * 8 + 21 * 8) {
+      const int e = (idx - 78 * 8) >> 3, q = (idx - 78 * 8) & 7;
+      int i, j;
+      tri_ij(e, i, j);
+      const V3 X = gauss_xyz(q);
+      PP[e * 8 + q] = (i >> 1) == (j >> 1) ? w8 * rt_val(i, X, h) * rt_val(j, X, h) : 0.0;
+    } else if (idx < 78 * 8 + 21 * 8 + 21) {
+      int i, j;
+      tri_ij(idx - 78 * 8 - 21 * 8, i, j);
+      DD[idx - 78 * 8 - 21 * 8] = w8 * rt_div(i, h) * rt_div(j, h);
+    } else {
+      const int e = idx - 78 * 8 - 21 * 8 - 21;
+      int i, j;
+      tri_ij(e, i, j);
+      cmp[e] = sym6(kLineAx[i], kLineAx[j]);
+    }
+  }
   __syncthreads();
 
   // ---- local matrices, curl coupling, loads
@@ -342,26 +420,16 @@ __global__ __launch_bounds__(Nd<L>::NT) void nrt_basis_kernel(NrtArgs g) {
     const double* co = cf + c * 64;
     double acc = 0.0;
     if (e < 78) {
-      int i, j;
-      tri_ij(e, i, j);
-      const int ci = sym6(kLineAx[i], kLineAx[j]);
-      for (int q = 0; q < 8; ++q) {
-        const V3 X = gauss_xyz(q);
-        acc += ned_val(i, X, h) * ned_val(j, X, h) * co[q * 8 + ci];
-      }
-      Ml[c * 78 + e] = acc * vol * 0.125;
+      const int ci = cmp[e];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) acc += Wt[e * 8 + q] * co[q * 8 + ci];
+      Ml[c * 78 + e] = acc;
     } else {
-      int i, j;
-      tri_ij(e - 78, i, j);
-      const double dd = rt_div(i, h) * rt_div(j, h);
-      const bool same = (i >> 1) == (j >> 1);
-      for (int q = 0; q < 8; ++q) {
-        const V3 X = gauss_xyz(q);
-        double v = co[q * 8 + 6] * dd;
-        if (same) v += co[q * 8 + 7] * rt_val(i, X, h) * rt_val(j, X, h);
-        acc += v;
-      }
-      Dl[c * 21 + (e - 78)] = acc * vol * 0.125;
+      const int e2 = e - 78;
+      const double dd = DD[e2];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) acc += co[q * 8 + 6] * dd + co[q * 8 + 7] * PP[e2 * 8 + q];
+      Dl[c * 21 + e2] = acc;
     }
   }
   for (int idx = tid; idx < 72 + D::nfc * 6 + 12 * D::nu; idx += D::NT) {
@@ -372,7 +440,7 @@ __global__ __launch_bounds__(Nd<L>::NT) void nrt_basis_kernel(NrtArgs g) {
         const V3 X = gauss_xyz(q);
         acc += rt_val(F, X, h) * ned_curl(Ln, X, h, ax);
       }
-      Cc[idx] = acc * vol * 0.125;
+      Cc[idx] = acc * w8;
     } else if (idx < 72 + D::nfc * 6) {
       const int c = (idx - 72) / 6, F = (idx - 72) % 6, ax = F >> 1;
       double acc = 0.0;
@@ -383,23 +451,78 @@ __global__ __launch_bounds__(Nd<L>::NT) void nrt_basis_kernel(NrtArgs g) {
           acc += rt_val(F, X, h) * f[q * 3 + ax];
         }
       }
-      Fl[c * 6 + F] = acc * vol * 0.125;
+      Fl[c * 6 + F] = acc * w8;
     } else {
-      // (psi_I, curl N_n) over the two fine cells of interior face I
+      // (psi_I, curl N_n) over the two fine cells of interior face I: along
+      // the face normal d both factors depend on the d-coordinate only, so
+      // QGauss(2) reduces to its 2 points along d (4 points each)
       const int t = idx - 72 - D::nfc * 6, nb = t / D::nu, J = t % D::nu;
       double acc = 0.0;
       for (int k = 0; k < 2; ++k) {
         const int e = ic[(D::ns + J) * 4 + k];
         const int c = e >> 5, F = (e & 31) - 12, ax = F >> 1;
         const int c0 = c % D::m, c1 = (c / D::m) % D::m, c2 = c / (D::m * D::m);
-        for (int q = 0; q < 8; ++q) {
-          const V3 X = gauss_xyz(q);
+        for (int qd = 0; qd < 2; ++qd) {
+          const double gd = gauss_pt(qd);
+          const V3 X{ax == 0 ? gd : 0.5, ax == 1 ? gd : 0.5, ax == 2 ? gd : 0.5};
           const V3 Xc{(c0 + X.x) / D::m, (c1 + X.y) / D::m, (c2 + X.z) / D::m};
           acc += rt_val(F, X, h) * ned_curl(nb, Xc, Hc, ax);
         }
       }
-      Fc[nb * D::nu + J] = acc * vol * 0.125;
+      Fc[nb * D::nu + J] = acc * 4.0 * w8;
     }
+  }
+  __syncthreads();
+
+  // ---- right-hand sides b' = S F - K'_IB x_B of the 18 bases, rb[r][J], one
+  // thread per interior dof J: the curl loads (face rows negated in K'), then
+  // the lift of the boundary values over the fine cells of J (a boundary edge
+  // carries the 4 edge bases along its axis, a boundary face one face basis)
+  // symmetric equilibration: K'' = S K' S with S = |diag K'|^-1/2, so that
+  // every pivot is of order one (the solution is x = S y)
+  double* scl = sm + D::oSc;
+  for (int I = tid; I < D::n; I += D::NT) {
+    double dg = 0.0;
+    for (int kc = 0; kc < 4; ++kc) {
+      const int e = ic[I * 4 + kc];
+      if (e >= 0) dg += kloc<false>(Ml, Dl, Cc, e >> 5, e & 31, e & 31);
+    }
+    const double a = fabs(dg);
+    scl[I] = (a > 0.0 && isfinite(a)) ? 1.0 / sqrt(a) : 1.0;
+  }
+  double* rb = sm + D::oRb;
+  for (int J = tid; J < D::n; J += D::NT) {
+    double y[kNB];
+#pragma unroll
+    for (int r = 0; r < kNB; ++r) y[r] = (r < 12 && J >= D::ns) ? -Fc[r * D::nu + (J - D::ns)] : 0.0;
+    for (int kc = 0; kc < 4; ++kc) {
+      const int e = ic[J * 4 + kc];
+      if (e < 0) continue;
+      const int c = e >> 5, lj = e & 31;
+      for (int lb = 0; lb < 18; ++lb) {
+        const int full = cdl[c * 18 + lb];
+        if (f2l[full] >= 0) continue;
+        const double kv = kloc<false>(Ml, Dl, Cc, c, lj, lb);
+        int t, p0, p1, p2;
+        dof_pos<D::m>(full, t, p0, p1, p2);
+        if (t < 3) {
+          const double x1 = (double)(t == 0 ? p1 : p0) / D::m;
+          const double x2 = (double)(t == 2 ? p1 : p2) / D::m;
+#pragma unroll
+          for (int r = 0; r < 12; ++r)
+            if (kLAx[r] == t) y[r] -= kv * (lag(kLF1[r], x1) * lag(kLF2[r], x2) / D::m);
+        } else {
+          const int d = t - 3, pd = d == 0 ? p0 : (d == 1 ? p1 : p2);
+          const int F = 2 * d + (pd == D::m ? 1 : 0);
+#pragma unroll
+          for (int r = 0; r < 6; ++r)
+            if (r == F) y[12 + r] -= kv * (1.0 / (D::m * D::m));
+        }
+      }
+    }
+    const double sj = scl[J];
+#pragma unroll
+    for (int r = 0; r < kNB; ++r) rb[r * D::n + J] = y[r] * sj;
   }
   __syncthreads();
 
@@ -427,12 +550,13 @@ __global__ __launch_bounds__(Nd<L>::NT) void nrt_basis_kernel(NrtArgs g) {
     constexpr int s = decltype(sc)::value;
     int P, Q;
     slot_pq(s, P, Q);
-    T[s] = nrt_tile<L>(P, Q, ll, lh, ic, Ml, Dl, Cc, Fc, g.cdof, g.f2i, g.bval);
+    T[s] = nrt_tile<L>(P, Q, ll, lh, ic, Ml, Dl, Cc, rb, scl);
   });
   // first pivot columns (0..3) to LDS
   double* stg = sm + D::oStage;
   double* vb = sm + D::oV;
   double* vdb = sm + D::oVD;
+  double* pinv = sm + D::oPi;
   SlotLoop<0, D::NS>::run([&](auto sc) {
     constexpr int s = decltype(sc)::value;
     int P, Q;
@@ -442,9 +566,16 @@ __global__ __launch_bounds__(Nd<L>::NT) void nrt_basis_kernel(NrtArgs g) {
       for (int gg = 0; gg < 4; ++gg) stg[(16 * P + 4 * gg + lh) * 4 + ll] = T[s][gg];
   });
 
-  // ---- L D L^T, 4 pivots per step, right-hand-side rows alongside
+  // ---- block L D L^T, 4 pivots per step, right-hand-side rows alongside.
+  // Phase A (every thread): the pivot block P (the current Schur complement,
+  // staged in LDS) = L1 D1 L1^T, its inverse to LDS for the backward sweep,
+  // and, for row i below it, V_i = A_i L1^-T and V_i D1^-1 to LDS.  Phase B
+  // (the tile owners): T -= (V D1^-1) V^T, one MFMA per tile, then the next
+  // pivot columns to LDS.  V vanishes on the pivot rows, so a pivot column
+  // keeps its Schur-complement values A_i: the backward sweep works with
+  // them and the stored inverses (block back substitution).
   for (int k = 0; k < D::NP; k += 4) {
-    const int Qk = k >> 4, G = (k >> 2) & 3;
+    const int Qk = k >> 4;
     const int ll = opaque_v(tid) & 15, lh = (opaque_v(tid) >> 4) & 3;
     const int w = opaque_s(__builtin_amdgcn_readfirstlane(tid >> 6)), Pb = D::TN - 1 - w;
     auto slot_pq = [&](int s, int& P, int& Q) {
@@ -460,26 +591,26 @@ __global__ __launch_bounds__(Nd<L>::NT) void nrt_basis_kernel(NrtArgs g) {
     };
     __syncthreads();
     {
-      // pivot block = L1 D1 L1^T, factored by every thread
       double p[4][4];
 #pragma unroll
       for (int r = 0; r < 4; ++r)
 #pragma unroll
         for (int c = 0; c <= r; ++c) p[r][c] = stg[(k + r) * 4 + c];
-      double d[4], l1[4][4], u[4][4];
+      double d[4], di[4], l1[4][4], u[4][4];
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
         double dc = p[c][c];
 #pragma unroll
         for (int t = 0; t < c; ++t) dc -= l1[c][t] * u[c][t];
         d[c] = dc;
+        di[c] = 1.0 / dc;
 #pragma unroll
         for (int r = c + 1; r < 4; ++r) {
           double v = p[r][c];
 #pragma unroll
           for (int t = 0; t < c; ++t) v -= l1[r][t] * u[c][t];
           u[r][c] = v;
-          l1[r][c] = v / dc;
+          l1[r][c] = v * di[c];
         }
       }
       // rows of L1^-1 (unit lower)
@@ -496,17 +627,24 @@ __global__ __launch_bounds__(Nd<L>::NT) void nrt_basis_kernel(NrtArgs g) {
           li[r][c] = v;
         }
       }
-      if (tid == 0) {
-        // L1 below the diagonal, D1 on it, for the pivot columns; the sign check
+      if (tid < 16) {
+        // entry (r, c) of P^-1 = L1^-T D1^-1 L1^-1; the sign check of the pivots
+        double pv = 0.0;
 #pragma unroll
         for (int r = 0; r < 4; ++r)
 #pragma unroll
-          for (int c = 0; c <= r; ++c) s_pblk[r * 4 + c] = r == c ? d[c] : l1[r][c];
+          for (int c = 0; c < 4; ++c) {
+            double v = 0.0;
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
-          const int idx = k + c;
+            for (int t = 0; t < 4; ++t) v += li[t][r] * di[t] * li[t][c];
+            pv = tid == r * 4 + c ? v : pv;
+          }
+        pinv[(k >> 2) * 16 + tid] = pv;
+        if (tid < 4) {
+          const double dc = tid == 0 ? d[0] : (tid == 1 ? d[1] : (tid == 2 ? d[2] : d[3]));
+          const int idx = k + tid;
           const double sg = (idx < D::ns || idx >= D::n) ? 1.0 : -1.0;
-          if (!(sg * d[c] > 0.0) || !isfinite(d[c])) s_flag = MSFEC_CELL_NOT_SPD;
+          if (!(sg * dc > 0.0) || !isfinite(dc)) s_flag = MSFEC_CELL_NOT_SPD;
         }
       }
       for (int row = tid; row < D::NR; row += D::NT) {
@@ -520,61 +658,62 @@ __global__ __launch_bounds__(Nd<L>::NT) void nrt_basis_kernel(NrtArgs g) {
 #pragma unroll
           for (int c = 0; c <= kk; ++c) v += a[c] * li[kk][c];
           vb[row * 4 + kk] = v;
-          vdb[row * 4 + kk] = act ? v / d[kk] : 0.0;
+          vdb[row * 4 + kk] = act ? v * di[kk] : 0.0;
         }
       }
     }
     __syncthreads();
     {
-      // operands: V D^-1 of the wave's 4 tile rows, V of every active tile column
+      // operands, all read up front: V D^-1 of the wave's 4 tile rows, V of
+      // every active tile column; updates by tile column, left to right
       const double aPb = vdb[(16 * Pb + ll) * 4 + lh], aPa = vdb[(16 * w + ll) * 4 + lh];
       const double aR0 = vdb[(16 * D::TN + ll) * 4 + lh];
       const double aR1 = vdb[(16 * (D::TN + 1) + ll) * 4 + lh];
       double bq[D::TN];
 #pragma unroll
       for (int q = 0; q < D::TN; ++q) bq[q] = q >= Qk ? vb[(16 * q + ll) * 4 + lh] : 0.0;
-      SlotLoop<0, D::NS>::run([&](auto sc) {
-        constexpr int s = decltype(sc)::value;
-        if constexpr (s <= D::TN) {
-          const bool lo = s <= Pb;
-          const int Q = lo ? s : D::TN - s;
-          constexpr int qh = (D::TN - s) < D::TN ? D::TN - s : 0;
-          if (Q >= Qk)
-            T[s] = __builtin_amdgcn_mfma_f64_16x16x4f64(lo ? aPb : aPa, lo ? bq[s < D::TN ? s : 0] : bq[qh],
-                                                         T[s], 0, 0, 1);
-        } else {
-          constexpr int r = s - D::TN - 1;
-          const int Q = (r & 1) ? Pb : w;
-          if (Q >= Qk)
-            T[s] = __builtin_amdgcn_mfma_f64_16x16x4f64((r >> 1) ? aR1 : aR0,
-                                                         vb[(16 * Q + ll) * 4 + lh], T[s], 0, 0, 1);
+      const double bR0 = w >= Qk ? vb[(16 * w + ll) * 4 + lh] : 0.0;
+      const double bR1 = Pb >= Qk ? vb[(16 * Pb + ll) * 4 + lh] : 0.0;
+      SlotLoop<0, D::TN>::run([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        if (q >= Qk) {
+          // tile (Pb, q) is slot q, tile (w, q) slot TN - q
+          if (q <= Pb) T[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(aPb, bq[q], T[q], 0, 0, 1);
+          if (q <= w)
+            T[D::TN - q] = __builtin_amdgcn_mfma_f64_16x16x4f64(aPa, bq[q], T[D::TN - q], 0, 0, 1);
+          if (q == w) {
+            T[D::TN + 1] = __builtin_amdgcn_mfma_f64_16x16x4f64(aR0, bR0, T[D::TN + 1], 0, 0, 1);
+            T[D::TN + 3] = __builtin_amdgcn_mfma_f64_16x16x4f64(aR1, bR0, T[D::TN + 3], 0, 0, 1);
+          }
+          if (q == Pb) {
+            T[D::TN + 2] = __builtin_amdgcn_mfma_f64_16x16x4f64(aR0, bR1, T[D::TN + 2], 0, 0, 1);
+            T[D::TN + 4] = __builtin_amdgcn_mfma_f64_16x16x4f64(aR1, bR1, T[D::TN + 4], 0, 0, 1);
+          }
         }
       });
     }
-    // the pivot columns become L (unit lower, D on the diagonal), then the
-    // next pivot columns go to LDS (read after the barrier at the top)
+    // next pivot columns to LDS (read after the barrier at the top)
     const int k2 = k + 4, Q2 = k2 >> 4, G2 = (k2 >> 2) & 3;
-    SlotLoop<0, D::NS>::run([&](auto sc) {
-      constexpr int s = decltype(sc)::value;
-      int P, Q;
-      slot_pq(s, P, Q);
-      if (Q == Qk && (ll >> 2) == G) {
-        const int c = ll & 3;
+    if (k2 < D::NP) {
+      SlotLoop<0, D::NS>::run([&](auto sc) {
+        constexpr int s = decltype(sc)::value;
+        int P, Q;
+        slot_pq(s, P, Q);
+        if (Q == Q2 && (ll >> 2) == G2)
 #pragma unroll
-        for (int gg = 0; gg < 4; ++gg) {
-          const int row = 16 * P + 4 * gg + lh;
-          if (row > k + 3) T[s][gg] = vdb[row * 4 + c];
-          else if (row >= k + c) T[s][gg] = s_pblk[(row - k) * 4 + c];
-        }
-      }
-      if (k2 < D::NP && Q == Q2 && (ll >> 2) == G2)
-#pragma unroll
-        for (int gg = 0; gg < 4; ++gg) stg[(16 * P + 4 * gg + lh) * 4 + (ll & 3)] = T[s][gg];
-    });
+          for (int gg = 0; gg < 4; ++gg) stg[(16 * P + 4 * gg + lh) * 4 + (ll & 3)] = T[s][gg];
+      });
+    }
   }
   __syncthreads();
 
-  // ---- backward sweep L^T x = z; z = D^-1 L^-1 b sits in the rhs tiles
+  // ---- block back substitution, tile column by tile column from the right:
+  // x_b = P_b^-1 (r_b - A_b^T x_below) for the 4 pivot blocks b of a tile
+  // column, with r_b the right-hand sides' entries in the pivot columns and
+  // A_b the Schur-complement columns below the block.  The owners of the
+  // tiles below the diagonal form A^T x on the matrix cores (their
+  // accumulator registers are the transposed A operand); the diagonal tile's
+  // owner finishes the 4 blocks.
   double* xb = sm + D::oX;
   double* part = sm + D::oPart;
   double* Ls = sm + D::oLs;
@@ -609,8 +748,6 @@ __global__ __launch_bounds__(Nd<L>::NT) void nrt_basis_kernel(NrtArgs g) {
       int P, Q;
       slot_pq(s, P, Q);
       if (Q == Qb && P > Qb) {
-        // (L_PQ^T x_P): the accumulator layout of the tile is the A operand
-        // of its transpose, slice sp = register sp
 #pragma unroll
         for (int sp = 0; sp < 4; ++sp) {
           const int rx = 16 * P + 4 * sp + lh;
@@ -646,9 +783,17 @@ __global__ __launch_bounds__(Nd<L>::NT) void nrt_basis_kernel(NrtArgs g) {
           x[i] = v;
         }
 #pragma unroll
-        for (int i = 15; i >= 0; --i) {
+        for (int b = 3; b >= 0; --b) {
+          // r_b - A_b^T x_below (the rows of the later blocks of this tile)
 #pragma unroll
-          for (int r = i + 1; r < 16; ++r) x[i] -= Ls[r * 16 + i] * x[r];
+          for (int c = 4 * b; c < 4 * b + 4; ++c)
+#pragma unroll
+            for (int r = 4 * b + 4; r < 16; ++r) x[c] -= Ls[r * 16 + c] * x[r];
+          const double* pi = pinv + (4 * Qb + b) * 16;
+          const double t0 = x[4 * b], t1 = x[4 * b + 1], t2 = x[4 * b + 2], t3 = x[4 * b + 3];
+#pragma unroll
+          for (int c = 0; c < 4; ++c)
+            x[4 * b + c] = pi[c * 4] * t0 + pi[c * 4 + 1] * t1 + pi[c * 4 + 2] * t2 + pi[c * 4 + 3] * t3;
         }
 #pragma unroll
         for (int i = 0; i < 16; ++i) xb[(16 * Qb + i) * D::XS + lane] = x[i];
@@ -657,13 +802,20 @@ __global__ __launch_bounds__(Nd<L>::NT) void nrt_basis_kernel(NrtArgs g) {
     __syncthreads();
   }
 
+  // x = S y
+  for (int idx = tid; idx < D::n * kNB; idx += D::NT) {
+    const int I = idx / kNB, r = idx - I * kNB;
+    xb[I * D::XS + r] *= scl[I];
+  }
+  __syncthreads();
+
   // ---- the fine vectors of the bases: interior from x, boundary values fixed
   {
     double* Xo = g.X + (size_t)cell * kNB * N;
     for (int idx = tid; idx < kNB * N; idx += D::NT) {
       const int r = idx / N, f = idx - r * N;
-      const int I = g.f2i[f];
-      Xo[idx] = I >= 0 ? xb[I * D::XS + r] : g.bval[(size_t)f * kNB + r];
+      const int I = f2l[f];
+      Xo[idx] = I >= 0 ? xb[I * D::XS + r] : bnd_value<D::m>(f, r);
     }
   }
 
@@ -685,9 +837,9 @@ __global__ __launch_bounds__(Nd<L>::NT) void nrt_basis_kernel(NrtArgs g) {
         a = 12 + (e - 144) / 6;
         j = 12 + (e - 144) % 6;
       }
-      const int full = g.cdof[c * 18 + a];
-      const int I = g.f2i[full];
-      XL[idx] = I >= 0 ? xb[I * D::XS + j] : g.bval[(size_t)full * kNB + j];
+      const int full = cdl[c * 18 + a];
+      const int I = f2l[full];
+      XL[idx] = I >= 0 ? xb[I * D::XS + j] : bnd_value<D::m>(full, j);
     }
     __syncthreads();
     // Y[cl][a][j] = (K_loc x_j)[a] over the block of basis j
@@ -787,7 +939,6 @@ msfec_status nrt_fail(msfec_status st, const std::string& msg) {
 struct NrtTables {
   int m = 0, nx = 0, ne = 0, nf1 = 0, N = 0, n = 0, ns = 0;
   std::vector<int> icell, cdof, f2i;
-  std::vector<double> bval;
 };
 
 NrtTables nrt_tables(int l) {
@@ -826,10 +977,9 @@ NrtTables nrt_tables(int l) {
         d[17] = fz(a, b, c3 + 1);
       }
   // position of every full dof: type (0..2 edges along x/y/z, 3..5 faces
-  // normal to x/y/z) and its integer coordinates
+  // normal to x/y/z) and its integer coordinates; interior dofs are numbered
+  // in full order (edges first)
   t.f2i.assign(t.N, -1);
-  t.bval.assign((size_t)t.N * kNB, 0.0);
-  auto lagf = [](int s, double x) { return s ? x : 1.0 - x; };
   for (int f = 0; f < t.N; ++f) {
     int ty, p[3];
     if (f < t.ne) {
@@ -859,23 +1009,7 @@ NrtTables nrt_tables(int l) {
     } else {
       bnd = p[ty - 3] == 0 || p[ty - 3] == m;
     }
-    if (!bnd) {
-      t.f2i[f] = t.n++;
-      continue;
-    }
-    // fixed values: line integral of coarse Nedelec function L along a
-    // boundary edge, flux of coarse Raviart-Thomas function F through a
-    // boundary face
-    for (int L = 0; L < 12; ++L) {
-      if (ty != ax[L]) continue;
-      const int o1 = ax[L] == 0 ? 1 : 0, o2 = ax[L] == 2 ? 1 : 2;
-      t.bval[(size_t)f * kNB + L] = lagf(f1[L], (double)p[o1] / m) * lagf(f2[L], (double)p[o2] / m) / m;
-    }
-    if (ty >= 3) {
-      const int axf = ty - 3;
-      const int F = 2 * axf + (p[axf] == m ? 1 : 0);
-      t.bval[(size_t)f * kNB + 12 + F] = 1.0 / ((double)m * m);
-    }
+    if (!bnd) t.f2i[f] = t.n++;
   }
   t.ns = 0;
   for (int f = 0; f < t.ne; ++f)
@@ -905,7 +1039,6 @@ struct msfec_nrt_batch {
   int* d_icell = nullptr;
   int* d_cdof = nullptr;
   int* d_f2i = nullptr;
-  double* d_bval = nullptr;
   double* d_X = nullptr;
   double* d_M = nullptr;
   double* d_rhs = nullptr;
@@ -927,7 +1060,7 @@ msfec_status nrt_alloc(T** p, size_t count) {
 
 void nrt_free(msfec_nrt_batch* h) {
   void* ptrs[] = {h->d_verts, h->d_ainv, h->d_b, h->d_r,   h->d_f,   h->d_icell,  h->d_cdof,
-                  h->d_f2i,   h->d_bval, h->d_X, h->d_M,   h->d_rhs, h->d_w,      h->d_status};
+                  h->d_f2i,   h->d_X, h->d_M,   h->d_rhs, h->d_w,      h->d_status};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (auto& set : h->ev_sets)
@@ -968,7 +1101,6 @@ msfec_status nrt_launch(msfec_nrt_batch* h, hipStream_t s) {
   a.icell = h->d_icell;
   a.cdof = h->d_cdof;
   a.f2i = h->d_f2i;
-  a.bval = h->d_bval;
   a.X = h->d_X;
   a.M = h->d_M;
   a.b = h->d_rhs;
@@ -1038,7 +1170,6 @@ msfec_status msfec_nrt_create(const msfec_nrt_params* params, int n_cells, const
   NRT_TRY(nrt_alloc(&h->d_icell, h->tab.icell.size()));
   NRT_TRY(nrt_alloc(&h->d_cdof, h->tab.cdof.size()));
   NRT_TRY(nrt_alloc(&h->d_f2i, h->tab.f2i.size()));
-  NRT_TRY(nrt_alloc(&h->d_bval, h->tab.bval.size()));
   NRT_TRY(nrt_alloc(&h->d_X, nc * kNB * h->tab.N));
   NRT_TRY(nrt_alloc(&h->d_M, nc * kNB * kNB));
   NRT_TRY(nrt_alloc(&h->d_rhs, nc * kNB));
@@ -1048,7 +1179,6 @@ msfec_status msfec_nrt_create(const msfec_nrt_params* params, int n_cells, const
   NRT_TRY(nrt_in(h->stream, h->d_icell, h->tab.icell.data(), h->tab.icell.size() * sizeof(int), MSFEC_HOST));
   NRT_TRY(nrt_in(h->stream, h->d_cdof, h->tab.cdof.data(), h->tab.cdof.size() * sizeof(int), MSFEC_HOST));
   NRT_TRY(nrt_in(h->stream, h->d_f2i, h->tab.f2i.data(), h->tab.f2i.size() * sizeof(int), MSFEC_HOST));
-  NRT_TRY(nrt_in(h->stream, h->d_bval, h->tab.bval.data(), h->tab.bval.size() * sizeof(double), MSFEC_HOST));
 #undef NRT_TRY
   *out = h;
   return MSFEC_OK;
