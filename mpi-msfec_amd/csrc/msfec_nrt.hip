@@ -113,6 +113,7 @@ struct Nd {
   static constexpr int oW = 0, oPP = 624, oDD = 624 + 168, oCmp = 624 + 168 + 21;
   static_assert(oCmp + 39 <= szA, "phase-0 tables exceed region A");
   static_assert(total * 8 <= 160 * 1024, "LDS budget");
+  static_assert(oCc + 72 <= 0x8000, "local-matrix offsets must fit the 15 bits of the assembly map");
 };
 
 // deal.II's line order on the unit cube: axis of line L and the values of
@@ -275,6 +276,8 @@ struct NrtArgs {
   const int* icell;     // n x 4: fine cell * 32 + local dof of each cell holding the interior dof, -1
   const int* cdof;      // nfc x 18: full dof of every local dof
   const int* f2i;       // N: interior index of a full dof, -1 on the boundary
+  const unsigned short* amap;  // assembly map of the lower tiles (see nrt_tile)
+  const unsigned char* tnz;    // the lower tiles with any entry
   double* X;            // n_cells x 18 x N
   double* M;            // n_cells x 18 x 18
   double* b;            // n_cells x 18
@@ -301,40 +304,47 @@ __device__ __forceinline__ double kloc(const double* Ml, const double* Dl, const
   return neg ? -v : v;
 }
 
+// The entries of tile (P, Q) held by this lane (column 16 Q + ll, rows
+// 16 P + 4 gg + lh): K'' for P < TN (identity on the padding), else the
+// right-hand sides r = 16 (P - TN) + 4 gg + lh from rb.  An entry of K' is
+// the sum of at most 4 local-matrix entries (the fine cells the two dofs
+// share); amap lists them per tile, register and lane as 16-bit LDS offsets
+// with the sign in the top bit (0xffff: none), and tnz flags the tiles that
+// hold any.
 template <int L>
-__device__ __forceinline__ nrt_d4 nrt_tile(int P, int Q, int ll, int lh, const int* ic,
-                                        const double* Ml, const double* Dl, const double* Cc,
-                                        const double* rb, const double* sc) {
+__device__ __forceinline__ nrt_d4 nrt_tile(int P, int Q, int lane, const double* sm,
+                                        const double* rb, const double* sc,
+                                        const unsigned short* amap, const unsigned char* tnz) {
   typedef Nd<L> D;
+  const int ll = lane & 15, lh = lane >> 4;
   nrt_d4 t = {0.0, 0.0, 0.0, 0.0};
   const int J = 16 * Q + ll;
-  if (J >= D::n) {
-    if (P < D::TN)
-      for (int gg = 0; gg < 4; ++gg) t[gg] = (16 * P + 4 * gg + lh == J) ? 1.0 : 0.0;
-    return t;
-  }
-  int cj[4];
-  for (int k = 0; k < 4; ++k) cj[k] = ic[J * 4 + k];
   if (P < D::TN) {
+    const int tix = P * (P + 1) / 2 + Q;
+    if (J >= D::n) {
+      for (int gg = 0; gg < 4; ++gg) t[gg] = (16 * P + 4 * gg + lh == J) ? 1.0 : 0.0;
+      return t;
+    }
+    if (!tnz[tix]) return t;
+    const double sj = sc[J];
     for (int gg = 0; gg < 4; ++gg) {
       const int I = 16 * P + 4 * gg + lh;
+      const uint2 e = reinterpret_cast<const uint2*>(amap)[((size_t)tix * 4 + gg) * 64 + lane];
+      const unsigned hs[4] = {e.x & 0xffffu, e.x >> 16, e.y & 0xffffu, e.y >> 16};
       double v = 0.0;
-      if (I < D::n) {
-        for (int a = 0; a < 4; ++a) {
-          const int ei = ic[I * 4 + a];
-          if (ei < 0) continue;
-          for (int bb = 0; bb < 4; ++bb) {
-            const int ej = cj[bb];
-            if (ej >= 0 && (ej >> 5) == (ei >> 5))
-              v += kloc<false>(Ml, Dl, Cc, ei >> 5, ei & 31, ej & 31);
-          }
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const unsigned h = hs[k];
+        if (h != 0xffffu) {
+          const double x = sm[h & 0x7fffu];
+          v += (h & 0x8000u) ? -x : x;
         }
-        v *= sc[I] * sc[J];
       }
-      t[gg] = v;
+      t[gg] = I < D::n ? v * sc[I] * sj : 0.0;
     }
     return t;
   }
+  if (J >= D::n) return t;
   for (int gg = 0; gg < 4; ++gg) {
     const int r = 16 * (P - D::TN) + 4 * gg + lh;
     t[gg] = r < kNB ? rb[r * D::n + J] : 0.0;
@@ -490,14 +500,16 @@ __global__ __launch_bounds__(Nd<L>::NT) void nrt_basis_kernel(NrtArgs g) {
     const double a = fabs(dg);
     scl[I] = (a > 0.0 && isfinite(a)) ? 1.0 / sqrt(a) : 1.0;
   }
+  __syncthreads();
   double* rb = sm + D::oRb;
-  for (int J = tid; J < D::n; J += D::NT) {
+  // four lanes per dof J, one per fine cell of J, summed by lane shuffles
+  for (int base = 0; base < 4 * D::n; base += D::NT) {
+    const int idx = base + tid, J = idx >> 2, kc = idx & 3;
     double y[kNB];
 #pragma unroll
-    for (int r = 0; r < kNB; ++r) y[r] = (r < 12 && J >= D::ns) ? -Fc[r * D::nu + (J - D::ns)] : 0.0;
-    for (int kc = 0; kc < 4; ++kc) {
-      const int e = ic[J * 4 + kc];
-      if (e < 0) continue;
+    for (int r = 0; r < kNB; ++r) y[r] = 0.0;
+    const int e = J < D::n ? ic[J * 4 + kc] : -1;
+    if (e >= 0) {
       const int c = e >> 5, lj = e & 31;
       for (int lb = 0; lb < 18; ++lb) {
         const int full = cdl[c * 18 + lb];
@@ -520,9 +532,19 @@ __global__ __launch_bounds__(Nd<L>::NT) void nrt_basis_kernel(NrtArgs g) {
         }
       }
     }
-    const double sj = scl[J];
 #pragma unroll
-    for (int r = 0; r < kNB; ++r) rb[r * D::n + J] = y[r] * sj;
+    for (int r = 0; r < kNB; ++r) {
+      y[r] += __shfl_xor(y[r], 1);
+      y[r] += __shfl_xor(y[r], 2);
+    }
+    if (J < D::n && kc == 0) {
+      const double sj = scl[J];
+#pragma unroll
+      for (int r = 0; r < kNB; ++r) {
+        const double f0 = (r < 12 && J >= D::ns) ? -Fc[r * D::nu + (J - D::ns)] : 0.0;
+        rb[r * D::n + J] = (f0 + y[r]) * sj;
+      }
+    }
   }
   __syncthreads();
 
@@ -550,7 +572,7 @@ __global__ __launch_bounds__(Nd<L>::NT) void nrt_basis_kernel(NrtArgs g) {
     constexpr int s = decltype(sc)::value;
     int P, Q;
     slot_pq(s, P, Q);
-    T[s] = nrt_tile<L>(P, Q, ll, lh, ic, Ml, Dl, Cc, rb, scl);
+    T[s] = nrt_tile<L>(P, Q, lane, sm, rb, scl, g.amap, g.tnz);
   });
   // first pivot columns (0..3) to LDS
   double* stg = sm + D::oStage;
@@ -1025,6 +1047,46 @@ NrtTables nrt_tables(int l) {
   return t;
 }
 
+// The assembly map of nrt_tile: for lower tile (P, Q), register gg and lane
+// (ll, lh) -- the entry (I, J) = (16 P + 4 gg + lh, 16 Q + ll) of K' --
+// the LDS offsets of the local-matrix entries of the fine cells that I and J
+// share, with the sign of K' (every block but the edge block negated).
+template <int L>
+void nrt_amap(const NrtTables& t, std::vector<unsigned short>& amap, std::vector<unsigned char>& tnz) {
+  typedef Nd<L> D;
+  const int ntiles = D::TN * (D::TN + 1) / 2;
+  amap.assign((size_t)ntiles * 4 * 64 * 4, 0xffff);
+  tnz.assign(ntiles, 0);
+  auto tri = [](int i, int j) { const int hi = std::max(i, j), lo = std::min(i, j); return hi * (hi + 1) / 2 + lo; };
+  for (int P = 0; P < D::TN; ++P)
+    for (int Q = 0; Q <= P; ++Q) {
+      const int tix = P * (P + 1) / 2 + Q;
+      for (int gg = 0; gg < 4; ++gg)
+        for (int lane = 0; lane < 64; ++lane) {
+          const int I = 16 * P + 4 * gg + (lane >> 4), J = 16 * Q + (lane & 15);
+          if (I >= t.n || J >= t.n) continue;
+          unsigned short* out = &amap[(((size_t)tix * 4 + gg) * 64 + lane) * 4];
+          int cnt = 0;
+          for (int a = 0; a < 4; ++a) {
+            const int ei = t.icell[(size_t)I * 4 + a];
+            if (ei < 0) continue;
+            for (int b = 0; b < 4; ++b) {
+              const int ej = t.icell[(size_t)J * 4 + b];
+              if (ej < 0 || (ej >> 5) != (ei >> 5)) continue;
+              const int c = ei >> 5, li = ei & 31, lj = ej & 31;
+              int off;
+              if (li < 12 && lj < 12) off = D::oMl + c * 78 + tri(li, lj);
+              else if (li >= 12 && lj >= 12) off = D::oDl + c * 21 + tri(li - 12, lj - 12);
+              else off = D::oCc + (std::max(li, lj) - 12) * 12 + std::min(li, lj);
+              const bool neg = !(li < 12 && lj < 12);
+              out[cnt++] = (unsigned short)(off | (neg ? 0x8000 : 0));
+              tnz[tix] = 1;
+            }
+          }
+        }
+    }
+}
+
 }  // namespace
 
 struct msfec_nrt_batch {
@@ -1039,6 +1101,8 @@ struct msfec_nrt_batch {
   int* d_icell = nullptr;
   int* d_cdof = nullptr;
   int* d_f2i = nullptr;
+  unsigned short* d_amap = nullptr;
+  unsigned char* d_tnz = nullptr;
   double* d_X = nullptr;
   double* d_M = nullptr;
   double* d_rhs = nullptr;
@@ -1060,7 +1124,7 @@ msfec_status nrt_alloc(T** p, size_t count) {
 
 void nrt_free(msfec_nrt_batch* h) {
   void* ptrs[] = {h->d_verts, h->d_ainv, h->d_b, h->d_r,   h->d_f,   h->d_icell,  h->d_cdof,
-                  h->d_f2i,   h->d_X, h->d_M,   h->d_rhs, h->d_w,      h->d_status};
+                  h->d_f2i,   h->d_amap, h->d_tnz, h->d_X, h->d_M,   h->d_rhs, h->d_w,      h->d_status};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (auto& set : h->ev_sets)
@@ -1101,6 +1165,8 @@ msfec_status nrt_launch(msfec_nrt_batch* h, hipStream_t s) {
   a.icell = h->d_icell;
   a.cdof = h->d_cdof;
   a.f2i = h->d_f2i;
+  a.amap = h->d_amap;
+  a.tnz = h->d_tnz;
   a.X = h->d_X;
   a.M = h->d_M;
   a.b = h->d_rhs;
@@ -1179,6 +1245,16 @@ msfec_status msfec_nrt_create(const msfec_nrt_params* params, int n_cells, const
   NRT_TRY(nrt_in(h->stream, h->d_icell, h->tab.icell.data(), h->tab.icell.size() * sizeof(int), MSFEC_HOST));
   NRT_TRY(nrt_in(h->stream, h->d_cdof, h->tab.cdof.data(), h->tab.cdof.size() * sizeof(int), MSFEC_HOST));
   NRT_TRY(nrt_in(h->stream, h->d_f2i, h->tab.f2i.data(), h->tab.f2i.size() * sizeof(int), MSFEC_HOST));
+  {
+    std::vector<unsigned short> amap;
+    std::vector<unsigned char> tnz;
+    if (l == 1) nrt_amap<1>(h->tab, amap, tnz);
+    else nrt_amap<2>(h->tab, amap, tnz);
+    NRT_TRY(nrt_alloc(&h->d_amap, amap.size()));
+    NRT_TRY(nrt_alloc(&h->d_tnz, tnz.size()));
+    NRT_TRY(nrt_in(h->stream, h->d_amap, amap.data(), amap.size() * sizeof(unsigned short), MSFEC_HOST));
+    NRT_TRY(nrt_in(h->stream, h->d_tnz, tnz.data(), tnz.size(), MSFEC_HOST));
+  }
 #undef NRT_TRY
   *out = h;
   return MSFEC_OK;
