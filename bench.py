@@ -429,7 +429,7 @@ def nrt_config(steps: int, warmup: int, check: bool, cpu: bool):
         from oracle import oracle as O
         O.build()
         cores = host_cores()
-        samp = np.random.default_rng(13).choice(n_cells, min(n_cells, 16 * cores) if cpu else 8,
+        samp = np.random.default_rng(13).choice(n_cells, min(n_cells, 128 * cores) if cpu else 8,
                                                 replace=False)
         t0 = time.perf_counter()
         o = O.nrt_basis_batch(NRT_NREF, verts[samp], ainv[samp], bq[samp], None, fq[samp],

@@ -327,3 +327,130 @@ def test_multiscale_driver_uneven_levels(oracle_mod):
         assert np.abs(w[c] - wc).max() < 1e-9 * np.abs(u).max()
         ufo = wc @ Po[c]
         assert np.abs(uf[c] - ufo).max() < 1e-9 * np.abs(ufo).max(), c
+
+
+NRT_DRIVER = os.path.join(REPO, "mpi-msfec_amd", "cpp", "ned_rt_driver")
+
+
+def nrt_driver_inputs(rng, l, ncell):
+    """Seeded SPD A^-1 (6 entries), B and f at the 8 Gauss points of every fine cell."""
+    nq = (1 << l) ** 3
+    Lm = rng.normal(size=(ncell, nq, 8, 3, 3)) * 0.3 + np.eye(3)
+    G = Lm @ np.swapaxes(Lm, -1, -2) + 0.1 * np.eye(3)
+    ainv = np.stack([G[..., 0, 0], G[..., 0, 1], G[..., 0, 2], G[..., 1, 1], G[..., 1, 2],
+                     G[..., 2, 2]], -1)
+    bq = np.exp(rng.normal(size=(ncell, nq, 8)))
+    fq = rng.normal(size=(ncell, nq, 8, 3))
+    return ainv, bq, fq
+
+
+def run_nrt_driver(l, nc, ainv, bq, fq, td, outdir=None):
+    fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+    with open(fin, "wb") as fh:
+        fh.write(np.array([l, nc], dtype=np.int32).tobytes())
+        for x in (ainv, bq, fq):
+            fh.write(np.ascontiguousarray(x, dtype=np.float64).tobytes())
+    args = [NRT_DRIVER, fin, fout] + ([outdir] if outdir else [])
+    r = subprocess.run(args, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    return np.frombuffer(open(fout, "rb").read(), dtype=np.float64)
+
+
+def unit_cube_boxes(oracle_mod, nc):
+    h = 1.0 / nc
+    return np.array([oracle_mod.box_vertices_3d([cx * h, cy * h, cz * h],
+                                                [(cx + 1) * h, (cy + 1) * h, (cz + 1) * h])
+                     for cz in range(nc) for cy in range(nc) for cx in range(nc)])
+
+
+def test_ned_rt_driver(oracle_mod):
+    """The C++ NedRTBasis shim: map of cells, run(), getters and
+    set_global_weights (sigma = sum_{i<12} w_i sigma_i on the edges, u =
+    sum_{i>=12} w_i u_i on the faces, ned_rt_basis.cc:1156-1181) against the
+    oracle."""
+    print("case ned_rt_shim_driver l=2 nc=2", flush=True)
+    assert os.path.exists(NRT_DRIVER), "driver not built"
+    l, nc = 2, 2
+    ncell = nc ** 3
+    m, nx, ne, nf1, nf, N = oracle_mod.nrt_sizes(l)
+    ainv, bq, fq = nrt_driver_inputs(np.random.default_rng(31), l, ncell)
+    with tempfile.TemporaryDirectory() as td:
+        d = run_nrt_driver(l, nc, ainv, bq, fq, td)
+    o = 0
+    M = d[o:o + ncell * 324].reshape(ncell, 18, 18); o += ncell * 324
+    b = d[o:o + ncell * 18].reshape(ncell, 18); o += ncell * 18
+    X = d[o:o + ncell * 18 * N].reshape(ncell, 18, N); o += ncell * 18 * N
+    sol = d[o:o + ncell * N].reshape(ncell, N); o += ncell * N
+    assert o == d.size
+    verts = unit_cube_boxes(oracle_mod, nc)
+    ref = oracle_mod.nrt_basis_batch(l, verts, ainv, bq, None, fq, n_threads=8)
+    assert (ref["status"] == 0).all()
+    assert np.abs(M - ref["M"]).max() < 1e-11 * np.abs(ref["M"]).max()
+    assert np.abs(b - ref["b"]).max() < 1e-11 * np.abs(ref["b"]).max()
+    assert np.abs(X - ref["X"]).max() < 1e-11 * np.abs(ref["X"]).max()
+    w = np.arange(1.0, 19.0)
+    expect = np.concatenate([np.einsum("i,cie->ce", w[:12], X[:, :12, :ne]),
+                             np.einsum("i,cie->ce", w[12:], X[:, 12:, ne:])], axis=1)
+    assert np.abs(sol - expect).max() <= 1e-13 * np.abs(expect).max()
+
+
+def nrt_cell_centre_fields(v, l, h):
+    """Nedelec field (edge line integrals averaged over the 4 parallel edges, over
+    the edge length) and Raviart-Thomas field (face fluxes averaged over the 2
+    opposite faces, over the face area) at the fine cell centres of a cube of
+    fine width h, (m^3, 3) each, cells x fastest."""
+    m = 1 << l
+    nx, nf1 = m * (m + 1) ** 2, m * m * (m + 1)
+    ex = v[:nx].reshape(m + 1, m + 1, m)             # [k, j, i]
+    ey = v[nx:2 * nx].reshape(m + 1, m, m + 1)
+    ez = v[2 * nx:3 * nx].reshape(m, m + 1, m + 1)
+    f = v[3 * nx:]
+    fx = f[:nf1].reshape(m, m, m + 1)
+    fy = f[nf1:2 * nf1].reshape(m, m + 1, m)
+    fz = f[2 * nf1:].reshape(m + 1, m, m)
+    sx = (ex[:-1, :-1] + ex[1:, :-1] + ex[:-1, 1:] + ex[1:, 1:]) / (4 * h)
+    sy = (ey[:-1, :, :-1] + ey[1:, :, :-1] + ey[:-1, :, 1:] + ey[1:, :, 1:]) / (4 * h)
+    sz = (ez[:, :-1, :-1] + ez[:, 1:, :-1] + ez[:, :-1, 1:] + ez[:, 1:, 1:]) / (4 * h)
+    ux = (fx[:, :, :-1] + fx[:, :, 1:]) / (2 * h * h)
+    uy = (fy[:, :-1] + fy[:, 1:]) / (2 * h * h)
+    uz = (fz[:-1] + fz[1:]) / (2 * h * h)
+    sig = np.stack([sx.ravel(), sy.ravel(), sz.ravel()], 1)
+    u = np.stack([ux.ravel(), uy.ravel(), uz.ravel()], 1)
+    return sig, u
+
+
+def test_ned_rt_driver_vtu_output(oracle_mod):
+    """NedRTBasis output_basis (first cell: 12 .curl and 6 .div files) and
+    output_global_solution_in_cell: file names, the Nedelec and Raviart-Thomas
+    fields at the fine cell centres."""
+    print("case ned_rt_shim_driver_vtu l=1 nc=2", flush=True)
+    l, nc = 1, 2
+    ncell = nc ** 3
+    m, nx, ne, nf1, nf, N = oracle_mod.nrt_sizes(l)
+    ainv, bq, fq = nrt_driver_inputs(np.random.default_rng(37), l, ncell)
+    with tempfile.TemporaryDirectory() as td:
+        od = os.path.join(td, "vtu")
+        os.mkdir(od)
+        d = run_nrt_driver(l, nc, ainv, bq, fq, td, od)
+        names = sorted(os.listdir(od))
+        expect = sorted([f"basis_ned-rt.curl.00000.cell-0.index-{i:02d}.vtu" for i in range(12)] +
+                        [f"basis_ned-rt.div.00000.cell-0.index-{i:02d}.vtu" for i in range(6)] +
+                        [f"solution.00000.cell-{c}.vtu" for c in range(ncell)])
+        assert names == expect
+        verts = unit_cube_boxes(oracle_mod, nc)
+        ref = oracle_mod.nrt_basis_batch(l, verts[:1], ainv[:1], bq[:1], None, fq[:1])
+        h = 1.0 / nc / m
+        for i in range(18):
+            kind, j = ("curl", i) if i < 12 else ("div", i - 12)
+            cf = read_vtu_cells(os.path.join(od, f"basis_ned-rt.{kind}.00000.cell-0.index-{j:02d}.vtu"))
+            sig, u = nrt_cell_centre_fields(ref["X"][0, i], l, h)
+            scale = max(np.abs(sig).max(), np.abs(u).max())
+            np.testing.assert_allclose(cf["sigma"], sig, rtol=0, atol=1e-10 * scale)
+            np.testing.assert_allclose(cf["u"], u, rtol=0, atol=1e-10 * scale)
+        sol = d[-ncell * N:].reshape(ncell, N)
+        c = ncell - 1
+        cf = read_vtu_cells(os.path.join(od, f"solution.00000.cell-{c}.vtu"))
+        sig, u = nrt_cell_centre_fields(sol[c], l, h)
+        scale = max(np.abs(sig).max(), np.abs(u).max())
+        np.testing.assert_allclose(cf["sigma"], sig, rtol=0, atol=1e-10 * scale)
+        np.testing.assert_allclose(cf["u"], u, rtol=0, atol=1e-10 * scale)
