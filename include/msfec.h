@@ -507,8 +507,15 @@ msfec_status msfec_rt_destroy(msfec_rt_handle h);
  * The element matrix is 18 x 18 (row = test basis; edge bases first).
  * ------------------------------------------------------------------------- */
 typedef struct {
-  int n_refine_local; /* l, 1..2 (the dense interior system holds up to 256 unknowns) */
+  int n_refine_local; /* l, 1..4                                                      */
   int flags;          /* MSFEC_FLAG_*                                                 */
+  int solver;         /* 0: the dense direct solve for l <= 2 (the interior system of
+                         up to 256 unknowns factored on the matrix cores) and MINRES
+                         above; MSFEC_SOLVER_CG: matrix-free MINRES for every l      */
+  int max_iter;       /* MINRES iteration limit, 0: 20000; a solve that reaches it
+                         flags its cell MSFEC_CELL_NOT_CONVERGED                      */
+  double rtol;        /* MINRES stop: preconditioned residual <= rtol x the initial
+                         one, 0: 1e-14                                                */
 } msfec_nrt_params;
 
 typedef struct msfec_nrt_batch *msfec_nrt_handle;
@@ -517,7 +524,9 @@ typedef struct msfec_nrt_batch *msfec_nrt_handle;
  * deal.II vertex order.  Replaces NedRTBasis::NedRTBasis + setup_grid /
  * setup_system_matrix / setup_basis_dofs_curl / setup_basis_dofs_div
  * (source/Ned_RT/ned_rt_basis.cc:7-359, include/Ned_RT/ned_rt_basis.h).
- * MSFEC_ERR_UNSUPPORTED for l > 2. */
+ * MSFEC_ERR_UNSUPPORTED for l > 4.  The MINRES path holds 6 fine vectors per
+ * basis and 99 doubles per fine cell in device memory (l = 4: 25 MiB a
+ * cell). */
 msfec_status msfec_nrt_create(const msfec_nrt_params *params, int n_cells,
                               const double *vertices, int vertices_loc,
                               msfec_nrt_handle *out);
@@ -557,9 +566,15 @@ msfec_status msfec_nrt_reconstruct(msfec_nrt_handle h, const double *w, int w_lo
 
 /* Per-cell status: MSFEC_CELL_OK, MSFEC_CELL_NOT_SPD when a pivot of the
  * symmetric indefinite factorisation has the wrong sign (edge pivots must be
- * positive, face pivots negative), MSFEC_CELL_NONFINITE for a non-finite
- * element matrix. */
+ * positive, face pivots negative), MSFEC_CELL_NOT_CONVERGED when a MINRES
+ * solve reached max_iter, MSFEC_CELL_NONFINITE for non-finite data. */
 msfec_status msfec_nrt_get_cell_status(msfec_nrt_handle h, int *status);
+
+/* MINRES iterations of the last compute, n_cells x 18 ints on the host (the
+ * reference's solver_control.last_step() of solve_iterative,
+ * ned_rt_basis.cc:796-801); MSFEC_ERR_STATE for a handle on the dense path.
+ * Synchronizes. */
+msfec_status msfec_nrt_get_iterations(msfec_nrt_handle h, int *iterations);
 
 /* Mean time in ms of the basis kernel over the computes made with
  * MSFEC_FLAG_TIMING since the previous call. */

@@ -56,6 +56,9 @@ void NedRTBatch::compute() {
   msfec_nrt_params p{};
   p.n_refine_local = (int)prm_.n_refine_local;
   p.flags = 0;
+  p.solver = prm_.solver;
+  p.max_iter = prm_.max_iter;
+  p.rtol = prm_.rtol;
   check(msfec_nrt_create(&p, (int)n_cells_, verts_.data(), MSFEC_HOST, &handle_), "msfec_nrt_create");
   check(msfec_nrt_set_coefficients(handle_, ainv_.data(), b_.data(), any_r_ ? r_.data() : nullptr,
                                    any_f_ ? f_.data() : nullptr, MSFEC_HOST),

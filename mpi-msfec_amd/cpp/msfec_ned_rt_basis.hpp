@@ -18,7 +18,14 @@
 namespace msfec {
 
 struct ParametersNedRT {
-  unsigned n_refine_local = 2;  // ParametersMs::n_refine_local (1 or 2)
+  unsigned n_refine_local = 2;  // ParametersMs::n_refine_local (1 to 4)
+  // the basis solver (use_direct_solver of the reference's ParametersBasis):
+  // 0 solves l <= 2 directly and runs MINRES above, MSFEC_SOLVER_CG runs
+  // MINRES at every l, to the relative tolerance rtol (0: 1e-14) within
+  // max_iter iterations (0: 20000)
+  int solver = 0;
+  int max_iter = 0;
+  double rtol = 0.0;
   // output (the ParametersBasis fields of include/Ned_RT/ned_rt_parameters.h)
   std::string dirname_output = "Ned_RT";
   std::string filename_output = "solution";

@@ -945,6 +945,524 @@ __global__ __launch_bounds__(256) void nrt_reconstruct_kernel(const double* X, c
     }
 }
 
+// ===========================================================================
+// Beyond the dense limit (n_refine_local 3 and 4, or any l on request): the
+// interior system is never formed.  nrt_it_local_kernel writes the local
+// matrices of every fine cell to HBM once; nrt_it_diag_kernel the
+// preconditioner; nrt_minres_kernel solves each basis in a workgroup of its
+// own by preconditioned MINRES (Paige-Saunders), applying K' row by row from
+// the local matrices of the fine cells that hold the row's dof; and
+// nrt_it_element_kernel forms the element matrix from the assembled products.
+// The reference solves these systems either directly (solve_direct,
+// ned_rt_basis.cc:579-634) or by CG on the face Schur complement
+// (solve_iterative, :636-846); MINRES on the whole symmetric form reaches the
+// direct solution to a relative tolerance on the preconditioned residual.
+
+constexpr int kItMaxL = 4;
+constexpr int kItNT = 512;  // threads of a MINRES workgroup
+constexpr int kItNW = kItNT / 64;
+constexpr int kLM = 99;     // local-matrix entries per fine cell: Me (78), then Du (21)
+
+struct NrtItArgs {
+  const double* verts;
+  const double* ainv;
+  const double* bq;
+  const double* rq;  // or null
+  const double* fq;  // or null
+  const int* idof;   // n: the full index of every interior dof
+  double* LM;        // n_cells x 99 x nfc (entry-major, fine cell fastest)
+  double* CC;        // n_cells x 72: C[F][L] = (psi_F, curl phi_L) of one fine cell
+  double* FL;        // n_cells x nfc x 6: (psi_F, f) per fine cell
+  double* PD;        // n_cells x N: the inverse preconditioner diagonal (0 on the boundary)
+  double* WK;        // n_cells x 18 x 5 x N: MINRES vectors v, r1, r2, w, w2
+  double* X;         // n_cells x 18 x N
+  double* M;         // n_cells x 18 x 18
+  double* b;         // n_cells x 18
+  int* its;          // n_cells x 18: MINRES iterations
+  int* flg;          // n_cells x 18: MSFEC_CELL_* of every solve
+  int* status;       // n_cells
+  int n_cells, N, n, max_iter;
+  double rtol;
+};
+
+__device__ __forceinline__ void it_sizes(const double* v, V3& Hc, V3& h, int m) {
+  Hc = V3{v[3] - v[0], v[7] - v[1], v[14] - v[2]};
+  h = V3{Hc.x / m, Hc.y / m, Hc.z / m};
+}
+
+// full dof of local dof b of fine cell (c0, c1, c2)
+template <int M>
+__device__ __forceinline__ int it_cdof(int c0, int c1, int c2, int b) {
+  constexpr int nx = M * (M + 1) * (M + 1), ne = 3 * nx, nf1 = M * M * (M + 1);
+  if (b < 12) {
+    const int ax = kLAx[b], f1 = kLF1[b], f2 = kLF2[b];
+    if (ax == 0) return ((c2 + f2) * (M + 1) + c1 + f1) * M + c0;
+    if (ax == 1) return nx + ((c2 + f2) * M + c1) * (M + 1) + c0 + f1;
+    return 2 * nx + (c2 * (M + 1) + c1 + f2) * (M + 1) + c0 + f1;
+  }
+  const int F = b - 12, d = F >> 1, s = F & 1;
+  if (d == 0) return ne + (c2 * M + c1) * (M + 1) + c0 + s;
+  if (d == 1) return ne + nf1 + (c2 * (M + 1) + c1 + s) * M + c0;
+  return ne + 2 * nf1 + ((c2 + s) * M + c1) * M + c0;
+}
+
+// Fine cell k (0..3 for an edge, 0..1 for a face) of the dof of type t at
+// (p0, p1, p2) (dof_pos) and the local index a of the dof in it; false when
+// that cell lies outside the coarse cell.  An edge along axis ax sits in the
+// cells below and above it along the two other axes (line (f1, f2) of those
+// cells), a face in the cells on its two sides.
+template <int M>
+__device__ __forceinline__ bool it_cell(int t, int p0, int p1, int p2, int k, int& c0, int& c1,
+                                        int& c2, int& a) {
+  const int u = k & 1, v = k >> 1;
+  if (t == 0) {
+    c0 = p0; c1 = p1 - u; c2 = p2 - v; a = 2 + u + 4 * v;  // lines 2, 3, 6, 7
+  } else if (t == 1) {
+    c0 = p0 - u; c1 = p1; c2 = p2 - v; a = u + 4 * v;      // lines 0, 1, 4, 5
+  } else if (t == 2) {
+    c0 = p0 - u; c1 = p1 - v; c2 = p2; a = 8 + u + 2 * v;  // lines 8 .. 11
+  } else {
+    const int d = t - 3;
+    c0 = p0 - (d == 0 ? u : 0); c1 = p1 - (d == 1 ? u : 0); c2 = p2 - (d == 2 ? u : 0);
+    a = 12 + 2 * d + u;
+    if (k >= 2) return false;
+  }
+  return c0 >= 0 && c0 < M && c1 >= 0 && c1 < M && c2 >= 0 && c2 < M;
+}
+
+template <int M>
+__device__ __forceinline__ bool it_on_boundary(int t, int p0, int p1, int p2) {
+  if (t == 0) return p1 == 0 || p1 == M || p2 == 0 || p2 == M;
+  if (t == 1) return p0 == 0 || p0 == M || p2 == 0 || p2 == M;
+  if (t == 2) return p0 == 0 || p0 == M || p1 == 0 || p1 == M;
+  const int pd = t == 3 ? p0 : (t == 4 ? p1 : p2);
+  return pd == 0 || pd == M;
+}
+
+// entry (a, b) of the local matrix of fine cell c (signs as kloc), from the
+// entry-major HBM copy
+template <bool Orig>
+__device__ __forceinline__ double it_kloc(const double* lm, const double* Cc, int nfc, int c, int a,
+                                          int b) {
+  const bool ea = a < 12, eb = b < 12;
+  const int hi = max(a, b), lo = min(a, b);
+  double v;
+  if (ea && eb) v = lm[(size_t)(hi * (hi + 1) / 2 + lo) * nfc + c];
+  else if (!ea && !eb) v = lm[(size_t)(78 + (hi - 12) * (hi - 11) / 2 + (lo - 12)) * nfc + c];
+  else v = Cc[(hi - 12) * 12 + lo];
+  const bool neg = Orig ? (ea && !eb) : !(ea && eb);
+  return neg ? -v : v;
+}
+
+// Row f of K' (or, with Orig, of the assembled matrix, row = test) times x,
+// over the columns Cols selects (1: edges, 2: faces, 3: all)
+template <int M, bool Orig, int Cols>
+__device__ __forceinline__ double it_row(int f, const double* x, const double* lm, const double* Cc) {
+  constexpr int nfc = M * M * M;
+  int t, p0, p1, p2;
+  dof_pos<M>(f, t, p0, p1, p2);
+  double acc = 0.0;
+  const int nk = t < 3 ? 4 : 2;
+  for (int k = 0; k < nk; ++k) {
+    int c0, c1, c2, a;
+    if (!it_cell<M>(t, p0, p1, p2, k, c0, c1, c2, a)) continue;
+    const int c = (c2 * M + c1) * M + c0;
+#pragma unroll
+    for (int b = 0; b < 18; ++b) {
+      if ((b < 12 && !(Cols & 1)) || (b >= 12 && !(Cols & 2))) continue;
+      acc += it_kloc<Orig>(lm, Cc, nfc, c, a, b) * x[it_cdof<M>(c0, c1, c2, b)];
+    }
+  }
+  return acc;
+}
+
+// (psi_f, curl N_nb) of the face dof of type t at (p0, p1, p2), over its two
+// fine cells (QGauss(2) reduced to its 2 points along the face normal, as
+// both factors depend on that coordinate only)
+template <int M>
+__device__ double it_curl_load(int t, int p0, int p1, int p2, int nb, V3 h, V3 Hc) {
+  double acc = 0.0;
+  for (int k = 0; k < 2; ++k) {
+    int c0, c1, c2, a;
+    if (!it_cell<M>(t, p0, p1, p2, k, c0, c1, c2, a)) continue;
+    const int F = a - 12, ax = F >> 1;
+    for (int qd = 0; qd < 2; ++qd) {
+      const double gd = gauss_pt(qd);
+      const V3 X{ax == 0 ? gd : 0.5, ax == 1 ? gd : 0.5, ax == 2 ? gd : 0.5};
+      const V3 Xc{(c0 + X.x) / M, (c1 + X.y) / M, (c2 + X.z) / M};
+      acc += rt_val(F, X, h) * ned_curl(nb, Xc, Hc, ax);
+    }
+  }
+  return acc * 4.0 * (h.x * h.y * h.z * 0.125);
+}
+
+// The local matrices (edge mass with A^-1 and face div-div + reaction at the
+// 8 Gauss points, the formulas of nrt_basis_kernel), the curl coupling and
+// the loads (psi_F, f) of every fine cell of one coarse cell.
+template <int L>
+__global__ __launch_bounds__(256) void nrt_it_local_kernel(NrtItArgs g) {
+  constexpr int M = 1 << L, nfc = M * M * M;
+  __shared__ double Wt[78 * 8], PP[21 * 8], DD[21];
+  __shared__ int cmp[78];
+  const int cell = blockIdx.x, tid = threadIdx.x;
+  V3 Hc, h;
+  it_sizes(g.verts + (size_t)cell * 24, Hc, h, M);
+  const double w8 = h.x * h.y * h.z * 0.125;
+  for (int idx = tid; idx < 78 * 8 + 21 * 8 + 21 + 78; idx += 256) {
+    if (idx < 78 * 8) {
+      const int e = idx >> 3, q = idx & 7;
+      int i, j;
+      tri_ij(e, i, j);
+      const V3 X = gauss_xyz(q);
+      Wt[idx] = w8 * ned_val(i, X, h) * ned_val(j, X, h);
+    } else if (idx < 78 * 8 + 21 * 8) {
+      const int e = (idx - 78 * 8) >> 3, q = (idx - 78 * 8) & 7;
+      int i, j;
+      tri_ij(e, i, j);
+      const V3 X = gauss_xyz(q);
+      PP[e * 8 + q] = (i >> 1) == (j >> 1) ? w8 * rt_val(i, X, h) * rt_val(j, X, h) : 0.0;
+    } else if (idx < 78 * 8 + 21 * 8 + 21) {
+      int i, j;
+      tri_ij(idx - 78 * 8 - 21 * 8, i, j);
+      DD[idx - 78 * 8 - 21 * 8] = w8 * rt_div(i, h) * rt_div(j, h);
+    } else {
+      const int e = idx - 78 * 8 - 21 * 8 - 21;
+      int i, j;
+      tri_ij(e, i, j);
+      cmp[e] = sym6(kLineAx[i], kLineAx[j]);
+    }
+  }
+  __syncthreads();
+  double* lm = g.LM + (size_t)cell * kLM * nfc;
+  for (int idx = tid; idx < nfc * kLM; idx += 256) {
+    const int e = idx / nfc, c = idx - e * nfc;
+    const size_t gi = (size_t)cell * nfc + c;
+    double acc = 0.0;
+    if (e < 78) {
+      const int ci = cmp[e];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) acc += Wt[e * 8 + q] * g.ainv[(gi * 8 + q) * 6 + ci];
+    } else {
+      const int e2 = e - 78;
+      const double dd = DD[e2];
+#pragma unroll
+      for (int q = 0; q < 8; ++q)
+        acc += g.bq[gi * 8 + q] * dd + (g.rq ? g.rq[gi * 8 + q] : 0.0) * PP[e2 * 8 + q];
+    }
+    lm[(size_t)e * nfc + c] = acc;
+  }
+  for (int idx = tid; idx < 72 + nfc * 6; idx += 256) {
+    if (idx < 72) {
+      const int F = idx / 12, Ln = idx % 12, ax = F >> 1;
+      double acc = 0.0;
+      for (int q = 0; q < 8; ++q) {
+        const V3 X = gauss_xyz(q);
+        acc += rt_val(F, X, h) * ned_curl(Ln, X, h, ax);
+      }
+      g.CC[(size_t)cell * 72 + idx] = acc * w8;
+    } else {
+      const int c = (idx - 72) / 6, F = (idx - 72) % 6, ax = F >> 1;
+      double acc = 0.0;
+      if (g.fq) {
+        const double* f = g.fq + ((size_t)cell * nfc + c) * 24;
+        for (int q = 0; q < 8; ++q) {
+          const V3 X = gauss_xyz(q);
+          acc += rt_val(F, X, h) * f[q * 3 + ax];
+        }
+      }
+      g.FL[((size_t)cell * nfc + c) * 6 + F] = acc * w8;
+    }
+  }
+}
+
+// diagonal of the edge block at full edge dof e: the sum over its fine cells
+template <int M>
+__device__ __forceinline__ double it_edge_diag(int e, const double* lm) {
+  constexpr int nfc = M * M * M;
+  int t, p0, p1, p2;
+  dof_pos<M>(e, t, p0, p1, p2);
+  double d = 0.0;
+  for (int k = 0; k < 4; ++k) {
+    int c0, c1, c2, a;
+    if (it_cell<M>(t, p0, p1, p2, k, c0, c1, c2, a))
+      d += lm[(size_t)(a * (a + 1) / 2 + a) * nfc + (c2 * M + c1) * M + c0];
+  }
+  return d;
+}
+
+// The MINRES preconditioner, the inverse of a positive diagonal: the edge
+// mass diagonal on the edges, and on the faces the diagonal of the Schur
+// complement D + C diag(Me)^-1 C^T (interior edges only).  Zero on the
+// boundary.
+template <int L>
+__global__ __launch_bounds__(256) void nrt_it_diag_kernel(NrtItArgs g) {
+  constexpr int M = 1 << L, nfc = M * M * M;
+  const int cell = blockIdx.x;
+  const double* lm = g.LM + (size_t)cell * kLM * nfc;
+  const double* Cc = g.CC + (size_t)cell * 72;
+  double* pd = g.PD + (size_t)cell * g.N;
+  for (int f = threadIdx.x; f < g.N; f += 256) {
+    int t, p0, p1, p2;
+    dof_pos<M>(f, t, p0, p1, p2);
+    if (it_on_boundary<M>(t, p0, p1, p2)) {
+      pd[f] = 0.0;
+      continue;
+    }
+    double d;
+    if (t < 3) {
+      d = it_edge_diag<M>(f, lm);
+    } else {
+      int cc[2][3], aa[2];
+      bool in[2];
+      d = 0.0;
+      for (int k = 0; k < 2; ++k) {
+        in[k] = it_cell<M>(t, p0, p1, p2, k, cc[k][0], cc[k][1], cc[k][2], aa[k]);
+        if (in[k]) {
+          const int a = aa[k] - 12;
+          d += lm[(size_t)(78 + a * (a + 1) / 2 + a) * nfc + (cc[k][2] * M + cc[k][1]) * M + cc[k][0]];
+        }
+      }
+      // C_fe summed over the fine cells both dofs share
+      for (int k = 0; k < 2; ++k) {
+        if (!in[k]) continue;
+        for (int Ln = 0; Ln < 12; ++Ln) {
+          const int e = it_cdof<M>(cc[k][0], cc[k][1], cc[k][2], Ln);
+          int te, q0, q1, q2;
+          dof_pos<M>(e, te, q0, q1, q2);
+          if (it_on_boundary<M>(te, q0, q1, q2)) continue;
+          double cfe = Cc[(aa[k] - 12) * 12 + Ln];
+          bool dup = false;
+          const int o = 1 - k;
+          if (in[o])
+            for (int L2 = 0; L2 < 12; ++L2)
+              if (it_cdof<M>(cc[o][0], cc[o][1], cc[o][2], L2) == e) {
+                if (k == 1) dup = true;  // counted from cell 0
+                cfe += Cc[(aa[o] - 12) * 12 + L2];
+              }
+          if (!dup) d += cfe * cfe / it_edge_diag<M>(e, lm);
+        }
+      }
+    }
+    pd[f] = (d > 0.0 && isfinite(d)) ? 1.0 / d : 0.0;
+  }
+}
+
+// sum over the workgroup; red holds one value per wave and must not be
+// written again before every thread has read it
+__device__ __forceinline__ double it_block_sum(double v, double* red) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double s = 0.0;
+#pragma unroll
+  for (int i = 0; i < kItNW; ++i) s += red[i];
+  return s;
+}
+
+// One basis of one coarse cell per workgroup.  Workgroups are numbered so
+// that those dispatched to the same XCD (block index mod 8) take the 18
+// bases of one cell one after another, which keeps that cell's local
+// matrices in the XCD's L2.
+template <int L>
+__global__ __launch_bounds__(kItNT) void nrt_minres_kernel(NrtItArgs g) {
+  constexpr int M = 1 << L, nfc = M * M * M;
+  __shared__ double sCc[72];
+  __shared__ double redA[kItNW], redB[kItNW];
+  const int bid = blockIdx.x, slot = bid >> 3;
+  const int cell = (slot / kNB) * 8 + (bid & 7), r = slot % kNB;
+  if (cell >= g.n_cells) return;
+  const int tid = threadIdx.x, N = g.N, n = g.n;
+  V3 Hc, h;
+  it_sizes(g.verts + (size_t)cell * 24, Hc, h, M);
+  const double* lm = g.LM + (size_t)cell * kLM * nfc;
+  const double* pd = g.PD + (size_t)cell * N;
+  double* x = g.X + ((size_t)cell * kNB + r) * N;
+  double* v = g.WK + ((size_t)cell * kNB + r) * 5 * N;
+  double* P = v + N;   // r1
+  double* Q = v + 2 * N;  // r2
+  double* W0 = v + 3 * N;  // w
+  double* W1 = v + 4 * N;  // w2
+  if (tid < 72) sCc[tid] = g.CC[(size_t)cell * 72 + tid];
+  // x = the boundary values, zero inside; the work vectors zero
+  for (int f = tid; f < N; f += kItNT) {
+    int t, p0, p1, p2;
+    dof_pos<M>(f, t, p0, p1, p2);
+    x[f] = it_on_boundary<M>(t, p0, p1, p2) ? bnd_value<M>(f, r) : 0.0;
+    v[f] = 0.0;
+    W0[f] = 0.0;
+    W1[f] = 0.0;
+  }
+  __syncthreads();
+  // r1 = r2 = S F - K' x (the curl loads of the edge bases on the face rows)
+  double acc = 0.0;
+  for (int i = tid; i < n; i += kItNT) {
+    const int f = g.idof[i];
+    double rr = -it_row<M, false, 3>(f, x, lm, sCc);
+    if (r < 12) {
+      int t, p0, p1, p2;
+      dof_pos<M>(f, t, p0, p1, p2);
+      if (t >= 3) rr -= it_curl_load<M>(t, p0, p1, p2, r, h, Hc);
+    }
+    P[f] = rr;
+    Q[f] = rr;
+    acc += rr * rr * pd[f];
+  }
+  const double beta1 = sqrt(it_block_sum(acc, redB));
+  int flag = isfinite(beta1) ? MSFEC_CELL_OK : MSFEC_CELL_NONFINITE;
+  int itn = 0;
+  if (flag == MSFEC_CELL_OK && beta1 > 0.0) {
+    for (int i = tid; i < n; i += kItNT) {
+      const int f = g.idof[i];
+      v[f] = pd[f] * Q[f] / beta1;
+    }
+    __syncthreads();
+    double oldb = 0.0, beta = beta1, dbar = 0.0, epsln = 0.0, phibar = beta1, cs = -1.0, sn = 0.0;
+    const double stop = g.rtol * beta1;
+    flag = MSFEC_CELL_NOT_CONVERGED;
+    while (itn < g.max_iter) {
+      ++itn;
+      // A: t = K' v - (beta / oldb) r1 into r1's slot; alfa = v . t
+      const double c1 = itn > 1 ? beta / oldb : 0.0;
+      acc = 0.0;
+      for (int i = tid; i < n; i += kItNT) {
+        const int f = g.idof[i];
+        const double t = it_row<M, false, 3>(f, v, lm, sCc) - c1 * P[f];
+        P[f] = t;
+        acc += v[f] * t;
+      }
+      const double alfa = it_block_sum(acc, redA);
+      // B: r2' = t - (alfa / beta) r2 into r1's slot, then r1 <-> r2;
+      // beta' = |r2'| in the preconditioner's norm
+      const double c2 = alfa / beta;
+      acc = 0.0;
+      for (int i = tid; i < n; i += kItNT) {
+        const int f = g.idof[i];
+        const double y = P[f] - c2 * Q[f];
+        P[f] = y;
+        acc += y * y * pd[f];
+      }
+      {
+        double* tmp = P;
+        P = Q;
+        Q = tmp;
+      }
+      oldb = beta;
+      beta = sqrt(it_block_sum(acc, redB));
+      // the QR update of the Lanczos tridiagonal (Givens rotations)
+      const double oldeps = epsln;
+      const double delta = cs * dbar + sn * alfa;
+      const double gbar = sn * dbar - cs * alfa;
+      epsln = sn * beta;
+      dbar = -cs * beta;
+      double gamma = sqrt(gbar * gbar + beta * beta);
+      gamma = fmax(gamma, 1e-300);
+      cs = gbar / gamma;
+      sn = beta / gamma;
+      const double phi = cs * phibar;
+      phibar = sn * phibar;
+      const double denom = 1.0 / gamma;
+      const double sb = beta > 0.0 ? 1.0 / beta : 0.0;
+      // C: w' = (v - oldeps w2 - delta w) / gamma into w2's slot, then
+      // w <-> w2; x += phi w'; the next v = M^-1 r2 / beta
+      for (int i = tid; i < n; i += kItNT) {
+        const int f = g.idof[i];
+        const double wn = (v[f] - oldeps * W1[f] - delta * W0[f]) * denom;
+        W1[f] = wn;
+        x[f] += phi * wn;
+        v[f] = pd[f] * Q[f] * sb;
+      }
+      {
+        double* tmp = W0;
+        W0 = W1;
+        W1 = tmp;
+      }
+      __syncthreads();
+      if (!isfinite(phibar) || !isfinite(beta)) {
+        flag = MSFEC_CELL_NONFINITE;
+        break;
+      }
+      if (phibar <= stop || beta == 0.0) {
+        flag = MSFEC_CELL_OK;
+        break;
+      }
+    }
+  }
+  // the assembled matrix times the block of this basis that enters the
+  // element matrix (edges for an edge basis, faces for a face basis), over
+  // every row, into r1's slot of the basis for nrt_it_element_kernel
+  double* Z = g.WK + ((size_t)cell * kNB + r) * 5 * N + N;
+  for (int f = tid; f < N; f += kItNT)
+    Z[f] = r < 12 ? it_row<M, true, 1>(f, x, lm, sCc) : it_row<M, true, 2>(f, x, lm, sCc);
+  if (tid == 0) {
+    g.its[(size_t)cell * kNB + r] = itn;
+    g.flg[(size_t)cell * kNB + r] = flag;
+  }
+}
+
+// Row i of the element matrix and load of one coarse cell per workgroup
+// (assemble_global_element_matrix, ned_rt_basis.cc:850-948): M(i, j) = the
+// block of basis i times the assembled matrix times the block of basis j,
+// the products of nrt_minres_kernel; b(i) = the face block of a face basis
+// times the load (v, f).
+template <int L>
+__global__ __launch_bounds__(256) void nrt_it_element_kernel(NrtItArgs g) {
+  constexpr int M = 1 << L, nfc = M * M * M;
+  constexpr int nx = M * (M + 1) * (M + 1), ne = 3 * nx;
+  __shared__ double red[4][kNB + 1];
+  const int cell = blockIdx.x / kNB, i = blockIdx.x % kNB, tid = threadIdx.x;
+  const int N = g.N;
+  const double* xi = g.X + ((size_t)cell * kNB + i) * N;
+  const double* fl = g.FL + (size_t)cell * nfc * 6;
+  double acc[kNB + 1];
+#pragma unroll
+  for (int j = 0; j <= kNB; ++j) acc[j] = 0.0;
+  const int r0 = i < 12 ? 0 : ne, r1 = i < 12 ? ne : N;
+  for (int f = r0 + tid; f < r1; f += 256) {
+    const double xv = xi[f];
+#pragma unroll
+    for (int j = 0; j < kNB; ++j) acc[j] += xv * g.WK[(((size_t)cell * kNB + j) * 5 + 1) * N + f];
+    if (i >= 12) {
+      int t, p0, p1, p2;
+      dof_pos<M>(f, t, p0, p1, p2);
+      double lf = 0.0;
+      for (int k = 0; k < 2; ++k) {
+        int c0, c1, c2, a;
+        if (it_cell<M>(t, p0, p1, p2, k, c0, c1, c2, a))
+          lf += fl[((c2 * M + c1) * M + c0) * 6 + a - 12];
+      }
+      acc[kNB] += xv * lf;
+    }
+  }
+#pragma unroll
+  for (int j = 0; j <= kNB; ++j) {
+    double s = acc[j];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    if ((tid & 63) == 0) red[tid >> 6][j] = s;
+  }
+  __syncthreads();
+  if (tid <= kNB) {
+    const double s = red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid];
+    if (tid < kNB) g.M[((size_t)cell * kNB + i) * kNB + tid] = s;
+    else g.b[(size_t)cell * kNB + i] = s;
+  }
+}
+
+// Per-cell status from the 18 solves and the element matrix.
+__global__ __launch_bounds__(256) void nrt_it_status_kernel(NrtItArgs g) {
+  const int cell = blockIdx.x * 256 + threadIdx.x;
+  if (cell >= g.n_cells) return;
+  int st = MSFEC_CELL_OK;
+  for (int r = 0; r < kNB; ++r) {
+    const int f = g.flg[(size_t)cell * kNB + r];
+    if (f != MSFEC_CELL_OK && st == MSFEC_CELL_OK) st = f;
+  }
+  for (int k = 0; k < kNB * kNB && st == MSFEC_CELL_OK; ++k)
+    if (!isfinite(g.M[(size_t)cell * kNB * kNB + k])) st = MSFEC_CELL_NONFINITE;
+  g.status[cell] = st;
+}
+
 msfec_status nrt_fail(msfec_status st, const std::string& msg) {
   msfec_detail::g_last_error = msg;
   return st;
@@ -1108,6 +1626,16 @@ struct msfec_nrt_batch {
   double* d_rhs = nullptr;
   double* d_w = nullptr;
   int* d_status = nullptr;
+  // MINRES path
+  bool iter = false;
+  int* d_idof = nullptr;
+  double* d_LM = nullptr;
+  double* d_CC = nullptr;
+  double* d_FL = nullptr;
+  double* d_PD = nullptr;
+  double* d_WK = nullptr;
+  int* d_its = nullptr;
+  int* d_flg = nullptr;
   bool coef_set = false, has_r = false, has_f = false, computed = false;
   hipStream_t stream = nullptr;
   std::vector<std::array<hipEvent_t, 2>> ev_sets;
@@ -1123,8 +1651,10 @@ msfec_status nrt_alloc(T** p, size_t count) {
 }
 
 void nrt_free(msfec_nrt_batch* h) {
-  void* ptrs[] = {h->d_verts, h->d_ainv, h->d_b, h->d_r,   h->d_f,   h->d_icell,  h->d_cdof,
-                  h->d_f2i,   h->d_amap, h->d_tnz, h->d_X, h->d_M,   h->d_rhs, h->d_w,      h->d_status};
+  void* ptrs[] = {h->d_verts, h->d_ainv, h->d_b,  h->d_r,   h->d_f,  h->d_icell, h->d_cdof,
+                  h->d_f2i,   h->d_amap, h->d_tnz, h->d_X,  h->d_M,  h->d_rhs,   h->d_w,
+                  h->d_status, h->d_idof, h->d_LM, h->d_CC, h->d_FL, h->d_PD,    h->d_WK,
+                  h->d_its,   h->d_flg};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (auto& set : h->ev_sets)
@@ -1177,6 +1707,47 @@ msfec_status nrt_launch(msfec_nrt_batch* h, hipStream_t s) {
   return MSFEC_OK;
 }
 
+template <int L>
+msfec_status nrt_it_launch(msfec_nrt_batch* h, hipStream_t s) {
+  NrtItArgs a{};
+  a.verts = h->d_verts;
+  a.ainv = h->d_ainv;
+  a.bq = h->d_b;
+  a.rq = h->has_r ? h->d_r : nullptr;
+  a.fq = h->has_f ? h->d_f : nullptr;
+  a.idof = h->d_idof;
+  a.LM = h->d_LM;
+  a.CC = h->d_CC;
+  a.FL = h->d_FL;
+  a.PD = h->d_PD;
+  a.WK = h->d_WK;
+  a.X = h->d_X;
+  a.M = h->d_M;
+  a.b = h->d_rhs;
+  a.its = h->d_its;
+  a.flg = h->d_flg;
+  a.status = h->d_status;
+  a.n_cells = h->n_cells;
+  a.N = h->tab.N;
+  a.n = h->tab.n;
+  a.max_iter = h->prm.max_iter > 0 ? h->prm.max_iter : 20000;
+  a.rtol = h->prm.rtol > 0.0 ? h->prm.rtol : 1e-14;
+  const int nc = h->n_cells;
+  hipLaunchKernelGGL(nrt_it_local_kernel<L>, dim3(nc), dim3(256), 0, s, a);
+  NRT_HIP(hipGetLastError());
+  hipLaunchKernelGGL(nrt_it_diag_kernel<L>, dim3(nc), dim3(256), 0, s, a);
+  NRT_HIP(hipGetLastError());
+  // cells in groups of 8 (one per XCD), 18 bases each
+  const int groups = (nc + 7) / 8;
+  hipLaunchKernelGGL(nrt_minres_kernel<L>, dim3(groups * 8 * kNB), dim3(kItNT), 0, s, a);
+  NRT_HIP(hipGetLastError());
+  hipLaunchKernelGGL(nrt_it_element_kernel<L>, dim3(nc * kNB), dim3(256), 0, s, a);
+  NRT_HIP(hipGetLastError());
+  hipLaunchKernelGGL(nrt_it_status_kernel, dim3((nc + 255) / 256), dim3(256), 0, s, a);
+  NRT_HIP(hipGetLastError());
+  return MSFEC_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1188,9 +1759,11 @@ msfec_status msfec_nrt_create(const msfec_nrt_params* params, int n_cells, const
   if (n_cells <= 0) return nrt_fail(MSFEC_ERR_INVALID, "n_cells must be positive");
   const int l = params->n_refine_local;
   if (l < 1) return nrt_fail(MSFEC_ERR_INVALID, "n_refine_local must be at least 1");
-  if (l > kMaxL)
-    return nrt_fail(MSFEC_ERR_UNSUPPORTED,
-                    "n_refine_local above 2: the dense interior system exceeds 256 unknowns");
+  if (l > kItMaxL) return nrt_fail(MSFEC_ERR_UNSUPPORTED, "n_refine_local above 4");
+  if (params->solver != 0 && params->solver != MSFEC_SOLVER_CG)
+    return nrt_fail(MSFEC_ERR_INVALID, "solver must be 0 or MSFEC_SOLVER_CG");
+  if (params->max_iter < 0 || !(params->rtol >= 0.0))
+    return nrt_fail(MSFEC_ERR_INVALID, "max_iter and rtol must not be negative");
   if (vertices_loc != MSFEC_HOST && vertices_loc != MSFEC_DEVICE)
     return nrt_fail(MSFEC_ERR_INVALID, "bad vertices location");
   std::vector<double> hv((size_t)n_cells * 24);
@@ -1217,6 +1790,7 @@ msfec_status msfec_nrt_create(const msfec_nrt_params* params, int n_cells, const
   h->n_cells = n_cells;
   h->l = l;
   h->tab = nrt_tables(l);
+  h->iter = l > kMaxL || params->solver == MSFEC_SOLVER_CG;
   const size_t nc = (size_t)n_cells, nfc = (size_t)1 << (3 * l);
   msfec_status st = MSFEC_OK;
 #define NRT_TRY(x)        \
@@ -1245,7 +1819,21 @@ msfec_status msfec_nrt_create(const msfec_nrt_params* params, int n_cells, const
   NRT_TRY(nrt_in(h->stream, h->d_icell, h->tab.icell.data(), h->tab.icell.size() * sizeof(int), MSFEC_HOST));
   NRT_TRY(nrt_in(h->stream, h->d_cdof, h->tab.cdof.data(), h->tab.cdof.size() * sizeof(int), MSFEC_HOST));
   NRT_TRY(nrt_in(h->stream, h->d_f2i, h->tab.f2i.data(), h->tab.f2i.size() * sizeof(int), MSFEC_HOST));
-  {
+  if (h->iter) {
+    const size_t N = (size_t)h->tab.N;
+    std::vector<int> idof;
+    for (int f = 0; f < h->tab.N; ++f)
+      if (h->tab.f2i[f] >= 0) idof.push_back(f);
+    NRT_TRY(nrt_alloc(&h->d_idof, idof.size()));
+    NRT_TRY(nrt_in(h->stream, h->d_idof, idof.data(), idof.size() * sizeof(int), MSFEC_HOST));
+    NRT_TRY(nrt_alloc(&h->d_LM, nc * kLM * nfc));
+    NRT_TRY(nrt_alloc(&h->d_CC, nc * 72));
+    NRT_TRY(nrt_alloc(&h->d_FL, nc * nfc * 6));
+    NRT_TRY(nrt_alloc(&h->d_PD, nc * N));
+    NRT_TRY(nrt_alloc(&h->d_WK, nc * kNB * 5 * N));
+    NRT_TRY(nrt_alloc(&h->d_its, nc * kNB));
+    NRT_TRY(nrt_alloc(&h->d_flg, nc * kNB));
+  } else {
     std::vector<unsigned short> amap;
     std::vector<unsigned char> tnz;
     if (l == 1) nrt_amap<1>(h->tab, amap, tnz);
@@ -1291,7 +1879,17 @@ msfec_status msfec_nrt_compute(msfec_nrt_handle h, void* stream) {
     ev = &h->ev_sets[h->ev_used++];
     NRT_HIP(hipEventRecord((*ev)[0], s));
   }
-  const msfec_status st = h->l == 1 ? nrt_launch<1>(h, s) : nrt_launch<2>(h, s);
+  msfec_status st;
+  if (h->iter) {
+    switch (h->l) {
+      case 1: st = nrt_it_launch<1>(h, s); break;
+      case 2: st = nrt_it_launch<2>(h, s); break;
+      case 3: st = nrt_it_launch<3>(h, s); break;
+      default: st = nrt_it_launch<4>(h, s); break;
+    }
+  } else {
+    st = h->l == 1 ? nrt_launch<1>(h, s) : nrt_launch<2>(h, s);
+  }
   if (st != MSFEC_OK) return st;
   if (ev) NRT_HIP(hipEventRecord((*ev)[1], s));
   h->computed = true;
@@ -1368,6 +1966,15 @@ msfec_status msfec_nrt_get_cell_status(msfec_nrt_handle h, int* status) {
   if (!h->computed) return nrt_fail(MSFEC_ERR_STATE, "compute not called");
   NRT_HIP(hipStreamSynchronize(h->stream));
   NRT_HIP(hipMemcpy(status, h->d_status, (size_t)h->n_cells * sizeof(int), hipMemcpyDeviceToHost));
+  return MSFEC_OK;
+}
+
+msfec_status msfec_nrt_get_iterations(msfec_nrt_handle h, int* iterations) {
+  if (!h || !iterations) return nrt_fail(MSFEC_ERR_INVALID, "null argument");
+  if (!h->iter) return nrt_fail(MSFEC_ERR_STATE, "the handle solves directly (no MINRES iterations)");
+  if (!h->computed) return nrt_fail(MSFEC_ERR_STATE, "compute not called");
+  NRT_HIP(hipStreamSynchronize(h->stream));
+  NRT_HIP(hipMemcpy(iterations, h->d_its, (size_t)h->n_cells * kNB * sizeof(int), hipMemcpyDeviceToHost));
   return MSFEC_OK;
 }
 

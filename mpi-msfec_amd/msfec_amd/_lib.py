@@ -38,8 +38,8 @@ EXPORTED = (
     "msfec_rt_get_cell_status", "msfec_rt_kernel_times", "msfec_rt_destroy",
     "msfec_nrt_create", "msfec_nrt_set_coefficients", "msfec_nrt_compute",
     "msfec_nrt_synchronize", "msfec_nrt_get_element", "msfec_nrt_get_basis",
-    "msfec_nrt_reconstruct", "msfec_nrt_get_cell_status", "msfec_nrt_kernel_times",
-    "msfec_nrt_destroy",
+    "msfec_nrt_reconstruct", "msfec_nrt_get_cell_status", "msfec_nrt_get_iterations",
+    "msfec_nrt_kernel_times", "msfec_nrt_destroy",
     "msfec_qu_create", "msfec_qu_offsets", "msfec_qu_set_coefficient", "msfec_qu_set_rhs",
     "msfec_qu_set_reaction", "msfec_qu_tune", "msfec_qu_level_info", "msfec_qu_level_handle",
     "msfec_qu_compute", "msfec_qu_synchronize", "msfec_qu_get_element", "msfec_qu_get_basis",
@@ -88,6 +88,9 @@ class NrtParams(ctypes.Structure):
     _fields_ = [
         ("n_refine_local", ctypes.c_int),
         ("flags", ctypes.c_int),
+        ("solver", ctypes.c_int),
+        ("max_iter", ctypes.c_int),
+        ("rtol", ctypes.c_double),
     ]
 
 
@@ -180,6 +183,7 @@ def load(path: str | None = None):
     L.msfec_nrt_get_basis.argtypes = [vp, vp, ctypes.c_int]
     L.msfec_nrt_reconstruct.argtypes = [vp, vp, ctypes.c_int, vp, vp, ctypes.c_int]
     L.msfec_nrt_get_cell_status.argtypes = [vp, ip]
+    L.msfec_nrt_get_iterations.argtypes = [vp, ip]
     L.msfec_nrt_kernel_times.argtypes = [vp, ctypes.POINTER(ctypes.c_float), ctypes.c_int, ip]
     L.msfec_nrt_destroy.argtypes = [vp]
     sp = ctypes.POINTER(ctypes.c_size_t)

@@ -26,13 +26,18 @@ def nrt_sizes(n_refine_local):
 
 
 class NedRTBasisBatch:
-    def __init__(self, vertices, n_refine_local=2, timing=False):
-        """vertices: n_cells x 8 x 3, axis-aligned boxes in deal.II vertex order."""
+    def __init__(self, vertices, n_refine_local=2, timing=False, solver=0, max_iter=0, rtol=0.0):
+        """vertices: n_cells x 8 x 3, axis-aligned boxes in deal.II vertex order.
+        solver 0: the dense direct solve for l <= 2, MINRES above;
+        SOLVER_CG: MINRES at every l (max_iter 0: 20000, rtol 0: 1e-14)."""
         self._lib = L.load()
         self._h = ctypes.c_void_p()
         p = L.NrtParams()
         p.n_refine_local = n_refine_local
         p.flags = L.FLAG_TIMING if timing else 0
+        p.solver = solver
+        p.max_iter = max_iter
+        p.rtol = rtol
         vptr, vloc, keep = _ptr(vertices)
         n_cells = int(np.prod(getattr(vertices, "shape", np.shape(vertices)))) // 24
         L.check(self._lib.msfec_nrt_create(ctypes.byref(p), n_cells, vptr, vloc,
@@ -111,6 +116,13 @@ class NedRTBasisBatch:
         L.check(self._lib.msfec_nrt_get_cell_status(
             self._h, st.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
         return st
+
+    def get_iterations(self):
+        """MINRES iterations of every basis solve, n_cells x 18 (MINRES path only)."""
+        its = np.empty((self.n_cells, 18), dtype=np.int32)
+        L.check(self._lib.msfec_nrt_get_iterations(
+            self._h, its.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
+        return its
 
     def kernel_times(self):
         ms = (ctypes.c_float * 2)()

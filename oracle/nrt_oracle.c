@@ -47,8 +47,8 @@ typedef struct {
   double x0[3], H[3], h[3];
 } nrt_mesh;
 
-static int nrt_mesh_init(int l, const double *v, nrt_mesh *g) {
-  if (l < 1 || l > 3) return 1;
+static int nrt_mesh_init(int l, int lmax, const double *v, nrt_mesh *g) {
+  if (l < 1 || l > lmax) return 1;
   g->m = 1 << l;
   const int m = g->m;
   g->nx = m * (m + 1) * (m + 1);
@@ -414,14 +414,14 @@ static int nrt_solve(int n, const double *S, int nr, double *B) {
 int oq_nrt_assemble(int l, const double *verts, const double *ainv, const double *bq,
                     const double *rq, const double *fq, double *A, double *F) {
   nrt_mesh g;
-  if (nrt_mesh_init(l, verts, &g)) return 1;
+  if (nrt_mesh_init(l, 4, verts, &g)) return 1;
   nrt_assemble(&g, ainv, bq, rq, fq, A, F);
   return 0;
 }
 
 int oq_nrt_curl_loads(int l, const double *verts, double *Fc) {
   nrt_mesh g;
-  if (nrt_mesh_init(l, verts, &g)) return 1;
+  if (nrt_mesh_init(l, 4, verts, &g)) return 1;
   nrt_curl_loads(&g, Fc);
   return 0;
 }
@@ -429,7 +429,7 @@ int oq_nrt_curl_loads(int l, const double *verts, double *Fc) {
 int oq_nrt_basis_cell(int l, const double *verts, const double *ainv, const double *bq,
                       const double *rq, const double *fq, double *M, double *b, double *X) {
   nrt_mesh g;
-  if (nrt_mesh_init(l, verts, &g)) return 1;
+  if (nrt_mesh_init(l, 3, verts, &g)) return 1;
   const int N = g.N, ne = g.ne;
   double *A = (double *)malloc(sizeof(double) * (size_t)N * N);
   double *F = (double *)malloc(sizeof(double) * N);

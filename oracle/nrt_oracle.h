@@ -62,12 +62,13 @@ int oq_nrt_basis_cell(int l, const double *verts, const double *ainv, const doub
                       const double *rq, const double *fq, double *M, double *b, double *X);
 
 /* The assembled matrix of assemble_system over all fine dofs ((ne + nf)^2,
- * row = test function) and the load (v, f) (zero on the edges). */
+ * row = test function) and the load (v, f) (zero on the edges); l in 1..4
+ * (l = 4: 5.8 GB). */
 int oq_nrt_assemble(int l, const double *verts, const double *ainv, const double *bq,
                     const double *rq, const double *fq, double *A, double *F);
 
 /* The loads (v, curl N_n) of the 12 edge bases (12 x (ne + nf), zero on the
- * edges), by the same quadrature as assemble_system. */
+ * edges), by the same quadrature as assemble_system; l in 1..4. */
 int oq_nrt_curl_loads(int l, const double *verts, double *Fc);
 
 #ifdef __cplusplus

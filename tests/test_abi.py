@@ -259,10 +259,11 @@ def test_uneven_create_rejects_on_the_host(lib):
         assert st == want, (fields, levels, st, L.msfec_last_error())
 
 
-def _nrt_create(lib, l=2, verts=None):
+def _nrt_create(lib, l=2, verts=None, solver=0, max_iter=0, rtol=0.0):
     L = lib.load()
     p = lib.NrtParams()
     p.n_refine_local, p.flags = l, 0
+    p.solver, p.max_iter, p.rtol = solver, max_iter, rtol
     if verts is None:
         verts = box_vertices(3, [0.0] * 3, [1.0] * 3)[None]
     verts = np.ascontiguousarray(verts, dtype=np.float64)
@@ -279,8 +280,14 @@ def test_nrt_rejects_non_box_and_bad_level(lib):
     assert st == lib.ERR_INVALID and "axis-aligned" in msg
     st, msg = _nrt_create(lib, l=0)
     assert st == lib.ERR_INVALID
-    st, msg = _nrt_create(lib, l=3)
-    assert st == lib.ERR_UNSUPPORTED and "256 unknowns" in msg
+    st, msg = _nrt_create(lib, l=5)
+    assert st == lib.ERR_UNSUPPORTED and "above 4" in msg
+    st, msg = _nrt_create(lib, solver=lib.SOLVER_BAND_MIXED)
+    assert st == lib.ERR_INVALID and "solver" in msg
+    st, msg = _nrt_create(lib, max_iter=-1)
+    assert st == lib.ERR_INVALID
+    st, msg = _nrt_create(lib, rtol=float("nan"))
+    assert st == lib.ERR_INVALID
     L = lib.load()
     assert L.msfec_nrt_create(None, 1, None, 0, None) == lib.ERR_INVALID
     assert L.msfec_nrt_destroy(None) == lib.OK
