@@ -576,8 +576,10 @@ msfec_status msfec_nrt_get_cell_status(msfec_nrt_handle h, int *status);
  * Synchronizes. */
 msfec_status msfec_nrt_get_iterations(msfec_nrt_handle h, int *iterations);
 
-/* Mean time in ms of the basis kernel over the computes made with
- * MSFEC_FLAG_TIMING since the previous call. */
+/* Mean times in ms over the computes made with MSFEC_FLAG_TIMING since the
+ * previous call: the whole compute (the dense path's one kernel) and, on the
+ * MINRES path, its local-matrix + preconditioner, MINRES and element-matrix
+ * parts (up to max values, *n of them written). */
 msfec_status msfec_nrt_kernel_times(msfec_nrt_handle h, float *ms, int max, int *n);
 msfec_status msfec_nrt_destroy(msfec_nrt_handle h);
 

@@ -959,7 +959,7 @@ __global__ __launch_bounds__(256) void nrt_reconstruct_kernel(const double* X, c
 // direct solution to a relative tolerance on the preconditioned residual.
 
 constexpr int kItMaxL = 4;
-constexpr int kItNT = 512;  // threads of a MINRES workgroup
+constexpr int kItNT = 1024;  // threads of a MINRES workgroup
 constexpr int kItNW = kItNT / 64;
 constexpr int kLM = 99;     // local-matrix entries per fine cell: Me (78), then Du (21)
 
@@ -1072,6 +1072,28 @@ __device__ __forceinline__ double it_row(int f, const double* x, const double* l
       if ((b < 12 && !(Cols & 1)) || (b >= 12 && !(Cols & 2))) continue;
       acc += it_kloc<Orig>(lm, Cc, nfc, c, a, b) * x[it_cdof<M>(c0, c1, c2, b)];
     }
+  }
+  return acc;
+}
+
+// Row f of K' times x for an interior dof: all 4 (edge) or 2 (face) fine
+// cells exist, so no bounds checks; one cell's 18 terms at a time.
+template <int M>
+__device__ __forceinline__ double it_row_interior(int f, const double* __restrict__ x,
+                                                  const double* __restrict__ lm, const double* Cc) {
+  constexpr int nfc = M * M * M;
+  int t, p0, p1, p2;
+  dof_pos<M>(f, t, p0, p1, p2);
+  double acc = 0.0;
+  const int nk = t < 3 ? 4 : 2;
+#pragma unroll 1
+  for (int k = 0; k < nk; ++k) {
+    int c0, c1, c2, a;
+    it_cell<M>(t, p0, p1, p2, k, c0, c1, c2, a);
+    const int c = (c2 * M + c1) * M + c0;
+#pragma unroll
+    for (int b = 0; b < 18; ++b)
+      acc += it_kloc<false>(lm, Cc, nfc, c, a, b) * x[it_cdof<M>(c0, c1, c2, b)];
   }
   return acc;
 }
@@ -1267,7 +1289,6 @@ __device__ __forceinline__ double it_block_sum(double v, double* red) {
 template <int L>
 __global__ __launch_bounds__(kItNT) void nrt_minres_kernel(NrtItArgs g) {
   constexpr int M = 1 << L, nfc = M * M * M;
-  __shared__ double sCc[72];
   __shared__ double redA[kItNW], redB[kItNW];
   const int bid = blockIdx.x, slot = bid >> 3;
   const int cell = (slot / kNB) * 8 + (bid & 7), r = slot % kNB;
@@ -1277,13 +1298,13 @@ __global__ __launch_bounds__(kItNT) void nrt_minres_kernel(NrtItArgs g) {
   it_sizes(g.verts + (size_t)cell * 24, Hc, h, M);
   const double* lm = g.LM + (size_t)cell * kLM * nfc;
   const double* pd = g.PD + (size_t)cell * N;
+  const double* gCc = g.CC + (size_t)cell * 72;  // uniform: read through the scalar cache
   double* x = g.X + ((size_t)cell * kNB + r) * N;
   double* v = g.WK + ((size_t)cell * kNB + r) * 5 * N;
   double* P = v + N;   // r1
   double* Q = v + 2 * N;  // r2
   double* W0 = v + 3 * N;  // w
   double* W1 = v + 4 * N;  // w2
-  if (tid < 72) sCc[tid] = g.CC[(size_t)cell * 72 + tid];
   // x = the boundary values, zero inside; the work vectors zero
   for (int f = tid; f < N; f += kItNT) {
     int t, p0, p1, p2;
@@ -1298,7 +1319,7 @@ __global__ __launch_bounds__(kItNT) void nrt_minres_kernel(NrtItArgs g) {
   double acc = 0.0;
   for (int i = tid; i < n; i += kItNT) {
     const int f = g.idof[i];
-    double rr = -it_row<M, false, 3>(f, x, lm, sCc);
+    double rr = -it_row<M, false, 3>(f, x, lm, gCc);
     if (r < 12) {
       int t, p0, p1, p2;
       dof_pos<M>(f, t, p0, p1, p2);
@@ -1327,7 +1348,7 @@ __global__ __launch_bounds__(kItNT) void nrt_minres_kernel(NrtItArgs g) {
       acc = 0.0;
       for (int i = tid; i < n; i += kItNT) {
         const int f = g.idof[i];
-        const double t = it_row<M, false, 3>(f, v, lm, sCc) - c1 * P[f];
+        const double t = it_row_interior<M>(f, v, lm, gCc) - c1 * P[f];
         P[f] = t;
         acc += v[f] * t;
       }
@@ -1393,7 +1414,7 @@ __global__ __launch_bounds__(kItNT) void nrt_minres_kernel(NrtItArgs g) {
   // every row, into r1's slot of the basis for nrt_it_element_kernel
   double* Z = g.WK + ((size_t)cell * kNB + r) * 5 * N + N;
   for (int f = tid; f < N; f += kItNT)
-    Z[f] = r < 12 ? it_row<M, true, 1>(f, x, lm, sCc) : it_row<M, true, 2>(f, x, lm, sCc);
+    Z[f] = r < 12 ? it_row<M, true, 1>(f, x, lm, gCc) : it_row<M, true, 2>(f, x, lm, gCc);
   if (tid == 0) {
     g.its[(size_t)cell * kNB + r] = itn;
     g.flg[(size_t)cell * kNB + r] = flag;
@@ -1638,7 +1659,9 @@ struct msfec_nrt_batch {
   int* d_flg = nullptr;
   bool coef_set = false, has_r = false, has_f = false, computed = false;
   hipStream_t stream = nullptr;
-  std::vector<std::array<hipEvent_t, 2>> ev_sets;
+  // per timed compute: start, after the local matrices and preconditioner,
+  // after MINRES, end (the middle two only on the MINRES path)
+  std::vector<std::array<hipEvent_t, 4>> ev_sets;
   size_t ev_used = 0;
 };
 
@@ -1708,7 +1731,7 @@ msfec_status nrt_launch(msfec_nrt_batch* h, hipStream_t s) {
 }
 
 template <int L>
-msfec_status nrt_it_launch(msfec_nrt_batch* h, hipStream_t s) {
+msfec_status nrt_it_launch(msfec_nrt_batch* h, hipStream_t s, std::array<hipEvent_t, 4>* ev) {
   NrtItArgs a{};
   a.verts = h->d_verts;
   a.ainv = h->d_ainv;
@@ -1737,10 +1760,12 @@ msfec_status nrt_it_launch(msfec_nrt_batch* h, hipStream_t s) {
   NRT_HIP(hipGetLastError());
   hipLaunchKernelGGL(nrt_it_diag_kernel<L>, dim3(nc), dim3(256), 0, s, a);
   NRT_HIP(hipGetLastError());
+  if (ev) NRT_HIP(hipEventRecord((*ev)[1], s));
   // cells in groups of 8 (one per XCD), 18 bases each
   const int groups = (nc + 7) / 8;
   hipLaunchKernelGGL(nrt_minres_kernel<L>, dim3(groups * 8 * kNB), dim3(kItNT), 0, s, a);
   NRT_HIP(hipGetLastError());
+  if (ev) NRT_HIP(hipEventRecord((*ev)[2], s));
   hipLaunchKernelGGL(nrt_it_element_kernel<L>, dim3(nc * kNB), dim3(256), 0, s, a);
   NRT_HIP(hipGetLastError());
   hipLaunchKernelGGL(nrt_it_status_kernel, dim3((nc + 255) / 256), dim3(256), 0, s, a);
@@ -1869,10 +1894,10 @@ msfec_status msfec_nrt_compute(msfec_nrt_handle h, void* stream) {
   if (!h->coef_set) return nrt_fail(MSFEC_ERR_STATE, "coefficients not set");
   hipStream_t s = (hipStream_t)stream;
   h->stream = s;
-  std::array<hipEvent_t, 2>* ev = nullptr;
+  std::array<hipEvent_t, 4>* ev = nullptr;
   if (h->prm.flags & MSFEC_FLAG_TIMING) {
     if (h->ev_used == h->ev_sets.size()) {
-      std::array<hipEvent_t, 2> set{};
+      std::array<hipEvent_t, 4> set{};
       for (auto& e : set) NRT_HIP(hipEventCreate(&e));
       h->ev_sets.push_back(set);
     }
@@ -1882,16 +1907,22 @@ msfec_status msfec_nrt_compute(msfec_nrt_handle h, void* stream) {
   msfec_status st;
   if (h->iter) {
     switch (h->l) {
-      case 1: st = nrt_it_launch<1>(h, s); break;
-      case 2: st = nrt_it_launch<2>(h, s); break;
-      case 3: st = nrt_it_launch<3>(h, s); break;
-      default: st = nrt_it_launch<4>(h, s); break;
+      case 1: st = nrt_it_launch<1>(h, s, ev); break;
+      case 2: st = nrt_it_launch<2>(h, s, ev); break;
+      case 3: st = nrt_it_launch<3>(h, s, ev); break;
+      default: st = nrt_it_launch<4>(h, s, ev); break;
     }
   } else {
     st = h->l == 1 ? nrt_launch<1>(h, s) : nrt_launch<2>(h, s);
   }
   if (st != MSFEC_OK) return st;
-  if (ev) NRT_HIP(hipEventRecord((*ev)[1], s));
+  if (ev) {
+    if (!h->iter) {
+      NRT_HIP(hipEventRecord((*ev)[1], s));
+      NRT_HIP(hipEventRecord((*ev)[2], s));
+    }
+    NRT_HIP(hipEventRecord((*ev)[3], s));
+  }
   h->computed = true;
   return MSFEC_OK;
 }
@@ -1981,15 +2012,22 @@ msfec_status msfec_nrt_get_iterations(msfec_nrt_handle h, int* iterations) {
 msfec_status msfec_nrt_kernel_times(msfec_nrt_handle h, float* ms, int max, int* n) {
   if (!h || !ms) return nrt_fail(MSFEC_ERR_INVALID, "null argument");
   if (h->ev_used == 0) return nrt_fail(MSFEC_ERR_STATE, "no timed compute (set MSFEC_FLAG_TIMING)");
-  double acc = 0.0;
+  // the whole compute, then (MINRES path) local matrices + preconditioner,
+  // MINRES, element matrix + status
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
   for (size_t c = 0; c < h->ev_used; ++c) {
-    NRT_HIP(hipEventSynchronize(h->ev_sets[c][1]));
-    float t = 0.f;
-    NRT_HIP(hipEventElapsedTime(&t, h->ev_sets[c][0], h->ev_sets[c][1]));
-    acc += t;
+    const auto& e = h->ev_sets[c];
+    NRT_HIP(hipEventSynchronize(e[3]));
+    float t[4] = {0.f, 0.f, 0.f, 0.f};
+    NRT_HIP(hipEventElapsedTime(&t[0], e[0], e[3]));
+    NRT_HIP(hipEventElapsedTime(&t[1], e[0], e[1]));
+    NRT_HIP(hipEventElapsedTime(&t[2], e[1], e[2]));
+    NRT_HIP(hipEventElapsedTime(&t[3], e[2], e[3]));
+    for (int k = 0; k < 4; ++k) acc[k] += t[k];
   }
-  if (max > 0) ms[0] = (float)(acc / (double)h->ev_used);
-  if (n) *n = max > 0 ? 1 : 0;
+  const int cnt = std::min(max, h->iter ? 4 : 1);
+  for (int k = 0; k < cnt; ++k) ms[k] = (float)(acc[k] / (double)h->ev_used);
+  if (n) *n = std::max(cnt, 0);
   h->ev_used = 0;
   return MSFEC_OK;
 }

@@ -125,9 +125,12 @@ class NedRTBasisBatch:
         return its
 
     def kernel_times(self):
-        ms = (ctypes.c_float * 2)()
+        """Mean ms of the timed computes: [whole compute] on the dense path,
+        [whole, local matrices + preconditioner, MINRES, element matrix] on the
+        MINRES path."""
+        ms = (ctypes.c_float * 4)()
         n = ctypes.c_int(0)
-        L.check(self._lib.msfec_nrt_kernel_times(self._h, ms, 2, ctypes.byref(n)))
+        L.check(self._lib.msfec_nrt_kernel_times(self._h, ms, 4, ctypes.byref(n)))
         return [ms[i] for i in range(n.value)]
 
     def close(self):

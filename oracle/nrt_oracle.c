@@ -426,6 +426,46 @@ int oq_nrt_curl_loads(int l, const double *verts, double *Fc) {
   return 0;
 }
 
+int oq_nrt_assemble_coo(int l, const double *verts, const double *ainv, const double *bq,
+                        const double *rq, const double *fq, int *rows, int *cols, double *vals,
+                        double *F) {
+  nrt_mesh g;
+  if (nrt_mesh_init(l, 4, verts, &g)) return 1;
+  const int m = g.m;
+  memset(F, 0, sizeof(double) * g.N);
+  size_t k = 0;
+  for (int c3 = 0; c3 < m; ++c3)
+    for (int b = 0; b < m; ++b)
+      for (int a = 0; a < m; ++a) {
+        const int c = (c3 * m + b) * m + a;
+        double K[18][18], Fl[18];
+        int d[18];
+        nrt_local(&g, c, ainv, bq, rq, fq, K, Fl);
+        nrt_cell_dofs(&g, a, b, c3, d);
+        for (int i = 0; i < 18; ++i) {
+          for (int j = 0; j < 18; ++j, ++k) {
+            rows[k] = d[i];
+            cols[k] = d[j];
+            vals[k] = K[i][j];
+          }
+          F[d[i]] += Fl[i];
+        }
+      }
+  return 0;
+}
+
+int oq_nrt_boundary(int l, const double *verts, double *G, int *on_boundary) {
+  nrt_mesh g;
+  if (nrt_mesh_init(l, 4, verts, &g)) return 1;
+  for (int d = 0; d < g.N; ++d) {
+    int t, p[3];
+    on_boundary[d] = nrt_on_boundary(&g, d, &t, p);
+    for (int n = 0; n < 18; ++n)
+      G[(size_t)n * g.N + d] = on_boundary[d] ? nrt_boundary_value(&g, t, p, n) : 0.0;
+  }
+  return 0;
+}
+
 int oq_nrt_basis_cell(int l, const double *verts, const double *ainv, const double *bq,
                       const double *rq, const double *fq, double *M, double *b, double *X) {
   nrt_mesh g;

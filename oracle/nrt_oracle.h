@@ -70,6 +70,14 @@ int oq_nrt_assemble(int l, const double *verts, const double *ainv, const double
 /* The loads (v, curl N_n) of the 12 edge bases (12 x (ne + nf), zero on the
  * edges), by the same quadrature as assemble_system; l in 1..4. */
 int oq_nrt_curl_loads(int l, const double *verts, double *Fc);
+/* The same assembly as (row, column, value) triplets, 18 x 18 per fine cell
+ * in fine-cell order (duplicates summed by the reader), l in 1..4. */
+int oq_nrt_assemble_coo(int l, const double *verts, const double *ainv, const double *bq,
+                        const double *rq, const double *fq, int *rows, int *cols, double *vals,
+                        double *F);
+/* The boundary dofs (on_boundary, N ints) and the fixed values of the 18
+ * bases on them (G, 18 x N, zero inside), l in 1..4. */
+int oq_nrt_boundary(int l, const double *verts, double *G, int *on_boundary);
 
 #ifdef __cplusplus
 }

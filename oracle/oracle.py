@@ -103,6 +103,11 @@ def lib():
         L.oq_nrt_assemble.restype = ctypes.c_int
         L.oq_nrt_curl_loads.argtypes = [ctypes.c_int, dp, dp]
         L.oq_nrt_curl_loads.restype = ctypes.c_int
+        ip = ctypes.POINTER(ctypes.c_int)
+        L.oq_nrt_assemble_coo.argtypes = [ctypes.c_int, dp, dp, dp, dp, dp, ip, ip, dp, dp]
+        L.oq_nrt_assemble_coo.restype = ctypes.c_int
+        L.oq_nrt_boundary.argtypes = [ctypes.c_int, dp, dp, ip]
+        L.oq_nrt_boundary.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -459,6 +464,76 @@ def nrt_curl_loads(l, verts):
     st = lib().oq_nrt_curl_loads(l, _dp(v), _dp(Fc))
     assert st == 0
     return Fc
+
+
+def nrt_assemble_sparse(l, verts, ainv, bq=1.0, rq=None, fq=None):
+    """The assembled mixed system over all fine dofs as a scipy CSR matrix (row
+    = test) and the load (v, f); l up to 4."""
+    import scipy.sparse as sp
+    ainv, bq, rq, fq = _nrt_coefs(l, ainv, bq, rq, fq)
+    v = np.ascontiguousarray(verts, dtype=np.float64).reshape(8, 3)
+    N = nrt_sizes(l)[5]
+    nt = (1 << (3 * l)) * 324
+    rows = np.empty(nt, dtype=np.int32)
+    cols = np.empty(nt, dtype=np.int32)
+    vals = np.empty(nt)
+    F = np.zeros(N)
+    ip = ctypes.POINTER(ctypes.c_int)
+    st = lib().oq_nrt_assemble_coo(l, _dp(v), _dp(ainv), _dp(bq), _dp(rq), _dp(fq),
+                                   rows.ctypes.data_as(ip), cols.ctypes.data_as(ip), _dp(vals),
+                                   _dp(F))
+    assert st == 0
+    return sp.coo_matrix((vals, (rows, cols)), shape=(N, N)).tocsr(), F
+
+
+def nrt_boundary(l, verts):
+    """(G 18 x N, the fixed values of the bases on the boundary dofs, zero
+    inside; on_boundary, N bools)."""
+    v = np.ascontiguousarray(verts, dtype=np.float64).reshape(8, 3)
+    N = nrt_sizes(l)[5]
+    G = np.zeros((18, N))
+    b = np.zeros(N, dtype=np.int32)
+    st = lib().oq_nrt_boundary(l, _dp(v), _dp(G), b.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+    assert st == 0
+    return G, b.astype(bool)
+
+
+def nrt_basis_cell_sparse(l, verts, ainv, bq=1.0, rq=None, fq=None):
+    """One cell by a sparse direct solve (SuperLU, scipy.sparse.linalg.splu)
+    of the assembled interior system with the 18 right-hand sides: the
+    reference's solve_direct (UMFPACK, ned_rt_basis.cc:579-634) restated for
+    l up to 4, where the dense LU of nrt_basis_cell does not reach, with two
+    steps of iterative refinement.  Returns
+    (M 18x18, b 18, X 18 x N); M and b by the block rule of
+    assemble_global_element_matrix (:850-948)."""
+    import scipy.sparse.linalg as spla
+    A, F = nrt_assemble_sparse(l, verts, ainv, bq, rq, fq)
+    Fc = nrt_curl_loads(l, verts)
+    G, bnd = nrt_boundary(l, verts)
+    ne, N = nrt_sizes(l)[2], nrt_sizes(l)[5]
+    I = np.flatnonzero(~bnd)
+    B = np.flatnonzero(bnd)
+    AI = A[I]
+    rhs = np.concatenate([Fc[:, I].T, np.zeros((len(I), 6))], axis=1)
+    rhs -= AI[:, B] @ G[:, B].T
+    AII = AI[:, I].tocsc()
+    lu = spla.splu(AII)
+    x = lu.solve(rhs)
+    # two steps of iterative refinement, as nrt_solve does for the dense LU:
+    # the interior system is badly scaled (edge rows ~ h, face rows ~ 1/h^3)
+    for _ in range(2):
+        x += lu.solve(rhs - AII @ x)
+    X = G.copy()
+    X[:, I] = x.T
+    Ye = A[:, :ne] @ X[:, :ne].T
+    Yf = A[:, ne:] @ X[:, ne:].T
+    M = np.zeros((18, 18))
+    for i in range(18):
+        ri = slice(0, ne) if i < 12 else slice(ne, N)
+        for j in range(18):
+            M[i, j] = X[i, ri] @ (Ye if j < 12 else Yf)[ri, j]
+    b = np.array([0.0] * 12 + [X[i, ne:] @ F[ne:] for i in range(12, 18)])
+    return M, b, X
 
 
 def box_vertices_3d(x0, x1):

@@ -6,10 +6,11 @@ default), so its bases carry an iteration error the direct solves do not;
 the tolerance TOL_IT = 1e-8 (relative to the largest entry of each basis,
 and of M) holds it.  At l = 2 and 3 the oracle's direct solve is the
 reference; at l = 4 (the reference's example refinement, 22 320 interior
-unknowns, beyond the oracle's dense LU) the bases are checked against the
-oracle's assembled system (interior residual, boundary traces, the element
-matrix by the reference's block rule) and, for constant coefficients,
-against the closed forms (the bases are the coarse functions).
+unknowns, beyond the oracle's dense LU) against the oracle's sparse direct
+solve, against its assembled system (interior residual, boundary traces,
+the element matrix by the reference's block rule) and, for constant
+coefficients, against the closed forms (the bases are the coarse
+functions).
 """
 import numpy as np
 import pytest
@@ -146,6 +147,11 @@ def test_minres_l4_reference_example_against_assembled_system(oracle_mod):
     Bv[:12, :ne] = E
     Bv[12:, ne:] = Fv
     np.testing.assert_allclose(Xc[:, bnd], Bv[:, bnd], rtol=0, atol=1e-15)
+    # and the sparse direct solve of the same system
+    Ms, bs, Xs = oracle_mod.nrt_basis_cell_sparse(l, verts[0], ainv[0], bq[0], None, fq[0])
+    ex = max(rel(Xc[i], Xs[i]) for i in range(18))
+    print(f"vs sparse direct: basis {ex:.2e} M {rel(M[0], Ms):.2e} b {rel(b[0], bs):.2e}", flush=True)
+    assert ex <= TOL_IT and rel(M[0], Ms) <= TOL_IT and rel(b[0], bs) <= TOL_IT
 
 
 def test_minres_iteration_limit_flags_the_cell():

@@ -442,6 +442,41 @@ def test_bases_solve_the_assembled_system(l):
             assert M[i, 12 + j] == pytest.approx(-want, rel=1e-10, abs=1e-12 * np.abs(M).max())
 
 
+@pytest.mark.parametrize("l", [1, 2, 3])
+def test_sparse_direct_solve_matches_dense(l):
+    """nrt_basis_cell_sparse (SuperLU on the triplet assembly, the path that
+    reaches l = 4) against the dense LU of nrt_basis_cell."""
+    ainv, bq, rq, fq = rand_coefs(l, 60 + l)
+    M, b, X = O.nrt_basis_cell_sparse(l, VERTS, ainv, bq, rq, fq)
+    Mo, bo, Xo, st = O.nrt_basis_cell(l, VERTS, ainv, bq, rq, fq)
+    assert st == 0
+    np.testing.assert_allclose(M, Mo, rtol=0, atol=1e-12 * np.abs(Mo).max())
+    np.testing.assert_allclose(b, bo, rtol=0, atol=1e-12 * np.abs(bo).max())
+    np.testing.assert_allclose(X, Xo, rtol=0, atol=1e-10 * np.abs(Xo).max())
+
+
+def test_triplet_assembly_equals_dense_assembly():
+    ainv, bq, rq, fq = rand_coefs(2, 71)
+    A, F = O.nrt_assemble(2, VERTS, ainv, bq, rq, fq)
+    As, Fs = O.nrt_assemble_sparse(2, VERTS, ainv, bq, rq, fq)
+    np.testing.assert_allclose(As.toarray(), A, rtol=0, atol=1e-14 * np.abs(A).max())
+    np.testing.assert_array_equal(Fs, F)
+
+
+def test_boundary_values_are_the_coarse_traces():
+    for l in (1, 3):
+        D = Dofs(l)
+        G, bnd = O.nrt_boundary(l, VERTS)
+        for e, t, p in D.edges():
+            assert bnd[e] == D.edge_on_boundary(t, p)
+        for f, t, p in D.faces():
+            assert bnd[f] == D.face_on_boundary(t, p)
+        E, Fv = coarse_edge_values(D), coarse_face_values(D)
+        np.testing.assert_allclose(G[:12, :D.ne], np.where(bnd[:D.ne], E, 0.0), rtol=0, atol=1e-15)
+        np.testing.assert_allclose(G[12:, D.ne:], np.where(bnd[D.ne:], Fv, 0.0), rtol=0, atol=1e-15)
+        assert not G[:12, D.ne:].any() and not G[12:, :D.ne].any()
+
+
 def test_rejects_bad_cells():
     v = VERTS.copy()
     v[7, 0] += 0.01
