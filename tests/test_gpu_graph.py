@@ -1,5 +1,5 @@
 """The compute entry points under HIP graph capture: msfec_q_compute,
-msfec_ned_compute and msfec_rt_compute enqueue kernels only (no memsets, no
+msfec_ned_compute, msfec_rt_compute and msfec_nrt_compute enqueue kernels only (no memsets, no
 host synchronisation), so a stream capture of one call replays the whole
 batch.  Each graph is replayed three times with new coefficients uploaded in
 between (same device buffers), with eager work and synchronisations between
@@ -140,3 +140,20 @@ def test_uneven_compute_graph_replay(stream):
         replay_check(ub, stream, ub.set_coefficients,
                      lambda: (ub.get_global_element_matrix(), ub.get_global_element_rhs(),
                               ub.get_basis_flat(), ub.get_cell_status()), data)
+
+
+@pytest.mark.parametrize("l,solver", [(2, 0), (2, 3), (3, 0)])
+def test_nrt_compute_graph_replay(stream, l, solver):
+    """msfec_nrt_compute on both paths (the dense kernel; MINRES with its
+    five launches) replayed from a graph, bit for bit the eager results."""
+    print(f"case graph nrt l={l} solver={solver}", flush=True)
+    from msfec_amd import NedRTBasisBatch
+    from test_gpu_nrt import boxes, rand_coefs
+    rng = np.random.default_rng(51 + l)
+    n = 8
+    verts = boxes(rng, n)
+    data = [rand_coefs(rng, n, l) for _ in range(2)]
+    with NedRTBasisBatch(verts, n_refine_local=l, solver=solver) as nb:
+        replay_check(nb, stream, lambda d: nb.set_coefficients(*d),
+                     lambda: (nb.get_global_element_matrix(), nb.get_global_element_rhs(),
+                              nb.get_basis(), nb.get_cell_status()), data)

@@ -122,3 +122,17 @@ def test_rt_uploads_ordered_on_nonblocking_stream(nb_stream):
     with RTDQBasisBatch(verts, n_refine_local=4) as rb:
         run_overlapped(rb, s, rb.set_coefficients,
                        lambda: (rb.get_global_element_matrix(), rb.get_basis()), *data)
+
+
+@pytest.mark.parametrize("l,n", [(2, 1024), (3, 8)])
+def test_nrt_uploads_ordered_on_nonblocking_stream(nb_stream, l, n):
+    print(f"case streams nrt l={l} n={n}", flush=True)
+    from msfec_amd import NedRTBasisBatch
+    from test_gpu_nrt import boxes, rand_coefs
+    hip, s = nb_stream
+    rng = np.random.default_rng(37 + l)
+    verts = boxes(rng, n)
+    d0, d1 = rand_coefs(rng, n, l), rand_coefs(rng, n, l)
+    with NedRTBasisBatch(verts, n_refine_local=l) as nb:
+        run_overlapped(nb, s, lambda d: nb.set_coefficients(*d),
+                       lambda: (nb.get_global_element_matrix(), nb.get_basis()), d0, d1)
