@@ -981,6 +981,8 @@ struct NrtItArgs {
   double* FL;        // n_cells x nfc x 6: (psi_F, f) per fine cell
   double* PD;        // n_cells x N: the inverse preconditioner diagonal (0 on the boundary)
   double* WK;        // n_cells x 18 x 5 x N: MINRES vectors v, r1, r2, w, w2
+  double* S0;        // n_cells x NC x NC: the coarse matrix, then its inverse
+  const int* agg;    // n: the aggregate of every interior face dof, -1 for an edge
   double* X;         // n_cells x 18 x N
   double* M;         // n_cells x 18 x 18
   double* b;         // n_cells x 18
@@ -1341,6 +1343,161 @@ __global__ __launch_bounds__(256) void nrt_it_diag_kernel(NrtItArgs g) {
   }
 }
 
+// The coarse space of the MINRES preconditioner: the interior faces of each
+// orientation aggregated in NB^3 blocks of B^3 fine cells (NC = 3 NB^3
+// coarse dofs, 192 from l = 3 on).  The preconditioner is the additive
+// two-level
+//   z = diag^-1 r + R0^T S0^-1 R0 r_faces,  S0 = R0 (D + C diag(Me)^-1 C^T) R0^T,
+// R0 summing the faces of each aggregate: symmetric positive definite, as
+// MINRES needs, and it removes the smooth face modes that make the Schur
+// block's condition number grow as h^-2.
+template <int M>
+struct ItCoarse {
+  static constexpr int NB = M <= 4 ? M / 2 : 4;
+  static constexpr int B = M / NB;
+  static constexpr int NC = 3 * NB * NB * NB;
+};
+
+// aggregate of the face of orientation d at (q0, q1, q2) (its level along d,
+// its cell indices along the other axes)
+template <int M>
+__device__ __forceinline__ int it_agg(int d, int q0, int q1, int q2) {
+  typedef ItCoarse<M> C;
+  const int a0 = min(q0 / C::B, C::NB - 1), a1 = min(q1 / C::B, C::NB - 1),
+            a2 = min(q2 / C::B, C::NB - 1);
+  return d * C::NB * C::NB * C::NB + (a2 * C::NB + a1) * C::NB + a0;
+}
+
+// whether local face G of fine cell (c0, c1, c2) is an interior face of
+// aggregate J
+template <int M>
+__device__ __forceinline__ bool it_face_in(int c0, int c1, int c2, int G, int J) {
+  const int d = G >> 1, s = G & 1;
+  const int q0 = c0 + (d == 0 ? s : 0), q1 = c1 + (d == 1 ? s : 0), q2 = c2 + (d == 2 ? s : 0);
+  const int lev = d == 0 ? q0 : (d == 1 ? q1 : q2);
+  return lev > 0 && lev < M && it_agg<M>(d, q0, q1, q2) == J;
+}
+
+// The inclusive range of aggregate I along axis k (levels along its
+// normal, interior only; cells along the others)
+template <int M>
+__device__ __forceinline__ void it_agg_range(int I, int k, int& lo, int& hi) {
+  typedef ItCoarse<M> C;
+  constexpr int NB3 = C::NB * C::NB * C::NB;
+  const int d = I / NB3, r = I % NB3;
+  const int a = k == 0 ? r % C::NB : (k == 1 ? (r / C::NB) % C::NB : r / (C::NB * C::NB));
+  lo = a * C::B;
+  hi = a == C::NB - 1 ? (k == d ? M - 1 : M - 1) : a * C::B + C::B - 1;
+  if (k == d) lo = max(lo, 1);
+}
+
+// full dof of the face of orientation d at (q0, q1, q2)
+template <int M>
+__device__ __forceinline__ int it_face_dof(int d, int q0, int q1, int q2) {
+  constexpr int nx = M * (M + 1) * (M + 1), ne = 3 * nx, nf1 = M * M * (M + 1);
+  if (d == 0) return ne + (q2 * M + q1) * (M + 1) + q0;
+  if (d == 1) return ne + nf1 + (q2 * (M + 1) + q1) * M + q0;
+  return ne + 2 * nf1 + (q2 * M + q1) * M + q0;
+}
+
+// Column J of S0 for one coarse cell per workgroup: u = (D + C diag(Me)^-1
+// C^T) 1_J on every interior face into LDS (faces too far from the
+// aggregate are zero and skipped), then S0[I][J] = the sum of u over the
+// faces of I, one thread per I in a fixed order (no atomics: the
+// preconditioner, and so every result, is reproducible bit for bit).
+template <int L>
+__global__ __launch_bounds__(256) void nrt_it_coarse_kernel(NrtItArgs g) {
+  constexpr int M = 1 << L, nfc = M * M * M, nx = M * (M + 1) * (M + 1), ne = 3 * nx;
+  constexpr int NF = 3 * M * M * (M + 1);
+  typedef ItCoarse<M> CS;
+  extern __shared__ double uf[];  // NF
+  const int cell = blockIdx.x / CS::NC, J = blockIdx.x % CS::NC, tid = threadIdx.x;
+  const double* lm = g.LM + (size_t)cell * kLM * nfc;
+  const double* Cc = g.CC + (size_t)cell * 72;
+  // the cells with a face in J, widened by the reach of C diag(Me)^-1 C^T
+  int jlo[3], jhi[3];
+  for (int k = 0; k < 3; ++k) {
+    it_agg_range<M>(J, k, jlo[k], jhi[k]);
+    if (k == J / (CS::NB * CS::NB * CS::NB)) jlo[k] -= 1;  // the cell below the lowest level
+    jlo[k] -= 2;
+    jhi[k] += 2;
+  }
+  for (int f = tid; f < NF; f += 256) {
+    int t, p0, p1, p2;
+    dof_pos<M>(ne + f, t, p0, p1, p2);
+    double u = 0.0;
+    const bool near = p0 >= jlo[0] && p0 <= jhi[0] + 1 && p1 >= jlo[1] && p1 <= jhi[1] + 1 &&
+                      p2 >= jlo[2] && p2 <= jhi[2] + 1;
+    if (near && !it_on_boundary<M>(t, p0, p1, p2)) {
+      for (int k = 0; k < 2; ++k) {
+        int c0, c1, c2, a;
+        it_cell<M>(t, p0, p1, p2, k, c0, c1, c2, a);
+        const int F = a - 12;
+        const int sc = it_lmc<M>(c0, c1, c2);
+        for (int G = 0; G < 6; ++G)
+          if (it_face_in<M>(c0, c1, c2, G, J)) {
+            const int hi = max(F, G), lo = min(F, G);
+            u += lm[(size_t)(78 + hi * (hi + 1) / 2 + lo) * nfc + sc];
+          }
+        for (int Ln = 0; Ln < 12; ++Ln) {
+          const int e = it_cdof<M>(c0, c1, c2, Ln);
+          int te, q0, q1, q2;
+          dof_pos<M>(e, te, q0, q1, q2);
+          if (it_on_boundary<M>(te, q0, q1, q2)) continue;
+          double we = 0.0, me = 0.0;
+          for (int k2 = 0; k2 < 4; ++k2) {
+            int d0, d1, d2, a2;
+            it_cell<M>(te, q0, q1, q2, k2, d0, d1, d2, a2);
+            me += lm[(size_t)(a2 * (a2 + 1) / 2 + a2) * nfc + it_lmc<M>(d0, d1, d2)];
+            for (int G = 0; G < 6; ++G)
+              if (it_face_in<M>(d0, d1, d2, G, J)) we += Cc[G * 12 + a2];
+          }
+          if (we != 0.0) u += Cc[F * 12 + Ln] * we / me;
+        }
+      }
+    }
+    uf[f] = u;
+  }
+  __syncthreads();
+  for (int I = tid; I < CS::NC; I += 256) {
+    const int d = I / (CS::NB * CS::NB * CS::NB);
+    int lo[3], hi[3];
+    for (int k = 0; k < 3; ++k) it_agg_range<M>(I, k, lo[k], hi[k]);
+    double sum = 0.0;
+    for (int q2 = lo[2]; q2 <= hi[2]; ++q2)
+      for (int q1 = lo[1]; q1 <= hi[1]; ++q1)
+        for (int q0 = lo[0]; q0 <= hi[0]; ++q0) sum += uf[it_face_dof<M>(d, q0, q1, q2) - ne];
+    g.S0[((size_t)cell * CS::NC + I) * CS::NC + J] = sum;
+  }
+}
+
+// S0 <- S0^-1 in place, one coarse cell per workgroup: Gauss-Jordan without
+// pivoting (S0 is symmetric positive definite), two barriers per pivot.
+template <int L>
+__global__ __launch_bounds__(1024) void nrt_it_coarse_inv_kernel(NrtItArgs g) {
+  constexpr int M = 1 << L;
+  constexpr int NC = ItCoarse<M>::NC;
+  double* A = g.S0 + (size_t)blockIdx.x * NC * NC;
+  for (int k = 0; k < NC; ++k) {
+    const double p = 1.0 / A[k * NC + k];
+    for (int idx = threadIdx.x; idx < NC * NC; idx += 1024) {
+      const int i = idx / NC, j = idx % NC;
+      if (i != k && j != k) A[idx] -= A[i * NC + k] * A[k * NC + j] * p;
+    }
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < 2 * NC; idx += 1024) {
+      if (idx < NC) {
+        if (idx != k) A[idx * NC + k] *= -p;
+      } else if (idx - NC != k) {
+        A[k * NC + idx - NC] *= p;
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) A[k * NC + k] = p;
+    __syncthreads();
+  }
+}
+
 // sum over the workgroup; red holds one value per wave and must not be
 // written again before every thread has read it
 __device__ __forceinline__ double it_block_sum(double v, double* red) {
@@ -1354,6 +1511,40 @@ __device__ __forceinline__ double it_block_sum(double v, double* red) {
   return s;
 }
 
+// cq = S0^-1 R0 y (y's face values; LDS q and cq of NC values): four
+// threads per aggregate sum every fourth of its faces in a fixed order and
+// combine by lane shuffles, then one thread per coarse row forms the product
+// with the inverse.  Deterministic; ends with a barrier.
+template <int M>
+__device__ __forceinline__ void it_coarse_correct(const double* y, const double* S0i, double* q,
+                                                  double* cq) {
+  typedef ItCoarse<M> CS;
+  static_assert(4 * CS::NC <= kItNT, "four threads per aggregate");
+  const int tid = threadIdx.x, I = tid >> 2, part = tid & 3;
+  double sum = 0.0;
+  if (I < CS::NC) {
+    const int d = I / (CS::NB * CS::NB * CS::NB);
+    int lo[3], hi[3];
+    for (int k = 0; k < 3; ++k) it_agg_range<M>(I, k, lo[k], hi[k]);
+    int cnt = 0;
+    for (int q2 = lo[2]; q2 <= hi[2]; ++q2)
+      for (int q1 = lo[1]; q1 <= hi[1]; ++q1)
+        for (int q0 = lo[0]; q0 <= hi[0]; ++q0, ++cnt)
+          if ((cnt & 3) == part) sum += y[it_face_dof<M>(d, q0, q1, q2)];
+  }
+  sum += __shfl_xor(sum, 1);
+  sum += __shfl_xor(sum, 2);
+  if (I < CS::NC && part == 0) q[I] = sum;
+  __syncthreads();
+  if (tid < CS::NC) {
+    const double* row = S0i + (size_t)tid * CS::NC;
+    double c = 0.0;
+    for (int J = 0; J < CS::NC; ++J) c += row[J] * q[J];
+    cq[tid] = c;
+  }
+  __syncthreads();
+}
+
 // One basis of one coarse cell per workgroup.  Workgroups are numbered so
 // that those dispatched to the same XCD (block index mod 8) take the 18
 // bases of one cell one after another, which keeps that cell's local
@@ -1361,7 +1552,9 @@ __device__ __forceinline__ double it_block_sum(double v, double* red) {
 template <int L>
 __global__ __launch_bounds__(kItNT) void nrt_minres_kernel(NrtItArgs g) {
   constexpr int M = 1 << L, nfc = M * M * M;
+  constexpr int NC = ItCoarse<M>::NC;
   __shared__ double redA[kItNW], redB[kItNW];
+  __shared__ double sq[NC], scq[NC];
   const int bid = blockIdx.x, slot = bid >> 3;
   const int cell = (slot / kNB) * 8 + (bid & 7), r = slot % kNB;
   if (cell >= g.n_cells) return;
@@ -1371,6 +1564,7 @@ __global__ __launch_bounds__(kItNT) void nrt_minres_kernel(NrtItArgs g) {
   const double* lm = g.LM + (size_t)cell * kLM * nfc;
   const double* pd = g.PD + (size_t)cell * N;
   const double* gCc = g.CC + (size_t)cell * 72;  // uniform: read through the scalar cache
+  const double* S0i = g.S0 + (size_t)cell * NC * NC;
   double* x = g.X + ((size_t)cell * kNB + r) * N;
   double* v = g.WK + ((size_t)cell * kNB + r) * 5 * N;
   double* P = v + N;   // r1
@@ -1399,15 +1593,21 @@ __global__ __launch_bounds__(kItNT) void nrt_minres_kernel(NrtItArgs g) {
     }
     P[f] = rr;
     Q[f] = rr;
-    acc += rr * rr * pd[f];
+  }
+  __syncthreads();
+  // z = M^-1 r: the diagonal, plus the coarse correction on the faces
+  it_coarse_correct<M>(Q, S0i, sq, scq);
+  for (int i = tid; i < n; i += kItNT) {
+    const int f = g.idof[i], ag = g.agg[i];
+    acc += Q[f] * (pd[f] * Q[f] + (ag >= 0 ? scq[ag] : 0.0));
   }
   const double beta1 = sqrt(it_block_sum(acc, redB));
   int flag = isfinite(beta1) ? MSFEC_CELL_OK : MSFEC_CELL_NONFINITE;
   int itn = 0;
   if (flag == MSFEC_CELL_OK && beta1 > 0.0) {
     for (int i = tid; i < n; i += kItNT) {
-      const int f = g.idof[i];
-      v[f] = pd[f] * Q[f] / beta1;
+      const int f = g.idof[i], ag = g.agg[i];
+      v[f] = (pd[f] * Q[f] + (ag >= 0 ? scq[ag] : 0.0)) / beta1;
     }
     __syncthreads();
     double oldb = 0.0, beta = beta1, dbar = 0.0, epsln = 0.0, phibar = beta1, cs = -1.0, sn = 0.0;
@@ -1448,14 +1648,20 @@ __global__ __launch_bounds__(kItNT) void nrt_minres_kernel(NrtItArgs g) {
       }
       const double alfa = it_block_sum(acc, redA);
       // B: r2' = t - (alfa / beta) r2 into r1's slot, then r1 <-> r2;
-      // beta' = |r2'| in the preconditioner's norm
+      // beta' = |r2'| in the preconditioner's norm (its coarse part formed
+      // between the two passes)
       const double c2 = alfa / beta;
-      acc = 0.0;
       for (int i = tid; i < n; i += kItNT) {
         const int f = g.idof[i];
-        const double y = P[f] - c2 * Q[f];
-        P[f] = y;
-        acc += y * y * pd[f];
+        P[f] = P[f] - c2 * Q[f];
+      }
+      __syncthreads();
+      it_coarse_correct<M>(P, S0i, sq, scq);
+      acc = 0.0;
+      for (int i = tid; i < n; i += kItNT) {
+        const int f = g.idof[i], ag = g.agg[i];
+        const double y = P[f];
+        acc += y * (pd[f] * y + (ag >= 0 ? scq[ag] : 0.0));
       }
       {
         double* tmp = P;
@@ -1481,11 +1687,11 @@ __global__ __launch_bounds__(kItNT) void nrt_minres_kernel(NrtItArgs g) {
       // C: w' = (v - oldeps w2 - delta w) / gamma into w2's slot, then
       // w <-> w2; x += phi w'; the next v = M^-1 r2 / beta
       for (int i = tid; i < n; i += kItNT) {
-        const int f = g.idof[i];
+        const int f = g.idof[i], ag = g.agg[i];
         const double wn = (v[f] - oldeps * W1[f] - delta * W0[f]) * denom;
         W1[f] = wn;
         x[f] += phi * wn;
-        v[f] = pd[f] * Q[f] * sb;
+        v[f] = (pd[f] * Q[f] + (ag >= 0 ? scq[ag] : 0.0)) * sb;
       }
       {
         double* tmp = W0;
@@ -1751,6 +1957,8 @@ struct msfec_nrt_batch {
   double* d_WK = nullptr;
   int* d_its = nullptr;
   int* d_flg = nullptr;
+  double* d_S0 = nullptr;
+  int* d_agg = nullptr;
   bool coef_set = false, has_r = false, has_f = false, computed = false;
   hipStream_t stream = nullptr;
   // per timed compute: start, after the local matrices and preconditioner,
@@ -1771,7 +1979,7 @@ void nrt_free(msfec_nrt_batch* h) {
   void* ptrs[] = {h->d_verts, h->d_ainv, h->d_b,  h->d_r,   h->d_f,  h->d_icell, h->d_cdof,
                   h->d_f2i,   h->d_amap, h->d_tnz, h->d_X,  h->d_M,  h->d_rhs,   h->d_w,
                   h->d_status, h->d_idof, h->d_LM, h->d_CC, h->d_FL, h->d_PD,    h->d_WK,
-                  h->d_its,   h->d_flg};
+                  h->d_its,   h->d_flg,  h->d_S0, h->d_agg};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (auto& set : h->ev_sets)
@@ -1843,6 +2051,8 @@ msfec_status nrt_it_launch(msfec_nrt_batch* h, hipStream_t s, std::array<hipEven
   a.b = h->d_rhs;
   a.its = h->d_its;
   a.flg = h->d_flg;
+  a.S0 = h->d_S0;
+  a.agg = h->d_agg;
   a.status = h->d_status;
   a.n_cells = h->n_cells;
   a.N = h->tab.N;
@@ -1854,6 +2064,16 @@ msfec_status nrt_it_launch(msfec_nrt_batch* h, hipStream_t s, std::array<hipEven
   NRT_HIP(hipGetLastError());
   hipLaunchKernelGGL(nrt_it_diag_kernel<L>, dim3(nc), dim3(256), 0, s, a);
   NRT_HIP(hipGetLastError());
+  {
+    constexpr int M = 1 << L, NC = ItCoarse<M>::NC;
+    const size_t lds = (size_t)3 * M * M * (M + 1) * sizeof(double);
+    NRT_HIP(hipFuncSetAttribute((const void*)nrt_it_coarse_kernel<L>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(nrt_it_coarse_kernel<L>, dim3(nc * NC), dim3(256), lds, s, a);
+    NRT_HIP(hipGetLastError());
+    hipLaunchKernelGGL(nrt_it_coarse_inv_kernel<L>, dim3(nc), dim3(1024), 0, s, a);
+    NRT_HIP(hipGetLastError());
+  }
   if (ev) NRT_HIP(hipEventRecord((*ev)[1], s));
   // cells in groups of 8 (one per XCD), 18 bases each
   const int groups = (nc + 7) / 8;
@@ -1952,6 +2172,30 @@ msfec_status msfec_nrt_create(const msfec_nrt_params* params, int n_cells, const
     NRT_TRY(nrt_alloc(&h->d_WK, nc * kNB * 5 * N));
     NRT_TRY(nrt_alloc(&h->d_its, nc * kNB));
     NRT_TRY(nrt_alloc(&h->d_flg, nc * kNB));
+    // the coarse space: aggregate of every interior face dof (-1 for edges)
+    const int m = 1 << l, NB = m <= 4 ? m / 2 : 4, B = m / NB, NC = 3 * NB * NB * NB;
+    std::vector<int> agg;
+    for (int f : idof) {
+      if (f < h->tab.ne) {
+        agg.push_back(-1);
+        continue;
+      }
+      const int d = (f - h->tab.ne) / h->tab.nf1;
+      int r = (f - h->tab.ne) % h->tab.nf1;
+      int ext[3] = {m, m, m};
+      ext[d] = m + 1;
+      int q[3];
+      q[0] = r % ext[0];
+      r /= ext[0];
+      q[1] = r % ext[1];
+      q[2] = r / ext[1];
+      int a[3];
+      for (int k = 0; k < 3; ++k) a[k] = std::min(q[k] / B, NB - 1);
+      agg.push_back(d * NB * NB * NB + (a[2] * NB + a[1]) * NB + a[0]);
+    }
+    NRT_TRY(nrt_alloc(&h->d_agg, agg.size()));
+    NRT_TRY(nrt_in(h->stream, h->d_agg, agg.data(), agg.size() * sizeof(int), MSFEC_HOST));
+    NRT_TRY(nrt_alloc(&h->d_S0, nc * NC * NC));
   } else {
     std::vector<unsigned short> amap;
     std::vector<unsigned char> tnz;
