@@ -948,8 +948,10 @@ __global__ __launch_bounds__(256) void nrt_reconstruct_kernel(const double* X, c
 // ===========================================================================
 // Beyond the dense limit (n_refine_local 3 and 4, or any l on request): the
 // interior system is never formed.  nrt_it_local_kernel writes the local
-// matrices of every fine cell to HBM once; nrt_it_diag_kernel the
-// preconditioner; nrt_minres_kernel solves each basis in a workgroup of its
+// matrices of every fine cell to HBM once; nrt_it_diag_kernel,
+// nrt_it_coarse_kernel and nrt_it_coarse_inv_kernel the preconditioner (a
+// diagonal plus a coarse correction on the faces); nrt_minres_kernel solves
+// each basis in a workgroup of its
 // own by preconditioned MINRES (Paige-Saunders), applying K' row by row from
 // the local matrices of the fine cells that hold the row's dof; and
 // nrt_it_element_kernel forms the element matrix from the assembled products.
@@ -1286,10 +1288,10 @@ __device__ __forceinline__ double it_edge_diag(int e, const double* lm) {
   return d;
 }
 
-// The MINRES preconditioner, the inverse of a positive diagonal: the edge
-// mass diagonal on the edges, and on the faces the diagonal of the Schur
-// complement D + C diag(Me)^-1 C^T (interior edges only).  Zero on the
-// boundary.
+// The diagonal part of the MINRES preconditioner, the inverse of a positive
+// diagonal: the edge mass diagonal on the edges, and on the faces the
+// diagonal of the Schur complement D + C diag(Me)^-1 C^T (interior edges
+// only).  Zero on the boundary.
 template <int L>
 __global__ __launch_bounds__(256) void nrt_it_diag_kernel(NrtItArgs g) {
   constexpr int M = 1 << L, nfc = M * M * M;
