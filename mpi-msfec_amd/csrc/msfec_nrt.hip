@@ -952,8 +952,9 @@ __global__ __launch_bounds__(256) void nrt_reconstruct_kernel(const double* X, c
 // nrt_it_coarse_kernel and nrt_it_coarse_inv_kernel the preconditioner (a
 // diagonal plus a coarse correction on the faces); nrt_minres_kernel solves
 // each basis in a workgroup of its
-// own by preconditioned MINRES (Paige-Saunders), applying K' row by row from
-// the local matrices of the fine cells that hold the row's dof; and
+// own by preconditioned MINRES (Paige-Saunders), applying K' from the local
+// matrices of the fine cells (element by element at l = 4, row by row
+// below); and
 // nrt_it_element_kernel forms the element matrix from the assembled products.
 // The reference solves these systems either directly (solve_direct,
 // ned_rt_basis.cc:579-634) or by CG on the face Schur complement
