@@ -1517,10 +1517,11 @@ __device__ __forceinline__ double it_block_sum(double v, double* red) {
 // cq = S0^-1 R0 y (y's face values; LDS q and cq of NC values): four
 // threads per aggregate sum every fourth of its faces in a fixed order and
 // combine by lane shuffles, then one thread per coarse row forms the product
-// with the inverse.  Deterministic; ends with a barrier.
+// with the inverse.  Returns this thread's part of q . cq (the coarse part of
+// y . M^-1 y).  Deterministic; ends with a barrier.
 template <int M>
-__device__ __forceinline__ void it_coarse_correct(const double* y, const double* S0i, double* q,
-                                                  double* cq) {
+__device__ __forceinline__ double it_coarse_correct(const double* y, const double* S0i, double* q,
+                                                    double* cq) {
   typedef ItCoarse<M> CS;
   static_assert(4 * CS::NC <= kItNT, "four threads per aggregate");
   const int tid = threadIdx.x, I = tid >> 2, part = tid & 3;
@@ -1539,13 +1540,16 @@ __device__ __forceinline__ void it_coarse_correct(const double* y, const double*
   sum += __shfl_xor(sum, 2);
   if (I < CS::NC && part == 0) q[I] = sum;
   __syncthreads();
+  double qcq = 0.0;
   if (tid < CS::NC) {
     const double* row = S0i + (size_t)tid * CS::NC;
     double c = 0.0;
     for (int J = 0; J < CS::NC; ++J) c += row[J] * q[J];
     cq[tid] = c;
+    qcq = q[tid] * c;
   }
   __syncthreads();
+  return qcq;
 }
 
 // One basis of one coarse cell per workgroup.  Workgroups are numbered so
@@ -1594,16 +1598,13 @@ __global__ __launch_bounds__(kItNT) void nrt_minres_kernel(NrtItArgs g) {
       dof_pos<M>(f, t, p0, p1, p2);
       if (t >= 3) rr -= it_curl_load<M>(t, p0, p1, p2, r, h, Hc);
     }
-    P[f] = rr;
+    P[f] = 0.0;  // r1 enters the first iteration scaled by 0
     Q[f] = rr;
+    acc += rr * rr * pd[f];
   }
   __syncthreads();
-  // z = M^-1 r: the diagonal, plus the coarse correction on the faces
-  it_coarse_correct<M>(Q, S0i, sq, scq);
-  for (int i = tid; i < n; i += kItNT) {
-    const int f = g.idof[i], ag = g.agg[i];
-    acc += Q[f] * (pd[f] * Q[f] + (ag >= 0 ? scq[ag] : 0.0));
-  }
+  // r . M^-1 r: the diagonal part, plus q . S0^-1 q of the coarse correction
+  acc += it_coarse_correct<M>(Q, S0i, sq, scq);
   const double beta1 = sqrt(it_block_sum(acc, redB));
   int flag = isfinite(beta1) ? MSFEC_CELL_OK : MSFEC_CELL_NONFINITE;
   int itn = 0;
@@ -1623,6 +1624,7 @@ __global__ __launch_bounds__(kItNT) void nrt_minres_kernel(NrtItArgs g) {
       // class; below, row by row); alfa = v . t
       const double c1 = itn > 1 ? beta / oldb : 0.0;
       acc = 0.0;
+      double alfa;
       if constexpr (kItCellProduct<L>) {
         for (int i = tid; i < n; i += kItNT) {
           const int f = g.idof[i];
@@ -1649,23 +1651,20 @@ __global__ __launch_bounds__(kItNT) void nrt_minres_kernel(NrtItArgs g) {
           acc += v[f] * t;
         }
       }
-      const double alfa = it_block_sum(acc, redA);
+      alfa = it_block_sum(acc, redA);
       // B: r2' = t - (alfa / beta) r2 into r1's slot, then r1 <-> r2;
       // beta' = |r2'| in the preconditioner's norm (its coarse part formed
       // between the two passes)
       const double c2 = alfa / beta;
-      for (int i = tid; i < n; i += kItNT) {
-        const int f = g.idof[i];
-        P[f] = P[f] - c2 * Q[f];
-      }
-      __syncthreads();
-      it_coarse_correct<M>(P, S0i, sq, scq);
       acc = 0.0;
       for (int i = tid; i < n; i += kItNT) {
-        const int f = g.idof[i], ag = g.agg[i];
-        const double y = P[f];
-        acc += y * (pd[f] * y + (ag >= 0 ? scq[ag] : 0.0));
+        const int f = g.idof[i];
+        const double y = P[f] - c2 * Q[f];
+        P[f] = y;
+        acc += y * y * pd[f];
       }
+      __syncthreads();
+      acc += it_coarse_correct<M>(P, S0i, sq, scq);
       {
         double* tmp = P;
         P = Q;
