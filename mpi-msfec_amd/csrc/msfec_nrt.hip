@@ -969,6 +969,13 @@ constexpr int kItNT = 1024;  // threads of a MINRES workgroup
 // Measured on one cell at l = 4: 0.51 s against 0.71 s row by row.
 template <int L>
 constexpr bool kItCellProduct = L >= 4;
+// The edge block of the preconditioner: an element additive Schwarz at
+// l = 4 (64 cells of the reference example: 690 iterations and 1.93 s
+// against 1 117 and 2.17 s with the diagonal), the diagonal below (l = 3:
+// its 8 colour passes of 64 cells cost more than the 10 % fewer iterations
+// save).
+template <int L>
+constexpr bool kItEdgeSchwarz = L >= 4;
 constexpr int kItNW = kItNT / 64;
 constexpr int kLM = 99;     // local-matrix entries per fine cell: Me (78), then Du (21)
 
@@ -983,8 +990,10 @@ struct NrtItArgs {
   double* CC;        // n_cells x 72: C[F][L] = (psi_F, curl phi_L) of one fine cell
   double* FL;        // n_cells x nfc x 6: (psi_F, f) per fine cell
   double* PD;        // n_cells x N: the inverse preconditioner diagonal (0 on the boundary)
-  double* WK;        // n_cells x 18 x 5 x N: MINRES vectors v, r1, r2, w, w2
+  double* WK;        // n_cells x 18 x 6 x N: MINRES vectors v, r1, r2, w, w2, z
   double* S0;        // n_cells x NC x NC: the coarse matrix, then its inverse
+  double* EI;        // n_cells x 78 x nfc: per fine cell the inverse of its edge mass block
+                     // over its interior edges (packed lower, entry-major, colour-major cells)
   const int* agg;    // n: the aggregate of every interior face dof, -1 for an edge
   double* X;         // n_cells x 18 x N
   double* M;         // n_cells x 18 x 18
@@ -1474,6 +1483,75 @@ __global__ __launch_bounds__(256) void nrt_it_coarse_kernel(NrtItArgs g) {
   }
 }
 
+// The edge part of the MINRES preconditioner, an element additive Schwarz:
+// z_e = sum over fine cells of (Me_c restricted to its interior edges)^-1
+// r_c.  One thread per fine cell inverts its 12 x 12 block in LDS by
+// Gauss-Jordan (a boundary edge becomes a unit row and column, then zero).
+template <int L>
+__global__ __launch_bounds__(64) void nrt_it_edgeas_kernel(NrtItArgs g) {
+  constexpr int M = 1 << L, nfc = M * M * M;
+  constexpr int CPB = nfc < 64 ? nfc : 64;  // fine cells per workgroup
+  __shared__ double A[64][145];
+  const int tid = threadIdx.x;
+  const int cell = blockIdx.x / (nfc / CPB), c = (blockIdx.x % (nfc / CPB)) * CPB + tid;
+  if (tid >= CPB) return;
+  const int c0 = c % M, c1 = (c / M) % M, c2 = c / (M * M);
+  const int sc = it_lmc<M>(c0, c1, c2);
+  const double* lm = g.LM + (size_t)cell * kLM * nfc;
+  double* a = A[tid];
+  bool in[12];
+  for (int i = 0; i < 12; ++i) in[i] = !it_local_on_boundary<M>(c0, c1, c2, i);
+  for (int i = 0; i < 12; ++i)
+    for (int j = 0; j <= i; ++j) {
+      const double v = (in[i] && in[j]) ? lm[(size_t)(i * (i + 1) / 2 + j) * nfc + sc]
+                                         : (i == j ? 1.0 : 0.0);
+      a[i * 12 + j] = v;
+      a[j * 12 + i] = v;
+    }
+  for (int k = 0; k < 12; ++k) {
+    const double p = 1.0 / a[k * 12 + k];
+    for (int i = 0; i < 12; ++i)
+      for (int j = 0; j < 12; ++j)
+        if (i != k && j != k) a[i * 12 + j] -= a[i * 12 + k] * a[k * 12 + j] * p;
+    for (int i = 0; i < 12; ++i)
+      if (i != k) {
+        a[i * 12 + k] *= -p;
+        a[k * 12 + i] *= p;
+      }
+    a[k * 12 + k] = p;
+  }
+  double* ei = g.EI + (size_t)cell * 78 * nfc;
+  for (int i = 0; i < 12; ++i)
+    for (int j = 0; j <= i; ++j)
+      ei[(size_t)(i * (i + 1) / 2 + j) * nfc + sc] =
+          (in[i] && in[j]) ? 0.5 * (a[i * 12 + j] + a[j * 12 + i]) : 0.0;
+}
+
+// z += EI_c y_c on the interior edges of fine cell (c0, c1, c2); returns
+// y_c . EI_c y_c, this cell's part of y . z
+template <int M>
+__device__ __forceinline__ double it_cell_edge_precond(int c0, int c1, int c2, const double* y,
+                                                       double* z, const double* ei) {
+  constexpr int nfc = M * M * M;
+  const unsigned sc = (unsigned)it_lmc<M>(c0, c1, c2);
+  double yv[12];
+#pragma unroll
+  for (int b = 0; b < 12; ++b) yv[b] = it_local_on_boundary<M>(c0, c1, c2, b) ? 0.0 : y[it_cdof<M>(c0, c1, c2, b)];
+  double quad = 0.0;
+#pragma unroll 1
+  for (int i = 0; i < 12; ++i) {
+    double acc = 0.0;
+#pragma unroll
+    for (int j = 0; j < 12; ++j) {
+      const int e = i >= j ? i * (i + 1) / 2 + j : j * (j + 1) / 2 + i;
+      acc += (ei + (size_t)e * nfc)[sc] * yv[j];
+    }
+    quad += yv[i] * acc;
+    if (!it_local_on_boundary<M>(c0, c1, c2, i)) z[it_cdof<M>(c0, c1, c2, i)] += acc;
+  }
+  return quad;
+}
+
 // S0 <- S0^-1 in place, one coarse cell per workgroup: Gauss-Jordan without
 // pivoting (S0 is symmetric positive definite), two barriers per pivot.
 template <int L>
@@ -1558,7 +1636,7 @@ __device__ __forceinline__ double it_coarse_correct(const double* y, const doubl
 // matrices in the XCD's L2.
 template <int L>
 __global__ __launch_bounds__(kItNT) void nrt_minres_kernel(NrtItArgs g) {
-  constexpr int M = 1 << L, nfc = M * M * M;
+  constexpr int M = 1 << L, nfc = M * M * M, ne = 3 * M * (M + 1) * (M + 1);
   constexpr int NC = ItCoarse<M>::NC;
   __shared__ double redA[kItNW], redB[kItNW];
   __shared__ double sq[NC], scq[NC];
@@ -1573,11 +1651,27 @@ __global__ __launch_bounds__(kItNT) void nrt_minres_kernel(NrtItArgs g) {
   const double* gCc = g.CC + (size_t)cell * 72;  // uniform: read through the scalar cache
   const double* S0i = g.S0 + (size_t)cell * NC * NC;
   double* x = g.X + ((size_t)cell * kNB + r) * N;
-  double* v = g.WK + ((size_t)cell * kNB + r) * 5 * N;
+  double* v = g.WK + ((size_t)cell * kNB + r) * 6 * N;
   double* P = v + N;   // r1
   double* Q = v + 2 * N;  // r2
   double* W0 = v + 3 * N;  // w
   double* W1 = v + 4 * N;  // w2
+  double* Zb = v + 5 * N;  // z = M^-1 r2 on the edges
+  const double* ei = g.EI + (size_t)cell * 78 * nfc;
+  constexpr int H = M / 2;
+  // z_e = the element additive Schwarz of y's edge values, colour class by
+  // colour class (Zb zeroed on the interior edges beforehand); adds y_e . z_e
+  auto edge_precond = [&](const double* y) {
+    double quad = 0.0;
+#pragma unroll 1
+    for (int col = 0; col < 8; ++col) {
+      for (int q = tid; q < H * H * H; q += kItNT)
+        quad += it_cell_edge_precond<M>(2 * (q % H) + (col & 1), 2 * ((q / H) % H) + ((col >> 1) & 1),
+                                        2 * (q / (H * H)) + (col >> 2), y, Zb, ei);
+      __syncthreads();
+    }
+    return quad;
+  };
   // x = the boundary values, zero inside; the work vectors zero
   for (int f = tid; f < N; f += kItNT) {
     int t, p0, p1, p2;
@@ -1600,18 +1694,22 @@ __global__ __launch_bounds__(kItNT) void nrt_minres_kernel(NrtItArgs g) {
     }
     P[f] = 0.0;  // r1 enters the first iteration scaled by 0
     Q[f] = rr;
-    acc += rr * rr * pd[f];
+    if (kItEdgeSchwarz<L> && f < ne) Zb[f] = 0.0;
+    else acc += rr * rr * pd[f];
   }
   __syncthreads();
-  // r . M^-1 r: the diagonal part, plus q . S0^-1 q of the coarse correction
+  // r . M^-1 r: on the faces the diagonal part plus q . S0^-1 q of the
+  // coarse correction; on the edges the diagonal part, or at l = 4 the
+  // cells' quadratic forms of the additive Schwarz
   acc += it_coarse_correct<M>(Q, S0i, sq, scq);
+  if constexpr (kItEdgeSchwarz<L>) acc += edge_precond(Q);
   const double beta1 = sqrt(it_block_sum(acc, redB));
   int flag = isfinite(beta1) ? MSFEC_CELL_OK : MSFEC_CELL_NONFINITE;
   int itn = 0;
   if (flag == MSFEC_CELL_OK && beta1 > 0.0) {
     for (int i = tid; i < n; i += kItNT) {
       const int f = g.idof[i], ag = g.agg[i];
-      v[f] = (pd[f] * Q[f] + (ag >= 0 ? scq[ag] : 0.0)) / beta1;
+      v[f] = (ag >= 0 ? pd[f] * Q[f] + scq[ag] : (kItEdgeSchwarz<L> ? Zb[f] : pd[f] * Q[f])) / beta1;
     }
     __syncthreads();
     double oldb = 0.0, beta = beta1, dbar = 0.0, epsln = 0.0, phibar = beta1, cs = -1.0, sn = 0.0;
@@ -1631,7 +1729,6 @@ __global__ __launch_bounds__(kItNT) void nrt_minres_kernel(NrtItArgs g) {
           P[f] = -c1 * P[f];
         }
         __syncthreads();
-        constexpr int H = M / 2;
 #pragma unroll 1
         for (int col = 0; col < 8; ++col) {
           for (int q = tid; q < H * H * H; q += kItNT)
@@ -1661,10 +1758,12 @@ __global__ __launch_bounds__(kItNT) void nrt_minres_kernel(NrtItArgs g) {
         const int f = g.idof[i];
         const double y = P[f] - c2 * Q[f];
         P[f] = y;
-        acc += y * y * pd[f];
+        if (kItEdgeSchwarz<L> && f < ne) Zb[f] = 0.0;
+        else acc += y * y * pd[f];
       }
       __syncthreads();
       acc += it_coarse_correct<M>(P, S0i, sq, scq);
+      if constexpr (kItEdgeSchwarz<L>) acc += edge_precond(P);
       {
         double* tmp = P;
         P = Q;
@@ -1693,7 +1792,7 @@ __global__ __launch_bounds__(kItNT) void nrt_minres_kernel(NrtItArgs g) {
         const double wn = (v[f] - oldeps * W1[f] - delta * W0[f]) * denom;
         W1[f] = wn;
         x[f] += phi * wn;
-        v[f] = (pd[f] * Q[f] + (ag >= 0 ? scq[ag] : 0.0)) * sb;
+        v[f] = (ag >= 0 ? pd[f] * Q[f] + scq[ag] : (kItEdgeSchwarz<L> ? Zb[f] : pd[f] * Q[f])) * sb;
       }
       {
         double* tmp = W0;
@@ -1714,7 +1813,7 @@ __global__ __launch_bounds__(kItNT) void nrt_minres_kernel(NrtItArgs g) {
   // the assembled matrix times the block of this basis that enters the
   // element matrix (edges for an edge basis, faces for a face basis), over
   // every row, into r1's slot of the basis for nrt_it_element_kernel
-  double* Z = g.WK + ((size_t)cell * kNB + r) * 5 * N + N;
+  double* Z = g.WK + ((size_t)cell * kNB + r) * 6 * N + N;
   for (int f = tid; f < N; f += kItNT)
     Z[f] = r < 12 ? it_row<M, true, 1>(f, x, lm, gCc) : it_row<M, true, 2>(f, x, lm, gCc);
   if (tid == 0) {
@@ -1744,7 +1843,7 @@ __global__ __launch_bounds__(256) void nrt_it_element_kernel(NrtItArgs g) {
   for (int f = r0 + tid; f < r1; f += 256) {
     const double xv = xi[f];
 #pragma unroll
-    for (int j = 0; j < kNB; ++j) acc[j] += xv * g.WK[(((size_t)cell * kNB + j) * 5 + 1) * N + f];
+    for (int j = 0; j < kNB; ++j) acc[j] += xv * g.WK[(((size_t)cell * kNB + j) * 6 + 1) * N + f];
     if (i >= 12) {
       int t, p0, p1, p2;
       dof_pos<M>(f, t, p0, p1, p2);
@@ -1961,6 +2060,7 @@ struct msfec_nrt_batch {
   int* d_flg = nullptr;
   double* d_S0 = nullptr;
   int* d_agg = nullptr;
+  double* d_EI = nullptr;
   bool coef_set = false, has_r = false, has_f = false, computed = false;
   hipStream_t stream = nullptr;
   // per timed compute: start, after the local matrices and preconditioner,
@@ -1981,7 +2081,7 @@ void nrt_free(msfec_nrt_batch* h) {
   void* ptrs[] = {h->d_verts, h->d_ainv, h->d_b,  h->d_r,   h->d_f,  h->d_icell, h->d_cdof,
                   h->d_f2i,   h->d_amap, h->d_tnz, h->d_X,  h->d_M,  h->d_rhs,   h->d_w,
                   h->d_status, h->d_idof, h->d_LM, h->d_CC, h->d_FL, h->d_PD,    h->d_WK,
-                  h->d_its,   h->d_flg,  h->d_S0, h->d_agg};
+                  h->d_its,   h->d_flg,  h->d_S0, h->d_agg, h->d_EI};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (auto& set : h->ev_sets)
@@ -2055,6 +2155,7 @@ msfec_status nrt_it_launch(msfec_nrt_batch* h, hipStream_t s, std::array<hipEven
   a.flg = h->d_flg;
   a.S0 = h->d_S0;
   a.agg = h->d_agg;
+  a.EI = h->d_EI;
   a.status = h->d_status;
   a.n_cells = h->n_cells;
   a.N = h->tab.N;
@@ -2075,6 +2176,11 @@ msfec_status nrt_it_launch(msfec_nrt_batch* h, hipStream_t s, std::array<hipEven
     NRT_HIP(hipGetLastError());
     hipLaunchKernelGGL(nrt_it_coarse_inv_kernel<L>, dim3(nc), dim3(1024), 0, s, a);
     NRT_HIP(hipGetLastError());
+    constexpr int nfc = M * M * M, CPB = nfc < 64 ? nfc : 64;
+    if (kItEdgeSchwarz<L>) {
+      hipLaunchKernelGGL(nrt_it_edgeas_kernel<L>, dim3(nc * (nfc / CPB)), dim3(64), 0, s, a);
+      NRT_HIP(hipGetLastError());
+    }
   }
   if (ev) NRT_HIP(hipEventRecord((*ev)[1], s));
   // cells in groups of 8 (one per XCD), 18 bases each
@@ -2171,7 +2277,8 @@ msfec_status msfec_nrt_create(const msfec_nrt_params* params, int n_cells, const
     NRT_TRY(nrt_alloc(&h->d_CC, nc * 72));
     NRT_TRY(nrt_alloc(&h->d_FL, nc * nfc * 6));
     NRT_TRY(nrt_alloc(&h->d_PD, nc * N));
-    NRT_TRY(nrt_alloc(&h->d_WK, nc * kNB * 5 * N));
+    NRT_TRY(nrt_alloc(&h->d_WK, nc * kNB * 6 * N));
+    NRT_TRY(nrt_alloc(&h->d_EI, nc * 78 * nfc));
     NRT_TRY(nrt_alloc(&h->d_its, nc * kNB));
     NRT_TRY(nrt_alloc(&h->d_flg, nc * kNB));
     // the coarse space: aggregate of every interior face dof (-1 for edges)
