@@ -525,7 +525,7 @@ typedef struct msfec_nrt_batch *msfec_nrt_handle;
  * setup_system_matrix / setup_basis_dofs_curl / setup_basis_dofs_div
  * (source/Ned_RT/ned_rt_basis.cc:7-359, include/Ned_RT/ned_rt_basis.h).
  * MSFEC_ERR_UNSUPPORTED for l > 4.  The MINRES path holds 7 fine vectors per
- * basis and 177 doubles per fine cell in device memory (l = 4: 30 MiB a
+ * basis and 177 doubles per fine cell in device memory (l = 4: 32 MiB a
  * cell). */
 msfec_status msfec_nrt_create(const msfec_nrt_params *params, int n_cells,
                               const double *vertices, int vertices_loc,
