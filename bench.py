@@ -457,14 +457,22 @@ def nrt_config(steps: int, warmup: int, check: bool, cpu: bool):
 NRT4_NREF, NRT4_NC = 4, 4
 
 
-def nrt_minres_bytes_per_iteration(n: int, nfc: int) -> float:
+def nrt_minres_bytes_per_iteration(n: int, ns: int, nu: int, nfc: int, nc0: int,
+                                   edge_schwarz: bool) -> float:
     """Algorithmic bytes of one MINRES iteration of one basis: 16 passes over
     the n interior values (the product reads v once and reads and writes the
     first residual slot; the second pass reads both residuals and the
     preconditioner and writes one; the third reads v, w, w2, x, the new
-    residual and the preconditioner and writes w, x and v) and a share of the
-    cell's local matrices (99 values per fine cell, read by the 18 bases)."""
-    return 8.0 * (16 * n + 99 * nfc / 18.0)
+    residual and the preconditioner and writes w, x and v); a share of the
+    cell's local matrices (99 values per fine cell, read by the 18 bases);
+    the coarse correction (the nu face values restricted, the nc0 x nc0
+    inverse read); and, with the edge additive Schwarz, 5 passes over the ns
+    interior edges (z cleared, y gathered, z read and written, z read for
+    the next v) and a share of the 78 inverse entries per fine cell."""
+    b = 16 * n + 99 * nfc / 18.0 + nu + nc0 * nc0
+    if edge_schwarz:
+        b += 5 * ns + 78 * nfc / 18.0
+    return 8.0 * b
 
 
 def nrt4_config(steps: int, warmup: int, check: bool, cpu: bool):
@@ -495,14 +503,17 @@ def nrt4_config(steps: int, warmup: int, check: bool, cpu: bool):
     kt = nb.kernel_times()
     its = nb.get_iterations()
     m = 1 << NRT4_NREF
-    n_int = 3 * m * (m - 1) ** 2 + 3 * (m - 1) * m * m
-    bpi = nrt_minres_bytes_per_iteration(n_int, m ** 3)
+    ns, nu = 3 * m * (m - 1) ** 2, 3 * (m - 1) * m * m
+    nb = m // 2 if m <= 4 else 4
+    bpi = nrt_minres_bytes_per_iteration(ns + nu, ns, nu, m ** 3, 3 * nb ** 3, NRT4_NREF >= 4)
     gbs = float(its.sum()) * bpi / (kt[2] * 1e-3) / 1e9
     out = {"workload": "3D Nedelec x Raviart-Thomas basis, the reference's prm_ned_rt_test-01: "
                        "4^3=64 coarse cells x 16^3 fine cells (13872 edges + 13056 faces, "
                        "22320 interior unknowns), 12 edge + 6 face bases, its coefficients at "
-                       "the Gauss points; matrix-free preconditioned MINRES (relative "
-                       "tolerance 1e-14), one workgroup per basis",
+                       "the Gauss points; matrix-free MINRES (relative tolerance 1e-14) "
+                       "preconditioned by an element additive Schwarz on the edges and a "
+                       "diagonal plus 192-dof coarse correction on the faces, one workgroup "
+                       "per basis",
            "cells_per_s": round(n_cells / (wall / steps), 2),
            "ms_per_step": round(wall * 1e3 / steps, 3),
            "kernel_ms": {"total": round(kt[0], 3), "local_and_preconditioner": round(kt[1], 3),
