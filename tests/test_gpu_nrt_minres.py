@@ -154,6 +154,26 @@ def test_minres_l4_reference_example_against_assembled_system(oracle_mod):
     assert ex <= TOL_IT and rel(M[0], Ms) <= TOL_IT and rel(b[0], bs) <= TOL_IT
 
 
+def test_minres_l4_random_coefficients_against_sparse_direct(oracle_mod):
+    """l = 4 with random boxes, rotated SPD A^-1 tensors, B, a reaction rate
+    and loads at the Gauss points (the element additive Schwarz and coarse
+    correction on coefficients unlike the reference example's) against the
+    oracle's sparse direct solve."""
+    print("case nrt minres random l=4 n=2", flush=True)
+    rng = np.random.default_rng(961)
+    n, l = 2, 4
+    verts = boxes(rng, n)
+    ainv, bq, rq, fq = rand_coefs(rng, n, l)
+    M, b, X, st, its = run_minres(verts, l, ainv, bq, rq, fq)
+    assert (st == 0).all(), st
+    print("iterations", its.min(), its.max(), flush=True)
+    for c in range(n):
+        Ms, bs, Xs = oracle_mod.nrt_basis_cell_sparse(l, verts[c], ainv[c], bq[c], rq[c], fq[c])
+        ex = max(rel(X[c, i], Xs[i]) for i in range(18))
+        print(f"cell {c}: basis {ex:.2e} M {rel(M[c], Ms):.2e} b {rel(b[c], bs):.2e}", flush=True)
+        assert ex <= TOL_IT and rel(M[c], Ms) <= TOL_IT and rel(b[c], bs) <= TOL_IT
+
+
 def test_minres_iteration_limit_flags_the_cell():
     """A limit below what the solve needs leaves MSFEC_CELL_NOT_CONVERGED."""
     print("case nrt minres iteration limit", flush=True)
