@@ -394,6 +394,26 @@ def test_ned_rt_driver(oracle_mod):
     assert np.abs(sol - expect).max() <= 1e-13 * np.abs(expect).max()
 
 
+def test_ned_rt_driver_minres_path(oracle_mod):
+    """The NedRTBasis shim at l = 3, where the library solves by MINRES,
+    against the oracle's sparse direct solve."""
+    print("case ned_rt_shim_driver l=3 nc=2", flush=True)
+    assert os.path.exists(NRT_DRIVER), "driver not built"
+    l, nc = 3, 2
+    ncell = nc ** 3
+    m, nx, ne, nf1, nf, N = oracle_mod.nrt_sizes(l)
+    ainv, bq, fq = nrt_driver_inputs(np.random.default_rng(43), l, ncell)
+    with tempfile.TemporaryDirectory() as td:
+        d = run_nrt_driver(l, nc, ainv, bq, fq, td)
+    M = d[:ncell * 324].reshape(ncell, 18, 18)
+    X = d[ncell * 342:ncell * 342 + ncell * 18 * N].reshape(ncell, 18, N)
+    verts = unit_cube_boxes(oracle_mod, nc)
+    for c in range(ncell):
+        Ms, bs, Xs = oracle_mod.nrt_basis_cell_sparse(l, verts[c], ainv[c], bq[c], None, fq[c])
+        assert np.abs(M[c] - Ms).max() <= 1e-8 * np.abs(Ms).max()
+        assert np.abs(X[c] - Xs).max() <= 1e-8 * np.abs(Xs).max()
+
+
 def nrt_cell_centre_fields(v, l, h):
     """Nedelec field (edge line integrals averaged over the 4 parallel edges, over
     the edge length) and Raviart-Thomas field (face fluxes averaged over the 2
