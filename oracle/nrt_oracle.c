@@ -554,3 +554,154 @@ int oq_nrt_basis_cell(int l, const double *verts, const double *ainv, const doub
   free(pos);
   return st;
 }
+
+/* ---------------------------------------------------------------------------
+ * RT_DQ in three dimensions (source/RT_DQ/rt_dq_basis.cc), on the same fine
+ * grid: sigma on the fine faces (FE_RaviartThomas(0)), u per fine cell
+ * (FE_DGQ(0)):
+ *   (A^-1 sigma, tau) - (div tau, u) = 0,  (v, div sigma) + (R u, v) = -+(v, 1/|K|)
+ * (:488-538; R = 0, as eqn_coeff_R.cc returns).  Basis n fixes the normal
+ * trace of the coarse Raviart-Thomas function of face n on the boundary
+ * (:267-380); u is pinned in the last cell (the pure Neumann problem fixes
+ * it up to a constant) and its mean removed; set_u_to_std sets u = 1
+ * (:1039-1043) before the 7 x 7 element matrix (:822-914).  Dofs: the fine
+ * faces in the Ned_RT face numbering without the edge offset, then the fine
+ * cells. */
+static void rt3_local(const nrt_mesh *g, int c, const double *ainv, const double *fq,
+                      double K[7][7], double Fl[7]) {
+  memset(K, 0, sizeof(double) * 49);
+  memset(Fl, 0, sizeof(double) * 7);
+  const double vol = g->h[0] * g->h[1] * g->h[2];
+  for (int q = 0; q < 8; ++q) {
+    const double X[3] = {kNrtG[q & 1], kNrtG[(q >> 1) & 1], kNrtG[(q >> 2) & 1]};
+    const double w = vol / 8.0;
+    double psi[6][3], dv[6];
+    for (int F = 0; F < 6; ++F) rt_shape(F, X, g->h, psi[F], &dv[F]);
+    const double *t = ainv + ((size_t)c * 8 + q) * 6;
+    const double G[3][3] = {{t[0], t[1], t[2]}, {t[1], t[3], t[4]}, {t[2], t[4], t[5]}};
+    for (int i = 0; i < 6; ++i) {
+      for (int j = 0; j < 6; ++j) {
+        double v = 0.0;
+        for (int r = 0; r < 3; ++r)
+          for (int s = 0; s < 3; ++s) v += psi[i][r] * G[r][s] * psi[j][s];
+        K[i][j] += v * w;
+      }
+      K[i][6] -= dv[i] * w; /* - (div tau, u) */
+      K[6][i] += dv[i] * w; /* (v, div sigma) */
+    }
+    if (fq) Fl[6] += fq[(size_t)c * 8 + q] * w;
+  }
+}
+
+static void rt3_assemble(const nrt_mesh *g, const double *ainv, const double *fq, double *A,
+                         double *F) {
+  const int m = g->m, nf = g->nf, N = nf + m * m * m;
+  memset(A, 0, sizeof(double) * (size_t)N * N);
+  memset(F, 0, sizeof(double) * N);
+  for (int c3 = 0; c3 < m; ++c3)
+    for (int b = 0; b < m; ++b)
+      for (int a = 0; a < m; ++a) {
+        const int c = (c3 * m + b) * m + a;
+        double K[7][7], Fl[7];
+        int d18[18], d[7];
+        rt3_local(g, c, ainv, fq, K, Fl);
+        nrt_cell_dofs(g, a, b, c3, d18);
+        for (int i = 0; i < 6; ++i) d[i] = d18[12 + i] - g->ne;
+        d[6] = nf + c;
+        for (int i = 0; i < 7; ++i) {
+          for (int j = 0; j < 7; ++j) A[(size_t)d[i] * N + d[j]] += K[i][j];
+          F[d[i]] += Fl[i];
+        }
+      }
+}
+
+int oq_rt3_assemble(int l, const double *verts, const double *ainv, const double *fq, double *A,
+                    double *F) {
+  nrt_mesh g;
+  if (nrt_mesh_init(l, 3, verts, &g)) return 1;
+  rt3_assemble(&g, ainv, fq, A, F);
+  return 0;
+}
+
+int oq_rt3_basis_cell(int l, const double *verts, const double *ainv, const double *fq,
+                      double *M, double *b, double *Sigma) {
+  nrt_mesh g;
+  if (nrt_mesh_init(l, 3, verts, &g)) return 1;
+  const int nf = g.nf, nc = g.m * g.m * g.m, N = nf + nc;
+  double *A = (double *)malloc(sizeof(double) * (size_t)N * N);
+  double *F = (double *)malloc(sizeof(double) * N);
+  double *fix = (double *)calloc((size_t)N * 6, sizeof(double));
+  int *pos = (int *)malloc(sizeof(int) * N);
+  int st = (A && F && fix && pos) ? 0 : 3;
+  double *S = NULL, *B = NULL;
+  int n = 0;
+  if (!st) {
+    rt3_assemble(&g, ainv, fq, A, F);
+    for (int d = 0; d < N; ++d) {
+      int t, p[3];
+      if (d < nf && nrt_on_boundary(&g, g.ne + d, &t, p)) {
+        pos[d] = -1;
+        for (int k = 0; k < 6; ++k) fix[(size_t)d * 6 + k] = nrt_boundary_value(&g, t, p, 12 + k);
+      } else if (d == N - 1) {
+        pos[d] = -1; /* u of the last cell pinned to 0 */
+      } else {
+        pos[d] = n++;
+      }
+    }
+    S = (double *)malloc(sizeof(double) * (size_t)n * n);
+    B = (double *)calloc((size_t)n * 6, sizeof(double));
+    if (!S || !B) st = 3;
+  }
+  if (!st) {
+    const double volK = g.H[0] * g.H[1] * g.H[2], vol = g.h[0] * g.h[1] * g.h[2];
+    for (int r = 0; r < N; ++r) {
+      const int pr = pos[r];
+      if (pr < 0) continue;
+      for (int c = 0; c < N; ++c) {
+        const double v = A[(size_t)r * N + c];
+        if (pos[c] >= 0) S[(size_t)pr * n + pos[c]] = v;
+        else if (v != 0.0)
+          for (int k = 0; k < 6; ++k) B[(size_t)pr * 6 + k] -= v * fix[(size_t)c * 6 + k];
+      }
+      if (r >= nf) /* -+ (v, 1/|K|): minus for the lower face of an axis (:516-538) */
+        for (int k = 0; k < 6; ++k) B[(size_t)pr * 6 + k] += (k % 2 ? 1.0 : -1.0) * vol / volK;
+    }
+    st = nrt_solve(n, S, 6, B);
+  }
+  if (!st) {
+    double *sol = fix; /* reuse: the full solution, boundary values already there */
+    for (int d = 0; d < N; ++d)
+      if (pos[d] >= 0)
+        for (int k = 0; k < 6; ++k) sol[(size_t)d * 6 + k] = B[(size_t)pos[d] * 6 + k];
+    if (Sigma)
+      for (int k = 0; k < 6; ++k)
+        for (int f = 0; f < nf; ++f) Sigma[(size_t)k * nf + f] = sol[(size_t)f * 6 + k];
+    /* set_u_to_std, then the block products: test and trial vectors are the
+     * flux block of basis i (i < 6) or the u block of basis 0 (i = 6), now
+     * the constant 1 */
+    for (int it = 0; it < 7; ++it) {
+      for (int jt = 0; jt < 7; ++jt) {
+        double acc = 0.0;
+        for (int r = (it < 6 ? 0 : nf); r < (it < 6 ? nf : N); ++r) {
+          const double tv = it < 6 ? sol[(size_t)r * 6 + it] : 1.0;
+          double row = 0.0;
+          for (int c = (jt < 6 ? 0 : nf); c < (jt < 6 ? nf : N); ++c)
+            row += A[(size_t)r * N + c] * (jt < 6 ? sol[(size_t)c * 6 + jt] : 1.0);
+          acc += tv * row;
+        }
+        M[it * 7 + jt] = acc;
+      }
+      double rb = 0.0;
+      if (it == 6)
+        for (int c = 0; c < nc; ++c) rb += F[nf + c];
+      b[it] = rb;
+    }
+  }
+  free(S);
+  free(B);
+  free(A);
+  free(F);
+  free(fix);
+  free(pos);
+  return st;
+}

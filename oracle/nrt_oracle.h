@@ -78,6 +78,16 @@ int oq_nrt_assemble_coo(int l, const double *verts, const double *ainv, const do
 /* The boundary dofs (on_boundary, N ints) and the fixed values of the 18
  * bases on them (G, 18 x N, zero inside), l in 1..4. */
 int oq_nrt_boundary(int l, const double *verts, double *G, int *on_boundary);
+/* RT_DQ in three dimensions (source/RT_DQ/rt_dq_basis.cc) on the same grid:
+ * dofs the fine faces (numbered as above without the edge offset, nf of
+ * them), then the fine cells.  ainv as above; fq: the scalar load at the 8
+ * points of every fine cell (NULL: zero); no reaction.  Outputs M (7 x 7: the
+ * 6 face bases in deal.II face order, then the DQ0 basis), b (7) and Sigma
+ * (6 x nf, the fine fluxes of the face bases; their u is 1).  l in 1..3. */
+int oq_rt3_assemble(int l, const double *verts, const double *ainv, const double *fq, double *A,
+                    double *F);
+int oq_rt3_basis_cell(int l, const double *verts, const double *ainv, const double *fq,
+                      double *M, double *b, double *Sigma);
 
 #ifdef __cplusplus
 }

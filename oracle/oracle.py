@@ -108,6 +108,10 @@ def lib():
         L.oq_nrt_assemble_coo.restype = ctypes.c_int
         L.oq_nrt_boundary.argtypes = [ctypes.c_int, dp, dp, ip]
         L.oq_nrt_boundary.restype = ctypes.c_int
+        L.oq_rt3_assemble.argtypes = [ctypes.c_int, dp, dp, dp, dp, dp]
+        L.oq_rt3_assemble.restype = ctypes.c_int
+        L.oq_rt3_basis_cell.argtypes = [ctypes.c_int, dp, dp, dp, dp, dp, dp]
+        L.oq_rt3_basis_cell.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -534,6 +538,42 @@ def nrt_basis_cell_sparse(l, verts, ainv, bq=1.0, rq=None, fq=None):
             M[i, j] = X[i, ri] @ (Ye if j < 12 else Yf)[ri, j]
     b = np.array([0.0] * 12 + [X[i, ne:] @ F[ne:] for i in range(12, 18)])
     return M, b, X
+
+
+def rt3_sizes(l):
+    """(m, nf, nc, N): fine cells per direction, fine faces, fine cells, all
+    dofs of the 3-D RT_DQ system."""
+    m = 1 << l
+    nf = 3 * m * m * (m + 1)
+    return m, nf, m ** 3, nf + m ** 3
+
+
+def rt3_basis_cell(l, verts, ainv, fq=None):
+    """3-D RT_DQ basis of one box cell: (M 7x7, b 7, Sigma 6 x nf, status)."""
+    nc = (1 << l) ** 3
+    ainv = np.ascontiguousarray(np.broadcast_to(np.asarray(ainv, dtype=np.float64), (nc, 8, 6)))
+    fq = None if fq is None else np.ascontiguousarray(
+        np.broadcast_to(np.asarray(fq, dtype=np.float64), (nc, 8)))
+    v = np.ascontiguousarray(verts, dtype=np.float64).reshape(8, 3)
+    nf = rt3_sizes(l)[1]
+    M = np.zeros((7, 7))
+    b = np.zeros(7)
+    S = np.zeros((6, nf))
+    st = lib().oq_rt3_basis_cell(l, _dp(v), _dp(ainv), _dp(fq), _dp(M), _dp(b), _dp(S))
+    return M, b, S, st
+
+
+def rt3_assemble(l, verts, ainv, fq=None):
+    nc = (1 << l) ** 3
+    ainv = np.ascontiguousarray(np.broadcast_to(np.asarray(ainv, dtype=np.float64), (nc, 8, 6)))
+    fq = None if fq is None else np.ascontiguousarray(
+        np.broadcast_to(np.asarray(fq, dtype=np.float64), (nc, 8)))
+    v = np.ascontiguousarray(verts, dtype=np.float64).reshape(8, 3)
+    N = rt3_sizes(l)[3]
+    A = np.zeros((N, N))
+    F = np.zeros(N)
+    assert lib().oq_rt3_assemble(l, _dp(v), _dp(ainv), _dp(fq), _dp(A), _dp(F)) == 0
+    return A, F
 
 
 def box_vertices_3d(x0, x1):
