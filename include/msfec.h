@@ -583,6 +583,60 @@ msfec_status msfec_nrt_get_iterations(msfec_nrt_handle h, int *iterations);
 msfec_status msfec_nrt_kernel_times(msfec_nrt_handle h, float *ms, int max, int *n);
 msfec_status msfec_nrt_destroy(msfec_nrt_handle h);
 
+/* ---------------------------------------------------------------------------
+ * RT_DQ in three dimensions, as the reference's RTDQBasis
+ * (source/RT_DQ/rt_dq_basis.cc): the 6 face bases of the mixed
+ * Raviart-Thomas x DQ0 system per coarse box cell, on the 2^l x 2^l x 2^l
+ * fine grid (fine faces numbered as the faces of the Ned_RT section without
+ * the edge offset: nf = 3 m^2 (m+1)).  The flux of basis n is the coarse
+ * Raviart-Thomas function of face n plus the curl of an interior Nedelec
+ * field, solved by Jacobi-PCG on the curl-curl system; u is 1
+ * (set_u_to_std).  No reaction term (the reference's ReactionRate is zero).
+ * ------------------------------------------------------------------------- */
+typedef struct {
+  int n_refine_local; /* l, 1..4                                                  */
+  int flags;          /* MSFEC_FLAG_*                                             */
+  int max_iter;       /* CG iteration limit, 0: 20000 (MSFEC_CELL_NOT_CONVERGED) */
+  double rtol;        /* CG stop: |r| <= rtol |b|, 0: 1e-13                       */
+} msfec_rt3_params;
+
+typedef struct msfec_rt3_batch *msfec_rt3_handle;
+
+/* vertices: n_cells x 8 x 3, axis-aligned boxes in deal.II vertex order.
+ * Replaces RTDQBasis::RTDQBasis + setup_grid / setup_system_matrix /
+ * setup_basis_dofs (rt_dq_basis.cc:7-380). */
+msfec_status msfec_rt3_create(const msfec_rt3_params *params, int n_cells,
+                              const double *vertices, int vertices_loc,
+                              msfec_rt3_handle *out);
+/* ainv: n_cells x 8^l x 8 x 6 (xx, xy, xz, yy, yz, zz of DiffusionInverse_A at
+ * the QGauss(2) points of every fine cell, fine cells x fastest); f:
+ * n_cells x 8^l x 8 (the right-hand side at the same points), NULL for zero
+ * (rt_dq_basis.cc:455-470). */
+msfec_status msfec_rt3_set_coefficients(msfec_rt3_handle h, const double *ainv, const double *f,
+                                        int loc);
+/* All cells: the 6 basis solves and the element matrix, enqueued on stream.
+ * Replaces RTDQBasis::run (rt_dq_basis.cc:1114-1170) and
+ * assemble_global_element_matrix (:822-914). */
+msfec_status msfec_rt3_compute(msfec_rt3_handle h, void *stream);
+msfec_status msfec_rt3_synchronize(msfec_rt3_handle h);
+/* M: n_cells x 7 x 7 (row = test basis: the 6 face bases in deal.II face
+ * order, then the DQ0 basis), b: n_cells x 7 (zero but b_6 = (f, 1))
+ * (get_global_element_matrix/_rhs, rt_dq_basis.cc:1096-1106). */
+msfec_status msfec_rt3_get_element(msfec_rt3_handle h, double *M, double *b, int loc);
+/* sigma: n_cells x 6 x nf, the fine fluxes of the face bases in the +x, +y,
+ * +z directions (the sigma block of basis_div_v; its u block is 1). */
+msfec_status msfec_rt3_get_basis(msfec_rt3_handle h, double *sigma, int loc);
+/* MSFEC_CELL_OK, MSFEC_CELL_NOT_CONVERGED, MSFEC_CELL_NOT_SPD (a breakdown of
+ * CG), MSFEC_CELL_NONFINITE.  Synchronizes. */
+msfec_status msfec_rt3_get_cell_status(msfec_rt3_handle h, int *status);
+/* CG iterations of the last compute, n_cells x 6 ints on the host.
+ * Synchronizes. */
+msfec_status msfec_rt3_get_iterations(msfec_rt3_handle h, int *iterations);
+/* Mean times in ms over the computes made with MSFEC_FLAG_TIMING since the
+ * previous call: whole, face masses + preconditioner, CG, element matrix. */
+msfec_status msfec_rt3_kernel_times(msfec_rt3_handle h, float *ms, int max, int *n);
+msfec_status msfec_rt3_destroy(msfec_rt3_handle h);
+
 /* CG iterations of the last compute, n_cells x 2^dim ints on the host
  * (basis-minor), for a handle created with MSFEC_SOLVER_CG.  The reference
  * reports CG iterations per basis solve in verbose mode

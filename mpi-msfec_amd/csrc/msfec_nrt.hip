@@ -1885,6 +1885,396 @@ __global__ __launch_bounds__(256) void nrt_it_status_kernel(NrtItArgs g) {
   g.status[cell] = st;
 }
 
+// ===========================================================================
+// RT_DQ in three dimensions (source/RT_DQ/rt_dq_basis.cc): the 6 face bases
+// of the mixed Raviart-Thomas x DQ0 system.  As in two dimensions the flux
+// of basis n is the coarse Raviart-Thomas function c_n (which has the
+// prescribed divergence -+1/|K| and boundary flux) plus a divergence-free
+// correction with zero boundary flux; in three dimensions those fields are
+// the curls of the interior Nedelec fields (the discrete exact sequence), so
+// sigma_n = c_n + D psi_n with D the face circulation of the edge values and
+// psi_n solving the curl-curl system
+//   D^T M_A D psi = -D^T M_A c_n   (M_A the face mass with A^-1)
+// over the interior edges: the first mixed equation tested with the
+// divergence-free fields, the second holding by construction.  The system
+// is semi-definite (gradients are its kernel) and consistent; Jacobi-PCG
+// converges on its range, and sigma does not see the kernel.  u is 1
+// (set_u_to_std, :1039-1043).  No reaction term (ReactionRate is zero,
+// eqn_coeff_R.cc:8-14).
+
+struct Rt3Args {
+  const double* verts;  // n_cells x 8 x 3
+  const double* ainv;   // n_cells x nfc x 8 x 6
+  const double* fq;     // n_cells x nfc x 8, or null
+  const int* iedge;     // ns: the interior edges
+  double* MF;           // n_cells x 21 x nfc: face mass with A^-1 per fine cell (colour-major)
+  double* PD;           // n_cells x ne: inverse Jacobi diagonal (0 on the boundary)
+  double* WK;           // n_cells x 6 x 4 x ne: x, r, p, q
+  double* S;            // n_cells x 6 x nf: the fine fluxes
+  double* M;            // n_cells x 7 x 7
+  double* b;            // n_cells x 7
+  int* its;             // n_cells x 6
+  int* flg;             // n_cells x 6
+  int* status;          // n_cells
+  int n_cells, ns, max_iter;
+  double rtol;
+};
+
+// Circulation sign of local line L in local face F of a fine cell (counter-
+// clockwise seen from the face's +axis side), 0 when L is not on F: the
+// coefficient of face F in the curl of the Nedelec function of L.
+constexpr int rt3_inc(int F, int L) {
+  const int ax = kLAx[L], o1 = ax == 0 ? 1 : 0, o2 = ax == 2 ? 1 : 2;
+  const int f1 = kLF1[L], f2 = kLF2[L], d = F / 2, s = F % 2;
+  if (d == ax) return 0;
+  if ((d == o1 ? f1 : f2) != s) return 0;
+  const int a = (d + 1) % 3, bb = (d + 2) % 3;
+  if (ax == a) return (bb == o1 ? f1 : f2) == 0 ? 1 : -1;
+  return (a == o1 ? f1 : f2) == 1 ? 1 : -1;
+}
+
+// the coarse Raviart-Thomas function of face n: its flux through local face
+// F of fine cell (c0, c1, c2)
+template <int M>
+__device__ __forceinline__ double rt3_coarse_flux(int c0, int c1, int c2, int F, int n) {
+  const int d = F >> 1;
+  if (d != (n >> 1)) return 0.0;
+  const int lev = (d == 0 ? c0 : (d == 1 ? c1 : c2)) + (F & 1);
+  return lag(n & 1, (double)lev / M) / (M * M);
+}
+
+// fine face index (without the edge offset) of local face F
+template <int M>
+__device__ __forceinline__ int rt3_face(int c0, int c1, int c2, int F) {
+  constexpr int nx = M * (M + 1) * (M + 1), ne = 3 * nx;
+  return it_cdof<M>(c0, c1, c2, 12 + F) - ne;
+}
+
+// The face mass with A^-1 at the 8 Gauss points of every fine cell
+// ((psi_F, A^-1 psi_G): psi_F has only its axis component), 21 packed
+// entries, entry-major over colour-major cells.
+template <int L>
+__global__ __launch_bounds__(256) void rt3_local_kernel(Rt3Args g) {
+  constexpr int M = 1 << L, nfc = M * M * M;
+  const int cell = blockIdx.x;
+  V3 Hc, h;
+  it_sizes(g.verts + (size_t)cell * 24, Hc, h, M);
+  const double w8 = h.x * h.y * h.z * 0.125;
+  double* mf = g.MF + (size_t)cell * 21 * nfc;
+  for (int idx = threadIdx.x; idx < nfc * 21; idx += 256) {
+    const int e = idx / nfc, c = idx - e * nfc;
+    int F, G;
+    tri_ij(e, F, G);
+    const int comp = sym6(F >> 1, G >> 1);
+    const double* a = g.ainv + ((size_t)cell * nfc + c) * 48;
+    double acc = 0.0;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const V3 X = gauss_xyz(q);
+      acc += rt_val(F, X, h) * rt_val(G, X, h) * a[q * 6 + comp];
+    }
+    mf[(size_t)e * nfc + it_lmc<M>(c % M, (c / M) % M, c / (M * M))] = acc * w8;
+  }
+}
+
+// the inverse Jacobi diagonal of D^T M_A D on the interior edges
+template <int L>
+__global__ __launch_bounds__(256) void rt3_diag_kernel(Rt3Args g) {
+  constexpr int M = 1 << L, nfc = M * M * M, ne = 3 * M * (M + 1) * (M + 1);
+  const int cell = blockIdx.x;
+  const double* mf = g.MF + (size_t)cell * 21 * nfc;
+  double* pd = g.PD + (size_t)cell * ne;
+  for (int e = threadIdx.x; e < ne; e += 256) {
+    int t, p0, p1, p2;
+    dof_pos<M>(e, t, p0, p1, p2);
+    if (it_on_boundary<M>(t, p0, p1, p2)) {
+      pd[e] = 0.0;
+      continue;
+    }
+    double d = 0.0;
+    for (int k = 0; k < 4; ++k) {
+      int c0, c1, c2, a;
+      it_cell<M>(t, p0, p1, p2, k, c0, c1, c2, a);
+      const int sc = it_lmc<M>(c0, c1, c2);
+      for (int F = 0; F < 6; ++F)
+        for (int G = 0; G < 6; ++G) {
+          const int sF = rt3_inc(F, a), sG = rt3_inc(G, a);
+          if (sF && sG) {
+            const int hi = max(F, G), lo = min(F, G);
+            d += sF * sG * mf[(size_t)(hi * (hi + 1) / 2 + lo) * nfc + sc];
+          }
+        }
+    }
+    pd[e] = d > 0.0 ? 1.0 / d : 0.0;
+  }
+}
+
+// For fine cell (c0, c1, c2): the circulations D_c p of its 6 faces from
+// its 12 edge values (plus the coarse flux cf when given), z = M_A,c D_c p,
+// and y += +-D_c^T z on its interior edges (sign sg); returns (D_c p) . z.
+template <int M>
+__device__ __forceinline__ double rt3_cell_apply(int c0, int c1, int c2, const double* p, double* y,
+                                                 const double* mf, int nb, double sg) {
+  constexpr int nfc = M * M * M;
+  const unsigned sc = (unsigned)it_lmc<M>(c0, c1, c2);
+  double circ[6];
+  if (nb < 0) {
+    double pv[12];
+#pragma unroll
+    for (int Ln = 0; Ln < 12; ++Ln) pv[Ln] = p[it_cdof<M>(c0, c1, c2, Ln)];
+#pragma unroll
+    for (int F = 0; F < 6; ++F) {
+      double s = 0.0;
+#pragma unroll
+      for (int Ln = 0; Ln < 12; ++Ln) {
+        const int sgn = rt3_inc(F, Ln);
+        if (sgn) s += sgn * pv[Ln];
+      }
+      circ[F] = s;
+    }
+  } else {
+#pragma unroll
+    for (int F = 0; F < 6; ++F) circ[F] = rt3_coarse_flux<M>(c0, c1, c2, F, nb);
+  }
+  double z[6];
+  double quad = 0.0;
+#pragma unroll
+  for (int F = 0; F < 6; ++F) {
+    double s = 0.0;
+#pragma unroll
+    for (int G = 0; G < 6; ++G) {
+      const int e = F >= G ? F * (F + 1) / 2 + G : G * (G + 1) / 2 + F;
+      s += (mf + (size_t)e * nfc)[sc] * circ[G];
+    }
+    z[F] = s;
+    quad += circ[F] * s;
+  }
+#pragma unroll
+  for (int Ln = 0; Ln < 12; ++Ln) {
+    if (it_local_on_boundary<M>(c0, c1, c2, Ln)) continue;
+    double s = 0.0;
+#pragma unroll
+    for (int F = 0; F < 6; ++F) {
+      const int sgn = rt3_inc(F, Ln);
+      if (sgn) s += sgn * z[F];
+    }
+    y[it_cdof<M>(c0, c1, c2, Ln)] += sg * s;
+  }
+  return quad;
+}
+
+// One face basis of one coarse cell per workgroup: Jacobi-PCG on the
+// curl-curl system (the product element by element in 8 colour passes, its
+// quadratic form p . A p accumulated on the way), then the fine fluxes
+// sigma = c_n + D psi on every face.
+template <int L>
+__global__ __launch_bounds__(kItNT) void rt3_cg_kernel(Rt3Args g) {
+  constexpr int M = 1 << L, nfc = M * M * M, nx = M * (M + 1) * (M + 1), ne = 3 * nx;
+  constexpr int nf1 = M * M * (M + 1), nf = 3 * nf1, H = M / 2;
+  __shared__ double redA[kItNW], redB[kItNW];
+  const int bid = blockIdx.x, slot = bid >> 3;
+  const int cell = (slot / 6) * 8 + (bid & 7), nb = slot % 6;
+  if (cell >= g.n_cells) return;
+  const int tid = threadIdx.x, ns = g.ns;
+  const double* mf = g.MF + (size_t)cell * 21 * nfc;
+  const double* pd = g.PD + (size_t)cell * ne;
+  double* x = g.WK + ((size_t)cell * 6 + nb) * 4 * ne;
+  double* r = x + ne;
+  double* p = x + 2 * ne;
+  double* q = x + 3 * ne;
+  for (int e = tid; e < ne; e += kItNT) {
+    x[e] = 0.0;
+    r[e] = 0.0;
+    p[e] = 0.0;
+    q[e] = 0.0;
+  }
+  __syncthreads();
+  // r = b = -D^T M_A c_n
+#pragma unroll 1
+  for (int col = 0; col < 8; ++col) {
+    for (int cc = tid; cc < H * H * H; cc += kItNT)
+      rt3_cell_apply<M>(2 * (cc % H) + (col & 1), 2 * ((cc / H) % H) + ((col >> 1) & 1),
+                        2 * (cc / (H * H)) + (col >> 2), nullptr, r, mf, nb, -1.0);
+    __syncthreads();
+  }
+  double rz = 0.0, rr = 0.0;
+  for (int i = tid; i < ns; i += kItNT) {
+    const int e = g.iedge[i];
+    const double z = pd[e] * r[e];
+    p[e] = z;
+    rz += r[e] * z;
+    rr += r[e] * r[e];
+  }
+  rz = it_block_sum(rz, redA);
+  const double bnorm = sqrt(it_block_sum(rr, redB));
+  int flag = isfinite(bnorm) ? MSFEC_CELL_OK : MSFEC_CELL_NONFINITE;
+  int itn = 0;
+  if (flag == MSFEC_CELL_OK && bnorm > 0.0) {
+    const double stop = g.rtol * bnorm;
+    flag = MSFEC_CELL_NOT_CONVERGED;
+    while (itn < g.max_iter) {
+      ++itn;
+      // q = A p (q cleared by the previous pass); p . A p from the cells
+      double pq = 0.0;
+#pragma unroll 1
+      for (int col = 0; col < 8; ++col) {
+        for (int cc = tid; cc < H * H * H; cc += kItNT)
+          pq += rt3_cell_apply<M>(2 * (cc % H) + (col & 1), 2 * ((cc / H) % H) + ((col >> 1) & 1),
+                                  2 * (cc / (H * H)) + (col >> 2), p, q, mf, -1, 1.0);
+        if (col < 7) __syncthreads();
+      }
+      pq = it_block_sum(pq, redA);
+      if (!(pq > 0.0)) {
+        flag = isfinite(pq) ? MSFEC_CELL_NOT_SPD : MSFEC_CELL_NONFINITE;
+        break;
+      }
+      const double alpha = rz / pq;
+      double rzn = 0.0;
+      rr = 0.0;
+      for (int i = tid; i < ns; i += kItNT) {
+        const int e = g.iedge[i];
+        x[e] += alpha * p[e];
+        const double re = r[e] - alpha * q[e];
+        r[e] = re;
+        rzn += re * pd[e] * re;
+        rr += re * re;
+      }
+      rzn = it_block_sum(rzn, redB);
+      rr = sqrt(it_block_sum(rr, redA));
+      if (!isfinite(rr)) {
+        flag = MSFEC_CELL_NONFINITE;
+        break;
+      }
+      if (rr <= stop) {
+        flag = MSFEC_CELL_OK;
+        break;
+      }
+      const double beta = rzn / rz;
+      rz = rzn;
+      for (int i = tid; i < ns; i += kItNT) {
+        const int e = g.iedge[i];
+        p[e] = pd[e] * r[e] + beta * p[e];
+        q[e] = 0.0;
+      }
+      __syncthreads();
+    }
+  }
+  __syncthreads();
+  // sigma = c_n + D psi on every face (a face's circulation from its 4 edges)
+  double* sg = g.S + ((size_t)cell * 6 + nb) * nf;
+  auto EX = [](int i, int j, int k) { return (k * (M + 1) + j) * M + i; };
+  auto EY = [](int i, int j, int k) { return nx + (k * M + j) * (M + 1) + i; };
+  auto EZ = [](int i, int j, int k) { return 2 * nx + (k * (M + 1) + j) * (M + 1) + i; };
+  for (int f = tid; f < nf; f += kItNT) {
+    int t, i, j, k;
+    dof_pos<M>(ne + f, t, i, j, k);
+    double circ, c = 0.0;
+    const int d = t - 3, lev = d == 0 ? i : (d == 1 ? j : k);
+    if (d == 0) circ = x[EY(i, j, k)] + x[EZ(i, j + 1, k)] - x[EY(i, j, k + 1)] - x[EZ(i, j, k)];
+    else if (d == 1) circ = x[EZ(i, j, k)] + x[EX(i, j, k + 1)] - x[EZ(i + 1, j, k)] - x[EX(i, j, k)];
+    else circ = x[EX(i, j, k)] + x[EY(i + 1, j, k)] - x[EX(i, j + 1, k)] - x[EY(i, j, k)];
+    if (d == (nb >> 1)) c = lag(nb & 1, (double)lev / M) / (M * M);
+    sg[f] = c + circ;
+  }
+  if (tid == 0) {
+    g.its[(size_t)cell * 6 + nb] = itn;
+    g.flg[(size_t)cell * 6 + nb] = flag;
+  }
+}
+
+// Row i of the 7 x 7 element matrix of one coarse cell per workgroup
+// (assemble_global_element_matrix, rt_dq_basis.cc:822-914, with u = 1):
+// the flux block sigma_i^T M_A sigma_j from the fine cells' face masses,
+// the divergence entries -+ the net outflow, b_6 the integral of f.
+template <int L>
+__global__ __launch_bounds__(256) void rt3_element_kernel(Rt3Args g) {
+  constexpr int M = 1 << L, nfc = M * M * M, nf = 3 * M * M * (M + 1);
+  __shared__ double red[4][8];
+  const int cell = blockIdx.x / 7, i = blockIdx.x % 7, tid = threadIdx.x;
+  const double* mf = g.MF + (size_t)cell * 21 * nfc;
+  const double* S = g.S + (size_t)cell * 6 * nf;
+  double acc[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) acc[j] = 0.0;
+  for (int c = tid; c < nfc; c += 256) {
+    const int c0 = c % M, c1 = (c / M) % M, c2 = c / (M * M);
+    const int sc = it_lmc<M>(c0, c1, c2);
+    int fi[6];
+#pragma unroll
+    for (int F = 0; F < 6; ++F) fi[F] = rt3_face<M>(c0, c1, c2, F);
+    if (i < 6) {
+      double si[6];
+#pragma unroll
+      for (int F = 0; F < 6; ++F) si[F] = S[(size_t)i * nf + fi[F]];
+      double ms[6];
+#pragma unroll
+      for (int F = 0; F < 6; ++F) {
+        double s = 0.0;
+#pragma unroll
+        for (int G = 0; G < 6; ++G) {
+          const int e = F >= G ? F * (F + 1) / 2 + G : G * (G + 1) / 2 + F;
+          s += mf[(size_t)e * nfc + sc] * si[G];
+        }
+        ms[F] = s;
+      }
+#pragma unroll
+      for (int j = 0; j < 6; ++j) {
+        double s = 0.0;
+#pragma unroll
+        for (int F = 0; F < 6; ++F) s += ms[F] * S[(size_t)j * nf + fi[F]];
+        acc[j] += s;
+      }
+      // net outflow of sigma_i from this fine cell
+#pragma unroll
+      for (int F = 0; F < 6; ++F) acc[6] += (F & 1 ? 1.0 : -1.0) * si[F];
+    } else if (g.fq) {
+      V3 Hc, h;
+      it_sizes(g.verts + (size_t)cell * 24, Hc, h, M);
+      const double* f = g.fq + ((size_t)cell * nfc + c) * 8;
+      double s = 0.0;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) s += f[q];
+      acc[7] += s * h.x * h.y * h.z * 0.125;
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    double s = acc[j];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    if ((tid & 63) == 0) red[tid >> 6][j] = s;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double t[8];
+    for (int j = 0; j < 8; ++j) t[j] = red[0][j] + red[1][j] + red[2][j] + red[3][j];
+    double* Mc = g.M + (size_t)cell * 49;
+    double* bc = g.b + (size_t)cell * 7;
+    if (i < 6) {
+      for (int j = 0; j < 6; ++j) Mc[i * 7 + j] = t[j];
+      Mc[i * 7 + 6] = -t[6];  // -(div sigma_i, 1)
+      Mc[6 * 7 + i] = t[6];   // (1, div sigma_i)
+      bc[i] = 0.0;
+    } else {
+      Mc[6 * 7 + 6] = 0.0;    // (R u, v) with R = 0
+      bc[6] = t[7];
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void rt3_status_kernel(Rt3Args g) {
+  const int cell = blockIdx.x * 256 + threadIdx.x;
+  if (cell >= g.n_cells) return;
+  int st = MSFEC_CELL_OK;
+  for (int n = 0; n < 6; ++n) {
+    const int f = g.flg[(size_t)cell * 6 + n];
+    if (f != MSFEC_CELL_OK && st == MSFEC_CELL_OK) st = f;
+  }
+  for (int k = 0; k < 49 && st == MSFEC_CELL_OK; ++k)
+    if (!isfinite(g.M[(size_t)cell * 49 + k])) st = MSFEC_CELL_NONFINITE;
+  g.status[cell] = st;
+}
+
 msfec_status nrt_fail(msfec_status st, const std::string& msg) {
   msfec_detail::g_last_error = msg;
   return st;
@@ -2483,6 +2873,267 @@ msfec_status msfec_nrt_destroy(msfec_nrt_handle h) {
   if (!h) return MSFEC_OK;
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   nrt_free(h);
+  delete h;
+  return MSFEC_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Host side of the 3-D RT_DQ basis (msfec_rt3_*).
+
+struct msfec_rt3_batch {
+  msfec_rt3_params prm{};
+  int n_cells = 0, l = 0, ne = 0, nf = 0, ns = 0;
+  double* d_verts = nullptr;
+  double* d_ainv = nullptr;
+  double* d_f = nullptr;
+  int* d_iedge = nullptr;
+  double* d_MF = nullptr;
+  double* d_PD = nullptr;
+  double* d_WK = nullptr;
+  double* d_S = nullptr;
+  double* d_M = nullptr;
+  double* d_b = nullptr;
+  int* d_its = nullptr;
+  int* d_flg = nullptr;
+  int* d_status = nullptr;
+  bool coef_set = false, has_f = false, computed = false;
+  hipStream_t stream = nullptr;
+  std::vector<std::array<hipEvent_t, 4>> ev_sets;
+  size_t ev_used = 0;
+};
+
+namespace {
+
+void rt3_free(msfec_rt3_batch* h) {
+  void* ptrs[] = {h->d_verts, h->d_ainv, h->d_f, h->d_iedge, h->d_MF,  h->d_PD,    h->d_WK,
+                  h->d_S,     h->d_M,    h->d_b, h->d_its,   h->d_flg, h->d_status};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  for (auto& set : h->ev_sets)
+    for (auto& e : set)
+      if (e) (void)hipEventDestroy(e);
+  h->ev_sets.clear();
+}
+
+template <int L>
+msfec_status rt3_launch(msfec_rt3_batch* h, hipStream_t s, std::array<hipEvent_t, 4>* ev) {
+  Rt3Args a{};
+  a.verts = h->d_verts;
+  a.ainv = h->d_ainv;
+  a.fq = h->has_f ? h->d_f : nullptr;
+  a.iedge = h->d_iedge;
+  a.MF = h->d_MF;
+  a.PD = h->d_PD;
+  a.WK = h->d_WK;
+  a.S = h->d_S;
+  a.M = h->d_M;
+  a.b = h->d_b;
+  a.its = h->d_its;
+  a.flg = h->d_flg;
+  a.status = h->d_status;
+  a.n_cells = h->n_cells;
+  a.ns = h->ns;
+  a.max_iter = h->prm.max_iter > 0 ? h->prm.max_iter : 20000;
+  a.rtol = h->prm.rtol > 0.0 ? h->prm.rtol : 1e-13;
+  const int nc = h->n_cells;
+  hipLaunchKernelGGL(rt3_local_kernel<L>, dim3(nc), dim3(256), 0, s, a);
+  NRT_HIP(hipGetLastError());
+  hipLaunchKernelGGL(rt3_diag_kernel<L>, dim3(nc), dim3(256), 0, s, a);
+  NRT_HIP(hipGetLastError());
+  if (ev) NRT_HIP(hipEventRecord((*ev)[1], s));
+  const int groups = (nc + 7) / 8;
+  hipLaunchKernelGGL(rt3_cg_kernel<L>, dim3(groups * 8 * 6), dim3(kItNT), 0, s, a);
+  NRT_HIP(hipGetLastError());
+  if (ev) NRT_HIP(hipEventRecord((*ev)[2], s));
+  hipLaunchKernelGGL(rt3_element_kernel<L>, dim3(nc * 7), dim3(256), 0, s, a);
+  NRT_HIP(hipGetLastError());
+  hipLaunchKernelGGL(rt3_status_kernel, dim3((nc + 255) / 256), dim3(256), 0, s, a);
+  NRT_HIP(hipGetLastError());
+  return MSFEC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+msfec_status msfec_rt3_create(const msfec_rt3_params* params, int n_cells, const double* vertices,
+                              int vertices_loc, msfec_rt3_handle* out) {
+  if (!params || !vertices || !out) return nrt_fail(MSFEC_ERR_INVALID, "null argument");
+  *out = nullptr;
+  if (n_cells <= 0) return nrt_fail(MSFEC_ERR_INVALID, "n_cells must be positive");
+  const int l = params->n_refine_local;
+  if (l < 1) return nrt_fail(MSFEC_ERR_INVALID, "n_refine_local must be at least 1");
+  if (l > 4) return nrt_fail(MSFEC_ERR_UNSUPPORTED, "n_refine_local above 4");
+  if (params->max_iter < 0 || !(params->rtol >= 0.0))
+    return nrt_fail(MSFEC_ERR_INVALID, "max_iter and rtol must not be negative");
+  if (vertices_loc != MSFEC_HOST && vertices_loc != MSFEC_DEVICE)
+    return nrt_fail(MSFEC_ERR_INVALID, "bad vertices location");
+  std::vector<double> hv((size_t)n_cells * 24);
+  if (vertices_loc == MSFEC_DEVICE) {
+    NRT_HIP(hipMemcpy(hv.data(), vertices, hv.size() * sizeof(double), hipMemcpyDeviceToHost));
+  } else {
+    std::copy(vertices, vertices + hv.size(), hv.begin());
+  }
+  for (int c = 0; c < n_cells; ++c) {
+    const double* v = &hv[(size_t)c * 24];
+    bool box = true;
+    for (int d = 0; d < 3; ++d) {
+      const double Hd = v[3 * (1 << d) + d] - v[d];
+      if (!(Hd > 0)) box = false;
+      for (int k = 0; k < 8 && box; ++k)
+        if (std::fabs(v[3 * k + d] - (v[d] + ((k >> d) & 1) * Hd)) > 1e-14 * Hd) box = false;
+    }
+    if (!box)
+      return nrt_fail(MSFEC_ERR_INVALID,
+                      "cell " + std::to_string(c) + " is not an axis-aligned box in deal.II vertex order");
+  }
+  msfec_rt3_batch* h = new msfec_rt3_batch();
+  h->prm = *params;
+  h->n_cells = n_cells;
+  h->l = l;
+  const NrtTables tab = nrt_tables(l);
+  h->ne = tab.ne;
+  h->nf = tab.N - tab.ne;
+  std::vector<int> iedge;
+  for (int e = 0; e < tab.ne; ++e)
+    if (tab.f2i[e] >= 0) iedge.push_back(e);
+  h->ns = (int)iedge.size();
+  const size_t nc = (size_t)n_cells, nfc = (size_t)1 << (3 * l);
+  msfec_status st = MSFEC_OK;
+#define RT3_TRY(x)        \
+  do {                    \
+    st = (x);             \
+    if (st != MSFEC_OK) { \
+      rt3_free(h);        \
+      delete h;           \
+      return st;          \
+    }                     \
+  } while (0)
+  RT3_TRY(nrt_alloc(&h->d_verts, nc * 24));
+  RT3_TRY(nrt_alloc(&h->d_ainv, nc * nfc * 8 * 6));
+  RT3_TRY(nrt_alloc(&h->d_f, nc * nfc * 8));
+  RT3_TRY(nrt_alloc(&h->d_iedge, iedge.size()));
+  RT3_TRY(nrt_alloc(&h->d_MF, nc * 21 * nfc));
+  RT3_TRY(nrt_alloc(&h->d_PD, nc * h->ne));
+  RT3_TRY(nrt_alloc(&h->d_WK, nc * 6 * 4 * h->ne));
+  RT3_TRY(nrt_alloc(&h->d_S, nc * 6 * h->nf));
+  RT3_TRY(nrt_alloc(&h->d_M, nc * 49));
+  RT3_TRY(nrt_alloc(&h->d_b, nc * 7));
+  RT3_TRY(nrt_alloc(&h->d_its, nc * 6));
+  RT3_TRY(nrt_alloc(&h->d_flg, nc * 6));
+  RT3_TRY(nrt_alloc(&h->d_status, nc));
+  RT3_TRY(nrt_in(h->stream, h->d_verts, hv.data(), hv.size() * sizeof(double), MSFEC_HOST));
+  RT3_TRY(nrt_in(h->stream, h->d_iedge, iedge.data(), iedge.size() * sizeof(int), MSFEC_HOST));
+#undef RT3_TRY
+  *out = h;
+  return MSFEC_OK;
+}
+
+msfec_status msfec_rt3_set_coefficients(msfec_rt3_handle h, const double* ainv, const double* f,
+                                        int loc) {
+  if (!h || !ainv) return nrt_fail(MSFEC_ERR_INVALID, "null argument");
+  if (loc != MSFEC_HOST && loc != MSFEC_DEVICE) return nrt_fail(MSFEC_ERR_INVALID, "bad location");
+  const size_t pts = (size_t)h->n_cells * ((size_t)1 << (3 * h->l)) * 8;
+  msfec_status st = nrt_in(h->stream, h->d_ainv, ainv, pts * 6 * sizeof(double), loc);
+  if (st == MSFEC_OK && f) st = nrt_in(h->stream, h->d_f, f, pts * sizeof(double), loc);
+  if (st != MSFEC_OK) return st;
+  h->has_f = f != nullptr;
+  h->coef_set = true;
+  return MSFEC_OK;
+}
+
+msfec_status msfec_rt3_compute(msfec_rt3_handle h, void* stream) {
+  if (!h) return nrt_fail(MSFEC_ERR_INVALID, "null handle");
+  if (!h->coef_set) return nrt_fail(MSFEC_ERR_STATE, "coefficients not set");
+  hipStream_t s = (hipStream_t)stream;
+  h->stream = s;
+  std::array<hipEvent_t, 4>* ev = nullptr;
+  if (h->prm.flags & MSFEC_FLAG_TIMING) {
+    if (h->ev_used == h->ev_sets.size()) {
+      std::array<hipEvent_t, 4> set{};
+      for (auto& e : set) NRT_HIP(hipEventCreate(&e));
+      h->ev_sets.push_back(set);
+    }
+    ev = &h->ev_sets[h->ev_used++];
+    NRT_HIP(hipEventRecord((*ev)[0], s));
+  }
+  msfec_status st;
+  switch (h->l) {
+    case 1: st = rt3_launch<1>(h, s, ev); break;
+    case 2: st = rt3_launch<2>(h, s, ev); break;
+    case 3: st = rt3_launch<3>(h, s, ev); break;
+    default: st = rt3_launch<4>(h, s, ev); break;
+  }
+  if (st != MSFEC_OK) return st;
+  if (ev) NRT_HIP(hipEventRecord((*ev)[3], s));
+  h->computed = true;
+  return MSFEC_OK;
+}
+
+msfec_status msfec_rt3_synchronize(msfec_rt3_handle h) {
+  if (!h) return nrt_fail(MSFEC_ERR_INVALID, "null handle");
+  NRT_HIP(hipStreamSynchronize(h->stream));
+  return MSFEC_OK;
+}
+
+msfec_status msfec_rt3_get_element(msfec_rt3_handle h, double* M, double* b, int loc) {
+  if (!h) return nrt_fail(MSFEC_ERR_INVALID, "null handle");
+  if (!h->computed) return nrt_fail(MSFEC_ERR_STATE, "compute not called");
+  msfec_status st = MSFEC_OK;
+  if (M) st = nrt_out(h->stream, M, h->d_M, (size_t)h->n_cells * 49, loc);
+  if (st == MSFEC_OK && b) st = nrt_out(h->stream, b, h->d_b, (size_t)h->n_cells * 7, loc);
+  return st;
+}
+
+msfec_status msfec_rt3_get_basis(msfec_rt3_handle h, double* sigma, int loc) {
+  if (!h) return nrt_fail(MSFEC_ERR_INVALID, "null handle");
+  if (!h->computed) return nrt_fail(MSFEC_ERR_STATE, "compute not called");
+  return nrt_out(h->stream, sigma, h->d_S, (size_t)h->n_cells * 6 * h->nf, loc);
+}
+
+msfec_status msfec_rt3_get_cell_status(msfec_rt3_handle h, int* status) {
+  if (!h || !status) return nrt_fail(MSFEC_ERR_INVALID, "null argument");
+  if (!h->computed) return nrt_fail(MSFEC_ERR_STATE, "compute not called");
+  NRT_HIP(hipStreamSynchronize(h->stream));
+  NRT_HIP(hipMemcpy(status, h->d_status, (size_t)h->n_cells * sizeof(int), hipMemcpyDeviceToHost));
+  return MSFEC_OK;
+}
+
+msfec_status msfec_rt3_get_iterations(msfec_rt3_handle h, int* iterations) {
+  if (!h || !iterations) return nrt_fail(MSFEC_ERR_INVALID, "null argument");
+  if (!h->computed) return nrt_fail(MSFEC_ERR_STATE, "compute not called");
+  NRT_HIP(hipStreamSynchronize(h->stream));
+  NRT_HIP(hipMemcpy(iterations, h->d_its, (size_t)h->n_cells * 6 * sizeof(int), hipMemcpyDeviceToHost));
+  return MSFEC_OK;
+}
+
+msfec_status msfec_rt3_kernel_times(msfec_rt3_handle h, float* ms, int max, int* n) {
+  if (!h || !ms) return nrt_fail(MSFEC_ERR_INVALID, "null argument");
+  if (h->ev_used == 0) return nrt_fail(MSFEC_ERR_STATE, "no timed compute (set MSFEC_FLAG_TIMING)");
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  for (size_t c = 0; c < h->ev_used; ++c) {
+    const auto& e = h->ev_sets[c];
+    NRT_HIP(hipEventSynchronize(e[3]));
+    float t[4] = {0.f, 0.f, 0.f, 0.f};
+    NRT_HIP(hipEventElapsedTime(&t[0], e[0], e[3]));
+    NRT_HIP(hipEventElapsedTime(&t[1], e[0], e[1]));
+    NRT_HIP(hipEventElapsedTime(&t[2], e[1], e[2]));
+    NRT_HIP(hipEventElapsedTime(&t[3], e[2], e[3]));
+    for (int k = 0; k < 4; ++k) acc[k] += t[k];
+  }
+  const int cnt = std::min(max, 4);
+  for (int k = 0; k < cnt; ++k) ms[k] = (float)(acc[k] / (double)h->ev_used);
+  if (n) *n = std::max(cnt, 0);
+  h->ev_used = 0;
+  return MSFEC_OK;
+}
+
+msfec_status msfec_rt3_destroy(msfec_rt3_handle h) {
+  if (!h) return MSFEC_OK;
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  rt3_free(h);
   delete h;
   return MSFEC_OK;
 }

@@ -40,6 +40,10 @@ EXPORTED = (
     "msfec_nrt_synchronize", "msfec_nrt_get_element", "msfec_nrt_get_basis",
     "msfec_nrt_reconstruct", "msfec_nrt_get_cell_status", "msfec_nrt_get_iterations",
     "msfec_nrt_kernel_times", "msfec_nrt_destroy",
+    "msfec_rt3_create", "msfec_rt3_set_coefficients", "msfec_rt3_compute",
+    "msfec_rt3_synchronize", "msfec_rt3_get_element", "msfec_rt3_get_basis",
+    "msfec_rt3_get_cell_status", "msfec_rt3_get_iterations", "msfec_rt3_kernel_times",
+    "msfec_rt3_destroy",
     "msfec_qu_create", "msfec_qu_offsets", "msfec_qu_set_coefficient", "msfec_qu_set_rhs",
     "msfec_qu_set_reaction", "msfec_qu_tune", "msfec_qu_level_info", "msfec_qu_level_handle",
     "msfec_qu_compute", "msfec_qu_synchronize", "msfec_qu_get_element", "msfec_qu_get_basis",
@@ -81,6 +85,15 @@ class RtParams(ctypes.Structure):
         ("n_refine_local", ctypes.c_int),
         ("flags", ctypes.c_int),
         ("coef_mode", ctypes.c_int),
+    ]
+
+
+class Rt3Params(ctypes.Structure):
+    _fields_ = [
+        ("n_refine_local", ctypes.c_int),
+        ("flags", ctypes.c_int),
+        ("max_iter", ctypes.c_int),
+        ("rtol", ctypes.c_double),
     ]
 
 
@@ -184,6 +197,17 @@ def load(path: str | None = None):
     L.msfec_nrt_reconstruct.argtypes = [vp, vp, ctypes.c_int, vp, vp, ctypes.c_int]
     L.msfec_nrt_get_cell_status.argtypes = [vp, ip]
     L.msfec_nrt_get_iterations.argtypes = [vp, ip]
+    L.msfec_rt3_create.argtypes = [ctypes.POINTER(Rt3Params), ctypes.c_int, vp, ctypes.c_int,
+                                   ctypes.POINTER(vp)]
+    L.msfec_rt3_set_coefficients.argtypes = [vp, vp, vp, ctypes.c_int]
+    L.msfec_rt3_compute.argtypes = [vp, vp]
+    L.msfec_rt3_synchronize.argtypes = [vp]
+    L.msfec_rt3_get_element.argtypes = [vp, vp, vp, ctypes.c_int]
+    L.msfec_rt3_get_basis.argtypes = [vp, vp, ctypes.c_int]
+    L.msfec_rt3_get_cell_status.argtypes = [vp, ip]
+    L.msfec_rt3_get_iterations.argtypes = [vp, ip]
+    L.msfec_rt3_kernel_times.argtypes = [vp, ctypes.POINTER(ctypes.c_float), ctypes.c_int, ip]
+    L.msfec_rt3_destroy.argtypes = [vp]
     L.msfec_nrt_kernel_times.argtypes = [vp, ctypes.POINTER(ctypes.c_float), ctypes.c_int, ip]
     L.msfec_nrt_destroy.argtypes = [vp]
     sp = ctypes.POINTER(ctypes.c_size_t)
