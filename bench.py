@@ -549,6 +549,89 @@ def nrt4_config(steps: int, warmup: int, check: bool, cpu: bool):
     return out
 
 
+RT3_NREF, RT3_NC = 4, 4
+
+
+def rt3_cg_bytes_per_iteration(nfc: int, ne: int, ns: int) -> float:
+    """Algorithmic HBM bytes of one Jacobi-PCG iteration of one 3-D RT_DQ
+    basis: the 21 face-mass entries of every fine cell, p read and q read and
+    written once per edge in the element-by-element product, and per
+    interior edge its index (4 B) plus x, r read and written, p, q, the
+    Jacobi diagonal read in the update (60 B) and the index, diagonal, r, p
+    read, p and q written in the direction update (44 B)."""
+    return 8.0 * 21 * nfc + 24.0 * ne + 104.0 * ns
+
+
+def rt3_config(steps: int, warmup: int, check: bool, cpu: bool):
+    """The reference's own 3-D RT_DQ example, prm_rt_dq_test-01 (global
+    refinements 2: 4^3 = 64 coarse cells of the unit cube; local refinements
+    4: 16^3 fine cells, 13 056 faces + 4096 cell values per basis), its
+    DiffusionInverse_A and right-hand side at the Gauss points, 6 face bases."""
+    from msfec_amd import RTDQ3BasisBatch, fields as F
+    verts = F.box_grid_3d(RT3_NC)
+    pts = F.gauss_points_3d(verts, RT3_NREF)
+    ainv = F.diffusion_inverse_A_tensor(pts)
+    fq = F.rt_dq_rhs_3d(pts)
+    n_cells = verts.shape[0]
+    rb = RTDQ3BasisBatch(verts, n_refine_local=RT3_NREF, timing=True)
+    rb.set_coefficients(ainv, fq)
+    for _ in range(warmup):
+        rb.run()
+    rb.synchronize()
+    if warmup > 0:
+        rb.kernel_times()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        rb.run()
+    rb.synchronize()
+    wall = time.perf_counter() - t0
+    kt = rb.kernel_times()
+    its = rb.get_iterations()
+    m = 1 << RT3_NREF
+    ne, ns = 3 * m * (m + 1) ** 2, 3 * m * (m - 1) ** 2
+    bpi = rt3_cg_bytes_per_iteration(m ** 3, ne, ns)
+    gbs = float(its.sum()) * bpi / (kt[2] * 1e-3) / 1e9
+    out = {"workload": "3D Raviart-Thomas x DQ0 basis, the reference's prm_rt_dq_test-01: "
+                       "4^3=64 coarse cells x 16^3 fine cells (13056 faces + 4096 cell values), "
+                       "6 face bases, its coefficients and right-hand side at the Gauss points; "
+                       "the flux is the coarse function plus the curl of an interior Nedelec "
+                       f"field ({ns} interior edges), curl-curl system by Jacobi-PCG "
+                       "(relative tolerance 1e-13), one workgroup per basis",
+           "cells_per_s": round(n_cells / (wall / steps), 2),
+           "ms_per_step": round(wall * 1e3 / steps, 3),
+           "kernel_ms": {"total": round(kt[0], 3), "local_and_diagonal": round(kt[1], 3),
+                         "cg": round(kt[2], 3), "element": round(kt[3], 3)},
+           "cg_iterations": {"mean": round(float(its.mean()), 1), "min": int(its.min()),
+                             "max": int(its.max())},
+           "failed_cells": int((rb.get_cell_status() != 0).sum()),
+           "roofline": {"bound": "hbm", "kernel": "rt3_cg_kernel<4>",
+                        "bytes_per_iteration_basis": bpi, "achieved_gbs": round(gbs, 1),
+                        "peak_gbs": HBM_PEAK_GBS, "frac": round(gbs / HBM_PEAK_GBS, 4)}}
+    if check or cpu:
+        from oracle import oracle as O
+        O.build()
+        c = 21
+        t0 = time.perf_counter()
+        Mo, bo, So = O.rt3_basis_cell_sparse(RT3_NREF, verts[c], ainv[c], fq[c])
+        t_cpu = time.perf_counter() - t0
+        if check:
+            M, b, S = rb.get_global_element_matrix(), rb.get_global_element_rhs(), rb.get_basis()
+            out["rel_err_vs_fp64_oracle"] = {
+                "M": rel_err(M[c:c + 1], Mo[None]), "b": rel_err(b[c:c + 1], bo[None]),
+                "Sigma": rel_err(S[c:c + 1], So[None]), "cells_checked": 1,
+                "oracle": "sparse direct solve of the saddle-point system (SuperLU)"}
+        if cpu:
+            out["cpu_baseline"] = {"value": round(1.0 / t_cpu, 4), "unit": "cells/s", "cores": 1,
+                                   "kind": "port",
+                                   "sample": "1 of the 64 cells (cell 21): triplet assembly of "
+                                             "the mixed system, sparse LU (SuperLU, scipy) of "
+                                             "the saddle-point system with the 6 right-hand "
+                                             "sides (the UMFPACK direct solve restated), element "
+                                             f"matrix; one thread, wall {t_cpu:.2f} s"}
+    rb.close()
+    return out
+
+
 def cg_flops_per_iteration(dim: int, level: int) -> float:
     """Algorithmic FP64 flops of one Jacobi-PCG iteration of one basis on one
     cell: the matrix-free 3^dim-point product over the interior rows (a
@@ -847,7 +930,7 @@ def main():
     ap.add_argument("--no-ned", action="store_true",
                     help="skip the 2-D Q1 x Nedelec configuration timed beside the headline")
     ap.add_argument("--no-rt", action="store_true",
-                    help="skip the 2-D Raviart-Thomas x DQ0 configuration timed beside the headline")
+                    help="skip the 2-D and 3-D Raviart-Thomas x DQ0 configurations timed beside the headline")
     ap.add_argument("--no-nrt", action="store_true",
                     help="skip the 3-D Nedelec x Raviart-Thomas configuration timed beside the "
                          "headline")
@@ -1036,6 +1119,10 @@ def main():
             not args.no_cpu_baseline)
     if not args.no_nrt:
         others_cfg["ned_rt_3d_4x4x4_l4"] = nrt4_config(
+            max(2, args.steps // 4), min(args.warmup, 1), not args.no_check,
+            not args.no_cpu_baseline)
+    if not args.no_rt:
+        others_cfg["rt_dq_3d_4x4x4_l4"] = rt3_config(
             max(2, args.steps // 4), min(args.warmup, 1), not args.no_check,
             not args.no_cpu_baseline)
     if not args.no_uneven:

@@ -150,6 +150,12 @@ def diffusion_B(points: np.ndarray, freq=14, scale=1.0, alpha=0.9) -> np.ndarray
     return scale * (1.0 - alpha * np.sin(2 * np.pi * freq * points[..., 0]))
 
 
+def rt_dq_rhs_3d(points: np.ndarray) -> np.ndarray:
+    """The right-hand side of example_parameters/prm_rt_dq_test-01.prm,
+    sin(2 pi x) sin(2 pi y) sin(2 pi z), per point."""
+    return np.prod(np.sin(2 * np.pi * points), axis=-1)
+
+
 def ned_rt_rhs(points: np.ndarray, scale=100.0) -> np.ndarray:
     """The right-hand side of prm_ned_rt_test-01.prm: scale times the gradient
     of x y z (x-1) (y-1) (z-1)."""

@@ -623,6 +623,38 @@ int oq_rt3_assemble(int l, const double *verts, const double *ainv, const double
   return 0;
 }
 
+/* The assembled 3-D RT_DQ system as 49 triplets per fine cell (row = test
+ * dof, duplicates summed by the caller) and the load, for a sparse solve at
+ * refinements where the dense system does not fit. */
+int oq_rt3_assemble_coo(int l, const double *verts, const double *ainv, const double *fq,
+                        int *rows, int *cols, double *vals, double *F) {
+  nrt_mesh g;
+  if (nrt_mesh_init(l, 4, verts, &g)) return 1;
+  const int m = g.m, nf = g.nf;
+  memset(F, 0, sizeof(double) * (size_t)(nf + m * m * m));
+  size_t k = 0;
+  for (int c3 = 0; c3 < m; ++c3)
+    for (int b = 0; b < m; ++b)
+      for (int a = 0; a < m; ++a) {
+        const int c = (c3 * m + b) * m + a;
+        double K[7][7], Fl[7];
+        int d18[18], d[7];
+        rt3_local(&g, c, ainv, fq, K, Fl);
+        nrt_cell_dofs(&g, a, b, c3, d18);
+        for (int i = 0; i < 6; ++i) d[i] = d18[12 + i] - g.ne;
+        d[6] = nf + c;
+        for (int i = 0; i < 7; ++i) {
+          for (int j = 0; j < 7; ++j, ++k) {
+            rows[k] = d[i];
+            cols[k] = d[j];
+            vals[k] = K[i][j];
+          }
+          F[d[i]] += Fl[i];
+        }
+      }
+  return 0;
+}
+
 int oq_rt3_basis_cell(int l, const double *verts, const double *ainv, const double *fq,
                       double *M, double *b, double *Sigma) {
   nrt_mesh g;

@@ -86,6 +86,8 @@ int oq_nrt_boundary(int l, const double *verts, double *G, int *on_boundary);
  * (6 x nf, the fine fluxes of the face bases; their u is 1).  l in 1..3. */
 int oq_rt3_assemble(int l, const double *verts, const double *ainv, const double *fq, double *A,
                     double *F);
+int oq_rt3_assemble_coo(int l, const double *verts, const double *ainv, const double *fq,
+                        int *rows, int *cols, double *vals, double *F);
 int oq_rt3_basis_cell(int l, const double *verts, const double *ainv, const double *fq,
                       double *M, double *b, double *Sigma);
 

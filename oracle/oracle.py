@@ -112,6 +112,8 @@ def lib():
         L.oq_rt3_assemble.restype = ctypes.c_int
         L.oq_rt3_basis_cell.argtypes = [ctypes.c_int, dp, dp, dp, dp, dp, dp]
         L.oq_rt3_basis_cell.restype = ctypes.c_int
+        L.oq_rt3_assemble_coo.argtypes = [ctypes.c_int, dp, dp, dp, ip, ip, dp, dp]
+        L.oq_rt3_assemble_coo.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -574,6 +576,67 @@ def rt3_assemble(l, verts, ainv, fq=None):
     F = np.zeros(N)
     assert lib().oq_rt3_assemble(l, _dp(v), _dp(ainv), _dp(fq), _dp(A), _dp(F)) == 0
     return A, F
+
+
+def rt3_assemble_sparse(l, verts, ainv, fq=None):
+    """The assembled 3-D RT_DQ system (fine faces, then fine cells) as a scipy
+    CSR matrix (row = test) and the load; l up to 4."""
+    import scipy.sparse as sp
+    nc = (1 << l) ** 3
+    ainv = np.ascontiguousarray(np.broadcast_to(np.asarray(ainv, dtype=np.float64), (nc, 8, 6)))
+    fq = None if fq is None else np.ascontiguousarray(
+        np.broadcast_to(np.asarray(fq, dtype=np.float64), (nc, 8)))
+    v = np.ascontiguousarray(verts, dtype=np.float64).reshape(8, 3)
+    N = rt3_sizes(l)[3]
+    rows = np.empty(nc * 49, dtype=np.int32)
+    cols = np.empty(nc * 49, dtype=np.int32)
+    vals = np.empty(nc * 49)
+    F = np.zeros(N)
+    ip = ctypes.POINTER(ctypes.c_int)
+    assert lib().oq_rt3_assemble_coo(l, _dp(v), _dp(ainv), _dp(fq), rows.ctypes.data_as(ip),
+                                     cols.ctypes.data_as(ip), _dp(vals), _dp(F)) == 0
+    return sp.coo_matrix((vals, (rows, cols)), shape=(N, N)).tocsr(), F
+
+
+def rt3_basis_cell_sparse(l, verts, ainv, fq=None):
+    """oq_rt3_basis_cell by a sparse direct solve (SuperLU) with two steps of
+    iterative refinement, for l up to 4: the same boundary fluxes (the coarse
+    traces), u pinned in the last fine cell, the loads -+|c|/|K| on the cell
+    rows (rt_dq_basis.cc:516-538), and M, b by the block products with u = 1
+    (:822-914).  Returns (M 7x7, b 7, Sigma 6 x nf)."""
+    import scipy.sparse.linalg as spla
+    A, F = rt3_assemble_sparse(l, verts, ainv, fq)
+    m, nf, nc, N = rt3_sizes(l)
+    G, bnd = nrt_boundary(l, verts)
+    ne = nrt_sizes(l)[2]
+    Gf, fb = G[12:, ne:], bnd[ne:]
+    fix = np.zeros((6, N))
+    fix[:, :nf] = np.where(fb, Gf, 0.0)
+    free = np.ones(N, dtype=bool)
+    free[:nf] = ~fb
+    free[N - 1] = False
+    I = np.flatnonzero(free)
+    rhs = -(A[I] @ fix.T)
+    sgn = np.array([-1.0, 1, -1, 1, -1, 1])
+    rhs[I >= nf] += sgn / m ** 3
+    AII = A[I][:, I].tocsc()
+    lu = spla.splu(AII)
+    x = lu.solve(rhs)
+    for _ in range(2):
+        x += lu.solve(rhs - AII @ x)
+    X = fix.copy()
+    X[:, I] = x.T
+    S = X[:, :nf]
+    Aff, Afc, Acf = A[:nf, :nf], A[:nf, nf:], A[nf:, :nf]
+    one = np.ones(nc)
+    M = np.zeros((7, 7))
+    M[:6, :6] = S @ (Aff @ S.T)
+    M[:6, 6] = S @ (Afc @ one)
+    M[6, :6] = one @ (Acf @ S.T)
+    M[6, 6] = one @ (A[nf:, nf:] @ one)
+    b = np.zeros(7)
+    b[6] = F[nf:].sum()
+    return M, b, S
 
 
 def box_vertices_3d(x0, x1):

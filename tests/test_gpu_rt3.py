@@ -9,8 +9,9 @@ an interior Nedelec field and solves the curl-curl system by Jacobi-PCG to
 each basis's fluxes and of M.  At l = 4 (the reference examples' local
 refinement), beyond the oracle's dense LU, the fluxes are checked through
 the balance of every fine cell, the boundary traces, the symmetric positive
-flux block and the complementary energy bound, and against the closed form
-for constant coefficients.
+flux block and the complementary energy bound, against the oracle's sparse
+direct solve of the saddle-point system, and against the closed form for
+constant coefficients.
 """
 import numpy as np
 import pytest
@@ -100,8 +101,9 @@ def test_rt3_l4_reference_coefficients_invariants(oracle_mod):
     """Two cells of the reference example's 4^3 mesh at l = 4 with its
     rotated DiffusionInverse_A: every fine cell's net outflow is -+|c|/|K|,
     the boundary fluxes are the coarse traces, the flux block is symmetric
-    positive definite and each basis's energy lies below that of the coarse
-    function (the complementary energy principle)."""
+    positive definite, each basis's energy lies below that of the coarse
+    function (the complementary energy principle), and the fluxes and M equal
+    the oracle's sparse direct solve."""
     print("case rt3 reference coefficients l=4", flush=True)
     from msfec_amd import fields as F
     import test_nrt_oracle as T
@@ -131,7 +133,11 @@ def test_rt3_l4_reference_coefficients_invariants(oracle_mod):
         E = coarse_energy(verts[c], l, ainv[c])
         assert (np.diag(Mf) <= E * (1 + 1e-12)).all(), (np.diag(Mf), E)
         assert (np.diag(Mf) < E).any()
-    print("flux block diagonal", np.diag(M[1, :6, :6]), flush=True)
+        # and the oracle's sparse direct solve of the saddle-point system
+        Ms, bs, Ss = oracle_mod.rt3_basis_cell_sparse(l, verts[c], ainv[c])
+        es = max(rel(S[c, i], Ss[i]) for i in range(6))
+        print(f"cell {c} vs sparse direct: sigma {es:.2e} M {rel(M[c], Ms):.2e}", flush=True)
+        assert es <= TOL and rel(M[c], Ms) <= TOL
 
 
 def test_rt3_iteration_limit_flags_the_cell():

@@ -92,3 +92,31 @@ def test_rejects_bad_cells_and_levels():
     v[5, 1] += 0.02
     assert O.rt3_basis_cell(1, v, [1, 0, 0, 1, 0, 1])[3] == 1
     assert O.rt3_basis_cell(4, VERTS, [1, 0, 0, 1, 0, 1])[3] == 1
+
+
+@pytest.mark.parametrize("l", [1, 2, 3])
+def test_sparse_solve_matches_dense(l):
+    """rt3_basis_cell_sparse (SuperLU on the triplet assembly) against the
+    dense LU of oq_rt3_basis_cell."""
+    ainv, _, _, _ = rand_coefs(l, 120 + l, reaction=False)
+    f = np.random.default_rng(6).normal(size=((1 << l) ** 3, 8))
+    M, b, S, st = O.rt3_basis_cell(l, VERTS, ainv, f)
+    Ms, bs, Ss = O.rt3_basis_cell_sparse(l, VERTS, ainv, f)
+    assert st == 0
+    np.testing.assert_allclose(Ms, M, rtol=0, atol=1e-13 * np.abs(M).max())
+    np.testing.assert_allclose(Ss, S, rtol=0, atol=1e-13 * np.abs(S).max())
+    np.testing.assert_allclose(bs, b, rtol=1e-13, atol=1e-15)
+    A, F = O.rt3_assemble(l, VERTS, ainv, f)
+    As, Fs = O.rt3_assemble_sparse(l, VERTS, ainv, f)
+    np.testing.assert_allclose(As.toarray(), A, rtol=0, atol=1e-14 * np.abs(A).max())
+    np.testing.assert_allclose(Fs, F, rtol=0, atol=1e-15)
+
+
+def test_sparse_solve_l4_constant_coefficients():
+    """At l = 4 (beyond the dense LU) the sparse solve reproduces the closed
+    forms for A^-1 = a I."""
+    a = 0.9
+    M, b, S = O.rt3_basis_cell_sparse(4, VERTS, [a, 0, 0, a, 0, a], 1.0)
+    np.testing.assert_allclose(M[:6, :6], flux_block_closed_form(a), rtol=0, atol=1e-12)
+    np.testing.assert_allclose(S, coarse_face_values(Dofs(4)), rtol=0, atol=1e-12)
+    np.testing.assert_allclose(b[6], np.prod(H), rtol=1e-12)
