@@ -291,3 +291,35 @@ def test_nrt_rejects_non_box_and_bad_level(lib):
     L = lib.load()
     assert L.msfec_nrt_create(None, 1, None, 0, None) == lib.ERR_INVALID
     assert L.msfec_nrt_destroy(None) == lib.OK
+
+
+def _rt3_create(lib, l=2, verts=None, max_iter=0, rtol=0.0, n=None):
+    L = lib.load()
+    p = lib.Rt3Params()
+    p.n_refine_local, p.flags, p.max_iter, p.rtol = l, 0, max_iter, rtol
+    if verts is None:
+        verts = box_vertices(3, [0.0] * 3, [1.0] * 3)[None]
+    verts = np.ascontiguousarray(verts, dtype=np.float64)
+    h = ctypes.c_void_p()
+    st = L.msfec_rt3_create(ctypes.byref(p), verts.shape[0] if n is None else n,
+                            ctypes.c_void_p(verts.ctypes.data), lib.HOST, ctypes.byref(h))
+    return st, L.msfec_last_error().decode()
+
+
+def test_rt3_rejects_non_box_bad_level_and_arguments(lib):
+    """The 3-D RT_DQ batch checks its arguments before any device call."""
+    v = box_vertices(3, [0, 0, 0], [1, 1, 1])
+    v[3, 0] += 0.05
+    st, msg = _rt3_create(lib, verts=v[None])
+    assert st == lib.ERR_INVALID and "axis-aligned" in msg
+    assert _rt3_create(lib, l=0)[0] == lib.ERR_INVALID
+    st, msg = _rt3_create(lib, l=5)
+    assert st == lib.ERR_UNSUPPORTED and "above 4" in msg
+    assert _rt3_create(lib, max_iter=-1)[0] == lib.ERR_INVALID
+    assert _rt3_create(lib, rtol=float("nan"))[0] == lib.ERR_INVALID
+    assert _rt3_create(lib, n=0)[0] == lib.ERR_INVALID
+    L = lib.load()
+    assert L.msfec_rt3_create(None, 1, None, 0, None) == lib.ERR_INVALID
+    assert L.msfec_rt3_destroy(None) == lib.OK
+    assert L.msfec_rt3_compute(None, None) == lib.ERR_INVALID
+    assert L.msfec_rt3_get_element(None, None, None, 0) == lib.ERR_INVALID
