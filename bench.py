@@ -552,14 +552,14 @@ def nrt4_config(steps: int, warmup: int, check: bool, cpu: bool):
 RT3_NREF, RT3_NC = 4, 4
 
 
-def rt3_cg_bytes_per_iteration(nfc: int, ne: int, ns: int) -> float:
+def rt3_cg_bytes_per_iteration(nfc: int, ns: int) -> float:
     """Algorithmic HBM bytes of one Jacobi-PCG iteration of one 3-D RT_DQ
-    basis: the 21 face-mass entries of every fine cell, p read and q read and
-    written once per edge in the element-by-element product, and per
-    interior edge its index (4 B) plus x, r read and written, p, q, the
-    Jacobi diagonal read in the update (60 B) and the index, diagonal, r, p
-    read, p and q written in the direction update (44 B)."""
-    return 8.0 * 21 * nfc + 24.0 * ne + 104.0 * ns
+    basis (vectors over the ns interior edges): the 21 face-mass entries of
+    every fine cell, p read and q read and written in the element-by-element
+    product, then per edge x and r read and written, p, q and the Jacobi
+    diagonal read in the update (56 B), and the diagonal, r, p read, p and q
+    written in the direction update (40 B)."""
+    return 8.0 * 21 * nfc + 24.0 * ns + 96.0 * ns
 
 
 def rt3_config(steps: int, warmup: int, check: bool, cpu: bool):
@@ -588,8 +588,8 @@ def rt3_config(steps: int, warmup: int, check: bool, cpu: bool):
     kt = rb.kernel_times()
     its = rb.get_iterations()
     m = 1 << RT3_NREF
-    ne, ns = 3 * m * (m + 1) ** 2, 3 * m * (m - 1) ** 2
-    bpi = rt3_cg_bytes_per_iteration(m ** 3, ne, ns)
+    ns = 3 * m * (m - 1) ** 2
+    bpi = rt3_cg_bytes_per_iteration(m ** 3, ns)
     gbs = float(its.sum()) * bpi / (kt[2] * 1e-3) / 1e9
     out = {"workload": "3D Raviart-Thomas x DQ0 basis, the reference's prm_rt_dq_test-01: "
                        "4^3=64 coarse cells x 16^3 fine cells (13056 faces + 4096 cell values), "

@@ -1906,10 +1906,9 @@ struct Rt3Args {
   const double* verts;  // n_cells x 8 x 3
   const double* ainv;   // n_cells x nfc x 8 x 6
   const double* fq;     // n_cells x nfc x 8, or null
-  const int* iedge;     // ns: the interior edges
   double* MF;           // n_cells x 21 x nfc: face mass with A^-1 per fine cell (colour-major)
-  double* PD;           // n_cells x ne: inverse Jacobi diagonal (0 on the boundary)
-  double* WK;           // n_cells x 6 x 4 x ne: x, r, p, q
+  double* PD;           // n_cells x ns: inverse Jacobi diagonal
+  double* WK;           // n_cells x 6 x 4 x ns: x, r, p, q over the interior edges
   double* S;            // n_cells x 6 x nf: the fine fluxes
   double* M;            // n_cells x 7 x 7
   double* b;            // n_cells x 7
@@ -1950,6 +1949,26 @@ __device__ __forceinline__ int rt3_face(int c0, int c1, int c2, int F) {
   return it_cdof<M>(c0, c1, c2, 12 + F) - ne;
 }
 
+// Index among the interior edges (x-, y-, then z-edges, each in the full
+// numbering's order) of the edge along axis t starting at node (i, j, k),
+// or -1 on the coarse boundary.
+template <int M>
+__device__ __forceinline__ int rt3_iidx(int t, int i, int j, int k) {
+  constexpr int S1 = M * (M - 1) * (M - 1);
+  auto in = [](int q) { return q >= 1 && q <= M - 1; };
+  if (t == 0) return in(j) && in(k) ? ((k - 1) * (M - 1) + (j - 1)) * M + i : -1;
+  if (t == 1) return in(i) && in(k) ? S1 + ((k - 1) * M + j) * (M - 1) + (i - 1) : -1;
+  return in(i) && in(j) ? 2 * S1 + (k * (M - 1) + (j - 1)) * (M - 1) + (i - 1) : -1;
+}
+// ... of local line Ln of fine cell (c0, c1, c2)
+template <int M>
+__device__ __forceinline__ int rt3_line_iidx(int c0, int c1, int c2, int Ln) {
+  const int ax = kLAx[Ln], f1 = kLF1[Ln], f2 = kLF2[Ln];
+  if (ax == 0) return rt3_iidx<M>(0, c0, c1 + f1, c2 + f2);
+  if (ax == 1) return rt3_iidx<M>(1, c0 + f1, c1, c2 + f2);
+  return rt3_iidx<M>(2, c0 + f1, c1 + f2, c2);
+}
+
 // The face mass with A^-1 at the 8 Gauss points of every fine cell
 // ((psi_F, A^-1 psi_G): psi_F has only its axis component), 21 packed
 // entries, entry-major over colour-major cells.
@@ -1983,14 +2002,12 @@ __global__ __launch_bounds__(256) void rt3_diag_kernel(Rt3Args g) {
   constexpr int M = 1 << L, nfc = M * M * M, ne = 3 * M * (M + 1) * (M + 1);
   const int cell = blockIdx.x;
   const double* mf = g.MF + (size_t)cell * 21 * nfc;
-  double* pd = g.PD + (size_t)cell * ne;
+  double* pd = g.PD + (size_t)cell * g.ns;
   for (int e = threadIdx.x; e < ne; e += 256) {
     int t, p0, p1, p2;
     dof_pos<M>(e, t, p0, p1, p2);
-    if (it_on_boundary<M>(t, p0, p1, p2)) {
-      pd[e] = 0.0;
-      continue;
-    }
+    const int ie = rt3_iidx<M>(t, p0, p1, p2);
+    if (ie < 0) continue;
     double d = 0.0;
     for (int k = 0; k < 4; ++k) {
       int c0, c1, c2, a;
@@ -2005,7 +2022,7 @@ __global__ __launch_bounds__(256) void rt3_diag_kernel(Rt3Args g) {
           }
         }
     }
-    pd[e] = d > 0.0 ? 1.0 / d : 0.0;
+    pd[ie] = d > 0.0 ? 1.0 / d : 0.0;
   }
 }
 
@@ -2021,7 +2038,10 @@ __device__ __forceinline__ double rt3_cell_apply(int c0, int c1, int c2, const d
   if (nb < 0) {
     double pv[12];
 #pragma unroll
-    for (int Ln = 0; Ln < 12; ++Ln) pv[Ln] = p[it_cdof<M>(c0, c1, c2, Ln)];
+    for (int Ln = 0; Ln < 12; ++Ln) {
+      const int ie = rt3_line_iidx<M>(c0, c1, c2, Ln);
+      pv[Ln] = ie >= 0 ? p[ie] : 0.0;
+    }
 #pragma unroll
     for (int F = 0; F < 6; ++F) {
       double s = 0.0;
@@ -2051,14 +2071,15 @@ __device__ __forceinline__ double rt3_cell_apply(int c0, int c1, int c2, const d
   }
 #pragma unroll
   for (int Ln = 0; Ln < 12; ++Ln) {
-    if (it_local_on_boundary<M>(c0, c1, c2, Ln)) continue;
+    const int ie = rt3_line_iidx<M>(c0, c1, c2, Ln);
+    if (ie < 0) continue;
     double s = 0.0;
 #pragma unroll
     for (int F = 0; F < 6; ++F) {
       const int sgn = rt3_inc(F, Ln);
       if (sgn) s += sgn * z[F];
     }
-    y[it_cdof<M>(c0, c1, c2, Ln)] += sg * s;
+    y[ie] += sg * s;
   }
   return quad;
 }
@@ -2069,23 +2090,23 @@ __device__ __forceinline__ double rt3_cell_apply(int c0, int c1, int c2, const d
 // sigma = c_n + D psi on every face.
 template <int L>
 __global__ __launch_bounds__(kItNT) void rt3_cg_kernel(Rt3Args g) {
-  constexpr int M = 1 << L, nfc = M * M * M, nx = M * (M + 1) * (M + 1), ne = 3 * nx;
+  constexpr int M = 1 << L, nfc = M * M * M;
   constexpr int nf1 = M * M * (M + 1), nf = 3 * nf1, H = M / 2;
-  __shared__ double redA[kItNW], redB[kItNW];
+  constexpr int ns = 3 * M * (M - 1) * (M - 1);
+  __shared__ double redA[kItNW], redB[2 * kItNW];
   const int bid = blockIdx.x, slot = bid >> 3;
   const int cell = (slot / 6) * 8 + (bid & 7), nb = slot % 6;
   if (cell >= g.n_cells) return;
-  const int tid = threadIdx.x, ns = g.ns;
+  const int tid = threadIdx.x;
   const double* mf = g.MF + (size_t)cell * 21 * nfc;
-  const double* pd = g.PD + (size_t)cell * ne;
-  double* x = g.WK + ((size_t)cell * 6 + nb) * 4 * ne;
-  double* r = x + ne;
-  double* p = x + 2 * ne;
-  double* q = x + 3 * ne;
-  for (int e = tid; e < ne; e += kItNT) {
+  const double* pd = g.PD + (size_t)cell * ns;
+  double* x = g.WK + ((size_t)cell * 6 + nb) * 4 * ns;
+  double* r = x + ns;
+  double* p = x + 2 * ns;
+  double* q = x + 3 * ns;
+  for (int e = tid; e < ns; e += kItNT) {
     x[e] = 0.0;
     r[e] = 0.0;
-    p[e] = 0.0;
     q[e] = 0.0;
   }
   __syncthreads();
@@ -2098,12 +2119,11 @@ __global__ __launch_bounds__(kItNT) void rt3_cg_kernel(Rt3Args g) {
     __syncthreads();
   }
   double rz = 0.0, rr = 0.0;
-  for (int i = tid; i < ns; i += kItNT) {
-    const int e = g.iedge[i];
-    const double z = pd[e] * r[e];
+  for (int e = tid; e < ns; e += kItNT) {
+    const double re = r[e], z = pd[e] * re;
     p[e] = z;
-    rz += r[e] * z;
-    rr += r[e] * r[e];
+    rz += re * z;
+    rr += re * re;
   }
   rz = it_block_sum(rz, redA);
   const double bnorm = sqrt(it_block_sum(rr, redB));
@@ -2112,6 +2132,7 @@ __global__ __launch_bounds__(kItNT) void rt3_cg_kernel(Rt3Args g) {
   if (flag == MSFEC_CELL_OK && bnorm > 0.0) {
     const double stop = g.rtol * bnorm;
     flag = MSFEC_CELL_NOT_CONVERGED;
+    __syncthreads();
     while (itn < g.max_iter) {
       ++itn;
       // q = A p (q cleared by the previous pass); p . A p from the cells
@@ -2131,16 +2152,32 @@ __global__ __launch_bounds__(kItNT) void rt3_cg_kernel(Rt3Args g) {
       const double alpha = rz / pq;
       double rzn = 0.0;
       rr = 0.0;
-      for (int i = tid; i < ns; i += kItNT) {
-        const int e = g.iedge[i];
+      for (int e = tid; e < ns; e += kItNT) {
         x[e] += alpha * p[e];
         const double re = r[e] - alpha * q[e];
         r[e] = re;
         rzn += re * pd[e] * re;
         rr += re * re;
       }
-      rzn = it_block_sum(rzn, redB);
-      rr = sqrt(it_block_sum(rr, redA));
+      // both sums in one reduction
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        rzn += __shfl_xor(rzn, off);
+        rr += __shfl_xor(rr, off);
+      }
+      if ((tid & 63) == 0) {
+        redB[tid >> 6] = rzn;
+        redB[kItNW + (tid >> 6)] = rr;
+      }
+      __syncthreads();
+      rzn = 0.0;
+      rr = 0.0;
+#pragma unroll
+      for (int w = 0; w < kItNW; ++w) {
+        rzn += redB[w];
+        rr += redB[kItNW + w];
+      }
+      rr = sqrt(rr);
       if (!isfinite(rr)) {
         flag = MSFEC_CELL_NONFINITE;
         break;
@@ -2151,8 +2188,7 @@ __global__ __launch_bounds__(kItNT) void rt3_cg_kernel(Rt3Args g) {
       }
       const double beta = rzn / rz;
       rz = rzn;
-      for (int i = tid; i < ns; i += kItNT) {
-        const int e = g.iedge[i];
+      for (int e = tid; e < ns; e += kItNT) {
         p[e] = pd[e] * r[e] + beta * p[e];
         q[e] = 0.0;
       }
@@ -2162,17 +2198,18 @@ __global__ __launch_bounds__(kItNT) void rt3_cg_kernel(Rt3Args g) {
   __syncthreads();
   // sigma = c_n + D psi on every face (a face's circulation from its 4 edges)
   double* sg = g.S + ((size_t)cell * 6 + nb) * nf;
-  auto EX = [](int i, int j, int k) { return (k * (M + 1) + j) * M + i; };
-  auto EY = [](int i, int j, int k) { return nx + (k * M + j) * (M + 1) + i; };
-  auto EZ = [](int i, int j, int k) { return 2 * nx + (k * (M + 1) + j) * (M + 1) + i; };
+  auto X = [&](int t, int i, int j, int k) {
+    const int ie = rt3_iidx<M>(t, i, j, k);
+    return ie >= 0 ? x[ie] : 0.0;
+  };
   for (int f = tid; f < nf; f += kItNT) {
     int t, i, j, k;
-    dof_pos<M>(ne + f, t, i, j, k);
+    dof_pos<M>(3 * M * (M + 1) * (M + 1) + f, t, i, j, k);
     double circ, c = 0.0;
     const int d = t - 3, lev = d == 0 ? i : (d == 1 ? j : k);
-    if (d == 0) circ = x[EY(i, j, k)] + x[EZ(i, j + 1, k)] - x[EY(i, j, k + 1)] - x[EZ(i, j, k)];
-    else if (d == 1) circ = x[EZ(i, j, k)] + x[EX(i, j, k + 1)] - x[EZ(i + 1, j, k)] - x[EX(i, j, k)];
-    else circ = x[EX(i, j, k)] + x[EY(i + 1, j, k)] - x[EX(i, j + 1, k)] - x[EY(i, j, k)];
+    if (d == 0) circ = X(1, i, j, k) + X(2, i, j + 1, k) - X(1, i, j, k + 1) - X(2, i, j, k);
+    else if (d == 1) circ = X(2, i, j, k) + X(0, i, j, k + 1) - X(2, i + 1, j, k) - X(0, i, j, k);
+    else circ = X(0, i, j, k) + X(1, i + 1, j, k) - X(0, i, j + 1, k) - X(1, i, j, k);
     if (d == (nb >> 1)) c = lag(nb & 1, (double)lev / M) / (M * M);
     sg[f] = c + circ;
   }
@@ -2888,7 +2925,6 @@ struct msfec_rt3_batch {
   double* d_verts = nullptr;
   double* d_ainv = nullptr;
   double* d_f = nullptr;
-  int* d_iedge = nullptr;
   double* d_MF = nullptr;
   double* d_PD = nullptr;
   double* d_WK = nullptr;
@@ -2907,8 +2943,8 @@ struct msfec_rt3_batch {
 namespace {
 
 void rt3_free(msfec_rt3_batch* h) {
-  void* ptrs[] = {h->d_verts, h->d_ainv, h->d_f, h->d_iedge, h->d_MF,  h->d_PD,    h->d_WK,
-                  h->d_S,     h->d_M,    h->d_b, h->d_its,   h->d_flg, h->d_status};
+  void* ptrs[] = {h->d_verts, h->d_ainv, h->d_f,   h->d_MF,  h->d_PD,    h->d_WK,
+                  h->d_S,     h->d_M,    h->d_b,   h->d_its, h->d_flg, h->d_status};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (auto& set : h->ev_sets)
@@ -2923,7 +2959,6 @@ msfec_status rt3_launch(msfec_rt3_batch* h, hipStream_t s, std::array<hipEvent_t
   a.verts = h->d_verts;
   a.ainv = h->d_ainv;
   a.fq = h->has_f ? h->d_f : nullptr;
-  a.iedge = h->d_iedge;
   a.MF = h->d_MF;
   a.PD = h->d_PD;
   a.WK = h->d_WK;
@@ -2996,10 +3031,8 @@ msfec_status msfec_rt3_create(const msfec_rt3_params* params, int n_cells, const
   const NrtTables tab = nrt_tables(l);
   h->ne = tab.ne;
   h->nf = tab.N - tab.ne;
-  std::vector<int> iedge;
-  for (int e = 0; e < tab.ne; ++e)
-    if (tab.f2i[e] >= 0) iedge.push_back(e);
-  h->ns = (int)iedge.size();
+  const int m = 1 << l;
+  h->ns = 3 * m * (m - 1) * (m - 1);  // interior edges
   const size_t nc = (size_t)n_cells, nfc = (size_t)1 << (3 * l);
   msfec_status st = MSFEC_OK;
 #define RT3_TRY(x)        \
@@ -3014,10 +3047,9 @@ msfec_status msfec_rt3_create(const msfec_rt3_params* params, int n_cells, const
   RT3_TRY(nrt_alloc(&h->d_verts, nc * 24));
   RT3_TRY(nrt_alloc(&h->d_ainv, nc * nfc * 8 * 6));
   RT3_TRY(nrt_alloc(&h->d_f, nc * nfc * 8));
-  RT3_TRY(nrt_alloc(&h->d_iedge, iedge.size()));
   RT3_TRY(nrt_alloc(&h->d_MF, nc * 21 * nfc));
-  RT3_TRY(nrt_alloc(&h->d_PD, nc * h->ne));
-  RT3_TRY(nrt_alloc(&h->d_WK, nc * 6 * 4 * h->ne));
+  RT3_TRY(nrt_alloc(&h->d_PD, nc * h->ns));
+  RT3_TRY(nrt_alloc(&h->d_WK, nc * 6 * 4 * h->ns));
   RT3_TRY(nrt_alloc(&h->d_S, nc * 6 * h->nf));
   RT3_TRY(nrt_alloc(&h->d_M, nc * 49));
   RT3_TRY(nrt_alloc(&h->d_b, nc * 7));
@@ -3025,7 +3057,6 @@ msfec_status msfec_rt3_create(const msfec_rt3_params* params, int n_cells, const
   RT3_TRY(nrt_alloc(&h->d_flg, nc * 6));
   RT3_TRY(nrt_alloc(&h->d_status, nc));
   RT3_TRY(nrt_in(h->stream, h->d_verts, hv.data(), hv.size() * sizeof(double), MSFEC_HOST));
-  RT3_TRY(nrt_in(h->stream, h->d_iedge, iedge.data(), iedge.size() * sizeof(int), MSFEC_HOST));
 #undef RT3_TRY
   *out = h;
   return MSFEC_OK;
