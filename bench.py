@@ -504,8 +504,8 @@ def nrt4_config(steps: int, warmup: int, check: bool, cpu: bool):
     its = nb.get_iterations()
     m = 1 << NRT4_NREF
     ns, nu = 3 * m * (m - 1) ** 2, 3 * (m - 1) * m * m
-    nb = m // 2 if m <= 4 else 4
-    bpi = nrt_minres_bytes_per_iteration(ns + nu, ns, nu, m ** 3, 3 * nb ** 3, NRT4_NREF >= 4)
+    nblk = m // 2 if m <= 4 else 4  # aggregates per direction of the coarse correction
+    bpi = nrt_minres_bytes_per_iteration(ns + nu, ns, nu, m ** 3, 3 * nblk ** 3, NRT4_NREF >= 4)
     gbs = float(its.sum()) * bpi / (kt[2] * 1e-3) / 1e9
     out = {"workload": "3D Nedelec x Raviart-Thomas basis, the reference's prm_ned_rt_test-01: "
                        "4^3=64 coarse cells x 16^3 fine cells (13872 edges + 13056 faces, "

@@ -35,15 +35,17 @@ DRIVER = os.path.join(CPP, "q_multiscale_driver")
 NED_DRIVER = os.path.join(CPP, "q_ned_driver")
 RT_DRIVER = os.path.join(CPP, "rt_dq_driver")
 NRT_DRIVER = os.path.join(CPP, "ned_rt_driver")
+RT3_DRIVER = os.path.join(CPP, "rt_dq3_driver")
 
 
 def build_driver(verbose: bool = False) -> str:
-    """The QBasis, QNedBasis, RTDQBasis and NedRTBasis C++ shims and their drivers:
+    """The QBasis, QNedBasis, RTDQBasis (2-D and 3-D) and NedRTBasis C++ shims and their drivers:
     host C++ only, linked against the HIP library through the C-ABI."""
     for shim, main, exe in (("msfec_q_basis.cpp", "q_multiscale_driver.cpp", DRIVER),
                             ("msfec_q_ned_basis.cpp", "q_ned_driver.cpp", NED_DRIVER),
                             ("msfec_rt_dq_basis.cpp", "rt_dq_driver.cpp", RT_DRIVER),
-                            ("msfec_ned_rt_basis.cpp", "ned_rt_driver.cpp", NRT_DRIVER)):
+                            ("msfec_ned_rt_basis.cpp", "ned_rt_driver.cpp", NRT_DRIVER),
+                            ("msfec_rt_dq3_basis.cpp", "rt_dq3_driver.cpp", RT3_DRIVER)):
         cmd = ["g++", "-O2", "-std=c++17", "-Wall", f"-I{INCLUDE}", f"-I{CPP}",
                os.path.join(CPP, shim), os.path.join(CPP, "msfec_vtu.cpp"), os.path.join(CPP, main),
                f"-L{HERE}", "-l:libmsfec_hip.so", "-Wl,-rpath,$ORIGIN/../msfec_amd",

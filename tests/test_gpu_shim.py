@@ -474,3 +474,80 @@ def test_ned_rt_driver_vtu_output(oracle_mod):
         scale = max(np.abs(sig).max(), np.abs(u).max())
         np.testing.assert_allclose(cf["sigma"], sig, rtol=0, atol=1e-10 * scale)
         np.testing.assert_allclose(cf["u"], u, rtol=0, atol=1e-10 * scale)
+
+
+RT3_DRIVER = os.path.join(REPO, "mpi-msfec_amd", "cpp", "rt_dq3_driver")
+
+
+def run_rt3_driver(l, nc, ainv, fq, td, outdir=None):
+    fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+    with open(fin, "wb") as fh:
+        fh.write(np.array([l, nc], dtype=np.int32).tobytes())
+        for x in (ainv, fq):
+            fh.write(np.ascontiguousarray(x, dtype=np.float64).tobytes())
+    args = [RT3_DRIVER, fin, fout] + ([outdir] if outdir else [])
+    r = subprocess.run(args, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    return np.frombuffer(open(fout, "rb").read(), dtype=np.float64)
+
+
+def test_rt_dq3_driver(oracle_mod):
+    """The C++ RTDQ3Basis shim (the reference's 3-D RTDQBasis surface): map of
+    cells, run(), getters, set_global_weights (fluxes sum_{i<6} w_i sigma_i,
+    u = w_6, rt_dq_basis.cc:1011-1036) against the oracle, and the VTU files
+    of output_basis (6 .div files of the first cell) and
+    output_global_solution_in_cell (flux at the fine cell centres, u)."""
+    print("case rt_dq3_shim_driver l=2 nc=2", flush=True)
+    assert os.path.exists(RT3_DRIVER), "driver not built"
+    l, nc = 2, 2
+    ncell = nc ** 3
+    m, nf, nfc, N = oracle_mod.rt3_sizes(l)
+    ainv, _, _ = nrt_driver_inputs(np.random.default_rng(41), l, ncell)
+    fq = np.random.default_rng(42).normal(size=(ncell, nfc, 8))
+    with tempfile.TemporaryDirectory() as td:
+        od = os.path.join(td, "vtu")
+        os.mkdir(od)
+        d = run_rt3_driver(l, nc, ainv, fq, td, od)
+        names = sorted(os.listdir(od))
+        expect = sorted([f"basis_rt-dq.div.00000.cell-0.index-{i:02d}.vtu" for i in range(6)] +
+                        [f"solution.00000.cell-{c}.vtu" for c in range(ncell)])
+        assert names == expect
+        o = 0
+        M = d[o:o + ncell * 49].reshape(ncell, 7, 7); o += ncell * 49
+        b = d[o:o + ncell * 7].reshape(ncell, 7); o += ncell * 7
+        S = d[o:o + ncell * 6 * nf].reshape(ncell, 6, nf); o += ncell * 6 * nf
+        sol = d[o:o + ncell * N].reshape(ncell, N); o += ncell * N
+        assert o == d.size
+        verts = unit_cube_boxes(oracle_mod, nc)
+        for c in range(ncell):
+            Mo, bo, So, st = oracle_mod.rt3_basis_cell(l, verts[c], ainv[c], fq[c])
+            assert st == 0
+            assert np.abs(M[c] - Mo).max() < 1e-9 * np.abs(Mo).max()
+            assert np.abs(b[c] - bo).max() < 1e-12 * np.abs(bo).max()
+            assert np.abs(S[c] - So).max() < 1e-9 * np.abs(So).max()
+        w = np.arange(1.0, 8.0)
+        np.testing.assert_allclose(sol[:, :nf], np.einsum("i,cif->cf", w[:6], S), rtol=0,
+                                   atol=1e-13 * np.abs(S).max() * w.sum())
+        assert (sol[:, nf:] == 7.0).all()
+        # the VTU cell fields: the flux at the fine cell centres
+        h = 1.0 / nc / m
+        nf1 = m * m * (m + 1)
+
+        def centre(s):
+            fx = s[:nf1].reshape(m, m, m + 1)
+            fy = s[nf1:2 * nf1].reshape(m, m + 1, m)
+            fz = s[2 * nf1:].reshape(m + 1, m, m)
+            return np.stack([0.5 * (fx[..., 1:] + fx[..., :-1]) / h ** 2,
+                             0.5 * (fy[:, 1:] + fy[:, :-1]) / h ** 2,
+                             0.5 * (fz[1:] + fz[:-1]) / h ** 2], -1).reshape(-1, 3)
+
+        for i in range(6):
+            cf = read_vtu_cells(os.path.join(od, f"basis_rt-dq.div.00000.cell-0.index-{i:02d}.vtu"))
+            sig = centre(S[0, i])
+            np.testing.assert_allclose(cf["sigma"], sig, rtol=0, atol=1e-10 * np.abs(sig).max())
+            np.testing.assert_allclose(cf["u"], 1.0)
+        c = ncell - 1
+        cf = read_vtu_cells(os.path.join(od, f"solution.00000.cell-{c}.vtu"))
+        sig = centre(sol[c, :nf])
+        np.testing.assert_allclose(cf["sigma"], sig, rtol=0, atol=1e-10 * np.abs(sig).max())
+        np.testing.assert_allclose(cf["u"], 7.0)
