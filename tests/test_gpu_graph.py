@@ -157,3 +157,21 @@ def test_nrt_compute_graph_replay(stream, l, solver):
         replay_check(nb, stream, lambda d: nb.set_coefficients(*d),
                      lambda: (nb.get_global_element_matrix(), nb.get_global_element_rhs(),
                               nb.get_basis(), nb.get_cell_status()), data)
+
+
+@pytest.mark.parametrize("l", [2, 3])
+def test_rt3_compute_graph_replay(stream, l):
+    """msfec_rt3_compute (five kernel launches, no memset) replayed from a
+    graph, bit for bit the eager results."""
+    print(f"case graph rt3 l={l}", flush=True)
+    from msfec_amd import RTDQ3BasisBatch
+    from test_gpu_nrt import boxes, rand_coefs
+    rng = np.random.default_rng(71 + l)
+    n = 8
+    verts = boxes(rng, n)
+    data = [(rand_coefs(rng, n, l, reaction=False, load=False)[0],
+             rng.normal(size=(n, (1 << l) ** 3, 8))) for _ in range(2)]
+    with RTDQ3BasisBatch(verts, n_refine_local=l) as rb:
+        replay_check(rb, stream, lambda d: rb.set_coefficients(*d),
+                     lambda: (rb.get_global_element_matrix(), rb.get_global_element_rhs(),
+                              rb.get_basis(), rb.get_cell_status()), data)

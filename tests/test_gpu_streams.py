@@ -136,3 +136,18 @@ def test_nrt_uploads_ordered_on_nonblocking_stream(nb_stream, l, n):
     with NedRTBasisBatch(verts, n_refine_local=l) as nb:
         run_overlapped(nb, s, lambda d: nb.set_coefficients(*d),
                        lambda: (nb.get_global_element_matrix(), nb.get_basis()), d0, d1)
+
+
+def test_rt3_uploads_ordered_on_nonblocking_stream(nb_stream):
+    print("case streams rt3 l=3 n=8", flush=True)
+    from msfec_amd import RTDQ3BasisBatch
+    from test_gpu_nrt import boxes, rand_coefs
+    hip, s = nb_stream
+    rng = np.random.default_rng(61)
+    n, l = 8, 3
+    verts = boxes(rng, n)
+    d0, d1 = [(rand_coefs(rng, n, l, reaction=False, load=False)[0],
+               rng.normal(size=(n, (1 << l) ** 3, 8))) for _ in range(2)]
+    with RTDQ3BasisBatch(verts, n_refine_local=l) as rb:
+        run_overlapped(rb, s, lambda d: rb.set_coefficients(*d),
+                       lambda: (rb.get_global_element_matrix(), rb.get_basis()), d0, d1)
