@@ -149,3 +149,22 @@ def test_rt3_iteration_limit_flags_the_cell():
     M, b, S, st, its = run_gpu(verts, 3, ainv, None, max_iter=3)
     assert (st == L.CELL_NOT_CONVERGED).all()
     assert (its == 3).all()
+
+
+def test_rt3_nonfinite_cell_is_flagged_alone(oracle_mod):
+    """A NaN in one cell's A^-1 marks that cell MSFEC_CELL_NONFINITE; the
+    other cells of the batch still match the oracle."""
+    print("case rt3 non-finite cell l=2 n=4", flush=True)
+    from msfec_amd import _lib as L
+    rng = np.random.default_rng(561)
+    n, l = 4, 2
+    verts = boxes(rng, n)
+    ainv, _, _, _ = rand_coefs(rng, n, l, reaction=False, load=False)
+    ainv[2, 5, 3, 0] = np.nan
+    M, b, S, st, its = run_gpu(verts, l, ainv, None)
+    assert st[2] == L.CELL_NONFINITE
+    for c in (0, 1, 3):
+        assert st[c] == 0
+        Mo, bo, So, sto = oracle_mod.rt3_basis_cell(l, verts[c], ainv[c])
+        assert rel(M[c], Mo) <= TOL
+        assert max(rel(S[c, i], So[i]) for i in range(6)) <= TOL
